@@ -1,0 +1,191 @@
+/*
+ * tdgl_hip.h -- C ABI of libtdglhip.so, the MI355X (gfx950) implementation of
+ * tdgl3d's trapezoidal time step (omedeiro/nanowire_tdgl).
+ *
+ * The reference exposes no FFI: its seam is the Python call chain
+ *   solve (src/tdgl3d/solvers/runner.py:40-163)
+ *     -> trapezoidal (src/tdgl3d/solvers/integrators.py:92-206)
+ *       -> newton_gcr_trap (src/tdgl3d/solvers/newton.py:76-120)
+ *         -> tgcr_matrix_free_trap (src/tdgl3d/solvers/tgcr.py:96-156)
+ *           -> eval_f (src/tdgl3d/physics/rhs.py:147-239)
+ * Each entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *   - Every call returns an int status: TDGL_OK (0) or a negative TDGL_E_*.
+ *     tdgl_last_error() describes the last failure on a context.
+ *   - Pointers named *_dev are device pointers owned by the caller (for
+ *     example torch tensors); they must stay valid until the work queued on
+ *     the context's stream has finished.  Host pointers are read during the
+ *     call only.
+ *   - All work is queued on the context's stream (tdgl_set_stream); calls
+ *     that return host values synchronise that stream.
+ *   - A context is not thread-safe; use one context per stream.
+ *   - "State" buffers use the library's internal layout: interior nodes only,
+ *     i fastest, then j, then k; psi as interleaved (re, im) pairs followed
+ *     by real phi_x, phi_y and (3-D only) phi_z arrays:
+ *         [psi re/im: 2n][phi_x: n][phi_y: n][phi_z: n]   (n = n_interior)
+ *     tdgl_permute_in/out convert from/to the reference's complex128 state
+ *     vector [psi; phi_x; phi_y; phi_z] in its interior order (i slowest,
+ *     k fastest, src/tdgl3d/mesh/indices.py:271-278).
+ */
+#ifndef TDGL_HIP_H
+#define TDGL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDGL_OK 0
+#define TDGL_E_INVALID (-1)    /* bad argument or unsupported configuration */
+#define TDGL_E_HIP (-2)        /* a HIP runtime call failed */
+#define TDGL_E_NOMEM (-3)      /* device allocation failed */
+#define TDGL_E_COMPLEX_PHI (-4) /* a link variable has a non-zero imaginary part */
+#define TDGL_E_NONFINITE (-5)  /* a norm came back NaN or Inf */
+
+#define TDGL_PREC_F64 0        /* everything in FP64 (plain method) */
+#define TDGL_PREC_MIXED 1      /* FP32 stencil/JVP/Krylov vectors, FP64 Newton and Gram scalars */
+
+typedef struct tdgl_ctx tdgl_ctx;
+
+/* Grid and physics parameters; mirrors SimulationParameters
+ * (src/tdgl3d/core/parameters.py:9-94).  Nx, Ny, Nz count cells. */
+typedef struct tdgl_grid {
+  int32_t Nx, Ny, Nz;
+  int32_t periodic_x, periodic_y, periodic_z;
+  double hx, hy, hz;
+  double kappa;
+} tdgl_grid;
+
+/* Newton / TGCR options; mirrors the keyword arguments of
+ * trapezoidal (integrators.py:100-111), newton_gcr_trap (newton.py:82-87)
+ * and tgcr_matrix_free_trap (tgcr.py:100-103).  tgcr_max_iter <= 0 selects
+ * the reference default max(N, round(0.2 N)) with N = n_state. */
+typedef struct tdgl_solver_opts {
+  double newton_tol_f;
+  double newton_tol_dx;
+  int32_t newton_max_iter;
+  int32_t precision; /* TDGL_PREC_F64 or TDGL_PREC_MIXED */
+  double tol_gcr;
+  double eps_mf;
+  int64_t tgcr_max_iter;
+} tdgl_solver_opts;
+
+/* Per-step counters (the reference keeps none; they make iteration-count
+ * parity testable). */
+typedef struct tdgl_step_stats {
+  int32_t converged;        /* Newton converged (newton.py:118) */
+  int32_t newton_iters;     /* value newton_gcr_trap returns as `iterations` */
+  int32_t tgcr_failed;      /* the last TGCR returned the empty array (tgcr.py:154-155) */
+  int32_t tgcr_iters_last;  /* Krylov iterations of the last TGCR call */
+  int32_t tgcr_iters_total; /* Krylov iterations summed over the step */
+  int32_t rhs_evals;        /* stencil passes (RHS, predictor, Newton residual) */
+  int32_t jvp_evals;        /* fused finite-difference JVP passes */
+  int32_t reserved;
+  double g_inf;             /* last ||g||_inf (newton.py:112) */
+  double dx_inf;            /* last ||dx||_inf (newton.py:113) */
+  double bytes_moved;       /* algorithmic HBM bytes of the step */
+} tdgl_step_stats;
+
+/* Context: grid geometry + device buffers.  Replaces construct_indices'
+ * role for the RHS (src/tdgl3d/mesh/indices.py:239-450): index maps become
+ * index arithmetic on the internal layout. */
+int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out);
+int tdgl_ctx_destroy(tdgl_ctx* ctx);
+int tdgl_set_stream(tdgl_ctx* ctx, void* hip_stream);
+const char* tdgl_last_error(const tdgl_ctx* ctx);
+/* Number of doubles in one internal state buffer ((2 + n_phi) * n_interior). */
+int64_t tdgl_state_len(const tdgl_ctx* ctx);
+int64_t tdgl_n_interior(const tdgl_ctx* ctx);
+
+/* Per-node material and hole data (host arrays, copied to the device).
+ *   link_bits_full : dim_x bytes in the reference's full-grid numbering
+ *                    m = i + (Nx+1) j + (Nx+1)(Ny+1) k; bit 0/1/2 = the x/y/z
+ *                    link at m is a hole-boundary link (GridIndices.hole_*_bc_mask,
+ *                    indices.py:126-134, zeroed by rhs.py:132-137 and rhs.py:229-234).
+ *                    NULL = no holes.
+ *   kappa_interior : n doubles, MaterialMap.kappa at the interior nodes in the
+ *                    reference interior order (sparse_operators.py:19-28); NULL
+ *                    = uniform grid->kappa (no material).
+ *   sc_interior    : n doubles, MaterialMap.interior_sc_mask (material.py:35-41,
+ *                    read by sparse_operators.py:221-224); NULL = no material.
+ * Replaces MaterialMap / Device.add_hole (core/material.py:26-112,
+ * core/device.py:80-153) as seen by eval_f. */
+int tdgl_set_material(tdgl_ctx* ctx, const uint8_t* link_bits_full,
+                      const double* kappa_interior, const double* sc_interior);
+
+/* Applied field of BoundaryVectors (physics/rhs.py:32-40) built by
+ * build_boundary_field_vectors (physics/applied_field.py:59-91): uniform values
+ * on the face-inner nodes. */
+int tdgl_set_field(tdgl_ctx* ctx, double bx, double by, double bz);
+/* General BoundaryVectors: three host arrays of dim_x doubles (full-grid
+ * numbering); only their face-inner entries are read (rhs.py:95-127). */
+int tdgl_set_field_vectors(tdgl_ctx* ctx, const double* bx_full, const double* by_full,
+                           const double* bz_full);
+
+/* Layout conversion between the reference state vector (complex128, n_state,
+ * StateVector layout core/state.py:11-30) and the internal layout.
+ * tdgl_permute_in fails with TDGL_E_COMPLEX_PHI if any phi entry has a
+ * non-zero imaginary part (the build stores phi as real). */
+int tdgl_permute_in(tdgl_ctx* ctx, const void* ref_state_dev, double* state_dev);
+int tdgl_permute_out(tdgl_ctx* ctx, const double* state_dev, void* ref_state_dev);
+
+/* f = eval_f(x) (physics/rhs.py:147-239), FP64, internal layout. */
+int tdgl_rhs(tdgl_ctx* ctx, const double* x_dev, double* f_dev);
+
+/* Ap = p - dt/2 * (f(x + eps p) - f0) * (1/eps)  (tgcr.py:129-133), FP64,
+ * with f0 = f(x) supplied by the caller. */
+int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x_dev, const double* f0_dev,
+                     const double* p_dev, double eps, double dt, double* ap_dev);
+
+/* Reductions over internal-layout vectors (the reference's np.vdot(...).real,
+ * np.linalg.norm(., 2) and np.linalg.norm(., inf): tgcr.py:116,129,136,140,149,152;
+ * newton.py:100,112-113).  Results are returned to the host. */
+int tdgl_dot(tdgl_ctx* ctx, const double* a_dev, const double* b_dev, double* out);
+int tdgl_norm_inf(tdgl_ctx* ctx, const double* a_dev, double* out);
+
+/* One trapezoidal step attempt at the current field: predictor, Newton,
+ * TGCR (integrators.py:151-172 + newton.py:76-120 + tgcr.py:96-156).
+ * On convergence X_dev is overwritten with the new state and stats->converged
+ * is 1; otherwise X_dev is left unchanged (the caller divides dt by 10 as
+ * integrators.py:185-197 does). */
+int tdgl_trap_step(tdgl_ctx* ctx, double* X_dev, double dt, const tdgl_solver_opts* opts,
+                   tdgl_step_stats* stats);
+
+/* One Forward-Euler step X += dt f(X) (integrators.py:78-83). */
+int tdgl_euler_step(tdgl_ctx* ctx, double* X_dev, double dt);
+
+/* Check-path observables on an internal-layout state, written in the
+ * reference interior order (3 x n doubles each):
+ *   B  = curl(phi) on the expanded state with BCs at the current field
+ *        (core/solution.py:124-158 -> physics/bfield.py:12-78);
+ *   Js = Im[conj(psi) e^{-i phi} psi(+a)] on the expanded state without BCs
+ *        (core/solution.py:180-217 -> physics/current_density.py:14-72).
+ * Either output pointer may be NULL. */
+int tdgl_observables(tdgl_ctx* ctx, const double* x_dev, double* b_dev, double* js_dev);
+
+/* Instrumentation (no counterpart in the reference): when enabled, the driver
+ * brackets every kernel launch of the trapezoidal step with HIP events on the
+ * context's stream and accumulates, per kernel class, the summed duration,
+ * the launch count and the algorithmic HBM bytes (each vector pass counted
+ * once).  tdgl_profile_read synchronises the stream. */
+#define TDGL_K_PRED 0      /* predictor + gamma stencil */
+#define TDGL_K_NEWTON 1    /* Newton residual stencil (f, r = -g, norms) */
+#define TDGL_K_JVP 2       /* fused finite-difference JVP stencil */
+#define TDGL_K_MGS 3       /* MGS pass: Ap -= beta Ap_j, next beta */
+#define TDGL_K_MGS_LAST 4  /* last MGS update + ||Ap|| */
+#define TDGL_K_FINAL_P 5   /* p_k assembly + normalisation + alpha */
+#define TDGL_K_UPDATE_R 6  /* r -= alpha Ap + ||r|| */
+#define TDGL_K_ACCUM 7     /* dx = sum alpha p; x += dx; ||dx||_inf */
+#define TDGL_K_COUNT 8
+int tdgl_profile_enable(tdgl_ctx* ctx, int enable);
+int tdgl_profile_read(tdgl_ctx* ctx, int kernel_class, double* ms_total, int64_t* launches,
+                      double* bytes_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDGL_HIP_H */

@@ -1,0 +1,267 @@
+"""ctypes binding of libtdglhip.so (declared in include/tdgl_hip.h).
+
+The library is the only compute path: importing this module does not need a
+GPU, but creating a :class:`Context` does, and every failure (missing
+library, no GPU, HIP error) raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtdglhip.so"
+
+TDGL_OK = 0
+TDGL_E_INVALID = -1
+TDGL_E_HIP = -2
+TDGL_E_NOMEM = -3
+TDGL_E_COMPLEX_PHI = -4
+TDGL_E_NONFINITE = -5
+
+PREC_F64 = 0
+PREC_MIXED = 1
+
+EXPORTS = [
+    "tdgl_ctx_create", "tdgl_ctx_destroy", "tdgl_set_stream", "tdgl_last_error",
+    "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
+    "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
+    "tdgl_jvp_trap_fd", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
+]
+
+KERNEL_CLASSES = ["pred", "newton", "jvp", "mgs", "mgs_last", "final_p", "update_r", "accum"]
+
+
+class Grid(C.Structure):
+    _fields_ = [("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+                ("periodic_x", C.c_int32), ("periodic_y", C.c_int32), ("periodic_z", C.c_int32),
+                ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+                ("kappa", C.c_double)]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("newton_tol_f", C.c_double), ("newton_tol_dx", C.c_double),
+                ("newton_max_iter", C.c_int32), ("precision", C.c_int32),
+                ("tol_gcr", C.c_double), ("eps_mf", C.c_double),
+                ("tgcr_max_iter", C.c_int64)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("newton_iters", C.c_int32),
+                ("tgcr_failed", C.c_int32), ("tgcr_iters_last", C.c_int32),
+                ("tgcr_iters_total", C.c_int32), ("rhs_evals", C.c_int32),
+                ("jvp_evals", C.c_int32), ("reserved", C.c_int32),
+                ("g_inf", C.c_double), ("dx_inf", C.c_double), ("bytes_moved", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libtdglhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m nanowire_tdgl_amd.build` "
+            "(the HIP library is the only compute path)")
+    lib = C.CDLL(str(LIB_PATH))
+    P, D, I32, I64 = C.c_void_p, C.c_double, C.c_int32, C.c_int64
+    sig = {
+        "tdgl_ctx_create": ([C.POINTER(Grid), C.POINTER(P)], C.c_int),
+        "tdgl_ctx_destroy": ([P], C.c_int),
+        "tdgl_set_stream": ([P, P], C.c_int),
+        "tdgl_last_error": ([P], C.c_char_p),
+        "tdgl_state_len": ([P], I64),
+        "tdgl_n_interior": ([P], I64),
+        "tdgl_set_material": ([P, P, P, P], C.c_int),
+        "tdgl_set_field": ([P, D, D, D], C.c_int),
+        "tdgl_set_field_vectors": ([P, P, P, P], C.c_int),
+        "tdgl_permute_in": ([P, P, P], C.c_int),
+        "tdgl_permute_out": ([P, P, P], C.c_int),
+        "tdgl_rhs": ([P, P, P], C.c_int),
+        "tdgl_jvp_trap_fd": ([P, P, P, P, D, D, P], C.c_int),
+        "tdgl_dot": ([P, P, P, C.POINTER(D)], C.c_int),
+        "tdgl_norm_inf": ([P, P, C.POINTER(D)], C.c_int),
+        "tdgl_trap_step": ([P, P, D, C.POINTER(SolverOpts), C.POINTER(StepStats)], C.c_int),
+        "tdgl_euler_step": ([P, P, D], C.c_int),
+        "tdgl_observables": ([P, P, P, P], C.c_int),
+        "tdgl_profile_enable": ([P, C.c_int], C.c_int),
+        "tdgl_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names from include/tdgl_hip.h that the built library exports."""
+    lib = C.CDLL(str(LIB_PATH))
+    return [s for s in EXPORTS if hasattr(lib, s)]
+
+
+class TDGLError(RuntimeError):
+    pass
+
+
+def _raise(lib, ctx, rc: int, what: str) -> None:
+    msg = lib.tdgl_last_error(ctx).decode() if ctx else ""
+    text = f"{what} failed (code {rc}){': ' + msg if msg else ''}"
+    if rc in (TDGL_E_INVALID, TDGL_E_COMPLEX_PHI):
+        raise ValueError(text)
+    raise TDGLError(text)
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+class Context:
+    """One tdgl_ctx on the current device and torch's current stream."""
+
+    def __init__(self, Nx, Ny, Nz, hx, hy, hz, kappa, periodic=(False, False, False)):
+        import torch
+        if not torch.cuda.is_available():
+            raise TDGLError("no GPU visible: nanowire_tdgl_amd runs on MI355X (HIP) only")
+        self.lib = load()
+        self.torch = torch
+        g = Grid(int(Nx), int(Ny), int(Nz), int(periodic[0]), int(periodic[1]), int(periodic[2]),
+                 float(hx), float(hy), float(hz), float(kappa))
+        h = C.c_void_p()
+        rc = self.lib.tdgl_ctx_create(C.byref(g), C.byref(h))
+        if rc != TDGL_OK:
+            _raise(self.lib, None, rc, "tdgl_ctx_create")
+        self.h = h
+        self.grid = g
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.stream = torch.cuda.current_stream(self.device)
+        self._call("tdgl_set_stream", C.c_void_p(self.stream.cuda_stream))
+        self.len = int(self.lib.tdgl_state_len(h))
+        self.n = int(self.lib.tdgl_n_interior(h))
+        self.nphi = 3 if Nz > 1 else 2
+
+    def _call(self, name: str, *args) -> None:
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != TDGL_OK:
+            _raise(self.lib, self.h, rc, name)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.tdgl_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers ---------------------------------------------------------
+    def empty_state(self):
+        return self.torch.empty(self.len, dtype=self.torch.float64, device=self.device)
+
+    def to_internal(self, ref_state):
+        """complex128 reference-layout state (numpy or torch) -> internal FP64 buffer."""
+        t = self.torch
+        x = t.as_tensor(ref_state)
+        if x.dtype != t.complex128:
+            x = x.to(t.complex128)
+        x = x.to(self.device).contiguous()
+        if x.numel() != (1 + self.nphi) * self.n:
+            raise ValueError(f"state length {x.numel()} != n_state {(1 + self.nphi) * self.n}")
+        out = self.empty_state()
+        self._call("tdgl_permute_in", C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()))
+        return out
+
+    def to_reference(self, st):
+        """internal FP64 buffer -> complex128 reference-layout torch tensor (device)."""
+        t = self.torch
+        out = t.empty((1 + self.nphi) * self.n, dtype=t.complex128, device=self.device)
+        self._call("tdgl_permute_out", C.c_void_p(st.data_ptr()), C.c_void_p(out.data_ptr()))
+        return out
+
+    # -- configuration -----------------------------------------------------
+    def set_material(self, link_bits=None, kappa_int=None, sc_int=None) -> None:
+        import numpy as np
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return C.c_void_p(a.ctypes.data)
+        self._call("tdgl_set_material", arr(link_bits, np.uint8), arr(kappa_int, np.float64),
+                   arr(sc_int, np.float64))
+
+    def set_field(self, bx: float, by: float, bz: float) -> None:
+        self._call("tdgl_set_field", C.c_double(bx), C.c_double(by), C.c_double(bz))
+
+    def set_field_vectors(self, Bx, By, Bz) -> None:
+        import numpy as np
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (Bx, By, Bz)]
+        self._call("tdgl_set_field_vectors", *[C.c_void_p(v.ctypes.data) for v in a])
+
+    # -- compute -------------------------------------------------------------
+    def rhs(self, x, f) -> None:
+        self._call("tdgl_rhs", C.c_void_p(x.data_ptr()), C.c_void_p(f.data_ptr()))
+
+    def jvp_trap_fd(self, x, f0, p, eps: float, dt: float, ap) -> None:
+        self._call("tdgl_jvp_trap_fd", C.c_void_p(x.data_ptr()), C.c_void_p(f0.data_ptr()),
+                   C.c_void_p(p.data_ptr()), C.c_double(eps), C.c_double(dt),
+                   C.c_void_p(ap.data_ptr()))
+
+    def dot(self, a, b) -> float:
+        out = C.c_double()
+        self._call("tdgl_dot", C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.byref(out))
+        return out.value
+
+    def norm_inf(self, a) -> float:
+        out = C.c_double()
+        self._call("tdgl_norm_inf", C.c_void_p(a.data_ptr()), C.byref(out))
+        return out.value
+
+    def trap_step(self, X, dt: float, opts: SolverOpts) -> StepStats:
+        st = StepStats()
+        self._call("tdgl_trap_step", C.c_void_p(X.data_ptr()), C.c_double(dt), C.byref(opts),
+                   C.byref(st))
+        return st
+
+    def euler_step(self, X, dt: float) -> None:
+        self._call("tdgl_euler_step", C.c_void_p(X.data_ptr()), C.c_double(dt))
+
+    def profile_enable(self, on: bool = True) -> None:
+        self._call("tdgl_profile_enable", C.c_int(1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """Per kernel class: summed event-timed milliseconds, launches, algorithmic bytes."""
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+            self._call("tdgl_profile_read", C.c_int(k), C.byref(ms), C.byref(n), C.byref(by))
+            out[name] = {"ms": ms.value, "launches": n.value, "bytes": by.value}
+        return out
+
+    def observables(self, x, want_b=True, want_j=True):
+        t = self.torch
+        b = t.empty(3 * self.n, dtype=t.float64, device=self.device) if want_b else None
+        j = t.empty(3 * self.n, dtype=t.float64, device=self.device) if want_j else None
+        self._call("tdgl_observables", C.c_void_p(x.data_ptr()), C.c_void_p(_ptr(b)),
+                   C.c_void_p(_ptr(j)))
+        return b, j
+
+
+def solver_opts(newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,
+                eps_mf=1e-4, tgcr_max_iter=None, precision=PREC_F64) -> SolverOpts:
+    return SolverOpts(float(newton_tol_f), float(newton_tol_dx), int(newton_max_iter),
+                      int(precision), float(tol_gcr), float(eps_mf),
+                      int(tgcr_max_iter) if tgcr_max_iter is not None else 0)
+

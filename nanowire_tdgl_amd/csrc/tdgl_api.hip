@@ -1,0 +1,665 @@
+// C ABI of libtdglhip.so and the host driver of the trapezoidal step.
+//
+// The driver keeps tdgl3d's control flow exactly (integrators.py:151-172,
+// newton.py:97-120, tgcr.py:106-156) and reads scalars back only where the
+// reference branches: once per TGCR iteration (stop test + breakdown test)
+// and once per Newton iteration (convergence test).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/tdgl_hip.h"
+#include "tdgl_launch.hpp"
+
+using namespace tdgl;
+
+struct tdgl_ctx {
+  tdgl_grid grid{};
+  Geo g{};
+  Bnd b{};
+  Mat mt{};
+  StencilCfg cfg{1};
+  hipStream_t st = nullptr;
+  std::string err;
+  int device = 0;
+
+  uint8_t* d_flags = nullptr;
+  double* d_k2 = nullptr;
+  double* d_sc = nullptr;
+  double* d_bfull = nullptr;  // 3 * dim_x
+  RedBufs rb{};
+  int* d_bad = nullptr;
+  double* h_res = nullptr;  // pinned mirror of the scalar slots
+
+  long long len = 0;  // doubles per state buffer
+  double* w_x = nullptr;
+  double* w_gam = nullptr;
+  double* w_f = nullptr;
+  double* w_r = nullptr;
+  std::vector<double*> P, AP;
+  double** d_ptab = nullptr;
+  int ptab_cap = 0;
+
+  // per-kernel-class event timing (tdgl_profile_enable)
+  struct PendingEv {
+    hipEvent_t a, b;
+    int cls;
+    double bytes;
+  };
+  bool prof = false;
+  std::vector<PendingEv> pending;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  double prof_ms[TDGL_K_COUNT] = {0};
+  long long prof_n[TDGL_K_COUNT] = {0};
+  double prof_bytes[TDGL_K_COUNT] = {0};
+};
+
+namespace {
+
+int fail(tdgl_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, TDGL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+int alloc(tdgl_ctx* ctx, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(ctx, TDGL_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return TDGL_OK;
+}
+
+// Event bracket around one launch of kernel class `cls` (no-op unless profiling).
+int prof_begin(tdgl_ctx* ctx, int cls, double bytes) {
+  if (!ctx->prof) return -1;
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (!ctx->ev_pool.empty()) {
+    ev = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+  } else if (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess) {
+    return -1;
+  }
+  if (hipEventRecord(ev.first, ctx->st) != hipSuccess) return -1;
+  ctx->pending.push_back({ev.first, ev.second, cls, bytes});
+  return (int)ctx->pending.size() - 1;
+}
+void prof_end(tdgl_ctx* ctx, int tok) {
+  if (tok >= 0) hipEventRecord(ctx->pending[tok].b, ctx->st);
+}
+// Fold finished brackets into the per-class totals (stream already synchronised).
+void prof_harvest(tdgl_ctx* ctx) {
+  for (auto& p : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ctx->prof_ms[p.cls] += ms;
+      ctx->prof_n[p.cls] += 1;
+      ctx->prof_bytes[p.cls] += p.bytes;
+    }
+    ctx->ev_pool.push_back({p.a, p.b});
+  }
+  ctx->pending.clear();
+}
+
+// Scalars back to the host (synchronises the stream).
+int readback(tdgl_ctx* ctx, int slot, int count) {
+  HIPCHK(hipMemcpyAsync(ctx->h_res + slot, ctx->rb.res + slot, sizeof(double) * count,
+                        hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (!ctx->pending.empty()) prof_harvest(ctx);
+  return TDGL_OK;
+}
+
+int ensure_work(tdgl_ctx* ctx) {
+  if (ctx->w_x) return TDGL_OK;
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  int rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_x, bytes))) return rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_gam, bytes))) return rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_f, bytes))) return rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_r, bytes))) return rc;
+  return TDGL_OK;
+}
+
+// Krylov slot i (p_i and Ap_i) allocated on first use; the device table of
+// p pointers is refreshed when it grows.
+int ensure_vec(tdgl_ctx* ctx, int i) {
+  if (i >= kMaxKrylov)
+    return fail(ctx, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  while ((int)ctx->P.size() <= i) {
+    double *p = nullptr, *ap = nullptr;
+    int rc;
+    if ((rc = alloc(ctx, (void**)&p, bytes))) return rc;
+    if ((rc = alloc(ctx, (void**)&ap, bytes))) {
+      hipFree(p);
+      return rc;
+    }
+    ctx->P.push_back(p);
+    ctx->AP.push_back(ap);
+  }
+  if ((int)ctx->P.size() > ctx->ptab_cap) {
+    int cap = std::max(64, ctx->ptab_cap * 2);
+    while (cap < (int)ctx->P.size()) cap *= 2;
+    if (ctx->d_ptab) HIPCHK(hipFree(ctx->d_ptab));
+    ctx->d_ptab = nullptr;
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_ptab, sizeof(double*) * cap))) return rc;
+    ctx->ptab_cap = cap;
+    HIPCHK(hipMemcpyAsync(ctx->d_ptab, ctx->P.data(), sizeof(double*) * ctx->P.size(),
+                          hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+  } else {
+    HIPCHK(hipMemcpyAsync(ctx->d_ptab, ctx->P.data(), sizeof(double*) * ctx->P.size(),
+                          hipMemcpyHostToDevice, ctx->st));
+  }
+  return TDGL_OK;
+}
+
+void set_uniform_coeffs(tdgl_ctx* ctx, double kappa) {
+  Mat& m = ctx->mt;
+  const double k2 = kappa * kappa;  // sparse_operators.py:131 (kappa_m**2)
+  m.cx = k2 / m.hx2;
+  m.cy = k2 / m.hy2;
+  m.cz = ctx->g.is3d ? k2 / m.hz2 : 0.0;
+  m.dgx = (-2.0 * (m.cx + m.cz)) + (-2.0 * (m.cx + m.cy));
+  m.dgy = (-2.0 * (m.cy + m.cz)) + (-2.0 * (m.cx + m.cy));
+  m.dgz = (-2.0 * (m.cy + m.cz)) + (-2.0 * (m.cx + m.cz));
+}
+
+double state_bytes(const tdgl_ctx* ctx) { return 8.0 * (double)ctx->len; }
+
+}  // namespace
+
+extern "C" {
+
+int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
+  tdgl_ctx* ctx = nullptr;
+  if (!grid || !out) return TDGL_E_INVALID;
+  *out = nullptr;
+  if (grid->Nx < 2 || grid->Ny < 2 || grid->Nz < 1) return TDGL_E_INVALID;  // parameters.py:47-52
+  if (!(grid->hx > 0) || !(grid->hy > 0) || !(grid->hz > 0)) return TDGL_E_INVALID;
+  ctx = new tdgl_ctx();
+  ctx->grid = *grid;
+  HIPCHK(hipGetDevice(&ctx->device));
+  Geo& g = ctx->g;
+  g.Nx = grid->Nx;
+  g.Ny = grid->Ny;
+  g.Nz = grid->Nz;
+  g.is3d = grid->Nz > 1;
+  g.nx = grid->Nx - 1;
+  g.ny = grid->Ny - 1;
+  g.nz = g.is3d ? grid->Nz - 1 : 1;
+  g.per_x = grid->periodic_x != 0;
+  g.per_y = grid->periodic_y != 0;
+  g.per_z = grid->periodic_z != 0;
+  g.mj = grid->Nx + 1;
+  g.mk = (long long)(grid->Nx + 1) * (grid->Ny + 1);
+  g.n = (long long)g.nx * g.ny * g.nz;
+  g.nphi = g.is3d ? 3 : 2;
+  ctx->len = (2 + g.nphi) * g.n;
+
+  Bnd& b = ctx->b;
+  b.hx = grid->hx;
+  b.hy = grid->hy;
+  b.hz = grid->hz;
+  Mat& m = ctx->mt;
+  m.hx2 = grid->hx * grid->hx;
+  m.hy2 = grid->hy * grid->hy;
+  m.hz2 = grid->hz * grid->hz;
+  m.ihx2 = 1.0 / m.hx2;
+  m.ihy2 = 1.0 / m.hy2;
+  m.ihz2 = 1.0 / m.hz2;
+  m.dgpsi = (-2.0 * m.ihx2) + (-2.0 * m.ihy2);
+  if (g.is3d) m.dgpsi = m.dgpsi + (-2.0 * m.ihz2);
+  m.inv_tau = 1.0 / 0.1;
+  set_uniform_coeffs(ctx, grid->kappa);
+
+  // z planes per block: enough blocks to fill 256 CUs several times over
+  if (g.is3d) {
+    const long long tiles = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+    long long kz = tiles * g.nz / 1024;
+    ctx->cfg.kz = (int)std::max(1LL, std::min<long long>(kz, g.nz));
+  }
+
+  const int maxb = std::max(stencil_blocks(g, ctx->cfg), stream_blocks(g));
+  int rc;
+  if ((rc = alloc(ctx, (void**)&ctx->rb.part, sizeof(double) * 4 * (size_t)maxb))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->rb.ticket, sizeof(unsigned int) * 4))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->rb.res, sizeof(double) * kResLen))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_bad, sizeof(int)))) goto bad;
+  if (hipMemset(ctx->rb.ticket, 0, sizeof(unsigned int) * 4) != hipSuccess ||
+      hipMemset(ctx->rb.res, 0, sizeof(double) * kResLen) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_res, sizeof(double) * kResLen, hipHostMallocDefault) != hipSuccess) {
+    rc = TDGL_E_HIP;
+    goto bad;
+  }
+  *out = ctx;
+  return TDGL_OK;
+bad:
+  tdgl_ctx_destroy(ctx);
+  return rc;
+}
+
+int tdgl_ctx_destroy(tdgl_ctx* ctx) {
+  if (!ctx) return TDGL_OK;
+  if (ctx->st) hipStreamSynchronize(ctx->st);
+  else hipDeviceSynchronize();
+  for (void* p : {(void*)ctx->d_flags, (void*)ctx->d_k2, (void*)ctx->d_sc, (void*)ctx->d_bfull,
+                  (void*)ctx->rb.part, (void*)ctx->rb.ticket, (void*)ctx->rb.res,
+                  (void*)ctx->d_bad, (void*)ctx->w_x, (void*)ctx->w_gam, (void*)ctx->w_f,
+                  (void*)ctx->w_r, (void*)ctx->d_ptab})
+    if (p) hipFree(p);
+  for (double* p : ctx->P) hipFree(p);
+  for (double* p : ctx->AP) hipFree(p);
+  for (auto& e : ctx->ev_pool) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  for (auto& e : ctx->pending) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  if (ctx->h_res) hipHostFree(ctx->h_res);
+  delete ctx;
+  return TDGL_OK;
+}
+
+int tdgl_set_stream(tdgl_ctx* ctx, void* s) {
+  if (!ctx) return TDGL_E_INVALID;
+  ctx->st = (hipStream_t)s;
+  return TDGL_OK;
+}
+
+const char* tdgl_last_error(const tdgl_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+int64_t tdgl_state_len(const tdgl_ctx* ctx) { return ctx ? ctx->len : -1; }
+int64_t tdgl_n_interior(const tdgl_ctx* ctx) { return ctx ? ctx->g.n : -1; }
+
+int tdgl_set_material(tdgl_ctx* ctx, const uint8_t* link_bits_full, const double* kappa_int,
+                      const double* sc_int) {
+  if (!ctx) return TDGL_E_INVALID;
+  const Geo& g = ctx->g;
+  const long long dimx = g.mk * (g.is3d ? (g.Nz + 1) : 1);
+  const long long n = g.n;
+  Mat& m = ctx->mt;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  // reference interior order -> internal order
+  auto int_of_ref = [&](long long r) {
+    const long long k1 = r % g.nz, t = r / g.nz, j1 = t % g.ny, i1 = t / g.ny;
+    return i1 + g.nx * (j1 + g.ny * k1);
+  };
+  auto full_of_ref = [&](long long r) {
+    const long long k1 = r % g.nz, t = r / g.nz, j1 = t % g.ny, i1 = t / g.ny;
+    return (i1 + 1) + (long long)g.mj * (j1 + 1) + (g.is3d ? g.mk * (k1 + 1) : 0LL);
+  };
+  std::vector<uint8_t> flags;
+  bool any_flags = false;
+  if (link_bits_full) {
+    flags.assign(link_bits_full, link_bits_full + dimx);
+    for (auto& f : flags) f &= (kHoleX | kHoleY | kHoleZ);
+    any_flags = true;
+  }
+  // kappa: uniform -> scalar coefficients, otherwise kappa^2 per node
+  if (m.k2) {
+    hipFree(ctx->d_k2);
+    ctx->d_k2 = nullptr;
+    m.k2 = nullptr;
+  }
+  if (kappa_int) {
+    bool uniform = true;
+    for (long long r = 1; r < n && uniform; ++r) uniform = (kappa_int[r] == kappa_int[0]);
+    if (uniform) {
+      set_uniform_coeffs(ctx, kappa_int[0]);
+    } else {
+      std::vector<double> k2(n);
+      for (long long r = 0; r < n; ++r) k2[int_of_ref(r)] = kappa_int[r] * kappa_int[r];
+      int rc;
+      if ((rc = alloc(ctx, (void**)&ctx->d_k2, sizeof(double) * n))) return rc;
+      HIPCHK(hipMemcpy(ctx->d_k2, k2.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      m.k2 = ctx->d_k2;
+    }
+  } else {
+    set_uniform_coeffs(ctx, ctx->grid.kappa);
+  }
+  // sc: binary -> insulator bits, otherwise per node
+  if (ctx->d_sc) {
+    hipFree(ctx->d_sc);
+    ctx->d_sc = nullptr;
+  }
+  m.sc = nullptr;
+  m.sc_bits = 0;
+  m.has_mat = sc_int != nullptr;
+  if (sc_int) {
+    bool binary = true;
+    for (long long r = 0; r < n && binary; ++r) binary = (sc_int[r] == 0.0 || sc_int[r] == 1.0);
+    if (binary) {
+      if (!any_flags) {
+        flags.assign(dimx, 0);
+        any_flags = true;
+      }
+      for (long long r = 0; r < n; ++r)
+        if (sc_int[r] == 0.0) flags[full_of_ref(r)] |= kInsulator;
+      m.sc_bits = 1;
+    } else {
+      std::vector<double> sc(n);
+      for (long long r = 0; r < n; ++r) sc[int_of_ref(r)] = sc_int[r];
+      int rc;
+      if ((rc = alloc(ctx, (void**)&ctx->d_sc, sizeof(double) * n))) return rc;
+      HIPCHK(hipMemcpy(ctx->d_sc, sc.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      m.sc = ctx->d_sc;
+    }
+  }
+  if (ctx->d_flags) {
+    hipFree(ctx->d_flags);
+    ctx->d_flags = nullptr;
+  }
+  m.flags = nullptr;
+  if (any_flags) {
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_flags, (size_t)dimx))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_flags, flags.data(), (size_t)dimx, hipMemcpyHostToDevice));
+    m.flags = ctx->d_flags;
+  }
+  return TDGL_OK;
+}
+
+int tdgl_set_field(tdgl_ctx* ctx, double bx, double by, double bz) {
+  if (!ctx) return TDGL_E_INVALID;
+  Bnd& b = ctx->b;
+  b.cx = (bx * b.hy) * b.hz;
+  b.cy = (by * b.hz) * b.hx;
+  b.cz = (bz * b.hx) * b.hy;
+  b.bx = b.by = b.bz = nullptr;
+  return TDGL_OK;
+}
+
+int tdgl_set_field_vectors(tdgl_ctx* ctx, const double* bx, const double* by, const double* bz) {
+  if (!ctx || !bx || !by || !bz) return TDGL_E_INVALID;
+  const Geo& g = ctx->g;
+  const long long dimx = g.mk * (g.is3d ? (g.Nz + 1) : 1);
+  if (!ctx->d_bfull) {
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_bfull, sizeof(double) * 3 * dimx))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipMemcpy(ctx->d_bfull, bx, sizeof(double) * dimx, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_bfull + dimx, by, sizeof(double) * dimx, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_bfull + 2 * dimx, bz, sizeof(double) * dimx, hipMemcpyHostToDevice));
+  ctx->b.bx = ctx->d_bfull;
+  ctx->b.by = ctx->d_bfull + dimx;
+  ctx->b.bz = ctx->d_bfull + 2 * dimx;
+  return TDGL_OK;
+}
+
+int tdgl_permute_in(tdgl_ctx* ctx, const void* ref, double* out) {
+  if (!ctx || !ref || !out) return TDGL_E_INVALID;
+  HIPCHK(hipMemsetAsync(ctx->d_bad, 0, sizeof(int), ctx->st));
+  launch_permute_in(ctx->g, ref, out, ctx->d_bad, ctx->st);
+  LAUNCHCHK();
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, ctx->d_bad, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  if (bad) return fail(ctx, TDGL_E_COMPLEX_PHI, "link variables must be real (Im(phi) == 0)");
+  return TDGL_OK;
+}
+
+int tdgl_permute_out(tdgl_ctx* ctx, const double* in, void* ref) {
+  if (!ctx || !ref || !in) return TDGL_E_INVALID;
+  launch_permute_out(ctx->g, in, ref, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
+int tdgl_rhs(tdgl_ctx* ctx, const double* x, double* f) {
+  if (!ctx || !x || !f) return TDGL_E_INVALID;
+  Epi e{};
+  e.out0 = f;
+  launch_stencil(M_RHS, false, ctx->g, ctx->b, ctx->mt, x, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
+int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const double* p, double eps,
+                     double dt, double* ap) {
+  if (!ctx || !x || !f0 || !p || !ap) return TDGL_E_INVALID;
+  Epi e{};
+  e.out0 = ap;
+  e.f0 = f0;
+  e.ap0 = nullptr;
+  e.hdt = dt / 2.0;
+  e.inv_eps = 1.0 / eps;
+  launch_stencil(M_JVP, true, ctx->g, ctx->b, ctx->mt, x, p, eps, e, ctx->cfg, ctx->rb, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
+int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {
+  if (!ctx || !a || !b || !out) return TDGL_E_INVALID;
+  launch_dot(ctx->g, a, b, ctx->rb, ctx->st);
+  LAUNCHCHK();
+  int rc;
+  if ((rc = readback(ctx, RS_DOT, 1))) return rc;
+  *out = ctx->h_res[RS_DOT];
+  return TDGL_OK;
+}
+
+int tdgl_norm_inf(tdgl_ctx* ctx, const double* a, double* out) {
+  if (!ctx || !a || !out) return TDGL_E_INVALID;
+  launch_norm_inf(ctx->g, a, ctx->rb, ctx->st);
+  LAUNCHCHK();
+  int rc;
+  if ((rc = readback(ctx, RS_DOT, 1))) return rc;
+  *out = ctx->h_res[RS_DOT];
+  return TDGL_OK;
+}
+
+int tdgl_euler_step(tdgl_ctx* ctx, double* X, double dt) {
+  if (!ctx || !X) return TDGL_E_INVALID;
+  int rc;
+  if ((rc = ensure_work(ctx))) return rc;
+  Epi e{};
+  e.out0 = ctx->w_x;
+  e.dt = dt;
+  launch_stencil(M_EULER, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
+  return TDGL_OK;
+}
+
+int tdgl_observables(tdgl_ctx* ctx, const double* x, double* bout, double* jout) {
+  if (!ctx || !x) return TDGL_E_INVALID;
+  launch_observables(ctx->g, ctx->b, ctx->mt, x, bout, jout, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Newton residual at the current iterate w_x: f, r = -g, ||x||^2, ||r||^2, ||g||_inf.
+int newton_eval(tdgl_ctx* ctx, double hdt, tdgl_step_stats* s) {
+  Epi e{};
+  e.out0 = ctx->w_f;
+  e.out1 = ctx->w_r;
+  e.gam = ctx->w_gam;
+  e.hdt = hdt;
+  const double bytes = 4.0 * state_bytes(ctx);
+  const int tok = prof_begin(ctx, TDGL_K_NEWTON, bytes);
+  launch_stencil(M_NEWTON, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0, e, ctx->cfg,
+                 ctx->rb, ctx->st);
+  prof_end(ctx, tok);
+  LAUNCHCHK();
+  s->rhs_evals++;
+  s->bytes_moved += bytes;
+  return TDGL_OK;
+}
+
+// tgcr_matrix_free_trap (tgcr.py:96-156) on (I - dt/2 J) dx = r, r = -g held in
+// w_r, linearised at w_x with f(w_x) in w_f.  On success w_x += dx.
+// Returns TDGL_OK and sets *ok; *ok = 0 is the reference's empty-array return.
+int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double xx, int* ok,
+         tdgl_step_stats* s) {
+  const Geo& g = ctx->g;
+  const double S = state_bytes(ctx);
+  const long long N = (1 + g.nphi) * g.n;  // complex entries of the reference state
+  long long maxk = o->tgcr_max_iter;
+  if (maxk <= 0) maxk = std::max<long long>(N, llround(0.2 * (double)N));
+  const double r0 = sqrt(rr);
+  double rk = r0;
+  int nvec = 0, k = 0, rc;
+  *ok = 1;
+  if (r0 == 0.0) {  // tgcr.py:118-119: x = 0
+    s->tgcr_iters_last = 0;
+    return TDGL_OK;
+  }
+  const double xn = sqrt(xx);
+  while (rk / r0 > o->tol_gcr && k < maxk) {
+    ++k;
+    const int i = nvec;
+    if ((rc = ensure_vec(ctx, i))) return rc;
+    const double eps = o->eps_mf * (1.0 + xn) / rk;  // tgcr.py:129 (||pk|| == ||r||)
+    Epi e{};
+    e.out0 = ctx->AP[i];
+    e.f0 = ctx->w_f;
+    e.ap0 = (i > 0) ? ctx->AP[0] : nullptr;
+    e.hdt = hdt;
+    e.inv_eps = 1.0 / eps;
+    // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
+    double by = (i > 0 ? 5.0 : 4.0) * S;
+    int tok = prof_begin(ctx, TDGL_K_JVP, by);
+    launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, eps, e, ctx->cfg, ctx->rb, ctx->st);
+    prof_end(ctx, tok);
+    LAUNCHCHK();
+    s->jvp_evals++;
+    s->bytes_moved += by;
+    for (int j = 1; j < i; ++j) {  // read Ap, Ap_{j-1}, Ap_j; write Ap
+      tok = prof_begin(ctx, TDGL_K_MGS, 4.0 * S);
+      launch_mgs_pass(g, ctx->AP[i], ctx->AP[j - 1], RS_BETA + j - 1, ctx->AP[j], RS_BETA + j,
+                      ctx->rb, ctx->st);
+      prof_end(ctx, tok);
+      s->bytes_moved += 4.0 * S;
+    }
+    if (i > 0) {  // read Ap, Ap_{i-1}; write Ap
+      tok = prof_begin(ctx, TDGL_K_MGS_LAST, 3.0 * S);
+      launch_mgs_last(g, ctx->AP[i], ctx->AP[i - 1], RS_BETA + i - 1, ctx->rb, ctx->st);
+      prof_end(ctx, tok);
+      s->bytes_moved += 3.0 * S;
+    }
+    // read r, p_0..p_{i-1}, Ap; write p_i, Ap
+    tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);
+    launch_final_p(g, ctx->w_r, ctx->d_ptab, i, ctx->AP[i], ctx->P[i], RS_ALPHA + i, ctx->rb, ctx->st);
+    prof_end(ctx, tok);
+    tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
+    launch_update_r(g, ctx->w_r, ctx->AP[i], RS_ALPHA + i, ctx->rb, ctx->st);
+    prof_end(ctx, tok);
+    LAUNCHCHK();
+    if ((rc = readback(ctx, RS_BRK, 2))) return rc;
+    if (ctx->h_res[RS_BRK] != 0.0) break;  // tgcr.py:141-142
+    s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
+    rk = sqrt(ctx->h_res[RS_RR]);
+    nvec = i + 1;
+    if (!std::isfinite(rk)) return fail(ctx, TDGL_E_NONFINITE, "TGCR residual is not finite");
+  }
+  s->tgcr_iters_last = k;
+  s->tgcr_iters_total += k;
+  if (rk > o->tol_gcr * r0) {  // tgcr.py:154-155
+    *ok = 0;
+    return TDGL_OK;
+  }
+  const int tok = prof_begin(ctx, TDGL_K_ACCUM, (double)(nvec + 2) * S);  // read x, p_i; write x
+  launch_accum(g, ctx->w_x, ctx->d_ptab, nvec, ctx->rb, ctx->st);
+  prof_end(ctx, tok);
+  LAUNCHCHK();
+  s->bytes_moved += (double)(nvec + 2) * S;
+  return TDGL_OK;
+}
+
+}  // namespace
+
+extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
+                              tdgl_step_stats* s) {
+  if (!ctx || !X || !o || !s) return TDGL_E_INVALID;
+  if (o->precision != TDGL_PREC_F64) return fail(ctx, TDGL_E_INVALID, "only TDGL_PREC_F64 is built");
+  memset(s, 0, sizeof(*s));
+  int rc;
+  if ((rc = ensure_work(ctx))) return rc;
+  if ((rc = ensure_vec(ctx, 0))) return rc;
+  const double hdt = dt / 2.0;
+  const double S = state_bytes(ctx);
+  // predictor + gamma (integrators.py:156-158)
+  Epi e{};
+  e.out0 = ctx->w_x;
+  e.out1 = ctx->w_gam;
+  e.dt = dt;
+  e.hdt = hdt;
+  const int tok = prof_begin(ctx, TDGL_K_PRED, 3.0 * S);  // read X; write x_pred, gamma
+  launch_stencil(M_PRED, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  prof_end(ctx, tok);
+  LAUNCHCHK();
+  s->rhs_evals++;
+  s->bytes_moved += 3.0 * S;
+  if ((rc = newton_eval(ctx, hdt, s))) return rc;
+  if ((rc = readback(ctx, RS_RR, 3))) return rc;
+  for (int it = 1; it <= o->newton_max_iter; ++it) {
+    const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
+    int ok = 0;
+    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
+    if (!ok) {  // newton.py:102-106
+      s->tgcr_failed = 1;
+      s->newton_iters = it;
+      s->converged = 0;
+      return TDGL_OK;
+    }
+    if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
+    if ((rc = readback(ctx, RS_RR, 4))) return rc;
+    s->g_inf = ctx->h_res[RS_GINF];
+    s->dx_inf = ctx->h_res[RS_DXINF];
+    if (!std::isfinite(s->g_inf) || !std::isfinite(s->dx_inf))
+      return fail(ctx, TDGL_E_NONFINITE, "Newton residual is not finite");
+    if (s->g_inf <= o->newton_tol_f && s->dx_inf <= o->newton_tol_dx) {  // newton.py:118
+      HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
+      s->bytes_moved += 2.0 * S;
+      s->converged = 1;
+      s->newton_iters = it;
+      return TDGL_OK;
+    }
+  }
+  s->converged = 0;
+  s->newton_iters = o->newton_max_iter;
+  return TDGL_OK;
+}
+
+extern "C" int tdgl_profile_enable(tdgl_ctx* ctx, int enable) {
+  if (!ctx) return TDGL_E_INVALID;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  prof_harvest(ctx);
+  ctx->prof = enable != 0;
+  for (int c = 0; c < TDGL_K_COUNT; ++c) {
+    ctx->prof_ms[c] = 0.0;
+    ctx->prof_n[c] = 0;
+    ctx->prof_bytes[c] = 0.0;
+  }
+  return TDGL_OK;
+}
+
+extern "C" int tdgl_profile_read(tdgl_ctx* ctx, int cls, double* ms, int64_t* n, double* bytes) {
+  if (!ctx || cls < 0 || cls >= TDGL_K_COUNT) return TDGL_E_INVALID;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  prof_harvest(ctx);
+  if (ms) *ms = ctx->prof_ms[cls];
+  if (n) *n = ctx->prof_n[cls];
+  if (bytes) *bytes = ctx->prof_bytes[cls];
+  return TDGL_OK;
+}

@@ -1,0 +1,272 @@
+// Device-side building blocks shared by the TDGL kernels (gfx950 / CDNA4).
+//
+// Layout (internal): interior nodes only, i fastest:
+//   q = (i-1) + nx*((j-1) + ny*(k-1))        (3-D; 2-D uses k = 0 and no k term)
+// A state buffer of doubles is [psi re/im interleaved: 2n][phi_x: n][phi_y: n][phi_z: n].
+// Ghost (boundary) values are never stored: every kernel that needs them
+// rebuilds them from the interior values with the boundary rules of
+// tdgl3d's _apply_boundary_conditions (src/tdgl3d/physics/rhs.py:54-139).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tdgl_types.hpp"
+
+namespace tdgl {
+
+constexpr int kRedThreads = 256;  // block size of every reducing kernel
+
+__device__ __forceinline__ long long qix(const Geo& g, int i, int j, int k) {
+  return (long long)(i - 1) + (long long)g.nx * ((long long)(j - 1) + (long long)g.ny * (long long)(g.is3d ? k - 1 : 0));
+}
+
+__device__ __forceinline__ long long mfull(const Geo& g, int i, int j, int k) {
+  return (long long)i + (long long)g.mj * j + (g.is3d ? g.mk * (long long)k : 0LL);
+}
+
+// Raw interior values of a state, optionally perturbed: x + eps*p, the
+// argument of f(x_lin + epsilon*pk) in tgcr.py:130.  The two roundings of the
+// reference (eps*p, then +x) are kept: this file is compiled with
+// -ffp-contract=off.
+template <typename T, bool PERT>
+struct Src {
+  const T* x;
+  const T* p;
+  T eps;
+  long long n;
+  __device__ __forceinline__ T ld(long long off) const {
+    T v = x[off];
+    if (PERT) v = v + eps * p[off];
+    return v;
+  }
+  __device__ __forceinline__ void psi(long long q, T& re, T& im) const {
+    re = ld(2 * q);
+    im = ld(2 * q + 1);
+  }
+  __device__ __forceinline__ T phx(long long q) const { return ld(2 * n + q); }
+  __device__ __forceinline__ T phy(long long q) const { return ld(3 * n + q); }
+  __device__ __forceinline__ T phz(long long q) const { return ld(4 * n + q); }
+};
+
+// Effective (post-boundary-condition) values at one full-grid node.
+template <typename T>
+struct Cell {
+  T pr, pi, x, y, z;
+};
+
+// Field terms that the ghost rules add: cz = (Bz*hx)*hy, cy = (By*hz)*hx,
+// cx = (Bx*hy)*hz, exactly as rhs.py:95-127 groups them.
+template <typename T>
+__device__ __forceinline__ T fld_cz(const Geo& g, const Bnd& b, long long m) {
+  return b.bz ? (T)((b.bz[m] * b.hx) * b.hy) : (T)b.cz;
+}
+template <typename T>
+__device__ __forceinline__ T fld_cy(const Geo& g, const Bnd& b, long long m) {
+  return b.by ? (T)((b.by[m] * b.hz) * b.hx) : (T)b.cy;
+}
+template <typename T>
+__device__ __forceinline__ T fld_cx(const Geo& g, const Bnd& b, long long m) {
+  return b.bx ? (T)((b.bx[m] * b.hy) * b.hz) : (T)b.cx;
+}
+
+// The value tdgl3d's eval_f sees at full-grid node (i, j, k) after
+// _apply_boundary_conditions:
+//  (1) phi_x := 0 on i in {0, Nx-1}, phi_y on j in {0, Ny-1}, phi_z on k in {0, Nz-1}
+//      (rhs.py:73-76; this also hits the interior planes i = Nx-1 etc.);
+//  (2) face-inner ghosts from the zeroed copies (rhs.py:85-127): psi copied
+//      from the first interior layer (the e^{-+i phi} factor is e^0 there),
+//      transverse links shifted by the applied-field flux, or the periodic
+//      image;
+//  (3) edges and corners stay 0;
+//  (4) hole-boundary links := 0 last (rhs.py:132-137).
+template <typename T, bool IS3D, bool PERT>
+__device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, const Src<T, PERT>& s,
+                            int i, int j, int k) {
+  Cell<T> c{T(0), T(0), T(0), T(0), T(0)};
+  if (i < 0 || i > g.Nx || j < 0 || j > g.Ny) return c;
+  if (IS3D && (k < 0 || k > g.Nz)) return c;
+  const bool bi = (i == 0) || (i == g.Nx);
+  const bool bj = (j == 0) || (j == g.Ny);
+  const bool bk = IS3D && ((k == 0) || (k == g.Nz));
+  const int nb = (int)bi + (int)bj + (int)bk;
+  if (nb > 1) return c;  // edge or corner: never assigned
+  if (nb == 0) {
+    const long long q = qix(g, i, j, k);
+    s.psi(q, c.pr, c.pi);
+    c.x = (i == g.Nx - 1) ? T(0) : s.phx(q);
+    c.y = (j == g.Ny - 1) ? T(0) : s.phy(q);
+    if (IS3D) c.z = (k == g.Nz - 1) ? T(0) : s.phz(q);
+  } else {
+    const long long m = mfull(g, i, j, k);
+    if (bi) {
+      if (g.per_x) {  // rhs.py:85-89
+        const long long q = qix(g, (i == 0) ? g.Nx - 1 : 1, j, k);
+        s.psi(q, c.pr, c.pi);
+        c.x = (i == 0 || g.Nx - 1 == 1) ? T(0) : s.phx(q);
+      } else {        // rhs.py:91-98
+        const int si = (i == 0) ? 1 : g.Nx - 1;
+        const long long q = qix(g, si, j, k);
+        s.psi(q, c.pr, c.pi);
+        const T cz = fld_cz<T>(g, b, m);
+        const T sy = (j == g.Ny - 1) ? T(0) : s.phy(q);
+        c.y = (i == 0) ? (-cz + sy) : (cz + sy);
+        if (IS3D) {
+          const T cy = fld_cy<T>(g, b, m);
+          const T sz = (k == g.Nz - 1) ? T(0) : s.phz(q);
+          c.z = (i == 0) ? (cy + sz) : (-cy + sz);
+        }
+      }
+    } else if (bj) {
+      if (g.per_y) {  // rhs.py:101-105
+        const long long q = qix(g, i, (j == 0) ? g.Ny - 1 : 1, k);
+        s.psi(q, c.pr, c.pi);
+        c.y = (j == 0 || g.Ny - 1 == 1) ? T(0) : s.phy(q);
+      } else {        // rhs.py:107-112
+        const int sj = (j == 0) ? 1 : g.Ny - 1;
+        const long long q = qix(g, i, sj, k);
+        s.psi(q, c.pr, c.pi);
+        const T cz = fld_cz<T>(g, b, m);
+        const T sx = (i == g.Nx - 1) ? T(0) : s.phx(q);
+        c.x = (j == 0) ? (cz + sx) : (-cz + sx);
+        if (IS3D) {
+          const T cx = fld_cx<T>(g, b, m);
+          const T sz = (k == g.Nz - 1) ? T(0) : s.phz(q);
+          c.z = (j == 0) ? (-cx + sz) : (cx + sz);
+        }
+      }
+    } else {  // bk (3-D only)
+      if (g.per_z) {  // rhs.py:116-120
+        const long long q = qix(g, i, j, (k == 0) ? g.Nz - 1 : 1);
+        s.psi(q, c.pr, c.pi);
+        c.z = (k == 0 || g.Nz - 1 == 1) ? T(0) : s.phz(q);
+      } else {        // rhs.py:122-127
+        const int sk = (k == 0) ? 1 : g.Nz - 1;
+        const long long q = qix(g, i, j, sk);
+        s.psi(q, c.pr, c.pi);
+        const T cy = fld_cy<T>(g, b, m);
+        const T cx = fld_cx<T>(g, b, m);
+        const T sx = (i == g.Nx - 1) ? T(0) : s.phx(q);
+        const T sy = (j == g.Ny - 1) ? T(0) : s.phy(q);
+        c.x = (k == 0) ? (-cy + sx) : (cy + sx);
+        c.y = (k == 0) ? (cx + sy) : (-cx + sy);
+      }
+    }
+  }
+  if (flags) {  // rhs.py:132-137 (applied last)
+    const uint8_t f = flags[mfull(g, i, j, k)];
+    if (f & kHoleX) c.x = T(0);
+    if (f & kHoleY) c.y = T(0);
+    if (IS3D && (f & kHoleZ)) c.z = T(0);
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic grid reductions.
+//
+// Every block reduces its values in a fixed tree (wave shuffles, then the
+// waves in order), publishes them to `part`, and the last block to arrive
+// (ticket counter) sums the partials in block order.  The result does not
+// depend on block scheduling.  Partials and the counter go through
+// agent-scope atomics, so they are coherent across the XCDs' L2 caches.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ void st_agent(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                     (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_agent(const double* p) {
+  unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)u);
+}
+
+// NS sums followed by NM maxima, reduced over the whole grid.  Block size
+// must be kRedThreads.  Returns true in the one block that finalised; in
+// that block out[] holds the grid-wide results for every thread.
+template <int NS, int NM>
+struct GridRed {
+  static constexpr int NV = NS + NM;
+  double* part;          // [gridDim.x][NV]
+  unsigned int* ticket;  // zero between launches
+
+  __device__ bool run(double (&v)[NV], double (&out)[NV]) const {
+    __shared__ double sh[kRedThreads / 64][NV > 0 ? NV : 1];
+    __shared__ unsigned int s_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      double r = (a < NS) ? wave_sum(v[a]) : wave_max(v[a]);
+      if (lane == 0) sh[wid][a] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int a = 0; a < NV; ++a) {
+        double r = sh[0][a];
+        for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
+        st_agent(&part[(long long)blockIdx.x * NV + a], r);
+      }
+      __threadfence();
+      unsigned int t = atomicAdd(ticket, 1u);
+      s_last = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int a = 0; a < NV; ++a) acc[a] = (a < NS) ? 0.0 : -1.0;
+    for (unsigned int bi = threadIdx.x; bi < gridDim.x; bi += kRedThreads) {
+#pragma unroll
+      for (int a = 0; a < NV; ++a) {
+        double r = ld_agent(&part[(long long)bi * NV + a]);
+        acc[a] = (a < NS) ? acc[a] + r : fmax(acc[a], r);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      double r = (a < NS) ? wave_sum(acc[a]) : wave_max(acc[a]);
+      if (lane == 0) sh[wid][a] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      double r = sh[0][a];
+      for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
+      out[a] = r;
+    }
+    if (threadIdx.x == 0)
+      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+};
+
+// Block-index remap so that consecutive logical tiles land on the same XCD
+// (hardware dispatches block b to XCD b % 8), keeping stencil halos inside one L2.
+__device__ __forceinline__ unsigned int xcd_remap(unsigned int b, unsigned int nblocks) {
+  const unsigned int nx = 8;
+  const unsigned int per = (nblocks + nx - 1) / nx;
+  const unsigned int xcd = b % nx, slot = b / nx;
+  const unsigned int full = nblocks % nx == 0 ? nx : nblocks % nx;  // XCDs holding `per` blocks
+  unsigned int base = (xcd < full) ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + slot;
+}
+
+}  // namespace tdgl

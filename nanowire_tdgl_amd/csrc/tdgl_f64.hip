@@ -1,0 +1,674 @@
+// FP64 kernels of the trapezoidal TDGL step (plain method).
+//
+// Compiled with -ffp-contract=off: every product and sum is rounded on its
+// own, in the order tdgl3d's numpy/scipy code performs it, so the GPU result
+// differs from the reference only by the ulp-level differences of sin/cos and
+// of reduction order.
+//
+// Kernels
+//   k_stencil  : the fused RHS (psi Laplacian through the Peierls phases,
+//                link curl-curl, supercurrent, relaxation term, ghost rules,
+//                hole masking) with five epilogues: plain RHS, trapezoidal
+//                predictor, Newton residual, finite-difference JVP, Euler.
+//                xy tile of 64x4 nodes + 1-node halo staged in LDS, marching
+//                through z with a 3-plane ring.
+//   k_mgs_*, k_final_p, k_update_r, k_accum : TGCR with modified Gram-Schmidt
+//                (tgcr.py:135-152), vectors streamed node by node.
+//   k_permute_* : reference complex128 state <-> internal layout.
+//   k_obs      : B = curl(phi) and J_s (check-path observables).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "tdgl_device.hpp"
+#include "tdgl_launch.hpp"
+
+namespace tdgl {
+
+constexpr int TX = 64, TY = 4;
+constexpr int CX = TX + 2, CY = TY + 2, CELLS = CX * CY;
+constexpr int BS = TX * TY;
+static_assert(BS == kRedThreads, "stencil block must match the reduction block");
+
+// complex multiply-accumulate with numpy's formula (ar*br - ai*bi, ar*bi + ai*br)
+__device__ __forceinline__ void cmac(double& sr, double& si, double cr, double ci, double vr,
+                                     double vi) {
+  const double tr = cr * vr - ci * vi;
+  const double ti = cr * vi + ci * vr;
+  sr = sr + tr;
+  si = si + ti;
+}
+
+// Im[e^{-i phi} * conj(psi) * psi1] evaluated left to right as numpy does
+// (sparse_operators.py:245, :275, :308).
+__device__ __forceinline__ double supercurrent(double cs, double sn, double pr, double pi,
+                                               double qr, double qi) {
+  const double er = cs, ei = -sn;        // exp(-1j*phi)
+  const double cr = pr, ci = -pi;        // conj(psi)
+  const double tr = er * cr - ei * ci;
+  const double ti = er * ci + ei * cr;
+  return tr * qi + ti * qr;              // imag(t * psi1)
+}
+
+template <bool IS3D, bool PERT, int MODE>
+__global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double, PERT> s, Epi e,
+                                                int tiles_x, int tiles_y, int kz, RedBufs rb) {
+  constexpr int NSLOT = IS3D ? 3 : 1;
+  __shared__ double l_pr[NSLOT][CELLS], l_pi[NSLOT][CELLS], l_x[NSLOT][CELLS], l_y[NSLOT][CELLS],
+      l_z[IS3D ? NSLOT : 1][IS3D ? CELLS : 1];
+
+  const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile_x = (int)(L % (unsigned)tiles_x);
+  const unsigned int rest = L / (unsigned)tiles_x;
+  const int tile_y = (int)(rest % (unsigned)tiles_y);
+  const int zc = (int)(rest / (unsigned)tiles_y);
+  const int i0 = 1 + tile_x * TX, j0 = 1 + tile_y * TY;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
+  int kb = 0, ke = 1;
+  if (IS3D) {
+    kb = 1 + zc * kz;
+    ke = min(kb + kz, g.Nz);
+  }
+
+  constexpr int NS = (MODE == M_NEWTON) ? 2 : (MODE == M_JVP ? 1 : 0);
+  constexpr int NM = (MODE == M_NEWTON) ? 1 : 0;
+  double red[NS + NM > 0 ? NS + NM : 1];
+#pragma unroll
+  for (int a = 0; a < (NS + NM > 0 ? NS + NM : 1); ++a) red[a] = 0.0;
+
+  auto load_plane = [&](int kp, int slot) {
+    for (int c = threadIdx.x; c < CELLS; c += BS) {
+      const int ci = c % CX, cj = c / CX;
+      const Cell<double> v = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + ci, j0 - 1 + cj, kp);
+      l_pr[slot][c] = v.pr;
+      l_pi[slot][c] = v.pi;
+      l_x[slot][c] = v.x;
+      l_y[slot][c] = v.y;
+      if (IS3D) l_z[slot][c] = v.z;
+    }
+  };
+
+  if (IS3D) {
+    load_plane(kb - 1, (kb - 1) % 3);
+    load_plane(kb, kb % 3);
+  } else {
+    load_plane(0, 0);
+  }
+
+  for (int k = kb; k < ke; ++k) {
+    if (IS3D) load_plane(k + 1, (k + 1) % 3);
+    __syncthreads();
+    if (act) {
+      const int s0 = IS3D ? k % 3 : 0;
+      const int sm = IS3D ? (k + 2) % 3 : 0;  // k - 1
+      const int sp = IS3D ? (k + 1) % 3 : 0;  // k + 1
+      const int c0 = (ty + 1) * CX + (tx + 1);
+      const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
+      const long long q = qix(g, i, j, k);
+
+      // psi and links at the node and its neighbours (post-BC values)
+      const double pr = l_pr[s0][c0], pi = l_pi[s0][c0];
+      const double ex = l_x[s0][c0], ey = l_y[s0][c0];
+      const double ez = IS3D ? l_z[s0][c0] : 0.0;
+
+      // Peierls factors e^{i phi_a(m)} and e^{-i phi_a(m - a)}
+      double sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0, czp = 1, szm = 0, czm = 1;
+      sincos(ex, &sxp, &cxp_);
+      sincos(l_x[s0][cxm], &sxm, &cxm_);
+      sincos(ey, &syp, &cyp_);
+      sincos(l_y[s0][cym], &sym, &cym_);
+      if (IS3D) {
+        sincos(ez, &szp, &czp);
+        sincos(l_z[sm][c0], &szm, &czm);
+      }
+
+      // ---- dpsi: row m of (LPSIX/hx^2 + LPSIY/hy^2 + LPSIZ/hz^2) @ x, columns
+      // ascending (m-mk, m-mj, m-1, m, m+1, m+mj, m+mk)   rhs.py:220
+      double ar = 0.0, ai = 0.0;
+      if (IS3D) cmac(ar, ai, czm * mt.ihz2, -szm * mt.ihz2, l_pr[sm][c0], l_pi[sm][c0]);
+      cmac(ar, ai, cym_ * mt.ihy2, -sym * mt.ihy2, l_pr[s0][cym], l_pi[s0][cym]);
+      cmac(ar, ai, cxm_ * mt.ihx2, -sxm * mt.ihx2, l_pr[s0][cxm], l_pi[s0][cxm]);
+      cmac(ar, ai, mt.dgpsi, 0.0, pr, pi);
+      cmac(ar, ai, cxp_ * mt.ihx2, sxp * mt.ihx2, l_pr[s0][cxp], l_pi[s0][cxp]);
+      cmac(ar, ai, cyp_ * mt.ihy2, syp * mt.ihy2, l_pr[s0][cyp], l_pi[s0][cyp]);
+      if (IS3D) cmac(ar, ai, czp * mt.ihz2, szp * mt.ihz2, l_pr[sp][c0], l_pi[sp][c0]);
+
+      // FPSI (sparse_operators.py:202-226)
+      const double ss = pr * pr - (-pi) * pi;  // conj(psi)*psi
+      const double t1 = 1.0 - ss;
+      double fr = t1 * pr - 0.0 * pi, fi = t1 * pi + 0.0 * pr;
+      uint8_t fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
+      if (mt.has_mat) {
+        double sc;
+        if (mt.sc_bits) sc = (fl & kInsulator) ? 0.0 : 1.0;
+        else sc = mt.sc[q];
+        const double w = 1.0 - sc;
+        fr = sc * fr - (w * pr) * mt.inv_tau;
+        fi = sc * fi - (w * pi) * mt.inv_tau;
+      }
+      const double dpr = ar + fr, dpi = ai + fi;
+
+      // ---- link equations (rhs.py:223-225 + sparse_operators.py:229-317)
+      double cx, cy, cz, dgx, dgy, dgz;
+      if (mt.k2) {
+        const double k2 = mt.k2[q];
+        cx = k2 / mt.hx2;
+        cy = k2 / mt.hy2;
+        cz = IS3D ? k2 / mt.hz2 : 0.0;
+        dgx = (-2.0 * (cx + cz)) + (-2.0 * (cx + cy));
+        dgy = (-2.0 * (cy + cz)) + (-2.0 * (cx + cy));
+        dgz = (-2.0 * (cy + cz)) + (-2.0 * (cx + cz));
+      } else {
+        cx = mt.cx; cy = mt.cy; cz = mt.cz;
+        dgx = mt.dgx; dgy = mt.dgy; dgz = mt.dgz;
+      }
+      const double c2x = cx + cx, c2y = cy + cy, c2z = cz + cz;
+
+      // dphi_x = (LPHIY + LPHIZ) @ y1 + FPHIX
+      double dx_ = 0.0;
+      if (IS3D) dx_ = dx_ + cz * l_x[sm][c0];
+      dx_ = dx_ + cy * l_x[s0][cym];
+      dx_ = dx_ + c2x * l_x[s0][cxm];
+      dx_ = dx_ + dgx * ex;
+      dx_ = dx_ + c2x * l_x[s0][cxp];
+      dx_ = dx_ + cy * l_x[s0][cyp];
+      if (IS3D) dx_ = dx_ + cz * l_x[sp][c0];
+      double crl = cy * (((-l_y[s0][cxp] + ey) + l_y[s0][cxp - CX]) - l_y[s0][cym]);
+      if (IS3D) crl = crl + cz * (((-l_z[s0][cxp] + ez) + l_z[sm][cxp]) - l_z[sm][c0]);
+      dx_ = dx_ + (crl + supercurrent(cxp_, sxp, pr, pi, l_pr[s0][cxp], l_pi[s0][cxp]));
+
+      // dphi_y = (LPHIX + LPHIZ) @ y2 + FPHIY
+      double dy_ = 0.0;
+      if (IS3D) dy_ = dy_ + cz * l_y[sm][c0];
+      dy_ = dy_ + c2y * l_y[s0][cym];
+      dy_ = dy_ + cx * l_y[s0][cxm];
+      dy_ = dy_ + dgy * ey;
+      dy_ = dy_ + cx * l_y[s0][cxp];
+      dy_ = dy_ + c2y * l_y[s0][cyp];
+      if (IS3D) dy_ = dy_ + cz * l_y[sp][c0];
+      crl = cx * (((-l_x[s0][cyp] + ex) + l_x[s0][cyp - 1]) - l_x[s0][cxm]);
+      if (IS3D) crl = crl + cz * (((-l_z[s0][cyp] + ez) + l_z[sm][cyp]) - l_z[sm][c0]);
+      dy_ = dy_ + (crl + supercurrent(cyp_, syp, pr, pi, l_pr[s0][cyp], l_pi[s0][cyp]));
+
+      // dphi_z = (LPHIX + LPHIY) @ y3 + FPHIZ
+      double dz_ = 0.0;
+      if (IS3D) {
+        dz_ = dz_ + c2z * l_z[sm][c0];
+        dz_ = dz_ + cy * l_z[s0][cym];
+        dz_ = dz_ + cx * l_z[s0][cxm];
+        dz_ = dz_ + dgz * ez;
+        dz_ = dz_ + cx * l_z[s0][cxp];
+        dz_ = dz_ + cy * l_z[s0][cyp];
+        dz_ = dz_ + c2z * l_z[sp][c0];
+        crl = cx * (((-l_x[sp][c0] + ex) + l_x[sp][cxm]) - l_x[s0][cxm]);
+        crl = crl + cy * (((-l_y[sp][c0] + ey) + l_y[sp][cym]) - l_y[s0][cym]);
+        dz_ = dz_ + (crl + supercurrent(czp, szp, pr, pi, l_pr[sp][c0], l_pi[sp][c0]));
+      }
+      // hole-boundary links keep dphi = 0 (rhs.py:229-234)
+      if (fl & kHoleX) dx_ = 0.0;
+      if (fl & kHoleY) dy_ = 0.0;
+      if (IS3D && (fl & kHoleZ)) dz_ = 0.0;
+
+      // ---- epilogues
+      const long long n = g.n;
+      double2* o0 = reinterpret_cast<double2*>(e.out0);
+      if (MODE == M_RHS) {
+        o0[q] = make_double2(dpr, dpi);
+        e.out0[2 * n + q] = dx_;
+        e.out0[3 * n + q] = dy_;
+        if (IS3D) e.out0[4 * n + q] = dz_;
+      } else if (MODE == M_PRED || MODE == M_EULER) {
+        // X_pred = X + dt f ; gamma = X + dt/2 f   (integrators.py:156-158, :82)
+        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
+        const double Xx = s.x[2 * n + q], Xy = s.x[3 * n + q];
+        const double Xz = IS3D ? s.x[4 * n + q] : 0.0;
+        o0[q] = make_double2(X0.x + e.dt * dpr, X0.y + e.dt * dpi);
+        e.out0[2 * n + q] = Xx + e.dt * dx_;
+        e.out0[3 * n + q] = Xy + e.dt * dy_;
+        if (IS3D) e.out0[4 * n + q] = Xz + e.dt * dz_;
+        if (MODE == M_PRED) {
+          double2* o1 = reinterpret_cast<double2*>(e.out1);
+          o1[q] = make_double2(X0.x + e.hdt * dpr, X0.y + e.hdt * dpi);
+          e.out1[2 * n + q] = Xx + e.hdt * dx_;
+          e.out1[3 * n + q] = Xy + e.hdt * dy_;
+          if (IS3D) e.out1[4 * n + q] = Xz + e.hdt * dz_;
+        }
+      } else if (MODE == M_NEWTON) {
+        // f, r = -g with g = x - dt/2 f - gamma (newton.py:98-100, :109-111)
+        o0[q] = make_double2(dpr, dpi);
+        e.out0[2 * n + q] = dx_;
+        e.out0[3 * n + q] = dy_;
+        if (IS3D) e.out0[4 * n + q] = dz_;
+        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
+        const double2 G0 = reinterpret_cast<const double2*>(e.gam)[q];
+        const double gr = (X0.x - e.hdt * dpr) - G0.x;
+        const double gi = (X0.y - e.hdt * dpi) - G0.y;
+        const double Xx = s.x[2 * n + q], Xy = s.x[3 * n + q];
+        const double gx = (Xx - e.hdt * dx_) - e.gam[2 * n + q];
+        const double gy = (Xy - e.hdt * dy_) - e.gam[3 * n + q];
+        double Xz = 0.0, gz = 0.0;
+        if (IS3D) {
+          Xz = s.x[4 * n + q];
+          gz = (Xz - e.hdt * dz_) - e.gam[4 * n + q];
+        }
+        double2* o1 = reinterpret_cast<double2*>(e.out1);
+        o1[q] = make_double2(-gr, -gi);
+        e.out1[2 * n + q] = -gx;
+        e.out1[3 * n + q] = -gy;
+        if (IS3D) e.out1[4 * n + q] = -gz;
+        red[0] += ((X0.x * X0.x + X0.y * X0.y) + Xx * Xx) + (Xy * Xy + Xz * Xz);
+        red[1] += ((gr * gr + gi * gi) + gx * gx) + (gy * gy + gz * gz);
+        red[NS] = fmax(red[NS], fmax(hypot(gr, gi), fmax(fabs(gx), fmax(fabs(gy), fabs(gz)))));
+      } else if (MODE == M_JVP) {
+        // Ap = p - dt/2 * ((f(x+eps p) - f0) * (1/eps))   (tgcr.py:129-133)
+        const double2 F0 = reinterpret_cast<const double2*>(e.f0)[q];
+        const double2 P0 = reinterpret_cast<const double2*>(s.p)[q];
+        const double jr = (dpr - F0.x) * e.inv_eps, ji = (dpi - F0.y) * e.inv_eps;
+        const double jx = (dx_ - e.f0[2 * n + q]) * e.inv_eps;
+        const double jy = (dy_ - e.f0[3 * n + q]) * e.inv_eps;
+        const double apr = P0.x - e.hdt * jr, api = P0.y - e.hdt * ji;
+        const double apx = s.p[2 * n + q] - e.hdt * jx;
+        const double apy = s.p[3 * n + q] - e.hdt * jy;
+        double apz = 0.0;
+        if (IS3D) {
+          const double jz = (dz_ - e.f0[4 * n + q]) * e.inv_eps;
+          apz = s.p[4 * n + q] - e.hdt * jz;
+        }
+        o0[q] = make_double2(apr, api);
+        e.out0[2 * n + q] = apx;
+        e.out0[3 * n + q] = apy;
+        if (IS3D) e.out0[4 * n + q] = apz;
+        if (e.ap0) {
+          const double2 A0 = reinterpret_cast<const double2*>(e.ap0)[q];
+          double az0 = IS3D ? e.ap0[4 * n + q] : 0.0;
+          red[0] += ((apr * A0.x + api * A0.y) + apx * e.ap0[2 * n + q]) +
+                    (apy * e.ap0[3 * n + q] + apz * az0);
+        } else {
+          red[0] += ((apr * apr + api * api) + apx * apx) + (apy * apy + apz * apz);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if constexpr (NS + NM > 0) {
+    GridRed<NS, NM> gr{rb.part, rb.ticket};
+    double out[NS + NM];
+    if (gr.run(red, out) && threadIdx.x == 0) {
+      if (MODE == M_NEWTON) {
+        rb.res[RS_XX] = out[0];
+        rb.res[RS_RR] = out[1];
+        rb.res[RS_GINF] = out[2];
+      } else {  // JVP
+        if (e.ap0) {
+          rb.res[RS_BETA] = out[0];
+        } else {
+          const double nrm = sqrt(out[0]);
+          rb.res[RS_NRM] = nrm;
+          rb.res[RS_INV] = 1.0 / nrm;
+          rb.res[RS_BRK] = (nrm < 1e-14) ? 1.0 : 0.0;
+        }
+      }
+    }
+  }
+}
+
+int stencil_blocks(const Geo& g, const StencilCfg& c) {
+  const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
+  const int tz = g.is3d ? (g.nz + c.kz - 1) / c.kz : 1;
+  return tx * ty * tz;
+}
+
+template <bool IS3D, bool PERT, int MODE>
+static void go_stencil(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* p,
+                       double eps, const Epi& e, const StencilCfg& cfg, const RedBufs& rb,
+                       hipStream_t st) {
+  Src<double, PERT> s{x, p, eps, g.n};
+  const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
+  const int nb = stencil_blocks(g, cfg);
+  hipLaunchKernelGGL((k_stencil<IS3D, PERT, MODE>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty,
+                     cfg.kz, rb);
+}
+
+template <bool IS3D>
+static void go_mode(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
+                    const RedBufs& rb, hipStream_t st) {
+  switch (mode) {
+    case M_RHS: go_stencil<IS3D, false, M_RHS>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+    case M_PRED: go_stencil<IS3D, false, M_PRED>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+    case M_NEWTON: go_stencil<IS3D, false, M_NEWTON>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+    case M_JVP: go_stencil<IS3D, true, M_JVP>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+    case M_EULER: go_stencil<IS3D, false, M_EULER>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+  }
+  (void)pert;
+}
+
+void launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
+                    const RedBufs& rb, hipStream_t st) {
+  if (g.is3d) go_mode<true>(mode, pert, g, b, mt, x, p, eps, e, cfg, rb, st);
+  else go_mode<false>(mode, pert, g, b, mt, x, p, eps, e, cfg, rb, st);
+}
+
+// ---------------------------------------------------------------------------
+// Node-wise vector kernels.  A node carries (psi re, psi im, phi_x, phi_y[, phi_z]).
+// ---------------------------------------------------------------------------
+
+struct V5 {
+  double a, b, x, y, z;
+};
+
+template <bool IS3D>
+__device__ __forceinline__ V5 ld5(const double* v, long long n, long long q) {
+  const double2 p = reinterpret_cast<const double2*>(v)[q];
+  V5 r{p.x, p.y, v[2 * n + q], v[3 * n + q], IS3D ? v[4 * n + q] : 0.0};
+  return r;
+}
+template <bool IS3D>
+__device__ __forceinline__ void st5(double* v, long long n, long long q, const V5& r) {
+  reinterpret_cast<double2*>(v)[q] = make_double2(r.a, r.b);
+  v[2 * n + q] = r.x;
+  v[3 * n + q] = r.y;
+  if (IS3D) v[4 * n + q] = r.z;
+}
+__device__ __forceinline__ double dot5(const V5& u, const V5& v) {
+  return ((u.a * v.a + u.b * v.b) + u.x * v.x) + (u.y * v.y + u.z * v.z);
+}
+// u - c*v per component (numpy: `u - beta * v`)
+__device__ __forceinline__ V5 axmy(const V5& u, double c, const V5& v) {
+  return V5{u.a - c * v.a, u.b - c * v.b, u.x - c * v.x, u.y - c * v.y, u.z - c * v.z};
+}
+__device__ __forceinline__ V5 scal(const V5& u, double c) {
+  return V5{u.a * c, u.b * c, u.x * c, u.y * c, u.z * c};
+}
+
+constexpr int kStreamCap = 4096;
+
+int stream_blocks(const Geo& g) {
+  long long nb = (g.n + kRedThreads - 1) / kRedThreads;
+  return (int)(nb < kStreamCap ? nb : kStreamCap);
+}
+
+#define GRID_STRIDE(q) \
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n; \
+       q += (long long)gridDim.x * kRedThreads)
+
+// Ap -= beta_{j-1} Ap_{j-1};  beta_j = <Ap, Ap_j>      (tgcr.py:135-138)
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_mgs_pass(long long n, double* ap, const double* apm,
+                                                          int bm, const double* apn, int bn,
+                                                          RedBufs rb) {
+  const double beta = rb.res[bm];
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    const V5 a = axmy(ld5<IS3D>(ap, n, q), beta, ld5<IS3D>(apm, n, q));
+    st5<IS3D>(ap, n, q, a);
+    v[0] += dot5(a, ld5<IS3D>(apn, n, q));
+  }
+  GridRed<1, 0> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[bn] = out[0];
+}
+
+// Ap -= beta_last Ap_last;  ||Ap||, 1/||Ap||, breakdown flag   (tgcr.py:138-142)
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_mgs_last(long long n, double* ap, const double* apm,
+                                                          int bm, RedBufs rb) {
+  const double beta = rb.res[bm];
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    const V5 a = axmy(ld5<IS3D>(ap, n, q), beta, ld5<IS3D>(apm, n, q));
+    st5<IS3D>(ap, n, q, a);
+    v[0] += dot5(a, a);
+  }
+  GridRed<1, 0> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) {
+    const double nrm = sqrt(out[0]);
+    rb.res[RS_NRM] = nrm;
+    rb.res[RS_INV] = 1.0 / nrm;
+    rb.res[RS_BRK] = (nrm < 1e-14) ? 1.0 : 0.0;
+  }
+}
+
+// p_k = (r - sum_j beta_j p_j) / ||Ap|| ; Ap_k = Ap / ||Ap|| ; alpha = <r, Ap_k>
+// The p updates of tgcr.py:137 are applied here, per node, in the same j order,
+// which rounds exactly like the reference's vector-by-vector updates.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const double* r,
+                                                         const double* const* ptab, int nprev,
+                                                         double* ap, double* pout, int aslot,
+                                                         RedBufs rb) {
+  if (rb.res[RS_BRK] != 0.0) return;
+  const double inv = rb.res[RS_INV];
+  const double* beta = rb.res + RS_BETA;
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    const V5 rr = ld5<IS3D>(r, n, q);
+    V5 p = rr;
+    for (int jj = 0; jj < nprev; ++jj) p = axmy(p, beta[jj], ld5<IS3D>(ptab[jj], n, q));
+    const V5 a = scal(ld5<IS3D>(ap, n, q), inv);
+    p = scal(p, inv);
+    st5<IS3D>(ap, n, q, a);
+    st5<IS3D>(pout, n, q, p);
+    v[0] += dot5(rr, a);
+  }
+  GridRed<1, 0> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[aslot] = out[0];
+}
+
+// r -= alpha Ap ; ||r||^2   (tgcr.py:150-152)
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_update_r(long long n, double* r, const double* ap,
+                                                          int aslot, RedBufs rb) {
+  if (rb.res[RS_BRK] != 0.0) return;
+  const double alpha = rb.res[aslot];
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    const V5 a = axmy(ld5<IS3D>(r, n, q), alpha, ld5<IS3D>(ap, n, q));
+    st5<IS3D>(r, n, q, a);
+    v[0] += dot5(a, a);
+  }
+  GridRed<1, 0> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_RR] = out[0];
+}
+
+// dx = sum_i alpha_i p_i (the x += alpha p sequence of tgcr.py:148, per node);
+// x += dx (newton.py:108); ||dx||_inf (newton.py:113)
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_accum(long long n, double* x,
+                                                       const double* const* ptab, int nvec,
+                                                       RedBufs rb) {
+  const double* alpha = rb.res + RS_ALPHA;
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    V5 d{0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int ii = 0; ii < nvec; ++ii) {
+      const V5 p = ld5<IS3D>(ptab[ii], n, q);
+      const double a = alpha[ii];
+      d = V5{d.a + a * p.a, d.b + a * p.b, d.x + a * p.x, d.y + a * p.y, d.z + a * p.z};
+    }
+    const V5 xv = ld5<IS3D>(x, n, q);
+    st5<IS3D>(x, n, q, V5{xv.a + d.a, xv.b + d.b, xv.x + d.x, xv.y + d.y, xv.z + d.z});
+    v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
+  }
+  GridRed<0, 1> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DXINF] = out[0];
+}
+
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_dot(long long n, const double* a, const double* b,
+                                                     RedBufs rb) {
+  double v[1] = {0.0};
+  GRID_STRIDE(q) { v[0] += dot5(ld5<IS3D>(a, n, q), ld5<IS3D>(b, n, q)); }
+  GridRed<1, 0> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DOT] = out[0];
+}
+
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_norm_inf(long long n, const double* a, RedBufs rb) {
+  double v[1] = {0.0};
+  GRID_STRIDE(q) {
+    const V5 d = ld5<IS3D>(a, n, q);
+    v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
+  }
+  GridRed<0, 1> gr{rb.part, rb.ticket};
+  double out[1];
+  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DOT] = out[0];
+}
+
+#define LAUNCH_V(kern, ...)                                                                 \
+  do {                                                                                      \
+    if (g.is3d)                                                                             \
+      hipLaunchKernelGGL(kern<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
+    else                                                                                    \
+      hipLaunchKernelGGL(kern<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
+  } while (0)
+
+void launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int bm, const double* ap_next,
+                     int bn, const RedBufs& rb, hipStream_t st) {
+  LAUNCH_V(k_mgs_pass, g.n, ap, ap_prev, bm, ap_next, bn, rb);
+}
+void launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int bm, const RedBufs& rb,
+                     hipStream_t st) {
+  LAUNCH_V(k_mgs_last, g.n, ap, ap_prev, bm, rb);
+}
+void launch_final_p(const Geo& g, const double* r, const double* const* ptab, int nprev, double* ap,
+                    double* pout, int aslot, const RedBufs& rb, hipStream_t st) {
+  LAUNCH_V(k_final_p, g.n, r, ptab, nprev, ap, pout, aslot, rb);
+}
+void launch_update_r(const Geo& g, double* r, const double* ap, int aslot, const RedBufs& rb,
+                     hipStream_t st) {
+  LAUNCH_V(k_update_r, g.n, r, ap, aslot, rb);
+}
+void launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const RedBufs& rb,
+                  hipStream_t st) {
+  LAUNCH_V(k_accum, g.n, x, ptab, nvec, rb);
+}
+void launch_dot(const Geo& g, const double* a, const double* b, const RedBufs& rb, hipStream_t st) {
+  LAUNCH_V(k_dot, g.n, a, b, rb);
+}
+void launch_norm_inf(const Geo& g, const double* a, const RedBufs& rb, hipStream_t st) {
+  LAUNCH_V(k_norm_inf, g.n, a, rb);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion.  Reference interior order r = ((i-1)*ny + (j-1))*nz + (k-1)
+// (meshgrid 'ij', k fastest: indices.py:271-278).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ long long ref_of(const Geo& g, long long q) {
+  const long long i1 = q % g.nx;
+  const long long t = q / g.nx;
+  const long long j1 = t % g.ny;
+  const long long k1 = t / g.ny;
+  return (i1 * g.ny + j1) * g.nz + k1;
+}
+
+__global__ void k_permute_in(Geo g, const double2* ref, double* out, int* bad) {
+  const long long n = g.n;
+  GRID_STRIDE(q) {
+    const long long r = ref_of(g, q);
+    reinterpret_cast<double2*>(out)[q] = ref[r];
+    int nonreal = 0;
+    for (int a = 0; a < g.nphi; ++a) {
+      const double2 v = ref[(a + 1) * n + r];
+      out[(2 + a) * n + q] = v.x;
+      nonreal |= (v.y != 0.0);
+    }
+    if (nonreal) atomicOr(bad, 1);
+  }
+}
+
+__global__ void k_permute_out(Geo g, const double* in, double2* ref) {
+  const long long n = g.n;
+  GRID_STRIDE(q) {
+    const long long r = ref_of(g, q);
+    ref[r] = reinterpret_cast<const double2*>(in)[q];
+    for (int a = 0; a < g.nphi; ++a) ref[(a + 1) * n + r] = make_double2(in[(2 + a) * n + q], 0.0);
+  }
+}
+
+void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_permute_in, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g,
+                     static_cast<const double2*>(ref), st_out, bad);
+}
+void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st) {
+  hipLaunchKernelGGL(k_permute_out, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, st_in,
+                     static_cast<double2*>(ref));
+}
+
+// ---------------------------------------------------------------------------
+// Observables (bfield.py:58-68 with BCs; current_density.py:61-68 without)
+// ---------------------------------------------------------------------------
+
+template <bool IS3D>
+__global__ void k_obs(Geo g, Bnd b, const uint8_t* flags, const double* x, double* bout,
+                      double* jout) {
+  const long long n = g.n;
+  Src<double, false> s{x, nullptr, 0.0, n};
+  GRID_STRIDE(q) {
+    const int i = 1 + (int)(q % g.nx);
+    const long long t = q / g.nx;
+    const int j = 1 + (int)(t % g.ny);
+    const int k = IS3D ? 1 + (int)(t / g.ny) : 0;
+    const long long r = ref_of(g, q);
+    if (bout) {
+      const Cell<double> c = eff_cell<double, IS3D, false>(g, b, flags, s, i, j, k);
+      const Cell<double> ci = eff_cell<double, IS3D, false>(g, b, flags, s, i + 1, j, k);
+      const Cell<double> cj = eff_cell<double, IS3D, false>(g, b, flags, s, i, j + 1, k);
+      double bx = 0.0, by = 0.0;
+      if (IS3D) {
+        const Cell<double> ck = eff_cell<double, IS3D, false>(g, b, flags, s, i, j, k + 1);
+        bx = (1.0 / (b.hy * b.hz)) * (((c.y - ck.y) - c.z) + cj.z);
+        by = (1.0 / (b.hz * b.hx)) * (((c.z - ci.z) - c.x) + ck.x);
+      }
+      const double bz = (1.0 / (b.hx * b.hy)) * (((c.x - cj.x) - c.y) + ci.y);
+      bout[r] = bx;
+      bout[n + r] = by;
+      bout[2 * n + r] = bz;
+    }
+    if (jout) {
+      // expanded state without BCs: ghosts are 0, interior values raw
+      double pr, pi;
+      s.psi(q, pr, pi);
+      const double ph[3] = {s.phx(q), s.phy(q), IS3D ? s.phz(q) : 0.0};
+      const int nb[3][3] = {{i + 1, j, k}, {i, j + 1, k}, {i, j, k + 1}};
+      for (int a = 0; a < 3; ++a) {
+        double js = 0.0;
+        if (a < 2 || IS3D) {
+          const int ii = nb[a][0], jj = nb[a][1], kk = nb[a][2];
+          double qr = 0.0, qi = 0.0;
+          if (ii <= g.Nx - 1 && jj <= g.Ny - 1 && (!IS3D || kk <= g.Nz - 1))
+            s.psi(qix(g, ii, jj, kk), qr, qi);
+          double sn, cs;
+          sincos(ph[a], &sn, &cs);
+          // conj(psi) * exp(-1j*phi) * psi1
+          const double tr = pr * cs - (-pi) * (-sn);
+          const double ti = pr * (-sn) + (-pi) * cs;
+          js = tr * qi + ti * qr;
+        }
+        jout[a * n + r] = js;
+      }
+    }
+  }
+}
+
+void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,
+                        double* jout, hipStream_t st) {
+  if (g.is3d)
+    hipLaunchKernelGGL(k_obs<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b, mt.flags,
+                       x, bout, jout);
+  else
+    hipLaunchKernelGGL(k_obs<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b,
+                       mt.flags, x, bout, jout);
+}
+
+}  // namespace tdgl

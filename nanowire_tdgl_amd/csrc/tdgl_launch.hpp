@@ -1,0 +1,76 @@
+// Host-side launchers for the TDGL kernels (implemented in tdgl_f64.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tdgl_types.hpp"
+
+namespace tdgl {
+
+// Slots of the per-context device scalar array.
+enum ResSlot : int {
+  RS_NRM = 0,    // ||Ap||_2 after Gram-Schmidt
+  RS_INV = 1,    // 1/||Ap||_2
+  RS_BRK = 2,    // 1.0 when ||Ap|| < 1e-14 (tgcr.py:141-142)
+  RS_RR = 3,     // ||r||_2^2
+  RS_XX = 4,     // ||x_lin||_2^2
+  RS_GINF = 5,   // ||g||_inf
+  RS_DXINF = 6,  // ||dx||_inf
+  RS_DOT = 7,    // generic dot / norm result
+  RS_BETA = 16,  // beta_j, j < kMaxKrylov
+};
+constexpr int kMaxKrylov = 2048;
+constexpr int RS_ALPHA = RS_BETA + kMaxKrylov;
+constexpr int kResLen = RS_ALPHA + kMaxKrylov;
+
+enum StencilMode : int { M_RHS = 0, M_PRED = 1, M_NEWTON = 2, M_JVP = 3, M_EULER = 4 };
+
+struct RedBufs {
+  double* part;          // partial sums, >= max grid * 4 doubles
+  unsigned int* ticket;  // zero-initialised counter
+  double* res;           // kResLen doubles
+};
+
+// Extra inputs/outputs of the stencil epilogue.
+struct Epi {
+  double* out0;          // f | x_pred | f | Ap | X_new
+  double* out1;          // - | gamma  | r | - | -
+  const double* gam;     // NEWTON: gamma
+  const double* f0;      // JVP: f(x_lin)
+  const double* ap0;     // JVP: first Krylov vector (beta_0) or nullptr (norm)
+  double dt;             // PRED/EULER: dt
+  double hdt;            // dt / 2
+  double inv_eps;        // JVP: 1/eps
+};
+
+struct StencilCfg {
+  int kz;                // z planes per block
+};
+
+int stencil_blocks(const Geo& g, const StencilCfg& c);
+int stream_blocks(const Geo& g);
+
+void launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
+                    const RedBufs& rb, hipStream_t st);
+
+// Krylov / vector kernels (state-shaped vectors, internal layout).
+void launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int beta_prev_slot,
+                     const double* ap_next, int beta_next_slot, const RedBufs& rb, hipStream_t st);
+void launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int beta_slot,
+                     const RedBufs& rb, hipStream_t st);
+void launch_final_p(const Geo& g, const double* r, const double* const* ptab, int nprev, double* ap,
+                    double* pout, int alpha_slot, const RedBufs& rb, hipStream_t st);
+void launch_update_r(const Geo& g, double* r, const double* ap, int alpha_slot, const RedBufs& rb,
+                     hipStream_t st);
+void launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const RedBufs& rb,
+                  hipStream_t st);
+void launch_dot(const Geo& g, const double* a, const double* b, const RedBufs& rb, hipStream_t st);
+void launch_norm_inf(const Geo& g, const double* a, const RedBufs& rb, hipStream_t st);
+void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
+void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st);
+void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,
+                        double* jout, hipStream_t st);
+
+}  // namespace tdgl

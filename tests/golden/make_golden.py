@@ -1,0 +1,246 @@
+"""Write the golden fixtures that pin the oracle to tdgl3d's own numbers.
+
+Run in the development container only (the GPU box has no copy of tdgl3d):
+
+    python -B tests/golden/make_golden.py /root/reference/src
+
+It imports the reference package from the given source directory, runs it on
+the small seeded cases of ``specs.py`` and writes ``*.npz`` files (inputs and
+outputs only) next to this script.  The library never imports this script;
+the tests only read the ``.npz`` files.
+
+Fixtures
+--------
+indices_*.npz        construct_indices / define_hole_polygon arrays (ordered)
+indices_film128_hole digests of the 128x128x16 film's maps + its hole arrays
+evalf_*.npz          eval_f on seeded states (applied field, holes, material)
+tgcr_trap.npz        one tgcr_matrix_free_trap solve + its iteration count
+newton_trap.npz      one newton_gcr_trap solve + its iteration count
+trap_c1.npz          trapezoidal, 16x16x8 film, t in [0, 10] (snapshots + counters)
+trap_c1_abort.npz    the same film run to t = 20: the reference aborts (dt < dt_min)
+trap_hole.npz        trapezoidal on a small film with a hole (every step)
+trap_dtcut.npz       trapezoidal with TGCR capped at 4 iterations (dt/10 branch)
+euler_small.npz      forward_euler on a small film with a hole + B, J_s per snapshot
+obs_aniso.npz        Solution.bfield / supercurrent_density on a random state
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+import specs as S  # noqa: E402
+
+IDX_KEYS = [
+    "interior_to_full", "bfield_interior",
+    "x_face_lo", "x_first", "x_last", "x_face_hi",
+    "x_face_lo_inner", "x_first_inner", "x_last_inner", "x_face_hi_inner",
+    "y_face_lo", "y_first", "y_last", "y_face_hi",
+    "y_face_lo_inner", "y_first_inner", "y_last_inner", "y_face_hi_inner",
+    "z_face_lo", "z_first", "z_last", "z_face_hi",
+    "z_face_lo_inner", "z_first_inner", "z_last_inner", "z_face_hi_inner",
+    "x_normal_bc_mask", "y_normal_bc_mask", "z_normal_bc_mask",
+    "hole_x_bc_mask", "hole_y_bc_mask", "hole_z_bc_mask",
+    "hole_x_bc_interior", "hole_y_bc_interior", "hole_z_bc_interior",
+]
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def rand_state(n, nphi, seed, psi_noise=0.3, phi_scale=0.2):
+    rng = np.random.default_rng(seed)
+    psi = 1.0 + psi_noise * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    phi = phi_scale * rng.standard_normal(nphi * n)
+    return np.concatenate([psi, phi.astype(np.complex128)])
+
+
+def noisy_psi_state(n, nphi, seed, noise):
+    rng = np.random.default_rng(seed)
+    psi = 1.0 + noise * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    return np.concatenate([psi, np.zeros(nphi * n, dtype=np.complex128)])
+
+
+def main(src: str) -> None:
+    sys.dont_write_bytecode = True
+    sys.path.insert(0, src)
+    from tdgl3d.core.device import Device
+    from tdgl3d.core.material import Layer, Trilayer
+    from tdgl3d.core.parameters import SimulationParameters
+    from tdgl3d.core.solution import Solution
+    from tdgl3d.physics.applied_field import AppliedField, build_boundary_field_vectors
+    from tdgl3d.physics.rhs import BoundaryVectors, eval_f
+    from tdgl3d.solvers import integrators, newton, tgcr
+    from tdgl3d.solvers.runner import _make_eval_u
+
+    def make_device(case, **dev_kw):
+        p = SimulationParameters(**case["p"])
+        tri = None
+        if case.get("trilayer"):
+            b, i, t = case["trilayer"]
+            tri = Trilayer(bottom=Layer(*b), insulator=Layer(*i), top=Layer(*t))
+        dev = Device(p, trilayer=tri, **dev_kw)
+        for verts, zr in case["holes"]:
+            dev.add_hole(verts, zr)
+        return p, dev
+
+    # 1. index maps and hole masks -------------------------------------------------
+    for name, case in S.INDEX_CASES.items():
+        p, dev = make_device(case)
+        out = {k: np.asarray(getattr(dev.idx, k)) for k in IDX_KEYS}
+        if dev.material is not None:
+            out["sc_mask"] = dev.material.sc_mask
+            out["interior_sc_mask"] = dev.material.interior_sc_mask
+            out["kappa"] = dev.material.kappa
+        np.savez_compressed(HERE / f"indices_{name}.npz", **out)
+
+    p, dev = make_device(S.FILM128)
+    out = {k: np.asarray(getattr(dev.idx, k)) for k in IDX_KEYS if k.startswith("hole_")}
+    digests = {k: digest(np.asarray(getattr(dev.idx, k), dtype=np.int64)) for k in IDX_KEYS}
+    out["digest_keys"] = np.array(list(digests.keys()))
+    out["digest_vals"] = np.array(list(digests.values()))
+    out["sc_mask_digest"] = np.array(digest(dev.material.sc_mask))
+    out["n_sc_zero"] = np.array(int(np.sum(dev.material.sc_mask == 0.0)))
+    np.savez_compressed(HERE / "indices_film128_hole.npz", **out)
+
+    # 2. eval_f on seeded states ---------------------------------------------------
+    for name, case in S.EVALF_CASES.items():
+        p, dev = make_device(case)
+        X = rand_state(p.n_interior, 3 if p.is_3d else 2, case["seed"])
+        u = BoundaryVectors(*build_boundary_field_vectors(*case["field"], p, dev.idx))
+        F = eval_f(X, p, dev.idx, u, material=dev.material)
+        out = dict(X=X, F=F, field=np.array(case["field"]),
+                   params=np.array([p.Nx, p.Ny, p.Nz, p.hx, p.hy, p.hz, p.kappa,
+                                    p.periodic_x, p.periodic_y, p.periodic_z], dtype=np.float64))
+        if dev.material is not None:
+            out["kappa_full"] = dev.material.kappa
+            out["interior_sc_mask"] = dev.material.interior_sc_mask
+        np.savez_compressed(HERE / f"evalf_{name}.npz", **out)
+
+    # 3. one TGCR solve and one Newton solve on the 16x16x8 film -----------------------
+    case = S.EVALF_CASES["c1_bz"]
+    p, dev = make_device(case)
+    X = rand_state(p.n_interior, 3, 20, psi_noise=0.05, phi_scale=0.02)
+    u = BoundaryVectors(*build_boundary_field_vectors(0.0, 0.0, 0.3, p, dev.idx))
+    calls = [0]
+
+    def f(Y):
+        calls[0] += 1
+        return eval_f(Y, p, dev.idx, u)
+
+    dt = 0.1
+    fX = f(X)
+    gamma = X + (dt / 2.0) * fX
+    x_pred = X + dt * fX
+    g = x_pred - (dt / 2.0) * f(x_pred) - gamma
+    calls[0] = 0
+    dx = tgcr.tgcr_matrix_free_trap(f, x_pred, -g, dt, tol=1e-8, eps_mf=1e-4)
+    np.savez_compressed(HERE / "tgcr_trap.npz", X=X, x_lin=x_pred, b=-g, dt=dt, tol=1e-8,
+                        eps_mf=1e-4, dx=dx, n_iter=calls[0] // 2, field=np.array([0, 0, 0.3]))
+    calls[0] = 0
+    xn, conv, it = newton.newton_gcr_trap(f, x_pred, gamma, dt, tol_f=1e-3, tol_dx=1e-3,
+                                          max_iter=20, tol_gcr=1e-8, eps_mf=1e-4)
+    np.savez_compressed(HERE / "newton_trap.npz", X=X, x0=x_pred, gamma=gamma, dt=dt,
+                        x=xn, converged=conv, iters=it, n_evals=calls[0],
+                        field=np.array([0, 0, 0.3]))
+
+    # 4. trapezoidal trajectories with per-step Newton/TGCR counters -----------------
+    orig_newton = integrators.newton_gcr_trap
+    orig_tgcr = newton.tgcr_matrix_free_trap
+    log = {"newton": [], "tgcr": []}
+    tgcr_cap = [None]
+
+    def tgcr_wrap(ef, x_lin, b, dt, tol=1e-4, max_iter=None, eps_mf=1e-4):
+        cnt = [0]
+
+        def ef2(Y):
+            cnt[0] += 1
+            return ef(Y)
+        mi = tgcr_cap[0] if max_iter is None else max_iter
+        res = orig_tgcr(ef2, x_lin, b, dt, tol=tol, max_iter=mi, eps_mf=eps_mf)
+        log["tgcr"].append((cnt[0] // 2, res.size == 0))
+        return res
+
+    def newton_wrap(*a, **k):
+        r = orig_newton(*a, **k)
+        log["newton"].append((bool(r[1]), int(r[2]), float(a[3])))
+        return r
+
+    integrators.newton_gcr_trap = newton_wrap
+    newton.tgcr_matrix_free_trap = tgcr_wrap
+
+    def run_trap(case, keep, t_stop=None, save_every=1):
+        p, dev = make_device(case)
+        x0 = noisy_psi_state(p.n_interior, 3 if p.is_3d else 2, case["seed"], case["noise"])
+        log["newton"].clear()
+        log["tgcr"].clear()
+        tgcr_cap[0] = case["tgcr_max_iter"]
+        t_stop = case["t_stop"] if t_stop is None else t_stop
+        eval_u = _make_eval_u(AppliedField(**case["field"]), p, dev.idx, t_stop)
+        err = ""
+        try:
+            times, hist = integrators.trapezoidal(
+                x0, p, dev.idx, eval_u, 0.0, t_stop, case["dt"], tol_gcr=case["tol_gcr"],
+                save_every=save_every, progress=False, material=dev.material)
+        except RuntimeError as e:
+            err = str(e)
+            times, hist = np.zeros(0), np.zeros((x0.size, 0), dtype=np.complex128)
+        sel = [s for s in keep if s < hist.shape[1]]
+        if hist.shape[1] and (hist.shape[1] - 1) not in sel:
+            sel.append(hist.shape[1] - 1)
+        return dict(x0=x0, times=times, states=hist[:, sel], sel=np.array(sel, dtype=np.int64),
+                    newton=np.array(log["newton"], dtype=np.float64).reshape(-1, 3),
+                    tgcr=np.array(log["tgcr"], dtype=np.float64).reshape(-1, 2),
+                    error=np.array(err), t_stop=np.array(t_stop))
+
+    # The 16x16x8 film: the full run to t = 20 aborts in the reference (dt falls
+    # below dt_min near t = 12.9), so it is stored as an abort record and the
+    # trajectory for the completing prefix t in [0, 10].
+    res = run_trap(S.TRAP_C1, [0, 1, 2, 3, 5, 10, 30, 60])
+    np.savez_compressed(HERE / "trap_c1.npz", **res)
+    res = run_trap(S.TRAP_C1, [], t_stop=S.TRAP_C1["t_stop_full"])
+    np.savez_compressed(HERE / "trap_c1_abort.npz",
+                        **{k: v for k, v in res.items() if k not in ("states", "x0")})
+    res = run_trap(S.TRAP_HOLE, list(range(64)))
+    np.savez_compressed(HERE / "trap_hole.npz", **res)
+    res = run_trap(S.TRAP_DTCUT, [0, 1, 2, 5, 10, 50, 100, 200, 400])
+    np.savez_compressed(HERE / "trap_dtcut.npz", **res)
+    integrators.newton_gcr_trap = orig_newton
+    newton.tgcr_matrix_free_trap = orig_tgcr
+
+    # 5. forward Euler + observables ------------------------------------------------
+    case = S.EULER_SMALL
+    p, dev = make_device(case, applied_field=AppliedField(**case["field"]))
+    x0 = noisy_psi_state(p.n_interior, 3, case["seed"], case["noise"])
+    eval_u = _make_eval_u(dev.applied_field, p, dev.idx, case["t_stop"])
+    times, hist = integrators.forward_euler(x0, p, dev.idx, eval_u, 0.0, case["t_stop"],
+                                            case["dt"], save_every=case["save_every"],
+                                            progress=False, material=dev.material)
+    sol = Solution(times=times, states=hist, params=p, idx=dev.idx, device=dev)
+    bfs = np.stack([np.stack(sol.bfield(step=s)) for s in range(sol.n_steps)])
+    jss = np.stack([np.stack(sol.supercurrent_density(step=s)) for s in range(sol.n_steps)])
+    np.savez_compressed(HERE / "euler_small.npz", x0=x0, times=times, states=hist,
+                        bfield=bfs, supercurrent=jss)
+
+    case = S.OBS_ANISO
+    p, dev = make_device(case, applied_field=AppliedField(**case["field"]))
+    X = rand_state(p.n_interior, 3, case["seed"])
+    sol = Solution(times=np.array(case["times"]), states=np.stack([X, X, X], axis=1),
+                   params=p, idx=dev.idx, device=dev)
+    np.savez_compressed(HERE / "obs_aniso.npz", X=X,
+                        b_t0=np.stack(sol.bfield(step=0)), b_t1=np.stack(sol.bfield(step=1)),
+                        b_tend=np.stack(sol.bfield(step=2)),
+                        b_sub=np.stack(sol.bfield(step=1, full_interior=False)),
+                        js=np.stack(sol.supercurrent_density(step=1)))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.environ.get("TDGL3D_SRC", "/root/reference/src"))

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path against tdgl3d's own outputs (golden fixtures) and
+the CPU oracle, FP64.
+
+Tolerances: the RHS agrees with the reference to rounding (<= 1e-13 of the
+largest entry); one trapezoidal step started from the reference's state
+agrees to <= 1e-12 relative (the target for the plain method); Newton and
+TGCR iteration counts and dt/10 decisions are identical.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import specs as S
+from conftest import golden
+from helpers import device, oracle_film, rand_state, rel_err
+from oracle import tdgl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    import nanowire_tdgl_amd as T
+    from nanowire_tdgl_amd.engine import engine_for
+    from nanowire_tdgl_amd.physics import rhs
+    from nanowire_tdgl_amd.physics.applied_field import build_boundary_field_vectors
+    from nanowire_tdgl_amd.solvers import integrators
+    return T, engine_for, rhs, build_boundary_field_vectors, integrators
+
+
+@pytest.mark.parametrize("name", list(S.EVALF_CASES))
+def test_eval_f_matches_reference(name):
+    T, engine_for, rhs, bbfv, _ = _lib()
+    d = golden(f"evalf_{name}")
+    dev = device(S.EVALF_CASES[name])
+    u = rhs.BoundaryVectors.from_field(bbfv(*d["field"], dev.params, dev.idx))
+    F = rhs.eval_f(d["X"], dev.params, dev.idx, u, dev.material)
+    assert F.dtype == np.complex128 and F.shape == d["F"].shape
+    assert rel_err(F, d["F"]) < 1e-13, rel_err(F, d["F"])
+    n = dev.params.n_interior
+    assert np.all(F[n:].imag == 0)
+
+
+def test_eval_f_general_boundary_vectors():
+    """BoundaryVectors that did not come from build_boundary_field_vectors go
+    through the per-node field path."""
+    T, engine_for, rhs, bbfv, _ = _lib()
+    d = golden("evalf_aniso_bxyz")
+    dev = device(S.EVALF_CASES["aniso_bxyz"])
+    Bx, By, Bz = (np.array(v) for v in bbfv(*d["field"], dev.params, dev.idx))
+    F = rhs.eval_f(d["X"], dev.params, dev.idx, rhs.BoundaryVectors(Bx, By, Bz), dev.material)
+    assert rel_err(F, d["F"]) < 1e-13
+
+
+def test_permute_round_trip_and_complex_phi_rejected():
+    T, engine_for, rhs, bbfv, _ = _lib()
+    dev = device(S.EVALF_CASES["hole_zr"])
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    X = rand_state(dev.params.n_interior, 3, 4)
+    back = eng.ctx.to_reference(eng.ctx.to_internal(X)).cpu().numpy()
+    assert np.array_equal(back, X)
+    bad = X.copy()
+    bad[dev.params.n_interior + 5] += 1e-3j
+    with pytest.raises(ValueError, match="real"):
+        eng.ctx.to_internal(bad)
+
+
+def test_reductions_match_numpy():
+    T, engine_for, rhs, bbfv, _ = _lib()
+    dev = device(S.EVALF_CASES["c1_bz"])
+    ctx = engine_for(dev.params, dev.idx, dev.material).ctx
+    n = dev.params.n_interior
+    a = rand_state(n, 3, 1)
+    b = rand_state(n, 3, 2)
+    ga, gb = ctx.to_internal(a), ctx.to_internal(b)
+    assert ctx.dot(ga, gb) == pytest.approx(np.vdot(a, b).real, rel=1e-14)
+    assert ctx.norm_inf(ga) == np.max(np.abs(a))
+
+
+def test_fd_jvp_matches_oracle():
+    T, engine_for, rhs, bbfv, _ = _lib()
+    case = S.EVALF_CASES["hole_bz"]
+    dev = device(case)
+    f = oracle_film(case)
+    field = case["field"]
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    eng.set_field(*field)
+    u = O.boundary_vectors(*field, f.g, f.idx)
+    n = dev.params.n_interior
+    x = rand_state(n, 3, 7, 0.1, 0.05)
+    p = rand_state(n, 3, 8, 1.0, 1.0) * 1e-3
+    eps, dt = 0.37, 0.2
+    ctx = eng.ctx
+    gx, gp = ctx.to_internal(x), ctx.to_internal(p)
+    f0 = ctx.empty_state()
+    ctx.rhs(gx, f0)
+    ap = ctx.empty_state()
+    ctx.jvp_trap_fd(gx, f0, gp, eps, dt, ap)
+    got = ctx.to_reference(ap).cpu().numpy()
+    F = lambda Y: O.eval_f_csr(Y, f.g, f.idx, u, f.material)  # noqa: E731
+    want = p - (dt / 2.0) * ((F(x + eps * p) - F(x)) / eps)
+    assert rel_err(got, want) < 1e-11
+
+
+def _trap_one_step(dev, X, t, t_stop, field, dt, tol_gcr, tgcr_max_iter=None):
+    T, engine_for, rhs, bbfv, integ = _lib()
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    from nanowire_tdgl_amd import _native
+    af = T.AppliedField(**field)
+    eng.set_field(*af.evaluate(t, t_stop))
+    gX = eng.ctx.to_internal(X)
+    st = eng.ctx.trap_step(gX, dt, _native.solver_opts(tol_gcr=tol_gcr, tgcr_max_iter=tgcr_max_iter))
+    return eng.ctx.to_reference(gX).cpu().numpy(), st
+
+
+def test_single_steps_match_reference_per_step():
+    """Each step of the hole-film trajectory, started from tdgl3d's own state,
+    reproduces tdgl3d's next state to 1e-12 and its Newton/TGCR counts."""
+    d = golden("trap_hole")
+    case = S.TRAP_HOLE
+    dev = device(case)
+    states, times = d["states"], d["times"]
+    tg = [int(k) for k, _ in d["tgcr"]]
+    pos = 0
+    worst = 0.0
+    for s in range(len(d["sel"]) - 1):
+        assert d["sel"][s] == s
+        Xn, st = _trap_one_step(dev, states[:, s], times[s], float(d["t_stop"]), case["field"],
+                                case["dt"], case["tol_gcr"])
+        assert st.converged == 1
+        nit = int(d["newton"][s][1])
+        assert st.newton_iters == nit, s
+        assert st.tgcr_iters_total == sum(tg[pos:pos + nit]), s
+        pos += nit
+        err = rel_err(Xn, states[:, s + 1])
+        worst = max(worst, err)
+        assert err < 1e-12, (s, err)
+    print(f"worst per-step relative error {worst:.2e}")
+
+
+@pytest.mark.parametrize("which", ["trap_hole", "trap_c1", "trap_dtcut"])
+def test_trajectory_matches_reference(which):
+    T, engine_for, rhs, bbfv, integ = _lib()
+    from nanowire_tdgl_amd.solvers.runner import _make_eval_u
+    case = {"trap_hole": S.TRAP_HOLE, "trap_c1": S.TRAP_C1, "trap_dtcut": S.TRAP_DTCUT}[which]
+    d = golden(which)
+    dev = device(case)
+    t_stop = float(d["t_stop"])
+    eval_u = _make_eval_u(T.AppliedField(**case["field"]), dev.params, dev.idx, t_stop)
+    log = []
+    times, hist = integ.trapezoidal(d["x0"], dev.params, dev.idx, eval_u, 0.0, t_stop, case["dt"],
+                                    tol_gcr=case["tol_gcr"], progress=False, material=dev.material,
+                                    tgcr_max_iter=case["tgcr_max_iter"], step_log=log)
+    # identical accept/reject decisions and iteration counts
+    assert [(bool(r["converged"]), r["newton_iters"]) for r in log] == \
+        [(bool(c), int(i)) for c, i, _ in d["newton"]]
+    assert [r["dt"] for r in log] == [float(v) for v in d["newton"][:, 2]]
+    assert np.allclose(times, d["times"], rtol=0, atol=1e-12)
+    err = rel_err(hist[:, d["sel"]], d["states"])
+    print(f"{which}: {len(log)} step attempts, max relative state error {err:.2e}")
+    assert err < 1e-9
+
+
+def test_abort_matches_reference():
+    """The 16x16x8 film run to t = 20 aborts in tdgl3d (dt < dt_min); so must we,
+    at the same time."""
+    T, engine_for, rhs, bbfv, integ = _lib()
+    from nanowire_tdgl_amd.solvers.runner import _make_eval_u
+    case = S.TRAP_C1
+    d = golden("trap_c1_abort")
+    ref = golden("trap_c1")
+    dev = device(case)
+    t_stop = float(d["t_stop"])
+    eval_u = _make_eval_u(T.AppliedField(**case["field"]), dev.params, dev.idx, t_stop)
+    with pytest.raises(RuntimeError) as ei:
+        integ.trapezoidal(ref["x0"], dev.params, dev.idx, eval_u, 0.0, t_stop, case["dt"],
+                          tol_gcr=case["tol_gcr"], progress=False,
+                          tgcr_max_iter=case["tgcr_max_iter"])
+    assert str(ei.value) == str(d["error"])
+
+
+def test_euler_and_observables_match_reference():
+    T, engine_for, rhs, bbfv, integ = _lib()
+    d = golden("euler_small")
+    case = S.EULER_SMALL
+    dev = device(case, applied_field=T.AppliedField(**case["field"]))
+    sol = T.solve(dev, t_stop=case["t_stop"], dt=case["dt"], method="euler", x0=d["x0"],
+                  save_every=case["save_every"], progress=False, log_metadata=False)
+    assert np.array_equal(sol.times, d["times"])
+    assert rel_err(sol.states, d["states"]) < 1e-12
+    for s in range(sol.n_steps):
+        ref_sol = T.Solution(times=d["times"], states=d["states"], params=dev.params,
+                             idx=dev.idx, device=dev)
+        assert rel_err(np.stack(ref_sol.bfield(step=s)), d["bfield"][s]) < 1e-12
+        assert rel_err(np.stack(ref_sol.supercurrent_density(step=s)), d["supercurrent"][s]) < 1e-13
+
+
+def test_observables_aniso():
+    T, *_ = _lib()
+    d = golden("obs_aniso")
+    case = S.OBS_ANISO
+    dev = device(case, applied_field=T.AppliedField(**case["field"]))
+    X = d["X"]
+    sol = T.Solution(times=np.array(case["times"]), states=np.stack([X, X, X], 1),
+                     params=dev.params, idx=dev.idx, device=dev)
+    for s, key in enumerate(("b_t0", "b_t1", "b_tend")):
+        assert rel_err(np.stack(sol.bfield(step=s)), d[key]) < 1e-12, key
+    assert np.array_equal(np.stack(sol.bfield(step=1, full_interior=False)), d["b_sub"])
+    assert rel_err(np.stack(sol.supercurrent_density(step=1)), d["js"]) < 1e-13
+
+
+def test_solve_reference_smoke_cases(tmp_path):
+    T, *_ = _lib()
+    dev = T.Device(T.SimulationParameters(Nx=4, Ny=4, Nz=1, kappa=2.0), applied_field=T.AppliedField())
+    sol = T.solve(dev, t_stop=0.1, dt=0.05, method="trapezoidal", progress=False, log_dir=tmp_path)
+    assert sol.n_steps >= 2 and np.all(np.isfinite(sol.states))
+    assert len(list(tmp_path.glob("run_*.json"))) == 1
+    dev = T.Device(T.SimulationParameters(Nx=4, Ny=4, Nz=4, kappa=2.0), applied_field=T.AppliedField(Bz=0.5))
+    sol = T.solve(dev, t_stop=0.05, dt=0.01, method="euler", progress=False, log_metadata=False)
+    assert sol.states.shape[0] == dev.params.n_state and sol.n_steps > 1
+    with pytest.raises(ValueError):
+        T.solve(dev, method="rk4", progress=False, log_metadata=False)
+
+
+def test_film128_hole_step_invariants():
+    """128x128x16 film with the 32x32 hole: one step keeps phi = 0 on every
+    hole-boundary link (dphi = 0 there), keeps Im(phi) = 0 and stays finite."""
+    T, engine_for, rhs, bbfv, integ = _lib()
+    dev = device(S.FILM128)
+    p = dev.params
+    n = p.n_interior
+    rng = np.random.default_rng(1)
+    X = np.concatenate([1.0 + 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                        np.zeros(3 * n, complex)])
+    Xn, st = _trap_one_step(dev, X, 0.2, 200.0, dict(Bz=0.4), 0.2, 1e-8)
+    assert st.converged == 1 and np.all(np.isfinite(Xn))
+    for q, ax in enumerate("xyz"):
+        hi = getattr(dev.idx, f"hole_{ax}_bc_interior")
+        if hi.size:
+            assert np.all(Xn[(q + 1) * n:(q + 2) * n][hi] == 0.0)
+    assert np.all(Xn[n:].imag == 0)
+    # the psi relaxation inside the hole drives psi down
+    sc = dev.material.interior_sc_mask
+    assert np.abs(Xn[:n][sc == 0]).max() < np.abs(X[:n][sc == 0]).min()

@@ -1,0 +1,61 @@
+"""The C-ABI library loads on the CPU host and exports every entry point that
+include/tdgl_hip.h declares (no compute calls: there is no GPU here)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from nanowire_tdgl_amd import _native
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared() -> list[str]:
+    names = []
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:int|int64_t|const char\s*\*|void)\s+(tdgl_\w+)\s*\(", text, re.M)
+    return names
+
+
+def test_header_declares_entry_points():
+    assert len(_declared()) >= 15
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libtdglhip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_native.LIB_PATH))
+    missing = [s for s in _declared() if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(_native.EXPORTS) == set(_declared())
+
+
+@pytest.mark.skipif(not _native.LIB_PATH.exists(), reason="libtdglhip.so not built")
+def test_loader_sets_signatures():
+    lib = _native.load()
+    assert lib.tdgl_ctx_create.restype is ctypes.c_int
+    assert ctypes.sizeof(_native.StepStats) == 8 * 4 + 3 * 8
+    assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8
+
+
+def test_struct_layout_matches_header():
+    text = (ROOT / "include" / "tdgl_hip.h").read_text()
+    for st, fields in (("tdgl_grid", _native.Grid._fields_),
+                       ("tdgl_solver_opts", _native.SolverOpts._fields_),
+                       ("tdgl_step_stats", _native.StepStats._fields_)):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (st, st), text, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        declared = re.findall(r"(\w+)\s*(?:,|;)", body)
+        assert [f for f, _ in fields] == declared, st
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_native, "LIB_PATH", tmp_path / "libtdglhip.so")
+    monkeypatch.setattr(_native, "_lib", None)
+    with pytest.raises(ImportError, match="build it"):
+        _native.load()
