@@ -87,6 +87,8 @@ typedef struct tdgl_step_stats {
   double g_inf;             /* last ||g||_inf (newton.py:112) */
   double dx_inf;            /* last ||dx||_inf (newton.py:113) */
   double bytes_moved;       /* algorithmic HBM bytes of the step */
+  double tgcr_r0;           /* ||r0||_2 of the last TGCR call (tgcr.py:116) */
+  double tgcr_rk;           /* its final ||r||_2 */
 } tdgl_step_stats;
 
 /* Context: grid geometry + device buffers.  Replaces construct_indices'

@@ -52,7 +52,8 @@ class StepStats(C.Structure):
                 ("tgcr_failed", C.c_int32), ("tgcr_iters_last", C.c_int32),
                 ("tgcr_iters_total", C.c_int32), ("rhs_evals", C.c_int32),
                 ("jvp_evals", C.c_int32), ("reserved", C.c_int32),
-                ("g_inf", C.c_double), ("dx_inf", C.c_double), ("bytes_moved", C.c_double)]
+                ("g_inf", C.c_double), ("dx_inf", C.c_double), ("bytes_moved", C.c_double),
+                ("tgcr_r0", C.c_double), ("tgcr_rk", C.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

@@ -32,7 +32,10 @@ struct tdgl_ctx {
   double* d_k2 = nullptr;
   double* d_sc = nullptr;
   double* d_bfull = nullptr;  // 3 * dim_x
-  RedBufs rb{};
+  double* d_pb[2] = {nullptr, nullptr};  // double-buffered reduction partials
+  int pb_cur = 0;                         // buffer holding the last published partials
+  int pb_count = 0;                       // how many blocks published them
+  double* d_res = nullptr;                // scalar slots (kResLen)
   int* d_bad = nullptr;
   double* h_res = nullptr;  // pinned mirror of the scalar slots
 
@@ -44,12 +47,14 @@ struct tdgl_ctx {
   std::vector<double*> P, AP;
   double** d_ptab = nullptr;
   int ptab_cap = 0;
+  int ptab_len = 0;  // entries of d_ptab already copied
 
   // per-kernel-class event timing (tdgl_profile_enable)
   struct PendingEv {
     hipEvent_t a, b;
     int cls;
     double bytes;
+    int launches;  // consecutive launches of the class inside the bracket
   };
   bool prof = false;
   std::vector<PendingEv> pending;
@@ -82,7 +87,7 @@ int alloc(tdgl_ctx* ctx, void** p, size_t bytes) {
 }
 
 // Event bracket around one launch of kernel class `cls` (no-op unless profiling).
-int prof_begin(tdgl_ctx* ctx, int cls, double bytes) {
+int prof_begin(tdgl_ctx* ctx, int cls, double bytes, int launches = 1) {
   if (!ctx->prof) return -1;
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (!ctx->ev_pool.empty()) {
@@ -92,7 +97,7 @@ int prof_begin(tdgl_ctx* ctx, int cls, double bytes) {
     return -1;
   }
   if (hipEventRecord(ev.first, ctx->st) != hipSuccess) return -1;
-  ctx->pending.push_back({ev.first, ev.second, cls, bytes});
+  ctx->pending.push_back({ev.first, ev.second, cls, bytes, launches});
   return (int)ctx->pending.size() - 1;
 }
 void prof_end(tdgl_ctx* ctx, int tok) {
@@ -104,7 +109,7 @@ void prof_harvest(tdgl_ctx* ctx) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
       ctx->prof_ms[p.cls] += ms;
-      ctx->prof_n[p.cls] += 1;
+      ctx->prof_n[p.cls] += p.launches;
       ctx->prof_bytes[p.cls] += p.bytes;
     }
     ctx->ev_pool.push_back({p.a, p.b});
@@ -112,9 +117,28 @@ void prof_harvest(tdgl_ctx* ctx) {
   ctx->pending.clear();
 }
 
+// Reduction plumbing for the next launch: consume the last published partials,
+// publish into the other buffer.
+Red red_next(tdgl_ctx* ctx) {
+  Red r;
+  r.pin = ctx->d_pb[ctx->pb_cur];
+  r.pcount = ctx->pb_count;
+  r.pout = ctx->d_pb[ctx->pb_cur ^ 1];
+  r.res = ctx->d_res;
+  return r;
+}
+void red_commit(tdgl_ctx* ctx, int nblocks) {
+  ctx->pb_cur ^= 1;
+  ctx->pb_count = nblocks;
+}
+// Reduce the last published partials into res[slot0 ...] (one block).
+void red_finalize(tdgl_ctx* ctx, int nsum, int nmax, int slot0) {
+  launch_finalize(nsum, nmax, slot0, red_next(ctx), ctx->st);
+}
+
 // Scalars back to the host (synchronises the stream).
 int readback(tdgl_ctx* ctx, int slot, int count) {
-  HIPCHK(hipMemcpyAsync(ctx->h_res + slot, ctx->rb.res + slot, sizeof(double) * count,
+  HIPCHK(hipMemcpyAsync(ctx->h_res + slot, ctx->d_res + slot, sizeof(double) * count,
                         hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (!ctx->pending.empty()) prof_harvest(ctx);
@@ -152,17 +176,20 @@ int ensure_vec(tdgl_ctx* ctx, int i) {
   if ((int)ctx->P.size() > ctx->ptab_cap) {
     int cap = std::max(64, ctx->ptab_cap * 2);
     while (cap < (int)ctx->P.size()) cap *= 2;
+    HIPCHK(hipStreamSynchronize(ctx->st));  // no kernel may still read the old table
     if (ctx->d_ptab) HIPCHK(hipFree(ctx->d_ptab));
     ctx->d_ptab = nullptr;
     int rc;
     if ((rc = alloc(ctx, (void**)&ctx->d_ptab, sizeof(double*) * cap))) return rc;
     ctx->ptab_cap = cap;
-    HIPCHK(hipMemcpyAsync(ctx->d_ptab, ctx->P.data(), sizeof(double*) * ctx->P.size(),
-                          hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
-  } else {
-    HIPCHK(hipMemcpyAsync(ctx->d_ptab, ctx->P.data(), sizeof(double*) * ctx->P.size(),
-                          hipMemcpyHostToDevice, ctx->st));
+    ctx->ptab_len = 0;
+  }
+  if ((int)ctx->P.size() > ctx->ptab_len) {  // the table only grows: copy the new entries
+    HIPCHK(hipMemcpyAsync(ctx->d_ptab + ctx->ptab_len, ctx->P.data() + ctx->ptab_len,
+                          sizeof(double*) * (ctx->P.size() - ctx->ptab_len), hipMemcpyHostToDevice,
+                          ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));  // the host vector may reallocate later
+    ctx->ptab_len = (int)ctx->P.size();
   }
   return TDGL_OK;
 }
@@ -235,12 +262,13 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
 
   const int maxb = std::max(stencil_blocks(g, ctx->cfg), stream_blocks(g));
   int rc;
-  if ((rc = alloc(ctx, (void**)&ctx->rb.part, sizeof(double) * 4 * (size_t)maxb))) goto bad;
-  if ((rc = alloc(ctx, (void**)&ctx->rb.ticket, sizeof(unsigned int) * 4))) goto bad;
-  if ((rc = alloc(ctx, (void**)&ctx->rb.res, sizeof(double) * kResLen))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_pb[0], sizeof(double) * 4 * (size_t)maxb))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_pb[1], sizeof(double) * 4 * (size_t)maxb))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_res, sizeof(double) * kResLen))) goto bad;
   if ((rc = alloc(ctx, (void**)&ctx->d_bad, sizeof(int)))) goto bad;
-  if (hipMemset(ctx->rb.ticket, 0, sizeof(unsigned int) * 4) != hipSuccess ||
-      hipMemset(ctx->rb.res, 0, sizeof(double) * kResLen) != hipSuccess ||
+  if (hipMemset(ctx->d_pb[0], 0, sizeof(double) * 4 * (size_t)maxb) != hipSuccess ||
+      hipMemset(ctx->d_pb[1], 0, sizeof(double) * 4 * (size_t)maxb) != hipSuccess ||
+      hipMemset(ctx->d_res, 0, sizeof(double) * kResLen) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_res, sizeof(double) * kResLen, hipHostMallocDefault) != hipSuccess) {
     rc = TDGL_E_HIP;
     goto bad;
@@ -257,7 +285,7 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
   if (ctx->st) hipStreamSynchronize(ctx->st);
   else hipDeviceSynchronize();
   for (void* p : {(void*)ctx->d_flags, (void*)ctx->d_k2, (void*)ctx->d_sc, (void*)ctx->d_bfull,
-                  (void*)ctx->rb.part, (void*)ctx->rb.ticket, (void*)ctx->rb.res,
+                  (void*)ctx->d_pb[0], (void*)ctx->d_pb[1], (void*)ctx->d_res,
                   (void*)ctx->d_bad, (void*)ctx->w_x, (void*)ctx->w_gam, (void*)ctx->w_f,
                   (void*)ctx->w_r, (void*)ctx->d_ptab})
     if (p) hipFree(p);
@@ -425,7 +453,8 @@ int tdgl_rhs(tdgl_ctx* ctx, const double* x, double* f) {
   if (!ctx || !x || !f) return TDGL_E_INVALID;
   Epi e{};
   e.out0 = f;
-  launch_stencil(M_RHS, false, ctx->g, ctx->b, ctx->mt, x, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  launch_stencil(M_RHS, false, ctx->g, ctx->b, ctx->mt, x, nullptr, 0.0, e, ctx->cfg, red_next(ctx),
+                 ctx->st);
   LAUNCHCHK();
   return TDGL_OK;
 }
@@ -439,14 +468,16 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const dou
   e.ap0 = nullptr;
   e.hdt = dt / 2.0;
   e.inv_eps = 1.0 / eps;
-  launch_stencil(M_JVP, true, ctx->g, ctx->b, ctx->mt, x, p, eps, e, ctx->cfg, ctx->rb, ctx->st);
+  red_commit(ctx, launch_stencil(M_JVP, true, ctx->g, ctx->b, ctx->mt, x, p, eps, e, ctx->cfg,
+                                 red_next(ctx), ctx->st));
   LAUNCHCHK();
   return TDGL_OK;
 }
 
 int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {
   if (!ctx || !a || !b || !out) return TDGL_E_INVALID;
-  launch_dot(ctx->g, a, b, ctx->rb, ctx->st);
+  red_commit(ctx, launch_dot(ctx->g, a, b, red_next(ctx), ctx->st));
+  red_finalize(ctx, 1, 0, RS_DOT);
   LAUNCHCHK();
   int rc;
   if ((rc = readback(ctx, RS_DOT, 1))) return rc;
@@ -456,7 +487,8 @@ int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {
 
 int tdgl_norm_inf(tdgl_ctx* ctx, const double* a, double* out) {
   if (!ctx || !a || !out) return TDGL_E_INVALID;
-  launch_norm_inf(ctx->g, a, ctx->rb, ctx->st);
+  red_commit(ctx, launch_norm_inf(ctx->g, a, red_next(ctx), ctx->st));
+  red_finalize(ctx, 0, 1, RS_DOT);
   LAUNCHCHK();
   int rc;
   if ((rc = readback(ctx, RS_DOT, 1))) return rc;
@@ -471,7 +503,8 @@ int tdgl_euler_step(tdgl_ctx* ctx, double* X, double dt) {
   Epi e{};
   e.out0 = ctx->w_x;
   e.dt = dt;
-  launch_stencil(M_EULER, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  launch_stencil(M_EULER, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, red_next(ctx),
+                 ctx->st);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
   return TDGL_OK;
@@ -497,9 +530,10 @@ int newton_eval(tdgl_ctx* ctx, double hdt, tdgl_step_stats* s) {
   e.hdt = hdt;
   const double bytes = 4.0 * state_bytes(ctx);
   const int tok = prof_begin(ctx, TDGL_K_NEWTON, bytes);
-  launch_stencil(M_NEWTON, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0, e, ctx->cfg,
-                 ctx->rb, ctx->st);
+  red_commit(ctx, launch_stencil(M_NEWTON, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0, e,
+                                 ctx->cfg, red_next(ctx), ctx->st));
   prof_end(ctx, tok);
+  red_finalize(ctx, 2, 1, RS_XX);  // ||x||^2, ||r||^2, ||g||_inf
   LAUNCHCHK();
   s->rhs_evals++;
   s->bytes_moved += bytes;
@@ -520,6 +554,8 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
   double rk = r0;
   int nvec = 0, k = 0, rc;
   *ok = 1;
+  s->tgcr_r0 = r0;
+  s->tgcr_rk = r0;
   if (r0 == 0.0) {  // tgcr.py:118-119: x = 0
     s->tgcr_iters_last = 0;
     return TDGL_OK;
@@ -539,33 +575,38 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
     double by = (i > 0 ? 5.0 : 4.0) * S;
     int tok = prof_begin(ctx, TDGL_K_JVP, by);
-    launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, eps, e, ctx->cfg, ctx->rb, ctx->st);
+    red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, eps, e,
+                                   ctx->cfg, red_next(ctx), ctx->st));
     prof_end(ctx, tok);
     LAUNCHCHK();
     s->jvp_evals++;
     s->bytes_moved += by;
+    // the i-1 MGS passes run back to back: one event bracket around all of them
+    tok = (i > 1) ? prof_begin(ctx, TDGL_K_MGS, 4.0 * S * (i - 1), i - 1) : -1;
     for (int j = 1; j < i; ++j) {  // read Ap, Ap_{j-1}, Ap_j; write Ap
-      tok = prof_begin(ctx, TDGL_K_MGS, 4.0 * S);
-      launch_mgs_pass(g, ctx->AP[i], ctx->AP[j - 1], RS_BETA + j - 1, ctx->AP[j], RS_BETA + j,
-                      ctx->rb, ctx->st);
-      prof_end(ctx, tok);
+      red_commit(ctx, launch_mgs_pass(g, ctx->AP[i], ctx->AP[j - 1], RS_BETA + j - 1, ctx->AP[j],
+                                      red_next(ctx), ctx->st));
       s->bytes_moved += 4.0 * S;
     }
+    prof_end(ctx, tok);
     if (i > 0) {  // read Ap, Ap_{i-1}; write Ap
       tok = prof_begin(ctx, TDGL_K_MGS_LAST, 3.0 * S);
-      launch_mgs_last(g, ctx->AP[i], ctx->AP[i - 1], RS_BETA + i - 1, ctx->rb, ctx->st);
+      red_commit(ctx, launch_mgs_last(g, ctx->AP[i], ctx->AP[i - 1], RS_BETA + i - 1, red_next(ctx),
+                                      ctx->st));
       prof_end(ctx, tok);
       s->bytes_moved += 3.0 * S;
     }
     // read r, p_0..p_{i-1}, Ap; write p_i, Ap
     tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);
-    launch_final_p(g, ctx->w_r, ctx->d_ptab, i, ctx->AP[i], ctx->P[i], RS_ALPHA + i, ctx->rb, ctx->st);
+    red_commit(ctx, launch_final_p(g, ctx->w_r, ctx->d_ptab, i, ctx->AP[i], ctx->P[i], red_next(ctx),
+                                   ctx->st));
     prof_end(ctx, tok);
     tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
-    launch_update_r(g, ctx->w_r, ctx->AP[i], RS_ALPHA + i, ctx->rb, ctx->st);
+    red_commit(ctx, launch_update_r(g, ctx->w_r, ctx->AP[i], RS_ALPHA + i, red_next(ctx), ctx->st));
     prof_end(ctx, tok);
+    red_finalize(ctx, 1, 0, RS_RR);
     LAUNCHCHK();
-    if ((rc = readback(ctx, RS_BRK, 2))) return rc;
+    if ((rc = readback(ctx, RS_BRK, 3))) return rc;  // brk, (xx), rr
     if (ctx->h_res[RS_BRK] != 0.0) break;  // tgcr.py:141-142
     s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
     rk = sqrt(ctx->h_res[RS_RR]);
@@ -574,13 +615,15 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
   }
   s->tgcr_iters_last = k;
   s->tgcr_iters_total += k;
+  s->tgcr_rk = rk;
   if (rk > o->tol_gcr * r0) {  // tgcr.py:154-155
     *ok = 0;
     return TDGL_OK;
   }
   const int tok = prof_begin(ctx, TDGL_K_ACCUM, (double)(nvec + 2) * S);  // read x, p_i; write x
-  launch_accum(g, ctx->w_x, ctx->d_ptab, nvec, ctx->rb, ctx->st);
+  red_commit(ctx, launch_accum(g, ctx->w_x, ctx->d_ptab, nvec, red_next(ctx), ctx->st));
   prof_end(ctx, tok);
+  red_finalize(ctx, 0, 1, RS_DXINF);
   LAUNCHCHK();
   s->bytes_moved += (double)(nvec + 2) * S;
   return TDGL_OK;
@@ -605,13 +648,14 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   e.dt = dt;
   e.hdt = hdt;
   const int tok = prof_begin(ctx, TDGL_K_PRED, 3.0 * S);  // read X; write x_pred, gamma
-  launch_stencil(M_PRED, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, ctx->rb, ctx->st);
+  launch_stencil(M_PRED, false, ctx->g, ctx->b, ctx->mt, X, nullptr, 0.0, e, ctx->cfg, red_next(ctx),
+                 ctx->st);
   prof_end(ctx, tok);
   LAUNCHCHK();
   s->rhs_evals++;
   s->bytes_moved += 3.0 * S;
   if ((rc = newton_eval(ctx, hdt, s))) return rc;
-  if ((rc = readback(ctx, RS_RR, 3))) return rc;
+  if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
   for (int it = 1; it <= o->newton_max_iter; ++it) {
     const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
     int ok = 0;
@@ -623,7 +667,7 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
       return TDGL_OK;
     }
     if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
-    if ((rc = readback(ctx, RS_RR, 4))) return rc;
+    if ((rc = readback(ctx, RS_XX, 4))) return rc;  // xx, rr, ginf, dxinf
     s->g_inf = ctx->h_res[RS_GINF];
     s->dx_inf = ctx->h_res[RS_DXINF];
     if (!std::isfinite(s->g_inf) || !std::isfinite(s->dx_inf))

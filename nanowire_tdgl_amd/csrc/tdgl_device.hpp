@@ -163,13 +163,14 @@ __device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, co
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic grid reductions.
+// Deterministic grid reductions without atomics.
 //
-// Every block reduces its values in a fixed tree (wave shuffles, then the
-// waves in order), publishes them to `part`, and the last block to arrive
-// (ticket counter) sums the partials in block order.  The result does not
-// depend on block scheduling.  Partials and the counter go through
-// agent-scope atomics, so they are coherent across the XCDs' L2 caches.
+// A producing kernel reduces its values inside each block in a fixed tree
+// (wave shuffles, then the waves in order) and stores one partial per block.
+// The kernel boundary orders those stores before the next kernel, which
+// reduces the partials in block order in its prologue -- every consuming
+// block computes the same bits.  (A single ticket counter serialised ~1000
+// same-address atomics per launch: ~40 us at 242k nodes.)
 // ---------------------------------------------------------------------------
 
 template <typename T>
@@ -185,78 +186,53 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
-__device__ __forceinline__ void st_agent(double* p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
-                     (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double ld_agent(const double* p) {
-  unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)u);
-}
-
-// NS sums followed by NM maxima, reduced over the whole grid.  Block size
-// must be kRedThreads.  Returns true in the one block that finalised; in
-// that block out[] holds the grid-wide results for every thread.
+// NS sums then NM maxima over the block (blockDim.x == kRedThreads); the
+// result is valid in every thread.
 template <int NS, int NM>
-struct GridRed {
-  static constexpr int NV = NS + NM;
-  double* part;          // [gridDim.x][NV]
-  unsigned int* ticket;  // zero between launches
-
-  __device__ bool run(double (&v)[NV], double (&out)[NV]) const {
-    __shared__ double sh[kRedThreads / 64][NV > 0 ? NV : 1];
-    __shared__ unsigned int s_last;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+__device__ __forceinline__ void block_reduce(double (&v)[NS + NM]) {
+  constexpr int NV = NS + NM;
+  __shared__ double sh[kRedThreads / 64][NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int a = 0; a < NV; ++a) {
-      double r = (a < NS) ? wave_sum(v[a]) : wave_max(v[a]);
-      if (lane == 0) sh[wid][a] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int a = 0; a < NV; ++a) {
-        double r = sh[0][a];
-        for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
-        st_agent(&part[(long long)blockIdx.x * NV + a], r);
-      }
-      __threadfence();
-      unsigned int t = atomicAdd(ticket, 1u);
-      s_last = (t == gridDim.x - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int a = 0; a < NV; ++a) acc[a] = (a < NS) ? 0.0 : -1.0;
-    for (unsigned int bi = threadIdx.x; bi < gridDim.x; bi += kRedThreads) {
-#pragma unroll
-      for (int a = 0; a < NV; ++a) {
-        double r = ld_agent(&part[(long long)bi * NV + a]);
-        acc[a] = (a < NS) ? acc[a] + r : fmax(acc[a], r);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NV; ++a) {
-      double r = (a < NS) ? wave_sum(acc[a]) : wave_max(acc[a]);
-      if (lane == 0) sh[wid][a] = r;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < NV; ++a) {
-      double r = sh[0][a];
-      for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
-      out[a] = r;
-    }
-    if (threadIdx.x == 0)
-      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+  for (int a = 0; a < NV; ++a) {
+    const double r = (a < NS) ? wave_sum(v[a]) : wave_max(v[a]);
+    if (lane == 0) sh[wid][a] = r;
   }
-};
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    double r = sh[0][a];
+    for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
+    v[a] = r;
+  }
+  __syncthreads();
+}
+
+// Store this block's partials: pout[blockIdx.x * NV + a].
+template <int NS, int NM>
+__device__ __forceinline__ void publish(double (&v)[NS + NM], double* pout) {
+  block_reduce<NS, NM>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NS + NM; ++a) pout[(long long)blockIdx.x * (NS + NM) + a] = v[a];
+  }
+}
+
+// Reduce `count` published partials (block order); valid in every thread.
+template <int NS, int NM>
+__device__ __forceinline__ void gather(const double* pin, int count, double (&out)[NS + NM]) {
+  constexpr int NV = NS + NM;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) out[a] = 0.0;
+  for (int b = threadIdx.x; b < count; b += kRedThreads) {
+#pragma unroll
+    for (int a = 0; a < NV; ++a) {
+      const double r = pin[(long long)b * NV + a];
+      out[a] = (a < NS) ? out[a] + r : fmax(out[a], r);
+    }
+  }
+  block_reduce<NS, NM>(out);
+}
 
 // Block-index remap so that consecutive logical tiles land on the same XCD
 // (hardware dispatches block b to XCD b % 8), keeping stencil halos inside one L2.

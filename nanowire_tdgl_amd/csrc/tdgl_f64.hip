@@ -52,7 +52,7 @@ __device__ __forceinline__ double supercurrent(double cs, double sn, double pr, 
 
 template <bool IS3D, bool PERT, int MODE>
 __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double, PERT> s, Epi e,
-                                                int tiles_x, int tiles_y, int kz, RedBufs rb) {
+                                                int tiles_x, int tiles_y, int kz, Red r) {
   constexpr int NSLOT = IS3D ? 3 : 1;
   __shared__ double l_pr[NSLOT][CELLS], l_pi[NSLOT][CELLS], l_x[NSLOT][CELLS], l_y[NSLOT][CELLS],
       l_z[IS3D ? NSLOT : 1][IS3D ? CELLS : 1];
@@ -293,26 +293,7 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
     __syncthreads();
   }
 
-  if constexpr (NS + NM > 0) {
-    GridRed<NS, NM> gr{rb.part, rb.ticket};
-    double out[NS + NM];
-    if (gr.run(red, out) && threadIdx.x == 0) {
-      if (MODE == M_NEWTON) {
-        rb.res[RS_XX] = out[0];
-        rb.res[RS_RR] = out[1];
-        rb.res[RS_GINF] = out[2];
-      } else {  // JVP
-        if (e.ap0) {
-          rb.res[RS_BETA] = out[0];
-        } else {
-          const double nrm = sqrt(out[0]);
-          rb.res[RS_NRM] = nrm;
-          rb.res[RS_INV] = 1.0 / nrm;
-          rb.res[RS_BRK] = (nrm < 1e-14) ? 1.0 : 0.0;
-        }
-      }
-    }
-  }
+  if constexpr (NS + NM > 0) publish<NS, NM>(red, r.pout);
 }
 
 int stencil_blocks(const Geo& g, const StencilCfg& c) {
@@ -322,35 +303,36 @@ int stencil_blocks(const Geo& g, const StencilCfg& c) {
 }
 
 template <bool IS3D, bool PERT, int MODE>
-static void go_stencil(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* p,
-                       double eps, const Epi& e, const StencilCfg& cfg, const RedBufs& rb,
-                       hipStream_t st) {
+static int go_stencil(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* p,
+                      double eps, const Epi& e, const StencilCfg& cfg, const Red& r, hipStream_t st) {
   Src<double, PERT> s{x, p, eps, g.n};
   const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
   const int nb = stencil_blocks(g, cfg);
   hipLaunchKernelGGL((k_stencil<IS3D, PERT, MODE>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty,
-                     cfg.kz, rb);
+                     cfg.kz, r);
+  return nb;
 }
 
 template <bool IS3D>
-static void go_mode(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
-                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
-                    const RedBufs& rb, hipStream_t st) {
+static int go_mode(int mode, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                   const double* p, double eps, const Epi& e, const StencilCfg& cfg, const Red& r,
+                   hipStream_t st) {
   switch (mode) {
-    case M_RHS: go_stencil<IS3D, false, M_RHS>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
-    case M_PRED: go_stencil<IS3D, false, M_PRED>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
-    case M_NEWTON: go_stencil<IS3D, false, M_NEWTON>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
-    case M_JVP: go_stencil<IS3D, true, M_JVP>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
-    case M_EULER: go_stencil<IS3D, false, M_EULER>(g, b, mt, x, p, eps, e, cfg, rb, st); break;
+    case M_RHS: return go_stencil<IS3D, false, M_RHS>(g, b, mt, x, p, eps, e, cfg, r, st);
+    case M_PRED: return go_stencil<IS3D, false, M_PRED>(g, b, mt, x, p, eps, e, cfg, r, st);
+    case M_NEWTON: return go_stencil<IS3D, false, M_NEWTON>(g, b, mt, x, p, eps, e, cfg, r, st);
+    case M_JVP: return go_stencil<IS3D, true, M_JVP>(g, b, mt, x, p, eps, e, cfg, r, st);
+    case M_EULER: return go_stencil<IS3D, false, M_EULER>(g, b, mt, x, p, eps, e, cfg, r, st);
   }
-  (void)pert;
+  return 0;
 }
 
-void launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
-                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
-                    const RedBufs& rb, hipStream_t st) {
-  if (g.is3d) go_mode<true>(mode, pert, g, b, mt, x, p, eps, e, cfg, rb, st);
-  else go_mode<false>(mode, pert, g, b, mt, x, p, eps, e, cfg, rb, st);
+int launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                   const double* p, double eps, const Epi& e, const StencilCfg& cfg, const Red& r,
+                   hipStream_t st) {
+  (void)pert;
+  return g.is3d ? go_mode<true>(mode, g, b, mt, x, p, eps, e, cfg, r, st)
+                : go_mode<false>(mode, g, b, mt, x, p, eps, e, cfg, r, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -385,7 +367,9 @@ __device__ __forceinline__ V5 scal(const V5& u, double c) {
   return V5{u.a * c, u.b * c, u.x * c, u.y * c, u.z * c};
 }
 
-constexpr int kStreamCap = 4096;
+// Streaming grid: at most 1024 blocks (4 per CU), so consumers reduce at most
+// 1024 partials in their prologue.
+constexpr int kStreamCap = 1024;
 
 int stream_blocks(const Geo& g) {
   long long nb = (g.n + kRedThreads - 1) / kRedThreads;
@@ -396,58 +380,91 @@ int stream_blocks(const Geo& g) {
   for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n; \
        q += (long long)gridDim.x * kRedThreads)
 
-// Ap -= beta_{j-1} Ap_{j-1};  beta_j = <Ap, Ap_j>      (tgcr.py:135-138)
+// Ap -= beta_{j-1} Ap_{j-1};  partials of beta_j = <Ap, Ap_j>   (tgcr.py:135-138)
+// beta_{j-1} comes from the previous launch's partials; block 0 records it.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_mgs_pass(long long n, double* ap, const double* apm,
-                                                          int bm, const double* apn, int bn,
-                                                          RedBufs rb) {
-  const double beta = rb.res[bm];
+                                                          int bm, const double* apn, Red r) {
+  // the first node's loads go out before the partials are gathered, so the
+  // two memory latencies overlap
+  const long long q0 = (long long)blockIdx.x * kRedThreads + threadIdx.x;
+  const long long stride = (long long)gridDim.x * kRedThreads;
+  V5 a0{}, m0{}, n0{};
+  if (q0 < n) {
+    a0 = ld5<IS3D>(ap, n, q0);
+    m0 = ld5<IS3D>(apm, n, q0);
+    n0 = ld5<IS3D>(apn, n, q0);
+  }
+  double b1[1];
+  gather<1, 0>(r.pin, r.pcount, b1);
+  const double beta = b1[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) r.res[bm] = beta;
   double v[1] = {0.0};
-  GRID_STRIDE(q) {
+  if (q0 < n) {
+    const V5 a = axmy(a0, beta, m0);
+    st5<IS3D>(ap, n, q0, a);
+    v[0] += dot5(a, n0);
+  }
+  for (long long q = q0 + stride; q < n; q += stride) {
     const V5 a = axmy(ld5<IS3D>(ap, n, q), beta, ld5<IS3D>(apm, n, q));
     st5<IS3D>(ap, n, q, a);
     v[0] += dot5(a, ld5<IS3D>(apn, n, q));
   }
-  GridRed<1, 0> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[bn] = out[0];
+  publish<1, 0>(v, r.pout);
 }
 
-// Ap -= beta_last Ap_last;  ||Ap||, 1/||Ap||, breakdown flag   (tgcr.py:138-142)
+// Ap -= beta_last Ap_last;  partials of ||Ap||^2   (tgcr.py:138-140)
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_mgs_last(long long n, double* ap, const double* apm,
-                                                          int bm, RedBufs rb) {
-  const double beta = rb.res[bm];
+                                                          int bm, Red r) {
+  const long long q0 = (long long)blockIdx.x * kRedThreads + threadIdx.x;
+  const long long stride = (long long)gridDim.x * kRedThreads;
+  V5 a0{}, m0{};
+  if (q0 < n) {
+    a0 = ld5<IS3D>(ap, n, q0);
+    m0 = ld5<IS3D>(apm, n, q0);
+  }
+  double b1[1];
+  gather<1, 0>(r.pin, r.pcount, b1);
+  const double beta = b1[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) r.res[bm] = beta;
   double v[1] = {0.0};
-  GRID_STRIDE(q) {
+  if (q0 < n) {
+    const V5 a = axmy(a0, beta, m0);
+    st5<IS3D>(ap, n, q0, a);
+    v[0] += dot5(a, a);
+  }
+  for (long long q = q0 + stride; q < n; q += stride) {
     const V5 a = axmy(ld5<IS3D>(ap, n, q), beta, ld5<IS3D>(apm, n, q));
     st5<IS3D>(ap, n, q, a);
     v[0] += dot5(a, a);
   }
-  GridRed<1, 0> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) {
-    const double nrm = sqrt(out[0]);
-    rb.res[RS_NRM] = nrm;
-    rb.res[RS_INV] = 1.0 / nrm;
-    rb.res[RS_BRK] = (nrm < 1e-14) ? 1.0 : 0.0;
-  }
+  publish<1, 0>(v, r.pout);
 }
 
-// p_k = (r - sum_j beta_j p_j) / ||Ap|| ; Ap_k = Ap / ||Ap|| ; alpha = <r, Ap_k>
+// ||Ap|| from the previous partials; breakdown test (tgcr.py:140-142);
+// p_k = (r - sum_j beta_j p_j) / ||Ap|| ; Ap_k = Ap / ||Ap|| ; partials of alpha = <r, Ap_k>.
 // The p updates of tgcr.py:137 are applied here, per node, in the same j order,
 // which rounds exactly like the reference's vector-by-vector updates.
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const double* r,
+__global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const double* rv,
                                                          const double* const* ptab, int nprev,
-                                                         double* ap, double* pout, int aslot,
-                                                         RedBufs rb) {
-  if (rb.res[RS_BRK] != 0.0) return;
-  const double inv = rb.res[RS_INV];
-  const double* beta = rb.res + RS_BETA;
+                                                         double* ap, double* pout, Red r) {
+  double s1[1];
+  gather<1, 0>(r.pin, r.pcount, s1);
+  const double nrm = sqrt(s1[0]);
+  const double inv = 1.0 / nrm;
+  const bool brk = nrm < 1e-14;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    r.res[RS_NRM] = nrm;
+    r.res[RS_INV] = inv;
+    r.res[RS_BRK] = brk ? 1.0 : 0.0;
+  }
+  if (brk) return;
+  const double* beta = r.res + RS_BETA;
   double v[1] = {0.0};
   GRID_STRIDE(q) {
-    const V5 rr = ld5<IS3D>(r, n, q);
+    const V5 rr = ld5<IS3D>(rv, n, q);
     V5 p = rr;
     for (int jj = 0; jj < nprev; ++jj) p = axmy(p, beta[jj], ld5<IS3D>(ptab[jj], n, q));
     const V5 a = scal(ld5<IS3D>(ap, n, q), inv);
@@ -456,35 +473,46 @@ __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const doub
     st5<IS3D>(pout, n, q, p);
     v[0] += dot5(rr, a);
   }
-  GridRed<1, 0> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[aslot] = out[0];
+  publish<1, 0>(v, r.pout);
 }
 
-// r -= alpha Ap ; ||r||^2   (tgcr.py:150-152)
+// alpha from the previous partials (block 0 records it); r -= alpha Ap;
+// partials of ||r||^2   (tgcr.py:147-152)
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_update_r(long long n, double* r, const double* ap,
-                                                          int aslot, RedBufs rb) {
-  if (rb.res[RS_BRK] != 0.0) return;
-  const double alpha = rb.res[aslot];
+__global__ __launch_bounds__(kRedThreads) void k_update_r(long long n, double* rv, const double* ap,
+                                                          int aslot, Red r) {
+  if (r.res[RS_BRK] != 0.0) return;
+  const long long q0 = (long long)blockIdx.x * kRedThreads + threadIdx.x;
+  const long long stride = (long long)gridDim.x * kRedThreads;
+  V5 r0{}, a0{};
+  if (q0 < n) {
+    r0 = ld5<IS3D>(rv, n, q0);
+    a0 = ld5<IS3D>(ap, n, q0);
+  }
+  double a1[1];
+  gather<1, 0>(r.pin, r.pcount, a1);
+  const double alpha = a1[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) r.res[aslot] = alpha;
   double v[1] = {0.0};
-  GRID_STRIDE(q) {
-    const V5 a = axmy(ld5<IS3D>(r, n, q), alpha, ld5<IS3D>(ap, n, q));
-    st5<IS3D>(r, n, q, a);
+  if (q0 < n) {
+    const V5 a = axmy(r0, alpha, a0);
+    st5<IS3D>(rv, n, q0, a);
     v[0] += dot5(a, a);
   }
-  GridRed<1, 0> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_RR] = out[0];
+  for (long long q = q0 + stride; q < n; q += stride) {
+    const V5 a = axmy(ld5<IS3D>(rv, n, q), alpha, ld5<IS3D>(ap, n, q));
+    st5<IS3D>(rv, n, q, a);
+    v[0] += dot5(a, a);
+  }
+  publish<1, 0>(v, r.pout);
 }
 
 // dx = sum_i alpha_i p_i (the x += alpha p sequence of tgcr.py:148, per node);
-// x += dx (newton.py:108); ||dx||_inf (newton.py:113)
+// x += dx (newton.py:108); partials of ||dx||_inf (newton.py:113)
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_accum(long long n, double* x,
-                                                       const double* const* ptab, int nvec,
-                                                       RedBufs rb) {
-  const double* alpha = rb.res + RS_ALPHA;
+                                                       const double* const* ptab, int nvec, Red r) {
+  const double* alpha = r.res + RS_ALPHA;
   double v[1] = {0.0};
   GRID_STRIDE(q) {
     V5 d{0.0, 0.0, 0.0, 0.0, 0.0};
@@ -497,31 +525,33 @@ __global__ __launch_bounds__(kRedThreads) void k_accum(long long n, double* x,
     st5<IS3D>(x, n, q, V5{xv.a + d.a, xv.b + d.b, xv.x + d.x, xv.y + d.y, xv.z + d.z});
     v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
   }
-  GridRed<0, 1> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DXINF] = out[0];
+  publish<0, 1>(v, r.pout);
 }
 
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_dot(long long n, const double* a, const double* b,
-                                                     RedBufs rb) {
+                                                     Red r) {
   double v[1] = {0.0};
   GRID_STRIDE(q) { v[0] += dot5(ld5<IS3D>(a, n, q), ld5<IS3D>(b, n, q)); }
-  GridRed<1, 0> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DOT] = out[0];
+  publish<1, 0>(v, r.pout);
 }
 
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_norm_inf(long long n, const double* a, RedBufs rb) {
+__global__ __launch_bounds__(kRedThreads) void k_norm_inf(long long n, const double* a, Red r) {
   double v[1] = {0.0};
   GRID_STRIDE(q) {
     const V5 d = ld5<IS3D>(a, n, q);
     v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
   }
-  GridRed<0, 1> gr{rb.part, rb.ticket};
-  double out[1];
-  if (gr.run(v, out) && threadIdx.x == 0) rb.res[RS_DOT] = out[0];
+  publish<0, 1>(v, r.pout);
+}
+
+template <int NS, int NM>
+__global__ __launch_bounds__(kRedThreads) void k_finalize(Red r, int slot0) {
+  double out[NS + NM];
+  gather<NS, NM>(r.pin, r.pcount, out);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < NS + NM; ++a) r.res[slot0 + a] = out[a];
 }
 
 #define LAUNCH_V(kern, ...)                                                                 \
@@ -532,31 +562,46 @@ __global__ __launch_bounds__(kRedThreads) void k_norm_inf(long long n, const dou
       hipLaunchKernelGGL(kern<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
   } while (0)
 
-void launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int bm, const double* ap_next,
-                     int bn, const RedBufs& rb, hipStream_t st) {
-  LAUNCH_V(k_mgs_pass, g.n, ap, ap_prev, bm, ap_next, bn, rb);
+int launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int bm, const double* ap_next,
+                    const Red& r, hipStream_t st) {
+  LAUNCH_V(k_mgs_pass, g.n, ap, ap_prev, bm, ap_next, r);
+  return stream_blocks(g);
 }
-void launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int bm, const RedBufs& rb,
-                     hipStream_t st) {
-  LAUNCH_V(k_mgs_last, g.n, ap, ap_prev, bm, rb);
+int launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int bm, const Red& r,
+                    hipStream_t st) {
+  LAUNCH_V(k_mgs_last, g.n, ap, ap_prev, bm, r);
+  return stream_blocks(g);
 }
-void launch_final_p(const Geo& g, const double* r, const double* const* ptab, int nprev, double* ap,
-                    double* pout, int aslot, const RedBufs& rb, hipStream_t st) {
-  LAUNCH_V(k_final_p, g.n, r, ptab, nprev, ap, pout, aslot, rb);
+int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
+                   double* pout, const Red& r, hipStream_t st) {
+  LAUNCH_V(k_final_p, g.n, rv, ptab, nprev, ap, pout, r);
+  return stream_blocks(g);
 }
-void launch_update_r(const Geo& g, double* r, const double* ap, int aslot, const RedBufs& rb,
-                     hipStream_t st) {
-  LAUNCH_V(k_update_r, g.n, r, ap, aslot, rb);
+int launch_update_r(const Geo& g, double* rv, const double* ap, int aslot, const Red& r,
+                    hipStream_t st) {
+  LAUNCH_V(k_update_r, g.n, rv, ap, aslot, r);
+  return stream_blocks(g);
 }
-void launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const RedBufs& rb,
-                  hipStream_t st) {
-  LAUNCH_V(k_accum, g.n, x, ptab, nvec, rb);
+int launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const Red& r,
+                 hipStream_t st) {
+  LAUNCH_V(k_accum, g.n, x, ptab, nvec, r);
+  return stream_blocks(g);
 }
-void launch_dot(const Geo& g, const double* a, const double* b, const RedBufs& rb, hipStream_t st) {
-  LAUNCH_V(k_dot, g.n, a, b, rb);
+int launch_dot(const Geo& g, const double* a, const double* b, const Red& r, hipStream_t st) {
+  LAUNCH_V(k_dot, g.n, a, b, r);
+  return stream_blocks(g);
 }
-void launch_norm_inf(const Geo& g, const double* a, const RedBufs& rb, hipStream_t st) {
-  LAUNCH_V(k_norm_inf, g.n, a, rb);
+int launch_norm_inf(const Geo& g, const double* a, const Red& r, hipStream_t st) {
+  LAUNCH_V(k_norm_inf, g.n, a, r);
+  return stream_blocks(g);
+}
+void launch_finalize(int nsum, int nmax, int slot0, const Red& r, hipStream_t st) {
+  if (nsum == 1 && nmax == 0)
+    hipLaunchKernelGGL((k_finalize<1, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+  else if (nsum == 0 && nmax == 1)
+    hipLaunchKernelGGL((k_finalize<0, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+  else
+    hipLaunchKernelGGL((k_finalize<2, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
 }
 
 // ---------------------------------------------------------------------------

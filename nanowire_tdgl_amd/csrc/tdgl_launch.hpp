@@ -13,9 +13,9 @@ enum ResSlot : int {
   RS_NRM = 0,    // ||Ap||_2 after Gram-Schmidt
   RS_INV = 1,    // 1/||Ap||_2
   RS_BRK = 2,    // 1.0 when ||Ap|| < 1e-14 (tgcr.py:141-142)
-  RS_RR = 3,     // ||r||_2^2
-  RS_XX = 4,     // ||x_lin||_2^2
-  RS_GINF = 5,   // ||g||_inf
+  RS_XX = 3,     // ||x_lin||_2^2   } the Newton residual kernel's three
+  RS_RR = 4,     // ||r||_2^2       } results, in publish order
+  RS_GINF = 5,   // ||g||_inf       }
   RS_DXINF = 6,  // ||dx||_inf
   RS_DOT = 7,    // generic dot / norm result
   RS_BETA = 16,  // beta_j, j < kMaxKrylov
@@ -26,10 +26,14 @@ constexpr int kResLen = RS_ALPHA + kMaxKrylov;
 
 enum StencilMode : int { M_RHS = 0, M_PRED = 1, M_NEWTON = 2, M_JVP = 3, M_EULER = 4 };
 
-struct RedBufs {
-  double* part;          // partial sums, >= max grid * 4 doubles
-  unsigned int* ticket;  // zero-initialised counter
-  double* res;           // kResLen doubles
+// Reduction plumbing of one launch: the partials it consumes in its prologue
+// (published by the previous launch), where it publishes its own, and the
+// scalar slots.
+struct Red {
+  const double* pin;  // partials to consume, or nullptr
+  int pcount;         // number of partial blocks in pin
+  double* pout;       // this launch's partials (>= grid * 4 doubles)
+  double* res;        // kResLen doubles
 };
 
 // Extra inputs/outputs of the stencil epilogue.
@@ -51,23 +55,26 @@ struct StencilCfg {
 int stencil_blocks(const Geo& g, const StencilCfg& c);
 int stream_blocks(const Geo& g);
 
-void launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
-                    const double* p, double eps, const Epi& e, const StencilCfg& cfg,
-                    const RedBufs& rb, hipStream_t st);
+int launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& mt, const double* x,
+                   const double* p, double eps, const Epi& e, const StencilCfg& cfg, const Red& r,
+                   hipStream_t st);
 
-// Krylov / vector kernels (state-shaped vectors, internal layout).
-void launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int beta_prev_slot,
-                     const double* ap_next, int beta_next_slot, const RedBufs& rb, hipStream_t st);
-void launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int beta_slot,
-                     const RedBufs& rb, hipStream_t st);
-void launch_final_p(const Geo& g, const double* r, const double* const* ptab, int nprev, double* ap,
-                    double* pout, int alpha_slot, const RedBufs& rb, hipStream_t st);
-void launch_update_r(const Geo& g, double* r, const double* ap, int alpha_slot, const RedBufs& rb,
-                     hipStream_t st);
-void launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const RedBufs& rb,
-                  hipStream_t st);
-void launch_dot(const Geo& g, const double* a, const double* b, const RedBufs& rb, hipStream_t st);
-void launch_norm_inf(const Geo& g, const double* a, const RedBufs& rb, hipStream_t st);
+// Krylov / vector kernels (state-shaped vectors, internal layout).  Each
+// returns the number of partials it publishes.
+int launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int beta_prev_slot,
+                    const double* ap_next, const Red& r, hipStream_t st);
+int launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int beta_slot, const Red& r,
+                    hipStream_t st);
+int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
+                   double* pout, const Red& r, hipStream_t st);
+int launch_update_r(const Geo& g, double* rv, const double* ap, int alpha_slot, const Red& r,
+                    hipStream_t st);
+int launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const Red& r,
+                 hipStream_t st);
+int launch_dot(const Geo& g, const double* a, const double* b, const Red& r, hipStream_t st);
+int launch_norm_inf(const Geo& g, const double* a, const Red& r, hipStream_t st);
+// One block: reduce r.pin into res[slot0 ...] (nsum sums then nmax maxima).
+void launch_finalize(int nsum, int nmax, int slot0, const Red& r, hipStream_t st);
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
 void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st);
 void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,

@@ -592,6 +592,120 @@ def eval_f_stencil(X, g: Grid, idx: dict, u, mat: Optional[Material] = None) -> 
     return np.concatenate(res)
 
 
+def jvp_stencil(X, P, g: Grid, idx: dict, u, mat: Optional[Material] = None) -> np.ndarray:
+    """Exact directional derivative J(X) P of eval_f (the operator of rhs.py:147-239).
+
+    The boundary rules are affine in the state (the applied field enters as
+    constants), so the perturbation is expanded with the same rules and a zero
+    field.  Per term: d[e^{i t} psi] = e^{i t}(d psi + i dt psi);
+    d[(1-|psi|^2)psi] = (1 - 2|psi|^2) d psi - psi^2 conj(d psi);
+    d Im[e^{-i t} conj(psi) psi1] = Im[e^{-i t}(-i dt conj(psi) psi1 + conj(d psi) psi1
+    + conj(psi) d psi1)]; the link operators are linear.  Hole-boundary rows are 0.
+    """
+    X = np.asarray(X, np.complex128)
+    P = np.asarray(P, np.complex128)
+    zero = tuple(np.zeros(g.dim_x) for _ in range(3))
+    x, y1, y2, y3 = (v.reshape(g.full_shape) for v in expand_with_bc(X, g, idx, u))
+    dx, d1, d2, d3 = (v.reshape(g.full_shape) for v in expand_with_bc(P, g, idx, zero))
+    nz3 = g.is_3d
+    Ki = slice(1, g.Nz) if nz3 else slice(0, 1)
+    Ji, Ii = slice(1, g.Ny), slice(1, g.Nx)
+
+    def sh(A, di=0, dj=0, dk=0):
+        return A[Ki.start + dk:Ki.stop + dk, Ji.start + dj:Ji.stop + dj, Ii.start + di:Ii.stop + di]
+
+    c, dc = sh(x), sh(dx)
+    lap = 0.0
+    axes = [((1, 0, 0), y1, d1, g.hx), ((0, 1, 0), y2, d2, g.hy)]
+    if nz3:
+        axes.append(((0, 0, 1), y3, d3, g.hz))
+    for (di, dj, dk), y, dy, h in axes:
+        m_ = (-di, -dj, -dk)
+        lap = lap + (np.exp(-1j * sh(y, *m_)) * (sh(dx, *m_) - 1j * sh(dy, *m_) * sh(x, *m_))
+                     - 2.0 * dc
+                     + np.exp(1j * sh(y)) * (sh(dx, di, dj, dk) + 1j * sh(dy) * sh(x, di, dj, dk))) / h ** 2
+    m = idx["interior_to_full"]
+    dgl = (1.0 - 2.0 * np.abs(c) ** 2) * dc - c * c * np.conj(dc)
+    if mat is not None:
+        k2 = np.zeros(g.dim_x)
+        k2[m] = mat.kappa[m] ** 2
+        sc = np.zeros(g.dim_x)
+        sc[m] = mat.interior_sc_mask
+        k2 = sh(k2.reshape(g.full_shape))
+        sc = sh(sc.reshape(g.full_shape))
+        dfpsi = sc * dgl - (1.0 - sc) * dc / TAU_RELAX
+    else:
+        k2 = g.kappa ** 2
+        dfpsi = dgl
+    out = [lap + dfpsi]
+    ph = {"x": (y1, d1, (1, 0, 0), g.hx), "y": (y2, d2, (0, 1, 0), g.hy)}
+    if nz3:
+        ph["z"] = (y3, d3, (0, 0, 1), g.hz)
+    for a, (ya, da_, va, ha) in ph.items():
+        acc = 0.0
+        for b, (yb, db_, vb, hb) in ph.items():
+            acc = acc + (2.0 if b == a else 1.0) * k2 / hb ** 2 * (
+                sh(da_, *[-t for t in vb]) - 2.0 * sh(da_) + sh(da_, *vb))
+            if b != a:
+                apb = tuple(p - q for p, q in zip(va, vb))
+                mb = tuple(-q for q in vb)
+                acc = acc + k2 / hb ** 2 * (-sh(db_, *va) + sh(db_) + sh(db_, *apb) - sh(db_, *mb))
+        x1, dx1 = sh(x, *va), sh(dx, *va)
+        acc = acc + np.imag(np.exp(-1j * sh(ya)) * (-1j * sh(da_) * np.conj(c) * x1
+                                                   + np.conj(dc) * x1 + np.conj(c) * dx1))
+        out.append(acc.astype(np.complex128))
+
+    def to_ref(A):
+        return np.ascontiguousarray(A.transpose(2, 1, 0)).ravel()
+
+    res = [to_ref(o) for o in out]
+    for q, ax in enumerate("xyz"):
+        hm = idx[f"hole_{ax}_bc_interior"]
+        if hm.size and q + 1 < len(res):
+            res[q + 1][hm] = 0.0
+    return np.concatenate(res)
+
+
+def tgcr_trap_exact(jv: Callable, b, dt, tol=1e-4, max_iter=None, stats=None):
+    """TGCR of tgcr.py:96-156 with an exact product jv(p) = J p in place of the
+    finite difference (the mixed-precision path's operator)."""
+    N = len(b)
+    if max_iter is None:
+        max_iter = max(N, int(round(0.2 * N)))
+    x = np.zeros_like(b)
+    r = b.copy()
+    r0 = np.linalg.norm(r)
+    rk = r0
+    if r0 == 0.0:
+        return x
+    P, AP = [], []
+    k = 0
+    while rk / r0 > tol and k < max_iter:
+        k += 1
+        p = r.copy()
+        Ap = p - (dt / 2.0) * jv(p)
+        for pj, apj in zip(P, AP):
+            beta = np.vdot(Ap, apj).real
+            p = p - beta * pj
+            Ap = Ap - beta * apj
+        nrm = np.linalg.norm(Ap)
+        if nrm < 1e-14:
+            break
+        Ap /= nrm
+        p /= nrm
+        P.append(p)
+        AP.append(Ap)
+        alpha = np.vdot(r, Ap).real
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rk = np.linalg.norm(r)
+    if stats is not None:
+        stats["tgcr_iters"] = k
+    if rk > tol * r0:
+        return np.array([], dtype=np.complex128)
+    return x
+
+
 eval_f = eval_f_csr
 
 

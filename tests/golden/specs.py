@@ -58,7 +58,7 @@ EVALF_CASES = {
 # trapezoidal trajectories
 TRAP_C1 = dict(p=dict(Nx=16, Ny=16, Nz=8, hx=0.5, hy=0.5, hz=0.5, kappa=2.0), holes=[],
                field=dict(Bz=0.3, t_on_fraction=1.0), seed=0, noise=0.01, dt=0.1,
-               tol_gcr=1e-8, tgcr_max_iter=30, t_stop=10.0, t_stop_full=20.0)
+               tol_gcr=1e-8, tgcr_max_iter=30, t_stop=7.571, t_stop_abort=(10.0, 20.0))
 TRAP_HOLE = dict(p=dict(Nx=12, Ny=10, Nz=4, hx=0.5, hy=0.5, hz=0.5, kappa=4.0),
                  holes=[([(2.0, 1.5), (3.6, 1.5), (3.6, 3.4), (2.0, 3.4)], None)],
                  field=dict(Bz=0.4), seed=1, noise=1e-3, dt=0.2, tol_gcr=1e-8,
