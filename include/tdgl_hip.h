@@ -71,6 +71,11 @@ typedef struct tdgl_solver_opts {
   double tol_gcr;
   double eps_mf;
   int64_t tgcr_max_iter;
+  /* TDGL_PREC_MIXED only: the TGCR tolerance used is max(tol_gcr,
+   * mixed_tol_floor) (<= 0 selects 1e-6).  FP32 Krylov vectors cannot resolve
+   * relative residuals much below 1e-7; the FP64 Newton residual still
+   * decides convergence (inexact Newton). */
+  double mixed_tol_floor;
 } tdgl_solver_opts;
 
 /* Per-step counters (the reference keeps none; they make iteration-count
@@ -141,6 +146,12 @@ int tdgl_rhs(tdgl_ctx* ctx, const double* x_dev, double* f_dev);
  * with f0 = f(x) supplied by the caller. */
 int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x_dev, const double* f0_dev,
                      const double* p_dev, double eps, double dt, double* ap_dev);
+
+/* Ap = p - dt/2 * J(x) p with the analytic Jacobian of eval_f, FP32 vectors in
+ * the internal layout.  The TGCR of TDGL_PREC_MIXED uses this in place of the
+ * finite difference of tgcr.py:129-133 (its eps -> 0 limit). */
+int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32_dev, const float* p32_dev, double dt,
+                    float* ap32_dev);
 
 /* Reductions over internal-layout vectors (the reference's np.vdot(...).real,
  * np.linalg.norm(., 2) and np.linalg.norm(., inf): tgcr.py:116,129,136,140,149,152;

@@ -26,7 +26,7 @@ EXPORTS = [
     "tdgl_ctx_create", "tdgl_ctx_destroy", "tdgl_set_stream", "tdgl_last_error",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
-    "tdgl_jvp_trap_fd", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
 ]
 
@@ -44,7 +44,7 @@ class SolverOpts(C.Structure):
     _fields_ = [("newton_tol_f", C.c_double), ("newton_tol_dx", C.c_double),
                 ("newton_max_iter", C.c_int32), ("precision", C.c_int32),
                 ("tol_gcr", C.c_double), ("eps_mf", C.c_double),
-                ("tgcr_max_iter", C.c_int64)]
+                ("tgcr_max_iter", C.c_int64), ("mixed_tol_floor", C.c_double)]
 
 
 class StepStats(C.Structure):
@@ -220,6 +220,13 @@ class Context:
                    C.c_void_p(p.data_ptr()), C.c_double(eps), C.c_double(dt),
                    C.c_void_p(ap.data_ptr()))
 
+    def jvp_trap32(self, x32, p32, dt: float, ap32) -> None:
+        for v in (x32, p32, ap32):
+            if v.dtype != self.torch.float32 or v.numel() != self.len:
+                raise ValueError("jvp_trap32 takes float32 vectors of the internal state length")
+        self._call("tdgl_jvp_trap32", C.c_void_p(x32.data_ptr()), C.c_void_p(p32.data_ptr()),
+                   C.c_double(dt), C.c_void_p(ap32.data_ptr()))
+
     def dot(self, a, b) -> float:
         out = C.c_double()
         self._call("tdgl_dot", C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.byref(out))
@@ -261,8 +268,12 @@ class Context:
 
 
 def solver_opts(newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,
-                eps_mf=1e-4, tgcr_max_iter=None, precision=PREC_F64) -> SolverOpts:
+                eps_mf=1e-4, tgcr_max_iter=None, precision=PREC_F64,
+                mixed_tol_floor=1e-6) -> SolverOpts:
+    if isinstance(precision, str):
+        precision = {"f64": PREC_F64, "fp64": PREC_F64, "mixed": PREC_MIXED}[precision]
     return SolverOpts(float(newton_tol_f), float(newton_tol_dx), int(newton_max_iter),
                       int(precision), float(tol_gcr), float(eps_mf),
-                      int(tgcr_max_iter) if tgcr_max_iter is not None else 0)
+                      int(tgcr_max_iter) if tgcr_max_iter is not None else 0,
+                      float(mixed_tol_floor))
 

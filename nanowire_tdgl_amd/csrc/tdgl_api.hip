@@ -49,6 +49,15 @@ struct tdgl_ctx {
   int ptab_cap = 0;
   int ptab_len = 0;  // entries of d_ptab already copied
 
+  // mixed precision: FP32 Newton iterate copy, TGCR residual and Krylov basis
+  float* w_x32 = nullptr;
+  float* w_r32 = nullptr;
+  std::vector<float*> P32, AP32;
+  float** d_ptab32 = nullptr;   // p_j table
+  float** d_aptab32 = nullptr;  // Ap_j table
+  int ptab32_cap = 0, ptab32_len = 0;
+  double* d_mpart = nullptr;    // multi-dot partials: blocks x (kMaxKrylov + 1)
+
   // per-kernel-class event timing (tdgl_profile_enable)
   struct PendingEv {
     hipEvent_t a, b;
@@ -194,6 +203,58 @@ int ensure_vec(tdgl_ctx* ctx, int i) {
   return TDGL_OK;
 }
 
+int ensure_work32(tdgl_ctx* ctx) {
+  if (ctx->w_x32) return TDGL_OK;
+  const size_t bytes = sizeof(float) * (size_t)ctx->len;
+  int rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_x32, bytes))) return rc;
+  if ((rc = alloc(ctx, (void**)&ctx->w_r32, bytes))) return rc;
+  const size_t mp = sizeof(double) * (size_t)stream_blocks32(ctx->g) * (kMaxKrylov + 1);
+  if ((rc = alloc(ctx, (void**)&ctx->d_mpart, mp))) return rc;
+  return TDGL_OK;
+}
+
+// FP32 Krylov slot i; both pointer tables grow together.
+int ensure_vec32(tdgl_ctx* ctx, int i) {
+  if (i >= kMaxKrylov)
+    return fail(ctx, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
+  const size_t bytes = sizeof(float) * (size_t)ctx->len;
+  while ((int)ctx->P32.size() <= i) {
+    float *p = nullptr, *ap = nullptr;
+    int rc;
+    if ((rc = alloc(ctx, (void**)&p, bytes))) return rc;
+    if ((rc = alloc(ctx, (void**)&ap, bytes))) {
+      hipFree(p);
+      return rc;
+    }
+    ctx->P32.push_back(p);
+    ctx->AP32.push_back(ap);
+  }
+  if ((int)ctx->P32.size() > ctx->ptab32_cap) {
+    int cap = std::max(64, ctx->ptab32_cap * 2);
+    while (cap < (int)ctx->P32.size()) cap *= 2;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->d_ptab32) HIPCHK(hipFree(ctx->d_ptab32));
+    if (ctx->d_aptab32) HIPCHK(hipFree(ctx->d_aptab32));
+    ctx->d_ptab32 = ctx->d_aptab32 = nullptr;
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_ptab32, sizeof(float*) * cap))) return rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_aptab32, sizeof(float*) * cap))) return rc;
+    ctx->ptab32_cap = cap;
+    ctx->ptab32_len = 0;
+  }
+  if ((int)ctx->P32.size() > ctx->ptab32_len) {
+    const int from = ctx->ptab32_len, cnt = (int)ctx->P32.size() - from;
+    HIPCHK(hipMemcpyAsync(ctx->d_ptab32 + from, ctx->P32.data() + from, sizeof(float*) * cnt,
+                          hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->d_aptab32 + from, ctx->AP32.data() + from, sizeof(float*) * cnt,
+                          hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->ptab32_len = (int)ctx->P32.size();
+  }
+  return TDGL_OK;
+}
+
 void set_uniform_coeffs(tdgl_ctx* ctx, double kappa) {
   Mat& m = ctx->mt;
   const double k2 = kappa * kappa;  // sparse_operators.py:131 (kappa_m**2)
@@ -291,6 +352,11 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
     if (p) hipFree(p);
   for (double* p : ctx->P) hipFree(p);
   for (double* p : ctx->AP) hipFree(p);
+  for (float* p : ctx->P32) hipFree(p);
+  for (float* p : ctx->AP32) hipFree(p);
+  for (void* p : {(void*)ctx->w_x32, (void*)ctx->w_r32, (void*)ctx->d_ptab32,
+                  (void*)ctx->d_aptab32, (void*)ctx->d_mpart})
+    if (p) hipFree(p);
   for (auto& e : ctx->ev_pool) {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
@@ -474,6 +540,13 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const dou
   return TDGL_OK;
 }
 
+int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
+  if (!ctx || !x32 || !p32 || !ap32) return TDGL_E_INVALID;
+  launch_jvp32(ctx->g, ctx->b, ctx->mt, x32, p32, ap32, dt / 2.0, ctx->cfg, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
 int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {
   if (!ctx || !a || !b || !out) return TDGL_E_INVALID;
   red_commit(ctx, launch_dot(ctx->g, a, b, red_next(ctx), ctx->st));
@@ -629,16 +702,141 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
   return TDGL_OK;
 }
 
+// Mixed precision: the FP64 Newton residual, with x and r = -g also stored in FP32.
+int newton_eval32(tdgl_ctx* ctx, double hdt, tdgl_step_stats* s) {
+  Epi e{};
+  e.gam = ctx->w_gam;
+  e.hdt = hdt;
+  e.o32a = ctx->w_x32;
+  e.o32b = ctx->w_r32;
+  // read x, gamma (FP64); write x32, r32 (FP32)
+  const double bytes = 2.0 * state_bytes(ctx) + 2.0 * 4.0 * (double)ctx->len;
+  const int tok = prof_begin(ctx, TDGL_K_NEWTON, bytes);
+  red_commit(ctx, launch_stencil(M_NEWTON32, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0,
+                                 e, ctx->cfg, red_next(ctx), ctx->st));
+  prof_end(ctx, tok);
+  red_finalize(ctx, 2, 1, RS_XX);
+  LAUNCHCHK();
+  s->rhs_evals++;
+  s->bytes_moved += bytes;
+  return TDGL_OK;
+}
+
+// TGCR with the exact JVP, FP32 vectors and FP64 coefficients; classical
+// Gram-Schmidt with a conditional second pass (the device decides).  Same
+// stopping rules as tgcr.py:125-155.  On success w_x += dx (FP64).
+int tgcr32(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, int* ok,
+           tdgl_step_stats* s) {
+  const Geo& g = ctx->g;
+  const double S = 4.0 * (double)ctx->len;  // bytes of one FP32 state-shaped vector
+  const long long N = (1 + g.nphi) * g.n;
+  long long maxk = o->tgcr_max_iter;
+  if (maxk <= 0) maxk = std::max<long long>(N, llround(0.2 * (double)N));
+  const double floor_ = o->mixed_tol_floor > 0 ? o->mixed_tol_floor : 1e-6;
+  const double tol = std::max(o->tol_gcr, floor_);
+  const double r0 = sqrt(rr);
+  double rk = r0;
+  int nvec = 0, k = 0, rc;
+  *ok = 1;
+  s->tgcr_r0 = r0;
+  s->tgcr_rk = r0;
+  if (r0 == 0.0) {
+    s->tgcr_iters_last = 0;
+    return TDGL_OK;
+  }
+  const int nbm = stream_blocks32(g);
+  while (rk / r0 > tol && k < maxk) {
+    ++k;
+    const int i = nvec;
+    if ((rc = ensure_vec32(ctx, i))) return rc;
+    int tok = prof_begin(ctx, TDGL_K_JVP, 3.0 * S);  // read x32, p; write Ap
+    launch_jvp32(g, ctx->b, ctx->mt, ctx->w_x32, ctx->w_r32, ctx->AP32[i], hdt, ctx->cfg, ctx->st);
+    prof_end(ctx, tok);
+    s->jvp_evals++;
+    s->bytes_moved += 3.0 * S;
+    Red rm{nullptr, 0, ctx->d_mpart, ctx->d_res};
+    if (i == 0) {
+      tok = prof_begin(ctx, TDGL_K_MGS, S);
+      launch_mdot32(g, ctx->AP32[0], ctx->d_aptab32, 0, 1, 0, rm, ctx->st);
+      prof_end(ctx, tok);
+      launch_colsum(ctx->d_mpart, nbm, 1, 0, ctx->d_res, 0, RS_SELF, 0, ctx->st);
+      s->bytes_moved += S;
+    } else {
+      // pass 1: c = V^T Ap, ||Ap||^2; Ap -= V c
+      tok = prof_begin(ctx, TDGL_K_MGS, (double)(2 * i + 3) * S, 2);
+      launch_mdot32(g, ctx->AP32[i], ctx->d_aptab32, i, 1, 0, rm, ctx->st);
+      launch_colsum(ctx->d_mpart, nbm, i + 1, i, ctx->d_res, RS_BETA, RS_SELF, 0, ctx->st);
+      Red ru = red_next(ctx);
+      const int nbu = launch_cgs_upd32(g, ctx->AP32[i], ctx->d_aptab32, i, RS_BETA, 0, ru, ctx->st);
+      red_commit(ctx, nbu);
+      launch_colsum(ctx->d_pb[ctx->pb_cur], nbu, 1, 0, ctx->d_res, 0, RS_NRM1, 0, ctx->st);
+      prof_end(ctx, tok);
+      // pass 2 (runs on the device only when ||Ap'||^2 <= ||Ap||^2 / 2)
+      tok = prof_begin(ctx, TDGL_K_MGS, (double)(2 * i + 3) * S, 2);
+      launch_mdot32(g, ctx->AP32[i], ctx->d_aptab32, i, 0, 1, rm, ctx->st);
+      launch_colsum(ctx->d_mpart, nbm, i, i, ctx->d_res, RS_DBETA, RS_DOT, 1, ctx->st);
+      Red ru2 = red_next(ctx);
+      launch_cgs_upd32(g, ctx->AP32[i], ctx->d_aptab32, i, RS_DBETA, 1, ru2, ctx->st);
+      red_commit(ctx, nbu);
+      launch_colsum(ctx->d_pb[ctx->pb_cur], nbu, 1, 0, ctx->d_res, 0, RS_NRM2, 1, ctx->st);
+      prof_end(ctx, tok);
+      s->bytes_moved += (double)(2 * i + 3) * S;
+    }
+    tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);  // read r, p_j, Ap; write p_i, Ap
+    red_commit(ctx, launch_final_p32(g, ctx->w_r32, ctx->d_ptab32, i, ctx->AP32[i], ctx->P32[i],
+                                     RS_SELF, red_next(ctx), ctx->st));
+    prof_end(ctx, tok);
+    tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
+    red_commit(ctx, launch_update_r32(g, ctx->w_r32, ctx->AP32[i], RS_ALPHA + i, red_next(ctx),
+                                      ctx->st));
+    prof_end(ctx, tok);
+    red_finalize(ctx, 1, 0, RS_RR);
+    LAUNCHCHK();
+    if ((rc = readback(ctx, RS_BRK, 9))) return rc;  // brk .. nrm2
+    if (i > 0 && ctx->h_res[RS_NRM1] <= 0.5 * ctx->h_res[RS_SELF])
+      s->bytes_moved += (double)(2 * i + 3) * S;  // the second pass ran
+    else if (i > 0 && ctx->prof) {
+      ctx->prof_bytes[TDGL_K_MGS] -= (double)(2 * i + 3) * S;  // it did not
+      ctx->prof_n[TDGL_K_MGS] -= 2;
+    }
+    if (ctx->h_res[RS_BRK] != 0.0) break;
+    s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
+    rk = sqrt(ctx->h_res[RS_RR]);
+    nvec = i + 1;
+    if (!std::isfinite(rk)) return fail(ctx, TDGL_E_NONFINITE, "TGCR residual is not finite");
+  }
+  s->tgcr_iters_last = k;
+  s->tgcr_iters_total += k;
+  s->tgcr_rk = rk;
+  if (rk > tol * r0) {
+    *ok = 0;
+    return TDGL_OK;
+  }
+  const int tok = prof_begin(ctx, TDGL_K_ACCUM, (double)nvec * S + 2.0 * state_bytes(ctx));
+  red_commit(ctx, launch_accum32(g, ctx->w_x, ctx->d_ptab32, nvec, red_next(ctx), ctx->st));
+  prof_end(ctx, tok);
+  red_finalize(ctx, 0, 1, RS_DXINF);
+  LAUNCHCHK();
+  s->bytes_moved += (double)nvec * S + 2.0 * state_bytes(ctx);
+  return TDGL_OK;
+}
+
 }  // namespace
 
 extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
                               tdgl_step_stats* s) {
   if (!ctx || !X || !o || !s) return TDGL_E_INVALID;
-  if (o->precision != TDGL_PREC_F64) return fail(ctx, TDGL_E_INVALID, "only TDGL_PREC_F64 is built");
+  if (o->precision != TDGL_PREC_F64 && o->precision != TDGL_PREC_MIXED)
+    return fail(ctx, TDGL_E_INVALID, "precision must be TDGL_PREC_F64 or TDGL_PREC_MIXED");
+  const bool mixed = o->precision == TDGL_PREC_MIXED;
   memset(s, 0, sizeof(*s));
   int rc;
   if ((rc = ensure_work(ctx))) return rc;
-  if ((rc = ensure_vec(ctx, 0))) return rc;
+  if (mixed) {
+    if ((rc = ensure_work32(ctx))) return rc;
+  } else if ((rc = ensure_vec(ctx, 0))) {
+    return rc;
+  }
   const double hdt = dt / 2.0;
   const double S = state_bytes(ctx);
   // predictor + gamma (integrators.py:156-158)
@@ -654,19 +852,19 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   LAUNCHCHK();
   s->rhs_evals++;
   s->bytes_moved += 3.0 * S;
-  if ((rc = newton_eval(ctx, hdt, s))) return rc;
+  if ((rc = mixed ? newton_eval32(ctx, hdt, s) : newton_eval(ctx, hdt, s))) return rc;
   if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
   for (int it = 1; it <= o->newton_max_iter; ++it) {
     const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
     int ok = 0;
-    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
+    if ((rc = mixed ? tgcr32(ctx, hdt, o, rr, &ok, s) : tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
     if (!ok) {  // newton.py:102-106
       s->tgcr_failed = 1;
       s->newton_iters = it;
       s->converged = 0;
       return TDGL_OK;
     }
-    if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
+    if ((rc = mixed ? newton_eval32(ctx, hdt, s) : newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
     if ((rc = readback(ctx, RS_XX, 4))) return rc;  // xx, rr, ginf, dxinf
     s->g_inf = ctx->h_res[RS_GINF];
     s->dx_inf = ctx->h_res[RS_DXINF];

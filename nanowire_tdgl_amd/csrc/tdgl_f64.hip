@@ -72,8 +72,9 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
     ke = min(kb + kz, g.Nz);
   }
 
-  constexpr int NS = (MODE == M_NEWTON) ? 2 : (MODE == M_JVP ? 1 : 0);
-  constexpr int NM = (MODE == M_NEWTON) ? 1 : 0;
+  constexpr bool NEWTON = (MODE == M_NEWTON) || (MODE == M_NEWTON32);
+  constexpr int NS = NEWTON ? 2 : (MODE == M_JVP ? 1 : 0);
+  constexpr int NM = NEWTON ? 1 : 0;
   double red[NS + NM > 0 ? NS + NM : 1];
 #pragma unroll
   for (int a = 0; a < (NS + NM > 0 ? NS + NM : 1); ++a) red[a] = 0.0;
@@ -235,12 +236,15 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
           e.out1[3 * n + q] = Xy + e.hdt * dy_;
           if (IS3D) e.out1[4 * n + q] = Xz + e.hdt * dz_;
         }
-      } else if (MODE == M_NEWTON) {
-        // f, r = -g with g = x - dt/2 f - gamma (newton.py:98-100, :109-111)
-        o0[q] = make_double2(dpr, dpi);
-        e.out0[2 * n + q] = dx_;
-        e.out0[3 * n + q] = dy_;
-        if (IS3D) e.out0[4 * n + q] = dz_;
+      } else if (NEWTON) {
+        // f, r = -g with g = x - dt/2 f - gamma (newton.py:98-100, :109-111).
+        // NEWTON32 (mixed precision) stores x and r in FP32 instead of f and r.
+        if (MODE == M_NEWTON) {
+          o0[q] = make_double2(dpr, dpi);
+          e.out0[2 * n + q] = dx_;
+          e.out0[3 * n + q] = dy_;
+          if (IS3D) e.out0[4 * n + q] = dz_;
+        }
         const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
         const double2 G0 = reinterpret_cast<const double2*>(e.gam)[q];
         const double gr = (X0.x - e.hdt * dpr) - G0.x;
@@ -253,11 +257,24 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
           Xz = s.x[4 * n + q];
           gz = (Xz - e.hdt * dz_) - e.gam[4 * n + q];
         }
-        double2* o1 = reinterpret_cast<double2*>(e.out1);
-        o1[q] = make_double2(-gr, -gi);
-        e.out1[2 * n + q] = -gx;
-        e.out1[3 * n + q] = -gy;
-        if (IS3D) e.out1[4 * n + q] = -gz;
+        if (MODE == M_NEWTON) {
+          double2* o1 = reinterpret_cast<double2*>(e.out1);
+          o1[q] = make_double2(-gr, -gi);
+          e.out1[2 * n + q] = -gx;
+          e.out1[3 * n + q] = -gy;
+          if (IS3D) e.out1[4 * n + q] = -gz;
+        } else {
+          reinterpret_cast<float2*>(e.o32a)[q] = make_float2((float)X0.x, (float)X0.y);
+          e.o32a[2 * n + q] = (float)Xx;
+          e.o32a[3 * n + q] = (float)Xy;
+          reinterpret_cast<float2*>(e.o32b)[q] = make_float2((float)-gr, (float)-gi);
+          e.o32b[2 * n + q] = (float)-gx;
+          e.o32b[3 * n + q] = (float)-gy;
+          if (IS3D) {
+            e.o32a[4 * n + q] = (float)Xz;
+            e.o32b[4 * n + q] = (float)-gz;
+          }
+        }
         red[0] += ((X0.x * X0.x + X0.y * X0.y) + Xx * Xx) + (Xy * Xy + Xz * Xz);
         red[1] += ((gr * gr + gi * gi) + gx * gx) + (gy * gy + gz * gz);
         red[NS] = fmax(red[NS], fmax(hypot(gr, gi), fmax(fabs(gx), fmax(fabs(gy), fabs(gz)))));
@@ -323,6 +340,7 @@ static int go_mode(int mode, const Geo& g, const Bnd& b, const Mat& mt, const do
     case M_NEWTON: return go_stencil<IS3D, false, M_NEWTON>(g, b, mt, x, p, eps, e, cfg, r, st);
     case M_JVP: return go_stencil<IS3D, true, M_JVP>(g, b, mt, x, p, eps, e, cfg, r, st);
     case M_EULER: return go_stencil<IS3D, false, M_EULER>(g, b, mt, x, p, eps, e, cfg, r, st);
+    case M_NEWTON32: return go_stencil<IS3D, false, M_NEWTON32>(g, b, mt, x, p, eps, e, cfg, r, st);
   }
   return 0;
 }

@@ -73,19 +73,24 @@ def trapezoidal(x0, params: SimulationParameters, idx: GridIndices, eval_u: Call
                 save_every: int = 1, adaptive: bool = True, dt_min: float = 1e-8,
                 progress: bool = True, verbose: bool = False,
                 material: Optional[MaterialMap] = None, tgcr_max_iter: Optional[int] = None,
-                step_log: Optional[list] = None):
+                step_log: Optional[list] = None, precision: str = "f64",
+                mixed_tol_floor: float = 1e-6):
     """Implicit trapezoidal integration with Newton-TGCR (integrators.py:92-206).
 
     Extensions (defaults reproduce the reference): ``tgcr_max_iter`` caps the
     Krylov iterations per TGCR call (the reference always uses n_state);
-    ``step_log`` receives one dict of native counters per step attempt.
+    ``step_log`` receives one dict of native counters per step attempt;
+    ``precision="mixed"`` keeps the Newton residual and the state in FP64 but
+    runs TGCR with an analytic FP32 Jacobian-vector product and FP32 Krylov
+    vectors, with its tolerance floored at ``mixed_tol_floor``.
     """
     eng = engine_for(params, idx, material)
     ctx = eng.ctx
     X0 = np.array(x0, dtype=np.complex128)
     X = ctx.to_internal(X0)
     opts = _native.solver_opts(newton_tol_f, newton_tol_dx, newton_max_iter, tol_gcr, eps_mf,
-                               tgcr_max_iter)
+                               tgcr_max_iter, precision=precision,
+                               mixed_tol_floor=mixed_tol_floor)
     t = t_start
     cur = dt
     times, hist = [t], [X0.copy()]

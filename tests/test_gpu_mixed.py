@@ -1,0 +1,127 @@
+"""GPU: the mixed-precision trapezoidal path (analytic FP32 Jacobian-vector
+product and FP32 Krylov vectors inside an FP64 Newton iteration).
+
+* The FP32 JVP kernel is compared with the oracle's exact FP64 JVP evaluated
+  at the same FP32-rounded inputs: <= 2e-6 of the largest entry (FP32 unit
+  roundoff 6e-8 times the growth of a 7-point stencil with phase factors).
+* Mixed steps are compared with FP64 steps of the same library.  The Newton
+  residual is FP64 in both, so with a tight residual tolerance (||g||_inf <=
+  1e-10) both converge to the same solution of the FP64 step equations
+  (<= 1e-8 relative: 1e-10 times the conditioning of the step).  Much tighter
+  tolerances are out of reach for either path: TGCR's absolute breakdown test
+  ||Ap|| < 1e-14 (tgcr.py:141-142) ends the solve once the residual is that
+  small.  At the
+  reference's default Newton tolerances the two trajectories agree in |psi|^2,
+  B_z and J_s to <= 1e-6 relative (of the largest entry).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import specs as S
+from helpers import device, oracle_film, rand_state, rel_err
+from oracle import tdgl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _round32(v):
+    v = np.asarray(v, np.complex128)
+    return v.real.astype(np.float32).astype(np.float64) + 1j * v.imag.astype(np.float32)
+
+
+def _engine(dev):
+    from nanowire_tdgl_amd.engine import engine_for
+    return engine_for(dev.params, dev.idx, dev.material)
+
+
+@pytest.mark.parametrize("name", list(S.EVALF_CASES))
+def test_jvp32_matches_exact_jvp(name):
+    import torch
+    case = S.EVALF_CASES[name]
+    dev = device(case)
+    f = oracle_film(case)
+    eng = _engine(dev)
+    eng.set_field(*case["field"])
+    u = O.boundary_vectors(*case["field"], f.g, f.idx)
+    n = dev.params.n_interior
+    nphi = 3 if dev.params.Nz > 1 else 2
+    x = _round32(rand_state(n, nphi, 21, 0.3, 0.2))
+    p = _round32(rand_state(n, nphi, 22, 1.0, 1.0))
+    ctx = eng.ctx
+    gx, gp = ctx.to_internal(x).float(), ctx.to_internal(p).float()
+    ap = torch.empty_like(gx)
+    dt = 0.2
+    ctx.jvp_trap32(gx, gp, dt, ap)
+    got = ctx.to_reference(ap.double()).cpu().numpy()
+    want = p - (dt / 2.0) * O.jvp_stencil(x, p, f.g, f.idx, u, f.material)
+    err = rel_err(got, want)
+    print(f"{name}: jvp32 relative error {err:.2e}")
+    assert err < 2e-6
+
+
+def test_jvp32_rejects_wrong_vectors():
+    import torch
+    dev = device(S.EVALF_CASES["c1_bz"])
+    ctx = _engine(dev).ctx
+    x = torch.zeros(ctx.len, dtype=torch.float64, device=ctx.device)
+    with pytest.raises(ValueError):
+        ctx.jvp_trap32(x, x, 0.1, x)
+
+
+def _run(dev, X0, field, precision, steps, dt, **kw):
+    from nanowire_tdgl_amd import _native
+    import nanowire_tdgl_amd as T
+    eng = _engine(dev)
+    af = T.AppliedField(**field)
+    ctx = eng.ctx
+    X = ctx.to_internal(X0)
+    opts = _native.solver_opts(precision=precision, **kw)
+    stats = []
+    t = 0.0
+    for _ in range(steps):
+        eng.set_field(*af.evaluate(t, steps * dt))
+        st = ctx.trap_step(X, dt, opts)
+        assert st.converged == 1
+        stats.append(st.as_dict())
+        t += dt
+    b, j = ctx.observables(X)
+    psi = ctx.to_reference(X).cpu().numpy()[: dev.params.n_interior]
+    n = dev.params.n_interior
+    return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy(), stats
+
+
+def test_mixed_converges_to_the_fp64_solution():
+    case = S.TRAP_HOLE
+    dev = device(case)
+    n = dev.params.n_interior
+    X0 = rand_state(n, 3, 31, 0.05, 0.02)
+    tight = dict(newton_tol_f=1e-10, newton_tol_dx=1e-6, tol_gcr=1e-4)
+    a = _run(dev, X0, case["field"], "f64", 3, case["dt"], **tight)
+    b = _run(dev, X0, case["field"], "mixed", 3, case["dt"], **tight)
+    for k, name in enumerate(("|psi|^2", "B_z", "J_s")):
+        err = rel_err(b[k], a[k])
+        print(f"tight Newton: {name} relative difference {err:.2e}")
+        assert err < 1e-8, name
+
+
+@pytest.mark.parametrize("cname", ["TRAP_HOLE", "TRAP_C1"])
+def test_mixed_trajectory_error_vs_fp64(cname):
+    case = getattr(S, cname)
+    dev = device(case)
+    n = dev.params.n_interior
+    rng = np.random.default_rng(case["seed"])
+    X0 = np.concatenate([1.0 + case["noise"] * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                         np.zeros(3 * n, np.complex128)])
+    steps = 20
+    a = _run(dev, X0, case["field"], "f64", steps, case["dt"], tol_gcr=case["tol_gcr"])
+    b = _run(dev, X0, case["field"], "mixed", steps, case["dt"], tol_gcr=case["tol_gcr"])
+    for k, name in enumerate(("|psi|^2", "B_z", "J_s")):
+        err = rel_err(b[k], a[k])
+        print(f"{cname}: {name} relative difference after {steps} steps {err:.2e}")
+        assert err < 1e-6, name
+    print("newton f64/mixed:", sum(s["newton_iters"] for s in a[3]),
+          sum(s["newton_iters"] for s in b[3]),
+          "tgcr:", sum(s["tgcr_iters_total"] for s in a[3]), sum(s["tgcr_iters_total"] for s in b[3]))
