@@ -49,6 +49,7 @@ extern "C" {
 #define TDGL_PREC_MIXED 1      /* FP32 stencil/JVP/Krylov vectors, FP64 Newton and Gram scalars */
 
 typedef struct tdgl_ctx tdgl_ctx;
+typedef struct tdgl_batch tdgl_batch;
 
 /* Grid and physics parameters; mirrors SimulationParameters
  * (src/tdgl3d/core/parameters.py:9-94).  Nx, Ny, Nz count cells. */
@@ -102,6 +103,10 @@ typedef struct tdgl_step_stats {
 int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out);
 int tdgl_ctx_destroy(tdgl_ctx* ctx);
 int tdgl_set_stream(tdgl_ctx* ctx, void* hip_stream);
+/* Tuning (no counterpart in the reference): z planes per stencil work item
+ * (tiles are 64 x 4 x kz nodes) for this context's kernels, FP64 and mixed. */
+int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz);
+int tdgl_get_stencil_kz(const tdgl_ctx* ctx);
 const char* tdgl_last_error(const tdgl_ctx* ctx);
 /* Number of doubles in one internal state buffer ((2 + n_phi) * n_interior). */
 int64_t tdgl_state_len(const tdgl_ctx* ctx);
@@ -163,7 +168,9 @@ int tdgl_norm_inf(tdgl_ctx* ctx, const double* a_dev, double* out);
  * TGCR (integrators.py:151-172 + newton.py:76-120 + tgcr.py:96-156).
  * On convergence X_dev is overwritten with the new state and stats->converged
  * is 1; otherwise X_dev is left unchanged (the caller divides dt by 10 as
- * integrators.py:185-197 does). */
+ * integrators.py:185-197 does).  opts->precision selects the plain FP64
+ * method or the mixed-precision method (the film as a batch of one, see
+ * tdgl_batch_trap_step). */
 int tdgl_trap_step(tdgl_ctx* ctx, double* X_dev, double dt, const tdgl_solver_opts* opts,
                    tdgl_step_stats* stats);
 
@@ -179,6 +186,38 @@ int tdgl_euler_step(tdgl_ctx* ctx, double* X_dev, double dt);
  * Either output pointer may be NULL. */
 int tdgl_observables(tdgl_ctx* ctx, const double* x_dev, double* b_dev, double* js_dev);
 
+/* ---- Batches of films (mixed precision; no counterpart in the reference,
+ * whose trapezoidal() advances one film per call) ----
+ * A batch advances several films -- each a context with its own grid,
+ * material and field -- by one trapezoidal step, with tdgl3d's per-film
+ * control flow (integrators.py:151-172, newton.py:76-120, tgcr.py:96-156)
+ * run in lockstep: every kernel launch covers all films still iterating.
+ * The contexts must outlive the batch; field and material changes made on
+ * them are picked up at the next step.  All films are 3-D, or all 2-D.
+ * A batched state is one device buffer of tdgl_batch_total_len doubles; film
+ * f occupies tdgl_state_len(film f) doubles from tdgl_batch_offset(f) in the
+ * internal layout; the padding between films is never read. */
+int tdgl_batch_create(tdgl_ctx* const* films, int nfilm, tdgl_batch** out);
+int tdgl_batch_destroy(tdgl_batch* batch);
+int tdgl_batch_set_stream(tdgl_batch* batch, void* hip_stream);
+const char* tdgl_batch_last_error(const tdgl_batch* batch);
+int64_t tdgl_batch_total_len(const tdgl_batch* batch);
+int64_t tdgl_batch_offset(const tdgl_batch* batch, int film);
+/* Tuning: z planes per stencil work item of film `film` (the autotuner's knob). */
+int tdgl_batch_set_kz(tdgl_batch* batch, int film, int kz);
+int tdgl_batch_get_kz(const tdgl_batch* batch, int film);
+/* Average milliseconds of `reps` launches of the batch's FP32 JVP stencil over
+ * its work buffers, as currently laid out (what the autotuner measures). */
+int tdgl_batch_time_stencil(tdgl_batch* batch, int reps, double* ms);
+/* One step attempt for every film f with active[f] != 0 (active may be NULL:
+ * all), with its own dt[f] (host array).  stats[f] as tdgl_trap_step's;
+ * opts->precision must be TDGL_PREC_MIXED. */
+int tdgl_batch_trap_step(tdgl_batch* batch, double* X_dev, const double* dt, const int* active,
+                         const tdgl_solver_opts* opts, tdgl_step_stats* stats);
+int tdgl_batch_profile_enable(tdgl_batch* batch, int enable);
+int tdgl_batch_profile_read(tdgl_batch* batch, int kernel_class, double* ms_total,
+                            int64_t* launches, double* bytes_total);
+
 /* Instrumentation (no counterpart in the reference): when enabled, the driver
  * brackets every kernel launch of the trapezoidal step with HIP events on the
  * context's stream and accumulates, per kernel class, the summed duration,
@@ -186,12 +225,12 @@ int tdgl_observables(tdgl_ctx* ctx, const double* x_dev, double* b_dev, double* 
  * once).  tdgl_profile_read synchronises the stream. */
 #define TDGL_K_PRED 0      /* predictor + gamma stencil */
 #define TDGL_K_NEWTON 1    /* Newton residual stencil (f, r = -g, norms) */
-#define TDGL_K_JVP 2       /* fused finite-difference JVP stencil */
-#define TDGL_K_MGS 3       /* MGS pass: Ap -= beta Ap_j, next beta */
-#define TDGL_K_MGS_LAST 4  /* last MGS update + ||Ap|| */
-#define TDGL_K_FINAL_P 5   /* p_k assembly + normalisation + alpha */
-#define TDGL_K_UPDATE_R 6  /* r -= alpha Ap + ||r|| */
-#define TDGL_K_ACCUM 7     /* dx = sum alpha p; x += dx; ||dx||_inf */
+#define TDGL_K_JVP 2       /* fused JVP stencil (FP64 finite difference; mixed: analytic FP32) */
+#define TDGL_K_MGS 3       /* MGS pass: Ap -= beta Ap_j, next beta (mixed: first CGS pass) */
+#define TDGL_K_MGS_LAST 4  /* last MGS update + ||Ap|| (mixed: conditional second CGS pass) */
+#define TDGL_K_FINAL_P 5   /* p_k assembly + normalisation + alpha (mixed: Q_k, alpha, r update) */
+#define TDGL_K_UPDATE_R 6  /* r -= alpha Ap + ||r|| (mixed: stop tests) */
+#define TDGL_K_ACCUM 7     /* dx = sum alpha p; x += dx; ||dx||_inf (mixed: H y = alpha, x += R y) */
 #define TDGL_K_COUNT 8
 int tdgl_profile_enable(tdgl_ctx* ctx, int enable);
 int tdgl_profile_read(tdgl_ctx* ctx, int kernel_class, double* ms_total, int64_t* launches,

@@ -24,10 +24,15 @@ PREC_MIXED = 1
 
 EXPORTS = [
     "tdgl_ctx_create", "tdgl_ctx_destroy", "tdgl_set_stream", "tdgl_last_error",
+    "tdgl_set_stencil_kz", "tdgl_get_stencil_kz",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
     "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
+    "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
+    "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
+    "tdgl_batch_time_stencil",
+    "tdgl_batch_trap_step", "tdgl_batch_profile_enable", "tdgl_batch_profile_read",
 ]
 
 KERNEL_CLASSES = ["pred", "newton", "jvp", "mgs", "mgs_last", "final_p", "update_r", "accum"]
@@ -94,6 +99,21 @@ def load() -> C.CDLL:
         "tdgl_observables": ([P, P, P, P], C.c_int),
         "tdgl_profile_enable": ([P, C.c_int], C.c_int),
         "tdgl_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
+        "tdgl_jvp_trap32": ([P, P, P, D, P], C.c_int),
+        "tdgl_set_stencil_kz": ([P, C.c_int], C.c_int),
+        "tdgl_get_stencil_kz": ([P], C.c_int),
+        "tdgl_batch_create": ([P, C.c_int, C.POINTER(P)], C.c_int),
+        "tdgl_batch_destroy": ([P], C.c_int),
+        "tdgl_batch_set_stream": ([P, P], C.c_int),
+        "tdgl_batch_last_error": ([P], C.c_char_p),
+        "tdgl_batch_total_len": ([P], I64),
+        "tdgl_batch_offset": ([P, C.c_int], I64),
+        "tdgl_batch_set_kz": ([P, C.c_int, C.c_int], C.c_int),
+        "tdgl_batch_get_kz": ([P, C.c_int], C.c_int),
+        "tdgl_batch_time_stencil": ([P, C.c_int, C.POINTER(D)], C.c_int),
+        "tdgl_batch_trap_step": ([P, P, P, P, C.POINTER(SolverOpts), P], C.c_int),
+        "tdgl_batch_profile_enable": ([P, C.c_int], C.c_int),
+        "tdgl_batch_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -164,6 +184,12 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_stencil_kz(self, kz: int) -> None:
+        self._call("tdgl_set_stencil_kz", C.c_int(int(kz)))
+
+    def get_stencil_kz(self) -> int:
+        return int(self.lib.tdgl_get_stencil_kz(self.h))
 
     # -- buffers ---------------------------------------------------------
     def empty_state(self):
@@ -265,6 +291,114 @@ class Context:
         self._call("tdgl_observables", C.c_void_p(x.data_ptr()), C.c_void_p(_ptr(b)),
                    C.c_void_p(_ptr(j)))
         return b, j
+
+
+class Batch:
+    """A tdgl_batch over several Contexts (mixed-precision lockstep step).
+
+    The contexts must stay alive as long as the batch; their fields and
+    materials are read at every step.  A batched state is one float64 device
+    tensor of ``total_len`` elements; film f is ``X[offset(f) : offset(f) +
+    ctx.len]`` in the internal layout.
+    """
+
+    def __init__(self, contexts):
+        import torch
+        self.lib = load()
+        self.torch = torch
+        self.contexts = list(contexts)
+        if not self.contexts:
+            raise ValueError("a batch needs at least one film")
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = self.lib.tdgl_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc != TDGL_OK:
+            raise ValueError(f"tdgl_batch_create failed (code {rc}): films must all be 3-D or all 2-D")
+        self.h = h
+        self.device = self.contexts[0].device
+        self.stream = self.contexts[0].stream
+        self._call("tdgl_batch_set_stream", C.c_void_p(self.stream.cuda_stream))
+
+    def _call(self, name: str, *args) -> None:
+        rc = getattr(self.lib, name)(self.h, *args)
+        if rc != TDGL_OK:
+            msg = self.lib.tdgl_batch_last_error(self.h).decode() if self.h else ""
+            text = f"{name} failed (code {rc}){': ' + msg if msg else ''}"
+            if rc == TDGL_E_INVALID:
+                raise ValueError(text)
+            raise TDGLError(text)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.tdgl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nfilm(self) -> int:
+        return len(self.contexts)
+
+    @property
+    def total_len(self) -> int:
+        return int(self.lib.tdgl_batch_total_len(self.h))
+
+    def offset(self, f: int) -> int:
+        return int(self.lib.tdgl_batch_offset(self.h, int(f)))
+
+    def set_kz(self, f: int, kz: int) -> None:
+        self._call("tdgl_batch_set_kz", C.c_int(int(f)), C.c_int(int(kz)))
+
+    def get_kz(self, f: int) -> int:
+        return int(self.lib.tdgl_batch_get_kz(self.h, int(f)))
+
+    def time_stencil(self, reps: int = 5) -> float:
+        ms = C.c_double()
+        self._call("tdgl_batch_time_stencil", C.c_int(int(reps)), C.byref(ms))
+        return ms.value
+
+    def empty_state(self):
+        return self.torch.zeros(self.total_len, dtype=self.torch.float64, device=self.device)
+
+    def pack(self, states):
+        """Per-film internal-layout FP64 tensors -> one batched state."""
+        X = self.empty_state()
+        for f, (c, s) in enumerate(zip(self.contexts, states)):
+            o = self.offset(f)
+            X[o:o + c.len].copy_(s)
+        return X
+
+    def view(self, X, f: int):
+        o = self.offset(f)
+        return X[o:o + self.contexts[f].len]
+
+    def trap_step(self, X, dt, opts: SolverOpts, active=None) -> list:
+        import numpy as np
+        F = self.nfilm
+        if X.dtype != self.torch.float64 or X.numel() != self.total_len:
+            raise ValueError("X must be a float64 tensor of total_len elements")
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (F,)))
+        a = None if active is None else np.ascontiguousarray(np.asarray(active, dtype=np.int32))
+        stats = (StepStats * F)()
+        self._call("tdgl_batch_trap_step", C.c_void_p(X.data_ptr()), C.c_void_p(d.ctypes.data),
+                   C.c_void_p(a.ctypes.data) if a is not None else None, C.byref(opts),
+                   C.cast(stats, C.c_void_p))
+        return list(stats)
+
+    def profile_enable(self, on: bool = True) -> None:
+        self._call("tdgl_batch_profile_enable", C.c_int(1 if on else 0))
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+            self._call("tdgl_batch_profile_read", C.c_int(k), C.byref(ms), C.byref(n), C.byref(by))
+            out[name] = {"ms": ms.value, "launches": n.value, "bytes": by.value}
+        return out
 
 
 def solver_opts(newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,

@@ -66,6 +66,19 @@ class Device:
                                          interior_sc_mask=np.ones(p.n_interior, dtype=np.float64))
         self._material.carve_hole_polygon(vertices, z_range, self.params, self.idx)
 
+    def add_spheres(self, centers, radii) -> None:
+        """Insulating spherical inclusions (centres and radii in cells) in the
+        material's sc_mask; creates a uniform material map when none exists."""
+        self._ensure_material()
+        self._material.carve_spheres(centers, radii, self.params, self.idx)
+
+    def _ensure_material(self) -> None:
+        if self._material is None:
+            p = self.params
+            self._material = MaterialMap(kappa=np.full(p.dim_x, p.kappa, dtype=np.float64),
+                                         sc_mask=np.ones(p.dim_x, dtype=np.float64),
+                                         interior_sc_mask=np.ones(p.n_interior, dtype=np.float64))
+
     def __repr__(self) -> str:
         p = self.params
         tri = ""

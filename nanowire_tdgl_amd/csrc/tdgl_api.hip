@@ -14,64 +14,10 @@
 #include <vector>
 
 #include "../../include/tdgl_hip.h"
+#include "tdgl_ctx.hpp"
 #include "tdgl_launch.hpp"
 
 using namespace tdgl;
-
-struct tdgl_ctx {
-  tdgl_grid grid{};
-  Geo g{};
-  Bnd b{};
-  Mat mt{};
-  StencilCfg cfg{1};
-  hipStream_t st = nullptr;
-  std::string err;
-  int device = 0;
-
-  uint8_t* d_flags = nullptr;
-  double* d_k2 = nullptr;
-  double* d_sc = nullptr;
-  double* d_bfull = nullptr;  // 3 * dim_x
-  double* d_pb[2] = {nullptr, nullptr};  // double-buffered reduction partials
-  int pb_cur = 0;                         // buffer holding the last published partials
-  int pb_count = 0;                       // how many blocks published them
-  double* d_res = nullptr;                // scalar slots (kResLen)
-  int* d_bad = nullptr;
-  double* h_res = nullptr;  // pinned mirror of the scalar slots
-
-  long long len = 0;  // doubles per state buffer
-  double* w_x = nullptr;
-  double* w_gam = nullptr;
-  double* w_f = nullptr;
-  double* w_r = nullptr;
-  std::vector<double*> P, AP;
-  double** d_ptab = nullptr;
-  int ptab_cap = 0;
-  int ptab_len = 0;  // entries of d_ptab already copied
-
-  // mixed precision: FP32 Newton iterate copy, TGCR residual and Krylov basis
-  float* w_x32 = nullptr;
-  float* w_r32 = nullptr;
-  std::vector<float*> P32, AP32;
-  float** d_ptab32 = nullptr;   // p_j table
-  float** d_aptab32 = nullptr;  // Ap_j table
-  int ptab32_cap = 0, ptab32_len = 0;
-  double* d_mpart = nullptr;    // multi-dot partials: blocks x (kMaxKrylov + 1)
-
-  // per-kernel-class event timing (tdgl_profile_enable)
-  struct PendingEv {
-    hipEvent_t a, b;
-    int cls;
-    double bytes;
-    int launches;  // consecutive launches of the class inside the bracket
-  };
-  bool prof = false;
-  std::vector<PendingEv> pending;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
-  double prof_ms[TDGL_K_COUNT] = {0};
-  long long prof_n[TDGL_K_COUNT] = {0};
-  double prof_bytes[TDGL_K_COUNT] = {0};
-};
 
 namespace {
 
@@ -95,36 +41,11 @@ int alloc(tdgl_ctx* ctx, void** p, size_t bytes) {
   return TDGL_OK;
 }
 
-// Event bracket around one launch of kernel class `cls` (no-op unless profiling).
 int prof_begin(tdgl_ctx* ctx, int cls, double bytes, int launches = 1) {
-  if (!ctx->prof) return -1;
-  std::pair<hipEvent_t, hipEvent_t> ev;
-  if (!ctx->ev_pool.empty()) {
-    ev = ctx->ev_pool.back();
-    ctx->ev_pool.pop_back();
-  } else if (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess) {
-    return -1;
-  }
-  if (hipEventRecord(ev.first, ctx->st) != hipSuccess) return -1;
-  ctx->pending.push_back({ev.first, ev.second, cls, bytes, launches});
-  return (int)ctx->pending.size() - 1;
+  return ctx->prof.begin(ctx->st, cls, bytes, launches);
 }
-void prof_end(tdgl_ctx* ctx, int tok) {
-  if (tok >= 0) hipEventRecord(ctx->pending[tok].b, ctx->st);
-}
-// Fold finished brackets into the per-class totals (stream already synchronised).
-void prof_harvest(tdgl_ctx* ctx) {
-  for (auto& p : ctx->pending) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-      ctx->prof_ms[p.cls] += ms;
-      ctx->prof_n[p.cls] += p.launches;
-      ctx->prof_bytes[p.cls] += p.bytes;
-    }
-    ctx->ev_pool.push_back({p.a, p.b});
-  }
-  ctx->pending.clear();
-}
+void prof_end(tdgl_ctx* ctx, int tok) { ctx->prof.end(ctx->st, tok); }
+void prof_harvest(tdgl_ctx* ctx) { ctx->prof.harvest(); }
 
 // Reduction plumbing for the next launch: consume the last published partials,
 // publish into the other buffer.
@@ -150,7 +71,7 @@ int readback(tdgl_ctx* ctx, int slot, int count) {
   HIPCHK(hipMemcpyAsync(ctx->h_res + slot, ctx->d_res + slot, sizeof(double) * count,
                         hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
-  if (!ctx->pending.empty()) prof_harvest(ctx);
+  if (!ctx->prof.pending.empty()) prof_harvest(ctx);
   return TDGL_OK;
 }
 
@@ -203,58 +124,6 @@ int ensure_vec(tdgl_ctx* ctx, int i) {
   return TDGL_OK;
 }
 
-int ensure_work32(tdgl_ctx* ctx) {
-  if (ctx->w_x32) return TDGL_OK;
-  const size_t bytes = sizeof(float) * (size_t)ctx->len;
-  int rc;
-  if ((rc = alloc(ctx, (void**)&ctx->w_x32, bytes))) return rc;
-  if ((rc = alloc(ctx, (void**)&ctx->w_r32, bytes))) return rc;
-  const size_t mp = sizeof(double) * (size_t)stream_blocks32(ctx->g) * (kMaxKrylov + 1);
-  if ((rc = alloc(ctx, (void**)&ctx->d_mpart, mp))) return rc;
-  return TDGL_OK;
-}
-
-// FP32 Krylov slot i; both pointer tables grow together.
-int ensure_vec32(tdgl_ctx* ctx, int i) {
-  if (i >= kMaxKrylov)
-    return fail(ctx, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
-  const size_t bytes = sizeof(float) * (size_t)ctx->len;
-  while ((int)ctx->P32.size() <= i) {
-    float *p = nullptr, *ap = nullptr;
-    int rc;
-    if ((rc = alloc(ctx, (void**)&p, bytes))) return rc;
-    if ((rc = alloc(ctx, (void**)&ap, bytes))) {
-      hipFree(p);
-      return rc;
-    }
-    ctx->P32.push_back(p);
-    ctx->AP32.push_back(ap);
-  }
-  if ((int)ctx->P32.size() > ctx->ptab32_cap) {
-    int cap = std::max(64, ctx->ptab32_cap * 2);
-    while (cap < (int)ctx->P32.size()) cap *= 2;
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    if (ctx->d_ptab32) HIPCHK(hipFree(ctx->d_ptab32));
-    if (ctx->d_aptab32) HIPCHK(hipFree(ctx->d_aptab32));
-    ctx->d_ptab32 = ctx->d_aptab32 = nullptr;
-    int rc;
-    if ((rc = alloc(ctx, (void**)&ctx->d_ptab32, sizeof(float*) * cap))) return rc;
-    if ((rc = alloc(ctx, (void**)&ctx->d_aptab32, sizeof(float*) * cap))) return rc;
-    ctx->ptab32_cap = cap;
-    ctx->ptab32_len = 0;
-  }
-  if ((int)ctx->P32.size() > ctx->ptab32_len) {
-    const int from = ctx->ptab32_len, cnt = (int)ctx->P32.size() - from;
-    HIPCHK(hipMemcpyAsync(ctx->d_ptab32 + from, ctx->P32.data() + from, sizeof(float*) * cnt,
-                          hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(hipMemcpyAsync(ctx->d_aptab32 + from, ctx->AP32.data() + from, sizeof(float*) * cnt,
-                          hipMemcpyHostToDevice, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    ctx->ptab32_len = (int)ctx->P32.size();
-  }
-  return TDGL_OK;
-}
-
 void set_uniform_coeffs(tdgl_ctx* ctx, double kappa) {
   Mat& m = ctx->mt;
   const double k2 = kappa * kappa;  // sparse_operators.py:131 (kappa_m**2)
@@ -297,6 +166,10 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
   g.n = (long long)g.nx * g.ny * g.nz;
   g.nphi = g.is3d ? 3 : 2;
   ctx->len = (2 + g.nphi) * g.n;
+  if (g.mk * (long long)(grid->Nz + 1) >= (1LL << 31) || ctx->len >= (1LL << 31)) {
+    delete ctx;
+    return TDGL_E_INVALID;  // node indices are 32-bit on the device
+  }
 
   Bnd& b = ctx->b;
   b.hx = grid->hx;
@@ -322,6 +195,7 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
   }
 
   const int maxb = std::max(stencil_blocks(g, ctx->cfg), stream_blocks(g));
+  ctx->pb_blocks = maxb;
   int rc;
   if ((rc = alloc(ctx, (void**)&ctx->d_pb[0], sizeof(double) * 4 * (size_t)maxb))) goto bad;
   if ((rc = alloc(ctx, (void**)&ctx->d_pb[1], sizeof(double) * 4 * (size_t)maxb))) goto bad;
@@ -352,23 +226,43 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
     if (p) hipFree(p);
   for (double* p : ctx->P) hipFree(p);
   for (double* p : ctx->AP) hipFree(p);
-  for (float* p : ctx->P32) hipFree(p);
-  for (float* p : ctx->AP32) hipFree(p);
-  for (void* p : {(void*)ctx->w_x32, (void*)ctx->w_r32, (void*)ctx->d_ptab32,
-                  (void*)ctx->d_aptab32, (void*)ctx->d_mpart})
-    if (p) hipFree(p);
-  for (auto& e : ctx->ev_pool) {
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
-  }
-  for (auto& e : ctx->pending) {
-    hipEventDestroy(e.a);
-    hipEventDestroy(e.b);
-  }
+  if (ctx->own_batch) tdgl_batch_destroy(ctx->own_batch);
+  ctx->prof.destroy();
   if (ctx->h_res) hipHostFree(ctx->h_res);
   delete ctx;
   return TDGL_OK;
 }
+
+int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz) {
+  if (!ctx || kz < 1) return TDGL_E_INVALID;
+  const Geo& g = ctx->g;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  StencilCfg cfg{g.is3d ? std::min(kz, g.nz) : 1};
+  const int maxb = std::max(stencil_blocks(g, cfg), stream_blocks(g));
+  if (maxb > ctx->pb_blocks) {  // more work items than partial slots: grow both buffers
+    double* nb[2] = {nullptr, nullptr};
+    for (int h = 0; h < 2; ++h) {
+      int rc;
+      if ((rc = alloc(ctx, (void**)&nb[h], sizeof(double) * 4 * (size_t)maxb))) {
+        if (nb[0]) hipFree(nb[0]);
+        return rc;
+      }
+      HIPCHK(hipMemset(nb[h], 0, sizeof(double) * 4 * (size_t)maxb));
+    }
+    HIPCHK(hipFree(ctx->d_pb[0]));
+    HIPCHK(hipFree(ctx->d_pb[1]));
+    ctx->d_pb[0] = nb[0];
+    ctx->d_pb[1] = nb[1];
+    ctx->pb_count = 0;
+    ctx->pb_blocks = maxb;
+  }
+  ctx->cfg = cfg;
+  ctx->kz_user = cfg.kz;
+  if (ctx->own_batch) return tdgl_batch_set_kz(ctx->own_batch, 0, cfg.kz);
+  return TDGL_OK;
+}
+
+int tdgl_get_stencil_kz(const tdgl_ctx* ctx) { return ctx ? ctx->cfg.kz : -1; }
 
 int tdgl_set_stream(tdgl_ctx* ctx, void* s) {
   if (!ctx) return TDGL_E_INVALID;
@@ -542,9 +436,7 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const dou
 
 int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
   if (!ctx || !x32 || !p32 || !ap32) return TDGL_E_INVALID;
-  launch_jvp32(ctx->g, ctx->b, ctx->mt, x32, p32, ap32, dt / 2.0, ctx->cfg, ctx->st);
-  LAUNCHCHK();
-  return TDGL_OK;
+  return jvp32_single(ctx, x32, p32, dt, ap32);
 }
 
 int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {
@@ -702,125 +594,6 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
   return TDGL_OK;
 }
 
-// Mixed precision: the FP64 Newton residual, with x and r = -g also stored in FP32.
-int newton_eval32(tdgl_ctx* ctx, double hdt, tdgl_step_stats* s) {
-  Epi e{};
-  e.gam = ctx->w_gam;
-  e.hdt = hdt;
-  e.o32a = ctx->w_x32;
-  e.o32b = ctx->w_r32;
-  // read x, gamma (FP64); write x32, r32 (FP32)
-  const double bytes = 2.0 * state_bytes(ctx) + 2.0 * 4.0 * (double)ctx->len;
-  const int tok = prof_begin(ctx, TDGL_K_NEWTON, bytes);
-  red_commit(ctx, launch_stencil(M_NEWTON32, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0,
-                                 e, ctx->cfg, red_next(ctx), ctx->st));
-  prof_end(ctx, tok);
-  red_finalize(ctx, 2, 1, RS_XX);
-  LAUNCHCHK();
-  s->rhs_evals++;
-  s->bytes_moved += bytes;
-  return TDGL_OK;
-}
-
-// TGCR with the exact JVP, FP32 vectors and FP64 coefficients; classical
-// Gram-Schmidt with a conditional second pass (the device decides).  Same
-// stopping rules as tgcr.py:125-155.  On success w_x += dx (FP64).
-int tgcr32(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, int* ok,
-           tdgl_step_stats* s) {
-  const Geo& g = ctx->g;
-  const double S = 4.0 * (double)ctx->len;  // bytes of one FP32 state-shaped vector
-  const long long N = (1 + g.nphi) * g.n;
-  long long maxk = o->tgcr_max_iter;
-  if (maxk <= 0) maxk = std::max<long long>(N, llround(0.2 * (double)N));
-  const double floor_ = o->mixed_tol_floor > 0 ? o->mixed_tol_floor : 1e-6;
-  const double tol = std::max(o->tol_gcr, floor_);
-  const double r0 = sqrt(rr);
-  double rk = r0;
-  int nvec = 0, k = 0, rc;
-  *ok = 1;
-  s->tgcr_r0 = r0;
-  s->tgcr_rk = r0;
-  if (r0 == 0.0) {
-    s->tgcr_iters_last = 0;
-    return TDGL_OK;
-  }
-  const int nbm = stream_blocks32(g);
-  while (rk / r0 > tol && k < maxk) {
-    ++k;
-    const int i = nvec;
-    if ((rc = ensure_vec32(ctx, i))) return rc;
-    int tok = prof_begin(ctx, TDGL_K_JVP, 3.0 * S);  // read x32, p; write Ap
-    launch_jvp32(g, ctx->b, ctx->mt, ctx->w_x32, ctx->w_r32, ctx->AP32[i], hdt, ctx->cfg, ctx->st);
-    prof_end(ctx, tok);
-    s->jvp_evals++;
-    s->bytes_moved += 3.0 * S;
-    Red rm{nullptr, 0, ctx->d_mpart, ctx->d_res};
-    if (i == 0) {
-      tok = prof_begin(ctx, TDGL_K_MGS, S);
-      launch_mdot32(g, ctx->AP32[0], ctx->d_aptab32, 0, 1, 0, rm, ctx->st);
-      prof_end(ctx, tok);
-      launch_colsum(ctx->d_mpart, nbm, 1, 0, ctx->d_res, 0, RS_SELF, 0, ctx->st);
-      s->bytes_moved += S;
-    } else {
-      // pass 1: c = V^T Ap, ||Ap||^2; Ap -= V c
-      tok = prof_begin(ctx, TDGL_K_MGS, (double)(2 * i + 3) * S, 2);
-      launch_mdot32(g, ctx->AP32[i], ctx->d_aptab32, i, 1, 0, rm, ctx->st);
-      launch_colsum(ctx->d_mpart, nbm, i + 1, i, ctx->d_res, RS_BETA, RS_SELF, 0, ctx->st);
-      Red ru = red_next(ctx);
-      const int nbu = launch_cgs_upd32(g, ctx->AP32[i], ctx->d_aptab32, i, RS_BETA, 0, ru, ctx->st);
-      red_commit(ctx, nbu);
-      launch_colsum(ctx->d_pb[ctx->pb_cur], nbu, 1, 0, ctx->d_res, 0, RS_NRM1, 0, ctx->st);
-      prof_end(ctx, tok);
-      // pass 2 (runs on the device only when ||Ap'||^2 <= ||Ap||^2 / 2)
-      tok = prof_begin(ctx, TDGL_K_MGS, (double)(2 * i + 3) * S, 2);
-      launch_mdot32(g, ctx->AP32[i], ctx->d_aptab32, i, 0, 1, rm, ctx->st);
-      launch_colsum(ctx->d_mpart, nbm, i, i, ctx->d_res, RS_DBETA, RS_DOT, 1, ctx->st);
-      Red ru2 = red_next(ctx);
-      launch_cgs_upd32(g, ctx->AP32[i], ctx->d_aptab32, i, RS_DBETA, 1, ru2, ctx->st);
-      red_commit(ctx, nbu);
-      launch_colsum(ctx->d_pb[ctx->pb_cur], nbu, 1, 0, ctx->d_res, 0, RS_NRM2, 1, ctx->st);
-      prof_end(ctx, tok);
-      s->bytes_moved += (double)(2 * i + 3) * S;
-    }
-    tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);  // read r, p_j, Ap; write p_i, Ap
-    red_commit(ctx, launch_final_p32(g, ctx->w_r32, ctx->d_ptab32, i, ctx->AP32[i], ctx->P32[i],
-                                     RS_SELF, red_next(ctx), ctx->st));
-    prof_end(ctx, tok);
-    tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
-    red_commit(ctx, launch_update_r32(g, ctx->w_r32, ctx->AP32[i], RS_ALPHA + i, red_next(ctx),
-                                      ctx->st));
-    prof_end(ctx, tok);
-    red_finalize(ctx, 1, 0, RS_RR);
-    LAUNCHCHK();
-    if ((rc = readback(ctx, RS_BRK, 9))) return rc;  // brk .. nrm2
-    if (i > 0 && ctx->h_res[RS_NRM1] <= 0.5 * ctx->h_res[RS_SELF])
-      s->bytes_moved += (double)(2 * i + 3) * S;  // the second pass ran
-    else if (i > 0 && ctx->prof) {
-      ctx->prof_bytes[TDGL_K_MGS] -= (double)(2 * i + 3) * S;  // it did not
-      ctx->prof_n[TDGL_K_MGS] -= 2;
-    }
-    if (ctx->h_res[RS_BRK] != 0.0) break;
-    s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
-    rk = sqrt(ctx->h_res[RS_RR]);
-    nvec = i + 1;
-    if (!std::isfinite(rk)) return fail(ctx, TDGL_E_NONFINITE, "TGCR residual is not finite");
-  }
-  s->tgcr_iters_last = k;
-  s->tgcr_iters_total += k;
-  s->tgcr_rk = rk;
-  if (rk > tol * r0) {
-    *ok = 0;
-    return TDGL_OK;
-  }
-  const int tok = prof_begin(ctx, TDGL_K_ACCUM, (double)nvec * S + 2.0 * state_bytes(ctx));
-  red_commit(ctx, launch_accum32(g, ctx->w_x, ctx->d_ptab32, nvec, red_next(ctx), ctx->st));
-  prof_end(ctx, tok);
-  red_finalize(ctx, 0, 1, RS_DXINF);
-  LAUNCHCHK();
-  s->bytes_moved += (double)nvec * S + 2.0 * state_bytes(ctx);
-  return TDGL_OK;
-}
-
 }  // namespace
 
 extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
@@ -828,15 +601,11 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   if (!ctx || !X || !o || !s) return TDGL_E_INVALID;
   if (o->precision != TDGL_PREC_F64 && o->precision != TDGL_PREC_MIXED)
     return fail(ctx, TDGL_E_INVALID, "precision must be TDGL_PREC_F64 or TDGL_PREC_MIXED");
-  const bool mixed = o->precision == TDGL_PREC_MIXED;
+  if (o->precision == TDGL_PREC_MIXED) return mixed_step_single(ctx, X, dt, o, s);
   memset(s, 0, sizeof(*s));
   int rc;
   if ((rc = ensure_work(ctx))) return rc;
-  if (mixed) {
-    if ((rc = ensure_work32(ctx))) return rc;
-  } else if ((rc = ensure_vec(ctx, 0))) {
-    return rc;
-  }
+  if ((rc = ensure_vec(ctx, 0))) return rc;
   const double hdt = dt / 2.0;
   const double S = state_bytes(ctx);
   // predictor + gamma (integrators.py:156-158)
@@ -852,19 +621,19 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   LAUNCHCHK();
   s->rhs_evals++;
   s->bytes_moved += 3.0 * S;
-  if ((rc = mixed ? newton_eval32(ctx, hdt, s) : newton_eval(ctx, hdt, s))) return rc;
+  if ((rc = newton_eval(ctx, hdt, s))) return rc;
   if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
   for (int it = 1; it <= o->newton_max_iter; ++it) {
     const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
     int ok = 0;
-    if ((rc = mixed ? tgcr32(ctx, hdt, o, rr, &ok, s) : tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
+    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
     if (!ok) {  // newton.py:102-106
       s->tgcr_failed = 1;
       s->newton_iters = it;
       s->converged = 0;
       return TDGL_OK;
     }
-    if ((rc = mixed ? newton_eval32(ctx, hdt, s) : newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
+    if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
     if ((rc = readback(ctx, RS_XX, 4))) return rc;  // xx, rr, ginf, dxinf
     s->g_inf = ctx->h_res[RS_GINF];
     s->dx_inf = ctx->h_res[RS_DXINF];
@@ -887,12 +656,8 @@ extern "C" int tdgl_profile_enable(tdgl_ctx* ctx, int enable) {
   if (!ctx) return TDGL_E_INVALID;
   HIPCHK(hipStreamSynchronize(ctx->st));
   prof_harvest(ctx);
-  ctx->prof = enable != 0;
-  for (int c = 0; c < TDGL_K_COUNT; ++c) {
-    ctx->prof_ms[c] = 0.0;
-    ctx->prof_n[c] = 0;
-    ctx->prof_bytes[c] = 0.0;
-  }
+  ctx->prof.on = enable != 0;
+  ctx->prof.reset();
   return TDGL_OK;
 }
 
@@ -900,8 +665,8 @@ extern "C" int tdgl_profile_read(tdgl_ctx* ctx, int cls, double* ms, int64_t* n,
   if (!ctx || cls < 0 || cls >= TDGL_K_COUNT) return TDGL_E_INVALID;
   HIPCHK(hipStreamSynchronize(ctx->st));
   prof_harvest(ctx);
-  if (ms) *ms = ctx->prof_ms[cls];
-  if (n) *n = ctx->prof_n[cls];
-  if (bytes) *bytes = ctx->prof_bytes[cls];
+  if (ms) *ms = ctx->prof.ms[cls];
+  if (n) *n = ctx->prof.n[cls];
+  if (bytes) *bytes = ctx->prof.bytes[cls];
   return TDGL_OK;
 }

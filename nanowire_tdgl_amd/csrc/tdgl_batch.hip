@@ -1,0 +1,645 @@
+// Host driver of the mixed-precision trapezoidal step over a batch of films.
+//
+// Control flow per film is tdgl3d's (integrators.py:151-172, newton.py:76-120,
+// tgcr.py:96-156): predictor, Newton iterations on the FP64 residual, each
+// solving (I - dt/2 J) dx = -g with TGCR.  The films advance in lockstep: every
+// launch covers all films still taking part, a film that finishes (TGCR
+// converged, broke down or hit its cap; Newton converged or failed) drops out
+// through its device-side `act` flag, and the host reads back one small
+// per-film report per Newton iteration and one flag array per TGCR iteration.
+//
+// Differences from the FP64 path (the "mixed" method): the Jacobian-vector
+// product is the analytic derivative in FP32 instead of the finite difference
+// of tgcr.py:129-133, Krylov vectors are FP32 with FP64 accumulation,
+// orthogonalisation is classical Gram-Schmidt with a conditional second pass,
+// and the TGCR tolerance is max(tol_gcr, mixed_tol_floor).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/tdgl_hip.h"
+#include "tdgl_ctx.hpp"
+#include "tdgl_launch.hpp"
+
+using namespace tdgl;
+
+struct tdgl_batch {
+  std::vector<tdgl_ctx*> films;
+  int nfilm = 0;
+  bool is3d = true;
+  hipStream_t st = nullptr;
+  bool own_stream_from_ctx = false;  // a context's own batch follows the context's stream
+  std::string err;
+
+  std::vector<FilmDesc> hf;  // host copy of the descriptors
+  std::vector<int> kz;       // z planes per stencil block, per film
+  std::vector<int> item_film, sblk_film;
+  int nitems = 0, nsblk = 0;
+  long long total_len = 0;  // elements of a batched state vector
+  long long hlen = 0;       // doubles of the packed TGCR matrices
+  std::vector<int> kcap;    // per film: min(n_state, kMaxKrylov)
+
+  FilmDesc* d_films = nullptr;
+  int* d_item_film = nullptr;
+  int* d_sblk_film = nullptr;
+  int* d_act = nullptr;
+  double* d_res = nullptr;
+  double* d_dtv = nullptr;
+  int* d_fstat = nullptr;
+  int* d_maxk = nullptr;
+  int* d_iter = nullptr;
+  int* d_nact = nullptr;
+  double* d_pitem = nullptr;  // stencil partials, 3 per item
+  double* d_pm = nullptr;     // column partials, (kMaxKrylov + 2) per streaming block
+  double* d_H = nullptr;
+  double* d_rep = nullptr;
+
+  double* w_x = nullptr;
+  double* w_gam = nullptr;
+  float* x32 = nullptr;
+  float* r32 = nullptr;
+  float* r0_32 = nullptr;
+  std::vector<float*> Q;
+  float** d_qtab = nullptr;  // kMaxKrylov entries
+
+  double* h_rep = nullptr;  // pinned: nfilm * kReportLen
+  int* h_act = nullptr;     // pinned: nfilm
+
+  ProfSink own_prof;
+  ProfSink* prof = &own_prof;
+};
+
+namespace {
+
+int bfail(tdgl_batch* B, int code, const std::string& msg) {
+  if (B) B->err = msg;
+  return code;
+}
+
+#define BCHK(call)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return bfail(B, TDGL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+int balloc(tdgl_batch* B, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return bfail(B, TDGL_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) return bfail(B, TDGL_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  return TDGL_OK;
+}
+
+constexpr int kStencilTX = 64, kStencilTY = 4;
+constexpr long long kStreamChunk = 4096;  // elements per streaming block (256 threads x 4 float4)
+constexpr int kStreamMaxPerFilm = 2048;
+
+BatchK kview(const tdgl_batch* B) {
+  BatchK k;
+  k.films = B->d_films;
+  k.item_film = B->d_item_film;
+  k.sblk_film = B->d_sblk_film;
+  k.act = B->d_act;
+  k.res = B->d_res;
+  k.dtv = B->d_dtv;
+  k.nfilm = B->nfilm;
+  k.nitems = B->nitems;
+  k.nsblk = B->nsblk;
+  return k;
+}
+
+int default_kz(const Geo& g) {
+  if (!g.is3d) return 1;
+  const long long tiles = (long long)((g.nx + kStencilTX - 1) / kStencilTX) * ((g.ny + kStencilTY - 1) / kStencilTY);
+  long long kz = tiles * g.nz / 2048;  // several blocks per CU for large films
+  kz = std::max(1LL, std::min<long long>(kz, 8));
+  return (int)std::min<long long>(kz, g.nz);
+}
+
+// Work decomposition and buffer offsets.  Called at creation and whenever kz changes.
+int build_layout(tdgl_batch* B) {
+  const int F = B->nfilm;
+  B->hf.assign(F, FilmDesc{});
+  B->item_film.clear();
+  B->sblk_film.clear();
+  long long off = 0, hoff = 0;
+  for (int f = 0; f < F; ++f) {
+    const tdgl_ctx* c = B->films[f];
+    FilmDesc& d = B->hf[f];
+    d.g = c->g;
+    d.b = c->b;
+    d.mt = c->mt;
+    d.len = c->len;
+    d.off = off;
+    off += (c->len + 63) / 64 * 64;
+    d.kz = B->kz[f];
+    d.tiles_x = (c->g.nx + kStencilTX - 1) / kStencilTX;
+    d.tiles_y = (c->g.ny + kStencilTY - 1) / kStencilTY;
+    d.tz = c->g.is3d ? (c->g.nz + d.kz - 1) / d.kz : 1;
+    d.item0 = (int)B->item_film.size();
+    d.nitems = d.tiles_x * d.tiles_y * d.tz;
+    B->item_film.insert(B->item_film.end(), d.nitems, f);
+    d.sb0 = (int)B->sblk_film.size();
+    d.nsb = (int)std::min<long long>(kStreamMaxPerFilm, std::max<long long>(1, (c->len + kStreamChunk - 1) / kStreamChunk));
+    B->sblk_film.insert(B->sblk_film.end(), d.nsb, f);
+    d.hoff = hoff;
+    const long long K = B->kcap[f];
+    hoff += K * (K + 1) / 2;
+  }
+  B->total_len = off;
+  B->hlen = hoff;
+  B->nitems = (int)B->item_film.size();
+  B->nsblk = (int)B->sblk_film.size();
+  for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
+    if (p) BCHK(hipFree(p));
+  B->d_item_film = B->d_sblk_film = nullptr;
+  B->d_pitem = B->d_pm = nullptr;
+  int rc;
+  if ((rc = balloc(B, (void**)&B->d_item_film, sizeof(int) * B->nitems))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_sblk_film, sizeof(int) * B->nsblk))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_pitem, sizeof(double) * 3 * (size_t)B->nitems))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_pm, sizeof(double) * (kMaxKrylov + 2) * (size_t)B->nsblk))) return rc;
+  BCHK(hipMemcpy(B->d_item_film, B->item_film.data(), sizeof(int) * B->nitems, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(B->d_sblk_film, B->sblk_film.data(), sizeof(int) * B->nsblk, hipMemcpyHostToDevice));
+  return TDGL_OK;
+}
+
+// Current operators of the films (fields and materials may have changed).
+int refresh(tdgl_batch* B) {
+  for (int f = 0; f < B->nfilm; ++f) {
+    B->hf[f].b = B->films[f]->b;
+    B->hf[f].mt = B->films[f]->mt;
+  }
+  BCHK(hipMemcpyAsync(B->d_films, B->hf.data(), sizeof(FilmDesc) * B->nfilm, hipMemcpyHostToDevice, B->st));
+  return TDGL_OK;
+}
+
+int ensure_work(tdgl_batch* B) {
+  if (B->w_x) return TDGL_OK;
+  const size_t n = (size_t)B->total_len;
+  int rc;
+  if ((rc = balloc(B, (void**)&B->w_x, sizeof(double) * n))) return rc;
+  if ((rc = balloc(B, (void**)&B->w_gam, sizeof(double) * n))) return rc;
+  if ((rc = balloc(B, (void**)&B->x32, sizeof(float) * n))) return rc;
+  if ((rc = balloc(B, (void**)&B->r32, sizeof(float) * n))) return rc;
+  if ((rc = balloc(B, (void**)&B->r0_32, sizeof(float) * n))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_qtab, sizeof(float*) * kMaxKrylov))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_H, sizeof(double) * (size_t)std::max(1LL, B->hlen)))) return rc;
+  return TDGL_OK;
+}
+
+// Krylov vector Q_i (zero padding) and its table entry.
+int ensure_q(tdgl_batch* B, int i) {
+  if (i >= kMaxKrylov) return bfail(B, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
+  while ((int)B->Q.size() <= i) {
+    float* q = nullptr;
+    int rc;
+    if ((rc = balloc(B, (void**)&q, sizeof(float) * (size_t)B->total_len))) return rc;
+    B->Q.push_back(q);
+    const int k = (int)B->Q.size() - 1;
+    BCHK(hipMemcpy(B->d_qtab + k, &B->Q[k], sizeof(float*), hipMemcpyHostToDevice));
+  }
+  return TDGL_OK;
+}
+
+int set_act(tdgl_batch* B, const std::vector<int>& act) {
+  memcpy(B->h_act, act.data(), sizeof(int) * B->nfilm);
+  BCHK(hipMemcpyAsync(B->d_act, B->h_act, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->st));
+  return TDGL_OK;
+}
+
+int report(tdgl_batch* B) {
+  launch_breport(kview(B), B->d_fstat, B->d_rep, B->st);
+  BCHK(hipGetLastError());
+  BCHK(hipMemcpyAsync(B->h_rep, B->d_rep, sizeof(double) * kReportLen * B->nfilm, hipMemcpyDeviceToHost, B->st));
+  BCHK(hipStreamSynchronize(B->st));
+  if (!B->prof->pending.empty()) B->prof->harvest();
+  return TDGL_OK;
+}
+const double* rep(const tdgl_batch* B, int f) { return B->h_rep + (size_t)f * kReportLen; }
+
+double s32(const tdgl_batch* B, int f) { return 4.0 * (double)B->hf[f].len; }
+double s64(const tdgl_batch* B, int f) { return 8.0 * (double)B->hf[f].len; }
+double sum_over(const tdgl_batch* B, const std::vector<int>& act, double (*w)(const tdgl_batch*, int)) {
+  double s = 0.0;
+  for (int f = 0; f < B->nfilm; ++f)
+    if (act[f]) s += w(B, f);
+  return s;
+}
+
+// Newton residual in FP64 at w_x for the active films: x32, r32 = -g, and
+// res[RS_XX, RS_RR, RS_GINF].
+int newton_eval(tdgl_batch* B, const std::vector<int>& act, std::vector<tdgl_step_stats>& S) {
+  const double by = 2.0 * sum_over(B, act, s64) + 2.0 * sum_over(B, act, s32);
+  const int tok = B->prof->begin(B->st, TDGL_K_NEWTON, by);
+  launch_bstencil(kview(B), B->is3d, M_NEWTON32, B->w_x, nullptr, nullptr, B->w_gam, B->x32, B->r32,
+                  B->d_pitem, B->st);
+  launch_bfin(kview(B), 2, 1, B->d_pitem, RS_XX, B->st);
+  B->prof->end(B->st, tok);
+  BCHK(hipGetLastError());
+  for (int f = 0; f < B->nfilm; ++f)
+    if (act[f]) {
+      S[f].rhs_evals++;
+      S[f].bytes_moved += 2.0 * s64(B, f) + 2.0 * s32(B, f);
+    }
+  return TDGL_OK;
+}
+
+// TGCR for the active films on (I - dt/2 J) dx = r32 at x32; on return the
+// report holds each film's outcome.  `work[f]` collects per-film bytes.
+int tgcr(tdgl_batch* B, const std::vector<int>& act0, double tol, std::vector<tdgl_step_stats>& S) {
+  const int F = B->nfilm;
+  int rc;
+  if ((rc = set_act(B, act0))) return rc;
+  launch_btinit(kview(B), B->d_fstat, B->d_maxk, B->d_iter, B->st);
+  BCHK(hipMemcpyAsync(B->r0_32, B->r32, sizeof(float) * (size_t)B->total_len, hipMemcpyDeviceToDevice, B->st));
+  BCHK(hipGetLastError());
+  std::vector<int> act = act0;
+  const BatchK bk = kview(B);
+  for (int i = 0;; ++i) {
+    if ((rc = ensure_q(B, i))) return rc;
+    const double S3 = sum_over(B, act, s32);
+    int tok = B->prof->begin(B->st, TDGL_K_JVP, 3.0 * S3);
+    launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->st);
+    B->prof->end(B->st, tok);
+    if (i == 0) {
+      tok = B->prof->begin(B->st, TDGL_K_MGS, 2.0 * S3, 2);
+      launch_bmdot(bk, 0, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
+      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD0, 1, B->st);
+      B->prof->end(B->st, tok);
+    } else {
+      const int gy = std::min(i + 1, 32);
+      tok = B->prof->begin(B->st, TDGL_K_MGS, (2.0 * i + 4.0) * S3, 4);
+      launch_bmdot(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
+      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD1, gy, B->st);
+      launch_bupd(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
+      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP1, 1, B->st);
+      B->prof->end(B->st, tok);
+      // second pass: the kernels return at once for films that do not need it
+      tok = B->prof->begin(B->st, TDGL_K_MGS_LAST, 0.0, 4);
+      launch_bmdot(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
+      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD2, gy, B->st);
+      launch_bupd(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
+      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP2, 1, B->st);
+      B->prof->end(B->st, tok);
+    }
+    tok = B->prof->begin(B->st, TDGL_K_FINAL_P, 4.0 * S3);
+    launch_bfinish(bk, B->d_qtab, B->r32, B->d_iter, B->d_H, B->d_pm, B->st);
+    B->prof->end(B->st, tok);
+    tok = B->prof->begin(B->st, TDGL_K_UPDATE_R, 0.0, 2);
+    launch_bstatus(bk, B->d_fstat, B->d_iter, tol, B->d_pm, B->st);
+    launch_bnext(bk, B->d_iter, B->d_nact, B->st);
+    B->prof->end(B->st, tok);
+    BCHK(hipGetLastError());
+    for (int f = 0; f < F; ++f)
+      if (act[f]) {
+        S[f].jvp_evals++;
+        S[f].bytes_moved += (i == 0 ? 9.0 : 2.0 * i + 11.0) * s32(B, f);
+      }
+    BCHK(hipMemcpyAsync(B->h_act, B->d_act, sizeof(int) * F, hipMemcpyDeviceToHost, B->st));
+    BCHK(hipStreamSynchronize(B->st));
+    if (!B->prof->pending.empty()) B->prof->harvest();
+    int live = 0;
+    for (int f = 0; f < F; ++f) {
+      act[f] = B->h_act[f];
+      live += act[f];
+    }
+    if (!live) break;
+  }
+  return report(B);
+}
+
+int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
+               const tdgl_solver_opts* o, tdgl_step_stats* stats) {
+  const int F = B->nfilm;
+  int rc;
+  if ((rc = ensure_work(B))) return rc;
+  if ((rc = refresh(B))) return rc;
+  std::vector<double> dtv(2 * F);
+  std::vector<int> maxk(F), req(F);
+  for (int f = 0; f < F; ++f) {
+    dtv[2 * f] = dt[f];
+    dtv[2 * f + 1] = dt[f] / 2.0;
+    const long long N = (long long)(1 + B->hf[f].g.nphi) * B->hf[f].g.n;
+    // tgcr.py:110-112 default cap: the state length
+    long long r = o->tgcr_max_iter > 0 ? o->tgcr_max_iter : std::max<long long>(N, llround(0.2 * (double)N));
+    req[f] = (int)std::min<long long>(r, 1LL << 30);
+    maxk[f] = std::min(req[f], B->kcap[f]);
+  }
+  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * 2 * F, hipMemcpyHostToDevice, B->st));
+  BCHK(hipMemcpyAsync(B->d_maxk, maxk.data(), sizeof(int) * F, hipMemcpyHostToDevice, B->st));
+  const double floor_ = o->mixed_tol_floor > 0 ? o->mixed_tol_floor : 1e-6;
+  const double tol = std::max(o->tol_gcr, floor_);
+
+  std::vector<tdgl_step_stats> S(F);
+  memset(S.data(), 0, sizeof(tdgl_step_stats) * F);
+  std::vector<int> act(F);
+  for (int f = 0; f < F; ++f) act[f] = active ? (active[f] != 0) : 1;
+
+  // predictor + gamma (integrators.py:156-158)
+  if ((rc = set_act(B, act))) return rc;
+  {
+    const int tok = B->prof->begin(B->st, TDGL_K_PRED, 3.0 * sum_over(B, act, s64));
+    launch_bstencil(kview(B), B->is3d, M_PRED, X, B->w_x, B->w_gam, nullptr, nullptr, nullptr,
+                    B->d_pitem, B->st);
+    B->prof->end(B->st, tok);
+    BCHK(hipGetLastError());
+    for (int f = 0; f < F; ++f)
+      if (act[f]) {
+        S[f].rhs_evals++;
+        S[f].bytes_moved += 3.0 * s64(B, f);
+      }
+  }
+  if ((rc = newton_eval(B, act, S))) return rc;
+  if ((rc = report(B))) return rc;
+  for (int f = 0; f < F; ++f)
+    if (act[f]) S[f].g_inf = rep(B, f)[2];
+
+  std::vector<int> newton = act;
+  for (int it = 1; it <= o->newton_max_iter; ++it) {
+    if ((rc = tgcr(B, newton, tol, S))) return rc;
+    std::vector<int> ok(F, 0);
+    int nok = 0;
+    for (int f = 0; f < F; ++f) {
+      if (!newton[f]) continue;
+      const double* r = rep(B, f);
+      const int state = (int)r[6], kit = (int)r[8];
+      S[f].tgcr_iters_last = kit;
+      S[f].tgcr_iters_total += kit;
+      S[f].tgcr_r0 = r[4];
+      S[f].tgcr_rk = r[5];
+      if (state == TG_NONFINITE) return bfail(B, TDGL_E_NONFINITE, "TGCR residual is not finite");
+      if (state == TG_MAXITER && maxk[f] < req[f])
+        return bfail(B, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
+      if (state == TG_CONVERGED) {
+        ok[f] = 1;
+        ++nok;
+      } else {  // newton.py:102-106: TGCR failed, the step attempt fails
+        S[f].tgcr_failed = 1;
+        S[f].newton_iters = it;
+        S[f].converged = 0;
+        newton[f] = 0;
+      }
+    }
+    if (nok) {
+      if ((rc = set_act(B, ok))) return rc;
+      double by = 2.0 * sum_over(B, ok, s64);
+      for (int f = 0; f < F; ++f)
+        if (ok[f]) by += (rep(B, f)[7] + 1.0) * s32(B, f);
+      const int tok = B->prof->begin(B->st, TDGL_K_ACCUM, by, 3);
+      launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->st);
+      launch_baccum(kview(B), B->w_x, B->r0_32, B->d_qtab, B->d_fstat, B->d_pm, B->st);
+      launch_bcolsum(kview(B), B->d_iter, B->d_pm, CS_DXINF, 1, B->st);
+      B->prof->end(B->st, tok);
+      BCHK(hipGetLastError());
+      for (int f = 0; f < F; ++f)
+        if (ok[f]) S[f].bytes_moved += (rep(B, f)[7] + 1.0) * s32(B, f) + 2.0 * s64(B, f);
+      if ((rc = newton_eval(B, ok, S))) return rc;  // newton.py:109-113
+      if ((rc = report(B))) return rc;
+      for (int f = 0; f < F; ++f) {
+        if (!ok[f]) continue;
+        const double* r = rep(B, f);
+        S[f].g_inf = r[2];
+        S[f].dx_inf = r[3];
+        if (!std::isfinite(S[f].g_inf) || !std::isfinite(S[f].dx_inf))
+          return bfail(B, TDGL_E_NONFINITE, "Newton residual is not finite");
+        if (S[f].g_inf <= o->newton_tol_f && S[f].dx_inf <= o->newton_tol_dx) {  // newton.py:118
+          const FilmDesc& d = B->hf[f];
+          BCHK(hipMemcpyAsync(X + d.off, B->w_x + d.off, sizeof(double) * (size_t)d.len,
+                              hipMemcpyDeviceToDevice, B->st));
+          S[f].bytes_moved += 2.0 * s64(B, f);
+          S[f].converged = 1;
+          S[f].newton_iters = it;
+          newton[f] = 0;
+        }
+      }
+    }
+    bool any = false;
+    for (int f = 0; f < F; ++f) any = any || newton[f];
+    if (!any) break;
+  }
+  for (int f = 0; f < F; ++f)
+    if (newton[f]) {
+      S[f].converged = 0;
+      S[f].newton_iters = o->newton_max_iter;
+    }
+  BCHK(hipStreamSynchronize(B->st));
+  memcpy(stats, S.data(), sizeof(tdgl_step_stats) * F);
+  return TDGL_OK;
+}
+
+// A context's own batch of one follows the context's tuning.
+bool nfilm_is_own(const tdgl_batch* B) { return B->nfilm == 1 && B->own_stream_from_ctx; }
+
+int batch_init(tdgl_batch* B) {
+  const int F = B->nfilm;
+  int rc;
+  if ((rc = balloc(B, (void**)&B->d_films, sizeof(FilmDesc) * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_act, sizeof(int) * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_res, sizeof(double) * kResLen * (size_t)F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_dtv, sizeof(double) * 2 * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_fstat, sizeof(int) * 4 * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_maxk, sizeof(int) * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_iter, sizeof(int)))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_nact, sizeof(int)))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_rep, sizeof(double) * kReportLen * F))) return rc;
+  BCHK(hipHostMalloc((void**)&B->h_rep, sizeof(double) * kReportLen * F, hipHostMallocDefault));
+  BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * F, hipHostMallocDefault));
+  B->kz.resize(F);
+  B->kcap.resize(F);
+  for (int f = 0; f < F; ++f) {
+    const Geo& g = B->films[f]->g;
+    B->kz[f] = B->films[f]->kz_user > 0 && nfilm_is_own(B) ? B->films[f]->kz_user : default_kz(g);
+    B->kcap[f] = (int)std::min<long long>((long long)(1 + g.nphi) * g.n, kMaxKrylov);
+  }
+  return build_layout(B);
+}
+
+}  // namespace
+
+namespace tdgl {
+
+// The context's own batch of one: its profile and tuning are the context's.
+int own_batch(tdgl_ctx* ctx) {
+  if (ctx->own_batch) return TDGL_OK;
+  tdgl_batch* B = new tdgl_batch();
+  B->films.assign(1, ctx);
+  B->nfilm = 1;
+  B->is3d = ctx->g.is3d != 0;
+  B->st = ctx->st;
+  B->own_stream_from_ctx = true;
+  B->prof = &ctx->prof;
+  const int rc = batch_init(B);
+  if (rc) {
+    ctx->err = B->err;
+    tdgl_batch_destroy(B);
+    return rc;
+  }
+  ctx->own_batch = B;
+  return TDGL_OK;
+}
+
+int mixed_step_single(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
+                      tdgl_step_stats* s) {
+  int rc0;
+  if ((rc0 = own_batch(ctx))) return rc0;
+  tdgl_batch* B = ctx->own_batch;
+  B->st = ctx->st;
+  const int rc = batch_step(B, X, &dt, nullptr, o, s);
+  if (rc) ctx->err = B->err;
+  return rc;
+}
+
+int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
+  int rc0;
+  if ((rc0 = own_batch(ctx))) return rc0;
+  tdgl_batch* B = ctx->own_batch;
+  B->st = ctx->st;
+  int rc;
+  if ((rc = refresh(B))) {
+    ctx->err = B->err;
+    return rc;
+  }
+  const double dtv[2] = {dt, dt / 2.0};
+  const int one = 1;
+  if (hipMemcpyAsync(B->d_dtv, dtv, sizeof(dtv), hipMemcpyHostToDevice, B->st) != hipSuccess ||
+      hipMemcpyAsync(B->d_act, &one, sizeof(int), hipMemcpyHostToDevice, B->st) != hipSuccess)
+    return TDGL_E_HIP;
+  launch_bjvp32(kview(B), B->is3d, x32, p32, nullptr, nullptr, ap32, B->st);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
+  return TDGL_OK;
+}
+
+}  // namespace tdgl
+
+extern "C" {
+
+int tdgl_batch_create(tdgl_ctx* const* films, int nfilm, tdgl_batch** out) {
+  if (!films || nfilm <= 0 || !out) return TDGL_E_INVALID;
+  *out = nullptr;
+  const bool is3d = films[0] && films[0]->g.is3d;
+  for (int f = 0; f < nfilm; ++f)
+    if (!films[f] || (films[f]->g.is3d != 0) != is3d) return TDGL_E_INVALID;
+  tdgl_batch* B = new tdgl_batch();
+  B->films.assign(films, films + nfilm);
+  B->nfilm = nfilm;
+  B->is3d = is3d;
+  B->st = films[0]->st;
+  const int rc = batch_init(B);
+  if (rc) {
+    tdgl_batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return TDGL_OK;
+}
+
+int tdgl_batch_destroy(tdgl_batch* B) {
+  if (!B) return TDGL_OK;
+  if (B->st) (void)hipStreamSynchronize(B->st);
+  else (void)hipDeviceSynchronize();
+  for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_act,
+                  (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_fstat, (void*)B->d_maxk,
+                  (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
+                  (void*)B->d_H, (void*)B->d_rep, (void*)B->w_x, (void*)B->w_gam, (void*)B->x32,
+                  (void*)B->r32, (void*)B->r0_32, (void*)B->d_qtab})
+    if (p) (void)hipFree(p);
+  for (float* q : B->Q) (void)hipFree(q);
+  if (B->h_rep) (void)hipHostFree(B->h_rep);
+  if (B->h_act) (void)hipHostFree(B->h_act);
+  B->own_prof.destroy();
+  delete B;
+  return TDGL_OK;
+}
+
+int tdgl_batch_set_stream(tdgl_batch* B, void* s) {
+  if (!B) return TDGL_E_INVALID;
+  B->st = (hipStream_t)s;
+  return TDGL_OK;
+}
+
+const char* tdgl_batch_last_error(const tdgl_batch* B) { return B ? B->err.c_str() : "null batch"; }
+
+int64_t tdgl_batch_total_len(const tdgl_batch* B) { return B ? B->total_len : -1; }
+
+int64_t tdgl_batch_offset(const tdgl_batch* B, int f) {
+  if (!B || f < 0 || f >= B->nfilm) return -1;
+  return B->hf[f].off;
+}
+
+int tdgl_batch_set_kz(tdgl_batch* B, int f, int kz) {
+  if (!B || f < 0 || f >= B->nfilm || kz < 1) return TDGL_E_INVALID;
+  const Geo& g = B->films[f]->g;
+  B->kz[f] = g.is3d ? std::min(kz, g.nz) : 1;
+  (void)hipStreamSynchronize(B->st);
+  return build_layout(B);
+}
+
+int tdgl_batch_get_kz(const tdgl_batch* B, int f) {
+  if (!B || f < 0 || f >= B->nfilm) return -1;
+  return B->kz[f];
+}
+
+int tdgl_batch_time_stencil(tdgl_batch* B, int reps, double* ms) {
+  if (!B || reps < 1 || !ms) return TDGL_E_INVALID;
+  int rc;
+  if ((rc = ensure_work(B))) return rc;
+  if ((rc = ensure_q(B, 0))) return rc;
+  if ((rc = refresh(B))) return rc;
+  std::vector<int> all(B->nfilm, 1);
+  std::vector<double> dtv(2 * B->nfilm, 0.1);
+  if ((rc = set_act(B, all))) return rc;
+  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * dtv.size(), hipMemcpyHostToDevice, B->st));
+  hipEvent_t a, b;
+  BCHK(hipEventCreate(&a));
+  BCHK(hipEventCreate(&b));
+  launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->st);  // warm-up
+  BCHK(hipEventRecord(a, B->st));
+  for (int r = 0; r < reps; ++r)
+    launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->st);
+  BCHK(hipEventRecord(b, B->st));
+  BCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  BCHK(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = (double)t / reps;
+  return TDGL_OK;
+}
+
+int tdgl_batch_trap_step(tdgl_batch* B, double* X, const double* dt, const int* active,
+                         const tdgl_solver_opts* o, tdgl_step_stats* stats) {
+  if (!B || !X || !dt || !o || !stats) return TDGL_E_INVALID;
+  if (o->precision != TDGL_PREC_MIXED)
+    return bfail(B, TDGL_E_INVALID, "batches run the mixed-precision method (TDGL_PREC_MIXED)");
+  for (int f = 0; f < B->nfilm; ++f)
+    if (!(dt[f] > 0.0) && (!active || active[f])) return bfail(B, TDGL_E_INVALID, "dt must be > 0");
+  return batch_step(B, X, dt, active, o, stats);
+}
+
+int tdgl_batch_profile_enable(tdgl_batch* B, int enable) {
+  if (!B) return TDGL_E_INVALID;
+  if (hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
+  B->prof->harvest();
+  B->prof->on = enable != 0;
+  B->prof->reset();
+  return TDGL_OK;
+}
+
+int tdgl_batch_profile_read(tdgl_batch* B, int cls, double* ms, int64_t* n, double* bytes) {
+  if (!B || cls < 0 || cls >= TDGL_K_COUNT) return TDGL_E_INVALID;
+  if (hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
+  B->prof->harvest();
+  if (ms) *ms = B->prof->ms[cls];
+  if (n) *n = B->prof->n[cls];
+  if (bytes) *bytes = B->prof->bytes[cls];
+  return TDGL_OK;
+}
+
+}  // extern "C"

@@ -11,18 +11,22 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "tdgl_types.hpp"
 
 namespace tdgl {
 
 constexpr int kRedThreads = 256;  // block size of every reducing kernel
 
+// Node indices are formed in 32-bit arithmetic: the host rejects grids whose
+// full-grid size or state length reaches 2^31.
 __device__ __forceinline__ long long qix(const Geo& g, int i, int j, int k) {
-  return (long long)(i - 1) + (long long)g.nx * ((long long)(j - 1) + (long long)g.ny * (long long)(g.is3d ? k - 1 : 0));
+  return (long long)((i - 1) + g.nx * ((j - 1) + g.ny * (g.is3d ? k - 1 : 0)));
 }
 
 __device__ __forceinline__ long long mfull(const Geo& g, int i, int j, int k) {
-  return (long long)i + (long long)g.mj * j + (g.is3d ? g.mk * (long long)k : 0LL);
+  return (long long)(i + g.mj * (j + (g.Ny + 1) * (g.is3d ? k : 0)));
 }
 
 // Raw interior values of a state, optionally perturbed: x + eps*p, the
@@ -40,9 +44,17 @@ struct Src {
     if (PERT) v = v + eps * p[off];
     return v;
   }
+  // psi(q) as one vector load (re, im are adjacent; buffers are 8-byte aligned per pair)
   __device__ __forceinline__ void psi(long long q, T& re, T& im) const {
-    re = ld(2 * q);
-    im = ld(2 * q + 1);
+    using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    const V2 a = reinterpret_cast<const V2*>(x)[q];
+    re = a.x;
+    im = a.y;
+    if (PERT) {
+      const V2 d = reinterpret_cast<const V2*>(p)[q];
+      re = re + eps * d.x;
+      im = im + eps * d.y;
+    }
   }
   __device__ __forceinline__ T phx(long long q) const { return ld(2 * n + q); }
   __device__ __forceinline__ T phy(long long q) const { return ld(3 * n + q); }
@@ -162,6 +174,120 @@ __device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, co
   return c;
 }
 
+// eff_cell for a state x (applied field b) and a perturbation p (zero field)
+// at the same node: the boundary rules are affine, so p's ghosts follow the
+// same rules without the field terms.  One pass over the index logic.
+template <typename T, bool IS3D>
+__device__ __forceinline__ void eff_cell_pair(const Geo& g, const Bnd& b, const uint8_t* flags,
+                                              const Src<T, false>& sx, const Src<T, false>& sp,
+                                              int i, int j, int k, Cell<T>& cx, Cell<T>& cp) {
+  cx = Cell<T>{T(0), T(0), T(0), T(0), T(0)};
+  cp = cx;
+  if (i < 0 || i > g.Nx || j < 0 || j > g.Ny) return;
+  if (IS3D && (k < 0 || k > g.Nz)) return;
+  const bool bi = (i == 0) || (i == g.Nx);
+  const bool bj = (j == 0) || (j == g.Ny);
+  const bool bk = IS3D && ((k == 0) || (k == g.Nz));
+  const int nb = (int)bi + (int)bj + (int)bk;
+  if (nb > 1) return;
+  if (nb == 0) {
+    const long long q = qix(g, i, j, k);
+    sx.psi(q, cx.pr, cx.pi);
+    sp.psi(q, cp.pr, cp.pi);
+    if (i != g.Nx - 1) {
+      cx.x = sx.phx(q);
+      cp.x = sp.phx(q);
+    }
+    if (j != g.Ny - 1) {
+      cx.y = sx.phy(q);
+      cp.y = sp.phy(q);
+    }
+    if (IS3D && k != g.Nz - 1) {
+      cx.z = sx.phz(q);
+      cp.z = sp.phz(q);
+    }
+  } else {
+    const long long m = mfull(g, i, j, k);
+    if (bi) {
+      if (g.per_x) {
+        const long long q = qix(g, (i == 0) ? g.Nx - 1 : 1, j, k);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        if (!(i == 0 || g.Nx - 1 == 1)) {
+          cx.x = sx.phx(q);
+          cp.x = sp.phx(q);
+        }
+      } else {
+        const long long q = qix(g, (i == 0) ? 1 : g.Nx - 1, j, k);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        const T cz = fld_cz<T>(g, b, m);
+        const T yx = (j == g.Ny - 1) ? T(0) : sx.phy(q);
+        cp.y = (j == g.Ny - 1) ? T(0) : sp.phy(q);
+        cx.y = (i == 0) ? (-cz + yx) : (cz + yx);
+        if (IS3D) {
+          const T cy = fld_cy<T>(g, b, m);
+          const T zx = (k == g.Nz - 1) ? T(0) : sx.phz(q);
+          cp.z = (k == g.Nz - 1) ? T(0) : sp.phz(q);
+          cx.z = (i == 0) ? (cy + zx) : (-cy + zx);
+        }
+      }
+    } else if (bj) {
+      if (g.per_y) {
+        const long long q = qix(g, i, (j == 0) ? g.Ny - 1 : 1, k);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        if (!(j == 0 || g.Ny - 1 == 1)) {
+          cx.y = sx.phy(q);
+          cp.y = sp.phy(q);
+        }
+      } else {
+        const long long q = qix(g, i, (j == 0) ? 1 : g.Ny - 1, k);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        const T cz = fld_cz<T>(g, b, m);
+        const T xx = (i == g.Nx - 1) ? T(0) : sx.phx(q);
+        cp.x = (i == g.Nx - 1) ? T(0) : sp.phx(q);
+        cx.x = (j == 0) ? (cz + xx) : (-cz + xx);
+        if (IS3D) {
+          const T cxf = fld_cx<T>(g, b, m);
+          const T zx = (k == g.Nz - 1) ? T(0) : sx.phz(q);
+          cp.z = (k == g.Nz - 1) ? T(0) : sp.phz(q);
+          cx.z = (j == 0) ? (-cxf + zx) : (cxf + zx);
+        }
+      }
+    } else {
+      if (g.per_z) {
+        const long long q = qix(g, i, j, (k == 0) ? g.Nz - 1 : 1);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        if (!(k == 0 || g.Nz - 1 == 1)) {
+          cx.z = sx.phz(q);
+          cp.z = sp.phz(q);
+        }
+      } else {
+        const long long q = qix(g, i, j, (k == 0) ? 1 : g.Nz - 1);
+        sx.psi(q, cx.pr, cx.pi);
+        sp.psi(q, cp.pr, cp.pi);
+        const T cy = fld_cy<T>(g, b, m);
+        const T cxf = fld_cx<T>(g, b, m);
+        const T xx = (i == g.Nx - 1) ? T(0) : sx.phx(q);
+        const T yx = (j == g.Ny - 1) ? T(0) : sx.phy(q);
+        cp.x = (i == g.Nx - 1) ? T(0) : sp.phx(q);
+        cp.y = (j == g.Ny - 1) ? T(0) : sp.phy(q);
+        cx.x = (k == 0) ? (-cy + xx) : (cy + xx);
+        cx.y = (k == 0) ? (cxf + yx) : (-cxf + yx);
+      }
+    }
+  }
+  if (flags) {
+    const uint8_t f = flags[mfull(g, i, j, k)];
+    if (f & kHoleX) cx.x = cp.x = T(0);
+    if (f & kHoleY) cx.y = cp.y = T(0);
+    if (IS3D && (f & kHoleZ)) cx.z = cp.z = T(0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic grid reductions without atomics.
 //
@@ -215,6 +341,16 @@ __device__ __forceinline__ void publish(double (&v)[NS + NM], double* pout) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int a = 0; a < NS + NM; ++a) pout[(long long)blockIdx.x * (NS + NM) + a] = v[a];
+  }
+}
+
+// Store this block's partials at pout[idx * NV + a].
+template <int NS, int NM>
+__device__ __forceinline__ void publish_at(double (&v)[NS + NM], double* pout, long long idx) {
+  block_reduce<NS, NM>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < NS + NM; ++a) pout[idx * (NS + NM) + a] = v[a];
   }
 }
 

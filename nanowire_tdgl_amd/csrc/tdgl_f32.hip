@@ -1,14 +1,31 @@
-// Mixed-precision kernels of the trapezoidal step.
+// Mixed-precision kernels of the trapezoidal step, for a batch of films.
 //
-// FP32: the Jacobian-vector product (analytic, exact derivative of eval_f)
-// and the Krylov vectors p, Ap, r.  FP64: every dot product and norm is
-// accumulated in double, the Gram-Schmidt / alpha coefficients are double, and
-// the state update x += sum alpha_i p_i accumulates in double into the FP64
-// Newton iterate.  The Newton residual itself is the FP64 stencil (tdgl_f64.hip).
+// FP32: the Jacobian-vector product (analytic, the exact derivative of eval_f)
+// and the Krylov vectors (residual r, basis Q).  FP64: every dot product and
+// norm accumulates in double, the Gram-Schmidt coefficients and the small
+// triangular system are double, and the Newton update accumulates in double
+// into the FP64 iterate.  The Newton residual itself is the FP64 stencil
+// (tdgl_f64.hip).
 //
-// Orthogonalisation is classical Gram-Schmidt with one conditional
-// re-orthogonalisation (the Daniel-Gragg-Kaufman-Stewart test ||Ap'|| <
-// ||Ap|| / sqrt(2)), two passes over the basis instead of one per vector.
+// TGCR without a search-direction basis.  With p_k = r_k before
+// orthogonalisation (tgcr.py:125-127), A r_k = sum_{j<=k} H_jk Q_j where Q_j
+// are the normalised, orthogonalised A p_j, H_jk (j < k) the Gram-Schmidt
+// coefficients and H_kk = ||A p_k||.  GCR's update x = sum_k alpha_k p_k is
+// then x = R y with H y = alpha, R = [r_0 .. r_{K-1}], and since
+// r_k = r_0 - sum_{m<k} alpha_m Q_m,
+//     x = (sum_k y_k) r_0 - sum_m alpha_m (sum_{k>m} y_k) Q_m,
+// one pass over r_0 and the Q basis at the end instead of rebuilding p_k
+// from the previous p_j in every iteration.
+//
+// Orthogonalisation is classical Gram-Schmidt with a conditional second pass
+// (Daniel-Gragg-Kaufman-Stewart: repeat when ||Ap'||^2 <= ||Ap||^2 / 2).
+//
+// Every vector kernel covers all films of the batch in one launch: a block
+// belongs to one film (the block tables), inactive films' blocks return at
+// once, and per-film partials are summed in block order by one-block-per-film
+// kernels (deterministic, no atomics).  Vectors are processed as flat float4
+// arrays; each film's slice starts at a multiple of 64 elements and its
+// padding is zero.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -25,15 +42,14 @@ constexpr int BS = TX * TY;
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// k_jvp32: Ap = p - dt/2 * J(x) p, all in FP32.
+// k_bjvp32: Ap = p - dt/2 * J(x) p, all in FP32.
 // J p is the exact derivative of eval_f (rhs.py:147-239) along p.  The
 // boundary rules are affine in the state, so p's ghost values use the same
 // rules with a zero applied field; hole-boundary rows are zero.
 // ---------------------------------------------------------------------------
 template <bool IS3D>
-__global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, const float* x32,
-                                              const float* p32, float* ap, float hdt, int tiles_x,
-                                              int tiles_y, int kz) {
+__global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
+                                               float* const* Q, const int* iter, float* out_direct) {
   constexpr int NS = IS3D ? 3 : 1;
   constexpr int NZ = IS3D ? 3 : 1;
   constexpr int CZ = IS3D ? CELLS : 1;
@@ -41,59 +57,98 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
   __shared__ float P_r[NS][CELLS], P_i[NS][CELLS], P_x[NS][CELLS], P_y[NS][CELLS], P_z[NZ][CZ];
 
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int tile_x = (int)(L % (unsigned)tiles_x);
-  const unsigned int rest = L / (unsigned)tiles_x;
-  const int tile_y = (int)(rest % (unsigned)tiles_y);
-  const int zc = (int)(rest / (unsigned)tiles_y);
+  const int f = bk.item_film[L];
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  const Geo g = fd->g;  // by value: held in registers for the whole item
+  const Bnd b = fd->b;
+  const Mat mt = fd->mt;
+  const long long off = fd->off;
+  float* ap = (out_direct ? out_direct : Q[*iter]) + off;
+  const float* xs = x32 + off;
+  const float* ps = p32 + off;
+  const float hdt = (float)bk.dtv[2 * f + 1];
+  const int li = (int)L - fd->item0;
+  const int tile_x = li % fd->tiles_x, rest = li / fd->tiles_x;
+  const int tile_y = rest % fd->tiles_y, zc = rest / fd->tiles_y;
   const int i0 = 1 + tile_x * TX, j0 = 1 + tile_y * TY;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx, j = j0 + ty;
   const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
   int kb = 0, ke = 1;
   if (IS3D) {
-    kb = 1 + zc * kz;
-    ke = min(kb + kz, g.Nz);
+    kb = 1 + zc * fd->kz;
+    ke = min(kb + fd->kz, g.Nz);
   }
-  const Src<float, false> sx{x32, nullptr, 0.f, g.n};
-  const Src<float, false> sp{p32, nullptr, 0.f, g.n};
+  const long long n = g.n;
+  const Src<float, false> sx{xs, nullptr, 0.f, n};
+  const Src<float, false> sp{ps, nullptr, 0.f, n};
 
-  auto load_plane = [&](int kp, int slot) {
-    for (int c = threadIdx.x; c < CELLS; c += BS) {
-      const int ci = i0 - 1 + c % CX, cj = j0 - 1 + c / CX;
-      const Cell<float> v = eff_cell<float, IS3D, false>(g, b, mt.flags, sx, ci, cj, kp);
-      const Cell<float> w = eff_cell<float, IS3D, false>(g, b0, mt.flags, sp, ci, cj, kp);
-      X_r[slot][c] = v.pr;
-      X_i[slot][c] = v.pi;
-      X_x[slot][c] = v.x;
-      X_y[slot][c] = v.y;
-      P_r[slot][c] = w.pr;
-      P_i[slot][c] = w.pi;
-      P_x[slot][c] = w.x;
-      P_y[slot][c] = w.y;
-      if (IS3D) {
-        X_z[slot][c] = v.z;
-        P_z[slot][c] = w.z;
-      }
+  const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
+  const bool has1 = hc1 < CELLS;
+  Cell<float> vx0, vp0, vx1, vp1;
+  auto stage = [&](int kp) {
+    eff_cell_pair<float, IS3D>(g, b, mt.flags, sx, sp, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp, vx0, vp0);
+    if (has1)
+      eff_cell_pair<float, IS3D>(g, b, mt.flags, sx, sp, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp, vx1, vp1);
+  };
+  auto put = [&](int slot, int c, const Cell<float>& v, const Cell<float>& w) {
+    X_r[slot][c] = v.pr;
+    X_i[slot][c] = v.pi;
+    X_x[slot][c] = v.x;
+    X_y[slot][c] = v.y;
+    P_r[slot][c] = w.pr;
+    P_i[slot][c] = w.pi;
+    P_x[slot][c] = w.x;
+    P_y[slot][c] = w.y;
+    if (IS3D) {
+      X_z[slot][c] = v.z;
+      P_z[slot][c] = w.z;
     }
   };
+  auto commit = [&](int slot) {
+    put(slot, hc0, vx0, vp0);
+    if (has1) put(slot, hc1, vx1, vp1);
+  };
+
   if (IS3D) {
-    load_plane(kb - 1, (kb - 1) % 3);
-    load_plane(kb, kb % 3);
+    stage(kb - 1);
+    commit((kb - 1) % 3);
+    stage(kb);
+    commit(kb % 3);
+    stage(kb + 1);
   } else {
-    load_plane(0, 0);
+    stage(0);
   }
 
   const float ihx2 = (float)mt.ihx2, ihy2 = (float)mt.ihy2, ihz2 = (float)mt.ihz2;
   const float dgpsi = (float)mt.dgpsi, inv_tau = (float)mt.inv_tau;
 
   for (int k = kb; k < ke; ++k) {
-    if (IS3D) load_plane(k + 1, (k + 1) % 3);
+    commit(IS3D ? (k + 1) % 3 : 0);
     __syncthreads();
+    // own-node inputs, then the halo two planes ahead
+    long long q = 0;
+    uint8_t fl = 0;
+    float scv = 1.f, k2v = 0.f;
+    float2 p0 = make_float2(0.f, 0.f);
+    float praw_x = 0.f, praw_y = 0.f, praw_z = 0.f;
+    if (act) {
+      q = qix(g, i, j, k);
+      if (mt.flags) fl = mt.flags[mfull(g, i, j, k)];
+      if (mt.has_mat && !mt.sc_bits) scv = (float)mt.sc[q];
+      if (mt.k2) k2v = (float)mt.k2[q];
+      p0 = reinterpret_cast<const float2*>(ps)[q];
+      praw_x = ps[2 * n + q];
+      praw_y = ps[3 * n + q];
+      if (IS3D) praw_z = ps[4 * n + q];
+    }
+    if (IS3D && k + 2 <= ke) stage(k + 2);
+
     if (act) {
       const int s0 = IS3D ? k % 3 : 0, sm = IS3D ? (k + 2) % 3 : 0, sp1 = IS3D ? (k + 1) % 3 : 0;
       const int c0 = (ty + 1) * CX + (tx + 1);
       const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
-      const long long q = qix(g, i, j, k);
       const float pr = X_r[s0][c0], pi = X_i[s0][c0];
       const float dr = P_r[s0][c0], di = P_i[s0][c0];
       const float ex = X_x[s0][c0], ey = X_y[s0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
@@ -109,7 +164,8 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
         sincosf(X_z[sm][c0], &szm, &czm);
       }
 
-      // d[e^{-i t} psi(-a)] = e^{-i t}(d psi(-a) - i dt psi(-a)),  d[e^{i t} psi(+a)] = e^{i t}(d psi(+a) + i dt psi(+a))
+      // d[e^{-i t} psi(-a)] = e^{-i t}(d psi(-a) - i dt psi(-a)),
+      // d[e^{i t} psi(+a)]  = e^{i t}(d psi(+a) + i dt psi(+a))
       float ar = dgpsi * dr, ai = dgpsi * di;
       auto minus = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi, float h) {
         const float wr = dqr + dt_ * qi, wi = dqi - dt_ * qr;
@@ -133,11 +189,8 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
       const float m2 = pr * pr + pi * pi, a2 = pr * pr - pi * pi, b2 = 2.f * pr * pi;
       float fr = (1.f - 2.f * m2) * dr - (a2 * dr + b2 * di);
       float fi = (1.f - 2.f * m2) * di - (b2 * dr - a2 * di);
-      const uint8_t fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
       if (mt.has_mat) {
-        float sc;
-        if (mt.sc_bits) sc = (fl & kInsulator) ? 0.f : 1.f;
-        else sc = (float)mt.sc[q];
+        const float sc = mt.sc_bits ? ((fl & kInsulator) ? 0.f : 1.f) : scv;
         fr = sc * fr - (1.f - sc) * dr * inv_tau;
         fi = sc * fi - (1.f - sc) * di * inv_tau;
       }
@@ -145,10 +198,9 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
 
       float cx, cy, cz;
       if (mt.k2) {
-        const float k2 = (float)mt.k2[q];
-        cx = k2 / (float)mt.hx2;
-        cy = k2 / (float)mt.hy2;
-        cz = IS3D ? k2 / (float)mt.hz2 : 0.f;
+        cx = k2v / (float)mt.hx2;
+        cy = k2v / (float)mt.hy2;
+        cz = IS3D ? k2v / (float)mt.hz2 : 0.f;
       } else {
         cx = (float)mt.cx;
         cy = (float)mt.cy;
@@ -162,7 +214,7 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
         const float Ui = (dr * qi - di * qr) + (pr * dqi - pi * dqr);
         return (c * Ui - s * Ur) - dt_ * (c * Tr + s * Ti);
       };
-      // (LPHI_y + LPHI_z) etc. on the perturbation: 2 D_aa along the link, D_bb across
+      // link operators on the perturbation: 2 D_aa along the link, D_bb across
       float jx = 2.f * cx * (P_x[s0][cxm] - 2.f * dex + P_x[s0][cxp]) +
                  cy * (P_x[s0][cym] - 2.f * dex + P_x[s0][cyp]);
       float jy = 2.f * cy * (P_y[s0][cym] - 2.f * dey + P_y[s0][cyp]) +
@@ -188,40 +240,22 @@ __global__ __launch_bounds__(BS) void k_jvp32(Geo g, Bnd b, Bnd b0, Mat mt, cons
       if (fl & kHoleY) jy = 0.f;
       if (IS3D && (fl & kHoleZ)) jz = 0.f;
 
-      const long long n = g.n;
-      const float2 p0 = reinterpret_cast<const float2*>(p32)[q];
       reinterpret_cast<float2*>(ap)[q] = make_float2(p0.x - hdt * jr, p0.y - hdt * ji);
-      ap[2 * n + q] = p32[2 * n + q] - hdt * jx;
-      ap[3 * n + q] = p32[3 * n + q] - hdt * jy;
-      if (IS3D) ap[4 * n + q] = p32[4 * n + q] - hdt * jz;
+      ap[2 * n + q] = praw_x - hdt * jx;
+      ap[3 * n + q] = praw_y - hdt * jy;
+      if (IS3D) ap[4 * n + q] = praw_z - hdt * jz;
     }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-// FP32 vectors, FP64 accumulation.
+// Flat FP32 vectors with FP64 accumulation.
 // ---------------------------------------------------------------------------
 
-struct D5 {
-  double a, b, x, y, z;
-};
-
-template <bool IS3D>
-__device__ __forceinline__ D5 ldf(const float* v, long long n, long long q) {
-  const float2 p = reinterpret_cast<const float2*>(v)[q];
-  return D5{(double)p.x, (double)p.y, (double)v[2 * n + q], (double)v[3 * n + q],
-            IS3D ? (double)v[4 * n + q] : 0.0};
-}
-template <bool IS3D>
-__device__ __forceinline__ void stf(float* v, long long n, long long q, const D5& r) {
-  reinterpret_cast<float2*>(v)[q] = make_float2((float)r.a, (float)r.b);
-  v[2 * n + q] = (float)r.x;
-  v[3 * n + q] = (float)r.y;
-  if (IS3D) v[4 * n + q] = (float)r.z;
-}
-__device__ __forceinline__ double dotd(const D5& u, const D5& v) {
-  return ((u.a * v.a + u.b * v.b) + u.x * v.x) + (u.y * v.y + u.z * v.z);
+__device__ __forceinline__ double dot4(const float4& a, const float4& b) {
+  return ((double)a.x * (double)b.x + (double)a.y * (double)b.y) +
+         ((double)a.z * (double)b.z + (double)a.w * (double)b.w);
 }
 
 // Re-orthogonalise when the first pass removed more than half of ||Ap||^2.
@@ -229,235 +263,389 @@ __device__ __forceinline__ bool need_reorth(const double* res) {
   return res[RS_NRM1] <= 0.5 * res[RS_SELF];
 }
 
+#define FILM_STREAM(bk)                                                       \
+  const int f = (bk).sblk_film[blockIdx.x];                                   \
+  if (!(bk).act[f]) return;                                                   \
+  const FilmDesc* fd = (bk).films + f;                                        \
+  const long long off = fd->off, n4 = (fd->len + 3) >> 2;                     \
+  const long long e0 = (long long)(blockIdx.x - fd->sb0) * kRedThreads + threadIdx.x; \
+  const long long estride = (long long)fd->nsb * kRedThreads;                 \
+  double* res = (bk).res + (long long)f * kResLen;                            \
+  (void)res;                                                                  \
+  (void)off
+
 constexpr int kMdotChunk = 8;
 
-// partials[block][c] = <Ap, V_c> for c < ncols, and [ncols] = ||Ap||^2.
-// skip_unless_reorth: the second Gram-Schmidt pass, executed only when needed.
-template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_mdot32(long long n, const float* ap,
-                                                        const float* const* V, int ncols,
-                                                        int with_self, int second, Red r) {
-  if (second && !need_reorth(r.res)) return;
-  const int NC = ncols + (with_self ? 1 : 0);
-  for (int c0 = 0; c0 < NC; c0 += kMdotChunk) {
+// Column partials pm[c * nsblk + block] of
+//   PASS 0 (first iteration): ||W||^2, <r, W>
+//   PASS 1: <W, Q_j> for j < i, then ||W||^2
+//   PASS 2 (only when re-orthogonalising): <W, Q_j> for j < i
+// with W = Q_i, the new Jacobian-vector product.
+template <int PASS>
+__global__ __launch_bounds__(kRedThreads) void k_bmdot(BatchK bk, float* const* Q, const float* rv,
+                                                       const int* iter, double* pm) {
+  FILM_STREAM(bk);
+  const int i = *iter;
+  if (PASS == 2 && !need_reorth(res)) return;
+  const float4* W = reinterpret_cast<const float4*>(Q[i] + off);
+  const float4* R = reinterpret_cast<const float4*>(rv + off);
+  const int nq = (PASS == 0) ? 0 : i;
+  const int ncol = (PASS == 0) ? 2 : (PASS == 1 ? i + 1 : i);
+  for (int c0 = 0; c0 < ncol; c0 += kMdotChunk) {
     double acc[kMdotChunk];
+    const float4* V[kMdotChunk];
 #pragma unroll
-    for (int cc = 0; cc < kMdotChunk; ++cc) acc[cc] = 0.0;
-    for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-         q += (long long)gridDim.x * kRedThreads) {
-      const D5 a = ldf<IS3D>(ap, n, q);
+    for (int cc = 0; cc < kMdotChunk; ++cc) {
+      acc[cc] = 0.0;
+      const int c = c0 + cc;
+      V[cc] = (c < nq) ? reinterpret_cast<const float4*>(Q[c] + off) : nullptr;
+    }
+    for (long long e = e0; e < n4; e += estride) {
+      const float4 w = W[e];
 #pragma unroll
       for (int cc = 0; cc < kMdotChunk; ++cc) {
         const int c = c0 + cc;
-        if (c < ncols) acc[cc] += dotd(a, ldf<IS3D>(V[c], n, q));
-        else if (c == ncols && with_self) acc[cc] += dotd(a, a);
+        if (c < nq) acc[cc] += dot4(w, V[cc][e]);
+        else if (PASS != 2 && c == nq) acc[cc] += dot4(w, w);
+        else if (PASS == 0 && c == 1) acc[cc] += dot4(w, R[e]);
       }
     }
     block_reduce<kMdotChunk, 0>(acc);
     if (threadIdx.x == 0)
-      for (int cc = 0; cc < kMdotChunk && c0 + cc < NC; ++cc)
-        r.pout[(long long)blockIdx.x * NC + c0 + cc] = acc[cc];
+      for (int cc = 0; cc < kMdotChunk && c0 + cc < ncol; ++cc)
+        pm[(long long)(c0 + cc) * bk.nsblk + blockIdx.x] = acc[cc];
   }
 }
 
-// Column sums of partials[block][c] in block order, one block per column:
-// columns c < n_coef go to res[base + c], column n_coef (if present) to res[self_slot].
-__global__ __launch_bounds__(kRedThreads) void k_colsum(const double* pin, int count, int ncols,
-                                                        int n_coef, double* res, int base,
-                                                        int self_slot, int second) {
-  const int c = blockIdx.x;
-  const int dst = (c < n_coef) ? base + c : self_slot;
-  if (second && !need_reorth(res)) {
-    if (threadIdx.x == 0 && c < n_coef) res[dst] = 0.0;
+// W -= sum_{j<i} coef_j Q_j (coefficients from the pass's column sums), then
+// partials of ||W||^2 and <r, W> (columns 0 and 1).
+template <int PASS>
+__global__ __launch_bounds__(kRedThreads) void k_bupd(BatchK bk, float* const* Q, const float* rv,
+                                                      const int* iter, double* pm) {
+  FILM_STREAM(bk);
+  const int i = *iter;
+  if (PASS == 2 && !need_reorth(res)) return;
+  const double* coef = res + (PASS == 1 ? RS_BETA : RS_DBETA);
+  float4* W = reinterpret_cast<float4*>(Q[i] + off);
+  const float4* R = reinterpret_cast<const float4*>(rv + off);
+  double v[2] = {0.0, 0.0};
+  for (long long e = e0; e < n4; e += estride) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = 0;
+    for (; c + 4 <= i; c += 4) {
+      const float4 a = reinterpret_cast<const float4*>(Q[c] + off)[e];
+      const float4 bq = reinterpret_cast<const float4*>(Q[c + 1] + off)[e];
+      const float4 cq = reinterpret_cast<const float4*>(Q[c + 2] + off)[e];
+      const float4 dq = reinterpret_cast<const float4*>(Q[c + 3] + off)[e];
+      const double ka = coef[c], kb2 = coef[c + 1], kc = coef[c + 2], kd = coef[c + 3];
+      s0 += ((ka * a.x + kb2 * bq.x) + (kc * cq.x + kd * dq.x));
+      s1 += ((ka * a.y + kb2 * bq.y) + (kc * cq.y + kd * dq.y));
+      s2 += ((ka * a.z + kb2 * bq.z) + (kc * cq.z + kd * dq.z));
+      s3 += ((ka * a.w + kb2 * bq.w) + (kc * cq.w + kd * dq.w));
+    }
+    for (; c < i; ++c) {
+      const float4 a = reinterpret_cast<const float4*>(Q[c] + off)[e];
+      const double ka = coef[c];
+      s0 += ka * a.x;
+      s1 += ka * a.y;
+      s2 += ka * a.z;
+      s3 += ka * a.w;
+    }
+    const float4 w = W[e];
+    const float4 o = make_float4((float)((double)w.x - s0), (float)((double)w.y - s1),
+                                 (float)((double)w.z - s2), (float)((double)w.w - s3));
+    W[e] = o;
+    v[0] += dot4(o, o);
+    v[1] += dot4(R[e], o);
+  }
+  block_reduce<2, 0>(v);
+  if (threadIdx.x == 0) {
+    pm[blockIdx.x] = v[0];
+    pm[(long long)bk.nsblk + blockIdx.x] = v[1];
+  }
+}
+
+// Q_i = W / ||W||, alpha_i = <r, Q_i>, r -= alpha_i Q_i; H column i; partials
+// of ||r||^2.  Stops the film at ||W|| < 1e-14 (tgcr.py:141-142).
+__global__ __launch_bounds__(kRedThreads) void k_bfinish(BatchK bk, float* const* Q, float* rv,
+                                                         const int* iter, double* H, double* pm) {
+  FILM_STREAM(bk);
+  const int i = *iter;
+  const bool two = i > 0 && need_reorth(res);
+  const double nrm2 = two ? res[RS_NRM2] : res[RS_NRM1];
+  const double rw = two ? res[RS_RW2] : res[RS_RW1];
+  const double nrm = sqrt(nrm2);
+  const bool brk = nrm < 1e-14;
+  const double inv = 1.0 / nrm;
+  const double alpha = rw * inv;
+  if (blockIdx.x == fd->sb0) {
+    if (threadIdx.x == 0) {
+      res[RS_NRM] = nrm;
+      res[RS_BRK] = brk ? 1.0 : 0.0;
+      if (!brk) res[RS_ALPHA + i] = alpha;
+    }
+    if (!brk) {
+      double* col = H + fd->hoff + (long long)i * (i + 1) / 2;
+      for (int jj = threadIdx.x; jj < i; jj += kRedThreads)
+        col[jj] = two ? res[RS_BETA + jj] + res[RS_DBETA + jj] : res[RS_BETA + jj];
+      if (threadIdx.x == 0) col[i] = nrm;
+    }
+  }
+  if (brk) return;
+  float4* W = reinterpret_cast<float4*>(Q[i] + off);
+  float4* R = reinterpret_cast<float4*>(rv + off);
+  double v[1] = {0.0};
+  for (long long e = e0; e < n4; e += estride) {
+    const float4 w = W[e];
+    const float4 qv = make_float4((float)(w.x * inv), (float)(w.y * inv), (float)(w.z * inv),
+                                  (float)(w.w * inv));
+    W[e] = qv;
+    const float4 r = R[e];
+    const float4 rn = make_float4((float)((double)r.x - alpha * qv.x), (float)((double)r.y - alpha * qv.y),
+                                  (float)((double)r.z - alpha * qv.z), (float)((double)r.w - alpha * qv.w));
+    R[e] = rn;
+    v[0] += dot4(rn, rn);
+  }
+  block_reduce<1, 0>(v);
+  if (threadIdx.x == 0) pm[blockIdx.x] = v[0];
+}
+
+// Per-film column sums of pm in block order (grid: films x G).
+__global__ __launch_bounds__(kRedThreads) void k_bcolsum(BatchK bk, const int* iter, const double* pm,
+                                                         int kind) {
+  const int f = blockIdx.x;
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  double* res = bk.res + (long long)f * kResLen;
+  const int i = *iter;
+  int ncol = 1;
+  bool is_max = false;
+  switch (kind) {
+    case CS_MD0: ncol = 2; break;
+    case CS_MD1: ncol = i + 1; break;
+    case CS_MD2: if (!need_reorth(res)) return; ncol = i; break;
+    case CS_UP1: ncol = 2; break;
+    case CS_UP2: if (!need_reorth(res)) return; ncol = 2; break;
+    case CS_DXINF: ncol = 1; is_max = true; break;
+  }
+  for (int c = blockIdx.y; c < ncol; c += gridDim.y) {
+    double v[1] = {0.0};
+    const double* col = pm + (long long)c * bk.nsblk + fd->sb0;
+    for (int bb = threadIdx.x; bb < fd->nsb; bb += kRedThreads)
+      v[0] = is_max ? fmax(v[0], col[bb]) : v[0] + col[bb];
+    if (is_max) block_reduce<0, 1>(v);
+    else block_reduce<1, 0>(v);
+    if (threadIdx.x == 0) {
+      int slot = 0;
+      switch (kind) {
+        case CS_MD0: slot = c == 0 ? RS_NRM1 : RS_RW1; break;
+        case CS_MD1: slot = c < i ? RS_BETA + c : RS_SELF; break;
+        case CS_MD2: slot = RS_DBETA + c; break;
+        case CS_UP1: slot = c == 0 ? RS_NRM1 : RS_RW1; break;
+        case CS_UP2: slot = c == 0 ? RS_NRM2 : RS_RW2; break;
+        case CS_DXINF: slot = RS_DXINF; break;
+      }
+      res[slot] = v[0];
+    }
+  }
+}
+
+// End of a TGCR iteration: ||r||, the stop tests of tgcr.py:125 and :141,
+// and the film's status.  One block per film.
+__global__ __launch_bounds__(kRedThreads) void k_bstatus(BatchK bk, int* fstat, const int* iter,
+                                                         double tol, const double* pm) {
+  const int f = blockIdx.x;
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  double* res = bk.res + (long long)f * kResLen;
+  const int i = *iter;
+  int* st = fstat + 4 * f;
+  if (res[RS_BRK] != 0.0) {
+    if (threadIdx.x == 0) {
+      st[0] = TG_BREAK;
+      st[1] = i;
+      st[2] = i + 1;
+      bk.act[f] = 0;
+    }
     return;
   }
   double v[1] = {0.0};
-  for (int b = threadIdx.x; b < count; b += kRedThreads) v[0] += pin[(long long)b * ncols + c];
+  for (int bb = threadIdx.x; bb < fd->nsb; bb += kRedThreads) v[0] += pm[fd->sb0 + bb];
   block_reduce<1, 0>(v);
-  if (threadIdx.x == 0) res[dst] = v[0];
-}
-
-// Ap -= sum_c coef_c V_c  (FP64 accumulation); partials of ||Ap||^2.
-template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_cgs_upd32(long long n, float* ap,
-                                                           const float* const* V, int ncols,
-                                                           int coef_base, int second, Red r) {
-  if (second && !need_reorth(r.res)) return;
-  const double* coef = r.res + coef_base;
-  double v[1] = {0.0};
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    D5 s{0, 0, 0, 0, 0};
-    for (int c = 0; c < ncols; ++c) {
-      const D5 w = ldf<IS3D>(V[c], n, q);
-      const double cc = coef[c];
-      s = D5{s.a + cc * w.a, s.b + cc * w.b, s.x + cc * w.x, s.y + cc * w.y, s.z + cc * w.z};
+  if (threadIdx.x == 0) {
+    const double rk = sqrt(v[0]);
+    res[RS_RR] = v[0];
+    res[RS_RK] = rk;
+    st[1] = i + 1;
+    st[2] = i + 1;
+    if (!isfinite(rk)) {
+      st[0] = TG_NONFINITE;
+      bk.act[f] = 0;
+    } else if (!(rk / res[RS_R0] > tol)) {
+      st[0] = TG_CONVERGED;
+      bk.act[f] = 0;
+    } else if (i + 1 >= st[3]) {
+      st[0] = TG_MAXITER;
+      bk.act[f] = 0;
     }
-    const D5 a = ldf<IS3D>(ap, n, q);
-    const D5 o{a.a - s.a, a.b - s.b, a.x - s.x, a.y - s.y, a.z - s.z};
-    stf<IS3D>(ap, n, q, o);
-    const D5 of = ldf<IS3D>(ap, n, q);  // norm of the stored (rounded) vector
-    v[0] += dotd(of, of);
   }
-  publish<1, 0>(v, r.pout);
 }
 
-// p_k = (r - sum_j (c_j + d_j) p_j) / ||Ap||, Ap_k = Ap / ||Ap||, partials of <r, Ap_k>.
-// ||Ap||^2 is res[nrm_slot] when there is no previous vector, otherwise the
-// norm after the last Gram-Schmidt pass that ran.
-template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_final_p32(long long n, const float* rv,
-                                                           const float* const* P, int nprev,
-                                                           float* ap, float* pout, int nrm_slot,
-                                                           Red r) {
-  double nrm2;
-  if (nprev == 0) nrm2 = r.res[nrm_slot];
-  else nrm2 = need_reorth(r.res) ? r.res[RS_NRM2] : r.res[RS_NRM1];
-  const double nrm = sqrt(nrm2);
-  const double inv = 1.0 / nrm;
-  const bool brk = nrm < 1e-14;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    r.res[RS_NRM] = nrm;
-    r.res[RS_INV] = inv;
-    r.res[RS_BRK] = brk ? 1.0 : 0.0;
+// Advance the device iteration counter and count the films still iterating.
+__global__ void k_bnext(BatchK bk, int* iter, int* nact) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int c = 0;
+  for (int f = threadIdx.x; f < bk.nfilm; f += blockDim.x) c += bk.act[f] ? 1 : 0;
+  atomicAdd(&cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *iter = *iter + 1;
+    *nact = cnt;
   }
-  if (brk) return;
-  const double* c1 = r.res + RS_BETA;
-  const double* c2 = r.res + RS_DBETA;
-  const bool two = nprev > 0 && need_reorth(r.res);
-  double v[1] = {0.0};
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    const D5 rr = ldf<IS3D>(rv, n, q);
-    D5 s{0, 0, 0, 0, 0};
-    for (int jj = 0; jj < nprev; ++jj) {
-      const D5 w = ldf<IS3D>(P[jj], n, q);
-      const double cc = two ? c1[jj] + c2[jj] : c1[jj];
-      s = D5{s.a + cc * w.a, s.b + cc * w.b, s.x + cc * w.x, s.y + cc * w.y, s.z + cc * w.z};
+}
+
+// Start of TGCR for the active films: r0 = ||r||_2 from the Newton residual's
+// reduction (tgcr.py:116-121).  maxk: per-film iteration caps.
+__global__ void k_btinit(BatchK bk, int* fstat, const int* maxk, int* iter) {
+  for (int f = threadIdx.x; f < bk.nfilm; f += blockDim.x) {
+    if (!bk.act[f]) continue;
+    double* res = bk.res + (long long)f * kResLen;
+    const double r0 = sqrt(res[RS_RR]);
+    res[RS_R0] = r0;
+    res[RS_RK] = r0;
+    res[RS_BRK] = 0.0;
+    int* st = fstat + 4 * f;
+    st[0] = TG_RUNNING;
+    st[1] = 0;
+    st[2] = 0;
+    st[3] = maxk[f];
+    if (r0 == 0.0) {
+      st[0] = TG_CONVERGED;
+      bk.act[f] = 0;
+    } else if (!isfinite(r0)) {
+      st[0] = TG_NONFINITE;
+      bk.act[f] = 0;
     }
-    const D5 p{(rr.a - s.a) * inv, (rr.b - s.b) * inv, (rr.x - s.x) * inv, (rr.y - s.y) * inv,
-               (rr.z - s.z) * inv};
-    const D5 a0 = ldf<IS3D>(ap, n, q);
-    const D5 a{a0.a * inv, a0.b * inv, a0.x * inv, a0.y * inv, a0.z * inv};
-    stf<IS3D>(pout, n, q, p);
-    stf<IS3D>(ap, n, q, a);
-    v[0] += dotd(rr, ldf<IS3D>(ap, n, q));
   }
-  publish<1, 0>(v, r.pout);
+  if (threadIdx.x == 0) *iter = 0;
 }
 
-// r -= alpha Ap (alpha from the previous partials); partials of ||r||^2.
-template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_update_r32(long long n, float* rv, const float* ap,
-                                                            int aslot, Red r) {
-  if (r.res[RS_BRK] != 0.0) return;
-  double a1[1];
-  gather<1, 0>(r.pin, r.pcount, a1);
-  const double alpha = a1[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) r.res[aslot] = alpha;
-  double v[1] = {0.0};
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    const D5 x = ldf<IS3D>(rv, n, q), a = ldf<IS3D>(ap, n, q);
-    stf<IS3D>(rv, n, q, D5{x.a - alpha * a.a, x.b - alpha * a.b, x.x - alpha * a.x,
-                           x.y - alpha * a.y, x.z - alpha * a.z});
-    const D5 o = ldf<IS3D>(rv, n, q);
-    v[0] += dotd(o, o);
+// y = H^{-1} alpha (column-oriented back substitution), then the update's
+// coefficients: res[RS_GAMMA] for r_0 and res[RS_BETA + m] for Q_m, m < K-1.
+__global__ __launch_bounds__(1024) void k_btrisolve(BatchK bk, const int* fstat, const double* H) {
+  __shared__ double rhs[kMaxKrylov], ys[kMaxKrylov];
+  const int f = blockIdx.x;
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  double* res = bk.res + (long long)f * kResLen;
+  const int K = fstat[4 * f + 1];
+  const double* Hf = H + fd->hoff;
+  for (int jj = threadIdx.x; jj < K; jj += blockDim.x) rhs[jj] = res[RS_ALPHA + jj];
+  for (int k = K - 1; k >= 0; --k) {
+    __syncthreads();
+    const double* col = Hf + (long long)k * (k + 1) / 2;
+    const double yk = rhs[k] / col[k];
+    if (threadIdx.x == 0) ys[k] = yk;
+    for (int jj = threadIdx.x; jj < k; jj += blockDim.x) rhs[jj] = rhs[jj] - col[jj] * yk;
   }
-  publish<1, 0>(v, r.pout);
-}
-
-// x (FP64) += sum_i alpha_i p_i (FP64 accumulation of FP32 p); partials of ||dx||_inf.
-template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_accum32(long long n, double* x,
-                                                         const float* const* P, int nvec, Red r) {
-  const double* alpha = r.res + RS_ALPHA;
-  double v[1] = {0.0};
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    D5 d{0, 0, 0, 0, 0};
-    for (int ii = 0; ii < nvec; ++ii) {
-      const D5 p = ldf<IS3D>(P[ii], n, q);
-      const double a = alpha[ii];
-      d = D5{d.a + a * p.a, d.b + a * p.b, d.x + a * p.x, d.y + a * p.y, d.z + a * p.z};
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;  // s = sum_{k > m} y_k while m runs down
+    for (int m = K - 1; m >= 0; --m) {
+      if (m < K - 1) res[RS_BETA + m] = -res[RS_ALPHA + m] * s;
+      s += ys[m];
     }
-    double2* x2 = reinterpret_cast<double2*>(x);
-    const double2 xv = x2[q];
-    x2[q] = make_double2(xv.x + d.a, xv.y + d.b);
-    x[2 * n + q] += d.x;
-    x[3 * n + q] += d.y;
-    if (IS3D) x[4 * n + q] += d.z;
-    v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
+    res[RS_GAMMA] = K > 0 ? s : 0.0;
   }
-  publish<0, 1>(v, r.pout);
+}
+
+// x (FP64) += gamma r_0 + sum_{m<K-1} coef_m Q_m; partials of ||dx||_inf
+// (complex modulus on psi, |.| on phi).
+__global__ __launch_bounds__(kRedThreads) void k_baccum(BatchK bk, double* x, const float* r0v,
+                                                        float* const* Q, const int* fstat, double* pm) {
+  FILM_STREAM(bk);
+  const int K = fstat[4 * f + 1];
+  const double gam = res[RS_GAMMA];
+  const double* coef = res + RS_BETA;
+  const float4* R0 = reinterpret_cast<const float4*>(r0v + off);
+  double2* X = reinterpret_cast<double2*>(x + off);
+  const long long npsi = 2 * fd->g.n;  // psi occupies elements [0, 2n)
+  double v[1] = {0.0};
+  for (long long e = e0; e < n4; e += estride) {
+    const float4 r = R0[e];
+    double d0 = gam * r.x, d1 = gam * r.y, d2 = gam * r.z, d3 = gam * r.w;
+    for (int m = 0; m < K - 1; ++m) {
+      const float4 a = reinterpret_cast<const float4*>(Q[m] + off)[e];
+      const double c = coef[m];
+      d0 += c * a.x;
+      d1 += c * a.y;
+      d2 += c * a.z;
+      d3 += c * a.w;
+    }
+    const double2 xa = X[2 * e], xb = X[2 * e + 1];
+    X[2 * e] = make_double2(xa.x + d0, xa.y + d1);
+    X[2 * e + 1] = make_double2(xb.x + d2, xb.y + d3);
+    const long long el = 4 * e;
+    const double ma = (el < npsi) ? hypot(d0, d1) : fmax(fabs(d0), fabs(d1));
+    const double mb = (el + 2 < npsi) ? hypot(d2, d3) : fmax(fabs(d2), fabs(d3));
+    v[0] = fmax(v[0], fmax(ma, mb));
+  }
+  block_reduce<0, 1>(v);
+  if (threadIdx.x == 0) pm[blockIdx.x] = v[0];
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
-namespace {
-constexpr int kStream32Cap = 512;
-int blocks32(const Geo& g) {
-  long long nb = (g.n + kRedThreads - 1) / kRedThreads;
-  return (int)(nb < kStream32Cap ? nb : kStream32Cap);
-}
-}  // namespace
-
-int stream_blocks32(const Geo& g) { return blocks32(g); }
-
-void launch_jvp32(const Geo& g, const Bnd& b, const Mat& mt, const float* x32, const float* p32,
-                  float* ap, double hdt, const StencilCfg& cfg, hipStream_t st) {
-  Bnd b0 = b;
-  b0.cx = b0.cy = b0.cz = 0.0;
-  b0.bx = b0.by = b0.bz = nullptr;
-  const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
-  const int nb = stencil_blocks(g, cfg);
-  if (g.is3d)
-    hipLaunchKernelGGL(k_jvp32<true>, dim3(nb), dim3(BS), 0, st, g, b, b0, mt, x32, p32, ap,
-                       (float)hdt, tx, ty, cfg.kz);
+void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
+                   const int* iter, float* out_direct, hipStream_t st) {
+  if (is3d)
+    hipLaunchKernelGGL(k_bjvp32<true>, dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q, iter,
+                       out_direct);
   else
-    hipLaunchKernelGGL(k_jvp32<false>, dim3(nb), dim3(BS), 0, st, g, b, b0, mt, x32, p32, ap,
-                       (float)hdt, tx, ty, cfg.kz);
+    hipLaunchKernelGGL(k_bjvp32<false>, dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q, iter,
+                       out_direct);
 }
-
-#define LAUNCH_F(kern, ...)                                                                     \
-  do {                                                                                          \
-    if (g.is3d)                                                                                 \
-      hipLaunchKernelGGL(kern<true>, dim3(blocks32(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
-    else                                                                                        \
-      hipLaunchKernelGGL(kern<false>, dim3(blocks32(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
-  } while (0)
-
-int launch_mdot32(const Geo& g, const float* ap, const float* const* V, int ncols, int with_self,
-                  int second, const Red& r, hipStream_t st) {
-  LAUNCH_F(k_mdot32, g.n, ap, V, ncols, with_self, second, r);
-  return blocks32(g);
+void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
+                  double* pm, hipStream_t st) {
+  const dim3 gr(bk.nsblk), bl(kRedThreads);
+  if (pass == 0) hipLaunchKernelGGL(k_bmdot<0>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else if (pass == 1) hipLaunchKernelGGL(k_bmdot<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else hipLaunchKernelGGL(k_bmdot<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
 }
-void launch_colsum(const double* pin, int count, int ncols_total, int n_coef, double* res, int base,
-                   int self_slot, int second, hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum, dim3(ncols_total), dim3(kRedThreads), 0, st, pin, count, ncols_total,
-                     n_coef, res, base, self_slot, second);
+void launch_bupd(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
+                 double* pm, hipStream_t st) {
+  const dim3 gr(bk.nsblk), bl(kRedThreads);
+  if (pass == 1) hipLaunchKernelGGL(k_bupd<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else hipLaunchKernelGGL(k_bupd<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
 }
-int launch_cgs_upd32(const Geo& g, float* ap, const float* const* V, int ncols, int coef_base,
-                     int second, const Red& r, hipStream_t st) {
-  LAUNCH_F(k_cgs_upd32, g.n, ap, V, ncols, coef_base, second, r);
-  return blocks32(g);
+void launch_bfinish(const BatchK& bk, float* const* Q, float* rv, const int* iter, double* H,
+                    double* pm, hipStream_t st) {
+  hipLaunchKernelGGL(k_bfinish, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, Q, rv, iter, H, pm);
 }
-int launch_final_p32(const Geo& g, const float* rv, const float* const* P, int nprev, float* ap,
-                     float* pout, int nrm_slot, const Red& r, hipStream_t st) {
-  LAUNCH_F(k_final_p32, g.n, rv, P, nprev, ap, pout, nrm_slot, r);
-  return blocks32(g);
+void launch_bcolsum(const BatchK& bk, const int* iter, const double* pm, int kind, int gy,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_bcolsum, dim3(bk.nfilm, gy), dim3(kRedThreads), 0, st, bk, iter, pm, kind);
 }
-int launch_update_r32(const Geo& g, float* rv, const float* ap, int aslot, const Red& r,
-                      hipStream_t st) {
-  LAUNCH_F(k_update_r32, g.n, rv, ap, aslot, r);
-  return blocks32(g);
+void launch_bstatus(const BatchK& bk, int* fstat, const int* iter, double tol, const double* pm,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_bstatus, dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, fstat, iter, tol, pm);
 }
-int launch_accum32(const Geo& g, double* x, const float* const* P, int nvec, const Red& r,
-                   hipStream_t st) {
-  LAUNCH_F(k_accum32, g.n, x, P, nvec, r);
-  return blocks32(g);
+void launch_bnext(const BatchK& bk, int* iter, int* nact, hipStream_t st) {
+  hipLaunchKernelGGL(k_bnext, dim3(1), dim3(256), 0, st, bk, iter, nact);
+}
+void launch_btinit(const BatchK& bk, int* fstat, const int* maxk, int* iter, hipStream_t st) {
+  hipLaunchKernelGGL(k_btinit, dim3(1), dim3(256), 0, st, bk, fstat, maxk, iter);
+}
+void launch_btrisolve(const BatchK& bk, const int* fstat, const double* H, hipStream_t st) {
+  hipLaunchKernelGGL(k_btrisolve, dim3(bk.nfilm), dim3(1024), 0, st, bk, fstat, H);
+}
+void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* Q, const int* fstat,
+                   double* pm, hipStream_t st) {
+  hipLaunchKernelGGL(k_baccum, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, r0v, Q, fstat, pm);
 }
 
 }  // namespace tdgl

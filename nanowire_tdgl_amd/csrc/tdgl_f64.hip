@@ -50,18 +50,39 @@ __device__ __forceinline__ double supercurrent(double cs, double sn, double pr, 
   return tr * qi + ti * qr;              // imag(t * psi1)
 }
 
+// Raw interior values a stencil epilogue reads at its own node, loaded before
+// the next plane's halo so that they do not wait behind it.
+struct Ctr {
+  double xr, xi, xx, xy, xz;  // PRED/EULER/NEWTON: x;  JVP: p
+  double gr, gi, gx, gy, gz;  // NEWTON: gamma;  JVP: f0
+  double ar, ai, ax, ay, az;  // JVP with ap0: Ap_0
+  double sc, k2;
+  uint8_t fl;
+};
+
+template <int MODE, bool PERT>
+struct RedN {
+  static constexpr bool NEWTON = (MODE == M_NEWTON) || (MODE == M_NEWTON32);
+  static constexpr int NS = NEWTON ? 2 : (MODE == M_JVP ? 1 : 0);
+  static constexpr int NM = NEWTON ? 1 : 0;
+  static constexpr int NV = NS + NM > 0 ? NS + NM : 1;
+};
+
+// One work item of the fused stencil: an xy tile of TX x TY nodes, planes
+// [kb, ke) of a z chunk.  The tile and its 1-node halo are staged in LDS with
+// a 3-plane ring; the halo of plane k+2 is loaded into registers while plane
+// k is computed (software pipelining), and committed to the ring after.
 template <bool IS3D, bool PERT, int MODE>
-__global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double, PERT> s, Epi e,
-                                                int tiles_x, int tiles_y, int kz, Red r) {
+__device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const Mat& mt,
+                                             const Src<double, PERT>& s, const Epi& e, int tile_x,
+                                             int tile_y, int zc, int kz,
+                                             double (&red)[RedN<MODE, PERT>::NV]) {
   constexpr int NSLOT = IS3D ? 3 : 1;
   __shared__ double l_pr[NSLOT][CELLS], l_pi[NSLOT][CELLS], l_x[NSLOT][CELLS], l_y[NSLOT][CELLS],
       l_z[IS3D ? NSLOT : 1][IS3D ? CELLS : 1];
+  constexpr bool NEWTON = RedN<MODE, PERT>::NEWTON;
+  constexpr int NS = RedN<MODE, PERT>::NS;
 
-  const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int tile_x = (int)(L % (unsigned)tiles_x);
-  const unsigned int rest = L / (unsigned)tiles_x;
-  const int tile_y = (int)(rest % (unsigned)tiles_y);
-  const int zc = (int)(rest / (unsigned)tiles_y);
   const int i0 = 1 + tile_x * TX, j0 = 1 + tile_y * TY;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx, j = j0 + ty;
@@ -71,43 +92,101 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
     kb = 1 + zc * kz;
     ke = min(kb + kz, g.Nz);
   }
+  const long long n = g.n;
 
-  constexpr bool NEWTON = (MODE == M_NEWTON) || (MODE == M_NEWTON32);
-  constexpr int NS = NEWTON ? 2 : (MODE == M_JVP ? 1 : 0);
-  constexpr int NM = NEWTON ? 1 : 0;
-  double red[NS + NM > 0 ? NS + NM : 1];
-#pragma unroll
-  for (int a = 0; a < (NS + NM > 0 ? NS + NM : 1); ++a) red[a] = 0.0;
-
-  auto load_plane = [&](int kp, int slot) {
-    for (int c = threadIdx.x; c < CELLS; c += BS) {
-      const int ci = c % CX, cj = c / CX;
-      const Cell<double> v = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + ci, j0 - 1 + cj, kp);
-      l_pr[slot][c] = v.pr;
-      l_pi[slot][c] = v.pi;
-      l_x[slot][c] = v.x;
-      l_y[slot][c] = v.y;
-      if (IS3D) l_z[slot][c] = v.z;
+  // halo cells handled by this thread: c0 = tid, c1 = tid + BS (if < CELLS)
+  const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
+  const bool has1 = hc1 < CELLS;
+  Cell<double> v0, v1;
+  auto stage = [&](int kp) {
+    v0 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp);
+    if (has1)
+      v1 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp);
+  };
+  auto commit = [&](int slot) {
+    l_pr[slot][hc0] = v0.pr;
+    l_pi[slot][hc0] = v0.pi;
+    l_x[slot][hc0] = v0.x;
+    l_y[slot][hc0] = v0.y;
+    if (IS3D) l_z[slot][hc0] = v0.z;
+    if (has1) {
+      l_pr[slot][hc1] = v1.pr;
+      l_pi[slot][hc1] = v1.pi;
+      l_x[slot][hc1] = v1.x;
+      l_y[slot][hc1] = v1.y;
+      if (IS3D) l_z[slot][hc1] = v1.z;
     }
   };
 
   if (IS3D) {
-    load_plane(kb - 1, (kb - 1) % 3);
-    load_plane(kb, kb % 3);
+    stage(kb - 1);
+    commit((kb - 1) % 3);
+    stage(kb);
+    commit(kb % 3);
+    stage(kb + 1);
   } else {
-    load_plane(0, 0);
+    stage(0);
   }
 
   for (int k = kb; k < ke; ++k) {
-    if (IS3D) load_plane(k + 1, (k + 1) % 3);
+    commit(IS3D ? (k + 1) % 3 : 0);
     __syncthreads();
+
+    // own-node inputs first, then the halo two planes ahead
+    Ctr c{};
+    long long q = 0;
+    if (act) {
+      q = qix(g, i, j, k);
+      c.fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
+      if (mt.has_mat && !mt.sc_bits) c.sc = mt.sc[q];
+      if (mt.k2) c.k2 = mt.k2[q];
+      if (MODE == M_PRED || MODE == M_EULER || NEWTON) {
+        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
+        c.xr = X0.x;
+        c.xi = X0.y;
+        c.xx = s.x[2 * n + q];
+        c.xy = s.x[3 * n + q];
+        c.xz = IS3D ? s.x[4 * n + q] : 0.0;
+      }
+      if (NEWTON) {
+        const double2 G0 = reinterpret_cast<const double2*>(e.gam)[q];
+        c.gr = G0.x;
+        c.gi = G0.y;
+        c.gx = e.gam[2 * n + q];
+        c.gy = e.gam[3 * n + q];
+        c.gz = IS3D ? e.gam[4 * n + q] : 0.0;
+      }
+      if (MODE == M_JVP) {
+        const double2 P0 = reinterpret_cast<const double2*>(s.p)[q];
+        c.xr = P0.x;
+        c.xi = P0.y;
+        c.xx = s.p[2 * n + q];
+        c.xy = s.p[3 * n + q];
+        c.xz = IS3D ? s.p[4 * n + q] : 0.0;
+        const double2 F0 = reinterpret_cast<const double2*>(e.f0)[q];
+        c.gr = F0.x;
+        c.gi = F0.y;
+        c.gx = e.f0[2 * n + q];
+        c.gy = e.f0[3 * n + q];
+        c.gz = IS3D ? e.f0[4 * n + q] : 0.0;
+        if (e.ap0) {
+          const double2 A0 = reinterpret_cast<const double2*>(e.ap0)[q];
+          c.ar = A0.x;
+          c.ai = A0.y;
+          c.ax = e.ap0[2 * n + q];
+          c.ay = e.ap0[3 * n + q];
+          c.az = IS3D ? e.ap0[4 * n + q] : 0.0;
+        }
+      }
+    }
+    if (IS3D && k + 2 <= ke) stage(k + 2);
+
     if (act) {
       const int s0 = IS3D ? k % 3 : 0;
       const int sm = IS3D ? (k + 2) % 3 : 0;  // k - 1
       const int sp = IS3D ? (k + 1) % 3 : 0;  // k + 1
       const int c0 = (ty + 1) * CX + (tx + 1);
       const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
-      const long long q = qix(g, i, j, k);
 
       // psi and links at the node and its neighbours (post-BC values)
       const double pr = l_pr[s0][c0], pi = l_pi[s0][c0];
@@ -140,11 +219,11 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
       const double ss = pr * pr - (-pi) * pi;  // conj(psi)*psi
       const double t1 = 1.0 - ss;
       double fr = t1 * pr - 0.0 * pi, fi = t1 * pi + 0.0 * pr;
-      uint8_t fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
+      const uint8_t fl = c.fl;
       if (mt.has_mat) {
         double sc;
         if (mt.sc_bits) sc = (fl & kInsulator) ? 0.0 : 1.0;
-        else sc = mt.sc[q];
+        else sc = c.sc;
         const double w = 1.0 - sc;
         fr = sc * fr - (w * pr) * mt.inv_tau;
         fi = sc * fi - (w * pi) * mt.inv_tau;
@@ -154,7 +233,7 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
       // ---- link equations (rhs.py:223-225 + sparse_operators.py:229-317)
       double cx, cy, cz, dgx, dgy, dgz;
       if (mt.k2) {
-        const double k2 = mt.k2[q];
+        const double k2 = c.k2;
         cx = k2 / mt.hx2;
         cy = k2 / mt.hy2;
         cz = IS3D ? k2 / mt.hz2 : 0.0;
@@ -213,7 +292,6 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
       if (IS3D && (fl & kHoleZ)) dz_ = 0.0;
 
       // ---- epilogues
-      const long long n = g.n;
       double2* o0 = reinterpret_cast<double2*>(e.out0);
       if (MODE == M_RHS) {
         o0[q] = make_double2(dpr, dpi);
@@ -222,19 +300,16 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
         if (IS3D) e.out0[4 * n + q] = dz_;
       } else if (MODE == M_PRED || MODE == M_EULER) {
         // X_pred = X + dt f ; gamma = X + dt/2 f   (integrators.py:156-158, :82)
-        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
-        const double Xx = s.x[2 * n + q], Xy = s.x[3 * n + q];
-        const double Xz = IS3D ? s.x[4 * n + q] : 0.0;
-        o0[q] = make_double2(X0.x + e.dt * dpr, X0.y + e.dt * dpi);
-        e.out0[2 * n + q] = Xx + e.dt * dx_;
-        e.out0[3 * n + q] = Xy + e.dt * dy_;
-        if (IS3D) e.out0[4 * n + q] = Xz + e.dt * dz_;
+        o0[q] = make_double2(c.xr + e.dt * dpr, c.xi + e.dt * dpi);
+        e.out0[2 * n + q] = c.xx + e.dt * dx_;
+        e.out0[3 * n + q] = c.xy + e.dt * dy_;
+        if (IS3D) e.out0[4 * n + q] = c.xz + e.dt * dz_;
         if (MODE == M_PRED) {
           double2* o1 = reinterpret_cast<double2*>(e.out1);
-          o1[q] = make_double2(X0.x + e.hdt * dpr, X0.y + e.hdt * dpi);
-          e.out1[2 * n + q] = Xx + e.hdt * dx_;
-          e.out1[3 * n + q] = Xy + e.hdt * dy_;
-          if (IS3D) e.out1[4 * n + q] = Xz + e.hdt * dz_;
+          o1[q] = make_double2(c.xr + e.hdt * dpr, c.xi + e.hdt * dpi);
+          e.out1[2 * n + q] = c.xx + e.hdt * dx_;
+          e.out1[3 * n + q] = c.xy + e.hdt * dy_;
+          if (IS3D) e.out1[4 * n + q] = c.xz + e.hdt * dz_;
         }
       } else if (NEWTON) {
         // f, r = -g with g = x - dt/2 f - gamma (newton.py:98-100, :109-111).
@@ -245,18 +320,11 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
           e.out0[3 * n + q] = dy_;
           if (IS3D) e.out0[4 * n + q] = dz_;
         }
-        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
-        const double2 G0 = reinterpret_cast<const double2*>(e.gam)[q];
-        const double gr = (X0.x - e.hdt * dpr) - G0.x;
-        const double gi = (X0.y - e.hdt * dpi) - G0.y;
-        const double Xx = s.x[2 * n + q], Xy = s.x[3 * n + q];
-        const double gx = (Xx - e.hdt * dx_) - e.gam[2 * n + q];
-        const double gy = (Xy - e.hdt * dy_) - e.gam[3 * n + q];
-        double Xz = 0.0, gz = 0.0;
-        if (IS3D) {
-          Xz = s.x[4 * n + q];
-          gz = (Xz - e.hdt * dz_) - e.gam[4 * n + q];
-        }
+        const double gr = (c.xr - e.hdt * dpr) - c.gr;
+        const double gi = (c.xi - e.hdt * dpi) - c.gi;
+        const double gx = (c.xx - e.hdt * dx_) - c.gx;
+        const double gy = (c.xy - e.hdt * dy_) - c.gy;
+        const double gz = IS3D ? (c.xz - e.hdt * dz_) - c.gz : 0.0;
         if (MODE == M_NEWTON) {
           double2* o1 = reinterpret_cast<double2*>(e.out1);
           o1[q] = make_double2(-gr, -gi);
@@ -264,44 +332,39 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
           e.out1[3 * n + q] = -gy;
           if (IS3D) e.out1[4 * n + q] = -gz;
         } else {
-          reinterpret_cast<float2*>(e.o32a)[q] = make_float2((float)X0.x, (float)X0.y);
-          e.o32a[2 * n + q] = (float)Xx;
-          e.o32a[3 * n + q] = (float)Xy;
+          reinterpret_cast<float2*>(e.o32a)[q] = make_float2((float)c.xr, (float)c.xi);
+          e.o32a[2 * n + q] = (float)c.xx;
+          e.o32a[3 * n + q] = (float)c.xy;
           reinterpret_cast<float2*>(e.o32b)[q] = make_float2((float)-gr, (float)-gi);
           e.o32b[2 * n + q] = (float)-gx;
           e.o32b[3 * n + q] = (float)-gy;
           if (IS3D) {
-            e.o32a[4 * n + q] = (float)Xz;
+            e.o32a[4 * n + q] = (float)c.xz;
             e.o32b[4 * n + q] = (float)-gz;
           }
         }
-        red[0] += ((X0.x * X0.x + X0.y * X0.y) + Xx * Xx) + (Xy * Xy + Xz * Xz);
+        red[0] += ((c.xr * c.xr + c.xi * c.xi) + c.xx * c.xx) + (c.xy * c.xy + c.xz * c.xz);
         red[1] += ((gr * gr + gi * gi) + gx * gx) + (gy * gy + gz * gz);
         red[NS] = fmax(red[NS], fmax(hypot(gr, gi), fmax(fabs(gx), fmax(fabs(gy), fabs(gz)))));
       } else if (MODE == M_JVP) {
         // Ap = p - dt/2 * ((f(x+eps p) - f0) * (1/eps))   (tgcr.py:129-133)
-        const double2 F0 = reinterpret_cast<const double2*>(e.f0)[q];
-        const double2 P0 = reinterpret_cast<const double2*>(s.p)[q];
-        const double jr = (dpr - F0.x) * e.inv_eps, ji = (dpi - F0.y) * e.inv_eps;
-        const double jx = (dx_ - e.f0[2 * n + q]) * e.inv_eps;
-        const double jy = (dy_ - e.f0[3 * n + q]) * e.inv_eps;
-        const double apr = P0.x - e.hdt * jr, api = P0.y - e.hdt * ji;
-        const double apx = s.p[2 * n + q] - e.hdt * jx;
-        const double apy = s.p[3 * n + q] - e.hdt * jy;
+        const double jr = (dpr - c.gr) * e.inv_eps, ji = (dpi - c.gi) * e.inv_eps;
+        const double jx = (dx_ - c.gx) * e.inv_eps;
+        const double jy = (dy_ - c.gy) * e.inv_eps;
+        const double apr = c.xr - e.hdt * jr, api = c.xi - e.hdt * ji;
+        const double apx = c.xx - e.hdt * jx;
+        const double apy = c.xy - e.hdt * jy;
         double apz = 0.0;
         if (IS3D) {
-          const double jz = (dz_ - e.f0[4 * n + q]) * e.inv_eps;
-          apz = s.p[4 * n + q] - e.hdt * jz;
+          const double jz = (dz_ - c.gz) * e.inv_eps;
+          apz = c.xz - e.hdt * jz;
         }
         o0[q] = make_double2(apr, api);
         e.out0[2 * n + q] = apx;
         e.out0[3 * n + q] = apy;
         if (IS3D) e.out0[4 * n + q] = apz;
         if (e.ap0) {
-          const double2 A0 = reinterpret_cast<const double2*>(e.ap0)[q];
-          double az0 = IS3D ? e.ap0[4 * n + q] : 0.0;
-          red[0] += ((apr * A0.x + api * A0.y) + apx * e.ap0[2 * n + q]) +
-                    (apy * e.ap0[3 * n + q] + apz * az0);
+          red[0] += ((apr * c.ar + api * c.ai) + apx * c.ax) + (apy * c.ay + apz * c.az);
         } else {
           red[0] += ((apr * apr + api * api) + apx * apx) + (apy * apy + apz * apz);
         }
@@ -309,8 +372,58 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
     }
     __syncthreads();
   }
+}
 
-  if constexpr (NS + NM > 0) publish<NS, NM>(red, r.pout);
+template <bool IS3D, bool PERT, int MODE>
+__global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double, PERT> s, Epi e,
+                                                int tiles_x, int tiles_y, int kz, Red r) {
+  const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile_x = (int)(L % (unsigned)tiles_x);
+  const unsigned int rest = L / (unsigned)tiles_x;
+  const int tile_y = (int)(rest % (unsigned)tiles_y);
+  const int zc = (int)(rest / (unsigned)tiles_y);
+  using RN = RedN<MODE, PERT>;
+  double red[RN::NV];
+#pragma unroll
+  for (int a = 0; a < RN::NV; ++a) red[a] = 0.0;
+  stencil_tile<IS3D, PERT, MODE>(g, b, mt, s, e, tile_x, tile_y, zc, kz, red);
+  if constexpr (RN::NS + RN::NM > 0) publish<RN::NS, RN::NM>(red, r.pout);
+}
+
+// Batched films: block L (after the XCD remap) is work item L of the batch;
+// the item table names its film, whose descriptor gives the tile.  Partials
+// are published at index L, so each film's partials are one contiguous range.
+template <bool IS3D, int MODE>
+__global__ __launch_bounds__(BS) void k_bstencil(BatchK bk, const double* x, double* out0,
+                                                 double* out1, const double* gam, float* o32a,
+                                                 float* o32b, double* pout) {
+  const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int f = bk.item_film[L];
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  const Geo g = fd->g;  // by value: held in registers for the whole item
+  const Bnd b = fd->b;
+  const Mat mt = fd->mt;
+  const long long off = fd->off;
+  const int li = (int)L - fd->item0;
+  const int tiles_x = fd->tiles_x, tiles_y = fd->tiles_y;
+  const int tile_x = li % tiles_x, rest = li / tiles_x;
+  const int tile_y = rest % tiles_y, zc = rest / tiles_y;
+  Src<double, false> s{x + off, nullptr, 0.0, g.n};
+  Epi e{};
+  e.out0 = out0 ? out0 + off : nullptr;
+  e.out1 = out1 ? out1 + off : nullptr;
+  e.gam = gam ? gam + off : nullptr;
+  e.o32a = o32a ? o32a + off : nullptr;
+  e.o32b = o32b ? o32b + off : nullptr;
+  e.dt = bk.dtv[2 * f];
+  e.hdt = bk.dtv[2 * f + 1];
+  using RN = RedN<MODE, false>;
+  double red[RN::NV];
+#pragma unroll
+  for (int a = 0; a < RN::NV; ++a) red[a] = 0.0;
+  stencil_tile<IS3D, false, MODE>(g, b, mt, s, e, tile_x, tile_y, zc, fd->kz, red);
+  if constexpr (RN::NS + RN::NM > 0) publish_at<RN::NS, RN::NM>(red, pout, L);
 }
 
 int stencil_blocks(const Geo& g, const StencilCfg& c) {
@@ -351,6 +464,63 @@ int launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& m
   (void)pert;
   return g.is3d ? go_mode<true>(mode, g, b, mt, x, p, eps, e, cfg, r, st)
                 : go_mode<false>(mode, g, b, mt, x, p, eps, e, cfg, r, st);
+}
+
+void launch_bstencil(const BatchK& bk, bool is3d, int mode, const double* x, double* out0,
+                     double* out1, const double* gam, float* o32a, float* o32b, double* pout,
+                     hipStream_t st) {
+  const dim3 gr(bk.nitems), bl(BS);
+  if (mode == M_PRED) {
+    if (is3d)
+      hipLaunchKernelGGL((k_bstencil<true, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+    else
+      hipLaunchKernelGGL((k_bstencil<false, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+  } else {
+    if (is3d)
+      hipLaunchKernelGGL((k_bstencil<true, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+    else
+      hipLaunchKernelGGL((k_bstencil<false, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+  }
+}
+
+// Per film (one block each): reduce the film's stencil partials, in item order.
+template <int NS, int NM>
+__global__ __launch_bounds__(kRedThreads) void k_bfin(BatchK bk, const double* pin, int slot0) {
+  const int f = blockIdx.x;
+  if (!bk.act[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  double out[NS + NM];
+  gather<NS, NM>(pin + (long long)fd->item0 * (NS + NM), fd->nitems, out);
+  if (threadIdx.x == 0)
+    for (int a = 0; a < NS + NM; ++a) bk.res[(long long)f * kResLen + slot0 + a] = out[a];
+}
+
+void launch_bfin(const BatchK& bk, int nsum, int nmax, const double* pin, int slot0, hipStream_t st) {
+  (void)nsum;
+  (void)nmax;
+  hipLaunchKernelGGL((k_bfin<2, 1>), dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, pin, slot0);
+}
+
+// Host report: per film xx, rr, ginf, dxinf, r0, rk, state, basis size, iterations, breakdown.
+__global__ void k_breport(BatchK bk, const int* fstat, double* out) {
+  for (int f = threadIdx.x; f < bk.nfilm; f += blockDim.x) {
+    const double* res = bk.res + (long long)f * kResLen;
+    double* o = out + (long long)f * kReportLen;
+    o[0] = res[RS_XX];
+    o[1] = res[RS_RR];
+    o[2] = res[RS_GINF];
+    o[3] = res[RS_DXINF];
+    o[4] = res[RS_R0];
+    o[5] = res[RS_RK];
+    o[6] = (double)fstat[4 * f];
+    o[7] = (double)fstat[4 * f + 1];
+    o[8] = (double)fstat[4 * f + 2];
+    o[9] = res[RS_BRK];
+  }
+}
+
+void launch_breport(const BatchK& bk, const int* fstat, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_breport, dim3(1), dim3(256), 0, st, bk, fstat, out);
 }
 
 // ---------------------------------------------------------------------------

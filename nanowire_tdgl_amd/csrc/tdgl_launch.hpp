@@ -21,12 +21,22 @@ enum ResSlot : int {
   RS_SELF = 8,   // mixed: ||Ap||^2 before Gram-Schmidt
   RS_NRM1 = 9,   // mixed: ||Ap||^2 after the first classical Gram-Schmidt pass
   RS_NRM2 = 10,  // mixed: ||Ap||^2 after the re-orthogonalisation pass
-  RS_BETA = 16,  // beta_j (mixed: first-pass coefficients), j < kMaxKrylov
+  RS_RW1 = 11,   // mixed: <r, Ap> after the first pass
+  RS_RW2 = 12,   // mixed: <r, Ap> after the second pass
+  RS_R0 = 13,    // mixed: ||r_0||_2 of the current TGCR solve
+  RS_RK = 14,    // mixed: ||r_k||_2
+  RS_GAMMA = 15, // mixed: coefficient of r_0 in the Newton update
+  RS_BETA = 16,  // beta_j (mixed: first-pass coefficients; after the solve, the update's), j < kMaxKrylov
 };
 constexpr int kMaxKrylov = 2048;
 constexpr int RS_ALPHA = RS_BETA + kMaxKrylov;
 constexpr int RS_DBETA = RS_ALPHA + kMaxKrylov;  // mixed: second-pass coefficients
 constexpr int kResLen = RS_DBETA + kMaxKrylov;
+
+// Column-sum kinds of the mixed TGCR (k_bcolsum).
+enum ColSum : int { CS_MD0 = 0, CS_MD1 = 1, CS_MD2 = 2, CS_UP1 = 3, CS_UP2 = 4, CS_DXINF = 5 };
+// Per-film TGCR outcome (fstat[4 f]); fstat[4 f + 1] = basis size, [+2] = iterations, [+3] = cap.
+enum TgState : int { TG_RUNNING = 0, TG_CONVERGED = 1, TG_BREAK = 2, TG_MAXITER = 3, TG_NONFINITE = 4 };
 
 enum StencilMode : int { M_RHS = 0, M_PRED = 1, M_NEWTON = 2, M_JVP = 3, M_EULER = 4, M_NEWTON32 = 5 };
 
@@ -52,6 +62,19 @@ struct Epi {
   double inv_eps;        // JVP: 1/eps
   float* o32a;           // NEWTON32: x in FP32
   float* o32b;           // NEWTON32: r = -g in FP32
+};
+
+// Kernel-side view of a batch of films (all device pointers).
+struct BatchK {
+  const FilmDesc* films;
+  const int* item_film;  // stencil block -> film
+  const int* sblk_film;  // streaming block -> film
+  int* act;              // per film: takes part in the current stage
+  double* res;           // nfilm * kResLen scalar slots
+  const double* dtv;     // per film: dt, dt/2
+  int nfilm;
+  int nitems;            // stencil blocks of the batch
+  int nsblk;             // streaming blocks of the batch
 };
 
 struct StencilCfg {
@@ -81,22 +104,33 @@ int launch_dot(const Geo& g, const double* a, const double* b, const Red& r, hip
 int launch_norm_inf(const Geo& g, const double* a, const Red& r, hipStream_t st);
 // One block: reduce r.pin into res[slot0 ...] (nsum sums then nmax maxima).
 void launch_finalize(int nsum, int nmax, int slot0, const Red& r, hipStream_t st);
-// Mixed-precision kernels (tdgl_f32.hip).
-int stream_blocks32(const Geo& g);
-void launch_jvp32(const Geo& g, const Bnd& b, const Mat& mt, const float* x32, const float* p32,
-                  float* ap, double hdt, const StencilCfg& cfg, hipStream_t st);
-int launch_mdot32(const Geo& g, const float* ap, const float* const* V, int ncols, int with_self,
-                  int second, const Red& r, hipStream_t st);
-void launch_colsum(const double* pin, int count, int ncols_total, int n_coef, double* res, int base,
-                   int self_slot, int second, hipStream_t st);
-int launch_cgs_upd32(const Geo& g, float* ap, const float* const* V, int ncols, int coef_base,
-                     int second, const Red& r, hipStream_t st);
-int launch_final_p32(const Geo& g, const float* rv, const float* const* P, int nprev, float* ap,
-                     float* pout, int nrm_slot, const Red& r, hipStream_t st);
-int launch_update_r32(const Geo& g, float* rv, const float* ap, int aslot, const Red& r,
-                      hipStream_t st);
-int launch_accum32(const Geo& g, double* x, const float* const* P, int nvec, const Red& r,
-                   hipStream_t st);
+// Batched FP64 stencil (PRED or NEWTON32) over the batch's work items, the
+// per-film reduction of its partials into res[slot0 ..], and the per-film
+// report used by the host (tdgl_f64.hip).
+void launch_bstencil(const BatchK& bk, bool is3d, int mode, const double* x, double* out0,
+                     double* out1, const double* gam, float* o32a, float* o32b, double* pout,
+                     hipStream_t st);
+void launch_bfin(const BatchK& bk, int nsum, int nmax, const double* pin, int slot0, hipStream_t st);
+void launch_breport(const BatchK& bk, const int* fstat, double* out, hipStream_t st);
+constexpr int kReportLen = 10;  // doubles per film in the report
+// Mixed-precision TGCR kernels (tdgl_f32.hip).
+void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
+                   const int* iter, float* out_direct, hipStream_t st);
+void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
+                  double* pm, hipStream_t st);
+void launch_bupd(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
+                 double* pm, hipStream_t st);
+void launch_bfinish(const BatchK& bk, float* const* Q, float* rv, const int* iter, double* H,
+                    double* pm, hipStream_t st);
+void launch_bcolsum(const BatchK& bk, const int* iter, const double* pm, int kind, int gy,
+                    hipStream_t st);
+void launch_bstatus(const BatchK& bk, int* fstat, const int* iter, double tol, const double* pm,
+                    hipStream_t st);
+void launch_bnext(const BatchK& bk, int* iter, int* nact, hipStream_t st);
+void launch_btinit(const BatchK& bk, int* fstat, const int* maxk, int* iter, hipStream_t st);
+void launch_btrisolve(const BatchK& bk, const int* fstat, const double* H, hipStream_t st);
+void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* Q, const int* fstat,
+                   double* pm, hipStream_t st);
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
 void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st);
 void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,

@@ -47,4 +47,18 @@ struct Mat {
   double inv_tau;       // fl(1/0.1): numpy's complex/real division multiplies by the reciprocal
 };
 
+// One film of a batch: its operator, and where its work and data sit in the
+// batch's launches and buffers.
+struct FilmDesc {
+  Geo g;
+  Bnd b;
+  Mat mt;
+  long long off;                 // first element of the film in batched vectors (multiple of 64)
+  long long len;                 // elements of one state vector: (2 + nphi) n
+  int tiles_x, tiles_y, kz, tz;  // stencil decomposition: tiles_x*tiles_y*tz work items
+  int item0, nitems;             // the film's stencil blocks
+  int sb0, nsb;                  // the film's streaming blocks
+  long long hoff;                // the film's packed triangular TGCR matrix in the batch's H buffer
+};
+
 }  // namespace tdgl

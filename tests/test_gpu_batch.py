@@ -1,0 +1,156 @@
+"""GPU: batches of uneven films in lockstep (mixed precision).
+
+* A film's arithmetic does not depend on the other films of its batch: each
+  film owns its work items, streaming blocks and reduction order, so a batch
+  reproduces every film's own batch-of-one step bit for bit.
+* Films switched off for an attempt are left untouched.
+* Against FP64 steps of the same films: |psi|^2, B_z and J_s agree to 1e-6
+  relative (of the largest entry) after several steps.
+* The stencil tiling (z depth per work item) changes no node's arithmetic in
+  the JVP stencil: its output is bitwise identical for every depth.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _films():
+    import nanowire_tdgl_amd as T
+    from nanowire_tdgl_amd.solvers.batch import BatchFilm
+    specs = [
+        dict(p=dict(Nx=20, Ny=14, Nz=6, hx=0.5, hy=0.5, hz=0.5, kappa=2.0), bz=0.3,
+             hole=[(2.0, 1.5), (4.5, 1.5), (4.5, 4.0), (2.0, 4.0)]),
+        dict(p=dict(Nx=70, Ny=9, Nz=4, hx=0.5, hy=0.6, hz=0.7, kappa=3.5), bz=0.6, hole=None),
+        dict(p=dict(Nx=12, Ny=33, Nz=9, hx=0.5, hy=0.5, hz=0.5, kappa=1.5), bz=0.1, hole=None,
+             spheres=([(6.0, 16.0, 4.0)], [2.5])),
+    ]
+    films = []
+    for f, s in enumerate(specs):
+        p = T.SimulationParameters(**s["p"])
+        dev = T.Device(p, applied_field=T.AppliedField(Bz=s["bz"], t_on_fraction=1.0))
+        if s["hole"]:
+            dev.add_hole(s["hole"])
+        if s.get("spheres"):
+            dev.add_spheres(*s["spheres"])
+        n = p.n_interior
+        rng = np.random.default_rng(40 + f)
+        x0 = np.concatenate([1.0 + 0.02 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                             np.zeros(3 * n, complex)])
+        films.append(BatchFilm(dev, x0))
+    return films
+
+
+def _single(film, steps, dt, precision, **kw):
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.engine import engine_for
+    d = film.device
+    eng = engine_for(d.params, d.idx, d.material)
+    X = eng.ctx.to_internal(film.x0)
+    opts = _native.solver_opts(precision=precision, **kw)
+    stats = []
+    for s in range(steps):
+        eng.set_field(*d.applied_field.evaluate(s * dt, 100.0))
+        st = eng.ctx.trap_step(X, dt, opts)
+        assert st.converged == 1
+        stats.append(st.as_dict())
+    return eng, X, stats
+
+
+def _observables(eng, X, n):
+    b, j = eng.ctx.observables(X)
+    psi = eng.ctx.to_reference(X).cpu().numpy()[:n]
+    return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()
+
+
+def test_batch_reproduces_each_film_bitwise():
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    steps, dt = 3, 0.2
+    singles = [_single(f, steps, dt, "mixed") for f in films]
+    got = [s[1].clone() for s in singles]
+    runner = BatchRunner([f.device for f in films])
+    X = runner.pack([f.x0 for f in films])
+    opts = _native.solver_opts(precision="mixed")
+    for s in range(steps):
+        runner.set_fields([s * dt] * len(films), 100.0)
+        st = runner.batch.trap_step(X, dt, opts)
+        for f, sf in enumerate(st):
+            assert sf.converged == 1
+            ref = singles[f][2][s]
+            assert sf.newton_iters == ref["newton_iters"]
+            assert sf.tgcr_iters_total == ref["tgcr_iters_total"]
+    for f in range(len(films)):
+        assert runner.batch.view(X, f).equal(got[f]), f
+
+
+def test_inactive_films_are_untouched():
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    runner = BatchRunner([f.device for f in films])
+    X = runner.pack([f.x0 for f in films])
+    before = runner.batch.view(X, 1).clone()
+    runner.set_fields([0.2] * 3, 100.0)
+    st = runner.batch.trap_step(X, [0.2, 0.2, 0.2], _native.solver_opts(precision="mixed"),
+                                active=[1, 0, 1])
+    assert st[0].converged == 1 and st[2].converged == 1
+    assert st[1].converged == 0 and st[1].newton_iters == 0 and st[1].rhs_evals == 0
+    assert runner.batch.view(X, 1).equal(before)
+
+
+def test_batch_rejects_fp64_and_bad_dt():
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    runner = BatchRunner([f.device for f in films])
+    X = runner.pack([f.x0 for f in films])
+    with pytest.raises(ValueError, match="mixed"):
+        runner.batch.trap_step(X, 0.2, _native.solver_opts())
+    with pytest.raises(ValueError, match="dt"):
+        runner.batch.trap_step(X, [0.2, -1.0, 0.2], _native.solver_opts(precision="mixed"))
+
+
+def test_batch_error_against_fp64():
+    from nanowire_tdgl_amd.solvers.batch import trapezoidal_batch
+    films = _films()
+    steps, dt = 6, 0.2
+    res = trapezoidal_batch(films, 0.0, steps * dt, dt, tol_gcr=1e-8)
+    assert res.steps == [steps] * len(films)
+    for f, film in enumerate(films):
+        n = film.device.params.n_interior
+        eng, X, _ = _single(film, steps, dt, "f64", tol_gcr=1e-8)
+        a = _observables(eng, X, n)
+        Xm = eng.ctx.to_internal(res.states[f])
+        b = _observables(eng, Xm, n)
+        for k, name in enumerate(("|psi|^2", "B_z", "J_s")):
+            err = rel_err(b[k], a[k])
+            print(f"film {f}: {name} relative difference {err:.2e}")
+            assert err < 1e-6, (f, name)
+
+
+def test_stencil_depth_does_not_change_the_jvp():
+    import torch
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    runner = BatchRunner([f.device for f in films])
+    ctx = runner.engines[2].ctx
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = 0.5 + 0.1 * torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32)
+    p = torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32) - 0.5
+    outs = []
+    for kz in (1, 3, 8):
+        out = torch.empty_like(x)
+        ctx.set_stencil_kz(kz)
+        assert ctx.get_stencil_kz() == kz
+        ctx.jvp_trap32(x, p, 0.2, out)
+        outs.append(out)
+    assert all(o.equal(outs[0]) for o in outs[1:])
+    info = runner.tune(reps=2)
+    assert len(info["kz"]) == 3 and info["ms"] > 0
