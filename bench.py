@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: trapezoidal TDGL steps on one MI355X.
 
-Workload (the plain FP64 production film): 128x128x16 cells, h = 0.5,
+Headline workload (the plain FP64 production film): 128x128x16 cells, h = 0.5,
 kappa = 4.0, Bz = 0.4 applied through the boundary links (AppliedField
 default on-window, t_stop = 1000 dt), a centred 32x32-cell square hole through
 all z (tau = 0.1 relaxation in the hole), psi = 1 + 1e-3 complex Gaussian
@@ -15,6 +15,14 @@ One JSON line on stdout.  `roofline` is the dominant kernel class, measured
 with HIP events around every launch on the library's stream over a second,
 instrumented pass of the same K steps.  `cpu_baseline` is the CPU oracle
 (faithful FP64 restatement of tdgl3d) timed on a bounded sample on this host.
+
+Secondary sections of the same line (each with its own roofline):
+  mixed   -- the same film and steps in mixed precision; ratio and error of
+             |psi|^2, B_z, J_s against the FP64 run;
+  large   -- 256x256x64 film with 1% insulating spheres (kappa 3, Bz 0.5,
+             TGCR tol 1e-4): FP64 vs mixed steps and their error;
+  batched -- 64 uneven films (seed 3) in lockstep mixed-precision attempts,
+             against looping the plain FP64 step over the same films.
 """
 
 from __future__ import annotations
@@ -55,7 +63,7 @@ def build_film():
 class Stepper:
     """The adaptive trapezoidal loop of the integrator, without history."""
 
-    def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8):
+    def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8, precision="f64"):
         from nanowire_tdgl_amd import _native
         from nanowire_tdgl_amd.engine import engine_for
         self.dev = dev
@@ -63,7 +71,7 @@ class Stepper:
         self.ctx = self.eng.ctx
         self.X = self.ctx.to_internal(x0)
         self.dt, self.cur, self.t, self.t_stop = dt, dt, 0.0, t_stop
-        self.opts = _native.solver_opts(tol_gcr=tol_gcr)
+        self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision)
         self.log = []
 
     def accepted_steps(self, k: int) -> None:
@@ -89,6 +97,170 @@ class Stepper:
         X, self.t, self.cur, nlog = snap
         self.X.copy_(X)
         del self.log[nlog:]
+
+    def observables(self):
+        n = self.dev.params.n_interior
+        b, j = self.ctx.observables(self.X)
+        psi = self.ctx.to_reference(self.X).cpu().numpy()[:n]
+        return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()
+
+
+def rel_errors(a, b) -> dict:
+    """max |a - b| / max |b| for |psi|^2, B_z, J_s (b = the FP64 run)."""
+    out = {}
+    for name, u, v in zip(("psi2", "bz", "js"), a, b):
+        out[name] = float(np.max(np.abs(u - v)) / max(float(np.max(np.abs(v))), 1e-300))
+    return out
+
+
+def roofline_of(per_class: dict) -> dict:
+    dom = max(per_class, key=lambda k: per_class[k]["ms"])
+    c = per_class[dom]
+    avg_ms = c["ms"] / c["launches"]
+    bpl = c["bytes"] / c["launches"]
+    ach = bpl / (avg_ms * 1e-3) / 1e9
+    for k, v in per_class.items():
+        if v["launches"]:
+            v["avg_us"] = round(v["ms"] / v["launches"] * 1e3, 2)
+            v["GBps"] = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bpl}
+
+
+def timed_run(dev, x0, precision, steps, warmup, tol, profile=True):
+    """Warm up, time `steps` accepted steps, then (optionally) re-run them
+    instrumented from the same state.  Returns (Stepper, seconds, per_class, roofline)."""
+    import torch
+    s = Stepper(dev, x0, tol_gcr=tol, precision=precision)
+    s.accepted_steps(warmup)
+    torch.cuda.synchronize()
+    snap = s.snapshot()
+    t0 = time.perf_counter()
+    s.accepted_steps(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    final = s.snapshot()
+    per_class = roof = None
+    if profile:
+        s.restore(snap)
+        s.ctx.profile_enable(True)
+        s.accepted_steps(steps)
+        per_class = s.ctx.profile_read()
+        s.ctx.profile_enable(False)
+        roof = roofline_of(per_class)
+        s.restore(final)
+    s.timed = s.log[snap[3]:final[3]]
+    return s, el, per_class, roof
+
+
+def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
+    n = dev.params.n_interior
+    s, el, pc, roof = timed_run(dev, x0, "mixed", steps, warmup, tol)
+    v, v64 = n * steps / el, n * steps / fp64_el
+    err = rel_errors(s.observables(), fp64_stepper.observables())
+    log(f"[bench] mixed: {v:.4g} cell-steps/s ({v / v64:.2f}x FP64), err {err}")
+    return {"value": v, "unit": "cell-steps/s", "ms_per_step": el / steps * 1e3,
+            "speedup_vs_fp64": v / v64,
+            "rel_err_vs_fp64": err, "roofline": roof, "kernel_classes": pc,
+            "tgcr_iters_per_step": sum(r["tgcr_iters_total"] for r in s.timed) / steps}
+
+
+def section_large(steps, warmup):
+    from nanowire_tdgl_amd import workloads
+    t0 = time.perf_counter()
+    dev, x0 = workloads.film256_spheres()
+    n = dev.params.n_interior
+    log(f"[bench] large film built in {time.perf_counter() - t0:.1f} s, n_interior={n}, "
+        f"insulating fraction {1 - dev.material.interior_sc_mask.mean():.4f}")
+    tol = 1e-4
+    a, ela, _, roofa = timed_run(dev, x0, "f64", steps, warmup, tol, profile=False)
+    b, elb, pcb, roofb = timed_run(dev, x0, "mixed", steps, warmup, tol)
+    va, vb = n * steps / ela, n * steps / elb
+    err = rel_errors(b.observables(), a.observables())
+    log(f"[bench] large: FP64 {va:.4g}, mixed {vb:.4g} cell-steps/s ({vb / va:.2f}x), err {err}")
+    return {"workload": "256x256x64 film, h=0.5, kappa=3, Bz=0.5, 1% insulating spheres r in [2,4] "
+                        "(seed 2), psi = (1 + 1e-3 noise) * sc_mask, dt=0.2, TGCR tol 1e-4",
+            "n_interior": n, "steps": steps, "fp64_cell_steps_s": va, "mixed_cell_steps_s": vb,
+            "speedup": vb / va, "rel_err_vs_fp64": err, "roofline_mixed": roofb,
+            "kernel_classes_mixed": pcb,
+            "tgcr_iters_per_step": {"f64": sum(r["tgcr_iters_total"] for r in a.timed) / steps,
+                                    "mixed": sum(r["tgcr_iters_total"] for r in b.timed) / steps}}
+
+
+def section_batched(attempts, warmup, fp64_steps):
+    import torch
+    from nanowire_tdgl_amd import _native, workloads
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = workloads.batch_films()
+    ns = np.array([f.device.params.n_interior for f in films])
+    tol, dt = 1e-4, 0.2
+    runner = BatchRunner([f.device for f in films])
+    tune = runner.tune(reps=3)
+    X = runner.pack([f.x0 for f in films])
+    opts = _native.solver_opts(tol_gcr=tol, precision="mixed")
+    t = np.zeros(len(films))
+
+    def attempt():
+        runner.set_fields(t, 300 * dt)
+        st = runner.batch.trap_step(X, dt, opts)
+        ok = np.array([s.converged for s in st], dtype=bool)
+        t[ok] += dt
+        return st, ok
+
+    for _ in range(warmup):
+        attempt()
+    torch.cuda.synchronize()
+    X0, t0v = X.clone(), t.copy()
+    c0 = time.perf_counter()
+    acc = 0.0
+    for _ in range(attempts):
+        st, ok = attempt()
+        acc += float(ns[ok].sum())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - c0
+    v = acc / el
+    # instrumented repeat from the same state
+    X.copy_(X0)
+    t[:] = t0v
+    runner.batch.profile_enable(True)
+    for _ in range(attempts):
+        attempt()
+    pc = runner.batch.profile_read()
+    runner.batch.profile_enable(False)
+    roof = roofline_of(pc)
+    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films ({el:.2f} s)")
+    # baseline: loop the plain FP64 step over the films, same steps from the same start
+    steppers = [Stepper(f.device, f.x0, tol_gcr=tol) for f in films]
+    for s in steppers:
+        s.accepted_steps(warmup)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    for s in steppers:
+        s.accepted_steps(fp64_steps)
+    torch.cuda.synchronize()
+    el64 = time.perf_counter() - c0
+    v64 = float(ns.sum()) * fp64_steps / el64
+    log(f"[bench] FP64 loop: {v64:.4g} cell-steps/s ({el64:.2f} s); batched/loop = {v / v64:.2f}")
+    # error of the batched mixed states against the FP64 loop (films at the same time)
+    worst = {"psi2": 0.0, "bz": 0.0, "js": 0.0}
+    for f, s in enumerate(steppers):
+        if abs(t[f] - s.t) > 1e-9:
+            continue
+        eng = runner.engines[f]
+        n = f_n = films[f].device.params.n_interior
+        b, j = eng.ctx.observables(runner.batch.view(X, f))
+        psi = eng.ctx.to_reference(runner.batch.view(X, f)).cpu().numpy()[:f_n]
+        e = rel_errors((np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()), s.observables())
+        worst = {k: max(worst[k], e[k]) for k in worst}
+    log(f"[bench] batched vs FP64 loop, worst over films: {worst}")
+    return {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
+                        "Bz~U[0.05,0.8], holes on even films; dt=0.2, TGCR tol 1e-4",
+            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
+            "mixed_batched_cell_steps_s": v, "fp64_loop_cell_steps_s": v64,
+            "speedup_vs_fp64_loop": v / v64, "rel_err_vs_fp64_worst_film": worst,
+            "roofline": roof, "kernel_classes": pc,
+            "autotune": {"kz_policy_ms": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
 
 
 def cpu_baseline(step_stats: dict, n_interior: int, budget: float) -> dict:
@@ -118,6 +290,10 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=2)
+    ap.add_argument("--batch-attempts", type=int, default=3)
+    ap.add_argument("--sections", default="mixed,large,batched",
+                    help="comma list of secondary sections to run (empty: headline only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU split of "
@@ -128,40 +304,22 @@ def main() -> None:
     dev, x0 = build_film()
     n = dev.params.n_interior
     log(f"[bench] film {dev.params.Nx}x{dev.params.Ny}x{dev.params.Nz}, n_interior={n}")
-    s = Stepper(dev, x0)
-    s.accepted_steps(args.warmup)
-    torch.cuda.synchronize()
-    snap = s.snapshot()
-    log(f"[bench] warmup done ({args.warmup} steps), timing {args.steps} steps")
-    t0 = time.perf_counter()
-    s.accepted_steps(args.steps)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    timed = s.log[snap[3]:]
+    s, el, per_class, roof = timed_run(dev, x0, "f64", args.steps, args.warmup, 1e-8,
+                                       profile=not args.no_profile)
+    timed = s.timed
     value = n * args.steps / el
     log(f"[bench] {el:.3f} s for {args.steps} steps ({len(timed)} attempts): {value:.4g} cell-steps/s")
-
-    roof = None
-    per_class = None
-    if not args.no_profile:
-        s.restore(snap)
-        s.ctx.profile_enable(True)
-        s.accepted_steps(args.steps)
-        per_class = s.ctx.profile_read()
-        s.ctx.profile_enable(False)
-        dom = max(per_class, key=lambda k: per_class[k]["ms"])
-        c = per_class[dom]
-        avg_ms = c["ms"] / c["launches"]
-        bpl = c["bytes"] / c["launches"]
-        ach = bpl / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bpl}
-        for k, v in per_class.items():
-            if v["launches"]:
-                v["avg_us"] = round(v["ms"] / v["launches"] * 1e3, 2)
-                v["GBps"] = round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)
-        log(f"[bench] dominant kernel {dom}: {avg_ms * 1e3:.1f} us/launch, {ach:.0f} GB/s")
+    if roof:
+        log(f"[bench] dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
+            f"{roof['achieved']:.0f} GB/s")
+    sections = [x for x in args.sections.split(",") if x]
+    extra = {}
+    if "mixed" in sections:
+        extra["mixed"] = section_mixed(dev, x0, s, el, args.steps, args.warmup, 1e-8)
+    if "large" in sections:
+        extra["large"] = section_large(args.large_steps, 1)
+    if "batched" in sections:
+        extra["batched"] = section_batched(args.batch_attempts, 1, args.batch_attempts)
 
     first = s.log[0]
     cpu = None
@@ -189,6 +347,7 @@ def main() -> None:
         },
         "kernel_classes": per_class,
     }
+    res.update(extra)
     print(json.dumps(res), flush=True)
 
 

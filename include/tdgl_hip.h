@@ -209,6 +209,10 @@ int tdgl_batch_get_kz(const tdgl_batch* batch, int film);
 /* Average milliseconds of `reps` launches of the batch's FP32 JVP stencil over
  * its work buffers, as currently laid out (what the autotuner measures). */
 int tdgl_batch_time_stencil(tdgl_batch* batch, int reps, double* ms);
+/* TGCR iterations after the first replay one captured hipGraph of the
+ * iteration's kernels (default on); off: the same kernels launched eagerly.
+ * Results are identical either way. */
+int tdgl_batch_set_graphs(tdgl_batch* batch, int enable);
 /* One step attempt for every film f with active[f] != 0 (active may be NULL:
  * all), with its own dt[f] (host array).  stats[f] as tdgl_trap_step's;
  * opts->precision must be TDGL_PREC_MIXED. */
@@ -226,12 +230,13 @@ int tdgl_batch_profile_read(tdgl_batch* batch, int kernel_class, double* ms_tota
 #define TDGL_K_PRED 0      /* predictor + gamma stencil */
 #define TDGL_K_NEWTON 1    /* Newton residual stencil (f, r = -g, norms) */
 #define TDGL_K_JVP 2       /* fused JVP stencil (FP64 finite difference; mixed: analytic FP32) */
-#define TDGL_K_MGS 3       /* MGS pass: Ap -= beta Ap_j, next beta (mixed: first CGS pass) */
-#define TDGL_K_MGS_LAST 4  /* last MGS update + ||Ap|| (mixed: conditional second CGS pass) */
+#define TDGL_K_MGS 3       /* MGS pass: Ap -= beta Ap_j, next beta (mixed: CGS dot products) */
+#define TDGL_K_MGS_LAST 4  /* last MGS update + ||Ap|| (mixed: CGS update Ap -= Q c) */
 #define TDGL_K_FINAL_P 5   /* p_k assembly + normalisation + alpha (mixed: Q_k, alpha, r update) */
 #define TDGL_K_UPDATE_R 6  /* r -= alpha Ap + ||r|| (mixed: stop tests) */
 #define TDGL_K_ACCUM 7     /* dx = sum alpha p; x += dx; ||dx||_inf (mixed: H y = alpha, x += R y) */
-#define TDGL_K_COUNT 8
+#define TDGL_K_CGS2 8      /* mixed only: the conditional second CGS pass */
+#define TDGL_K_COUNT 9
 int tdgl_profile_enable(tdgl_ctx* ctx, int enable);
 int tdgl_profile_read(tdgl_ctx* ctx, int kernel_class, double* ms_total, int64_t* launches,
                       double* bytes_total);

@@ -31,11 +31,12 @@ EXPORTS = [
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
-    "tdgl_batch_time_stencil",
+    "tdgl_batch_time_stencil", "tdgl_batch_set_graphs",
     "tdgl_batch_trap_step", "tdgl_batch_profile_enable", "tdgl_batch_profile_read",
 ]
 
-KERNEL_CLASSES = ["pred", "newton", "jvp", "mgs", "mgs_last", "final_p", "update_r", "accum"]
+KERNEL_CLASSES = ["pred", "newton", "jvp", "mgs", "mgs_last", "final_p", "update_r", "accum",
+                  "cgs2"]
 
 
 class Grid(C.Structure):
@@ -111,6 +112,7 @@ def load() -> C.CDLL:
         "tdgl_batch_set_kz": ([P, C.c_int, C.c_int], C.c_int),
         "tdgl_batch_get_kz": ([P, C.c_int], C.c_int),
         "tdgl_batch_time_stencil": ([P, C.c_int, C.POINTER(D)], C.c_int),
+        "tdgl_batch_set_graphs": ([P, C.c_int], C.c_int),
         "tdgl_batch_trap_step": ([P, P, P, P, C.POINTER(SolverOpts), P], C.c_int),
         "tdgl_batch_profile_enable": ([P, C.c_int], C.c_int),
         "tdgl_batch_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
@@ -355,6 +357,9 @@ class Batch:
 
     def get_kz(self, f: int) -> int:
         return int(self.lib.tdgl_batch_get_kz(self.h, int(f)))
+
+    def set_graphs(self, on: bool) -> None:
+        self._call("tdgl_batch_set_graphs", C.c_int(1 if on else 0))
 
     def time_stencil(self, reps: int = 5) -> float:
         ms = C.c_double()

@@ -654,7 +654,7 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
 
 extern "C" int tdgl_profile_enable(tdgl_ctx* ctx, int enable) {
   if (!ctx) return TDGL_E_INVALID;
-  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipDeviceSynchronize());  // the context's own batch may have used its private stream
   prof_harvest(ctx);
   ctx->prof.on = enable != 0;
   ctx->prof.reset();
@@ -663,7 +663,7 @@ extern "C" int tdgl_profile_enable(tdgl_ctx* ctx, int enable) {
 
 extern "C" int tdgl_profile_read(tdgl_ctx* ctx, int cls, double* ms, int64_t* n, double* bytes) {
   if (!ctx || cls < 0 || cls >= TDGL_K_COUNT) return TDGL_E_INVALID;
-  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipDeviceSynchronize());
   prof_harvest(ctx);
   if (ms) *ms = ctx->prof.ms[cls];
   if (n) *n = ctx->prof.n[cls];

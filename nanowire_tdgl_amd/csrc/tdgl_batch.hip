@@ -68,7 +68,15 @@ struct tdgl_batch {
   float** d_qtab = nullptr;  // kMaxKrylov entries
 
   double* h_rep = nullptr;  // pinned: nfilm * kReportLen
-  int* h_act = nullptr;     // pinned: nfilm
+  int* h_act = nullptr;     // pinned: 3 * nfilm (set_act staging + two readback slots)
+
+  // Work runs on a private stream (graph capture needs a non-legacy stream);
+  // events order it after, and the caller's stream after it.
+  hipStream_t ps = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_act[2] = {nullptr, nullptr};
+  bool use_graph = true;
+  hipGraphExec_t body = nullptr;  // one TGCR iteration (i >= 1) for the current layout
+  double body_tol = -1.0;
 
   ProfSink own_prof;
   ProfSink* prof = &own_prof;
@@ -114,6 +122,19 @@ BatchK kview(const tdgl_batch* B) {
   return k;
 }
 
+// Order the private stream after the caller's stream (entry) and the caller's
+// stream after the private one (exit).
+int sync_in(tdgl_batch* B) {
+  BCHK(hipEventRecord(B->ev_in, B->st));
+  BCHK(hipStreamWaitEvent(B->ps, B->ev_in, 0));
+  return TDGL_OK;
+}
+int sync_out(tdgl_batch* B) {
+  BCHK(hipEventRecord(B->ev_out, B->ps));
+  BCHK(hipStreamWaitEvent(B->st, B->ev_out, 0));
+  return TDGL_OK;
+}
+
 int default_kz(const Geo& g) {
   if (!g.is3d) return 1;
   const long long tiles = (long long)((g.nx + kStencilTX - 1) / kStencilTX) * ((g.ny + kStencilTY - 1) / kStencilTY);
@@ -154,6 +175,10 @@ int build_layout(tdgl_batch* B) {
   }
   B->total_len = off;
   B->hlen = hoff;
+  if (B->body) {
+    (void)hipGraphExecDestroy(B->body);
+    B->body = nullptr;
+  }
   B->nitems = (int)B->item_film.size();
   B->nsblk = (int)B->sblk_film.size();
   for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
@@ -176,7 +201,7 @@ int refresh(tdgl_batch* B) {
     B->hf[f].b = B->films[f]->b;
     B->hf[f].mt = B->films[f]->mt;
   }
-  BCHK(hipMemcpyAsync(B->d_films, B->hf.data(), sizeof(FilmDesc) * B->nfilm, hipMemcpyHostToDevice, B->st));
+  BCHK(hipMemcpyAsync(B->d_films, B->hf.data(), sizeof(FilmDesc) * B->nfilm, hipMemcpyHostToDevice, B->ps));
   return TDGL_OK;
 }
 
@@ -210,15 +235,15 @@ int ensure_q(tdgl_batch* B, int i) {
 
 int set_act(tdgl_batch* B, const std::vector<int>& act) {
   memcpy(B->h_act, act.data(), sizeof(int) * B->nfilm);
-  BCHK(hipMemcpyAsync(B->d_act, B->h_act, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->st));
+  BCHK(hipMemcpyAsync(B->d_act, B->h_act, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->ps));
   return TDGL_OK;
 }
 
 int report(tdgl_batch* B) {
-  launch_breport(kview(B), B->d_fstat, B->d_rep, B->st);
+  launch_breport(kview(B), B->d_fstat, B->d_rep, B->ps);
   BCHK(hipGetLastError());
-  BCHK(hipMemcpyAsync(B->h_rep, B->d_rep, sizeof(double) * kReportLen * B->nfilm, hipMemcpyDeviceToHost, B->st));
-  BCHK(hipStreamSynchronize(B->st));
+  BCHK(hipMemcpyAsync(B->h_rep, B->d_rep, sizeof(double) * kReportLen * B->nfilm, hipMemcpyDeviceToHost, B->ps));
+  BCHK(hipStreamSynchronize(B->ps));
   if (!B->prof->pending.empty()) B->prof->harvest();
   return TDGL_OK;
 }
@@ -237,11 +262,11 @@ double sum_over(const tdgl_batch* B, const std::vector<int>& act, double (*w)(co
 // res[RS_XX, RS_RR, RS_GINF].
 int newton_eval(tdgl_batch* B, const std::vector<int>& act, std::vector<tdgl_step_stats>& S) {
   const double by = 2.0 * sum_over(B, act, s64) + 2.0 * sum_over(B, act, s32);
-  const int tok = B->prof->begin(B->st, TDGL_K_NEWTON, by);
+  const int tok = B->prof->begin(B->ps, TDGL_K_NEWTON, by);
   launch_bstencil(kview(B), B->is3d, M_NEWTON32, B->w_x, nullptr, nullptr, B->w_gam, B->x32, B->r32,
-                  B->d_pitem, B->st);
-  launch_bfin(kview(B), 2, 1, B->d_pitem, RS_XX, B->st);
-  B->prof->end(B->st, tok);
+                  B->d_pitem, B->ps);
+  launch_bfin(kview(B), 2, 1, B->d_pitem, RS_XX, B->ps);
+  B->prof->end(B->ps, tok);
   BCHK(hipGetLastError());
   for (int f = 0; f < B->nfilm; ++f)
     if (act[f]) {
@@ -251,72 +276,169 @@ int newton_eval(tdgl_batch* B, const std::vector<int>& act, std::vector<tdgl_ste
   return TDGL_OK;
 }
 
+constexpr int kColsumY = 32;  // column blocks per film in the coefficient sums
+
+// The kernels of TGCR iteration i >= 1 (every argument is iteration-independent:
+// the kernels read i from the device counter).  With `prof`, each pass is
+// bracketed with its algorithmic bytes for iteration i and the host's view
+// `act` of the films taking part.
+void enqueue_body(tdgl_batch* B, double tol, bool prof, int i, const std::vector<int>& act) {
+  const BatchK bk = kview(B);
+  const double S3 = prof ? sum_over(B, act, s32) : 0.0;
+  int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S3) : -1;
+  launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+  tok = prof ? B->prof->begin(B->ps, TDGL_K_MGS, (i + 1.0) * S3, 2) : -1;  // read Q_0..Q_{i-1}, W
+  launch_bmdot(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
+  launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD1, kColsumY, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+  tok = prof ? B->prof->begin(B->ps, TDGL_K_MGS_LAST, (i + 3.0) * S3, 2) : -1;  // Q_j, r; W r/w
+  launch_bupd(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
+  launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP1, 1, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+  // second pass: the kernels return at once for films that do not need it
+  tok = prof ? B->prof->begin(B->ps, TDGL_K_CGS2, 0.0, 4) : -1;
+  launch_bmdot(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
+  launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD2, kColsumY, B->ps);
+  launch_bupd(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
+  launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP2, 1, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+  tok = prof ? B->prof->begin(B->ps, TDGL_K_FINAL_P, 4.0 * S3) : -1;
+  launch_bfinish(bk, B->d_qtab, B->r32, B->d_iter, B->d_H, B->d_pm, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+  tok = prof ? B->prof->begin(B->ps, TDGL_K_UPDATE_R, 0.0, 2) : -1;
+  launch_bstatus(bk, B->d_fstat, B->d_iter, tol, B->d_pm, B->ps);
+  launch_bnext(bk, B->d_iter, B->d_nact, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+}
+
+// Iteration i >= 1 without profiling: replay the captured graph of the body.
+int launch_body(tdgl_batch* B, double tol) {
+  if (!B->use_graph) {
+    enqueue_body(B, tol, false, 1, {});
+    BCHK(hipGetLastError());
+    return TDGL_OK;
+  }
+  if (B->body && B->body_tol != tol) {
+    BCHK(hipGraphExecDestroy(B->body));
+    B->body = nullptr;
+  }
+  if (!B->body) {
+    hipGraph_t g = nullptr;
+    BCHK(hipStreamBeginCapture(B->ps, hipStreamCaptureModeThreadLocal));
+    enqueue_body(B, tol, false, 1, {});
+    const hipError_t e = hipGetLastError();
+    BCHK(hipStreamEndCapture(B->ps, &g));
+    BCHK(e);
+    const hipError_t ei = hipGraphInstantiate(&B->body, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    BCHK(ei);
+    B->body_tol = tol;
+  }
+  BCHK(hipGraphLaunch(B->body, B->ps));
+  return TDGL_OK;
+}
+
+// Queue the copy of the films' flags into readback slot `slot`.
+int post_flags(tdgl_batch* B, int slot) {
+  BCHK(hipMemcpyAsync(B->h_act + (size_t)B->nfilm * (1 + slot), B->d_act, sizeof(int) * B->nfilm,
+                      hipMemcpyDeviceToHost, B->ps));
+  BCHK(hipEventRecord(B->ev_act[slot], B->ps));
+  return TDGL_OK;
+}
+// Wait for slot `slot` and count the films still iterating.
+int read_flags(tdgl_batch* B, int slot, std::vector<int>& act, int* live) {
+  BCHK(hipEventSynchronize(B->ev_act[slot]));
+  const int* h = B->h_act + (size_t)B->nfilm * (1 + slot);
+  *live = 0;
+  for (int f = 0; f < B->nfilm; ++f) {
+    act[f] = h[f];
+    *live += act[f];
+  }
+  return TDGL_OK;
+}
+
 // TGCR for the active films on (I - dt/2 J) dx = r32 at x32; on return the
-// report holds each film's outcome.  `work[f]` collects per-film bytes.
+// report holds each film's outcome.  Without profiling the host keeps one
+// iteration queued ahead of the flags it reads (a film that stopped makes the
+// extra iteration a no-op for it), and iterations i >= 1 replay a hipGraph.
 int tgcr(tdgl_batch* B, const std::vector<int>& act0, double tol, std::vector<tdgl_step_stats>& S) {
   const int F = B->nfilm;
+  const bool prof = B->prof->on;
   int rc;
   if ((rc = set_act(B, act0))) return rc;
-  launch_btinit(kview(B), B->d_fstat, B->d_maxk, B->d_iter, B->st);
-  BCHK(hipMemcpyAsync(B->r0_32, B->r32, sizeof(float) * (size_t)B->total_len, hipMemcpyDeviceToDevice, B->st));
-  BCHK(hipGetLastError());
+  launch_btinit(kview(B), B->d_fstat, B->d_maxk, B->d_iter, B->ps);
+  BCHK(hipMemcpyAsync(B->r0_32, B->r32, sizeof(float) * (size_t)B->total_len, hipMemcpyDeviceToDevice, B->ps));
   std::vector<int> act = act0;
   const BatchK bk = kview(B);
-  for (int i = 0;; ++i) {
-    if ((rc = ensure_q(B, i))) return rc;
-    const double S3 = sum_over(B, act, s32);
-    int tok = B->prof->begin(B->st, TDGL_K_JVP, 3.0 * S3);
-    launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->st);
-    B->prof->end(B->st, tok);
-    if (i == 0) {
-      tok = B->prof->begin(B->st, TDGL_K_MGS, 2.0 * S3, 2);
-      launch_bmdot(bk, 0, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
-      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD0, 1, B->st);
-      B->prof->end(B->st, tok);
-    } else {
-      const int gy = std::min(i + 1, 32);
-      tok = B->prof->begin(B->st, TDGL_K_MGS, (2.0 * i + 4.0) * S3, 4);
-      launch_bmdot(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
-      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD1, gy, B->st);
-      launch_bupd(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
-      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP1, 1, B->st);
-      B->prof->end(B->st, tok);
-      // second pass: the kernels return at once for films that do not need it
-      tok = B->prof->begin(B->st, TDGL_K_MGS_LAST, 0.0, 4);
-      launch_bmdot(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
-      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD2, gy, B->st);
-      launch_bupd(bk, 2, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->st);
-      launch_bcolsum(bk, B->d_iter, B->d_pm, CS_UP2, 1, B->st);
-      B->prof->end(B->st, tok);
-    }
-    tok = B->prof->begin(B->st, TDGL_K_FINAL_P, 4.0 * S3);
-    launch_bfinish(bk, B->d_qtab, B->r32, B->d_iter, B->d_H, B->d_pm, B->st);
-    B->prof->end(B->st, tok);
-    tok = B->prof->begin(B->st, TDGL_K_UPDATE_R, 0.0, 2);
-    launch_bstatus(bk, B->d_fstat, B->d_iter, tol, B->d_pm, B->st);
-    launch_bnext(bk, B->d_iter, B->d_nact, B->st);
-    B->prof->end(B->st, tok);
+  if ((rc = ensure_q(B, 1))) return rc;
+  // iteration 0
+  {
+    const double S3 = prof ? sum_over(B, act, s32) : 0.0;
+    int tok = B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S3);
+    launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->ps);
+    B->prof->end(B->ps, tok);
+    tok = B->prof->begin(B->ps, TDGL_K_MGS, 2.0 * S3, 2);
+    launch_bmdot(bk, 0, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
+    launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD0, 1, B->ps);
+    B->prof->end(B->ps, tok);
+    tok = B->prof->begin(B->ps, TDGL_K_FINAL_P, 4.0 * S3);
+    launch_bfinish(bk, B->d_qtab, B->r32, B->d_iter, B->d_H, B->d_pm, B->ps);
+    B->prof->end(B->ps, tok);
+    tok = B->prof->begin(B->ps, TDGL_K_UPDATE_R, 0.0, 2);
+    launch_bstatus(bk, B->d_fstat, B->d_iter, tol, B->d_pm, B->ps);
+    launch_bnext(bk, B->d_iter, B->d_nact, B->ps);
+    B->prof->end(B->ps, tok);
     BCHK(hipGetLastError());
-    for (int f = 0; f < F; ++f)
-      if (act[f]) {
-        S[f].jvp_evals++;
-        S[f].bytes_moved += (i == 0 ? 9.0 : 2.0 * i + 11.0) * s32(B, f);
-      }
-    BCHK(hipMemcpyAsync(B->h_act, B->d_act, sizeof(int) * F, hipMemcpyDeviceToHost, B->st));
-    BCHK(hipStreamSynchronize(B->st));
-    if (!B->prof->pending.empty()) B->prof->harvest();
-    int live = 0;
-    for (int f = 0; f < F; ++f) {
-      act[f] = B->h_act[f];
-      live += act[f];
-    }
-    if (!live) break;
+    if ((rc = post_flags(B, 0))) return rc;
   }
-  return report(B);
+  int live = 0;
+  for (int i = 0;; ++i) {
+    // iteration i is queued; flags of iteration i are in slot i % 2
+    if (prof) {
+      if ((rc = read_flags(B, i % 2, act, &live))) return rc;
+      B->prof->harvest();
+      if (!live) break;
+      if ((rc = ensure_q(B, i + 1))) return rc;
+      enqueue_body(B, tol, true, i + 1, act);
+      BCHK(hipGetLastError());
+      if ((rc = post_flags(B, (i + 1) % 2))) return rc;
+      continue;
+    }
+    const bool ahead = i + 1 < kMaxKrylov;
+    if (ahead) {
+      if ((rc = ensure_q(B, std::min(kMaxKrylov - 1, ((i + 1) / 16 + 1) * 16)))) return rc;
+      if ((rc = launch_body(B, tol))) return rc;
+      if ((rc = post_flags(B, (i + 1) % 2))) return rc;
+    }
+    if ((rc = read_flags(B, i % 2, act, &live))) return rc;
+    if (!live) break;
+    if (!ahead) return bfail(B, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
+  }
+  if ((rc = report(B))) return rc;
+  for (int f = 0; f < F; ++f) {
+    if (!act0[f]) continue;
+    const double kit = rep(B, f)[8];
+    S[f].jvp_evals += (int)kit;
+    if (kit >= 1) S[f].bytes_moved += (9.0 + 11.0 * (kit - 1) + (kit - 1) * kit) * s32(B, f);
+  }
+  return TDGL_OK;
 }
+
+int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* active,
+                    const tdgl_solver_opts* o, tdgl_step_stats* stats);
 
 int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
                const tdgl_solver_opts* o, tdgl_step_stats* stats) {
+  int rc;
+  if ((rc = sync_in(B))) return rc;
+  rc = batch_step_impl(B, X, dt, active, o, stats);
+  const int rc2 = sync_out(B);
+  return rc ? rc : rc2;
+}
+
+int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* active,
+                    const tdgl_solver_opts* o, tdgl_step_stats* stats) {
   const int F = B->nfilm;
   int rc;
   if ((rc = ensure_work(B))) return rc;
@@ -332,8 +454,8 @@ int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
     req[f] = (int)std::min<long long>(r, 1LL << 30);
     maxk[f] = std::min(req[f], B->kcap[f]);
   }
-  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * 2 * F, hipMemcpyHostToDevice, B->st));
-  BCHK(hipMemcpyAsync(B->d_maxk, maxk.data(), sizeof(int) * F, hipMemcpyHostToDevice, B->st));
+  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * 2 * F, hipMemcpyHostToDevice, B->ps));
+  BCHK(hipMemcpyAsync(B->d_maxk, maxk.data(), sizeof(int) * F, hipMemcpyHostToDevice, B->ps));
   const double floor_ = o->mixed_tol_floor > 0 ? o->mixed_tol_floor : 1e-6;
   const double tol = std::max(o->tol_gcr, floor_);
 
@@ -345,10 +467,10 @@ int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
   // predictor + gamma (integrators.py:156-158)
   if ((rc = set_act(B, act))) return rc;
   {
-    const int tok = B->prof->begin(B->st, TDGL_K_PRED, 3.0 * sum_over(B, act, s64));
+    const int tok = B->prof->begin(B->ps, TDGL_K_PRED, 3.0 * sum_over(B, act, s64));
     launch_bstencil(kview(B), B->is3d, M_PRED, X, B->w_x, B->w_gam, nullptr, nullptr, nullptr,
-                    B->d_pitem, B->st);
-    B->prof->end(B->st, tok);
+                    B->d_pitem, B->ps);
+    B->prof->end(B->ps, tok);
     BCHK(hipGetLastError());
     for (int f = 0; f < F; ++f)
       if (act[f]) {
@@ -392,11 +514,11 @@ int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
       double by = 2.0 * sum_over(B, ok, s64);
       for (int f = 0; f < F; ++f)
         if (ok[f]) by += (rep(B, f)[7] + 1.0) * s32(B, f);
-      const int tok = B->prof->begin(B->st, TDGL_K_ACCUM, by, 3);
-      launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->st);
-      launch_baccum(kview(B), B->w_x, B->r0_32, B->d_qtab, B->d_fstat, B->d_pm, B->st);
-      launch_bcolsum(kview(B), B->d_iter, B->d_pm, CS_DXINF, 1, B->st);
-      B->prof->end(B->st, tok);
+      const int tok = B->prof->begin(B->ps, TDGL_K_ACCUM, by, 3);
+      launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->ps);
+      launch_baccum(kview(B), B->w_x, B->r0_32, B->d_qtab, B->d_fstat, B->d_pm, B->ps);
+      launch_bcolsum(kview(B), B->d_iter, B->d_pm, CS_DXINF, 1, B->ps);
+      B->prof->end(B->ps, tok);
       BCHK(hipGetLastError());
       for (int f = 0; f < F; ++f)
         if (ok[f]) S[f].bytes_moved += (rep(B, f)[7] + 1.0) * s32(B, f) + 2.0 * s64(B, f);
@@ -412,7 +534,7 @@ int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
         if (S[f].g_inf <= o->newton_tol_f && S[f].dx_inf <= o->newton_tol_dx) {  // newton.py:118
           const FilmDesc& d = B->hf[f];
           BCHK(hipMemcpyAsync(X + d.off, B->w_x + d.off, sizeof(double) * (size_t)d.len,
-                              hipMemcpyDeviceToDevice, B->st));
+                              hipMemcpyDeviceToDevice, B->ps));
           S[f].bytes_moved += 2.0 * s64(B, f);
           S[f].converged = 1;
           S[f].newton_iters = it;
@@ -429,7 +551,7 @@ int batch_step(tdgl_batch* B, double* X, const double* dt, const int* active,
       S[f].converged = 0;
       S[f].newton_iters = o->newton_max_iter;
     }
-  BCHK(hipStreamSynchronize(B->st));
+  BCHK(hipStreamSynchronize(B->ps));
   memcpy(stats, S.data(), sizeof(tdgl_step_stats) * F);
   return TDGL_OK;
 }
@@ -450,7 +572,12 @@ int batch_init(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_nact, sizeof(int)))) return rc;
   if ((rc = balloc(B, (void**)&B->d_rep, sizeof(double) * kReportLen * F))) return rc;
   BCHK(hipHostMalloc((void**)&B->h_rep, sizeof(double) * kReportLen * F, hipHostMallocDefault));
-  BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * F, hipHostMallocDefault));
+  BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * 3 * F, hipHostMallocDefault));
+  BCHK(hipStreamCreateWithFlags(&B->ps, hipStreamNonBlocking));
+  BCHK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+  BCHK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+  BCHK(hipEventCreateWithFlags(&B->ev_act[0], hipEventDisableTiming));
+  BCHK(hipEventCreateWithFlags(&B->ev_act[1], hipEventDisableTiming));
   B->kz.resize(F);
   B->kcap.resize(F);
   for (int f = 0; f < F; ++f) {
@@ -497,23 +624,22 @@ int mixed_step_single(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opt
 }
 
 int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
-  int rc0;
-  if ((rc0 = own_batch(ctx))) return rc0;
+  int rc;
+  if ((rc = own_batch(ctx))) return rc;
   tdgl_batch* B = ctx->own_batch;
   B->st = ctx->st;
-  int rc;
-  if ((rc = refresh(B))) {
+  if ((rc = sync_in(B)) || (rc = refresh(B))) {
     ctx->err = B->err;
     return rc;
   }
   const double dtv[2] = {dt, dt / 2.0};
-  const int one = 1;
-  if (hipMemcpyAsync(B->d_dtv, dtv, sizeof(dtv), hipMemcpyHostToDevice, B->st) != hipSuccess ||
-      hipMemcpyAsync(B->d_act, &one, sizeof(int), hipMemcpyHostToDevice, B->st) != hipSuccess)
+  std::vector<int> one(1, 1);
+  if ((rc = set_act(B, one))) return rc;
+  if (hipMemcpyAsync(B->d_dtv, dtv, sizeof(dtv), hipMemcpyHostToDevice, B->ps) != hipSuccess)
     return TDGL_E_HIP;
-  launch_bjvp32(kview(B), B->is3d, x32, p32, nullptr, nullptr, ap32, B->st);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
-  return TDGL_OK;
+  launch_bjvp32(kview(B), B->is3d, x32, p32, nullptr, nullptr, ap32, B->ps);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess) return TDGL_E_HIP;
+  return sync_out(B);
 }
 
 }  // namespace tdgl
@@ -542,8 +668,12 @@ int tdgl_batch_create(tdgl_ctx* const* films, int nfilm, tdgl_batch** out) {
 
 int tdgl_batch_destroy(tdgl_batch* B) {
   if (!B) return TDGL_OK;
-  if (B->st) (void)hipStreamSynchronize(B->st);
+  if (B->ps) (void)hipStreamSynchronize(B->ps);
   else (void)hipDeviceSynchronize();
+  if (B->body) (void)hipGraphExecDestroy(B->body);
+  for (hipEvent_t e : {B->ev_in, B->ev_out, B->ev_act[0], B->ev_act[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (B->ps) (void)hipStreamDestroy(B->ps);
   for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_act,
                   (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
@@ -577,7 +707,7 @@ int tdgl_batch_set_kz(tdgl_batch* B, int f, int kz) {
   if (!B || f < 0 || f >= B->nfilm || kz < 1) return TDGL_E_INVALID;
   const Geo& g = B->films[f]->g;
   B->kz[f] = g.is3d ? std::min(kz, g.nz) : 1;
-  (void)hipStreamSynchronize(B->st);
+  (void)hipStreamSynchronize(B->ps);
   return build_layout(B);
 }
 
@@ -591,25 +721,31 @@ int tdgl_batch_time_stencil(tdgl_batch* B, int reps, double* ms) {
   int rc;
   if ((rc = ensure_work(B))) return rc;
   if ((rc = ensure_q(B, 0))) return rc;
-  if ((rc = refresh(B))) return rc;
+  if ((rc = sync_in(B)) || (rc = refresh(B))) return rc;
   std::vector<int> all(B->nfilm, 1);
   std::vector<double> dtv(2 * B->nfilm, 0.1);
   if ((rc = set_act(B, all))) return rc;
-  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * dtv.size(), hipMemcpyHostToDevice, B->st));
+  BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * dtv.size(), hipMemcpyHostToDevice, B->ps));
   hipEvent_t a, b;
   BCHK(hipEventCreate(&a));
   BCHK(hipEventCreate(&b));
-  launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->st);  // warm-up
-  BCHK(hipEventRecord(a, B->st));
+  launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);  // warm-up
+  BCHK(hipEventRecord(a, B->ps));
   for (int r = 0; r < reps; ++r)
-    launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->st);
-  BCHK(hipEventRecord(b, B->st));
+    launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);
+  BCHK(hipEventRecord(b, B->ps));
   BCHK(hipEventSynchronize(b));
   float t = 0.f;
   BCHK(hipEventElapsedTime(&t, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   *ms = (double)t / reps;
+  return sync_out(B);
+}
+
+int tdgl_batch_set_graphs(tdgl_batch* B, int enable) {
+  if (!B) return TDGL_E_INVALID;
+  B->use_graph = enable != 0;
   return TDGL_OK;
 }
 
@@ -625,7 +761,7 @@ int tdgl_batch_trap_step(tdgl_batch* B, double* X, const double* dt, const int* 
 
 int tdgl_batch_profile_enable(tdgl_batch* B, int enable) {
   if (!B) return TDGL_E_INVALID;
-  if (hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
+  if (hipStreamSynchronize(B->ps) != hipSuccess) return TDGL_E_HIP;
   B->prof->harvest();
   B->prof->on = enable != 0;
   B->prof->reset();
@@ -634,7 +770,7 @@ int tdgl_batch_profile_enable(tdgl_batch* B, int enable) {
 
 int tdgl_batch_profile_read(tdgl_batch* B, int cls, double* ms, int64_t* n, double* bytes) {
   if (!B || cls < 0 || cls >= TDGL_K_COUNT) return TDGL_E_INVALID;
-  if (hipStreamSynchronize(B->st) != hipSuccess) return TDGL_E_HIP;
+  if (hipStreamSynchronize(B->ps) != hipSuccess) return TDGL_E_HIP;
   B->prof->harvest();
   if (ms) *ms = B->prof->ms[cls];
   if (n) *n = B->prof->n[cls];

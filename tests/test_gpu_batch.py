@@ -154,3 +154,25 @@ def test_stencil_depth_does_not_change_the_jvp():
     assert all(o.equal(outs[0]) for o in outs[1:])
     info = runner.tune(reps=2)
     assert len(info["kz"]) == 3 and info["ms"] > 0
+
+
+def test_graph_replay_matches_eager_launches():
+    """TGCR iterations replayed from the captured hipGraph give the same bits
+    as the same kernels launched one by one, over several steps with eager
+    work (field changes, observables) between the replays."""
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    opts = _native.solver_opts(precision="mixed")
+    outs = []
+    for graphs in (False, True):
+        runner = BatchRunner([f.device for f in films])
+        runner.batch.set_graphs(graphs)
+        X = runner.pack([f.x0 for f in films])
+        for s in range(4):
+            runner.set_fields([0.1 * s] * len(films), 100.0)
+            st = runner.batch.trap_step(X, 0.2, opts)
+            assert all(x.converged for x in st)
+            runner.engines[0].ctx.observables(runner.batch.view(X, 0))
+        outs.append(X.clone())
+    assert outs[0].equal(outs[1])

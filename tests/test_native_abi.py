@@ -40,6 +40,7 @@ def test_loader_sets_signatures():
     lib = _native.load()
     assert lib.tdgl_ctx_create.restype is ctypes.c_int
     assert ctypes.sizeof(_native.StepStats) == 8 * 4 + 5 * 8
+    assert len(_native.KERNEL_CLASSES) == 9
     assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8
 
 
