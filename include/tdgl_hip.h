@@ -158,6 +158,23 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x_dev, const double* f0_dev,
 int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32_dev, const float* p32_dev, double dt,
                     float* ap32_dev);
 
+/* Adjoint of eval_f (no counterpart in the reference; the hand-written
+ * transpose of the Jacobian of rhs.py:147-239 at x, at the current field and
+ * material), FP64:
+ *   gx_dev  = J(x)^T w                                       (internal layout)
+ *   dfield  = d<w, F(x)>/d(Bx, By, Bz) of the uniform applied field (host
+ *             double[3]; NULL to skip)
+ *   dkappa  = d<w, F(x)>/d kappa with every node's kappa tied (host; NULL to skip)
+ *   dsc_dev = d<w, F(x)>/d sc_q, sc the superconducting mask of interior
+ *             node q (internal order; NULL to skip)
+ * <a, b> is the real inner product of the internal layout (psi as re, im). */
+int tdgl_vjp_f(tdgl_ctx* ctx, const double* x_dev, const double* w_dev, double* gx_dev,
+               double* dfield, double* dkappa, double* dsc_dev);
+/* Adjoint of the B_z observable (core/solution.py:124-158 -> bfield.py:58-68)
+ * at the current field: gx_dev = (dB_z/dx)^T gbz (gbz in the reference interior
+ * order, n values), dfield = d<gbz, B_z>/d(Bx, By, Bz) (host double[3] or NULL). */
+int tdgl_vjp_bz(tdgl_ctx* ctx, const double* gbz_dev, double* gx_dev, double* dfield);
+
 /* Reductions over internal-layout vectors (the reference's np.vdot(...).real,
  * np.linalg.norm(., 2) and np.linalg.norm(., inf): tgcr.py:116,129,136,140,149,152;
  * newton.py:100,112-113).  Results are returned to the host. */

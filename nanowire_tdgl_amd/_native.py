@@ -27,7 +27,7 @@ EXPORTS = [
     "tdgl_set_stencil_kz", "tdgl_get_stencil_kz",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
-    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
@@ -102,6 +102,8 @@ def load() -> C.CDLL:
         "tdgl_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
         "tdgl_jvp_trap32": ([P, P, P, D, P], C.c_int),
         "tdgl_set_stencil_kz": ([P, C.c_int], C.c_int),
+        "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
+        "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
         "tdgl_get_stencil_kz": ([P], C.c_int),
         "tdgl_batch_create": ([P, C.c_int, C.POINTER(P)], C.c_int),
         "tdgl_batch_destroy": ([P], C.c_int),
@@ -254,6 +256,22 @@ class Context:
                 raise ValueError("jvp_trap32 takes float32 vectors of the internal state length")
         self._call("tdgl_jvp_trap32", C.c_void_p(x32.data_ptr()), C.c_void_p(p32.data_ptr()),
                    C.c_double(dt), C.c_void_p(ap32.data_ptr()))
+
+    def vjp_f(self, x, w, gx, field=False, kappa=False, dsc=None):
+        """gx = J(x)^T w; returns (d/d(Bx,By,Bz) or None, d/dkappa or None)."""
+        df = (C.c_double * 3)() if field else None
+        dk = C.c_double() if kappa else None
+        self._call("tdgl_vjp_f", C.c_void_p(x.data_ptr()), C.c_void_p(w.data_ptr()),
+                   C.c_void_p(gx.data_ptr()), C.cast(df, C.c_void_p) if field else None,
+                   C.byref(dk) if kappa else None, C.c_void_p(_ptr(dsc)) if dsc is not None else None)
+        return (tuple(df) if field else None), (dk.value if kappa else None)
+
+    def vjp_bz(self, gbz, gx, field=False):
+        """gx = (dB_z/dx)^T gbz (gbz in reference order); returns d/d(Bx,By,Bz) or None."""
+        df = (C.c_double * 3)() if field else None
+        self._call("tdgl_vjp_bz", C.c_void_p(gbz.data_ptr()), C.c_void_p(gx.data_ptr()),
+                   C.cast(df, C.c_void_p) if field else None)
+        return tuple(df) if field else None
 
     def dot(self, a, b) -> float:
         out = C.c_double()

@@ -1,0 +1,404 @@
+// Adjoint (vector-Jacobian) kernels of eval_f and of the B_z observable, FP64.
+//
+// The Jacobian-vector product of eval_f at x is a sum, over every output
+// component of every interior node m, of terms coef * (perturbation of one
+// component of one expanded-grid cell) -- the cells being the node, its six
+// neighbours and the six diagonal cells of the curl cross terms, all read
+// through the boundary rules (the ghost cells are affine in the interior
+// values).  The transpose is applied in two steps:
+//
+//   k_vjp_cells   each node m scatters w_m * coef into a full-grid cell
+//                 adjoint A[cell][component] (FP64 atomics);
+//   k_bc_gather   each interior node q gathers A over the cells whose values
+//                 come from q under the boundary rules of rhs.py:54-139 (its
+//                 own cell, the face ghosts copied from it, periodic images),
+//                 giving (J^T w)_q;
+//   k_field_grad  the face ghosts' transverse links also carry the applied
+//                 field flux (cx, cy, cz) of the ghost rules, so the same A
+//                 gives d<w, F>/d(cx, cy, cz).
+//
+// k_vjp_cells also emits the parameter derivatives of <w, F>: kappa (the
+// link operators scale with kappa^2, the supercurrent does not) and the
+// superconducting mask sc of every interior node (FPSI's sc * (1 - |psi|^2)
+// psi - (1 - sc) psi / tau, sparse_operators.py:202-226).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "tdgl_device.hpp"
+#include "tdgl_launch.hpp"
+
+namespace tdgl {
+
+namespace {
+
+__device__ __forceinline__ void acc(double* A, long long m, int comp, double v) {
+  if (v != 0.0) atomicAdd(A + m * 5 + comp, v);
+}
+enum { CR = 0, CI = 1, LX = 2, LY = 3, LZ = 4 };
+
+template <bool IS3D>
+__device__ __forceinline__ void node_of(const Geo& g, long long q, int& i, int& j, int& k) {
+  i = 1 + (int)(q % g.nx);
+  const long long t = q / g.nx;
+  j = 1 + (int)(t % g.ny);
+  k = IS3D ? 1 + (int)(t / g.ny) : 0;
+}
+
+}  // namespace
+
+// w: adjoint weights (internal layout).  A: full grid x 5, accumulated.
+// gsc: per interior node (internal order) d<w,F>/d(sc_q), or nullptr.
+// pk: per-block partials of d<w,F>/d(kappa) (uniform kappa `kappa`), or nullptr.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt, const double* x,
+                                                           const double* w, double* A, double* gsc,
+                                                           double kappa, double* pk) {
+  const long long n = g.n;
+  const Src<double, false> s{x, nullptr, 0.0, n};
+  double kap[1] = {0.0};
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
+       q += (long long)gridDim.x * kRedThreads) {
+    int i, j, k;
+    node_of<IS3D>(g, q, i, j, k);
+    const double2 w0 = reinterpret_cast<const double2*>(w)[q];
+    const double mr = w0.x, mi = w0.y;
+    const uint8_t fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
+    const double mx = (fl & kHoleX) ? 0.0 : w[2 * n + q];
+    const double my = (fl & kHoleY) ? 0.0 : w[3 * n + q];
+    const double mz = (IS3D && !(fl & kHoleZ)) ? w[4 * n + q] : 0.0;
+
+    auto cell = [&](int ii, int jj, int kk) { return eff_cell<double, IS3D, false>(g, b, mt.flags, s, ii, jj, kk); };
+    auto mf = [&](int ii, int jj, int kk) { return mfull(g, ii, jj, kk); };
+    const Cell<double> c0 = cell(i, j, k), xm = cell(i - 1, j, k), xp = cell(i + 1, j, k);
+    const Cell<double> ym = cell(i, j - 1, k), yp = cell(i, j + 1, k);
+    Cell<double> zm{}, zp{};
+    if (IS3D) {
+      zm = cell(i, j, k - 1);
+      zp = cell(i, j, k + 1);
+    }
+    const long long m0 = mf(i, j, k), mxm = mf(i - 1, j, k), mxp = mf(i + 1, j, k);
+    const long long mym = mf(i, j - 1, k), myp = mf(i, j + 1, k);
+    const long long mzm = IS3D ? mf(i, j, k - 1) : 0, mzp = IS3D ? mf(i, j, k + 1) : 0;
+
+    const double pr = c0.pr, pi = c0.pi;
+    double sxp, cxp, sxm, cxm, syp, cyp, sym, cym, szp = 0, czp = 1, szm = 0, czm = 1;
+    sincos(c0.x, &sxp, &cxp);
+    sincos(xm.x, &sxm, &cxm);
+    sincos(c0.y, &syp, &cyp);
+    sincos(ym.y, &sym, &cym);
+    if (IS3D) {
+      sincos(c0.z, &szp, &czp);
+      sincos(zm.z, &szm, &czm);
+    }
+
+    // ---- psi equation: d psi_m = dgpsi dpsi_0 + sum of the six hopping terms + FPSI'
+    acc(A, m0, CR, mt.dgpsi * mr);
+    acc(A, m0, CI, mt.dgpsi * mi);
+    // minus neighbour nb = m - a, its link t = phi_a(m - a):
+    //   w_r = dq_r + dt q_i, w_i = dq_i - dt q_r;  d(ar, ai) = h (c w_r + s w_i, c w_i - s w_r)
+    auto minus = [&](double c, double sn, double h, const Cell<double>& nb, long long mnb, int lc) {
+      const double gr = h * (c * mr - sn * mi), gi = h * (sn * mr + c * mi);
+      acc(A, mnb, CR, gr);
+      acc(A, mnb, CI, gi);
+      acc(A, mnb, lc, gr * nb.pi - gi * nb.pr);
+    };
+    // plus neighbour nb = m + a, the node's own link t = phi_a(m):
+    //   w_r = dq_r - dt q_i, w_i = dq_i + dt q_r;  d(ar, ai) = h (c w_r - s w_i, c w_i + s w_r)
+    auto plus = [&](double c, double sn, double h, const Cell<double>& nb, long long mnb, int lc) {
+      const double gr = h * (c * mr + sn * mi), gi = h * (-sn * mr + c * mi);
+      acc(A, mnb, CR, gr);
+      acc(A, mnb, CI, gi);
+      acc(A, m0, lc, -gr * nb.pi + gi * nb.pr);
+    };
+    minus(cxm, sxm, mt.ihx2, xm, mxm, LX);
+    plus(cxp, sxp, mt.ihx2, xp, mxp, LX);
+    minus(cym, sym, mt.ihy2, ym, mym, LY);
+    plus(cyp, syp, mt.ihy2, yp, myp, LY);
+    if (IS3D) {
+      minus(czm, szm, mt.ihz2, zm, mzm, LZ);
+      plus(czp, szp, mt.ihz2, zp, mzp, LZ);
+    }
+    // FPSI: [fr, fi] = M [dr, di], M symmetric
+    const double m2 = pr * pr + pi * pi, a2 = pr * pr - pi * pi, b2 = 2.0 * pr * pi;
+    double sc = 1.0;
+    if (mt.has_mat) sc = mt.sc_bits ? ((fl & kInsulator) ? 0.0 : 1.0) : mt.sc[q];
+    const double tau = mt.has_mat ? (1.0 - sc) * mt.inv_tau : 0.0;
+    const double M00 = sc * ((1.0 - 2.0 * m2) - a2) - tau, M11 = sc * ((1.0 - 2.0 * m2) + a2) - tau;
+    const double M01 = -sc * b2;
+    acc(A, m0, CR, M00 * mr + M01 * mi);
+    acc(A, m0, CI, M01 * mr + M11 * mi);
+    if (gsc) {  // d FPSI / d sc = (1 - |psi|^2) psi + psi / tau
+      const double t1 = 1.0 - m2;
+      gsc[q] = mr * (t1 * pr + pr * mt.inv_tau) + mi * (t1 * pi + pi * mt.inv_tau);
+    }
+
+    // ---- link equations
+    double cx, cy, cz;
+    if (mt.k2) {
+      const double k2 = mt.k2[q];
+      cx = k2 / mt.hx2;
+      cy = k2 / mt.hy2;
+      cz = IS3D ? k2 / mt.hz2 : 0.0;
+    } else {
+      cx = mt.cx;
+      cy = mt.cy;
+      cz = mt.cz;
+    }
+    // supercurrent derivative along link a with own phase (c, sn), neighbour nb = m + a:
+    //   (c U_i - s U_r) - dt (c T_r + s T_i)
+    auto dsup_t = [&](double c, double sn, double wa, const Cell<double>& nb, long long mnb, int lc) {
+      if (wa == 0.0) return;
+      const double qr = nb.pr, qi = nb.pi;
+      const double Tr = pr * qr + pi * qi, Ti = pr * qi - pi * qr;
+      const double gUi = c * wa, gUr = -sn * wa;
+      acc(A, m0, CR, gUr * qr + gUi * qi);
+      acc(A, m0, CI, gUr * qi - gUi * qr);
+      acc(A, mnb, CR, gUr * pr - gUi * pi);
+      acc(A, mnb, CI, gUr * pi + gUi * pr);
+      acc(A, m0, lc, -(c * Tr + sn * Ti) * wa);
+    };
+    // the kappa^2 part of each link equation, for d/d kappa
+    double kx = 0.0, ky = 0.0, kzv = 0.0;
+    if (mx != 0.0) {
+      // jx = 2cx (x[xm] - 2 x0 + x[xp]) + cy (x[ym] - 2 x0 + x[yp]) + cy ((-y[xp] + y0) + y[xp-y] - y[ym])
+      //    + cz (x[zm] - 2 x0 + x[zp]) + cz ((-z[xp] + z0) + z[xp-z] - z[zm]) + dsup_x
+      const double c2x = 2.0 * cx;
+      acc(A, mxm, LX, c2x * mx);
+      acc(A, mxp, LX, c2x * mx);
+      acc(A, mym, LX, cy * mx);
+      acc(A, myp, LX, cy * mx);
+      acc(A, m0, LX, (-2.0 * c2x - 2.0 * cy - (IS3D ? 2.0 * cz : 0.0)) * mx);
+      acc(A, mxp, LY, -cy * mx);
+      acc(A, m0, LY, cy * mx);
+      acc(A, mf(i + 1, j - 1, k), LY, cy * mx);
+      acc(A, mym, LY, -cy * mx);
+      const Cell<double> dxy = cell(i + 1, j - 1, k);
+      kx = c2x * (xm.x - 2.0 * c0.x + xp.x) + cy * (ym.x - 2.0 * c0.x + yp.x) +
+           cy * (((-xp.y + c0.y) + dxy.y) - ym.y);
+      if (IS3D) {
+        acc(A, mzm, LX, cz * mx);
+        acc(A, mzp, LX, cz * mx);
+        acc(A, mxp, LZ, -cz * mx);
+        acc(A, m0, LZ, cz * mx);
+        acc(A, mf(i + 1, j, k - 1), LZ, cz * mx);
+        acc(A, mzm, LZ, -cz * mx);
+        const Cell<double> dxz = cell(i + 1, j, k - 1);
+        kx += cz * (zm.x - 2.0 * c0.x + zp.x) + cz * (((-xp.z + c0.z) + dxz.z) - zm.z);
+      }
+      dsup_t(cxp, sxp, mx, xp, mxp, LX);
+    }
+    if (my != 0.0) {
+      // jy = 2cy (y[ym] - 2 y0 + y[yp]) + cx (y[xm] - 2 y0 + y[xp]) + cx ((-x[yp] + x0) + x[yp-x] - x[xm])
+      //    + cz (y[zm] - 2 y0 + y[zp]) + cz ((-z[yp] + z0) + z[yp-z] - z[zm]) + dsup_y
+      const double c2y = 2.0 * cy;
+      acc(A, mym, LY, c2y * my);
+      acc(A, myp, LY, c2y * my);
+      acc(A, mxm, LY, cx * my);
+      acc(A, mxp, LY, cx * my);
+      acc(A, m0, LY, (-2.0 * c2y - 2.0 * cx - (IS3D ? 2.0 * cz : 0.0)) * my);
+      acc(A, myp, LX, -cx * my);
+      acc(A, m0, LX, cx * my);
+      acc(A, mf(i - 1, j + 1, k), LX, cx * my);
+      acc(A, mxm, LX, -cx * my);
+      const Cell<double> dyx = cell(i - 1, j + 1, k);
+      ky = c2y * (ym.y - 2.0 * c0.y + yp.y) + cx * (xm.y - 2.0 * c0.y + xp.y) +
+           cx * (((-yp.x + c0.x) + dyx.x) - xm.x);
+      if (IS3D) {
+        acc(A, mzm, LY, cz * my);
+        acc(A, mzp, LY, cz * my);
+        acc(A, myp, LZ, -cz * my);
+        acc(A, m0, LZ, cz * my);
+        acc(A, mf(i, j + 1, k - 1), LZ, cz * my);
+        acc(A, mzm, LZ, -cz * my);
+        const Cell<double> dyz = cell(i, j + 1, k - 1);
+        ky += cz * (zm.y - 2.0 * c0.y + zp.y) + cz * (((-yp.z + c0.z) + dyz.z) - zm.z);
+      }
+      dsup_t(cyp, syp, my, yp, myp, LY);
+    }
+    if (IS3D && mz != 0.0) {
+      // jz = 2cz (z[zm] - 2 z0 + z[zp]) + cx (z[xm] - 2 z0 + z[xp]) + cy (z[ym] - 2 z0 + z[yp])
+      //    + cx ((-x[zp] + x0) + x[zp-x] - x[xm]) + cy ((-y[zp] + y0) + y[zp-y] - y[ym]) + dsup_z
+      const double c2z = 2.0 * cz;
+      acc(A, mzm, LZ, c2z * mz);
+      acc(A, mzp, LZ, c2z * mz);
+      acc(A, mxm, LZ, cx * mz);
+      acc(A, mxp, LZ, cx * mz);
+      acc(A, mym, LZ, cy * mz);
+      acc(A, myp, LZ, cy * mz);
+      acc(A, m0, LZ, (-2.0 * c2z - 2.0 * cx - 2.0 * cy) * mz);
+      acc(A, mzp, LX, -cx * mz);
+      acc(A, m0, LX, cx * mz);
+      acc(A, mf(i - 1, j, k + 1), LX, cx * mz);
+      acc(A, mxm, LX, -cx * mz);
+      acc(A, mzp, LY, -cy * mz);
+      acc(A, m0, LY, cy * mz);
+      acc(A, mf(i, j - 1, k + 1), LY, cy * mz);
+      acc(A, mym, LY, -cy * mz);
+      const Cell<double> dzx = cell(i - 1, j, k + 1), dzy = cell(i, j - 1, k + 1);
+      kzv = c2z * (zm.z - 2.0 * c0.z + zp.z) + cx * (xm.z - 2.0 * c0.z + xp.z) +
+            cy * (ym.z - 2.0 * c0.z + yp.z) + cx * (((-zp.x + c0.x) + dzx.x) - xm.x) +
+            cy * (((-zp.y + c0.y) + dzy.y) - ym.y);
+      dsup_t(czp, szp, mz, zp, mzp, LZ);
+    }
+    if (pk) {  // the link operators are proportional to kappa^2
+      const double kq = mt.k2 ? sqrt(mt.k2[q]) : kappa;
+      kap[0] += (2.0 / kq) * ((mx * kx + my * ky) + mz * kzv);
+    }
+  }
+  if (pk) publish<1, 0>(kap, pk);
+}
+
+// (J^T w)_q: the cell adjoint gathered over the cells whose values come from q.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_bc_gather(Geo g, const uint8_t* flags,
+                                                           const double* A, double* out) {
+  const long long n = g.n;
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
+       q += (long long)gridDim.x * kRedThreads) {
+    int i, j, k;
+    node_of<IS3D>(g, q, i, j, k);
+    auto F = [&](int ii, int jj, int kk) { return flags ? flags[mfull(g, ii, jj, kk)] : (uint8_t)0; };
+    auto a = [&](int ii, int jj, int kk, int c) { return A[mfull(g, ii, jj, kk) * 5 + c]; };
+    const uint8_t f0 = F(i, j, k);
+    double gr = a(i, j, k, CR), gi = a(i, j, k, CI);
+    double gx = (i != g.Nx - 1 && !(f0 & kHoleX)) ? a(i, j, k, LX) : 0.0;
+    double gy = (j != g.Ny - 1 && !(f0 & kHoleY)) ? a(i, j, k, LY) : 0.0;
+    double gz = (IS3D && k != g.Nz - 1 && !(f0 & kHoleZ)) ? a(i, j, k, LZ) : 0.0;
+    // x faces
+    for (int side = 0; side < 2; ++side) {
+      const int gi_ = side == 0 ? 0 : g.Nx;
+      const int src = g.per_x ? (side == 0 ? g.Nx - 1 : 1) : (side == 0 ? 1 : g.Nx - 1);
+      if (i != src) continue;
+      const uint8_t fg = F(gi_, j, k);
+      gr += a(gi_, j, k, CR);
+      gi += a(gi_, j, k, CI);
+      if (g.per_x) {
+        if (side == 1 && g.Nx - 1 != 1 && !(fg & kHoleX)) gx += a(gi_, j, k, LX);
+      } else {
+        if (j != g.Ny - 1 && !(fg & kHoleY)) gy += a(gi_, j, k, LY);
+        if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += a(gi_, j, k, LZ);
+      }
+    }
+    // y faces
+    for (int side = 0; side < 2; ++side) {
+      const int gj = side == 0 ? 0 : g.Ny;
+      const int src = g.per_y ? (side == 0 ? g.Ny - 1 : 1) : (side == 0 ? 1 : g.Ny - 1);
+      if (j != src) continue;
+      const uint8_t fg = F(i, gj, k);
+      gr += a(i, gj, k, CR);
+      gi += a(i, gj, k, CI);
+      if (g.per_y) {
+        if (side == 1 && g.Ny - 1 != 1 && !(fg & kHoleY)) gy += a(i, gj, k, LY);
+      } else {
+        if (i != g.Nx - 1 && !(fg & kHoleX)) gx += a(i, gj, k, LX);
+        if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += a(i, gj, k, LZ);
+      }
+    }
+    // z faces
+    if (IS3D) {
+      for (int side = 0; side < 2; ++side) {
+        const int gk = side == 0 ? 0 : g.Nz;
+        const int src = g.per_z ? (side == 0 ? g.Nz - 1 : 1) : (side == 0 ? 1 : g.Nz - 1);
+        if (k != src) continue;
+        const uint8_t fg = F(i, j, gk);
+        gr += a(i, j, gk, CR);
+        gi += a(i, j, gk, CI);
+        if (g.per_z) {
+          if (side == 1 && g.Nz - 1 != 1 && !(fg & kHoleZ)) gz += a(i, j, gk, LZ);
+        } else {
+          if (i != g.Nx - 1 && !(fg & kHoleX)) gx += a(i, j, gk, LX);
+          if (j != g.Ny - 1 && !(fg & kHoleY)) gy += a(i, j, gk, LY);
+        }
+      }
+    }
+    reinterpret_cast<double2*>(out)[q] = make_double2(gr, gi);
+    out[2 * n + q] = gx;
+    out[3 * n + q] = gy;
+    if (IS3D) out[4 * n + q] = gz;
+  }
+}
+
+// d/d(cx, cy, cz) of the face ghosts' transverse links (rhs.py:91-98, 107-112,
+// 122-127), weighted by the cell adjoint; per-block partials.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_field_grad(Geo g, const uint8_t* flags,
+                                                            const double* A, double* pout) {
+  double v[3] = {0.0, 0.0, 0.0};  // d/dcx, d/dcy, d/dcz
+  const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
+  for (long long m = (long long)blockIdx.x * kRedThreads + threadIdx.x; m < dimx;
+       m += (long long)gridDim.x * kRedThreads) {
+    const int i = (int)(m % g.mj);
+    const long long t = m / g.mj;
+    const int j = (int)(t % (g.Ny + 1));
+    const int k = IS3D ? (int)(t / (g.Ny + 1)) : 0;
+    const bool bi = (i == 0) || (i == g.Nx);
+    const bool bj = (j == 0) || (j == g.Ny);
+    const bool bk = IS3D && ((k == 0) || (k == g.Nz));
+    if ((int)bi + (int)bj + (int)bk != 1) continue;
+    const uint8_t fg = flags ? flags[m] : (uint8_t)0;
+    const double ax = (fg & kHoleX) ? 0.0 : A[m * 5 + LX];
+    const double ay = (fg & kHoleY) ? 0.0 : A[m * 5 + LY];
+    const double az = (IS3D && !(fg & kHoleZ)) ? A[m * 5 + LZ] : 0.0;
+    if (bi && !g.per_x) {
+      v[2] += (i == 0 ? -1.0 : 1.0) * ay;
+      if (IS3D) v[1] += (i == 0 ? 1.0 : -1.0) * az;
+    } else if (bj && !g.per_y) {
+      v[2] += (j == 0 ? 1.0 : -1.0) * ax;
+      if (IS3D) v[0] += (j == 0 ? -1.0 : 1.0) * az;
+    } else if (bk && !g.per_z) {
+      v[1] += (k == 0 ? -1.0 : 1.0) * ax;
+      v[0] += (k == 0 ? 1.0 : -1.0) * ay;
+    }
+  }
+  publish<3, 0>(v, pout);
+}
+
+// Cell adjoint of B_z = (((x - x[+y]) - y) + y[+x]) / (hx hy) at every
+// interior node (physics/bfield.py:58-68), weights in the reference order.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const double* gbz, double* A) {
+  const long long n = g.n;
+  const double s = 1.0 / (b.hx * b.hy);
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
+       q += (long long)gridDim.x * kRedThreads) {
+    int i, j, k;
+    node_of<IS3D>(g, q, i, j, k);
+    // reference order r = ((i-1) ny + (j-1)) nz + (k-1)
+    const long long r = ((long long)(i - 1) * g.ny + (j - 1)) * g.nz + (IS3D ? k - 1 : 0);
+    const double v = s * gbz[r];
+    acc(A, mfull(g, i, j, k), LX, v);
+    acc(A, mfull(g, i, j + 1, k), LX, -v);
+    acc(A, mfull(g, i, j, k), LY, -v);
+    acc(A, mfull(g, i + 1, j, k), LY, v);
+  }
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+
+void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
+                      double* A, double* gsc, double kappa, double* pk, hipStream_t st) {
+  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
+  if (g.is3d) hipLaunchKernelGGL(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
+  else hipLaunchKernelGGL(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
+}
+void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st) {
+  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
+  if (g.is3d) hipLaunchKernelGGL(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out);
+  else hipLaunchKernelGGL(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out);
+}
+int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st) {
+  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
+  if (g.is3d) hipLaunchKernelGGL(k_field_grad<true>, gr, bl, 0, st, g, flags, A, pout);
+  else hipLaunchKernelGGL(k_field_grad<false>, gr, bl, 0, st, g, flags, A, pout);
+  return stream_blocks(g);
+}
+void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st) {
+  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
+  if (g.is3d) hipLaunchKernelGGL(k_bz_cells<true>, gr, bl, 0, st, g, b, gbz, A);
+  else hipLaunchKernelGGL(k_bz_cells<false>, gr, bl, 0, st, g, b, gbz, A);
+}
+
+}  // namespace tdgl

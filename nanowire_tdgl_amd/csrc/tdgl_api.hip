@@ -125,6 +125,7 @@ int ensure_vec(tdgl_ctx* ctx, int i) {
 }
 
 void set_uniform_coeffs(tdgl_ctx* ctx, double kappa) {
+  ctx->kappa_uniform = kappa;
   Mat& m = ctx->mt;
   const double k2 = kappa * kappa;  // sparse_operators.py:131 (kappa_m**2)
   m.cx = k2 / m.hx2;
@@ -227,6 +228,7 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
   for (double* p : ctx->P) hipFree(p);
   for (double* p : ctx->AP) hipFree(p);
   if (ctx->own_batch) tdgl_batch_destroy(ctx->own_batch);
+  if (ctx->d_acell) hipFree(ctx->d_acell);
   ctx->prof.destroy();
   if (ctx->h_res) hipHostFree(ctx->h_res);
   delete ctx;
@@ -437,6 +439,75 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const dou
 int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
   if (!ctx || !x32 || !p32 || !ap32) return TDGL_E_INVALID;
   return jvp32_single(ctx, x32, p32, dt, ap32);
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_acell(tdgl_ctx* ctx) {
+  const Geo& g = ctx->g;
+  const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
+  if (!ctx->d_acell) {
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_acell, sizeof(double) * 5 * (size_t)dimx))) return rc;
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_acell, 0, sizeof(double) * 5 * (size_t)dimx, ctx->st));
+  return TDGL_OK;
+}
+
+// d<w, .>/d(Bx, By, Bz) from the cell adjoint (uniform applied field only).
+int field_grad(tdgl_ctx* ctx, double* dfield) {
+  if (ctx->b.bx || ctx->b.by || ctx->b.bz)
+    return fail(ctx, TDGL_E_INVALID, "field gradients need a uniform applied field (tdgl_set_field)");
+  red_commit(ctx, launch_field_grad(ctx->g, ctx->mt.flags, ctx->d_acell, red_next(ctx).pout, ctx->st));
+  red_finalize(ctx, 3, 0, RS_BETA);
+  LAUNCHCHK();
+  int rc;
+  if ((rc = readback(ctx, RS_BETA, 3))) return rc;
+  const Bnd& b = ctx->b;
+  // cx = (Bx hy) hz, cy = (By hz) hx, cz = (Bz hx) hy
+  dfield[0] = ctx->h_res[RS_BETA] * (b.hy * b.hz);
+  dfield[1] = ctx->h_res[RS_BETA + 1] * (b.hz * b.hx);
+  dfield[2] = ctx->h_res[RS_BETA + 2] * (b.hx * b.hy);
+  return TDGL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdgl_vjp_f(tdgl_ctx* ctx, const double* x, const double* w, double* gx, double* dfield,
+               double* dkappa, double* dsc) {
+  if (!ctx || !x || !w || !gx) return TDGL_E_INVALID;
+  int rc;
+  if ((rc = ensure_acell(ctx))) return rc;
+  Red r = red_next(ctx);
+  launch_vjp_cells(ctx->g, ctx->b, ctx->mt, x, w, ctx->d_acell, dsc, ctx->kappa_uniform,
+                   dkappa ? r.pout : nullptr, ctx->st);
+  LAUNCHCHK();
+  if (dkappa) {
+    red_commit(ctx, stream_blocks(ctx->g));
+    red_finalize(ctx, 1, 0, RS_DOT);
+    LAUNCHCHK();
+    if ((rc = readback(ctx, RS_DOT, 1))) return rc;
+    *dkappa = ctx->h_res[RS_DOT];
+  }
+  launch_bc_gather(ctx->g, ctx->mt.flags, ctx->d_acell, gx, ctx->st);
+  LAUNCHCHK();
+  if (dfield && (rc = field_grad(ctx, dfield))) return rc;
+  return TDGL_OK;
+}
+
+int tdgl_vjp_bz(tdgl_ctx* ctx, const double* gbz, double* gx, double* dfield) {
+  if (!ctx || !gbz || !gx) return TDGL_E_INVALID;
+  int rc;
+  if ((rc = ensure_acell(ctx))) return rc;
+  launch_bz_cells(ctx->g, ctx->b, gbz, ctx->d_acell, ctx->st);
+  launch_bc_gather(ctx->g, ctx->mt.flags, ctx->d_acell, gx, ctx->st);
+  LAUNCHCHK();
+  if (dfield && (rc = field_grad(ctx, dfield))) return rc;
+  return TDGL_OK;
 }
 
 int tdgl_dot(tdgl_ctx* ctx, const double* a, const double* b, double* out) {

@@ -117,6 +117,8 @@ struct tdgl_ctx {
   // mixed precision: this film as a batch of one (created on first use)
   tdgl_batch* own_batch = nullptr;
   int kz_user = 0;    // tdgl_set_stencil_kz (0: the built-in choice)
+  double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
+  double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
   int pb_blocks = 0;  // capacity of d_pb in blocks
 
   tdgl::ProfSink prof;  // tdgl_profile_enable

@@ -788,6 +788,8 @@ void launch_finalize(int nsum, int nmax, int slot0, const Red& r, hipStream_t st
     hipLaunchKernelGGL((k_finalize<1, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
   else if (nsum == 0 && nmax == 1)
     hipLaunchKernelGGL((k_finalize<0, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+  else if (nsum == 3 && nmax == 0)
+    hipLaunchKernelGGL((k_finalize<3, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
   else
     hipLaunchKernelGGL((k_finalize<2, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
 }

@@ -131,6 +131,12 @@ void launch_btinit(const BatchK& bk, int* fstat, const int* maxk, int* iter, hip
 void launch_btrisolve(const BatchK& bk, const int* fstat, const double* H, hipStream_t st);
 void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* Q, const int* fstat,
                    double* pm, hipStream_t st);
+// Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.
+void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
+                      double* A, double* gsc, double kappa, double* pk, hipStream_t st);
+void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
+int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
+void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
 void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st);
 void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,
