@@ -22,7 +22,11 @@ Secondary sections of the same line (each with its own roofline):
   large   -- 256x256x64 film with 1% insulating spheres (kappa 3, Bz 0.5,
              TGCR tol 1e-4): FP64 vs mixed steps and their error;
   batched -- 64 uneven films (seed 3) in lockstep mixed-precision attempts,
-             against looping the plain FP64 step over the same films.
+             against looping the plain FP64 step over the same films;
+  adjoint -- 96x96x12 film (kappa 2.5, Bz 0.3, mask w = 0.05 U(0,1) seed 4):
+             forward FP64 steps with checkpoints, the hand-written adjoint of
+             L = mean_midplane (B_z - T)^2, seconds per step of each, and
+             central differences (eps = 1e-6) for kappa and Bz.
 """
 
 from __future__ import annotations
@@ -282,6 +286,53 @@ def cpu_baseline(step_stats: dict, n_interior: int, budget: float) -> dict:
             "sec_per_step": sec_per_step}
 
 
+def section_adjoint(steps, every, fd=True):
+    import torch
+    import nanowire_tdgl_amd as T
+    from nanowire_tdgl_amd.adjoint import AdjointProblem, TrapezoidalBz, midplane_target_loss
+    p = T.SimulationParameters(Nx=96, Ny=96, Nz=12, hx=0.5, hy=0.5, hz=0.5, kappa=2.5)
+    n = p.n_interior
+    rng = np.random.default_rng(4)
+    x0 = np.concatenate([1.0 + 0.01 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                         np.zeros(3 * n, complex)])
+    w0 = 0.05 * np.random.default_rng(4).uniform(size=p.dim_x)
+    prob = AdjointProblem(p, x0, dt=0.2, steps=steps, checkpoint_every=every)
+    kappa = torch.tensor(2.5, dtype=torch.float64, requires_grad=True)
+    bz = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    loss = midplane_target_loss(prob, TrapezoidalBz.apply(prob, kappa, bz, w))
+    torch.cuda.synchronize()
+    c1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    c2 = time.perf_counter()
+    out = {"workload": "96x96x12 film, h=0.5, kappa=2.5, Bz=0.3 (t_on_fraction 1), w = 0.05 U(0,1) "
+                       f"(seed 4), psi = 1 + 0.01 noise (seed 4); {steps} FP64 steps dt=0.2, "
+                       f"checkpoints every {every}; L = mean over k=6 of (B_z - T)^2, T = 0.2 "
+                       "exp(-r^2/(2 15^2))",
+           "n_interior": n, "steps": steps, "loss": float(loss),
+           "forward_s_per_step": (c1 - c0) / steps, "adjoint_s_per_step": (c2 - c1) / steps,
+           "grad": {"kappa": kappa.grad.item(), "Bz": bz.grad.item(),
+                    "w_norm": float(w.grad.norm())},
+           "newton_iters": prob.stats.get("newton"), "gmres_iters": prob.stats.get("adjoint_gmres_iters")}
+    log(f"[bench] adjoint: forward {out['forward_s_per_step']:.3f} s/step, adjoint "
+        f"{out['adjoint_s_per_step']:.3f} s/step, grads {out['grad']}")
+    if fd:
+        def L(k, b):
+            _, bzo = prob.forward(k, b, w0)
+            return float(midplane_target_loss(prob, bzo))
+        eps = 1e-6
+        fdk = (L(2.5 + eps, 0.3) - L(2.5 - eps, 0.3)) / (2 * eps)
+        fdb = (L(2.5, 0.3 + eps) - L(2.5, 0.3 - eps)) / (2 * eps)
+        out["fd"] = {"eps": eps, "kappa": fdk, "Bz": fdb,
+                     "rel_diff_kappa": abs(fdk - out["grad"]["kappa"]) / abs(fdk),
+                     "rel_diff_Bz": abs(fdb - out["grad"]["Bz"]) / abs(fdb)}
+        log(f"[bench] adjoint vs central differences: {out['fd']}")
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,7 +343,8 @@ def main() -> None:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--large-steps", type=int, default=2)
     ap.add_argument("--batch-attempts", type=int, default=3)
-    ap.add_argument("--sections", default="mixed,large,batched",
+    ap.add_argument("--adjoint-steps", type=int, default=40)
+    ap.add_argument("--sections", default="mixed,large,batched,adjoint",
                     help="comma list of secondary sections to run (empty: headline only)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -320,6 +372,8 @@ def main() -> None:
         extra["large"] = section_large(args.large_steps, 1)
     if "batched" in sections:
         extra["batched"] = section_batched(args.batch_attempts, 1, args.batch_attempts)
+    if "adjoint" in sections:
+        extra["adjoint"] = section_adjoint(args.adjoint_steps, 20)
 
     first = s.log[0]
     cpu = None
