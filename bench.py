@@ -45,6 +45,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# Off with --no-profile: no second, event-instrumented pass in any section.
+PROFILE = True
 METRIC = "cell-steps/s mixed vs FP64 at 1e-6 rel err; achieved HBM GB/s; adjoint s/step"
 
 
@@ -67,7 +69,8 @@ def build_film():
 class Stepper:
     """The adaptive trapezoidal loop of the integrator, without history."""
 
-    def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8, precision="f64"):
+    def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8, precision="f64",
+                 precond="chebyshev"):
         from nanowire_tdgl_amd import _native
         from nanowire_tdgl_amd.engine import engine_for
         self.dev = dev
@@ -75,7 +78,7 @@ class Stepper:
         self.ctx = self.eng.ctx
         self.X = self.ctx.to_internal(x0)
         self.dt, self.cur, self.t, self.t_stop = dt, dt, 0.0, t_stop
-        self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision)
+        self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision, precond=precond)
         self.log = []
 
     def accepted_steps(self, k: int) -> None:
@@ -117,6 +120,19 @@ def rel_errors(a, b) -> dict:
     return out
 
 
+def pmc_traffic(profile: str) -> dict:
+    """HBM bytes per launch per kernel class from a committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size);
+    empty if the summary is absent."""
+    path = ROOT / "profiles" / profile
+    if not path.exists():
+        return {}
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic-json:"):
+            return json.loads(line.split(":", 1)[1])
+    return {}
+
+
 def roofline_of(per_class: dict) -> dict:
     dom = max(per_class, key=lambda k: per_class[k]["ms"])
     c = per_class[dom]
@@ -132,11 +148,11 @@ def roofline_of(per_class: dict) -> dict:
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bpl}
 
 
-def timed_run(dev, x0, precision, steps, warmup, tol, profile=True):
+def timed_run(dev, x0, precision, steps, warmup, tol, profile=None, precond="chebyshev"):
     """Warm up, time `steps` accepted steps, then (optionally) re-run them
     instrumented from the same state.  Returns (Stepper, seconds, per_class, roofline)."""
     import torch
-    s = Stepper(dev, x0, tol_gcr=tol, precision=precision)
+    s = Stepper(dev, x0, tol_gcr=tol, precision=precision, precond=precond)
     s.accepted_steps(warmup)
     torch.cuda.synchronize()
     snap = s.snapshot()
@@ -146,7 +162,7 @@ def timed_run(dev, x0, precision, steps, warmup, tol, profile=True):
     el = time.perf_counter() - t0
     final = s.snapshot()
     per_class = roof = None
-    if profile:
+    if PROFILE if profile is None else profile:
         s.restore(snap)
         s.ctx.profile_enable(True)
         s.accepted_steps(steps)
@@ -158,16 +174,30 @@ def timed_run(dev, x0, precision, steps, warmup, tol, profile=True):
     return s, el, per_class, roof
 
 
+def per_step(s, key, steps):
+    return sum(r[key] for r in s.timed) / steps
+
+
 def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
+    """Mixed precision with the Chebyshev-preconditioned TGCR (the default),
+    and without the preconditioner, on the headline film and steps."""
     n = dev.params.n_interior
     s, el, pc, roof = timed_run(dev, x0, "mixed", steps, warmup, tol)
     v, v64 = n * steps / el, n * steps / fp64_el
     err = rel_errors(s.observables(), fp64_stepper.observables())
     log(f"[bench] mixed: {v:.4g} cell-steps/s ({v / v64:.2f}x FP64), err {err}")
+    u, elu, _, _ = timed_run(dev, x0, "mixed", steps, warmup, tol, profile=False, precond="none")
+    vu = n * steps / elu
+    erru = rel_errors(u.observables(), fp64_stepper.observables())
+    log(f"[bench] mixed, no preconditioner: {vu:.4g} cell-steps/s ({vu / v64:.2f}x FP64), err {erru}")
     return {"value": v, "unit": "cell-steps/s", "ms_per_step": el / steps * 1e3,
-            "speedup_vs_fp64": v / v64,
+            "speedup_vs_fp64": v / v64, "precond": "chebyshev (automatic degree)",
             "rel_err_vs_fp64": err, "roofline": roof, "kernel_classes": pc,
-            "tgcr_iters_per_step": sum(r["tgcr_iters_total"] for r in s.timed) / steps}
+            "tgcr_iters_per_step": per_step(s, "tgcr_iters_total", steps),
+            "stencil_sweeps_per_step": per_step(s, "jvp_evals", steps),
+            "newton_iters_per_step": per_step(s, "newton_iters", steps),
+            "unpreconditioned": {"value": vu, "speedup_vs_fp64": vu / v64, "rel_err_vs_fp64": erru,
+                                 "tgcr_iters_per_step": per_step(u, "tgcr_iters_total", steps)}}
 
 
 def section_large(steps, warmup):
@@ -180,16 +210,21 @@ def section_large(steps, warmup):
     tol = 1e-4
     a, ela, _, roofa = timed_run(dev, x0, "f64", steps, warmup, tol, profile=False)
     b, elb, pcb, roofb = timed_run(dev, x0, "mixed", steps, warmup, tol)
-    va, vb = n * steps / ela, n * steps / elb
+    c, elc, _, _ = timed_run(dev, x0, "mixed", steps, warmup, tol, profile=False, precond="none")
+    va, vb, vc = n * steps / ela, n * steps / elb, n * steps / elc
     err = rel_errors(b.observables(), a.observables())
-    log(f"[bench] large: FP64 {va:.4g}, mixed {vb:.4g} cell-steps/s ({vb / va:.2f}x), err {err}")
+    log(f"[bench] large: FP64 {va:.4g}, mixed {vb:.4g} cell-steps/s ({vb / va:.2f}x), "
+        f"mixed without preconditioner {vc:.4g} ({vc / va:.2f}x), err {err}")
     return {"workload": "256x256x64 film, h=0.5, kappa=3, Bz=0.5, 1% insulating spheres r in [2,4] "
                         "(seed 2), psi = (1 + 1e-3 noise) * sc_mask, dt=0.2, TGCR tol 1e-4",
             "n_interior": n, "steps": steps, "fp64_cell_steps_s": va, "mixed_cell_steps_s": vb,
             "speedup": vb / va, "rel_err_vs_fp64": err, "roofline_mixed": roofb,
             "kernel_classes_mixed": pcb,
-            "tgcr_iters_per_step": {"f64": sum(r["tgcr_iters_total"] for r in a.timed) / steps,
-                                    "mixed": sum(r["tgcr_iters_total"] for r in b.timed) / steps}}
+            "mixed_unpreconditioned_cell_steps_s": vc, "speedup_unpreconditioned": vc / va,
+            "tgcr_iters_per_step": {"f64": per_step(a, "tgcr_iters_total", steps),
+                                    "mixed": per_step(b, "tgcr_iters_total", steps),
+                                    "mixed_unpreconditioned": per_step(c, "tgcr_iters_total", steps)},
+            "stencil_sweeps_per_step_mixed": per_step(b, "jvp_evals", steps)}
 
 
 def section_batched(attempts, warmup, fp64_steps):
@@ -205,9 +240,9 @@ def section_batched(attempts, warmup, fp64_steps):
     opts = _native.solver_opts(tol_gcr=tol, precision="mixed")
     t = np.zeros(len(films))
 
-    def attempt():
+    def attempt(o=opts):
         runner.set_fields(t, 300 * dt)
-        st = runner.batch.trap_step(X, dt, opts)
+        st = runner.batch.trap_step(X, dt, o)
         ok = np.array([s.converged for s in st], dtype=bool)
         t[ok] += dt
         return st, ok
@@ -224,16 +259,41 @@ def section_batched(attempts, warmup, fp64_steps):
     torch.cuda.synchronize()
     el = time.perf_counter() - c0
     v = acc / el
-    # instrumented repeat from the same state
+    pc = roof = None
+    if PROFILE:  # instrumented repeat from the same state
+        X.copy_(X0)
+        t[:] = t0v
+        runner.batch.profile_enable(True)
+        for _ in range(attempts):
+            attempt()
+        pc = runner.batch.profile_read()
+        runner.batch.profile_enable(False)
+        roof = roofline_of(pc)
+    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films ({el:.2f} s)")
+    # the same attempts without the preconditioner
+    Xp, tp = X.clone(), t.copy()
     X.copy_(X0)
     t[:] = t0v
-    runner.batch.profile_enable(True)
+    opts_none = _native.solver_opts(tol_gcr=tol, precision="mixed", precond="none")
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    accu = 0.0
     for _ in range(attempts):
-        attempt()
-    pc = runner.batch.profile_read()
-    runner.batch.profile_enable(False)
-    roof = roofline_of(pc)
-    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films ({el:.2f} s)")
+        st, ok = attempt(opts_none)
+        accu += float(ns[ok].sum())
+    torch.cuda.synchronize()
+    vu = accu / (time.perf_counter() - c0)
+    log(f"[bench] batched mixed, no preconditioner: {vu:.4g} cell-steps/s")
+    X.copy_(Xp)
+    t[:] = tp
+    out = {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
+                       "Bz~U[0.05,0.8], holes on even films; dt=0.2, TGCR tol 1e-4",
+           "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
+           "mixed_batched_cell_steps_s": v, "mixed_batched_unpreconditioned_cell_steps_s": vu,
+           "roofline": roof, "kernel_classes": pc,
+           "autotune": {"kz_policy_ms": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
+    if fp64_steps <= 0:
+        return out
     # baseline: loop the plain FP64 step over the films, same steps from the same start
     steppers = [Stepper(f.device, f.x0, tol_gcr=tol) for f in films]
     for s in steppers:
@@ -258,13 +318,10 @@ def section_batched(attempts, warmup, fp64_steps):
         e = rel_errors((np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()), s.observables())
         worst = {k: max(worst[k], e[k]) for k in worst}
     log(f"[bench] batched vs FP64 loop, worst over films: {worst}")
-    return {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
-                        "Bz~U[0.05,0.8], holes on even films; dt=0.2, TGCR tol 1e-4",
-            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
-            "mixed_batched_cell_steps_s": v, "fp64_loop_cell_steps_s": v64,
-            "speedup_vs_fp64_loop": v / v64, "rel_err_vs_fp64_worst_film": worst,
-            "roofline": roof, "kernel_classes": pc,
-            "autotune": {"kz_policy_ms": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
+    out.update({"fp64_loop_cell_steps_s": v64, "speedup_vs_fp64_loop": v / v64,
+                "speedup_unpreconditioned_vs_fp64_loop": vu / v64,
+                "rel_err_vs_fp64_worst_film": worst})
+    return out
 
 
 def cpu_baseline(step_stats: dict, n_interior: int, budget: float) -> dict:
@@ -333,8 +390,24 @@ def section_adjoint(steps, every, fd=True):
     return out
 
 
+def calibrate_copy(mib: int, reps: int) -> None:
+    """Copy a resident `mib` MiB buffer `reps` times (one copy kernel per
+    rep): the reference point that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes (each copy reads and writes exactly mib MiB)."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_copy": mib * (1 << 20),
+                      "bytes_written_per_copy": mib * (1 << 20), "copies": reps}), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel used to calibrate the PMC byte counters")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -343,10 +416,18 @@ def main() -> None:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--large-steps", type=int, default=2)
     ap.add_argument("--batch-attempts", type=int, default=3)
+    ap.add_argument("--batch-fp64-steps", type=int, default=-1,
+                    help="steps of the FP64 loop baseline in the batched section "
+                         "(default: --batch-attempts; 0 skips the baseline)")
     ap.add_argument("--adjoint-steps", type=int, default=40)
     ap.add_argument("--sections", default="mixed,large,batched,adjoint",
                     help="comma list of secondary sections to run (empty: headline only)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, 5)
+        return
+    global PROFILE
+    PROFILE = not args.no_profile
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU split of "
               "this path is a design note, not a measured run", file=sys.stderr)
@@ -356,14 +437,18 @@ def main() -> None:
     dev, x0 = build_film()
     n = dev.params.n_interior
     log(f"[bench] film {dev.params.Nx}x{dev.params.Ny}x{dev.params.Nz}, n_interior={n}")
-    s, el, per_class, roof = timed_run(dev, x0, "f64", args.steps, args.warmup, 1e-8,
-                                       profile=not args.no_profile)
+    s, el, per_class, roof = timed_run(dev, x0, "f64", args.steps, args.warmup, 1e-8)
     timed = s.timed
     value = n * args.steps / el
     log(f"[bench] {el:.3f} s for {args.steps} steps ({len(timed)} attempts): {value:.4g} cell-steps/s")
     if roof:
+        tr = pmc_traffic("pmc_traffic_film128_hole.md").get(roof["kernel"])
+        if tr:
+            roof["traffic"] = tr
+            roof["traffic_source"] = ("profiles/pmc_traffic_film128_hole.md (rocprofv3 FETCH_SIZE x 2.0 + "
+                                      "WRITE_SIZE, calibrated on a 1 GiB copy)")
         log(f"[bench] dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
-            f"{roof['achieved']:.0f} GB/s")
+            f"{roof['achieved']:.0f} GB/s, traffic {roof['traffic']}")
     sections = [x for x in args.sections.split(",") if x]
     extra = {}
     if "mixed" in sections:
@@ -371,7 +456,8 @@ def main() -> None:
     if "large" in sections:
         extra["large"] = section_large(args.large_steps, 1)
     if "batched" in sections:
-        extra["batched"] = section_batched(args.batch_attempts, 1, args.batch_attempts)
+        fp64_steps = args.batch_attempts if args.batch_fp64_steps < 0 else args.batch_fp64_steps
+        extra["batched"] = section_batched(args.batch_attempts, 1, fp64_steps)
     if "adjoint" in sections:
         extra["adjoint"] = section_adjoint(args.adjoint_steps, 20)
 

@@ -77,6 +77,16 @@ typedef struct tdgl_solver_opts {
    * relative residuals much below 1e-7; the FP64 Newton residual still
    * decides convergence (inexact Newton). */
   double mixed_tol_floor;
+  /* TDGL_PREC_MIXED only (no counterpart in the reference): right
+   * preconditioner of TGCR, a Chebyshev polynomial M = p(A) of A = I - dt/2 J
+   * on the interval [1 - dt/2, 1 + dt/2 ||J||_inf] (||J||_inf bounded from the
+   * stencil coefficients with |psi| <= 2).  0: none; > 0: polynomial degree
+   * (minus one) = this many stencil sweeps per TGCR iteration; < 0: the
+   * degree that contracts the spectrum to within 0.25 of 1.  TGCR's stop test
+   * stays on the true residual of (I - dt/2 J) dx = -g; the Newton residual
+   * is unchanged.  Off when dt/2 >= 0.9 (the interval would reach 0). */
+  int32_t mixed_precond;
+  int32_t mixed_reserved;
 } tdgl_solver_opts;
 
 /* Per-step counters (the reference keeps none; they make iteration-count
@@ -157,6 +167,18 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x_dev, const double* f0_dev,
  * finite difference of tgcr.py:129-133 (its eps -> 0 limit). */
 int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32_dev, const float* p32_dev, double dt,
                     float* ap32_dev);
+
+/* One application of the mixed TGCR's Chebyshev preconditioner (no
+ * counterpart in the reference; see tdgl_solver_opts.mixed_precond), FP32,
+ * internal layout, at the linearisation point x32 with A = I - dt/2 J(x32):
+ *   w32 = A M v = (I - P(A)) v,  z32 = M v,
+ * P(t) = T_d((theta - t)/delta) / T_d(sigma) on the interval [a, b] the
+ * library picks (theta = (a + b)/2, delta = (b - a)/2, sigma = theta/delta).
+ * degree: > 0 the polynomial degree d, < 0 the automatic choice.
+ * info (host double[4]) receives a, b, d, and 1.0 when d = 1 (z32 is then
+ * not written: z = v / theta). */
+int tdgl_precond_cheb32(tdgl_ctx* ctx, const float* x32_dev, const float* v32_dev, double dt,
+                        int degree, float* w32_dev, float* z32_dev, double* info);
 
 /* Adjoint of eval_f (no counterpart in the reference; the hand-written
  * transpose of the Jacobian of rhs.py:147-239 at x, at the current field and
@@ -253,7 +275,8 @@ int tdgl_batch_profile_read(tdgl_batch* batch, int kernel_class, double* ms_tota
 #define TDGL_K_UPDATE_R 6  /* r -= alpha Ap + ||r|| (mixed: stop tests) */
 #define TDGL_K_ACCUM 7     /* dx = sum alpha p; x += dx; ||dx||_inf (mixed: H y = alpha, x += R y) */
 #define TDGL_K_CGS2 8      /* mixed only: the conditional second CGS pass */
-#define TDGL_K_COUNT 9
+#define TDGL_K_CHEB 9      /* mixed only: Chebyshev preconditioner sweeps (stencil) */
+#define TDGL_K_COUNT 10
 int tdgl_profile_enable(tdgl_ctx* ctx, int enable);
 int tdgl_profile_read(tdgl_ctx* ctx, int kernel_class, double* ms_total, int64_t* launches,
                       double* bytes_total);

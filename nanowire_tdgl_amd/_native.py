@@ -10,6 +10,8 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
+import numpy as np
+
 LIB_PATH = Path(__file__).resolve().parent / "libtdglhip.so"
 
 TDGL_OK = 0
@@ -27,7 +29,7 @@ EXPORTS = [
     "tdgl_set_stencil_kz", "tdgl_get_stencil_kz",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
-    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
@@ -36,7 +38,7 @@ EXPORTS = [
 ]
 
 KERNEL_CLASSES = ["pred", "newton", "jvp", "mgs", "mgs_last", "final_p", "update_r", "accum",
-                  "cgs2"]
+                  "cgs2", "cheb"]
 
 
 class Grid(C.Structure):
@@ -50,7 +52,8 @@ class SolverOpts(C.Structure):
     _fields_ = [("newton_tol_f", C.c_double), ("newton_tol_dx", C.c_double),
                 ("newton_max_iter", C.c_int32), ("precision", C.c_int32),
                 ("tol_gcr", C.c_double), ("eps_mf", C.c_double),
-                ("tgcr_max_iter", C.c_int64), ("mixed_tol_floor", C.c_double)]
+                ("tgcr_max_iter", C.c_int64), ("mixed_tol_floor", C.c_double),
+                ("mixed_precond", C.c_int32), ("mixed_reserved", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -101,6 +104,7 @@ def load() -> C.CDLL:
         "tdgl_profile_enable": ([P, C.c_int], C.c_int),
         "tdgl_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
         "tdgl_jvp_trap32": ([P, P, P, D, P], C.c_int),
+        "tdgl_precond_cheb32": ([P, P, P, D, C.c_int, P, P, P], C.c_int),
         "tdgl_set_stencil_kz": ([P, C.c_int], C.c_int),
         "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
         "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
@@ -256,6 +260,19 @@ class Context:
                 raise ValueError("jvp_trap32 takes float32 vectors of the internal state length")
         self._call("tdgl_jvp_trap32", C.c_void_p(x32.data_ptr()), C.c_void_p(p32.data_ptr()),
                    C.c_double(dt), C.c_void_p(ap32.data_ptr()))
+
+    def precond_cheb32(self, x32, v32, dt: float, degree: int, w32, z32) -> dict:
+        """One application of the mixed TGCR's Chebyshev preconditioner:
+        w32 = A M v, z32 = M v (A = I - dt/2 J(x32)).  Returns the interval
+        (a, b), the degree d, and whether z32 was left unwritten (d = 1: z = v/theta)."""
+        for v in (x32, v32, w32, z32):
+            if v.dtype != self.torch.float32 or v.numel() != self.len:
+                raise ValueError("precond_cheb32 takes float32 vectors of the internal state length")
+        info = (C.c_double * 4)()
+        self._call("tdgl_precond_cheb32", C.c_void_p(x32.data_ptr()), C.c_void_p(v32.data_ptr()),
+                   C.c_double(dt), C.c_int(int(degree)), C.c_void_p(w32.data_ptr()),
+                   C.c_void_p(z32.data_ptr()), C.cast(info, C.c_void_p))
+        return {"a": info[0], "b": info[1], "degree": int(info[2]), "z_is_v_over_theta": info[3] == 1.0}
 
     def vjp_f(self, x, w, gx, field=False, kappa=False, dsc=None):
         """gx = J(x)^T w; returns (d/d(Bx,By,Bz) or None, d/dkappa or None)."""
@@ -424,13 +441,26 @@ class Batch:
         return out
 
 
+def precond_code(precond) -> int:
+    """Map the mixed path's `precond` option to tdgl_solver_opts.mixed_precond:
+    "chebyshev" (automatic degree) -> -1, "none" / None -> 0, an int d >= 1 ->
+    a Chebyshev preconditioner with d stencil sweeps per TGCR iteration."""
+    if precond is None or precond == "none":
+        return 0
+    if precond == "chebyshev":
+        return -1
+    if isinstance(precond, (int, np.integer)) and not isinstance(precond, bool) and precond >= 1:
+        return int(precond)
+    raise ValueError(f"precond must be 'chebyshev', 'none' or a degree >= 1, not {precond!r}")
+
+
 def solver_opts(newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,
                 eps_mf=1e-4, tgcr_max_iter=None, precision=PREC_F64,
-                mixed_tol_floor=1e-6) -> SolverOpts:
+                mixed_tol_floor=1e-6, precond="chebyshev") -> SolverOpts:
     if isinstance(precision, str):
         precision = {"f64": PREC_F64, "fp64": PREC_F64, "mixed": PREC_MIXED}[precision]
     return SolverOpts(float(newton_tol_f), float(newton_tol_dx), int(newton_max_iter),
                       int(precision), float(tol_gcr), float(eps_mf),
                       int(tgcr_max_iter) if tgcr_max_iter is not None else 0,
-                      float(mixed_tol_floor))
+                      float(mixed_tol_floor), precond_code(precond), 0)
 

@@ -128,6 +128,7 @@ void set_uniform_coeffs(tdgl_ctx* ctx, double kappa) {
   ctx->kappa_uniform = kappa;
   Mat& m = ctx->mt;
   const double k2 = kappa * kappa;  // sparse_operators.py:131 (kappa_m**2)
+  ctx->k2max = k2;
   m.cx = k2 / m.hx2;
   m.cy = k2 / m.hy2;
   m.cz = ctx->g.is3d ? k2 / m.hz2 : 0.0;
@@ -314,6 +315,7 @@ int tdgl_set_material(tdgl_ctx* ctx, const uint8_t* link_bits_full, const double
     } else {
       std::vector<double> k2(n);
       for (long long r = 0; r < n; ++r) k2[int_of_ref(r)] = kappa_int[r] * kappa_int[r];
+      ctx->k2max = *std::max_element(k2.begin(), k2.end());
       int rc;
       if ((rc = alloc(ctx, (void**)&ctx->d_k2, sizeof(double) * n))) return rc;
       HIPCHK(hipMemcpy(ctx->d_k2, k2.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -439,6 +441,13 @@ int tdgl_jvp_trap_fd(tdgl_ctx* ctx, const double* x, const double* f0, const dou
 int tdgl_jvp_trap32(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
   if (!ctx || !x32 || !p32 || !ap32) return TDGL_E_INVALID;
   return jvp32_single(ctx, x32, p32, dt, ap32);
+}
+
+int tdgl_precond_cheb32(tdgl_ctx* ctx, const float* x32, const float* v32, double dt, int degree,
+                        float* w32, float* z32, double* info) {
+  if (!ctx || !x32 || !v32 || !w32 || !z32 || !info || !(dt > 0.0)) return TDGL_E_INVALID;
+  if (degree > kMaxChebDeg) return fail(ctx, TDGL_E_INVALID, "Chebyshev degree above 64");
+  return precond_single(ctx, x32, v32, dt, degree, w32, z32, info);
 }
 
 }  // extern "C"

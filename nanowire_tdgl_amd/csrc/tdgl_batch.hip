@@ -64,6 +64,9 @@ struct tdgl_batch {
   float* x32 = nullptr;
   float* r32 = nullptr;
   float* r0_32 = nullptr;
+  float* t32[2] = {nullptr, nullptr};  // Chebyshev recurrence vectors
+  double* d_cheb = nullptr;            // nfilm * kChebStride: 1/theta, omega_1 .. omega_d
+  int cheb_deg = 0;                    // this step's preconditioner degree (0: none)
   std::vector<float*> Q;
   float** d_qtab = nullptr;  // kMaxKrylov entries
 
@@ -77,6 +80,7 @@ struct tdgl_batch {
   bool use_graph = true;
   hipGraphExec_t body = nullptr;  // one TGCR iteration (i >= 1) for the current layout
   double body_tol = -1.0;
+  int body_deg = -1;
 
   ProfSink own_prof;
   ProfSink* prof = &own_prof;
@@ -116,6 +120,7 @@ BatchK kview(const tdgl_batch* B) {
   k.act = B->d_act;
   k.res = B->d_res;
   k.dtv = B->d_dtv;
+  k.cheb = B->d_cheb;
   k.nfilm = B->nfilm;
   k.nitems = B->nitems;
   k.nsblk = B->nsblk;
@@ -219,6 +224,16 @@ int ensure_work(tdgl_batch* B) {
   return TDGL_OK;
 }
 
+int ensure_cheb(tdgl_batch* B) {
+  const size_t n = (size_t)B->total_len;
+  for (float*& t : B->t32) {
+    if (t) continue;
+    int rc;
+    if ((rc = balloc(B, (void**)&t, sizeof(float) * n))) return rc;
+  }
+  return TDGL_OK;
+}
+
 // Krylov vector Q_i (zero padding) and its table entry.
 int ensure_q(tdgl_batch* B, int i) {
   if (i >= kMaxKrylov) return bfail(B, TDGL_E_INVALID, "TGCR needs more than 2048 Krylov vectors");
@@ -258,6 +273,148 @@ double sum_over(const tdgl_batch* B, const std::vector<int>& act, double (*w)(co
   return s;
 }
 
+// Upper end of the spectrum of A = I - dt/2 J: 1 + dt/2 ||J||_inf, the row
+// sums of |J| read off k_bjvp32's coefficients (real representation, complex
+// rotations counted as sqrt(2)) with |psi| <= psimax.  psi rows: Laplacian
+// diagonal and neighbours, the link couplings, (1 - |psi|^2) psi and the tau
+// relaxation; link rows: 2 (4 c_a + 2 c_b + 2 c_c) (the doubled along-link
+// second difference and the transverse ones) + 4 (c_b + c_c) (curl cross
+// terms) = 8 (c_x + c_y + c_z), plus the supercurrent.
+double cheb_upper(const tdgl_ctx* c, double hdt) {
+  const double psimax = 2.0, r2 = sqrt(2.0);
+  const Mat& m = c->mt;
+  const bool d3 = c->g.is3d != 0;
+  const double ih = m.ihx2 + m.ihy2 + (d3 ? m.ihz2 : 0.0);
+  const double k2 = c->k2max;
+  const double cs = k2 * (m.ihx2 + m.ihy2 + (d3 ? m.ihz2 : 0.0));
+  const double row_psi = 2.0 * ih + 2.0 * r2 * (1.0 + psimax) * ih + 1.0 + 4.0 * psimax * psimax +
+                         (m.has_mat ? m.inv_tau : 0.0);
+  const double row_phi = 8.0 * cs + 4.0 * r2 * psimax + psimax * psimax;
+  return 1.0 + hdt * std::max(row_psi, row_phi);
+}
+
+// Chebyshev coefficients of this step for every film: interval [a, b] with
+// a = 1 - dt/2 (J's positive eigenvalues stay below 1: the (1 - |psi|^2) psi
+// term at psi = 0) and b = cheb_upper; sigma = (b + a)/(b - a),
+// omega_1 = 1, omega_2 = 2 sigma^2 / (2 sigma^2 - 1),
+// omega_{m+1} = 1 / (1 - omega_m / (4 sigma^2)).  The degree is common to the
+// batch: the request, or the largest over the active films of the degree
+// that brings the polynomial's deviation 1 / T_d(sigma) down to 0.25.
+int setup_cheb(tdgl_batch* B, int request, const std::vector<double>& dtv,
+               const std::vector<int>& act, double* interval0 = nullptr) {
+  B->cheb_deg = 0;
+  if (request == 0) return TDGL_OK;
+  const int F = B->nfilm;
+  std::vector<double> lo(F), hi(F);
+  int dmax = 1;
+  for (int f = 0; f < F; ++f) {
+    const double hdt = dtv[2 * f + 1];
+    lo[f] = 1.0 - hdt;
+    hi[f] = cheb_upper(B->films[f], hdt);
+    if (!act[f]) continue;
+    if (lo[f] < 0.1) return TDGL_OK;  // dt too large for a positive interval: no preconditioner
+    const double sig = (hi[f] + lo[f]) / (hi[f] - lo[f]);
+    const int d = (int)ceil(acosh(4.0) / acosh(sig));
+    dmax = std::max(dmax, d);
+  }
+  int d = request > 0 ? request : dmax;
+  d = std::max(1, std::min(d, kMaxChebDeg));
+  std::vector<double> tab((size_t)F * kChebStride, 0.0);
+  for (int f = 0; f < F; ++f) {
+    if (!(lo[f] < hi[f])) continue;
+    const double th = 0.5 * (hi[f] + lo[f]), de = 0.5 * (hi[f] - lo[f]);
+    const double sig = th / de, s2 = sig * sig;
+    double* t = tab.data() + (size_t)f * kChebStride;
+    t[0] = 1.0 / th;
+    t[1] = 1.0;
+    if (d >= 2) t[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
+    for (int mm = 3; mm <= d; ++mm) t[mm] = 1.0 / (1.0 - t[mm - 1] / (4.0 * s2));
+  }
+  int rc;
+  if ((rc = ensure_cheb(B))) return rc;
+  BCHK(hipMemcpyAsync(B->d_cheb, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, B->ps));
+  BCHK(hipStreamSynchronize(B->ps));  // tab is a host temporary
+  B->cheb_deg = d;
+  if (interval0) {
+    interval0[0] = lo[0];
+    interval0[1] = hi[0];
+  }
+  return TDGL_OK;
+}
+
+// W = A M v into `out` (nullptr: Q[*iter]) at the linearisation point x32:
+// the plain JVP (no preconditioner), or the d sweeps of the residual
+// recurrence rho_m = P_m(A) v, P_m(t) = T_m((theta - t)/delta) / T_m(sigma):
+// rho_1 = v - A v / theta, rho_m = omega_m (rho_{m-1} - A rho_{m-1} / theta)
+// - (omega_m - 1) rho_{m-2}, W = v - rho_d (= A z_d, z_d the Chebyshev
+// iterate for A z = v; M = (I - P_d(A)) A^{-1}).  rho_m lives in
+// t32[(m - 1) % 2]; each sweep overwrites its rho_{m-2} in place.
+void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const float* x32,
+               const float* v, float* out) {
+  const BatchK bk = kview(B);
+  const int d = B->cheb_deg;
+  const double S = prof ? sum_over(B, act, s32) : 0.0;
+  if (d == 0) {
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S) : -1;
+    launch_bjvp32(bk, B->is3d, x32, v, B->d_qtab, B->d_iter, out, B->ps);
+    if (prof) B->prof->end(B->ps, tok);
+    return;
+  }
+  // bytes: x, source and output, plus prev (m >= 2) and v at the last sweep (m >= 3)
+  const double by = prof ? S * (3.0 * d + (d >= 2 ? d - 1.0 : 0.0) + (d >= 3 ? 1.0 : 0.0)) : 0.0;
+  const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, d) : -1;
+  for (int m = 1; m <= d; ++m) {
+    ChebStep cs{};
+    cs.m = m;
+    const float* src = m == 1 ? v : B->t32[(m - 2) % 2];
+    cs.prev = m == 1 ? nullptr : (m == 2 ? v : B->t32[(m - 1) % 2]);
+    cs.fin = m == d;
+    cs.v = cs.fin ? v : nullptr;
+    cs.out = cs.fin ? out : B->t32[(m - 1) % 2];
+    launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+  }
+  if (prof) B->prof->end(B->ps, tok);
+}
+
+// z = M u: the Chebyshev iterate z_d of A z = u (z_1 = u / theta,
+// z_m = omega_m (z_{m-1} + (u - A z_{m-1}) / theta) - (omega_m - 1) z_{m-2}).
+// z_m lives in t32[m % 2]; returns z_d's buffer, or nullptr for d = 1
+// (z = u / theta, applied by the consumer).
+const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, const float* x32,
+                       const float* u) {
+  const int d = B->cheb_deg;
+  if (d < 2) return nullptr;
+  const BatchK bk = kview(B);
+  const double S = prof ? sum_over(B, act, s32) : 0.0;
+  const double by = prof ? S * (3.0 * (d - 1) + (d >= 3 ? 1.0 : 0.0) + (d >= 4 ? 2.0 * (d - 3) : 0.0)) : 0.0;
+  const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, d - 1) : -1;
+  for (int m = 2; m <= d; ++m) {
+    ChebStep cs{};
+    cs.m = m;
+    cs.cv = 1;
+    cs.v = u;
+    const float* src = m == 2 ? u : B->t32[(m - 1) % 2];
+    cs.src_th = m == 2;
+    cs.prev = m == 2 ? nullptr : (m == 3 ? u : B->t32[m % 2]);
+    cs.prev_th = m == 3;
+    cs.out = B->t32[m % 2];
+    launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+  }
+  if (prof) B->prof->end(B->ps, tok);
+  return B->t32[d % 2];
+}
+
+// x += M u for the films in `ok`, u = R y in r0_32, with ||M u||_inf partials.
+void enqueue_precond_update(tdgl_batch* B, bool prof, const std::vector<int>& ok) {
+  const BatchK bk = kview(B);
+  const float* z = enqueue_z(B, prof, ok, B->x32, B->r0_32);
+  const double S = prof ? sum_over(B, ok, s32) : 0.0;
+  const int tok = prof ? B->prof->begin(B->ps, TDGL_K_ACCUM, S + 2.0 * sum_over(B, ok, s64), 2) : -1;
+  launch_baddz(bk, B->w_x, z ? z : B->r0_32, z ? 0 : 1, B->d_pm, B->ps);
+  launch_bcolsum(bk, B->d_iter, B->d_pm, CS_DXINF, 1, B->ps);
+  if (prof) B->prof->end(B->ps, tok);
+}
+
 // Newton residual in FP64 at w_x for the active films: x32, r32 = -g, and
 // res[RS_XX, RS_RR, RS_GINF].
 int newton_eval(tdgl_batch* B, const std::vector<int>& act, std::vector<tdgl_step_stats>& S) {
@@ -285,10 +442,8 @@ constexpr int kColsumY = 32;  // column blocks per film in the coefficient sums
 void enqueue_body(tdgl_batch* B, double tol, bool prof, int i, const std::vector<int>& act) {
   const BatchK bk = kview(B);
   const double S3 = prof ? sum_over(B, act, s32) : 0.0;
-  int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S3) : -1;
-  launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->ps);
-  if (prof) B->prof->end(B->ps, tok);
-  tok = prof ? B->prof->begin(B->ps, TDGL_K_MGS, (i + 1.0) * S3, 2) : -1;  // read Q_0..Q_{i-1}, W
+  enqueue_w(B, prof, act, B->x32, B->r32, nullptr);
+  int tok = prof ? B->prof->begin(B->ps, TDGL_K_MGS, (i + 1.0) * S3, 2) : -1;  // read Q_0..Q_{i-1}, W
   launch_bmdot(bk, 1, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
   launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD1, kColsumY, B->ps);
   if (prof) B->prof->end(B->ps, tok);
@@ -319,7 +474,7 @@ int launch_body(tdgl_batch* B, double tol) {
     BCHK(hipGetLastError());
     return TDGL_OK;
   }
-  if (B->body && B->body_tol != tol) {
+  if (B->body && (B->body_tol != tol || B->body_deg != B->cheb_deg)) {
     BCHK(hipGraphExecDestroy(B->body));
     B->body = nullptr;
   }
@@ -334,6 +489,7 @@ int launch_body(tdgl_batch* B, double tol) {
     (void)hipGraphDestroy(g);
     BCHK(ei);
     B->body_tol = tol;
+    B->body_deg = B->cheb_deg;
   }
   BCHK(hipGraphLaunch(B->body, B->ps));
   return TDGL_OK;
@@ -375,10 +531,8 @@ int tgcr(tdgl_batch* B, const std::vector<int>& act0, double tol, std::vector<td
   // iteration 0
   {
     const double S3 = prof ? sum_over(B, act, s32) : 0.0;
-    int tok = B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S3);
-    launch_bjvp32(bk, B->is3d, B->x32, B->r32, B->d_qtab, B->d_iter, nullptr, B->ps);
-    B->prof->end(B->ps, tok);
-    tok = B->prof->begin(B->ps, TDGL_K_MGS, 2.0 * S3, 2);
+    enqueue_w(B, prof, act, B->x32, B->r32, nullptr);
+    int tok = B->prof->begin(B->ps, TDGL_K_MGS, 2.0 * S3, 2);
     launch_bmdot(bk, 0, B->d_qtab, B->r32, B->d_iter, B->d_pm, B->ps);
     launch_bcolsum(bk, B->d_iter, B->d_pm, CS_MD0, 1, B->ps);
     B->prof->end(B->ps, tok);
@@ -419,8 +573,11 @@ int tgcr(tdgl_batch* B, const std::vector<int>& act0, double tol, std::vector<td
   for (int f = 0; f < F; ++f) {
     if (!act0[f]) continue;
     const double kit = rep(B, f)[8];
-    S[f].jvp_evals += (int)kit;
-    if (kit >= 1) S[f].bytes_moved += (9.0 + 11.0 * (kit - 1) + (kit - 1) * kit) * s32(B, f);
+    const int d = B->cheb_deg;
+    S[f].jvp_evals += (int)kit * std::max(d, 1);
+    // iteration i: W (3 without a preconditioner), dots (i + 1), update (i + 3), finish 4
+    const double wb = d == 0 ? 3.0 : 3.0 * d + (d >= 2 ? d - 1.0 : 0.0) + (d >= 3 ? 1.0 : 0.0);
+    if (kit >= 1) S[f].bytes_moved += (wb + 6.0 + (wb + 8.0) * (kit - 1) + (kit - 1) * kit) * s32(B, f);
   }
   return TDGL_OK;
 }
@@ -463,6 +620,7 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
   memset(S.data(), 0, sizeof(tdgl_step_stats) * F);
   std::vector<int> act(F);
   for (int f = 0; f < F; ++f) act[f] = active ? (active[f] != 0) : 1;
+  if ((rc = setup_cheb(B, o->mixed_precond, dtv, act))) return rc;
 
   // predictor + gamma (integrators.py:156-158)
   if ((rc = set_act(B, act))) return rc;
@@ -511,17 +669,38 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
     }
     if (nok) {
       if ((rc = set_act(B, ok))) return rc;
-      double by = 2.0 * sum_over(B, ok, s64);
-      for (int f = 0; f < F; ++f)
-        if (ok[f]) by += (rep(B, f)[7] + 1.0) * s32(B, f);
-      const int tok = B->prof->begin(B->ps, TDGL_K_ACCUM, by, 3);
-      launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->ps);
-      launch_baccum(kview(B), B->w_x, B->r0_32, B->d_qtab, B->d_fstat, B->d_pm, B->ps);
-      launch_bcolsum(kview(B), B->d_iter, B->d_pm, CS_DXINF, 1, B->ps);
-      B->prof->end(B->ps, tok);
+      const int d = B->cheb_deg;
+      if (d == 0) {
+        double by = 2.0 * sum_over(B, ok, s64);
+        for (int f = 0; f < F; ++f)
+          if (ok[f]) by += (rep(B, f)[7] + 1.0) * s32(B, f);
+        const int tok = B->prof->begin(B->ps, TDGL_K_ACCUM, by, 3);
+        launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->ps);
+        launch_baccum(kview(B), B->w_x, B->r0_32, B->d_qtab, B->d_fstat, B->d_pm, B->ps);
+        launch_bcolsum(kview(B), B->d_iter, B->d_pm, CS_DXINF, 1, B->ps);
+        B->prof->end(B->ps, tok);
+      } else {  // u = R y into r0_32, then x += M u
+        double by = 0.0;
+        for (int f = 0; f < F; ++f)
+          if (ok[f]) by += (rep(B, f)[7] + 2.0) * s32(B, f);
+        const int tok = B->prof->begin(B->ps, TDGL_K_ACCUM, by, 2);
+        launch_btrisolve(kview(B), B->d_fstat, B->d_H, B->ps);
+        launch_baccum_u(kview(B), B->r0_32, B->r0_32, B->d_qtab, B->d_fstat, B->ps);
+        B->prof->end(B->ps, tok);
+        enqueue_precond_update(B, B->prof->on, ok);
+      }
       BCHK(hipGetLastError());
-      for (int f = 0; f < F; ++f)
-        if (ok[f]) S[f].bytes_moved += (rep(B, f)[7] + 1.0) * s32(B, f) + 2.0 * s64(B, f);
+      for (int f = 0; f < F; ++f) {
+        if (!ok[f]) continue;
+        const double kk = rep(B, f)[7];
+        if (d == 0) {
+          S[f].bytes_moved += (kk + 1.0) * s32(B, f) + 2.0 * s64(B, f);
+        } else {
+          const double zb = d >= 2 ? 3.0 * (d - 1) + (d >= 3 ? 1.0 : 0.0) + (d >= 4 ? 2.0 * (d - 3) : 0.0) : 0.0;
+          S[f].bytes_moved += (kk + 2.0 + zb + 1.0) * s32(B, f) + 2.0 * s64(B, f);
+          S[f].jvp_evals += std::max(d - 1, 0);
+        }
+      }
       if ((rc = newton_eval(B, ok, S))) return rc;  // newton.py:109-113
       if ((rc = report(B))) return rc;
       for (int f = 0; f < F; ++f) {
@@ -571,6 +750,7 @@ int batch_init(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_iter, sizeof(int)))) return rc;
   if ((rc = balloc(B, (void**)&B->d_nact, sizeof(int)))) return rc;
   if ((rc = balloc(B, (void**)&B->d_rep, sizeof(double) * kReportLen * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_cheb, sizeof(double) * kChebStride * F))) return rc;
   BCHK(hipHostMalloc((void**)&B->h_rep, sizeof(double) * kReportLen * F, hipHostMallocDefault));
   BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * 3 * F, hipHostMallocDefault));
   BCHK(hipStreamCreateWithFlags(&B->ps, hipStreamNonBlocking));
@@ -642,6 +822,39 @@ int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, f
   return sync_out(B);
 }
 
+int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt, int degree,
+                   float* w32, float* z32, double* info) {
+  int rc;
+  if ((rc = own_batch(ctx))) return rc;
+  tdgl_batch* B = ctx->own_batch;
+  B->st = ctx->st;
+  auto done = [&](int code) {
+    if (code) ctx->err = B->err;
+    return code;
+  };
+  if ((rc = ensure_work(B)) || (rc = sync_in(B)) || (rc = refresh(B))) return done(rc);
+  const std::vector<double> dtv = {dt, dt / 2.0};
+  std::vector<int> one(1, 1);
+  if ((rc = set_act(B, one))) return done(rc);
+  if (hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * 2, hipMemcpyHostToDevice, B->ps) != hipSuccess)
+    return done(bfail(B, TDGL_E_HIP, "hipMemcpyAsync"));
+  double iv[2] = {0.0, 0.0};
+  if ((rc = setup_cheb(B, degree, dtv, one, iv))) return done(rc);
+  if (B->cheb_deg == 0) return done(bfail(B, TDGL_E_INVALID, "no preconditioner for this degree and dt"));
+  enqueue_w(B, false, one, x32, v32, w32);
+  const float* z = enqueue_z(B, false, one, x32, v32);
+  if (z && hipMemcpyAsync(z32, z, sizeof(float) * (size_t)B->hf[0].len, hipMemcpyDeviceToDevice, B->ps) != hipSuccess)
+    return done(bfail(B, TDGL_E_HIP, "hipMemcpyAsync"));
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess)
+    return done(bfail(B, TDGL_E_HIP, "preconditioner kernels failed"));
+  // degree 1: z = v / theta
+  info[0] = iv[0];
+  info[1] = iv[1];
+  info[2] = (double)B->cheb_deg;
+  info[3] = z ? 0.0 : 1.0;
+  return done(sync_out(B));
+}
+
 }  // namespace tdgl
 
 extern "C" {
@@ -678,7 +891,8 @@ int tdgl_batch_destroy(tdgl_batch* B) {
                   (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
                   (void*)B->d_H, (void*)B->d_rep, (void*)B->w_x, (void*)B->w_gam, (void*)B->x32,
-                  (void*)B->r32, (void*)B->r0_32, (void*)B->d_qtab})
+                  (void*)B->r32, (void*)B->r0_32, (void*)B->d_qtab, (void*)B->d_cheb,
+                  (void*)B->t32[0], (void*)B->t32[1]})
     if (p) (void)hipFree(p);
   for (float* q : B->Q) (void)hipFree(q);
   if (B->h_rep) (void)hipHostFree(B->h_rep);

@@ -119,6 +119,7 @@ struct tdgl_ctx {
   int kz_user = 0;    // tdgl_set_stencil_kz (0: the built-in choice)
   double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
   double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
+  double k2max = 0.0;         // largest kappa^2 of any node (preconditioner bounds)
   int pb_blocks = 0;  // capacity of d_pb in blocks
 
   tdgl::ProfSink prof;  // tdgl_profile_enable
@@ -129,4 +130,7 @@ namespace tdgl {
 int mixed_step_single(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
                       tdgl_step_stats* s);
 int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32);
+// One application of the mixed TGCR's Chebyshev preconditioner (tdgl_batch.hip).
+int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt, int degree,
+                   float* w32, float* z32, double* info);
 }  // namespace tdgl

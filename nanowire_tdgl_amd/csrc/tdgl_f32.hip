@@ -47,9 +47,14 @@ constexpr int BS = TX * TY;
 // boundary rules are affine in the state, so p's ghost values use the same
 // rules with a zero applied field; hole-boundary rows are zero.
 // ---------------------------------------------------------------------------
-template <bool IS3D>
+//
+// CHEB: the same stencil, with the epilogue of one Chebyshev step (ChebStep)
+// instead of Ap.  A = I - dt/2 J is linear, so A (s p) = s A p: the source is
+// staged unscaled and s enters in the epilogue.
+template <bool IS3D, bool CHEB>
 __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
-                                               float* const* Q, const int* iter, float* out_direct) {
+                                               float* const* Q, const int* iter, float* out_direct,
+                                               ChebStep cs) {
   constexpr int NS = IS3D ? 3 : 1;
   constexpr int NZ = IS3D ? 3 : 1;
   constexpr int CZ = IS3D ? CELLS : 1;
@@ -68,6 +73,18 @@ __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, cons
   const float* xs = x32 + off;
   const float* ps = p32 + off;
   const float hdt = (float)bk.dtv[2 * f + 1];
+  float c_it = 0.f, c_om = 0.f, c_ss = 1.f, c_sp = 0.f;
+  const float* cv_v = nullptr;
+  const float* cv_p = nullptr;
+  if (CHEB) {
+    const double* cf = bk.cheb + (long long)f * kChebStride;
+    c_it = (float)cf[0];
+    c_om = (float)cf[cs.m];
+    c_ss = cs.src_th ? c_it : 1.f;
+    c_sp = cs.prev ? (c_om - 1.f) * (cs.prev_th ? c_it : 1.f) : 0.f;
+    if (cs.v) cv_v = cs.v + off;
+    if (cs.prev) cv_p = cs.prev + off;
+  }
   const int li = (int)L - fd->item0;
   const int tile_x = li % fd->tiles_x, rest = li / fd->tiles_x;
   const int tile_y = rest % fd->tiles_y, zc = rest / fd->tiles_y;
@@ -133,6 +150,8 @@ __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, cons
     float scv = 1.f, k2v = 0.f;
     float2 p0 = make_float2(0.f, 0.f);
     float praw_x = 0.f, praw_y = 0.f, praw_z = 0.f;
+    float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
+    float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
     if (act) {
       q = qix(g, i, j, k);
       if (mt.flags) fl = mt.flags[mfull(g, i, j, k)];
@@ -142,6 +161,18 @@ __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, cons
       praw_x = ps[2 * n + q];
       praw_y = ps[3 * n + q];
       if (IS3D) praw_z = ps[4 * n + q];
+      if (CHEB && cv_v) {
+        v0 = reinterpret_cast<const float2*>(cv_v)[q];
+        v_x = cv_v[2 * n + q];
+        v_y = cv_v[3 * n + q];
+        if (IS3D) v_z = cv_v[4 * n + q];
+      }
+      if (CHEB && cv_p) {
+        w0 = reinterpret_cast<const float2*>(cv_p)[q];
+        w_x = cv_p[2 * n + q];
+        w_y = cv_p[3 * n + q];
+        if (IS3D) w_z = cv_p[4 * n + q];
+      }
     }
     if (IS3D && k + 2 <= ke) stage(k + 2);
 
@@ -240,10 +271,25 @@ __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, cons
       if (fl & kHoleY) jy = 0.f;
       if (IS3D && (fl & kHoleZ)) jz = 0.f;
 
-      reinterpret_cast<float2*>(ap)[q] = make_float2(p0.x - hdt * jr, p0.y - hdt * ji);
-      ap[2 * n + q] = praw_x - hdt * jx;
-      ap[3 * n + q] = praw_y - hdt * jy;
-      if (IS3D) ap[4 * n + q] = praw_z - hdt * jz;
+      if (CHEB) {
+        const float cvf = cs.cv ? 1.f : 0.f;
+        auto step = [&](float pv, float jv, float vv, float wv) {
+          const float y = c_ss * pv;
+          const float ay = c_ss * (pv - hdt * jv);
+          const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
+          return cs.fin ? vv - res : res;
+        };
+        reinterpret_cast<float2*>(ap)[q] =
+            make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y));
+        ap[2 * n + q] = step(praw_x, jx, v_x, w_x);
+        ap[3 * n + q] = step(praw_y, jy, v_y, w_y);
+        if (IS3D) ap[4 * n + q] = step(praw_z, jz, v_z, w_z);
+      } else {
+        reinterpret_cast<float2*>(ap)[q] = make_float2(p0.x - hdt * jr, p0.y - hdt * ji);
+        ap[2 * n + q] = praw_x - hdt * jx;
+        ap[3 * n + q] = praw_y - hdt * jy;
+        if (IS3D) ap[4 * n + q] = praw_z - hdt * jz;
+      }
     }
     __syncthreads();
   }
@@ -596,18 +642,88 @@ __global__ __launch_bounds__(kRedThreads) void k_baccum(BatchK bk, double* x, co
   if (threadIdx.x == 0) pm[blockIdx.x] = v[0];
 }
 
+// u = gamma r_0 + sum_{m<K-1} coef_m Q_m in FP32: with the Chebyshev
+// preconditioner M the Newton update is x += M u (GCR with p_k = M r_k and a
+// fixed polynomial M gives x = M R y).  u may alias r0v (same element, same
+// thread).
+__global__ __launch_bounds__(kRedThreads) void k_baccum_u(BatchK bk, float* u, const float* r0v,
+                                                          float* const* Q, const int* fstat) {
+  FILM_STREAM(bk);
+  const int K = fstat[4 * f + 1];
+  const double gam = res[RS_GAMMA];
+  const double* coef = res + RS_BETA;
+  const float4* R0 = reinterpret_cast<const float4*>(r0v + off);
+  float4* U = reinterpret_cast<float4*>(u + off);
+  for (long long e = e0; e < n4; e += estride) {
+    const float4 r = R0[e];
+    double d0 = gam * r.x, d1 = gam * r.y, d2 = gam * r.z, d3 = gam * r.w;
+    for (int m = 0; m < K - 1; ++m) {
+      const float4 a = reinterpret_cast<const float4*>(Q[m] + off)[e];
+      const double c = coef[m];
+      d0 += c * a.x;
+      d1 += c * a.y;
+      d2 += c * a.z;
+      d3 += c * a.w;
+    }
+    U[e] = make_float4((float)d0, (float)d1, (float)d2, (float)d3);
+  }
+}
+
+// x (FP64) += s z, s = 1/theta (degree-1 preconditioner: M = 1/theta) or 1;
+// partials of ||s z||_inf (complex modulus on psi, |.| on phi).
+__global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, const float* z,
+                                                       int scale_th, double* pm) {
+  FILM_STREAM(bk);
+  const double s = scale_th ? bk.cheb[(long long)f * kChebStride] : 1.0;
+  const float4* Z = reinterpret_cast<const float4*>(z + off);
+  double2* X = reinterpret_cast<double2*>(x + off);
+  const long long npsi = 2 * fd->g.n;
+  double v[1] = {0.0};
+  for (long long e = e0; e < n4; e += estride) {
+    const float4 a = Z[e];
+    const double d0 = s * a.x, d1 = s * a.y, d2 = s * a.z, d3 = s * a.w;
+    const double2 xa = X[2 * e], xb = X[2 * e + 1];
+    X[2 * e] = make_double2(xa.x + d0, xa.y + d1);
+    X[2 * e + 1] = make_double2(xb.x + d2, xb.y + d3);
+    const long long el = 4 * e;
+    const double ma = (el < npsi) ? hypot(d0, d1) : fmax(fabs(d0), fabs(d1));
+    const double mb = (el + 2 < npsi) ? hypot(d2, d3) : fmax(fabs(d2), fabs(d3));
+    v[0] = fmax(v[0], fmax(ma, mb));
+  }
+  block_reduce<0, 1>(v);
+  if (threadIdx.x == 0) pm[blockIdx.x] = v[0];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
 void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                    const int* iter, float* out_direct, hipStream_t st) {
+  const ChebStep none{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0};
   if (is3d)
-    hipLaunchKernelGGL(k_bjvp32<true>, dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q, iter,
-                       out_direct);
+    hipLaunchKernelGGL((k_bjvp32<true, false>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
+                       iter, out_direct, none);
   else
-    hipLaunchKernelGGL(k_bjvp32<false>, dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q, iter,
-                       out_direct);
+    hipLaunchKernelGGL((k_bjvp32<false, false>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
+                       iter, out_direct, none);
+}
+void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
+                    const int* iter, const ChebStep& cs, hipStream_t st) {
+  if (is3d)
+    hipLaunchKernelGGL((k_bjvp32<true, true>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
+                       iter, cs.out, cs);
+  else
+    hipLaunchKernelGGL((k_bjvp32<false, true>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
+                       iter, cs.out, cs);
+}
+void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_baccum_u, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, u, r0v, Q, fstat);
+}
+void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_baddz, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, z, scale_th, pm);
 }
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                   double* pm, hipStream_t st) {

@@ -29,6 +29,10 @@ enum ResSlot : int {
   RS_BETA = 16,  // beta_j (mixed: first-pass coefficients; after the solve, the update's), j < kMaxKrylov
 };
 constexpr int kMaxKrylov = 2048;
+// Chebyshev polynomial preconditioner of the mixed TGCR: largest degree and
+// the per-film coefficient stride (1/theta, then omega_1 .. omega_d).
+constexpr int kMaxChebDeg = 64;
+constexpr int kChebStride = kMaxChebDeg + 2;
 constexpr int RS_ALPHA = RS_BETA + kMaxKrylov;
 constexpr int RS_DBETA = RS_ALPHA + kMaxKrylov;  // mixed: second-pass coefficients
 constexpr int kResLen = RS_DBETA + kMaxKrylov;
@@ -72,6 +76,7 @@ struct BatchK {
   int* act;              // per film: takes part in the current stage
   double* res;           // nfilm * kResLen scalar slots
   const double* dtv;     // per film: dt, dt/2
+  const double* cheb;    // per film, kChebStride doubles: 1/theta, omega_1 .. omega_d
   int nfilm;
   int nitems;            // stencil blocks of the batch
   int nsblk;             // streaming blocks of the batch
@@ -116,6 +121,25 @@ constexpr int kReportLen = 10;  // doubles per film in the report
 // Mixed-precision TGCR kernels (tdgl_f32.hip).
 void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                    const int* iter, float* out_direct, hipStream_t st);
+// One step of a Chebyshev recurrence for A = I - dt/2 J(x) (the stencil of
+// k_bjvp32 with another epilogue).  With y = s p (s = 1/theta when src_th,
+// else 1) and the film's omega_m, 1/theta:
+//   res = omega_m (y + (cv v - A y) / theta) - (omega_m - 1) s' prev
+//   out = fin ? v - res : res
+// (prev == nullptr: no prev term; s' = 1/theta when prev_th).  out may alias
+// prev; out == nullptr writes Q[*iter].
+struct ChebStep {
+  const float* v;
+  const float* prev;
+  float* out;
+  int m;
+  int src_th, prev_th, cv, fin;
+};
+void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
+                    const int* iter, const ChebStep& cs, hipStream_t st);
+// x (FP64) += s z with s = 1/theta of the film when scale_th, else 1; partials of ||s z||_inf.
+void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
+                  hipStream_t st);
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                   double* pm, hipStream_t st);
 void launch_bupd(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
@@ -131,6 +155,10 @@ void launch_btinit(const BatchK& bk, int* fstat, const int* maxk, int* iter, hip
 void launch_btrisolve(const BatchK& bk, const int* fstat, const double* H, hipStream_t st);
 void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* Q, const int* fstat,
                    double* pm, hipStream_t st);
+// u = gamma r_0 + sum_m coef_m Q_m in FP32 (the argument of the preconditioner
+// in x += M u); u may alias r0v.
+void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
+                     hipStream_t st);
 // Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
                       double* A, double* gsc, double kappa, double* pk, hipStream_t st);

@@ -84,20 +84,22 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
                       tgcr_max_iter: Optional[int] = None, mixed_tol_floor: float = 1e-6,
                       dt_min: float = 1e-8, max_attempts: Optional[int] = None,
                       tune: bool = False, runner: Optional[BatchRunner] = None,
-                      keep_log: bool = False) -> BatchResult:
+                      keep_log: bool = False, precond="chebyshev") -> BatchResult:
     """Advance every film from t_start to t_stop (mixed precision, lockstep).
 
     The Newton/TGCR parameters mean what they mean in
     :func:`nanowire_tdgl_amd.solvers.integrators.trapezoidal`; ``tune`` runs
     the stencil autotuner first; ``max_attempts`` bounds the number of
-    lockstep attempts (None: until every film reaches t_stop)."""
+    lockstep attempts (None: until every film reaches t_stop); ``precond`` as
+    in ``trapezoidal`` (the automatic degree is the largest any film asks for)."""
     runner = runner or BatchRunner([f.device for f in films])
     F = len(films)
     X = runner.pack([f.x0 for f in films])
     if tune:
         runner.tune()
     opts = _native.solver_opts(newton_tol_f, newton_tol_dx, newton_max_iter, tol_gcr, 1e-4,
-                               tgcr_max_iter, precision="mixed", mixed_tol_floor=mixed_tol_floor)
+                               tgcr_max_iter, precision="mixed", mixed_tol_floor=mixed_tol_floor,
+                               precond=precond)
     t = np.full(F, float(t_start))
     cur = np.full(F, float(dt))
     steps = np.zeros(F, dtype=np.int64)

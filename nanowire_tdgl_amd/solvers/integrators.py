@@ -74,7 +74,7 @@ def trapezoidal(x0, params: SimulationParameters, idx: GridIndices, eval_u: Call
                 progress: bool = True, verbose: bool = False,
                 material: Optional[MaterialMap] = None, tgcr_max_iter: Optional[int] = None,
                 step_log: Optional[list] = None, precision: str = "f64",
-                mixed_tol_floor: float = 1e-6):
+                mixed_tol_floor: float = 1e-6, precond="chebyshev"):
     """Implicit trapezoidal integration with Newton-TGCR (integrators.py:92-206).
 
     Extensions (defaults reproduce the reference): ``tgcr_max_iter`` caps the
@@ -82,7 +82,9 @@ def trapezoidal(x0, params: SimulationParameters, idx: GridIndices, eval_u: Call
     ``step_log`` receives one dict of native counters per step attempt;
     ``precision="mixed"`` keeps the Newton residual and the state in FP64 but
     runs TGCR with an analytic FP32 Jacobian-vector product and FP32 Krylov
-    vectors, with its tolerance floored at ``mixed_tol_floor``.
+    vectors, with its tolerance floored at ``mixed_tol_floor`` and (``precond``)
+    a Chebyshev polynomial right preconditioner: "chebyshev" (automatic
+    degree), "none", or a degree d >= 1.
     """
     eng = engine_for(params, idx, material)
     ctx = eng.ctx
@@ -90,7 +92,7 @@ def trapezoidal(x0, params: SimulationParameters, idx: GridIndices, eval_u: Call
     X = ctx.to_internal(X0)
     opts = _native.solver_opts(newton_tol_f, newton_tol_dx, newton_max_iter, tol_gcr, eps_mf,
                                tgcr_max_iter, precision=precision,
-                               mixed_tol_floor=mixed_tol_floor)
+                               mixed_tol_floor=mixed_tol_floor, precond=precond)
     t = t_start
     cur = dt
     times, hist = [t], [X0.copy()]

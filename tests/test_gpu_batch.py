@@ -2,7 +2,9 @@
 
 * A film's arithmetic does not depend on the other films of its batch: each
   film owns its work items, streaming blocks and reduction order, so a batch
-  reproduces every film's own batch-of-one step bit for bit.
+  reproduces every film's own batch-of-one step bit for bit (with or without
+  a Chebyshev preconditioner of a fixed degree; the automatic degree is the
+  largest any film of the batch asks for).
 * Films switched off for an attempt are left untouched.
 * Against FP64 steps of the same films: |psi|^2, B_z and J_s agree to 1e-6
   relative (of the largest entry) after several steps.
@@ -68,16 +70,18 @@ def _observables(eng, X, n):
     return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()
 
 
-def test_batch_reproduces_each_film_bitwise():
+@pytest.mark.parametrize("precond", ["none", 6])
+def test_batch_reproduces_each_film_bitwise(precond):
+    # a fixed preconditioner degree: the automatic one is the batch's largest
     from nanowire_tdgl_amd import _native
     from nanowire_tdgl_amd.solvers.batch import BatchRunner
     films = _films()
     steps, dt = 3, 0.2
-    singles = [_single(f, steps, dt, "mixed") for f in films]
+    singles = [_single(f, steps, dt, "mixed", precond=precond) for f in films]
     got = [s[1].clone() for s in singles]
     runner = BatchRunner([f.device for f in films])
     X = runner.pack([f.x0 for f in films])
-    opts = _native.solver_opts(precision="mixed")
+    opts = _native.solver_opts(precision="mixed", precond=precond)
     for s in range(steps):
         runner.set_fields([s * dt] * len(films), 100.0)
         st = runner.batch.trap_step(X, dt, opts)

@@ -1,0 +1,219 @@
+"""GPU: the Chebyshev polynomial preconditioner of the mixed-precision TGCR.
+
+The library applies M = p(A), A = I - dt/2 J(x), through two recurrences in
+FP32 stencil sweeps: w = A M v = v - P_d(A) v and z = M v, with
+P_d(t) = T_d((theta - t)/delta) / T_d(sigma) on the interval [a, b] it picks.
+
+* w is checked against P_d(A) v built from the textbook three-term recurrence
+  of the Chebyshev polynomials T_{k+1}(X) = 2 X T_k(X) - T_{k-1}(X) with
+  X = (theta I - A)/delta, A applied with the oracle's exact FP64 JVP, and
+  T_d(sigma) from numpy's Chebyshev series evaluation; z is checked through
+  A z = w.  Tolerance 2e-5 of the largest entry: FP32 sweeps (unit roundoff
+  6e-8) over degrees up to 12 with 7-point stencils.
+* The interval must contain the spectrum of A: dense eigenvalues of A on
+  small films (holes, insulating layers with kappa = 0, 2-D, periodic) lie in
+  [a, b] (real parts; the imaginary parts are small).
+* Full mixed steps with any preconditioner setting converge to the FP64
+  solution of the step equations (tight Newton tolerance), and a batch with a
+  fixed degree reproduces each film's own step bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import specs as S
+from helpers import device, oracle_film, rand_state, rel_err
+from oracle import tdgl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _round32(v):
+    v = np.asarray(v, np.complex128)
+    return v.real.astype(np.float32).astype(np.float64) + 1j * v.imag.astype(np.float32)
+
+
+def _setup(name, dt):
+    from nanowire_tdgl_amd.engine import engine_for
+    case = S.EVALF_CASES[name]
+    dev = device(case)
+    f = oracle_film(case)
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    eng.set_field(*case["field"])
+    u = O.boundary_vectors(*case["field"], f.g, f.idx)
+    n = dev.params.n_interior
+    nphi = 3 if dev.params.Nz > 1 else 2
+    x = _round32(rand_state(n, nphi, 51, 0.3, 0.2))
+    A = lambda y: y - (dt / 2.0) * O.jvp_stencil(x, y, f.g, f.idx, u, f.material)  # noqa: E731
+    return eng, x, A, n, nphi
+
+
+def _apply(eng, x, v, dt, degree):
+    import torch
+    ctx = eng.ctx
+    gx, gv = ctx.to_internal(x).float(), ctx.to_internal(v).float()
+    w = torch.zeros_like(gx)
+    z = torch.zeros_like(gx)
+    info = ctx.precond_cheb32(gx, gv, dt, degree, w, z)
+    a, b = info["a"], info["b"]
+    if info["z_is_v_over_theta"]:
+        z = gv / ((a + b) / 2.0)
+    back = lambda t: ctx.to_reference(t.double()).cpu().numpy()  # noqa: E731
+    return back(w), back(z), info
+
+
+def _chebyshev_w(A, v, a, b, d):
+    """v - T_d(X) v / T_d(sigma), X = (theta I - A)/delta, by T_{k+1} = 2 X T_k - T_{k-1}."""
+    th, de = (a + b) / 2.0, (b - a) / 2.0
+    X = lambda y: (th * y - A(y)) / de  # noqa: E731
+    t_prev, t_cur = v, X(v)
+    for _ in range(1, d):
+        t_prev, t_cur = t_cur, 2.0 * X(t_cur) - t_prev
+    td_sigma = np.polynomial.chebyshev.chebval(th / de, [0.0] * d + [1.0])
+    return v - t_cur / td_sigma
+
+
+@pytest.mark.parametrize("name,degree", [("c1_bz", 1), ("c1_bz", 2), ("c1_bz", 3), ("c1_bz", 12),
+                                         ("hole_bz", 5), ("plain2d", 4), ("periodic_xyz", 6),
+                                         ("trilayer_hole", 7), ("hole_zr", -1)])
+def test_preconditioner_is_the_chebyshev_polynomial(name, degree):
+    dt = 0.2
+    eng, x, A, n, nphi = _setup(name, dt)
+    v = _round32(rand_state(n, nphi, 52, 1.0, 1.0))
+    w, z, info = _apply(eng, x, v, dt, degree)
+    a, b, d = info["a"], info["b"], info["degree"]
+    if degree > 0:
+        assert d == degree
+    assert a == pytest.approx(1.0 - dt / 2.0)
+    want = _chebyshev_w(A, v, a, b, d)
+    err_w = rel_err(w, want)
+    err_z = rel_err(A(z), want)
+    print(f"{name} d={d} [a,b]=[{a:.3f},{b:.2f}]: |w - (I-P(A))v| {err_w:.2e}, |A z - w| {err_z:.2e}")
+    assert err_w < 2e-5
+    assert err_z < 2e-5
+
+
+def _dense_real(A, n, nphi):
+    """A in the real representation [re psi, im psi, phi] (oracle, FP64)."""
+    N = (2 + nphi) * n
+    M = np.zeros((N, N))
+    for k in range(N):
+        e = np.zeros(N)
+        e[k] = 1.0
+        y = np.concatenate([e[:n] + 1j * e[n:2 * n], e[2 * n:].astype(complex)])
+        r = A(y)
+        M[:, k] = np.concatenate([r[:n].real, r[:n].imag, r[n:].real])
+    return M
+
+
+SPECTRUM_CASES = {
+    "hole_k4": dict(p=dict(Nx=8, Ny=8, Nz=4, hx=0.5, hy=0.5, hz=0.5, kappa=4.0),
+                    field=(0.0, 0.0, 0.4), seed=61,
+                    holes=[([(1.0, 1.0), (2.5, 1.0), (2.5, 2.5), (1.0, 2.5)], None)]),
+    "aniso_k5": dict(p=dict(Nx=7, Ny=6, Nz=5, hx=0.5, hy=0.7, hz=0.9, kappa=5.0),
+                     field=(0.11, -0.23, 0.37), seed=62, holes=[]),
+    "twod_k3": dict(p=dict(Nx=10, Ny=9, Nz=1, hx=0.5, hy=0.5, kappa=3.0),
+                    field=(0.0, 0.0, 0.45), seed=63, holes=[]),
+    "trilayer": dict(p=dict(Nx=6, Ny=6, Nz=7, hx=0.5, hy=0.5, hz=0.5, kappa=2.0),
+                     field=(0.0, 0.1, 0.35), seed=64, holes=[],
+                     trilayer=((2, 2.0, True), (3, 0.0, False), (2, 3.0, True))),
+    "periodic": dict(p=dict(Nx=6, Ny=7, Nz=5, hx=0.5, hy=0.6, hz=0.7, kappa=2.0,
+                            periodic_x=True, periodic_y=True, periodic_z=True),
+                     field=(0.1, 0.2, 0.3), seed=65, holes=[]),
+}
+
+
+@pytest.mark.parametrize("name", list(SPECTRUM_CASES))
+@pytest.mark.parametrize("dt", [0.05, 0.2, 1.0])
+def test_interval_contains_the_spectrum(name, dt):
+    from nanowire_tdgl_amd.engine import engine_for
+    case = SPECTRUM_CASES[name]
+    dev = device(case)
+    f = oracle_film(case)
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    eng.set_field(*case["field"])
+    u = O.boundary_vectors(*case["field"], f.g, f.idx)
+    n = dev.params.n_interior
+    nphi = 3 if dev.params.Nz > 1 else 2
+    # |psi| up to ~1.3: the bound assumes |psi| <= 2
+    x = _round32(rand_state(n, nphi, case["seed"], 0.3, 0.5))
+    A = lambda y: y - (dt / 2.0) * O.jvp_stencil(x, y, f.g, f.idx, u, f.material)  # noqa: E731
+    _, _, info = _apply(eng, x, _round32(rand_state(n, nphi, 3, 1.0, 1.0)), dt, -1)
+    ev = np.linalg.eigvals(_dense_real(A, n, nphi))
+    a, b = info["a"], info["b"]
+    print(f"{name} dt={dt}: eig real [{ev.real.min():.3f}, {ev.real.max():.2f}], "
+          f"max |imag| {np.abs(ev.imag).max():.3f}; interval [{a:.3f}, {b:.2f}], degree {info['degree']}")
+    assert ev.real.max() <= b
+    assert np.abs(ev).max() <= b
+    # eigenvalues below a are damped less, never flipped: they only have to stay positive
+    assert ev.real.min() > 0.0
+
+
+def _run_step(dev, X0, field, steps, dt, **kw):
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.engine import engine_for
+    import nanowire_tdgl_amd as T
+    eng = engine_for(dev.params, dev.idx, dev.material)
+    af = T.AppliedField(**field)
+    X = eng.ctx.to_internal(X0)
+    opts = _native.solver_opts(**kw)
+    stats = []
+    for s in range(steps):
+        eng.set_field(*af.evaluate(s * dt, steps * dt))
+        st = eng.ctx.trap_step(X, dt, opts)
+        assert st.converged == 1
+        stats.append(st.as_dict())
+    n = dev.params.n_interior
+    b, j = eng.ctx.observables(X)
+    psi = eng.ctx.to_reference(X).cpu().numpy()[:n]
+    return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy(), stats
+
+
+@pytest.mark.parametrize("precond", ["none", "chebyshev", 1, 2, 3, 9])
+def test_preconditioned_step_converges_to_the_fp64_solution(precond):
+    case = S.TRAP_HOLE
+    dev = device(case)
+    n = dev.params.n_interior
+    X0 = rand_state(n, 3, 31, 0.05, 0.02)
+    tight = dict(newton_tol_f=1e-10, newton_tol_dx=1e-6, tol_gcr=1e-4)
+    a = _run_step(dev, X0, case["field"], 3, case["dt"], precision="f64", **tight)
+    b = _run_step(dev, X0, case["field"], 3, case["dt"], precision="mixed", precond=precond, **tight)
+    for k, name in enumerate(("|psi|^2", "B_z", "J_s")):
+        err = rel_err(b[k], a[k])
+        print(f"precond={precond}: {name} relative difference {err:.2e}")
+        assert err < 1e-8, name
+    print("tgcr iterations f64/mixed:", sum(s["tgcr_iters_total"] for s in a[3]),
+          sum(s["tgcr_iters_total"] for s in b[3]),
+          "stencil sweeps mixed:", sum(s["jvp_evals"] for s in b[3]))
+
+
+def test_preconditioner_cuts_krylov_iterations():
+    case = S.TRAP_HOLE
+    dev = device(case)
+    n = dev.params.n_interior
+    X0 = rand_state(n, 3, 32, 0.05, 0.02)
+    kw = dict(precision="mixed", tol_gcr=1e-8)
+    none = _run_step(dev, X0, case["field"], 2, case["dt"], precond="none", **kw)[3]
+    cheb = _run_step(dev, X0, case["field"], 2, case["dt"], precond="chebyshev", **kw)[3]
+    k0 = sum(s["tgcr_iters_total"] for s in none)
+    k1 = sum(s["tgcr_iters_total"] for s in cheb)
+    print(f"TGCR iterations: none {k0}, chebyshev {k1}")
+    assert k1 * 3 < k0
+
+
+def test_preconditioner_rejects_bad_arguments():
+    import torch
+    from nanowire_tdgl_amd.engine import engine_for
+    dev = device(S.EVALF_CASES["c1_bz"])
+    ctx = engine_for(dev.params, dev.idx, dev.material).ctx
+    x = torch.zeros(ctx.len, dtype=torch.float32, device=ctx.device)
+    with pytest.raises(ValueError):
+        ctx.precond_cheb32(x, x, 0.2, 65, x.clone(), x.clone())
+    with pytest.raises(ValueError):
+        ctx.precond_cheb32(x, x, -0.2, 3, x.clone(), x.clone())
+    with pytest.raises(ValueError):  # dt/2 >= 0.9: the interval would reach 0
+        ctx.precond_cheb32(x, x, 1.9, 3, x.clone(), x.clone())
+    with pytest.raises(ValueError):
+        ctx.precond_cheb32(x.double(), x, 0.2, 3, x.clone(), x.clone())

@@ -40,8 +40,19 @@ def test_loader_sets_signatures():
     lib = _native.load()
     assert lib.tdgl_ctx_create.restype is ctypes.c_int
     assert ctypes.sizeof(_native.StepStats) == 8 * 4 + 5 * 8
-    assert len(_native.KERNEL_CLASSES) == 9
-    assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8
+    text = (ROOT / "include" / "tdgl_hip.h").read_text()
+    assert len(_native.KERNEL_CLASSES) == int(re.search(r"#define TDGL_K_COUNT (\d+)", text).group(1))
+    assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8 + 4 + 4
+
+
+def test_precond_option_codes():
+    assert _native.precond_code("chebyshev") == -1
+    assert _native.precond_code("none") == 0 and _native.precond_code(None) == 0
+    assert _native.precond_code(7) == 7
+    for bad in (0, -2, True, "cheb", 2.5):
+        with pytest.raises(ValueError):
+            _native.precond_code(bad)
+    assert _native.solver_opts(precision="mixed").mixed_precond == -1
 
 
 def test_struct_layout_matches_header():
