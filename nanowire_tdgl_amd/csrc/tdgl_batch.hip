@@ -140,11 +140,15 @@ int sync_out(tdgl_batch* B) {
   return TDGL_OK;
 }
 
+// z planes per stencil work item: deep enough that the two extra staged planes
+// are a small overhead (measured best at 8 on 256x256x64 films and on the
+// 64-film batch), shallow enough that small films still spread over the CUs
+// (2 on 128x128x16: 512 work items).
 int default_kz(const Geo& g) {
   if (!g.is3d) return 1;
   const long long tiles = (long long)((g.nx + kStencilTX - 1) / kStencilTX) * ((g.ny + kStencilTY - 1) / kStencilTY);
-  long long kz = tiles * g.nz / 2048;  // several blocks per CU for large films
-  kz = std::max(1LL, std::min<long long>(kz, 8));
+  long long kz = tiles * g.nz / 512;
+  kz = std::max(2LL, std::min<long long>(kz, 8));
   return (int)std::min<long long>(kz, g.nz);
 }
 

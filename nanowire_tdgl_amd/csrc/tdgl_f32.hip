@@ -37,6 +37,11 @@ namespace tdgl {
 
 namespace {
 constexpr int TX = 64, TY = 4;
+
+__device__ __forceinline__ void fast_sincos(float a, float& s, float& c) {
+  s = __sinf(a);
+  c = __cosf(a);
+}
 constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
 constexpr int BS = TX * TY;
 }  // namespace
@@ -51,8 +56,14 @@ constexpr int BS = TX * TY;
 // CHEB: the same stencil, with the epilogue of one Chebyshev step (ChebStep)
 // instead of Ap.  A = I - dt/2 J is linear, so A (s p) = s A p: the source is
 // staged unscaled and s enters in the epilogue.
+//
+// Occupancy: at most 168 VGPRs (3 waves per SIMD, which the 47.5 KB of LDS per
+// block also allows: 3 blocks per CU).  The link phases use the hardware
+// sine/cosine (v_sin_f32 / v_cos_f32 on x / 2 pi): absolute error ~1e-6 for the
+// phases met here, far inside what the inexact Newton iteration needs from the
+// JVP (its residual is FP64).
 template <bool IS3D, bool CHEB>
-__global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
+__global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
                                                float* const* Q, const int* iter, float* out_direct,
                                                ChebStep cs) {
   constexpr int NS = IS3D ? 3 : 1;
@@ -186,13 +197,13 @@ __global__ __launch_bounds__(BS) void k_bjvp32(BatchK bk, const float* x32, cons
       const float dex = P_x[s0][c0], dey = P_y[s0][c0], dez = IS3D ? P_z[s0][c0] : 0.f;
 
       float sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0.f, czp = 1.f, szm = 0.f, czm = 1.f;
-      sincosf(ex, &sxp, &cxp_);
-      sincosf(X_x[s0][cxm], &sxm, &cxm_);
-      sincosf(ey, &syp, &cyp_);
-      sincosf(X_y[s0][cym], &sym, &cym_);
+      fast_sincos(ex, sxp, cxp_);
+      fast_sincos(X_x[s0][cxm], sxm, cxm_);
+      fast_sincos(ey, syp, cyp_);
+      fast_sincos(X_y[s0][cym], sym, cym_);
       if (IS3D) {
-        sincosf(ez, &szp, &czp);
-        sincosf(X_z[sm][c0], &szm, &czm);
+        fast_sincos(ez, szp, czp);
+        fast_sincos(X_z[sm][c0], szm, czm);
       }
 
       // d[e^{-i t} psi(-a)] = e^{-i t}(d psi(-a) - i dt psi(-a)),
