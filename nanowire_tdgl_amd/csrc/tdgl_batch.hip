@@ -109,6 +109,10 @@ int balloc(tdgl_batch* B, void** p, size_t bytes) {
 }
 
 constexpr int kStencilTX = 64, kStencilTY = 4;
+
+// The FP32 kernels index a film's vectors with 32-bit byte offsets (8 bytes
+// per node for psi).
+bool fits_sweeps(const tdgl_ctx* c) { return c->g.n < (1LL << 29); }
 constexpr long long kStreamChunk = 4096;  // elements per streaming block (256 threads x 4 float4)
 constexpr int kStreamMaxPerFilm = 2048;
 
@@ -779,6 +783,10 @@ namespace tdgl {
 // The context's own batch of one: its profile and tuning are the context's.
 int own_batch(tdgl_ctx* ctx) {
   if (ctx->own_batch) return TDGL_OK;
+  if (!fits_sweeps(ctx)) {
+    ctx->err = "film too large for the mixed-precision kernels (n_interior >= 2^29)";
+    return TDGL_E_INVALID;
+  }
   tdgl_batch* B = new tdgl_batch();
   B->films.assign(1, ctx);
   B->nfilm = 1;
@@ -868,7 +876,7 @@ int tdgl_batch_create(tdgl_ctx* const* films, int nfilm, tdgl_batch** out) {
   *out = nullptr;
   const bool is3d = films[0] && films[0]->g.is3d;
   for (int f = 0; f < nfilm; ++f)
-    if (!films[f] || (films[f]->g.is3d != 0) != is3d) return TDGL_E_INVALID;
+    if (!films[f] || (films[f]->g.is3d != 0) != is3d || !fits_sweeps(films[f])) return TDGL_E_INVALID;
   tdgl_batch* B = new tdgl_batch();
   B->films.assign(films, films + nfilm);
   B->nfilm = nfilm;

@@ -21,11 +21,11 @@ constexpr int kRedThreads = 256;  // block size of every reducing kernel
 
 // Node indices are formed in 32-bit arithmetic: the host rejects grids whose
 // full-grid size or state length reaches 2^31.
-__device__ __forceinline__ long long qix(const Geo& g, int i, int j, int k) {
+__host__ __device__ __forceinline__ long long qix(const Geo& g, int i, int j, int k) {
   return (long long)((i - 1) + g.nx * ((j - 1) + g.ny * (g.is3d ? k - 1 : 0)));
 }
 
-__device__ __forceinline__ long long mfull(const Geo& g, int i, int j, int k) {
+__host__ __device__ __forceinline__ long long mfull(const Geo& g, int i, int j, int k) {
   return (long long)(i + g.mj * (j + (g.Ny + 1) * (g.is3d ? k : 0)));
 }
 
@@ -39,13 +39,13 @@ struct Src {
   const T* p;
   T eps;
   long long n;
-  __device__ __forceinline__ T ld(long long off) const {
+  __host__ __device__ __forceinline__ T ld(long long off) const {
     T v = x[off];
     if (PERT) v = v + eps * p[off];
     return v;
   }
   // psi(q) as one vector load (re, im are adjacent; buffers are 8-byte aligned per pair)
-  __device__ __forceinline__ void psi(long long q, T& re, T& im) const {
+  __host__ __device__ __forceinline__ void psi(long long q, T& re, T& im) const {
     using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const V2 a = reinterpret_cast<const V2*>(x)[q];
     re = a.x;
@@ -56,9 +56,9 @@ struct Src {
       im = im + eps * d.y;
     }
   }
-  __device__ __forceinline__ T phx(long long q) const { return ld(2 * n + q); }
-  __device__ __forceinline__ T phy(long long q) const { return ld(3 * n + q); }
-  __device__ __forceinline__ T phz(long long q) const { return ld(4 * n + q); }
+  __host__ __device__ __forceinline__ T phx(long long q) const { return ld(2 * n + q); }
+  __host__ __device__ __forceinline__ T phy(long long q) const { return ld(3 * n + q); }
+  __host__ __device__ __forceinline__ T phz(long long q) const { return ld(4 * n + q); }
 };
 
 // Effective (post-boundary-condition) values at one full-grid node.
@@ -70,15 +70,15 @@ struct Cell {
 // Field terms that the ghost rules add: cz = (Bz*hx)*hy, cy = (By*hz)*hx,
 // cx = (Bx*hy)*hz, exactly as rhs.py:95-127 groups them.
 template <typename T>
-__device__ __forceinline__ T fld_cz(const Geo& g, const Bnd& b, long long m) {
+__host__ __device__ __forceinline__ T fld_cz(const Geo& g, const Bnd& b, long long m) {
   return b.bz ? (T)((b.bz[m] * b.hx) * b.hy) : (T)b.cz;
 }
 template <typename T>
-__device__ __forceinline__ T fld_cy(const Geo& g, const Bnd& b, long long m) {
+__host__ __device__ __forceinline__ T fld_cy(const Geo& g, const Bnd& b, long long m) {
   return b.by ? (T)((b.by[m] * b.hz) * b.hx) : (T)b.cy;
 }
 template <typename T>
-__device__ __forceinline__ T fld_cx(const Geo& g, const Bnd& b, long long m) {
+__host__ __device__ __forceinline__ T fld_cx(const Geo& g, const Bnd& b, long long m) {
   return b.bx ? (T)((b.bx[m] * b.hy) * b.hz) : (T)b.cx;
 }
 
@@ -178,7 +178,7 @@ __device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, co
 // at the same node: the boundary rules are affine, so p's ghosts follow the
 // same rules without the field terms.  One pass over the index logic.
 template <typename T, bool IS3D>
-__device__ __forceinline__ void eff_cell_pair(const Geo& g, const Bnd& b, const uint8_t* flags,
+__host__ __device__ __forceinline__ void eff_cell_pair(const Geo& g, const Bnd& b, const uint8_t* flags,
                                               const Src<T, false>& sx, const Src<T, false>& sp,
                                               int i, int j, int k, Cell<T>& cx, Cell<T>& cp) {
   cx = Cell<T>{T(0), T(0), T(0), T(0), T(0)};

@@ -32,6 +32,7 @@
 
 #include "tdgl_device.hpp"
 #include "tdgl_launch.hpp"
+#include "tdgl_stage32.hpp"
 
 namespace tdgl {
 
@@ -80,21 +81,21 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   const Bnd b = fd->b;
   const Mat mt = fd->mt;
   const long long off = fd->off;
+  const unsigned int n = (unsigned int)g.n;  // the host keeps n below 2^29
   float* ap = (out_direct ? out_direct : Q[*iter]) + off;
-  const float* xs = x32 + off;
-  const float* ps = p32 + off;
+  const V32 X = planes32(x32 + off, n), P = planes32(p32 + off, n);
   const float hdt = (float)bk.dtv[2 * f + 1];
   float c_it = 0.f, c_om = 0.f, c_ss = 1.f, c_sp = 0.f;
-  const float* cv_v = nullptr;
-  const float* cv_p = nullptr;
+  const bool has_v = CHEB && cs.v, has_p = CHEB && cs.prev;
+  V32 CV{}, CP{};
   if (CHEB) {
     const double* cf = bk.cheb + (long long)f * kChebStride;
     c_it = (float)cf[0];
     c_om = (float)cf[cs.m];
     c_ss = cs.src_th ? c_it : 1.f;
     c_sp = cs.prev ? (c_om - 1.f) * (cs.prev_th ? c_it : 1.f) : 0.f;
-    if (cs.v) cv_v = cs.v + off;
-    if (cs.prev) cv_p = cs.prev + off;
+    if (has_v) CV = planes32(cs.v + off, n);
+    if (has_p) CP = planes32(cs.prev + off, n);
   }
   const int li = (int)L - fd->item0;
   const int tile_x = li % fd->tiles_x, rest = li / fd->tiles_x;
@@ -108,17 +109,13 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     kb = 1 + zc * fd->kz;
     ke = min(kb + fd->kz, g.Nz);
   }
-  const long long n = g.n;
-  const Src<float, false> sx{xs, nullptr, 0.f, n};
-  const Src<float, false> sp{ps, nullptr, 0.f, n};
 
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
   const bool has1 = hc1 < CELLS;
   Cell<float> vx0, vp0, vx1, vp1;
   auto stage = [&](int kp) {
-    eff_cell_pair<float, IS3D>(g, b, mt.flags, sx, sp, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp, vx0, vp0);
-    if (has1)
-      eff_cell_pair<float, IS3D>(g, b, mt.flags, sx, sp, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp, vx1, vp1);
+    cell_pair32<IS3D>(g, b, mt.flags, X, P, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp, vx0, vp0);
+    if (has1) cell_pair32<IS3D>(g, b, mt.flags, X, P, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp, vx1, vp1);
   };
   auto put = [&](int slot, int c, const Cell<float>& v, const Cell<float>& w) {
     X_r[slot][c] = v.pr;
@@ -156,7 +153,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     commit(IS3D ? (k + 1) % 3 : 0);
     __syncthreads();
     // own-node inputs, then the halo two planes ahead
-    long long q = 0;
+    unsigned int q = 0;
     uint8_t fl = 0;
     float scv = 1.f, k2v = 0.f;
     float2 p0 = make_float2(0.f, 0.f);
@@ -164,25 +161,25 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
     float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
     if (act) {
-      q = qix(g, i, j, k);
-      if (mt.flags) fl = mt.flags[mfull(g, i, j, k)];
-      if (mt.has_mat && !mt.sc_bits) scv = (float)mt.sc[q];
-      if (mt.k2) k2v = (float)mt.k2[q];
-      p0 = reinterpret_cast<const float2*>(ps)[q];
-      praw_x = ps[2 * n + q];
-      praw_y = ps[3 * n + q];
-      if (IS3D) praw_z = ps[4 * n + q];
-      if (CHEB && cv_v) {
-        v0 = reinterpret_cast<const float2*>(cv_v)[q];
-        v_x = cv_v[2 * n + q];
-        v_y = cv_v[3 * n + q];
-        if (IS3D) v_z = cv_v[4 * n + q];
+      q = qix32(g, i, j, k, IS3D);
+      if (mt.flags) fl = ldu(mt.flags, (unsigned int)mfull(g, i, j, k));
+      if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
+      if (mt.k2) k2v = (float)ldu(mt.k2, q);
+      p0 = ldu(P.psi, q);
+      praw_x = ldu(P.px, q);
+      praw_y = ldu(P.py, q);
+      if (IS3D) praw_z = ldu(P.pz, q);
+      if (has_v) {
+        v0 = ldu(CV.psi, q);
+        v_x = ldu(CV.px, q);
+        v_y = ldu(CV.py, q);
+        if (IS3D) v_z = ldu(CV.pz, q);
       }
-      if (CHEB && cv_p) {
-        w0 = reinterpret_cast<const float2*>(cv_p)[q];
-        w_x = cv_p[2 * n + q];
-        w_y = cv_p[3 * n + q];
-        if (IS3D) w_z = cv_p[4 * n + q];
+      if (has_p) {
+        w0 = ldu(CP.psi, q);
+        w_x = ldu(CP.px, q);
+        w_y = ldu(CP.py, q);
+        if (IS3D) w_z = ldu(CP.pz, q);
       }
     }
     if (IS3D && k + 2 <= ke) stage(k + 2);
@@ -290,16 +287,16 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
           const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
           return cs.fin ? vv - res : res;
         };
-        reinterpret_cast<float2*>(ap)[q] =
-            make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y));
-        ap[2 * n + q] = step(praw_x, jx, v_x, w_x);
-        ap[3 * n + q] = step(praw_y, jy, v_y, w_y);
-        if (IS3D) ap[4 * n + q] = step(praw_z, jz, v_z, w_z);
+        stu(reinterpret_cast<float2*>(ap), q,
+            make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
+        stu(ap + 2u * n, q, step(praw_x, jx, v_x, w_x));
+        stu(ap + 3u * n, q, step(praw_y, jy, v_y, w_y));
+        if (IS3D) stu(ap + 4u * n, q, step(praw_z, jz, v_z, w_z));
       } else {
-        reinterpret_cast<float2*>(ap)[q] = make_float2(p0.x - hdt * jr, p0.y - hdt * ji);
-        ap[2 * n + q] = praw_x - hdt * jx;
-        ap[3 * n + q] = praw_y - hdt * jy;
-        if (IS3D) ap[4 * n + q] = praw_z - hdt * jz;
+        stu(reinterpret_cast<float2*>(ap), q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
+        stu(ap + 2u * n, q, praw_x - hdt * jx);
+        stu(ap + 3u * n, q, praw_y - hdt * jy);
+        if (IS3D) stu(ap + 4u * n, q, praw_z - hdt * jz);
       }
     }
     __syncthreads();
