@@ -291,7 +291,9 @@ def section_batched(attempts, warmup, fp64_steps):
            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
            "mixed_batched_cell_steps_s": v, "mixed_batched_unpreconditioned_cell_steps_s": vu,
            "roofline": roof, "kernel_classes": pc,
-           "autotune": {"kz_policy_ms": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
+           "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16"}[tune["shape"]],
+                                   "kz": tune["kz"]},
+                        "jvp_ms_by_shape_and_kz": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
     if fp64_steps <= 0:
         return out
     # baseline: loop the plain FP64 step over the films, same steps from the same start

@@ -117,6 +117,11 @@ int tdgl_set_stream(tdgl_ctx* ctx, void* hip_stream);
  * (tiles are 64 x 4 x kz nodes) for this context's kernels, FP64 and mixed. */
 int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz);
 int tdgl_get_stencil_kz(const tdgl_ctx* ctx);
+/* Tuning (no counterpart in the reference): tile shape of the mixed-precision
+ * stencil sweeps, nodes per plane: 0 = 64 x 4 (default), 1 = 32 x 8,
+ * 2 = 16 x 16.  Results are the same bits for every shape. */
+int tdgl_set_sweep_tile(tdgl_ctx* ctx, int shape);
+int tdgl_get_sweep_tile(const tdgl_ctx* ctx);
 const char* tdgl_last_error(const tdgl_ctx* ctx);
 /* Number of doubles in one internal state buffer ((2 + n_phi) * n_interior). */
 int64_t tdgl_state_len(const tdgl_ctx* ctx);
@@ -245,6 +250,10 @@ int64_t tdgl_batch_offset(const tdgl_batch* batch, int film);
 /* Tuning: z planes per stencil work item of film `film` (the autotuner's knob). */
 int tdgl_batch_set_kz(tdgl_batch* batch, int film, int kz);
 int tdgl_batch_get_kz(const tdgl_batch* batch, int film);
+/* Tuning: tile shape of the batch's FP32 sweeps (as tdgl_set_sweep_tile; the
+ * autotuner's other knob). */
+int tdgl_batch_set_sweep_tile(tdgl_batch* batch, int shape);
+int tdgl_batch_get_sweep_tile(const tdgl_batch* batch);
 /* Average milliseconds of `reps` launches of the batch's FP32 JVP stencil over
  * its work buffers, as currently laid out (what the autotuner measures). */
 int tdgl_batch_time_stencil(tdgl_batch* batch, int reps, double* ms);

@@ -26,13 +26,14 @@ PREC_MIXED = 1
 
 EXPORTS = [
     "tdgl_ctx_create", "tdgl_ctx_destroy", "tdgl_set_stream", "tdgl_last_error",
-    "tdgl_set_stencil_kz", "tdgl_get_stencil_kz",
+    "tdgl_set_stencil_kz", "tdgl_get_stencil_kz", "tdgl_set_sweep_tile", "tdgl_get_sweep_tile",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
     "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
+    "tdgl_batch_set_sweep_tile", "tdgl_batch_get_sweep_tile",
     "tdgl_batch_time_stencil", "tdgl_batch_set_graphs",
     "tdgl_batch_trap_step", "tdgl_batch_profile_enable", "tdgl_batch_profile_read",
 ]
@@ -106,6 +107,10 @@ def load() -> C.CDLL:
         "tdgl_jvp_trap32": ([P, P, P, D, P], C.c_int),
         "tdgl_precond_cheb32": ([P, P, P, D, C.c_int, P, P, P], C.c_int),
         "tdgl_set_stencil_kz": ([P, C.c_int], C.c_int),
+        "tdgl_set_sweep_tile": ([P, C.c_int], C.c_int),
+        "tdgl_get_sweep_tile": ([P], C.c_int),
+        "tdgl_batch_set_sweep_tile": ([P, C.c_int], C.c_int),
+        "tdgl_batch_get_sweep_tile": ([P], C.c_int),
         "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
         "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
         "tdgl_get_stencil_kz": ([P], C.c_int),
@@ -198,6 +203,13 @@ class Context:
 
     def get_stencil_kz(self) -> int:
         return int(self.lib.tdgl_get_stencil_kz(self.h))
+
+    def set_sweep_tile(self, shape: int) -> None:
+        """Tile shape of the mixed-precision sweeps: 0 = 64x4, 1 = 32x8, 2 = 16x16."""
+        self._call("tdgl_set_sweep_tile", C.c_int(int(shape)))
+
+    def get_sweep_tile(self) -> int:
+        return int(self.lib.tdgl_get_sweep_tile(self.h))
 
     # -- buffers ---------------------------------------------------------
     def empty_state(self):
@@ -392,6 +404,12 @@ class Batch:
 
     def get_kz(self, f: int) -> int:
         return int(self.lib.tdgl_batch_get_kz(self.h, int(f)))
+
+    def set_tile(self, shape: int) -> None:
+        self._call("tdgl_batch_set_sweep_tile", C.c_int(int(shape)))
+
+    def get_tile(self) -> int:
+        return int(self.lib.tdgl_batch_get_sweep_tile(self.h))
 
     def set_graphs(self, on: bool) -> None:
         self._call("tdgl_batch_set_graphs", C.c_int(1 if on else 0))

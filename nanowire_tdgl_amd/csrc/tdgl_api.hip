@@ -267,6 +267,15 @@ int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz) {
 
 int tdgl_get_stencil_kz(const tdgl_ctx* ctx) { return ctx ? ctx->cfg.kz : -1; }
 
+int tdgl_set_sweep_tile(tdgl_ctx* ctx, int shape) {
+  if (!ctx || shape < 0 || shape >= kSweepShapes) return TDGL_E_INVALID;
+  ctx->tile_user = shape;
+  if (ctx->own_batch) return tdgl_batch_set_sweep_tile(ctx->own_batch, shape);
+  return TDGL_OK;
+}
+
+int tdgl_get_sweep_tile(const tdgl_ctx* ctx) { return ctx ? ctx->tile_user : -1; }
+
 int tdgl_set_stream(tdgl_ctx* ctx, void* s) {
   if (!ctx) return TDGL_E_INVALID;
   ctx->st = (hipStream_t)s;

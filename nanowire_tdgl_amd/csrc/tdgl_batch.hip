@@ -38,14 +38,16 @@ struct tdgl_batch {
 
   std::vector<FilmDesc> hf;  // host copy of the descriptors
   std::vector<int> kz;       // z planes per stencil block, per film
-  std::vector<int> item_film, sblk_film;
-  int nitems = 0, nsblk = 0;
+  int shape = 0;             // FP32 sweep tile shape
+  std::vector<int> item_film, item_film32, sblk_film;
+  int nitems = 0, nitems32 = 0, nsblk = 0;
   long long total_len = 0;  // elements of a batched state vector
   long long hlen = 0;       // doubles of the packed TGCR matrices
   std::vector<int> kcap;    // per film: min(n_state, kMaxKrylov)
 
   FilmDesc* d_films = nullptr;
   int* d_item_film = nullptr;
+  int* d_item_film32 = nullptr;
   int* d_sblk_film = nullptr;
   int* d_act = nullptr;
   double* d_res = nullptr;
@@ -120,6 +122,7 @@ BatchK kview(const tdgl_batch* B) {
   BatchK k;
   k.films = B->d_films;
   k.item_film = B->d_item_film;
+  k.item_film32 = B->d_item_film32;
   k.sblk_film = B->d_sblk_film;
   k.act = B->d_act;
   k.res = B->d_res;
@@ -127,6 +130,8 @@ BatchK kview(const tdgl_batch* B) {
   k.cheb = B->d_cheb;
   k.nfilm = B->nfilm;
   k.nitems = B->nitems;
+  k.nitems32 = B->nitems32;
+  k.shape = B->shape;
   k.nsblk = B->nsblk;
   return k;
 }
@@ -161,6 +166,7 @@ int build_layout(tdgl_batch* B) {
   const int F = B->nfilm;
   B->hf.assign(F, FilmDesc{});
   B->item_film.clear();
+  B->item_film32.clear();
   B->sblk_film.clear();
   long long off = 0, hoff = 0;
   for (int f = 0; f < F; ++f) {
@@ -179,6 +185,11 @@ int build_layout(tdgl_batch* B) {
     d.item0 = (int)B->item_film.size();
     d.nitems = d.tiles_x * d.tiles_y * d.tz;
     B->item_film.insert(B->item_film.end(), d.nitems, f);
+    d.tiles32_x = (c->g.nx + sweep_tx(B->shape) - 1) / sweep_tx(B->shape);
+    d.tiles32_y = (c->g.ny + sweep_ty(B->shape) - 1) / sweep_ty(B->shape);
+    d.item32_0 = (int)B->item_film32.size();
+    d.nitems32 = d.tiles32_x * d.tiles32_y * d.tz;
+    B->item_film32.insert(B->item_film32.end(), d.nitems32, f);
     d.sb0 = (int)B->sblk_film.size();
     d.nsb = (int)std::min<long long>(kStreamMaxPerFilm, std::max<long long>(1, (c->len + kStreamChunk - 1) / kStreamChunk));
     B->sblk_film.insert(B->sblk_film.end(), d.nsb, f);
@@ -193,17 +204,21 @@ int build_layout(tdgl_batch* B) {
     B->body = nullptr;
   }
   B->nitems = (int)B->item_film.size();
+  B->nitems32 = (int)B->item_film32.size();
   B->nsblk = (int)B->sblk_film.size();
-  for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
+  for (void* p : {(void*)B->d_item_film, (void*)B->d_item_film32, (void*)B->d_sblk_film, (void*)B->d_pitem,
+                  (void*)B->d_pm})
     if (p) BCHK(hipFree(p));
-  B->d_item_film = B->d_sblk_film = nullptr;
+  B->d_item_film = B->d_item_film32 = B->d_sblk_film = nullptr;
   B->d_pitem = B->d_pm = nullptr;
   int rc;
   if ((rc = balloc(B, (void**)&B->d_item_film, sizeof(int) * B->nitems))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_item_film32, sizeof(int) * B->nitems32))) return rc;
   if ((rc = balloc(B, (void**)&B->d_sblk_film, sizeof(int) * B->nsblk))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pitem, sizeof(double) * 3 * (size_t)B->nitems))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pm, sizeof(double) * (kMaxKrylov + 2) * (size_t)B->nsblk))) return rc;
   BCHK(hipMemcpy(B->d_item_film, B->item_film.data(), sizeof(int) * B->nitems, hipMemcpyHostToDevice));
+  BCHK(hipMemcpy(B->d_item_film32, B->item_film32.data(), sizeof(int) * B->nitems32, hipMemcpyHostToDevice));
   BCHK(hipMemcpy(B->d_sblk_film, B->sblk_film.data(), sizeof(int) * B->nsblk, hipMemcpyHostToDevice));
   return TDGL_OK;
 }
@@ -768,6 +783,7 @@ int batch_init(tdgl_batch* B) {
   BCHK(hipEventCreateWithFlags(&B->ev_act[1], hipEventDisableTiming));
   B->kz.resize(F);
   B->kcap.resize(F);
+  if (nfilm_is_own(B)) B->shape = B->films[0]->tile_user;
   for (int f = 0; f < F; ++f) {
     const Geo& g = B->films[f]->g;
     B->kz[f] = B->films[f]->kz_user > 0 && nfilm_is_own(B) ? B->films[f]->kz_user : default_kz(g);
@@ -899,7 +915,8 @@ int tdgl_batch_destroy(tdgl_batch* B) {
   for (hipEvent_t e : {B->ev_in, B->ev_out, B->ev_act[0], B->ev_act[1]})
     if (e) (void)hipEventDestroy(e);
   if (B->ps) (void)hipStreamDestroy(B->ps);
-  for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_act,
+  for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_item_film32, (void*)B->d_sblk_film,
+                  (void*)B->d_act,
                   (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
                   (void*)B->d_H, (void*)B->d_rep, (void*)B->w_x, (void*)B->w_gam, (void*)B->x32,
@@ -941,6 +958,15 @@ int tdgl_batch_get_kz(const tdgl_batch* B, int f) {
   if (!B || f < 0 || f >= B->nfilm) return -1;
   return B->kz[f];
 }
+
+int tdgl_batch_set_sweep_tile(tdgl_batch* B, int shape) {
+  if (!B || shape < 0 || shape >= kSweepShapes) return TDGL_E_INVALID;
+  B->shape = shape;
+  (void)hipStreamSynchronize(B->ps);
+  return build_layout(B);
+}
+
+int tdgl_batch_get_sweep_tile(const tdgl_batch* B) { return B ? B->shape : -1; }
 
 int tdgl_batch_time_stencil(tdgl_batch* B, int reps, double* ms) {
   if (!B || reps < 1 || !ms) return TDGL_E_INVALID;

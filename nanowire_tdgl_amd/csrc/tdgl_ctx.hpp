@@ -117,6 +117,7 @@ struct tdgl_ctx {
   // mixed precision: this film as a batch of one (created on first use)
   tdgl_batch* own_batch = nullptr;
   int kz_user = 0;    // tdgl_set_stencil_kz (0: the built-in choice)
+  int tile_user = 0;  // tdgl_set_sweep_tile (mixed-precision sweeps)
   double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
   double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
   double k2max = 0.0;         // largest kappa^2 of any node (preconditioner bounds)

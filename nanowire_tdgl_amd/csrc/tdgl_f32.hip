@@ -37,14 +37,12 @@
 namespace tdgl {
 
 namespace {
-constexpr int TX = 64, TY = 4;
 
 __device__ __forceinline__ void fast_sincos(float a, float& s, float& c) {
   s = __sinf(a);
   c = __cosf(a);
 }
-constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
-constexpr int BS = TX * TY;
+constexpr int BS = 256;  // every sweep tile shape holds 256 nodes per plane
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -63,10 +61,16 @@ constexpr int BS = TX * TY;
 // sine/cosine (v_sin_f32 / v_cos_f32 on x / 2 pi): absolute error ~1e-6 for the
 // phases met here, far inside what the inexact Newton iteration needs from the
 // JVP (its residual is FP64).
-template <bool IS3D, bool CHEB>
+//
+// TS: the tile shape (sweep_tx(TS) x sweep_ty(TS) nodes per plane); every node's
+// arithmetic is the same for every shape and depth.
+template <bool IS3D, bool CHEB, int TS>
 __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
                                                float* const* Q, const int* iter, float* out_direct,
                                                ChebStep cs) {
+  constexpr int TX = sweep_tx(TS), TY = sweep_ty(TS);
+  static_assert(TX * TY == BS, "a sweep tile holds one node per thread and plane");
+  constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
   constexpr int NS = IS3D ? 3 : 1;
   constexpr int NZ = IS3D ? 3 : 1;
   constexpr int CZ = IS3D ? CELLS : 1;
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   __shared__ float P_r[NS][CELLS], P_i[NS][CELLS], P_x[NS][CELLS], P_y[NS][CELLS], P_z[NZ][CZ];
 
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int f = bk.item_film[L];
+  const int f = bk.item_film32[L];
   if (!bk.act[f]) return;
   const FilmDesc* fd = bk.films + f;
   const Geo g = fd->g;  // by value: held in registers for the whole item
@@ -97,9 +101,9 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     if (has_v) CV = planes32(cs.v + off, n);
     if (has_p) CP = planes32(cs.prev + off, n);
   }
-  const int li = (int)L - fd->item0;
-  const int tile_x = li % fd->tiles_x, rest = li / fd->tiles_x;
-  const int tile_y = rest % fd->tiles_y, zc = rest / fd->tiles_y;
+  const int li = (int)L - fd->item32_0;
+  const int tile_x = li % fd->tiles32_x, rest = li / fd->tiles32_x;
+  const int tile_y = rest % fd->tiles32_y, zc = rest / fd->tiles32_y;
   const int i0 = 1 + tile_x * TX, j0 = 1 + tile_y * TY;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx, j = j0 + ty;
@@ -706,24 +710,26 @@ __global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, con
 // launchers
 // ---------------------------------------------------------------------------
 
+template <bool IS3D, bool CHEB>
+void launch_sweep(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,
+                  float* out, const ChebStep& cs, hipStream_t st) {
+  const dim3 gr(bk.nitems32), bl(BS);
+  switch (bk.shape) {
+    case 1: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 1>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    case 2: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 2>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    default: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 0>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+  }
+}
 void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                    const int* iter, float* out_direct, hipStream_t st) {
   const ChebStep none{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0};
-  if (is3d)
-    hipLaunchKernelGGL((k_bjvp32<true, false>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
-                       iter, out_direct, none);
-  else
-    hipLaunchKernelGGL((k_bjvp32<false, false>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
-                       iter, out_direct, none);
+  if (is3d) launch_sweep<true, false>(bk, x32, p32, Q, iter, out_direct, none, st);
+  else launch_sweep<false, false>(bk, x32, p32, Q, iter, out_direct, none, st);
 }
 void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                     const int* iter, const ChebStep& cs, hipStream_t st) {
-  if (is3d)
-    hipLaunchKernelGGL((k_bjvp32<true, true>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
-                       iter, cs.out, cs);
-  else
-    hipLaunchKernelGGL((k_bjvp32<false, true>), dim3(bk.nitems), dim3(BS), 0, st, bk, x32, p32, Q,
-                       iter, cs.out, cs);
+  if (is3d) launch_sweep<true, true>(bk, x32, p32, Q, iter, cs.out, cs, st);
+  else launch_sweep<false, true>(bk, x32, p32, Q, iter, cs.out, cs, st);
 }
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st) {

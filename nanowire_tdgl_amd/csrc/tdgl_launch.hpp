@@ -29,6 +29,13 @@ enum ResSlot : int {
   RS_BETA = 16,  // beta_j (mixed: first-pass coefficients; after the solve, the update's), j < kMaxKrylov
 };
 constexpr int kMaxKrylov = 2048;
+// Tile shapes of the FP32 sweeps (k_bjvp32), nodes per plane: 0 = 64 x 4,
+// 1 = 32 x 8, 2 = 16 x 16 (x kz planes).  Wide rows coalesce best on large
+// films; squarer tiles waste fewer lanes on narrow ones (the batch autotuner
+// picks).
+constexpr int kSweepShapes = 3;
+__host__ __device__ constexpr int sweep_tx(int s) { return s == 1 ? 32 : (s == 2 ? 16 : 64); }
+__host__ __device__ constexpr int sweep_ty(int s) { return s == 1 ? 8 : (s == 2 ? 16 : 4); }
 // Chebyshev polynomial preconditioner of the mixed TGCR: largest degree and
 // the per-film coefficient stride (1/theta, then omega_1 .. omega_d).
 constexpr int kMaxChebDeg = 64;
@@ -71,14 +78,17 @@ struct Epi {
 // Kernel-side view of a batch of films (all device pointers).
 struct BatchK {
   const FilmDesc* films;
-  const int* item_film;  // stencil block -> film
+  const int* item_film;  // FP64 stencil block -> film
+  const int* item_film32;  // FP32 sweep block -> film
   const int* sblk_film;  // streaming block -> film
   int* act;              // per film: takes part in the current stage
   double* res;           // nfilm * kResLen scalar slots
   const double* dtv;     // per film: dt, dt/2
   const double* cheb;    // per film, kChebStride doubles: 1/theta, omega_1 .. omega_d
   int nfilm;
-  int nitems;            // stencil blocks of the batch
+  int nitems;            // FP64 stencil blocks of the batch
+  int nitems32;          // FP32 sweep blocks of the batch
+  int shape;             // FP32 sweep tile shape (sweep_tx / sweep_ty)
   int nsblk;             // streaming blocks of the batch
 };
 

@@ -55,8 +55,10 @@ struct FilmDesc {
   Mat mt;
   long long off;                 // first element of the film in batched vectors (multiple of 64)
   long long len;                 // elements of one state vector: (2 + nphi) n
-  int tiles_x, tiles_y, kz, tz;  // stencil decomposition: tiles_x*tiles_y*tz work items
-  int item0, nitems;             // the film's stencil blocks
+  int tiles_x, tiles_y, kz, tz;  // FP64 stencil decomposition: tiles_x*tiles_y*tz work items
+  int item0, nitems;             // the film's FP64 stencil blocks
+  int tiles32_x, tiles32_y;      // FP32 sweep decomposition (its own tile shape, same kz)
+  int item32_0, nitems32;        // the film's FP32 sweep blocks
   int sb0, nsb;                  // the film's streaming blocks
   long long hoff;                // the film's packed triangular TGCR matrix in the batch's H buffer
 };

@@ -54,28 +54,36 @@ class BatchRunner:
         for d, e, t in zip(self.devices, self.engines, times):
             e.set_field(*d.applied_field.evaluate(t, t_stop))
 
-    def tune(self, X=None, candidates=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), reps: int = 5) -> dict:
-        """Autotuner: pick the z depth of the stencil tiles (64 x 4 x kz nodes).
+    def tune(self, X=None, candidates=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), shapes=(0, 1, 2),
+             reps: int = 5) -> dict:
+        """Autotuner: pick the tile of the mixed-precision sweeps -- its shape
+        (0 = 64x4, 1 = 32x8, 2 = 16x16 nodes per plane) and its z depth kz.
 
-        Every candidate policy kz_f = min(K, nz_f) (and the built-in default)
-        is timed on the batch's own FP32 JVP stencil; the fastest is kept.
-        Returns {"kz": [...], "ms": best, "trials": {K: ms}}."""
+        Every shape with every candidate depth policy kz_f = min(K, nz_f), and
+        the current setting, is timed on the batch's own FP32 JVP stencil; the
+        fastest is kept.  Returns {"kz": [...], "shape": s, "ms": best,
+        "trials": {"<shape>:<K>": ms}}."""
         b = self.batch
         F = b.nfilm
-        default = [b.get_kz(f) for f in range(F)]
-        trials = {"default": b.time_stencil(reps)}
-        best_ms, best = trials["default"], default
-        for K in candidates:
-            kzs = [max(1, min(K, e.params.Nz - 1 if e.params.Nz > 1 else 1)) for e in self.engines]
-            for f, k in enumerate(kzs):
-                b.set_kz(f, k)
-            ms = b.time_stencil(reps)
-            trials[K] = ms
-            if ms < best_ms:
-                best_ms, best = ms, kzs
-        for f, k in enumerate(best):
+        names = {0: "64x4", 1: "32x8", 2: "16x16"}
+        best = ([b.get_kz(f) for f in range(F)], b.get_tile())
+        best_ms = b.time_stencil(reps)
+        trials = {"default": best_ms}
+        for sh in shapes:
+            b.set_tile(sh)
+            for K in candidates:
+                kzs = [max(1, min(K, e.params.Nz - 1 if e.params.Nz > 1 else 1)) for e in self.engines]
+                for f, k in enumerate(kzs):
+                    b.set_kz(f, k)
+                ms = b.time_stencil(reps)
+                trials[f"{names[sh]}:{K}"] = ms
+                if ms < best_ms:
+                    best_ms, best = ms, (kzs, sh)
+        kzs, sh = best
+        b.set_tile(sh)
+        for f, k in enumerate(kzs):
             b.set_kz(f, k)
-        return {"kz": best, "ms": best_ms, "trials": trials}
+        return {"kz": kzs, "shape": sh, "ms": best_ms, "trials": trials}
 
 
 def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float, dt: float, *,

@@ -8,8 +8,9 @@
 * Films switched off for an attempt are left untouched.
 * Against FP64 steps of the same films: |psi|^2, B_z and J_s agree to 1e-6
   relative (of the largest entry) after several steps.
-* The stencil tiling (z depth per work item) changes no node's arithmetic in
-  the JVP stencil: its output is bitwise identical for every depth.
+* The stencil tiling (tile shape and z depth per work item) changes no node's
+  arithmetic in the JVP stencil: its output is bitwise identical for every
+  shape and depth.
 """
 
 from __future__ import annotations
@@ -149,15 +150,21 @@ def test_stencil_depth_does_not_change_the_jvp():
     x = 0.5 + 0.1 * torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32)
     p = torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32) - 0.5
     outs = []
-    for kz in (1, 3, 8):
-        out = torch.empty_like(x)
-        ctx.set_stencil_kz(kz)
-        assert ctx.get_stencil_kz() == kz
-        ctx.jvp_trap32(x, p, 0.2, out)
-        outs.append(out)
+    for shape in (0, 1, 2):
+        ctx.set_sweep_tile(shape)
+        assert ctx.get_sweep_tile() == shape
+        for kz in (1, 3, 8):
+            out = torch.empty_like(x)
+            ctx.set_stencil_kz(kz)
+            assert ctx.get_stencil_kz() == kz
+            ctx.jvp_trap32(x, p, 0.2, out)
+            outs.append(out)
     assert all(o.equal(outs[0]) for o in outs[1:])
+    with pytest.raises(ValueError):
+        ctx.set_sweep_tile(3)
     info = runner.tune(reps=2)
-    assert len(info["kz"]) == 3 and info["ms"] > 0
+    assert len(info["kz"]) == 3 and info["ms"] > 0 and info["shape"] in (0, 1, 2)
+    assert runner.batch.get_tile() == info["shape"]
 
 
 def test_graph_replay_matches_eager_launches():
