@@ -112,9 +112,9 @@ int balloc(tdgl_batch* B, void** p, size_t bytes) {
 
 constexpr int kStencilTX = 64, kStencilTY = 4;
 
-// The FP32 kernels index a film's vectors with 32-bit byte offsets (8 bytes
-// per node for psi).
-bool fits_sweeps(const tdgl_ctx* c) { return c->g.n < (1LL << 29); }
+// The FP32 sweeps address a film's vector with 32-bit byte offsets from one
+// base (20 bytes per node).
+bool fits_sweeps(const tdgl_ctx* c) { return c->g.n * 20LL < (1LL << 32); }
 constexpr long long kStreamChunk = 4096;  // elements per streaming block (256 threads x 4 float4)
 constexpr int kStreamMaxPerFilm = 2048;
 
@@ -800,7 +800,7 @@ namespace tdgl {
 int own_batch(tdgl_ctx* ctx) {
   if (ctx->own_batch) return TDGL_OK;
   if (!fits_sweeps(ctx)) {
-    ctx->err = "film too large for the mixed-precision kernels (n_interior >= 2^29)";
+    ctx->err = "film too large for the mixed-precision kernels (20 n_interior >= 2^32)";
     return TDGL_E_INVALID;
   }
   tdgl_batch* B = new tdgl_batch();

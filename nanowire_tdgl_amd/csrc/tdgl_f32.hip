@@ -85,8 +85,8 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   const Bnd b = fd->b;
   const Mat mt = fd->mt;
   const long long off = fd->off;
-  const unsigned int n = (unsigned int)g.n;  // the host keeps n below 2^29
-  float* ap = (out_direct ? out_direct : Q[*iter]) + off;
+  const unsigned int n = (unsigned int)g.n;  // the host keeps 20 n below 2^32
+  const W32 out = wplanes32((out_direct ? out_direct : Q[*iter]) + off, n);
   const V32 X = planes32(x32 + off, n), P = planes32(p32 + off, n);
   const float hdt = (float)bk.dtv[2 * f + 1];
   float c_it = 0.f, c_om = 0.f, c_ss = 1.f, c_sp = 0.f;
@@ -116,10 +116,14 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
 
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
   const bool has1 = hc1 < CELLS;
+  // the columns this thread stages on every plane
+  const Fld32 F = fld32(b, &fd->b);
+  const Col32 col0 = col32(g, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
+  const Col32 col1 = col32(g, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
   Cell<float> vx0, vp0, vx1, vp1;
   auto stage = [&](int kp) {
-    cell_pair32<IS3D>(g, b, mt.flags, X, P, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp, vx0, vp0);
-    if (has1) cell_pair32<IS3D>(g, b, mt.flags, X, P, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp, vx1, vp1);
+    stage32<IS3D>(g, F, mt.flags, X, P, col0, kp, vx0, vp0);
+    if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col1, kp, vx1, vp1);
   };
   auto put = [&](int slot, int c, const Cell<float>& v, const Cell<float>& w) {
     X_r[slot][c] = v.pr;
@@ -169,21 +173,21 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
       if (mt.flags) fl = ldu(mt.flags, (unsigned int)mfull(g, i, j, k));
       if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
       if (mt.k2) k2v = (float)ldu(mt.k2, q);
-      p0 = ldu(P.psi, q);
-      praw_x = ldu(P.px, q);
-      praw_y = ldu(P.py, q);
-      if (IS3D) praw_z = ldu(P.pz, q);
+      p0 = P.psi(q);
+      praw_x = P.ph(2, q);
+      praw_y = P.ph(3, q);
+      if (IS3D) praw_z = P.ph(4, q);
       if (has_v) {
-        v0 = ldu(CV.psi, q);
-        v_x = ldu(CV.px, q);
-        v_y = ldu(CV.py, q);
-        if (IS3D) v_z = ldu(CV.pz, q);
+        v0 = CV.psi(q);
+        v_x = CV.ph(2, q);
+        v_y = CV.ph(3, q);
+        if (IS3D) v_z = CV.ph(4, q);
       }
       if (has_p) {
-        w0 = ldu(CP.psi, q);
-        w_x = ldu(CP.px, q);
-        w_y = ldu(CP.py, q);
-        if (IS3D) w_z = ldu(CP.pz, q);
+        w0 = CP.psi(q);
+        w_x = CP.ph(2, q);
+        w_y = CP.ph(3, q);
+        if (IS3D) w_z = CP.ph(4, q);
       }
     }
     if (IS3D && k + 2 <= ke) stage(k + 2);
@@ -291,16 +295,15 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
           const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
           return cs.fin ? vv - res : res;
         };
-        stu(reinterpret_cast<float2*>(ap), q,
-            make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
-        stu(ap + 2u * n, q, step(praw_x, jx, v_x, w_x));
-        stu(ap + 3u * n, q, step(praw_y, jy, v_y, w_y));
-        if (IS3D) stu(ap + 4u * n, q, step(praw_z, jz, v_z, w_z));
+        out.psi(q, make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
+        out.ph(2, q, step(praw_x, jx, v_x, w_x));
+        out.ph(3, q, step(praw_y, jy, v_y, w_y));
+        if (IS3D) out.ph(4, q, step(praw_z, jz, v_z, w_z));
       } else {
-        stu(reinterpret_cast<float2*>(ap), q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
-        stu(ap + 2u * n, q, praw_x - hdt * jx);
-        stu(ap + 3u * n, q, praw_y - hdt * jy);
-        if (IS3D) stu(ap + 4u * n, q, praw_z - hdt * jz);
+        out.psi(q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
+        out.ph(2, q, praw_x - hdt * jx);
+        out.ph(3, q, praw_y - hdt * jy);
+        if (IS3D) out.ph(4, q, praw_z - hdt * jz);
       }
     }
     __syncthreads();

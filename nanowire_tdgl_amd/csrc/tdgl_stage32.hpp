@@ -2,7 +2,8 @@
 // eval_f sees at a full-grid node after _apply_boundary_conditions
 // (src/tdgl3d/physics/rhs.py:54-139), for a state x (with the applied field)
 // and a perturbation p (the same affine rules without the field), as
-// eff_cell_pair computes them, in a form with few branches and 32-bit indices.
+// eff_cell_pair computes them, split into a per-column plan (computed once per
+// thread) and a per-plane step, on 32-bit offsets.
 //
 // Every cell takes its values from one source node -- itself, its face-inner
 // neighbour (ghost cells) or its periodic image -- and then, per link
@@ -28,15 +29,36 @@ __host__ __device__ __forceinline__ void stu(T* base, unsigned int e, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + e * (unsigned int)sizeof(T)) = v;
 }
 
-// The four planes of one FP32 state-shaped vector of n nodes.
+// One FP32 state-shaped vector of n nodes: a base pointer and the byte size of
+// one phi plane (4 n); psi of node q is at byte 8 q, phi_x at 2 (4n) + 4 q,
+// phi_y at 3 (4n) + 4 q, phi_z at 4 (4n) + 4 q.  One 64-bit base per vector
+// keeps the kernels' uniform state small; the host keeps 5 (4n) below 2^32.
 struct V32 {
-  const float2* psi;
-  const float* px;
-  const float* py;
-  const float* pz;
+  const char* base;
+  unsigned int pl;  // 4 n
+  __host__ __device__ __forceinline__ float2 psi(unsigned int q) const {
+    return *reinterpret_cast<const float2*>(base + 8u * q);
+  }
+  __host__ __device__ __forceinline__ float ph(int a, unsigned int q) const {  // a = 2, 3, 4
+    return *reinterpret_cast<const float*>(base + (unsigned int)a * pl + 4u * q);
+  }
 };
 __host__ __device__ __forceinline__ V32 planes32(const float* v, unsigned int n) {
-  return V32{reinterpret_cast<const float2*>(v), v + 2u * n, v + 3u * n, v + 4u * n};
+  return V32{reinterpret_cast<const char*>(v), 4u * n};
+}
+// The same for an output vector.
+struct W32 {
+  char* base;
+  unsigned int pl;
+  __host__ __device__ __forceinline__ void psi(unsigned int q, float2 v) const {
+    *reinterpret_cast<float2*>(base + 8u * q) = v;
+  }
+  __host__ __device__ __forceinline__ void ph(int a, unsigned int q, float v) const {
+    *reinterpret_cast<float*>(base + (unsigned int)a * pl + 4u * q) = v;
+  }
+};
+__host__ __device__ __forceinline__ W32 wplanes32(float* v, unsigned int n) {
+  return W32{reinterpret_cast<char*>(v), 4u * n};
 }
 
 __host__ __device__ __forceinline__ unsigned int qix32(const Geo& g, int i, int j, int k, bool is3d) {
@@ -44,52 +66,82 @@ __host__ __device__ __forceinline__ unsigned int qix32(const Geo& g, int i, int 
                                                        (is3d ? (unsigned int)g.ny * (unsigned int)(k - 1) : 0u));
 }
 
+// Field terms of the ghost rules in FP32: uniform values converted once, or the
+// general BoundaryVectors (rhs.py:32-40), read through `gen` per ghost cell.
+struct Fld32 {
+  float cx, cy, cz;  // (float)b.cx ... (uniform field)
+  const Bnd* gen;    // the film's Bnd when it holds general field vectors, else nullptr
+};
+__host__ __device__ __forceinline__ Fld32 fld32(const Bnd& b, const Bnd* where) {
+  return Fld32{(float)b.cx, (float)b.cy, (float)b.cz, b.bz ? where : nullptr};
+}
+
+// The part of a cell's staging that depends only on its column (i, j): a
+// thread stages the same columns on every plane of its z march.
+struct Col32 {
+  unsigned int qij;  // (si - 1) + nx (sj - 1): source column in the interior numbering
+  unsigned int mij;  // i + mj j: the cell's column in the full-grid numbering
+  bool okij;         // inside the grid and not an xy edge or corner
+  bool fi, fj;       // on an x / y face
+  bool lo_i, lo_j;   // on the low x / y face (i == 0, j == 0)
+  bool zx, zy;       // the source's phi_x / phi_y is zeroed (normal planes)
+};
+
+__host__ __device__ __forceinline__ Col32 col32(const Geo& g, int i, int j) {
+  Col32 c;
+  c.fi = (i == 0) || (i == g.Nx);
+  c.fj = (j == 0) || (j == g.Ny);
+  c.lo_i = i == 0;
+  c.lo_j = j == 0;
+  c.okij = i >= 0 && i <= g.Nx && j >= 0 && j <= g.Ny && !(c.fi && c.fj);
+  int si = c.fi ? (g.per_x ? (i == 0 ? g.Nx - 1 : 1) : (i == 0 ? 1 : g.Nx - 1)) : i;
+  int sj = c.fj ? (g.per_y ? (j == 0 ? g.Ny - 1 : 1) : (j == 0 ? 1 : g.Ny - 1)) : j;
+  if (!c.okij) si = sj = 1;  // any node in range; the cell is zeroed
+  c.qij = (unsigned int)(si - 1) + (unsigned int)g.nx * (unsigned int)(sj - 1);
+  c.mij = c.okij ? (unsigned int)i + (unsigned int)g.mj * (unsigned int)j : 0u;
+  c.zx = si == g.Nx - 1;
+  c.zy = sj == g.Ny - 1;
+  return c;
+}
+
+// Stage the cell of column c on plane k (k = 0 in 2-D).
 template <bool IS3D>
-__host__ __device__ __forceinline__ void cell_pair32(const Geo& g, const Bnd& b, const uint8_t* flags,
-                                                     const V32& X, const V32& P, int i, int j, int k,
-                                                     Cell<float>& cx, Cell<float>& cp) {
-  const bool out = i < 0 || i > g.Nx || j < 0 || j > g.Ny || (IS3D && (k < 0 || k > g.Nz));
-  const bool fi = (i == 0) || (i == g.Nx);
-  const bool fj = (j == 0) || (j == g.Ny);
+__host__ __device__ __forceinline__ void stage32(const Geo& g, const Fld32& F, const uint8_t* flags,
+                                                 const V32& X, const V32& P, const Col32& c, int k,
+                                                 Cell<float>& cx, Cell<float>& cp) {
   const bool fk = IS3D && ((k == 0) || (k == g.Nz));
-  const bool ok = !out && ((int)fi + (int)fj + (int)fk) <= 1;
-  // source node
-  int si = fi ? (g.per_x ? (i == 0 ? g.Nx - 1 : 1) : (i == 0 ? 1 : g.Nx - 1)) : i;
-  int sj = fj ? (g.per_y ? (j == 0 ? g.Ny - 1 : 1) : (j == 0 ? 1 : g.Ny - 1)) : j;
+  const bool ok = c.okij && (!IS3D || (k >= 0 && k <= g.Nz)) && !(fk && (c.fi || c.fj));
   int sk = IS3D ? (fk ? (g.per_z ? (k == 0 ? g.Nz - 1 : 1) : (k == 0 ? 1 : g.Nz - 1)) : k) : 1;
-  if (!ok) si = sj = sk = 1;  // any node in range; the cell is zeroed below
-  const unsigned int q = qix32(g, si, sj, sk, IS3D);
-  const float2 xa = ldu(X.psi, q), pa = ldu(P.psi, q);
-  float xx = ldu(X.px, q), xy = ldu(X.py, q), xz = IS3D ? ldu(X.pz, q) : 0.f;
-  float px = ldu(P.px, q), py = ldu(P.py, q), pz = IS3D ? ldu(P.pz, q) : 0.f;
-  // normal zeroing at the source node
-  if (si == g.Nx - 1) xx = px = 0.f;
-  if (sj == g.Ny - 1) xy = py = 0.f;
+  if (!ok) sk = 1;
+  const unsigned int q = c.qij + (IS3D ? (unsigned int)g.nx * (unsigned int)g.ny * (unsigned int)(sk - 1) : 0u);
+  const float2 xa = X.psi(q), pa = P.psi(q);
+  float xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.f;
+  float px = P.ph(2, q), py = P.ph(3, q), pz = IS3D ? P.ph(4, q) : 0.f;
+  if (c.zx) xx = px = 0.f;
+  if (c.zy) xy = py = 0.f;
   if (IS3D && sk == g.Nz - 1) xz = pz = 0.f;
-  const long long m = ok ? mfull(g, i, j, k) : 0;
-  // field flux of the ghost rules (ghost cells only)
-  float fz = 0.f, fy = 0.f, fx = 0.f;
-  if (ok && (fi || fj || fk)) {
-    fz = fld_cz<float>(g, b, m);
-    if (IS3D) {
-      fy = fld_cy<float>(g, b, m);
-      fx = fld_cx<float>(g, b, m);
-    }
+  const unsigned int m = ok ? c.mij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u) : 0u;
+  float fz = F.cz, fy = F.cy, fx = F.cx;
+  if (F.gen && ok && (c.fi || c.fj || fk)) {
+    const Bnd& B = *F.gen;
+    fz = (float)((B.bz[m] * B.hx) * B.hy);
+    fy = (float)((B.by[m] * B.hz) * B.hx);
+    fx = (float)((B.bx[m] * B.hy) * B.hz);
   }
   float gx = xx, gy = xy, gz = xz, hx = px, hy = py, hz = pz;
-  if (fi) {
+  if (c.fi) {
     gx = g.per_x ? xx : 0.f;
     hx = g.per_x ? px : 0.f;
-    gy = g.per_x ? 0.f : xy + (i == 0 ? -fz : fz);
+    gy = g.per_x ? 0.f : xy + (c.lo_i ? -fz : fz);
     hy = g.per_x ? 0.f : py;
-    gz = g.per_x ? 0.f : xz + (i == 0 ? fy : -fy);
+    gz = g.per_x ? 0.f : xz + (c.lo_i ? fy : -fy);
     hz = g.per_x ? 0.f : pz;
-  } else if (fj) {
-    gx = g.per_y ? 0.f : xx + (j == 0 ? fz : -fz);
+  } else if (c.fj) {
+    gx = g.per_y ? 0.f : xx + (c.lo_j ? fz : -fz);
     hx = g.per_y ? 0.f : px;
     gy = g.per_y ? xy : 0.f;
     hy = g.per_y ? py : 0.f;
-    gz = g.per_y ? 0.f : xz + (j == 0 ? -fx : fx);
+    gz = g.per_y ? 0.f : xz + (c.lo_j ? -fx : fx);
     hz = g.per_y ? 0.f : pz;
   } else if (fk) {
     gx = g.per_z ? 0.f : xx + (k == 0 ? -fy : fy);
@@ -99,8 +151,8 @@ __host__ __device__ __forceinline__ void cell_pair32(const Geo& g, const Bnd& b,
     gz = g.per_z ? xz : 0.f;
     hz = g.per_z ? pz : 0.f;
   }
-  if (ok && flags) {  // hole-boundary links at the cell, applied last
-    const uint8_t f = flags[m];
+  if (ok && flags) {
+    const uint8_t f = ldu(flags, m);
     if (f & kHoleX) gx = hx = 0.f;
     if (f & kHoleY) gy = hy = 0.f;
     if (IS3D && (f & kHoleZ)) gz = hz = 0.f;
@@ -112,6 +164,15 @@ __host__ __device__ __forceinline__ void cell_pair32(const Geo& g, const Bnd& b,
   }
   cx = Cell<float>{xa.x, xa.y, gx, gy, IS3D ? gz : 0.f};
   cp = Cell<float>{pa.x, pa.y, hx, hy, IS3D ? hz : 0.f};
+}
+
+// One cell, column plan included (the CPU check compares this with
+// eff_cell_pair; kernels build the plan once per column).
+template <bool IS3D>
+__host__ __device__ __forceinline__ void cell_pair32(const Geo& g, const Bnd& b, const uint8_t* flags,
+                                                     const V32& X, const V32& P, int i, int j, int k,
+                                                     Cell<float>& cx, Cell<float>& cp) {
+  stage32<IS3D>(g, fld32(b, &b), flags, X, P, col32(g, i, j), k, cx, cp);
 }
 
 }  // namespace tdgl
