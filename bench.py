@@ -133,6 +133,17 @@ def pmc_traffic(profile: str) -> dict:
     return {}
 
 
+def with_traffic(roof, profile: str, section: str):
+    """Attach the committed PMC traffic of the section's dominant kernel class."""
+    if roof:
+        tr = pmc_traffic(profile).get(section, {}).get(roof["kernel"])
+        if tr:
+            roof["traffic"] = tr
+            roof["traffic_source"] = (f"profiles/{profile} (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, "
+                                      "calibrated on a 1 GiB copy)")
+    return roof
+
+
 def roofline_of(per_class: dict) -> dict:
     dom = max(per_class, key=lambda k: per_class[k]["ms"])
     c = per_class[dom]
@@ -183,6 +194,7 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
     and without the preconditioner, on the headline film and steps."""
     n = dev.params.n_interior
     s, el, pc, roof = timed_run(dev, x0, "mixed", steps, warmup, tol)
+    roof = with_traffic(roof, "cheb_sweeps_mixed_sections.md", "mixed")
     v, v64 = n * steps / el, n * steps / fp64_el
     err = rel_errors(s.observables(), fp64_stepper.observables())
     log(f"[bench] mixed: {v:.4g} cell-steps/s ({v / v64:.2f}x FP64), err {err}")
@@ -210,6 +222,7 @@ def section_large(steps, warmup):
     tol = 1e-4
     a, ela, _, roofa = timed_run(dev, x0, "f64", steps, warmup, tol, profile=False)
     b, elb, pcb, roofb = timed_run(dev, x0, "mixed", steps, warmup, tol)
+    roofb = with_traffic(roofb, "cheb_sweeps_mixed_sections.md", "large")
     c, elc, _, _ = timed_run(dev, x0, "mixed", steps, warmup, tol, profile=False, precond="none")
     va, vb, vc = n * steps / ela, n * steps / elb, n * steps / elc
     err = rel_errors(b.observables(), a.observables())
@@ -268,13 +281,18 @@ def section_batched(attempts, warmup, fp64_steps):
             attempt()
         pc = runner.batch.profile_read()
         runner.batch.profile_enable(False)
-        roof = roofline_of(pc)
+        roof = with_traffic(roofline_of(pc), "cheb_sweeps_mixed_sections.md", "batched")
     log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films ({el:.2f} s)")
-    # the same attempts without the preconditioner
+    # the same attempts without the preconditioner, after the same warm-up (its
+    # larger Krylov basis is allocated and its iteration graph captured there)
     Xp, tp = X.clone(), t.copy()
+    opts_none = _native.solver_opts(tol_gcr=tol, precision="mixed", precond="none")
     X.copy_(X0)
     t[:] = t0v
-    opts_none = _native.solver_opts(tol_gcr=tol, precision="mixed", precond="none")
+    for _ in range(max(warmup, 1)):
+        attempt(opts_none)
+    X.copy_(X0)
+    t[:] = t0v
     torch.cuda.synchronize()
     c0 = time.perf_counter()
     accu = 0.0

@@ -383,9 +383,6 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
     if (prof) B->prof->end(B->ps, tok);
     return;
   }
-  // bytes: x, source and output, plus prev (m >= 2) and v at the last sweep (m >= 3)
-  const double by = prof ? S * (3.0 * d + (d >= 2 ? d - 1.0 : 0.0) + (d >= 3 ? 1.0 : 0.0)) : 0.0;
-  const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, d) : -1;
   for (int m = 1; m <= d; ++m) {
     ChebStep cs{};
     cs.m = m;
@@ -394,9 +391,12 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
     cs.fin = m == d;
     cs.v = cs.fin ? v : nullptr;
     cs.out = cs.fin ? out : B->t32[(m - 1) % 2];
+    // bytes: x, source, output; prev (m >= 2); v at the last sweep unless it is prev (m >= 3)
+    const double by = S * (3.0 + (m >= 2 ? 1.0 : 0.0) + (cs.fin && m >= 3 ? 1.0 : 0.0));
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+    if (prof) B->prof->end(B->ps, tok);
   }
-  if (prof) B->prof->end(B->ps, tok);
 }
 
 // z = M u: the Chebyshev iterate z_d of A z = u (z_1 = u / theta,
@@ -409,8 +409,6 @@ const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, co
   if (d < 2) return nullptr;
   const BatchK bk = kview(B);
   const double S = prof ? sum_over(B, act, s32) : 0.0;
-  const double by = prof ? S * (3.0 * (d - 1) + (d >= 3 ? 1.0 : 0.0) + (d >= 4 ? 2.0 * (d - 3) : 0.0)) : 0.0;
-  const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, d - 1) : -1;
   for (int m = 2; m <= d; ++m) {
     ChebStep cs{};
     cs.m = m;
@@ -421,9 +419,12 @@ const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, co
     cs.prev = m == 2 ? nullptr : (m == 3 ? u : B->t32[m % 2]);
     cs.prev_th = m == 3;
     cs.out = B->t32[m % 2];
+    // bytes: x, source, output; u unless it is the source (m >= 3); prev unless it is u (m >= 4)
+    const double by = S * (3.0 + (m >= 3 ? 1.0 : 0.0) + (m >= 4 ? 1.0 : 0.0));
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+    if (prof) B->prof->end(B->ps, tok);
   }
-  if (prof) B->prof->end(B->ps, tok);
   return B->t32[d % 2];
 }
 

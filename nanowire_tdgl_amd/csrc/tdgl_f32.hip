@@ -145,9 +145,13 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   };
 
   if (IS3D) {
-    stage(kb - 1);
-    commit((kb - 1) % 3);
+    // the first two planes' loads go out together (one memory round trip)
+    Cell<float> ax0, ap0, ax1, ap1;
+    stage32<IS3D>(g, F, mt.flags, X, P, col0, kb - 1, ax0, ap0);
+    if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col1, kb - 1, ax1, ap1);
     stage(kb);
+    put((kb - 1) % 3, hc0, ax0, ap0);
+    if (has1) put((kb - 1) % 3, hc1, ax1, ap1);
     commit(kb % 3);
     stage(kb + 1);
   } else {

@@ -104,66 +104,105 @@ __host__ __device__ __forceinline__ Col32 col32(const Geo& g, int i, int j) {
   return c;
 }
 
+// Where the cell of column c on plane k takes its values from.
+struct Src32 {
+  unsigned int q;  // source node (interior numbering; any valid node when !ok)
+  unsigned int m;  // the cell in the full-grid numbering (0 when !ok)
+  bool ok;         // inside the grid and not an edge or corner
+  bool fk;         // on a z face
+  bool zz;         // the source's phi_z is zeroed (normal plane)
+};
+
+template <bool IS3D>
+__host__ __device__ __forceinline__ Src32 src32(const Geo& g, const Col32& c, int k) {
+  Src32 r;
+  r.fk = IS3D && ((k == 0) || (k == g.Nz));
+  r.ok = c.okij && (!IS3D || (k >= 0 && k <= g.Nz)) && !(r.fk && (c.fi || c.fj));
+  int sk = IS3D ? (r.fk ? (g.per_z ? (k == 0 ? g.Nz - 1 : 1) : (k == 0 ? 1 : g.Nz - 1)) : k) : 1;
+  if (!r.ok) sk = 1;
+  r.q = c.qij + (IS3D ? (unsigned int)g.nx * (unsigned int)g.ny * (unsigned int)(sk - 1) : 0u);
+  r.m = r.ok ? c.mij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u) : 0u;
+  r.zz = IS3D && sk == g.Nz - 1;
+  return r;
+}
+
+// The perturbation's link values of a cell from its source node's raw values:
+// normal zeroing at the source, the face rules without field (components the
+// ghost rules do not assign are 0; a periodic face copies its normal link),
+// and the cell's hole bits.  Valid for ok cells.
+template <bool IS3D>
+__host__ __device__ __forceinline__ void fix_p32(const Geo& g, const Col32& c, const Src32& r,
+                                                 uint8_t hole, float& px, float& py, float& pz) {
+  if (c.zx) px = 0.f;
+  if (c.zy) py = 0.f;
+  if (r.zz) pz = 0.f;
+  if (c.fi) {
+    px = g.per_x ? px : 0.f;
+    py = g.per_x ? 0.f : py;
+    pz = g.per_x ? 0.f : pz;
+  } else if (c.fj) {
+    px = g.per_y ? 0.f : px;
+    py = g.per_y ? py : 0.f;
+    pz = g.per_y ? 0.f : pz;
+  } else if (r.fk) {
+    px = g.per_z ? 0.f : px;
+    py = g.per_z ? 0.f : py;
+    pz = g.per_z ? pz : 0.f;
+  }
+  if (hole & kHoleX) px = 0.f;
+  if (hole & kHoleY) py = 0.f;
+  if (IS3D && (hole & kHoleZ)) pz = 0.f;
+  if (!IS3D) pz = 0.f;
+}
+
 // Stage the cell of column c on plane k (k = 0 in 2-D).
 template <bool IS3D>
 __host__ __device__ __forceinline__ void stage32(const Geo& g, const Fld32& F, const uint8_t* flags,
                                                  const V32& X, const V32& P, const Col32& c, int k,
                                                  Cell<float>& cx, Cell<float>& cp) {
-  const bool fk = IS3D && ((k == 0) || (k == g.Nz));
-  const bool ok = c.okij && (!IS3D || (k >= 0 && k <= g.Nz)) && !(fk && (c.fi || c.fj));
-  int sk = IS3D ? (fk ? (g.per_z ? (k == 0 ? g.Nz - 1 : 1) : (k == 0 ? 1 : g.Nz - 1)) : k) : 1;
-  if (!ok) sk = 1;
-  const unsigned int q = c.qij + (IS3D ? (unsigned int)g.nx * (unsigned int)g.ny * (unsigned int)(sk - 1) : 0u);
+  const Src32 r = src32<IS3D>(g, c, k);
+  const unsigned int q = r.q, m = r.m;
   const float2 xa = X.psi(q), pa = P.psi(q);
   float xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.f;
   float px = P.ph(2, q), py = P.ph(3, q), pz = IS3D ? P.ph(4, q) : 0.f;
-  if (c.zx) xx = px = 0.f;
-  if (c.zy) xy = py = 0.f;
-  if (IS3D && sk == g.Nz - 1) xz = pz = 0.f;
-  const unsigned int m = ok ? c.mij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u) : 0u;
+  uint8_t hole = 0;
+  if (r.ok && flags) hole = ldu(flags, m);
+  fix_p32<IS3D>(g, c, r, hole, px, py, pz);
+  // the state's links: the same rules plus the field flux of the ghost rules (rhs.py:95-127)
+  if (c.zx) xx = 0.f;
+  if (c.zy) xy = 0.f;
+  if (r.zz) xz = 0.f;
   float fz = F.cz, fy = F.cy, fx = F.cx;
-  if (F.gen && ok && (c.fi || c.fj || fk)) {
+  if (F.gen && r.ok && (c.fi || c.fj || r.fk)) {
     const Bnd& B = *F.gen;
     fz = (float)((B.bz[m] * B.hx) * B.hy);
     fy = (float)((B.by[m] * B.hz) * B.hx);
     fx = (float)((B.bx[m] * B.hy) * B.hz);
   }
-  float gx = xx, gy = xy, gz = xz, hx = px, hy = py, hz = pz;
+  float gx = xx, gy = xy, gz = xz;
   if (c.fi) {
     gx = g.per_x ? xx : 0.f;
-    hx = g.per_x ? px : 0.f;
     gy = g.per_x ? 0.f : xy + (c.lo_i ? -fz : fz);
-    hy = g.per_x ? 0.f : py;
     gz = g.per_x ? 0.f : xz + (c.lo_i ? fy : -fy);
-    hz = g.per_x ? 0.f : pz;
   } else if (c.fj) {
     gx = g.per_y ? 0.f : xx + (c.lo_j ? fz : -fz);
-    hx = g.per_y ? 0.f : px;
     gy = g.per_y ? xy : 0.f;
-    hy = g.per_y ? py : 0.f;
     gz = g.per_y ? 0.f : xz + (c.lo_j ? -fx : fx);
-    hz = g.per_y ? 0.f : pz;
-  } else if (fk) {
+  } else if (r.fk) {
     gx = g.per_z ? 0.f : xx + (k == 0 ? -fy : fy);
-    hx = g.per_z ? 0.f : px;
     gy = g.per_z ? 0.f : xy + (k == 0 ? fx : -fx);
-    hy = g.per_z ? 0.f : py;
     gz = g.per_z ? xz : 0.f;
-    hz = g.per_z ? pz : 0.f;
   }
-  if (ok && flags) {
-    const uint8_t f = ldu(flags, m);
-    if (f & kHoleX) gx = hx = 0.f;
-    if (f & kHoleY) gy = hy = 0.f;
-    if (IS3D && (f & kHoleZ)) gz = hz = 0.f;
-  }
-  if (!ok) {
+  if (hole & kHoleX) gx = 0.f;
+  if (hole & kHoleY) gy = 0.f;
+  if (IS3D && (hole & kHoleZ)) gz = 0.f;
+  if (!r.ok) {
     cx = Cell<float>{0.f, 0.f, 0.f, 0.f, 0.f};
     cp = cx;
     return;
   }
   cx = Cell<float>{xa.x, xa.y, gx, gy, IS3D ? gz : 0.f};
-  cp = Cell<float>{pa.x, pa.y, hx, hy, IS3D ? hz : 0.f};
+  cp = Cell<float>{pa.x, pa.y, px, py, IS3D ? pz : 0.f};
 }
 
 // One cell, column plan included (the CPU check compares this with

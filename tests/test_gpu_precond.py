@@ -203,6 +203,28 @@ def test_preconditioner_cuts_krylov_iterations():
     assert k1 * 3 < k0
 
 
+@pytest.mark.parametrize("name", ["hole_zr", "trilayer_hole", "periodic_xyz", "aniso_bxyz"])
+def test_sweep_tiles_agree_bitwise(name):
+    """The Chebyshev sweeps give the same bits for every tile shape and depth."""
+    import torch
+    dt = 0.2
+    eng, x, _, n, nphi = _setup(name, dt)
+    ctx = eng.ctx
+    v = _round32(rand_state(n, nphi, 53, 1.0, 1.0))
+    gx, gv = ctx.to_internal(x).float(), ctx.to_internal(v).float()
+    ref = None
+    for shape in (0, 1, 2):
+        for kz in (1, 2, 5):
+            ctx.set_sweep_tile(shape)
+            ctx.set_stencil_kz(kz)
+            w, z = torch.zeros_like(gx), torch.zeros_like(gx)
+            ctx.precond_cheb32(gx, gv, dt, 7, w, z)
+            if ref is None:
+                ref = (w, z)
+            assert w.equal(ref[0]) and z.equal(ref[1]), (shape, kz)
+    ctx.set_sweep_tile(0)
+
+
 def test_preconditioner_rejects_bad_arguments():
     import torch
     from nanowire_tdgl_amd.engine import engine_for
