@@ -38,7 +38,7 @@ struct tdgl_batch {
 
   std::vector<FilmDesc> hf;  // host copy of the descriptors
   std::vector<int> kz;       // z planes per stencil block, per film
-  int shape = 0;             // FP32 sweep tile shape
+  int shape = kDefaultSweepShape;  // FP32 sweep tile shape
   std::vector<int> item_film, item_film32, sblk_film;
   int nitems = 0, nitems32 = 0, nsblk = 0;
   long long total_len = 0;  // elements of a batched state vector
@@ -83,6 +83,8 @@ struct tdgl_batch {
   hipGraphExec_t body = nullptr;  // one TGCR iteration (i >= 1) for the current layout
   double body_tol = -1.0;
   int body_deg = -1;
+  int gen = 0;  // some film has general applied-field vectors (set by refresh)
+  int body_gen = -1;
 
   ProfSink own_prof;
   ProfSink* prof = &own_prof;
@@ -132,6 +134,7 @@ BatchK kview(const tdgl_batch* B) {
   k.nitems = B->nitems;
   k.nitems32 = B->nitems32;
   k.shape = B->shape;
+  k.gen = B->gen;
   k.nsblk = B->nsblk;
   return k;
 }
@@ -151,8 +154,9 @@ int sync_out(tdgl_batch* B) {
 
 // z planes per stencil work item: deep enough that the two extra staged planes
 // are a small overhead (measured best at 8 on 256x256x64 films and on the
-// 64-film batch), shallow enough that small films still spread over the CUs
-// (2 on 128x128x16: 512 work items).
+// 64-film batch; 16 and 4 are 10 % slower there inside a solve), shallow
+// enough that small films still spread over the CUs (2 on 128x128x16: 512
+// work items).
 int default_kz(const Geo& g) {
   if (!g.is3d) return 1;
   const long long tiles = (long long)((g.nx + kStencilTX - 1) / kStencilTX) * ((g.ny + kStencilTY - 1) / kStencilTY);
@@ -225,9 +229,11 @@ int build_layout(tdgl_batch* B) {
 
 // Current operators of the films (fields and materials may have changed).
 int refresh(tdgl_batch* B) {
+  B->gen = 0;
   for (int f = 0; f < B->nfilm; ++f) {
     B->hf[f].b = B->films[f]->b;
     B->hf[f].mt = B->films[f]->mt;
+    if (B->hf[f].b.bz) B->gen = 1;
   }
   BCHK(hipMemcpyAsync(B->d_films, B->hf.data(), sizeof(FilmDesc) * B->nfilm, hipMemcpyHostToDevice, B->ps));
   return TDGL_OK;
@@ -498,7 +504,7 @@ int launch_body(tdgl_batch* B, double tol) {
     BCHK(hipGetLastError());
     return TDGL_OK;
   }
-  if (B->body && (B->body_tol != tol || B->body_deg != B->cheb_deg)) {
+  if (B->body && (B->body_tol != tol || B->body_deg != B->cheb_deg || B->body_gen != B->gen)) {
     BCHK(hipGraphExecDestroy(B->body));
     B->body = nullptr;
   }
@@ -514,6 +520,7 @@ int launch_body(tdgl_batch* B, double tol) {
     BCHK(ei);
     B->body_tol = tol;
     B->body_deg = B->cheb_deg;
+    B->body_gen = B->gen;
   }
   BCHK(hipGraphLaunch(B->body, B->ps));
   return TDGL_OK;
@@ -980,8 +987,8 @@ int tdgl_batch_time_stencil(tdgl_batch* B, int reps, double* ms) {
   if ((rc = set_act(B, all))) return rc;
   BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * dtv.size(), hipMemcpyHostToDevice, B->ps));
   hipEvent_t a, b;
-  BCHK(hipEventCreate(&a));
-  BCHK(hipEventCreate(&b));
+  BCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+  BCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
   launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);  // warm-up
   BCHK(hipEventRecord(a, B->ps));
   for (int r = 0; r < reps; ++r)

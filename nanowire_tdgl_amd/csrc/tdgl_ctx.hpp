@@ -14,7 +14,10 @@
 namespace tdgl {
 
 // Per-kernel-class event timing: a HIP event pair around every bracketed
-// launch (or run of launches) on the library's stream.
+// launch (or run of launches) on the library's stream.  The events carry no
+// system-scope fence: with one, every record writes back and invalidates the
+// L2, which would be timed with the kernel it brackets (and slow the next
+// one).  They are read only after the stream has been synchronised.
 struct ProfSink {
   struct Pending {
     hipEvent_t a, b;
@@ -35,7 +38,8 @@ struct ProfSink {
     if (!pool.empty()) {
       ev = pool.back();
       pool.pop_back();
-    } else if (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess) {
+    } else if (hipEventCreateWithFlags(&ev.first, hipEventDisableSystemFence) != hipSuccess ||
+               hipEventCreateWithFlags(&ev.second, hipEventDisableSystemFence) != hipSuccess) {
       return -1;
     }
     if (hipEventRecord(ev.first, st) != hipSuccess) return -1;
@@ -117,7 +121,7 @@ struct tdgl_ctx {
   // mixed precision: this film as a batch of one (created on first use)
   tdgl_batch* own_batch = nullptr;
   int kz_user = 0;    // tdgl_set_stencil_kz (0: the built-in choice)
-  int tile_user = 0;  // tdgl_set_sweep_tile (mixed-precision sweeps)
+  int tile_user = tdgl::kDefaultSweepShape;  // tdgl_set_sweep_tile (mixed-precision sweeps)
   double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
   double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
   double k2max = 0.0;         // largest kappa^2 of any node (preconditioner bounds)

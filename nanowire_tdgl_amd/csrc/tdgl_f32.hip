@@ -63,8 +63,11 @@ constexpr int BS = 256;  // every sweep tile shape holds 256 nodes per plane
 // JVP (its residual is FP64).
 //
 // TS: the tile shape (sweep_tx(TS) x sweep_ty(TS) nodes per plane); every node's
-// arithmetic is the same for every shape and depth.
-template <bool IS3D, bool CHEB, int TS>
+// arithmetic is the same for every shape and depth.  GEN: some film of the
+// batch has general applied-field vectors; its cells are staged by the
+// per-cell rules (stage32), every other film's by the mask plans (stage32u),
+// which give the same bits.
+template <bool IS3D, bool CHEB, int TS, bool GEN>
 __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
                                                float* const* Q, const int* iter, float* out_direct,
                                                ChebStep cs) {
@@ -108,6 +111,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx, j = j0 + ty;
   const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
+  const unsigned int m_ij = (unsigned int)i + (unsigned int)g.mj * (unsigned int)j;  // full-grid column
   int kb = 0, ke = 1;
   if (IS3D) {
     kb = 1 + zc * fd->kz;
@@ -116,15 +120,34 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
 
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
   const bool has1 = hc1 < CELLS;
-  // the columns this thread stages on every plane
+  // the columns this thread stages on every plane, as mask plans (uniform
+  // field); a general applied field takes the per-cell rules, the column's
+  // booleans recomputed on every plane (held across the march, they would
+  // occupy scalar registers as lane masks)
   const Fld32 F = fld32(b, &fd->b);
-  const Col32 col0 = col32(g, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
-  const Col32 col1 = col32(g, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
-  Cell<float> vx0, vp0, vx1, vp1;
-  auto stage = [&](int kp) {
-    stage32<IS3D>(g, F, mt.flags, X, P, col0, kp, vx0, vp0);
-    if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col1, kp, vx1, vp1);
+  const Plan32 pl0 = plan32(g, F, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
+  const Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
+  auto stage_into = [&](int kp, Cell<float>& x0, Cell<float>& p0, Cell<float>& x1,
+                        Cell<float>& p1) __attribute__((always_inline)) {
+    if (GEN && F.gen) {
+      unsigned int h0 = hc0, h1 = hc1;
+      opaque(h0);
+      opaque(h1);
+      stage32<IS3D>(g, F, mt.flags, X, P, col32(g, i0 - 1 + h0 % CX, j0 - 1 + h0 / CX), kp, x0, p0);
+      if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col32(g, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX), kp, x1, p1);
+      return;
+    }
+    const Pl32 pp = plane32<IS3D>(g, F, kp);
+    if (IS3D && pp.fk) {
+      stage32u<IS3D, true>(mt.flags, X, P, pl0, pp, x0, p0);
+      if (has1) stage32u<IS3D, true>(mt.flags, X, P, pl1, pp, x1, p1);
+    } else {
+      stage32u<IS3D, false>(mt.flags, X, P, pl0, pp, x0, p0);
+      if (has1) stage32u<IS3D, false>(mt.flags, X, P, pl1, pp, x1, p1);
+    }
   };
+  Cell<float> vx0, vp0, vx1, vp1;
+  auto stage = [&](int kp) __attribute__((always_inline)) { stage_into(kp, vx0, vp0, vx1, vp1); };
   auto put = [&](int slot, int c, const Cell<float>& v, const Cell<float>& w) {
     X_r[slot][c] = v.pr;
     X_i[slot][c] = v.pi;
@@ -147,8 +170,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   if (IS3D) {
     // the first two planes' loads go out together (one memory round trip)
     Cell<float> ax0, ap0, ax1, ap1;
-    stage32<IS3D>(g, F, mt.flags, X, P, col0, kb - 1, ax0, ap0);
-    if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col1, kb - 1, ax1, ap1);
+    stage_into(kb - 1, ax0, ap0, ax1, ap1);
     stage(kb);
     put((kb - 1) % 3, hc0, ax0, ap0);
     if (has1) put((kb - 1) % 3, hc1, ax1, ap1);
@@ -174,7 +196,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
     if (act) {
       q = qix32(g, i, j, k, IS3D);
-      if (mt.flags) fl = ldu(mt.flags, (unsigned int)mfull(g, i, j, k));
+      if (mt.flags) fl = ldu(mt.flags, m_ij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u));
       if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
       if (mt.k2) k2v = (float)ldu(mt.k2, q);
       p0 = P.psi(q);
@@ -717,15 +739,21 @@ __global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, con
 // launchers
 // ---------------------------------------------------------------------------
 
+template <bool IS3D, bool CHEB, bool GEN>
+void launch_sweep_g(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,
+                    float* out, const ChebStep& cs, hipStream_t st) {
+  const dim3 gr(bk.nitems32), bl(BS);
+  switch (bk.shape) {
+    case 1: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 1, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    case 2: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 2, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    default: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 0, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+  }
+}
 template <bool IS3D, bool CHEB>
 void launch_sweep(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,
                   float* out, const ChebStep& cs, hipStream_t st) {
-  const dim3 gr(bk.nitems32), bl(BS);
-  switch (bk.shape) {
-    case 1: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 1>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    case 2: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 2>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    default: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 0>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-  }
+  if (bk.gen) launch_sweep_g<IS3D, CHEB, true>(bk, x32, p32, Q, iter, out, cs, st);
+  else launch_sweep_g<IS3D, CHEB, false>(bk, x32, p32, Q, iter, out, cs, st);
 }
 void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                    const int* iter, float* out_direct, hipStream_t st) {

@@ -32,8 +32,11 @@ constexpr int kMaxKrylov = 2048;
 // Tile shapes of the FP32 sweeps (k_bjvp32), nodes per plane: 0 = 64 x 4,
 // 1 = 32 x 8, 2 = 16 x 16 (x kz planes).  Wide rows coalesce best on large
 // films; squarer tiles waste fewer lanes on narrow ones (the batch autotuner
-// picks).
+// picks).  Inside a solve, 64 x 4 was the fastest on single films (128x128x16:
+// 15.5 us against 19.1 us per Chebyshev sweep for 32 x 8; equal on 256x256x64),
+// although 32 x 8 wins when the sweep is timed alone on cache-resident vectors.
 constexpr int kSweepShapes = 3;
+constexpr int kDefaultSweepShape = 0;
 __host__ __device__ constexpr int sweep_tx(int s) { return s == 1 ? 32 : (s == 2 ? 16 : 64); }
 __host__ __device__ constexpr int sweep_ty(int s) { return s == 1 ? 8 : (s == 2 ? 16 : 4); }
 // Chebyshev polynomial preconditioner of the mixed TGCR: largest degree and
@@ -89,6 +92,7 @@ struct BatchK {
   int nitems;            // FP64 stencil blocks of the batch
   int nitems32;          // FP32 sweep blocks of the batch
   int shape;             // FP32 sweep tile shape (sweep_tx / sweep_ty)
+  int gen;               // some film has general applied-field vectors (per-cell ghost rules)
   int nsblk;             // streaming blocks of the batch
 };
 

@@ -17,16 +17,35 @@
 
 namespace tdgl {
 
-// Element e of an array from a uniform base with a 32-bit byte offset (the
-// global_load saddr form: no 64-bit address arithmetic per lane).  Callers keep
-// arrays below 2^30 bytes / sizeof(T) elements.
+// Device code accesses the vectors through global-address-space pointers, so
+// that loads and stores take the global forms (a flat access, which a pointer
+// read from memory otherwise gets, also waits with the LDS traffic).  The CPU
+// build of these helpers (the staging check) uses plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define TDGL_GLOBAL __attribute__((address_space(1)))
+#else
+#define TDGL_GLOBAL
+#endif
+
+// A T at byte offset `off` from a uniform base.  The offset is one 32-bit sum
+// (the global_load saddr form: a 32-bit offset per lane, no 64-bit address
+// arithmetic, and no per-plane 64-bit bases hoisted into scalar registers).
+template <typename T>
+__host__ __device__ __forceinline__ T ldb(const void* base, unsigned int off) {
+  return *(const TDGL_GLOBAL T*)((const TDGL_GLOBAL char*)base + off);
+}
+template <typename T>
+__host__ __device__ __forceinline__ void stb(void* base, unsigned int off, T v) {
+  *(TDGL_GLOBAL T*)((TDGL_GLOBAL char*)base + off) = v;
+}
+// Element e of an array (callers keep arrays below 2^32 bytes).
 template <typename T>
 __host__ __device__ __forceinline__ T ldu(const T* base, unsigned int e) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + e * (unsigned int)sizeof(T));
+  return ldb<T>(base, e * (unsigned int)sizeof(T));
 }
 template <typename T>
 __host__ __device__ __forceinline__ void stu(T* base, unsigned int e, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + e * (unsigned int)sizeof(T)) = v;
+  stb<T>(base, e * (unsigned int)sizeof(T), v);
 }
 
 // One FP32 state-shaped vector of n nodes: a base pointer and the byte size of
@@ -36,11 +55,9 @@ __host__ __device__ __forceinline__ void stu(T* base, unsigned int e, T v) {
 struct V32 {
   const char* base;
   unsigned int pl;  // 4 n
-  __host__ __device__ __forceinline__ float2 psi(unsigned int q) const {
-    return *reinterpret_cast<const float2*>(base + 8u * q);
-  }
+  __host__ __device__ __forceinline__ float2 psi(unsigned int q) const { return ldb<float2>(base, 8u * q); }
   __host__ __device__ __forceinline__ float ph(int a, unsigned int q) const {  // a = 2, 3, 4
-    return *reinterpret_cast<const float*>(base + (unsigned int)a * pl + 4u * q);
+    return ldb<float>(base, (unsigned int)a * pl + 4u * q);
   }
 };
 __host__ __device__ __forceinline__ V32 planes32(const float* v, unsigned int n) {
@@ -50,11 +67,9 @@ __host__ __device__ __forceinline__ V32 planes32(const float* v, unsigned int n)
 struct W32 {
   char* base;
   unsigned int pl;
-  __host__ __device__ __forceinline__ void psi(unsigned int q, float2 v) const {
-    *reinterpret_cast<float2*>(base + 8u * q) = v;
-  }
+  __host__ __device__ __forceinline__ void psi(unsigned int q, float2 v) const { stb<float2>(base, 8u * q, v); }
   __host__ __device__ __forceinline__ void ph(int a, unsigned int q, float v) const {
-    *reinterpret_cast<float*>(base + (unsigned int)a * pl + 4u * q) = v;
+    stb<float>(base, (unsigned int)a * pl + 4u * q, v);
   }
 };
 __host__ __device__ __forceinline__ W32 wplanes32(float* v, unsigned int n) {
@@ -175,9 +190,9 @@ __host__ __device__ __forceinline__ void stage32(const Geo& g, const Fld32& F, c
   float fz = F.cz, fy = F.cy, fx = F.cx;
   if (F.gen && r.ok && (c.fi || c.fj || r.fk)) {
     const Bnd& B = *F.gen;
-    fz = (float)((B.bz[m] * B.hx) * B.hy);
-    fy = (float)((B.by[m] * B.hz) * B.hx);
-    fx = (float)((B.bx[m] * B.hy) * B.hz);
+    fz = (float)((ldu(B.bz, m) * B.hx) * B.hy);
+    fy = (float)((ldu(B.by, m) * B.hz) * B.hx);
+    fx = (float)((ldu(B.bx, m) * B.hy) * B.hz);
   }
   float gx = xx, gy = xy, gz = xz;
   if (c.fi) {
@@ -205,13 +220,166 @@ __host__ __device__ __forceinline__ void stage32(const Geo& g, const Fld32& F, c
   cp = Cell<float>{pa.x, pa.y, px, py, IS3D ? pz : 0.f};
 }
 
+// ---------------------------------------------------------------------------
+// The same staging for a uniform applied field, as masks.  Every rule above
+// either keeps a value, zeroes it, or adds a field flux, so a cell's link
+// component is g = (((x & s) + a) & k) bit for bit: s the source's normal
+// zeroing, a the flux (-0.f when none: x + -0.f == x for every x), k the face
+// rule, the hole bits and !ok.  The column's part of s, a and k is computed
+// once per column as 32-bit words and floats: lane values that stay in vector
+// registers across the z march, where the booleans of Col32 would be held as
+// 64-bit lane masks in scalar registers.  Planes on a z face replace the
+// column's face rule by the z one (only cells on no x / y face stay).
+// ---------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ float fmask(float x, unsigned int m) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(unsigned int, x) & m);
+}
+__host__ __device__ __forceinline__ unsigned int keep(bool b) { return b ? ~0u : 0u; }
+// Hide a lane value's origin from the optimiser, which would otherwise turn
+// x & keep(b) back into a select on b and hold b as a lane mask.
+__host__ __device__ __forceinline__ void opaque(unsigned int& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+}
+
+struct Plan32 {
+  unsigned int qij, mij;    // as in Col32
+  unsigned int ok;          // keep(okij): psi is kept
+  unsigned int sx, sy;      // keep(!zx), keep(!zy)
+  unsigned int kx, ky, kz;  // the x / y face rule and ok, per component
+  float ax, ay, az;         // flux the x / y face rule adds to the state's link (-0.f: none)
+};
+
+__host__ __device__ __forceinline__ Plan32 plan32(const Geo& g, const Fld32& F, int i, int j) {
+  const Col32 c = col32(g, i, j);
+  Plan32 p;
+  p.qij = c.qij;
+  p.mij = c.mij;
+  p.ok = keep(c.okij);
+  p.sx = keep(!c.zx);
+  p.sy = keep(!c.zy);
+  bool kx = true, ky = true, kz = true;
+  p.ax = p.ay = p.az = -0.f;
+  if (c.fi) {
+    kx = g.per_x;
+    ky = kz = !g.per_x;
+    if (!g.per_x) {
+      p.ay = c.lo_i ? -F.cz : F.cz;
+      p.az = c.lo_i ? F.cy : -F.cy;
+    }
+  } else if (c.fj) {
+    ky = g.per_y;
+    kx = kz = !g.per_y;
+    if (!g.per_y) {
+      p.ax = c.lo_j ? F.cz : -F.cz;
+      p.az = c.lo_j ? -F.cx : F.cx;
+    }
+  }
+  p.kx = keep(c.okij && kx);
+  p.ky = keep(c.okij && ky);
+  p.kz = keep(c.okij && kz);
+  opaque(p.ok);
+  opaque(p.sx);
+  opaque(p.sy);
+  opaque(p.kx);
+  opaque(p.ky);
+  opaque(p.kz);
+  return p;
+}
+
+// The plane's part (uniform over a block): source plane, z-normal zeroing, and
+// on a z face its rule (rhs.py:73-76, 95-127).  Planes 0 <= k <= Nz (3-D; the
+// sweeps stage no others), k = 0 in 2-D.
+struct Pl32 {
+  unsigned int qoff, moff;  // source plane offset (interior numbering), cell plane offset (full grid)
+  unsigned int sz;          // keep(!zz)
+  bool fk;                  // a z face
+  unsigned int kx, ky, kz;  // the z face rule per component (fk only)
+  float ax, ay;             // its flux (fk only)
+};
+
+template <bool IS3D>
+__host__ __device__ __forceinline__ Pl32 plane32(const Geo& g, const Fld32& F, int k) {
+  Pl32 p;
+  p.fk = IS3D && ((k == 0) || (k == g.Nz));
+  const int sk = IS3D ? (p.fk ? (g.per_z ? (k == 0 ? g.Nz - 1 : 1) : (k == 0 ? 1 : g.Nz - 1)) : k) : 1;
+  p.qoff = IS3D ? (unsigned int)g.nx * (unsigned int)g.ny * (unsigned int)(sk - 1) : 0u;
+  p.moff = IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u;
+  p.sz = keep(!(IS3D && sk == g.Nz - 1));
+  p.kx = p.ky = keep(!g.per_z);
+  p.kz = keep(g.per_z);
+  p.ax = g.per_z ? -0.f : (k == 0 ? -F.cy : F.cy);
+  p.ay = g.per_z ? -0.f : (k == 0 ? F.cx : -F.cx);
+  return p;
+}
+
+// Stage the cell of column plan c on plane p (uniform field).  FK: p is a z
+// face (the caller branches on p.fk, which is uniform).
+template <bool IS3D, bool FK>
+__host__ __device__ __forceinline__ void stage32u(const uint8_t* flags, const V32& X, const V32& P,
+                                                  const Plan32& c, const Pl32& p, Cell<float>& cx,
+                                                  Cell<float>& cp) {
+  const unsigned int q = c.qij + p.qoff;
+  unsigned int kx, ky, kz, ok;
+  float ax, ay, az;
+  if (FK) {  // only cells on no x / y face stay: their x / y rule keeps every component
+    ok = c.kx & c.ky & c.kz;
+    kx = ok & p.kx;
+    ky = ok & p.ky;
+    kz = ok & p.kz;
+    ax = p.ax;
+    ay = p.ay;
+    az = -0.f;
+  } else {
+    ok = c.ok;
+    kx = c.kx;
+    ky = c.ky;
+    kz = c.kz;
+    ax = c.ax;
+    ay = c.ay;
+    az = c.az;
+  }
+  const float2 xa = X.psi(q), pa = P.psi(q);
+  const float xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.f;
+  const float px = P.ph(2, q), py = P.ph(3, q), pz = IS3D ? P.ph(4, q) : 0.f;
+  if (flags) {
+    const uint8_t h = ldu(flags, c.mij + p.moff);
+    kx &= keep(!(h & kHoleX));
+    ky &= keep(!(h & kHoleY));
+    if (IS3D) kz &= keep(!(h & kHoleZ));
+  }
+  cx.pr = fmask(xa.x, ok);
+  cx.pi = fmask(xa.y, ok);
+  cx.x = fmask(fmask(xx, c.sx) + ax, kx);
+  cx.y = fmask(fmask(xy, c.sy) + ay, ky);
+  cx.z = IS3D ? fmask(fmask(xz, p.sz) + az, kz) : 0.f;
+  cp.pr = fmask(pa.x, ok);
+  cp.pi = fmask(pa.y, ok);
+  cp.x = fmask(px, c.sx & kx);
+  cp.y = fmask(py, c.sy & ky);
+  cp.z = IS3D ? fmask(pz, p.sz & kz) : 0.f;
+}
+
 // One cell, column plan included (the CPU check compares this with
-// eff_cell_pair; kernels build the plan once per column).
+// eff_cell_pair; kernels build the plan once per column).  `masks` picks the
+// uniform-field form (stage32u), which needs a uniform field.
 template <bool IS3D>
 __host__ __device__ __forceinline__ void cell_pair32(const Geo& g, const Bnd& b, const uint8_t* flags,
                                                      const V32& X, const V32& P, int i, int j, int k,
-                                                     Cell<float>& cx, Cell<float>& cp) {
-  stage32<IS3D>(g, fld32(b, &b), flags, X, P, col32(g, i, j), k, cx, cp);
+                                                     Cell<float>& cx, Cell<float>& cp, bool masks = false) {
+  const Fld32 F = fld32(b, &b);
+  if (masks && !F.gen) {
+    const Plan32 c = plan32(g, F, i, j);
+    const Pl32 p = plane32<IS3D>(g, F, k);
+    if (p.fk)
+      stage32u<IS3D, true>(flags, X, P, c, p, cx, cp);
+    else
+      stage32u<IS3D, false>(flags, X, P, c, p, cx, cp);
+    return;
+  }
+  stage32<IS3D>(g, F, flags, X, P, col32(g, i, j), k, cx, cp);
 }
 
 }  // namespace tdgl

@@ -2,8 +2,10 @@
 // must give bit for bit what eff_cell_pair gives for every cell of the
 // extended grid (ghost faces, edges, corners and cells outside the grid
 // included), on random 2-D and 3-D grids with periodic axes, uniform and
-// per-node applied fields and hole-boundary link bits.  Host code only:
-// hipcc compiles the __host__ __device__ functions for the CPU.
+// per-node applied fields and hole-boundary link bits; and the mask form of
+// the uniform-field staging (stage32u) must give exactly the bits of the
+// per-cell rules (stage32), signs of zeros included.  Host code only: hipcc
+// compiles the __host__ __device__ functions for the CPU.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -16,6 +18,7 @@
 using namespace tdgl;
 
 static bool same(float a, float b) { return memcmp(&a, &b, sizeof(float)) == 0 || (a == 0.f && b == 0.f); }
+static bool bits_equal(float a, float b) { return memcmp(&a, &b, sizeof(float)) == 0; }
 
 template <bool IS3D>
 static long long check(std::mt19937& rng, int trial) {
@@ -71,7 +74,7 @@ static long long check(std::mt19937& rng, int trial) {
   for (int k = IS3D ? -1 : 0; k <= (IS3D ? g.Nz + 1 : 0); ++k)
     for (int j = -1; j <= g.Ny + 1; ++j)
       for (int i = -1; i <= g.Nx + 1; ++i) {
-        Cell<float> ax, ap, cx, cp;
+        Cell<float> ax, ap, cx, cp, mx, mp;
         eff_cell_pair<float, IS3D>(g, b, fl, sx, sp, i, j, k, ax, ap);
         cell_pair32<IS3D>(g, b, fl, X, P, i, j, k, cx, cp);
         const float u[10] = {ax.pr, ax.pi, ax.x, ax.y, ax.z, ap.pr, ap.pi, ap.x, ap.y, ap.z};
@@ -84,6 +87,17 @@ static long long check(std::mt19937& rng, int trial) {
                      (int)IS3D, trial, g.Nx, g.Ny, g.Nz, g.per_x, g.per_y, g.per_z, i, j, k, c, u[c], w[c]);
             ++bad;
           }
+        if (!general && k >= 0 && k <= (IS3D ? g.Nz : 0)) {  // the planes a sweep stages
+          cell_pair32<IS3D>(g, b, fl, X, P, i, j, k, mx, mp, true);
+          const float v[10] = {mx.pr, mx.pi, mx.x, mx.y, mx.z, mp.pr, mp.pi, mp.x, mp.y, mp.z};
+          for (int c = 0; c < 10; ++c)
+            if (!bits_equal(w[c], v[c])) {
+              if (bad < 10)
+                printf("MASK MISMATCH 3d=%d trial=%d N=(%d,%d,%d) per=(%d,%d,%d) cell=(%d,%d,%d) comp=%d: %g vs %g\n",
+                       (int)IS3D, trial, g.Nx, g.Ny, g.Nz, g.per_x, g.per_y, g.per_z, i, j, k, c, w[c], v[c]);
+              ++bad;
+            }
+        }
       }
   return bad ? -bad : cells;
 }

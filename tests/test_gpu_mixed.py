@@ -62,6 +62,48 @@ def test_jvp32_matches_exact_jvp(name):
     assert err < 2e-6
 
 
+@pytest.mark.parametrize("name", ["c1_bz", "hole_bz", "aniso_bxyz", "periodic_xyz", "hole2d", "trilayer_hole"])
+def test_jvp32_general_field_vectors(name):
+    """The same applied field given as full-grid BoundaryVectors takes the
+    sweeps' per-cell ghost rules (the general-field kernels) instead of the
+    uniform-field masks: both agree with the oracle's exact JVP to 2e-6, and
+    with each other to 1e-6 (the flux is the same float at every face cell),
+    for the plain JVP and for one Chebyshev application."""
+    import torch
+    from nanowire_tdgl_amd.physics.applied_field import build_boundary_field_vectors
+    case = S.EVALF_CASES[name]
+    dev = device(case)
+    f = oracle_film(case)
+    eng = _engine(dev)
+    ctx = eng.ctx
+    u = O.boundary_vectors(*case["field"], f.g, f.idx)
+    n = dev.params.n_interior
+    nphi = 3 if dev.params.Nz > 1 else 2
+    x = _round32(rand_state(n, nphi, 31, 0.3, 0.2))
+    p = _round32(rand_state(n, nphi, 32, 1.0, 1.0))
+    gx, gp = ctx.to_internal(x).float(), ctx.to_internal(p).float()
+    dt = 0.2
+
+    def sweep():
+        ap = torch.empty_like(gx)
+        ctx.jvp_trap32(gx, gp, dt, ap)
+        w, z = torch.empty_like(gx), torch.empty_like(gx)
+        ctx.precond_cheb32(gx, gp, dt, 5, w, z)
+        return [ctx.to_reference(t.double()).cpu().numpy() for t in (ap, w, z)]
+
+    ctx.set_field(*case["field"])
+    uni = sweep()
+    Bx, By, Bz = (np.array(v) for v in build_boundary_field_vectors(*case["field"], dev.params, dev.idx))
+    ctx.set_field_vectors(Bx, By, Bz)
+    gen = sweep()
+    eng._field = None  # the engine's uniform-field cache no longer describes the context
+    want = p - (dt / 2.0) * O.jvp_stencil(x, p, f.g, f.idx, u, f.material)
+    errs = [rel_err(gen[0], want)] + [rel_err(a, b) for a, b in zip(gen, uni)]
+    print(f"{name}: general-field jvp32 vs oracle {errs[0]:.2e}; vs uniform {errs[1:]}")
+    assert errs[0] < 2e-6
+    assert max(errs[1:]) < 1e-6
+
+
 def test_jvp32_rejects_wrong_vectors():
     import torch
     dev = device(S.EVALF_CASES["c1_bz"])
