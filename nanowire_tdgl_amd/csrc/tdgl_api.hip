@@ -196,15 +196,15 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
     ctx->cfg.kz = (int)std::max(1LL, std::min<long long>(kz, g.nz));
   }
 
-  const int maxb = std::max(stencil_blocks(g, ctx->cfg), stream_blocks(g));
+  const int maxb = std::max({stencil_blocks(g, ctx->cfg), stream_blocks(g), mgs_blocks((2LL + g.nphi) * g.n)});
   ctx->pb_blocks = maxb;
   int rc;
-  if ((rc = alloc(ctx, (void**)&ctx->d_pb[0], sizeof(double) * 4 * (size_t)maxb))) goto bad;
-  if ((rc = alloc(ctx, (void**)&ctx->d_pb[1], sizeof(double) * 4 * (size_t)maxb))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_pb[0], sizeof(double) * kPartialStride * (size_t)maxb))) goto bad;
+  if ((rc = alloc(ctx, (void**)&ctx->d_pb[1], sizeof(double) * kPartialStride * (size_t)maxb))) goto bad;
   if ((rc = alloc(ctx, (void**)&ctx->d_res, sizeof(double) * kResLen))) goto bad;
   if ((rc = alloc(ctx, (void**)&ctx->d_bad, sizeof(int)))) goto bad;
-  if (hipMemset(ctx->d_pb[0], 0, sizeof(double) * 4 * (size_t)maxb) != hipSuccess ||
-      hipMemset(ctx->d_pb[1], 0, sizeof(double) * 4 * (size_t)maxb) != hipSuccess ||
+  if (hipMemset(ctx->d_pb[0], 0, sizeof(double) * kPartialStride * (size_t)maxb) != hipSuccess ||
+      hipMemset(ctx->d_pb[1], 0, sizeof(double) * kPartialStride * (size_t)maxb) != hipSuccess ||
       hipMemset(ctx->d_res, 0, sizeof(double) * kResLen) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_res, sizeof(double) * kResLen, hipHostMallocDefault) != hipSuccess) {
     rc = TDGL_E_HIP;
@@ -241,16 +241,16 @@ int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz) {
   const Geo& g = ctx->g;
   HIPCHK(hipStreamSynchronize(ctx->st));
   StencilCfg cfg{g.is3d ? std::min(kz, g.nz) : 1};
-  const int maxb = std::max(stencil_blocks(g, cfg), stream_blocks(g));
+  const int maxb = std::max({stencil_blocks(g, cfg), stream_blocks(g), mgs_blocks((2LL + g.nphi) * g.n)});
   if (maxb > ctx->pb_blocks) {  // more work items than partial slots: grow both buffers
     double* nb[2] = {nullptr, nullptr};
     for (int h = 0; h < 2; ++h) {
       int rc;
-      if ((rc = alloc(ctx, (void**)&nb[h], sizeof(double) * 4 * (size_t)maxb))) {
+      if ((rc = alloc(ctx, (void**)&nb[h], sizeof(double) * kPartialStride * (size_t)maxb))) {
         if (nb[0]) hipFree(nb[0]);
         return rc;
       }
-      HIPCHK(hipMemset(nb[h], 0, sizeof(double) * 4 * (size_t)maxb));
+      HIPCHK(hipMemset(nb[h], 0, sizeof(double) * kPartialStride * (size_t)maxb));
     }
     HIPCHK(hipFree(ctx->d_pb[0]));
     HIPCHK(hipFree(ctx->d_pb[1]));
@@ -635,20 +635,25 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     LAUNCHCHK();
     s->jvp_evals++;
     s->bytes_moved += by;
-    // the i-1 MGS passes run back to back: one event bracket around all of them
-    tok = (i > 1) ? prof_begin(ctx, TDGL_K_MGS, 4.0 * S * (i - 1), i - 1) : -1;
-    for (int j = 1; j < i; ++j) {  // read Ap, Ap_{j-1}, Ap_j; write Ap
-      red_commit(ctx, launch_mgs_pass(g, ctx->AP[i], ctx->AP[j - 1], RS_BETA + j - 1, ctx->AP[j],
-                                      red_next(ctx), ctx->st));
-      s->bytes_moved += 4.0 * S;
-    }
-    prof_end(ctx, tok);
-    if (i > 0) {  // read Ap, Ap_{i-1}; write Ap
-      tok = prof_begin(ctx, TDGL_K_MGS_LAST, 3.0 * S);
-      red_commit(ctx, launch_mgs_last(g, ctx->AP[i], ctx->AP[i - 1], RS_BETA + i - 1, red_next(ctx),
-                                      ctx->st));
+    // Gram-Schmidt against Ap_0 .. Ap_{i-1} (tgcr.py:135-140) in blocked
+    // passes: each subtracts the previous pass's new vectors (the first: Ap_0,
+    // whose beta the JVP reduced) and reduces the dots with the next
+    // kMgsBlock vectors; the last reduces ||Ap||^2.
+    for (int j = 0, no = 1, first = 1; i > 0; first = 0) {
+      const int nb = j + no;
+      const int nn = std::min(kMgsBlock, i - nb);
+      MgsVecs v{};
+      for (int l = 0; l < no; ++l) v.a[l] = ctx->AP[j + l];
+      for (int m = 0; m < nn; ++m) v.b[m] = ctx->AP[nb + m];
+      const double bym = (2.0 + no + nn) * S;  // read Ap, the old and new vectors; write Ap
+      tok = prof_begin(ctx, nn ? TDGL_K_MGS : TDGL_K_MGS_LAST, bym);
+      red_commit(ctx, launch_mgs_block(g, ctx->AP[i], v, no, nn, RS_BETA + j, first, red_next(ctx),
+                                       ctx->st));
       prof_end(ctx, tok);
-      s->bytes_moved += 3.0 * S;
+      s->bytes_moved += bym;
+      if (nn == 0) break;
+      j = nb;
+      no = nn;
     }
     // read r, p_0..p_{i-1}, Ap; write p_i, Ap
     tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);

@@ -312,12 +312,15 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
-// NS sums then NM maxima over the block (blockDim.x == kRedThreads); the
-// result is valid in every thread.
-template <int NS, int NM>
+// NS sums then NM maxima over the block (blockDim.x == BT); the result is
+// valid in every thread.  Wave results are combined in wave order by one
+// thread per quantity.
+template <int NS, int NM, int BT = kRedThreads>
 __device__ __forceinline__ void block_reduce(double (&v)[NS + NM]) {
   constexpr int NV = NS + NM;
-  __shared__ double sh[kRedThreads / 64][NV];
+  static_assert(NV <= BT, "one thread per reduced quantity");
+  __shared__ double sh[BT / 64][NV];
+  __shared__ double tot[NV];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int a = 0; a < NV; ++a) {
@@ -325,19 +328,22 @@ __device__ __forceinline__ void block_reduce(double (&v)[NS + NM]) {
     if (lane == 0) sh[wid][a] = r;
   }
   __syncthreads();
-#pragma unroll
-  for (int a = 0; a < NV; ++a) {
+  if (threadIdx.x < NV) {
+    const int a = threadIdx.x;
     double r = sh[0][a];
-    for (int w = 1; w < kRedThreads / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
-    v[a] = r;
+    for (int w = 1; w < BT / 64; ++w) r = (a < NS) ? r + sh[w][a] : fmax(r, sh[w][a]);
+    tot[a] = r;
   }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < NV; ++a) v[a] = tot[a];
   __syncthreads();
 }
 
 // Store this block's partials: pout[blockIdx.x * NV + a].
-template <int NS, int NM>
+template <int NS, int NM, int BT = kRedThreads>
 __device__ __forceinline__ void publish(double (&v)[NS + NM], double* pout) {
-  block_reduce<NS, NM>(v);
+  block_reduce<NS, NM, BT>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int a = 0; a < NS + NM; ++a) pout[(long long)blockIdx.x * (NS + NM) + a] = v[a];
@@ -355,19 +361,19 @@ __device__ __forceinline__ void publish_at(double (&v)[NS + NM], double* pout, l
 }
 
 // Reduce `count` published partials (block order); valid in every thread.
-template <int NS, int NM>
+template <int NS, int NM, int BT = kRedThreads>
 __device__ __forceinline__ void gather(const double* pin, int count, double (&out)[NS + NM]) {
   constexpr int NV = NS + NM;
 #pragma unroll
   for (int a = 0; a < NV; ++a) out[a] = 0.0;
-  for (int b = threadIdx.x; b < count; b += kRedThreads) {
+  for (int b = threadIdx.x; b < count; b += BT) {
 #pragma unroll
     for (int a = 0; a < NV; ++a) {
       const double r = pin[(long long)b * NV + a];
       out[a] = (a < NS) ? out[a] + r : fmax(out[a], r);
     }
   }
-  block_reduce<NS, NM>(out);
+  block_reduce<NS, NM, BT>(out);
 }
 
 // Block-index remap so that consecutive logical tiles land on the same XCD

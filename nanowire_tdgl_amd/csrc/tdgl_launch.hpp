@@ -57,11 +57,27 @@ enum StencilMode : int { M_RHS = 0, M_PRED = 1, M_NEWTON = 2, M_JVP = 3, M_EULER
 // Reduction plumbing of one launch: the partials it consumes in its prologue
 // (published by the previous launch), where it publishes its own, and the
 // scalar slots.
+// Blocked modified Gram-Schmidt of the FP64 TGCR: vectors per pass, and the
+// sums a pass publishes (kMgsBlock dots, then the Gram entries of its new
+// vectors, G_lm at mgs_g(l, m) for l < m).
+constexpr int kMgsBlock = 4;
+constexpr int kMgsSums = kMgsBlock + kMgsBlock * (kMgsBlock - 1) / 2;
+__host__ __device__ constexpr int mgs_g(int l, int m) { return kMgsBlock + m * (m - 1) / 2 + l; }
+// Partials per block of the largest publisher (the blocked MGS pass).
+constexpr int kPartialStride = kMgsSums;
+
 struct Red {
   const double* pin;  // partials to consume, or nullptr
   int pcount;         // number of partial blocks in pin
-  double* pout;       // this launch's partials (>= grid * 4 doubles)
+  double* pout;       // this launch's partials (>= grid * kPartialStride doubles)
   double* res;        // kResLen doubles
+};
+
+// Old vectors (their betas known) and new vectors (dots wanted) of one
+// blocked MGS pass.
+struct MgsVecs {
+  const double* a[kMgsBlock];
+  const double* b[kMgsBlock];
 };
 
 // Extra inputs/outputs of the stencil epilogue.
@@ -109,10 +125,12 @@ int launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& m
 
 // Krylov / vector kernels (state-shaped vectors, internal layout).  Each
 // returns the number of partials it publishes.
-int launch_mgs_pass(const Geo& g, double* ap, const double* ap_prev, int beta_prev_slot,
-                    const double* ap_next, const Red& r, hipStream_t st);
-int launch_mgs_last(const Geo& g, double* ap, const double* ap_prev, int beta_slot, const Red& r,
-                    hipStream_t st);
+// One blocked MGS pass: Ap -= sum_l beta_l a_l over `no` old vectors; sums of
+// <Ap, b_m> and <b_l, b_m> over `nn` new ones, or ||Ap||^2 when nn = 0.
+// first: the previous launch is the JVP (one partial: <Ap, a_0>).
+int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn, int beta_slot,
+                     bool first, const Red& r, hipStream_t st);
+int mgs_blocks(long long len);  // blocks of a blocked MGS pass over vectors of len doubles
 int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
                    double* pout, const Red& r, hipStream_t st);
 int launch_update_r(const Geo& g, double* rv, const double* ap, int alpha_slot, const Red& r,
