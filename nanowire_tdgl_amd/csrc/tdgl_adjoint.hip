@@ -381,24 +381,24 @@ __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const do
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
                       double* A, double* gsc, double kappa, double* pk, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) hipLaunchKernelGGL(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
-  else hipLaunchKernelGGL(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
+  if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
+  else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
 }
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) hipLaunchKernelGGL(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out);
-  else hipLaunchKernelGGL(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out);
+  if (g.is3d) launch(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out);
+  else launch(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out);
 }
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) hipLaunchKernelGGL(k_field_grad<true>, gr, bl, 0, st, g, flags, A, pout);
-  else hipLaunchKernelGGL(k_field_grad<false>, gr, bl, 0, st, g, flags, A, pout);
+  if (g.is3d) launch(k_field_grad<true>, gr, bl, 0, st, g, flags, A, pout);
+  else launch(k_field_grad<false>, gr, bl, 0, st, g, flags, A, pout);
   return stream_blocks(g);
 }
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) hipLaunchKernelGGL(k_bz_cells<true>, gr, bl, 0, st, g, b, gbz, A);
-  else hipLaunchKernelGGL(k_bz_cells<false>, gr, bl, 0, st, g, b, gbz, A);
+  if (g.is3d) launch(k_bz_cells<true>, gr, bl, 0, st, g, b, gbz, A);
+  else launch(k_bz_cells<false>, gr, bl, 0, st, g, b, gbz, A);
 }
 
 }  // namespace tdgl

@@ -19,6 +19,13 @@
 
 using namespace tdgl;
 
+namespace tdgl {
+KernelTiming& kernel_timing() {
+  thread_local KernelTiming t;
+  return t;
+}
+}  // namespace tdgl
+
 namespace {
 
 int fail(tdgl_ctx* c, int code, const std::string& msg) {
@@ -41,8 +48,9 @@ int alloc(tdgl_ctx* ctx, void** p, size_t bytes) {
   return TDGL_OK;
 }
 
+// Brackets of one kernel launch are timed on the kernel itself.
 int prof_begin(tdgl_ctx* ctx, int cls, double bytes, int launches = 1) {
-  return ctx->prof.begin(ctx->st, cls, bytes, launches);
+  return ctx->prof.begin(ctx->st, cls, bytes, launches, launches == 1);
 }
 void prof_end(tdgl_ctx* ctx, int tok) { ctx->prof.end(ctx->st, tok); }
 void prof_harvest(tdgl_ctx* ctx) { ctx->prof.harvest(); }

@@ -384,7 +384,7 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
   const int d = B->cheb_deg;
   const double S = prof ? sum_over(B, act, s32) : 0.0;
   if (d == 0) {
-    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S) : -1;
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S, 1, true) : -1;
     launch_bjvp32(bk, B->is3d, x32, v, B->d_qtab, B->d_iter, out, B->ps);
     if (prof) B->prof->end(B->ps, tok);
     return;
@@ -399,7 +399,7 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
     cs.out = cs.fin ? out : B->t32[(m - 1) % 2];
     // bytes: x, source, output; prev (m >= 2); v at the last sweep unless it is prev (m >= 3)
     const double by = S * (3.0 + (m >= 2 ? 1.0 : 0.0) + (cs.fin && m >= 3 ? 1.0 : 0.0));
-    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by) : -1;
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }
@@ -427,7 +427,7 @@ const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, co
     cs.out = B->t32[m % 2];
     // bytes: x, source, output; u unless it is the source (m >= 3); prev unless it is u (m >= 4)
     const double by = S * (3.0 + (m >= 3 ? 1.0 : 0.0) + (m >= 4 ? 1.0 : 0.0));
-    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by) : -1;
+    const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }

@@ -24,6 +24,8 @@ struct ProfSink {
     int cls;
     double bytes;
     int launches;  // consecutive launches of the class inside the bracket
+    bool kernel;   // the events ride on the bracket's one launch (KernelTiming)
+    bool valid;
   };
   bool on = false;
   std::vector<Pending> pending;
@@ -32,7 +34,9 @@ struct ProfSink {
   long long n[TDGL_K_COUNT] = {0};
   double bytes[TDGL_K_COUNT] = {0};
 
-  int begin(hipStream_t st, int cls, double by, int launches = 1) {
+  // kernel: the bracket holds exactly one launch made through launch(), which
+  // records the events at the kernel's start and end.
+  int begin(hipStream_t st, int cls, double by, int launches = 1, bool kernel = false) {
     if (!on) return -1;
     std::pair<hipEvent_t, hipEvent_t> ev;
     if (!pool.empty()) {
@@ -42,18 +46,34 @@ struct ProfSink {
                hipEventCreateWithFlags(&ev.second, hipEventDisableSystemFence) != hipSuccess) {
       return -1;
     }
-    if (hipEventRecord(ev.first, st) != hipSuccess) return -1;
-    pending.push_back({ev.first, ev.second, cls, by, launches});
+    if (kernel) {
+      KernelTiming& t = kernel_timing();
+      t.start = ev.first;
+      t.stop = ev.second;
+      t.armed = true;
+      t.used = false;
+    } else if (hipEventRecord(ev.first, st) != hipSuccess) {
+      return -1;
+    }
+    pending.push_back({ev.first, ev.second, cls, by, launches, kernel, true});
     return (int)pending.size() - 1;
   }
   void end(hipStream_t st, int tok) {
-    if (tok >= 0) (void)hipEventRecord(pending[tok].b, st);
+    if (tok < 0) return;
+    Pending& p = pending[tok];
+    if (p.kernel) {
+      KernelTiming& t = kernel_timing();
+      p.valid = t.used;  // no launch took the events: nothing was timed
+      t.armed = t.used = false;
+    } else {
+      (void)hipEventRecord(p.b, st);
+    }
   }
   // Fold finished brackets into the per-class totals (stream already synchronised).
   void harvest() {
     for (auto& p : pending) {
       float t = 0.f;
-      if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+      if (p.valid && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
         ms[p.cls] += t;
         n[p.cls] += p.launches;
         bytes[p.cls] += p.bytes;

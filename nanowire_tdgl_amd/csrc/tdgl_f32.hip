@@ -744,9 +744,9 @@ void launch_sweep_g(const BatchK& bk, const float* x32, const float* p32, float*
                     float* out, const ChebStep& cs, hipStream_t st) {
   const dim3 gr(bk.nitems32), bl(BS);
   switch (bk.shape) {
-    case 1: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 1, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    case 2: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 2, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    default: hipLaunchKernelGGL((k_bjvp32<IS3D, CHEB, 0, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    case 1: launch((k_bjvp32<IS3D, CHEB, 1, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    case 2: launch((k_bjvp32<IS3D, CHEB, 2, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    default: launch((k_bjvp32<IS3D, CHEB, 0, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
   }
 }
 template <bool IS3D, bool CHEB>
@@ -768,49 +768,49 @@ void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* 
 }
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st) {
-  hipLaunchKernelGGL(k_baccum_u, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, u, r0v, Q, fstat);
+  launch(k_baccum_u, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, u, r0v, Q, fstat);
 }
 void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_baddz, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, z, scale_th, pm);
+  launch(k_baddz, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, z, scale_th, pm);
 }
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                   double* pm, hipStream_t st) {
   const dim3 gr(bk.nsblk), bl(kRedThreads);
-  if (pass == 0) hipLaunchKernelGGL(k_bmdot<0>, gr, bl, 0, st, bk, Q, rv, iter, pm);
-  else if (pass == 1) hipLaunchKernelGGL(k_bmdot<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
-  else hipLaunchKernelGGL(k_bmdot<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  if (pass == 0) launch(k_bmdot<0>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else if (pass == 1) launch(k_bmdot<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else launch(k_bmdot<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
 }
 void launch_bupd(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                  double* pm, hipStream_t st) {
   const dim3 gr(bk.nsblk), bl(kRedThreads);
-  if (pass == 1) hipLaunchKernelGGL(k_bupd<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
-  else hipLaunchKernelGGL(k_bupd<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  if (pass == 1) launch(k_bupd<1>, gr, bl, 0, st, bk, Q, rv, iter, pm);
+  else launch(k_bupd<2>, gr, bl, 0, st, bk, Q, rv, iter, pm);
 }
 void launch_bfinish(const BatchK& bk, float* const* Q, float* rv, const int* iter, double* H,
                     double* pm, hipStream_t st) {
-  hipLaunchKernelGGL(k_bfinish, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, Q, rv, iter, H, pm);
+  launch(k_bfinish, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, Q, rv, iter, H, pm);
 }
 void launch_bcolsum(const BatchK& bk, const int* iter, const double* pm, int kind, int gy,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_bcolsum, dim3(bk.nfilm, gy), dim3(kRedThreads), 0, st, bk, iter, pm, kind);
+  launch(k_bcolsum, dim3(bk.nfilm, gy), dim3(kRedThreads), 0, st, bk, iter, pm, kind);
 }
 void launch_bstatus(const BatchK& bk, int* fstat, const int* iter, double tol, const double* pm,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_bstatus, dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, fstat, iter, tol, pm);
+  launch(k_bstatus, dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, fstat, iter, tol, pm);
 }
 void launch_bnext(const BatchK& bk, int* iter, int* nact, hipStream_t st) {
-  hipLaunchKernelGGL(k_bnext, dim3(1), dim3(256), 0, st, bk, iter, nact);
+  launch(k_bnext, dim3(1), dim3(256), 0, st, bk, iter, nact);
 }
 void launch_btinit(const BatchK& bk, int* fstat, const int* maxk, int* iter, hipStream_t st) {
-  hipLaunchKernelGGL(k_btinit, dim3(1), dim3(256), 0, st, bk, fstat, maxk, iter);
+  launch(k_btinit, dim3(1), dim3(256), 0, st, bk, fstat, maxk, iter);
 }
 void launch_btrisolve(const BatchK& bk, const int* fstat, const double* H, hipStream_t st) {
-  hipLaunchKernelGGL(k_btrisolve, dim3(bk.nfilm), dim3(1024), 0, st, bk, fstat, H);
+  launch(k_btrisolve, dim3(bk.nfilm), dim3(1024), 0, st, bk, fstat, H);
 }
 void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* Q, const int* fstat,
                    double* pm, hipStream_t st) {
-  hipLaunchKernelGGL(k_baccum, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, r0v, Q, fstat, pm);
+  launch(k_baccum, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, r0v, Q, fstat, pm);
 }
 
 }  // namespace tdgl

@@ -438,7 +438,7 @@ static int go_stencil(const Geo& g, const Bnd& b, const Mat& mt, const double* x
   Src<double, PERT> s{x, p, eps, g.n};
   const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
   const int nb = stencil_blocks(g, cfg);
-  hipLaunchKernelGGL((k_stencil<IS3D, PERT, MODE>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty,
+  launch((k_stencil<IS3D, PERT, MODE>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty,
                      cfg.kz, r);
   return nb;
 }
@@ -472,14 +472,14 @@ void launch_bstencil(const BatchK& bk, bool is3d, int mode, const double* x, dou
   const dim3 gr(bk.nitems), bl(BS);
   if (mode == M_PRED) {
     if (is3d)
-      hipLaunchKernelGGL((k_bstencil<true, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+      launch((k_bstencil<true, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
     else
-      hipLaunchKernelGGL((k_bstencil<false, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+      launch((k_bstencil<false, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
   } else {
     if (is3d)
-      hipLaunchKernelGGL((k_bstencil<true, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+      launch((k_bstencil<true, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
     else
-      hipLaunchKernelGGL((k_bstencil<false, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+      launch((k_bstencil<false, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
   }
 }
 
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kRedThreads) void k_bfin(BatchK bk, const double* p
 void launch_bfin(const BatchK& bk, int nsum, int nmax, const double* pin, int slot0, hipStream_t st) {
   (void)nsum;
   (void)nmax;
-  hipLaunchKernelGGL((k_bfin<2, 1>), dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, pin, slot0);
+  launch((k_bfin<2, 1>), dim3(bk.nfilm), dim3(kRedThreads), 0, st, bk, pin, slot0);
 }
 
 // Host report: per film xx, rr, ginf, dxinf, r0, rk, state, basis size, iterations, breakdown.
@@ -520,7 +520,7 @@ __global__ void k_breport(BatchK bk, const int* fstat, double* out) {
 }
 
 void launch_breport(const BatchK& bk, const int* fstat, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_breport, dim3(1), dim3(256), 0, st, bk, fstat, out);
+  launch(k_breport, dim3(1), dim3(256), 0, st, bk, fstat, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -598,7 +598,7 @@ int mgs_blocks(long long len) {
   return (int)(nb < 1 ? 1 : (nb < kMgsCap ? nb : kMgsCap));
 }
 
-template <int NVIN>
+template <int NVIN, bool LAST>
 __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double* ap, MgsVecs v, int no,
                                                            int nn, int bslot, Red r) {
   const long long n2 = len / 2;  // double2 elements; an odd last double is handled by thread 0 of block 0
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double
       }
     }
     X[e] = x;
-    if (nn == 0) {
+    if (LAST) {
       acc[0] += x.x * x.x;
       acc[0] += x.y * x.y;
       return;
@@ -700,7 +700,7 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double
 #pragma unroll
     for (int m = 0; m < kMgsBlock; ++m) b[m] = m < nn ? v.b[m][e] : 0.0;
     ap[e] = x;
-    if (nn == 0) {
+    if (LAST) {
       acc[0] += x * x;
     } else {
 #pragma unroll
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double
       }
     }
   }
-  if (nn == 0) {
+  if (LAST) {
     double t[1] = {acc[0]};
     publish<1, 0, kMgsThreads>(t, r.pout);
   } else {
@@ -836,17 +836,22 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(Red r, int slot0) {
 #define LAUNCH_V(kern, ...)                                                                 \
   do {                                                                                      \
     if (g.is3d)                                                                             \
-      hipLaunchKernelGGL(kern<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
+      launch(kern<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
     else                                                                                    \
-      hipLaunchKernelGGL(kern<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
+      launch(kern<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
   } while (0)
 
 int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn, int beta_slot,
                      bool first, const Red& r, hipStream_t st) {
   const long long len = (long long)(2 + g.nphi) * g.n;
   const dim3 gr(mgs_blocks(len)), bl(kMgsThreads);
-  if (first) hipLaunchKernelGGL((k_mgs_block<1>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
-  else hipLaunchKernelGGL((k_mgs_block<kMgsSums>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
+  if (first) {
+    if (nn) launch((k_mgs_block<1, false>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
+    else launch((k_mgs_block<1, true>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
+  } else {
+    if (nn) launch((k_mgs_block<kMgsSums, false>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
+    else launch((k_mgs_block<kMgsSums, true>), gr, bl, 0, st, len, ap, v, no, nn, beta_slot, r);
+  }
   return mgs_blocks(len);
 }
 int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
@@ -874,13 +879,13 @@ int launch_norm_inf(const Geo& g, const double* a, const Red& r, hipStream_t st)
 }
 void launch_finalize(int nsum, int nmax, int slot0, const Red& r, hipStream_t st) {
   if (nsum == 1 && nmax == 0)
-    hipLaunchKernelGGL((k_finalize<1, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+    launch((k_finalize<1, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
   else if (nsum == 0 && nmax == 1)
-    hipLaunchKernelGGL((k_finalize<0, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+    launch((k_finalize<0, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
   else if (nsum == 3 && nmax == 0)
-    hipLaunchKernelGGL((k_finalize<3, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+    launch((k_finalize<3, 0>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
   else
-    hipLaunchKernelGGL((k_finalize<2, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
+    launch((k_finalize<2, 1>), dim3(1), dim3(kRedThreads), 0, st, r, slot0);
 }
 
 // ---------------------------------------------------------------------------
@@ -921,11 +926,11 @@ __global__ void k_permute_out(Geo g, const double* in, double2* ref) {
 }
 
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st) {
-  hipLaunchKernelGGL(k_permute_in, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g,
+  launch(k_permute_in, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g,
                      static_cast<const double2*>(ref), st_out, bad);
 }
 void launch_permute_out(const Geo& g, const double* st_in, void* ref, hipStream_t st) {
-  hipLaunchKernelGGL(k_permute_out, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, st_in,
+  launch(k_permute_out, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, st_in,
                      static_cast<double2*>(ref));
 }
 
@@ -988,10 +993,10 @@ __global__ void k_obs(Geo g, Bnd b, const uint8_t* flags, const double* x, doubl
 void launch_observables(const Geo& g, const Bnd& b, const Mat& mt, const double* x, double* bout,
                         double* jout, hipStream_t st) {
   if (g.is3d)
-    hipLaunchKernelGGL(k_obs<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b, mt.flags,
+    launch(k_obs<true>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b, mt.flags,
                        x, bout, jout);
   else
-    hipLaunchKernelGGL(k_obs<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b,
+    launch(k_obs<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, g, b,
                        mt.flags, x, bout, jout);
 }
 

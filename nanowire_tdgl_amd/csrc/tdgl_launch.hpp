@@ -1,12 +1,35 @@
 // Host-side launchers for the TDGL kernels (implemented in tdgl_f64.hip).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "tdgl_types.hpp"
 
 namespace tdgl {
+
+// Kernel timing for the event profiler.  While armed, the next launch made
+// through launch() carries the two events of a profiling bracket
+// (hipExtLaunchKernelGGL): they then time that kernel alone, as a kernel
+// trace does, rather than the stream between two event records.
+struct KernelTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool armed = false, used = false;
+};
+KernelTiming& kernel_timing();  // the calling thread's
+
+template <typename F, typename... Args>
+inline void launch(F kern, dim3 gr, dim3 bl, unsigned int shmem, hipStream_t st, Args... args) {
+  KernelTiming& t = kernel_timing();
+  if (t.armed) {
+    t.armed = false;
+    t.used = true;
+    hipExtLaunchKernelGGL(kern, gr, bl, shmem, st, t.start, t.stop, 0, args...);
+  } else {
+    kern<<<gr, bl, shmem, st>>>(args...);
+  }
+}
 
 // Slots of the per-context device scalar array.
 enum ResSlot : int {
