@@ -214,7 +214,10 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
   if (hipMemset(ctx->d_pb[0], 0, sizeof(double) * kPartialStride * (size_t)maxb) != hipSuccess ||
       hipMemset(ctx->d_pb[1], 0, sizeof(double) * kPartialStride * (size_t)maxb) != hipSuccess ||
       hipMemset(ctx->d_res, 0, sizeof(double) * kResLen) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->h_res, sizeof(double) * kResLen, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->h_res, sizeof(double) * kResLen, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_la, sizeof(double) * 8, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_la[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_la[1], hipEventDisableTiming) != hipSuccess) {
     rc = TDGL_E_HIP;
     goto bad;
   }
@@ -240,6 +243,9 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
   if (ctx->d_acell) hipFree(ctx->d_acell);
   ctx->prof.destroy();
   if (ctx->h_res) hipHostFree(ctx->h_res);
+  if (ctx->h_la) hipHostFree(ctx->h_la);
+  for (hipEvent_t e : ctx->ev_la)
+    if (e) hipEventDestroy(e);
   delete ctx;
   return TDGL_OK;
 }
@@ -602,6 +608,20 @@ int newton_eval(tdgl_ctx* ctx, double hdt, tdgl_step_stats* s) {
   return TDGL_OK;
 }
 
+// Vectors the blocked Gram-Schmidt passes of TGCR iteration i move, in units
+// of S (each pass: Ap, its old and new vectors, Ap again).
+double mgs_vectors(int i) {
+  double v = 0.0;
+  for (int j = 0, no = 1; i > 0;) {
+    const int nb = j + no, nn = std::min(kMgsBlock, i - nb);
+    v += 2.0 + no + nn;
+    if (nn == 0) break;
+    j = nb;
+    no = nn;
+  }
+  return v;
+}
+
 // tgcr_matrix_free_trap (tgcr.py:96-156) on (I - dt/2 J) dx = r, r = -g held in
 // w_r, linearised at w_x with f(w_x) in w_f.  On success w_x += dx.
 // Returns TDGL_OK and sets *ok; *ok = 0 is the reference's empty-array return.
@@ -623,26 +643,32 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     return TDGL_OK;
   }
   const double xn = sqrt(xx);
-  while (rk / r0 > o->tol_gcr && k < maxk) {
-    ++k;
-    const int i = nvec;
-    if ((rc = ensure_vec(ctx, i))) return rc;
-    const double eps = o->eps_mf * (1.0 + xn) / rk;  // tgcr.py:129 (||pk|| == ||r||)
+  // Lookahead: without profiling, iteration k + 1 is queued before iteration
+  // k's residual comes back, so the GPU never waits for the host.  The device
+  // evaluates the loop's exit test itself (k_tgcr_check -> res[RS_STOP]); the
+  // kernels of an iteration queued after the exit return at once, so a
+  // speculative iteration leaves no trace.  eps (tgcr.py:129) is formed on the
+  // device from ||r||^2 with the host's expression.
+  const bool ahead = !ctx->prof.on;
+  HIPCHK(hipMemsetAsync(ctx->d_res + RS_STOP, 0, sizeof(double), ctx->st));
+  const double eps_num = o->eps_mf * (1.0 + xn);
+  long long queued = 0;
+  auto enqueue = [&](long long kk) -> int {
+    const int i = (int)(kk - 1);  // every earlier iteration added a vector
+    int rc2;
+    if ((rc2 = ensure_vec(ctx, i))) return rc2;
     Epi e{};
     e.out0 = ctx->AP[i];
     e.f0 = ctx->w_f;
     e.ap0 = (i > 0) ? ctx->AP[0] : nullptr;
     e.hdt = hdt;
-    e.inv_eps = 1.0 / eps;
+    e.tgcr = 1;
+    e.eps_num = eps_num;
     // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
-    double by = (i > 0 ? 5.0 : 4.0) * S;
-    int tok = prof_begin(ctx, TDGL_K_JVP, by);
-    red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, eps, e,
+    int tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S);
+    red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, 0.0, e,
                                    ctx->cfg, red_next(ctx), ctx->st));
     prof_end(ctx, tok);
-    LAUNCHCHK();
-    s->jvp_evals++;
-    s->bytes_moved += by;
     // Gram-Schmidt against Ap_0 .. Ap_{i-1} (tgcr.py:135-140) in blocked
     // passes: each subtracts the previous pass's new vectors (the first: Ap_0,
     // whose beta the JVP reduced) and reduces the dots with the next
@@ -653,12 +679,10 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
       MgsVecs v{};
       for (int l = 0; l < no; ++l) v.a[l] = ctx->AP[j + l];
       for (int m = 0; m < nn; ++m) v.b[m] = ctx->AP[nb + m];
-      const double bym = (2.0 + no + nn) * S;  // read Ap, the old and new vectors; write Ap
-      tok = prof_begin(ctx, nn ? TDGL_K_MGS : TDGL_K_MGS_LAST, bym);
+      tok = prof_begin(ctx, nn ? TDGL_K_MGS : TDGL_K_MGS_LAST, (2.0 + no + nn) * S);  // Ap, old, new; Ap
       red_commit(ctx, launch_mgs_block(g, ctx->AP[i], v, no, nn, RS_BETA + j, first, red_next(ctx),
                                        ctx->st));
       prof_end(ctx, tok);
-      s->bytes_moved += bym;
       if (nn == 0) break;
       j = nb;
       no = nn;
@@ -671,12 +695,27 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
     red_commit(ctx, launch_update_r(g, ctx->w_r, ctx->AP[i], RS_ALPHA + i, red_next(ctx), ctx->st));
     prof_end(ctx, tok);
-    red_finalize(ctx, 1, 0, RS_RR);
+    launch_tgcr_check(r0, o->tol_gcr, kk, maxk, red_next(ctx), ctx->st);
     LAUNCHCHK();
-    if ((rc = readback(ctx, RS_BRK, 3))) return rc;  // brk, (xx), rr
-    if (ctx->h_res[RS_BRK] != 0.0) break;  // tgcr.py:141-142
+    double* h = ctx->h_la + 4 * (kk & 1);
+    HIPCHK(hipMemcpyAsync(h, ctx->d_res + RS_BRK, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipEventRecord(ctx->ev_la[kk & 1], ctx->st));
+    queued = kk;
+    return TDGL_OK;
+  };
+  while (rk / r0 > o->tol_gcr && k < maxk) {
+    ++k;
+    const int i = nvec;
+    if (queued < k && (rc = enqueue(k))) return rc;
+    if (ahead && k < maxk && (rc = enqueue(k + 1))) return rc;
+    HIPCHK(hipEventSynchronize(ctx->ev_la[k & 1]));
+    if (!ctx->prof.pending.empty()) prof_harvest(ctx);
+    const double* h = ctx->h_la + 4 * (k & 1);  // brk, (xx), rr
+    s->jvp_evals++;
+    s->bytes_moved += ((i > 0 ? 5.0 : 4.0) + mgs_vectors(i)) * S;
+    if (h[0] != 0.0) break;  // tgcr.py:141-142
     s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
-    rk = sqrt(ctx->h_res[RS_RR]);
+    rk = sqrt(h[2]);
     nvec = i + 1;
     if (!std::isfinite(rk)) return fail(ctx, TDGL_E_NONFINITE, "TGCR residual is not finite");
   }

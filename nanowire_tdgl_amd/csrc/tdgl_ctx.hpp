@@ -127,6 +127,10 @@ struct tdgl_ctx {
   double* d_res = nullptr;                // scalar slots (kResLen)
   int* d_bad = nullptr;
   double* h_res = nullptr;  // pinned mirror of the scalar slots
+  // FP64 TGCR lookahead: per queued iteration (two in flight), the pinned
+  // copy of res[RS_BRK .. RS_RR] and the event that completes it
+  double* h_la = nullptr;
+  hipEvent_t ev_la[2] = {nullptr, nullptr};
 
   long long len = 0;  // doubles per state buffer
   double* w_x = nullptr;

@@ -383,6 +383,13 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
   const int tile_y = (int)(rest % (unsigned)tiles_y);
   const int zc = (int)(rest / (unsigned)tiles_y);
   using RN = RedN<MODE, PERT>;
+  if constexpr (MODE == M_JVP) {
+    if (e.tgcr) {  // tgcr.py:129 with ||p_k|| = ||r_k||, from the device's ||r||^2
+      if (r.res[RS_STOP] != 0.0) return;
+      s.eps = e.eps_num / sqrt(r.res[RS_RR]);
+      e.inv_eps = 1.0 / s.eps;
+    }
+  }
   double red[RN::NV];
 #pragma unroll
   for (int a = 0; a < RN::NV; ++a) red[a] = 0.0;
@@ -601,6 +608,7 @@ int mgs_blocks(long long len) {
 template <int NVIN, bool LAST>
 __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double* ap, MgsVecs v, int no,
                                                            int nn, int bslot, Red r) {
+  if (r.res[RS_STOP] != 0.0) return;  // a queued iteration after the loop ended
   const long long n2 = len / 2;  // double2 elements; an odd last double is handled by thread 0 of block 0
   const long long e0 = (long long)blockIdx.x * kMgsThreads + threadIdx.x;
   const long long stride = (long long)gridDim.x * kMgsThreads;
@@ -729,6 +737,7 @@ template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const double* rv,
                                                          const double* const* ptab, int nprev,
                                                          double* ap, double* pout, Red r) {
+  if (r.res[RS_STOP] != 0.0) return;  // a queued iteration after the loop ended
   double s1[1];
   gather<1, 0>(r.pin, r.pcount, s1);
   const double nrm = sqrt(s1[0]);
@@ -760,7 +769,7 @@ __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const doub
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_update_r(long long n, double* rv, const double* ap,
                                                           int aslot, Red r) {
-  if (r.res[RS_BRK] != 0.0) return;
+  if (r.res[RS_BRK] != 0.0 || r.res[RS_STOP] != 0.0) return;
   const long long q0 = (long long)blockIdx.x * kRedThreads + threadIdx.x;
   const long long stride = (long long)gridDim.x * kRedThreads;
   V5 r0{}, a0{};
@@ -840,6 +849,26 @@ __global__ __launch_bounds__(kRedThreads) void k_finalize(Red r, int slot0) {
     else                                                                                    \
       launch(kern<false>, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, __VA_ARGS__); \
   } while (0)
+
+// ||r||^2 into res[RS_RR]; res[RS_STOP] = the exit test of the TGCR loop
+// after iteration k (tgcr.py:124, 141-142: breakdown, ||r|| / ||r_0|| <= tol,
+// the iteration cap) or a non-finite residual, evaluated as the host does.
+__global__ __launch_bounds__(kRedThreads) void k_tgcr_check(Red r, double r0, double tol, long long k,
+                                                            long long maxk) {
+  if (r.res[RS_STOP] != 0.0) return;
+  double s1[1];
+  gather<1, 0>(r.pin, r.pcount, s1);
+  if (threadIdx.x == 0) {
+    const bool brk = r.res[RS_BRK] != 0.0;
+    r.res[RS_RR] = s1[0];
+    const double rk = sqrt(s1[0]);
+    const bool stop = brk || !isfinite(rk) || !(rk / r0 > tol) || k >= maxk;
+    r.res[RS_STOP] = stop ? 1.0 : 0.0;
+  }
+}
+void launch_tgcr_check(double r0, double tol, long long k, long long maxk, const Red& r, hipStream_t st) {
+  launch(k_tgcr_check, dim3(1), dim3(kRedThreads), 0, st, r, r0, tol, k, maxk);
+}
 
 int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn, int beta_slot,
                      bool first, const Red& r, hipStream_t st) {

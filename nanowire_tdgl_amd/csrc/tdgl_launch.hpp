@@ -68,7 +68,11 @@ constexpr int kMaxChebDeg = 64;
 constexpr int kChebStride = kMaxChebDeg + 2;
 constexpr int RS_ALPHA = RS_BETA + kMaxKrylov;
 constexpr int RS_DBETA = RS_ALPHA + kMaxKrylov;  // mixed: second-pass coefficients
-constexpr int kResLen = RS_DBETA + kMaxKrylov;
+// FP64 TGCR: 1.0 once the iteration loop has ended (converged, breakdown,
+// iteration cap, non-finite residual); the kernels of a later, speculatively
+// queued iteration then return at once.
+constexpr int RS_STOP = RS_DBETA + kMaxKrylov;
+constexpr int kResLen = RS_STOP + 1;
 
 // Column-sum kinds of the mixed TGCR (k_bcolsum).
 enum ColSum : int { CS_MD0 = 0, CS_MD1 = 1, CS_MD2 = 2, CS_UP1 = 3, CS_UP2 = 4, CS_DXINF = 5 };
@@ -115,6 +119,10 @@ struct Epi {
   double inv_eps;        // JVP: 1/eps
   float* o32a;           // NEWTON32: x in FP32
   float* o32b;           // NEWTON32: r = -g in FP32
+  // JVP inside TGCR: eps = eps_num / sqrt(res[RS_RR]) on the device (the
+  // host's formula, same rounding), and a return when res[RS_STOP] is set
+  int tgcr;
+  double eps_num;        // eps_mf * (1 + ||x||)
 };
 
 // Kernel-side view of a batch of films (all device pointers).
@@ -153,6 +161,9 @@ int launch_stencil(int mode, bool pert, const Geo& g, const Bnd& b, const Mat& m
 // first: the previous launch is the JVP (one partial: <Ap, a_0>).
 int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn, int beta_slot,
                      bool first, const Red& r, hipStream_t st);
+// End of one FP64 TGCR iteration: ||r||^2 from the previous partials into
+// res[RS_RR], and res[RS_STOP] = the loop's exit test for iteration k.
+void launch_tgcr_check(double r0, double tol, long long k, long long maxk, const Red& r, hipStream_t st);
 int mgs_blocks(long long len);  // blocks of a blocked MGS pass over vectors of len doubles
 int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
                    double* pout, const Red& r, hipStream_t st);
