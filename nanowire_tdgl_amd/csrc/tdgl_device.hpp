@@ -93,7 +93,7 @@ __host__ __device__ __forceinline__ T fld_cx(const Geo& g, const Bnd& b, long lo
 //  (3) edges and corners stay 0;
 //  (4) hole-boundary links := 0 last (rhs.py:132-137).
 template <typename T, bool IS3D, bool PERT>
-__device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, const Src<T, PERT>& s,
+__host__ __device__ Cell<T> eff_cell(const Geo& g, const Bnd& b, const uint8_t* flags, const Src<T, PERT>& s,
                             int i, int j, int k) {
   Cell<T> c{T(0), T(0), T(0), T(0), T(0)};
   if (i < 0 || i > g.Nx || j < 0 || j > g.Ny) return c;

@@ -22,6 +22,7 @@
 
 #include "tdgl_device.hpp"
 #include "tdgl_launch.hpp"
+#include "tdgl_stage32.hpp"
 
 namespace tdgl {
 
@@ -68,11 +69,29 @@ struct RedN {
   static constexpr int NV = NS + NM > 0 ? NS + NM : 1;
 };
 
+// One FP64 output vector on 32-bit byte offsets (as V64).
+struct W64 {
+  char* base;
+  unsigned int pl;  // 8 n
+  __device__ __forceinline__ void psi(unsigned int q, double2 v) const { stb<double2>(base, 16u * q, v); }
+  __device__ __forceinline__ void ph(int a, unsigned int q, double v) const {
+    stb<double>(base, (unsigned int)a * pl + 8u * q, v);
+  }
+};
+__device__ __forceinline__ W64 wplanes64(double* v, unsigned int n) {
+  return W64{reinterpret_cast<char*>(v), 8u * n};
+}
+
 // One work item of the fused stencil: an xy tile of TX x TY nodes, planes
 // [kb, ke) of a z chunk.  The tile and its 1-node halo are staged in LDS with
 // a 3-plane ring; the halo of plane k+2 is loaded into registers while plane
 // k is computed (software pipelining), and committed to the ring after.
-template <bool IS3D, bool PERT, int MODE>
+// Cells are staged by the column masks of stage64u (uniform field: the same
+// bits as eff_cell, held in vector registers across the z march); GEN (some
+// film has general field vectors) stages them with eff_cell, its column
+// booleans recomputed on every plane.  Own-node loads and stores use 32-bit
+// byte offsets from one base per vector (the host keeps 5 (8 n) < 2^32).
+template <bool IS3D, bool PERT, int MODE, bool GEN>
 __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const Mat& mt,
                                              const Src<double, PERT>& s, const Epi& e, int tile_x,
                                              int tile_y, int zc, int kz,
@@ -92,18 +111,39 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
     kb = 1 + zc * kz;
     ke = min(kb + kz, g.Nz);
   }
-  const long long n = g.n;
+  const unsigned int n = (unsigned int)g.n;
+  const V64 X = planes64(s.x, n), P = planes64(PERT ? s.p : s.x, n);
 
   // halo cells handled by this thread: c0 = tid, c1 = tid + BS (if < CELLS)
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
   const bool has1 = hc1 < CELLS;
+  Plan64 pl0{}, pl1{};
+  if constexpr (!GEN) {
+    pl0 = plan64(g, b, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
+    pl1 = plan64(g, b, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
+  }
   Cell<double> v0, v1;
-  auto stage = [&](int kp) {
-    v0 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX, kp);
-    if (has1)
-      v1 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX, kp);
+  auto stage = [&](int kp) __attribute__((always_inline)) {
+    if constexpr (GEN) {
+      unsigned int h0 = hc0, h1 = hc1;
+      opaque(h0);
+      opaque(h1);
+      v0 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + h0 % CX, j0 - 1 + h0 / CX, kp);
+      if (has1)
+        v1 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX, kp);
+    } else {
+      const Pl64 pp = plane64<IS3D>(g, b, kp);
+      if (IS3D && pp.fk) {
+        stage64u<IS3D, true, PERT>(mt.flags, X, P, s.eps, pl0, pp, v0);
+        if (has1) stage64u<IS3D, true, PERT>(mt.flags, X, P, s.eps, pl1, pp, v1);
+      } else {
+        stage64u<IS3D, false, PERT>(mt.flags, X, P, s.eps, pl0, pp, v0);
+        if (has1) stage64u<IS3D, false, PERT>(mt.flags, X, P, s.eps, pl1, pp, v1);
+      }
+    }
   };
-  auto commit = [&](int slot) {
+  const unsigned int m_ij = (unsigned int)i + (unsigned int)g.mj * (unsigned int)j;  // full-grid column
+  auto commit = [&](int slot) __attribute__((always_inline)) {
     l_pr[slot][hc0] = v0.pr;
     l_pi[slot][hc0] = v0.pi;
     l_x[slot][hc0] = v0.x;
@@ -134,49 +174,27 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
 
     // own-node inputs first, then the halo two planes ahead
     Ctr c{};
-    long long q = 0;
+    unsigned int q = 0;
+    auto ld5 = [&](const V64& v, double& a, double& bb, double& xx, double& yy, double& zz)
+                   __attribute__((always_inline)) {
+      const double2 t = v.psi(q);
+      a = t.x;
+      bb = t.y;
+      xx = v.ph(2, q);
+      yy = v.ph(3, q);
+      zz = IS3D ? v.ph(4, q) : 0.0;
+    };
     if (act) {
-      q = qix(g, i, j, k);
-      c.fl = mt.flags ? mt.flags[mfull(g, i, j, k)] : (uint8_t)0;
-      if (mt.has_mat && !mt.sc_bits) c.sc = mt.sc[q];
-      if (mt.k2) c.k2 = mt.k2[q];
-      if (MODE == M_PRED || MODE == M_EULER || NEWTON) {
-        const double2 X0 = reinterpret_cast<const double2*>(s.x)[q];
-        c.xr = X0.x;
-        c.xi = X0.y;
-        c.xx = s.x[2 * n + q];
-        c.xy = s.x[3 * n + q];
-        c.xz = IS3D ? s.x[4 * n + q] : 0.0;
-      }
-      if (NEWTON) {
-        const double2 G0 = reinterpret_cast<const double2*>(e.gam)[q];
-        c.gr = G0.x;
-        c.gi = G0.y;
-        c.gx = e.gam[2 * n + q];
-        c.gy = e.gam[3 * n + q];
-        c.gz = IS3D ? e.gam[4 * n + q] : 0.0;
-      }
+      q = qix32(g, i, j, k, IS3D);
+      c.fl = mt.flags ? ldu(mt.flags, m_ij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u)) : (uint8_t)0;
+      if (mt.has_mat && !mt.sc_bits) c.sc = ldu(mt.sc, q);
+      if (mt.k2) c.k2 = ldu(mt.k2, q);
+      if (MODE == M_PRED || MODE == M_EULER || NEWTON) ld5(X, c.xr, c.xi, c.xx, c.xy, c.xz);
+      if (NEWTON) ld5(planes64(e.gam, n), c.gr, c.gi, c.gx, c.gy, c.gz);
       if (MODE == M_JVP) {
-        const double2 P0 = reinterpret_cast<const double2*>(s.p)[q];
-        c.xr = P0.x;
-        c.xi = P0.y;
-        c.xx = s.p[2 * n + q];
-        c.xy = s.p[3 * n + q];
-        c.xz = IS3D ? s.p[4 * n + q] : 0.0;
-        const double2 F0 = reinterpret_cast<const double2*>(e.f0)[q];
-        c.gr = F0.x;
-        c.gi = F0.y;
-        c.gx = e.f0[2 * n + q];
-        c.gy = e.f0[3 * n + q];
-        c.gz = IS3D ? e.f0[4 * n + q] : 0.0;
-        if (e.ap0) {
-          const double2 A0 = reinterpret_cast<const double2*>(e.ap0)[q];
-          c.ar = A0.x;
-          c.ai = A0.y;
-          c.ax = e.ap0[2 * n + q];
-          c.ay = e.ap0[3 * n + q];
-          c.az = IS3D ? e.ap0[4 * n + q] : 0.0;
-        }
+        ld5(P, c.xr, c.xi, c.xx, c.xy, c.xz);
+        ld5(planes64(e.f0, n), c.gr, c.gi, c.gx, c.gy, c.gz);
+        if (e.ap0) ld5(planes64(e.ap0, n), c.ar, c.ai, c.ax, c.ay, c.az);
       }
     }
     if (IS3D && k + 2 <= ke) stage(k + 2);
@@ -292,55 +310,45 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
       if (IS3D && (fl & kHoleZ)) dz_ = 0.0;
 
       // ---- epilogues
-      double2* o0 = reinterpret_cast<double2*>(e.out0);
+      const W64 o0 = wplanes64(e.out0, n);
+      auto st5 = [&](const W64& w, double a, double bb, double xx, double yy, double zz)
+                     __attribute__((always_inline)) {
+        w.psi(q, make_double2(a, bb));
+        w.ph(2, q, xx);
+        w.ph(3, q, yy);
+        if (IS3D) w.ph(4, q, zz);
+      };
       if (MODE == M_RHS) {
-        o0[q] = make_double2(dpr, dpi);
-        e.out0[2 * n + q] = dx_;
-        e.out0[3 * n + q] = dy_;
-        if (IS3D) e.out0[4 * n + q] = dz_;
+        st5(o0, dpr, dpi, dx_, dy_, dz_);
       } else if (MODE == M_PRED || MODE == M_EULER) {
         // X_pred = X + dt f ; gamma = X + dt/2 f   (integrators.py:156-158, :82)
-        o0[q] = make_double2(c.xr + e.dt * dpr, c.xi + e.dt * dpi);
-        e.out0[2 * n + q] = c.xx + e.dt * dx_;
-        e.out0[3 * n + q] = c.xy + e.dt * dy_;
-        if (IS3D) e.out0[4 * n + q] = c.xz + e.dt * dz_;
-        if (MODE == M_PRED) {
-          double2* o1 = reinterpret_cast<double2*>(e.out1);
-          o1[q] = make_double2(c.xr + e.hdt * dpr, c.xi + e.hdt * dpi);
-          e.out1[2 * n + q] = c.xx + e.hdt * dx_;
-          e.out1[3 * n + q] = c.xy + e.hdt * dy_;
-          if (IS3D) e.out1[4 * n + q] = c.xz + e.hdt * dz_;
-        }
+        st5(o0, c.xr + e.dt * dpr, c.xi + e.dt * dpi, c.xx + e.dt * dx_, c.xy + e.dt * dy_,
+            c.xz + e.dt * dz_);
+        if (MODE == M_PRED)
+          st5(wplanes64(e.out1, n), c.xr + e.hdt * dpr, c.xi + e.hdt * dpi, c.xx + e.hdt * dx_,
+              c.xy + e.hdt * dy_, c.xz + e.hdt * dz_);
       } else if (NEWTON) {
         // f, r = -g with g = x - dt/2 f - gamma (newton.py:98-100, :109-111).
         // NEWTON32 (mixed precision) stores x and r in FP32 instead of f and r.
-        if (MODE == M_NEWTON) {
-          o0[q] = make_double2(dpr, dpi);
-          e.out0[2 * n + q] = dx_;
-          e.out0[3 * n + q] = dy_;
-          if (IS3D) e.out0[4 * n + q] = dz_;
-        }
+        if (MODE == M_NEWTON) st5(o0, dpr, dpi, dx_, dy_, dz_);
         const double gr = (c.xr - e.hdt * dpr) - c.gr;
         const double gi = (c.xi - e.hdt * dpi) - c.gi;
         const double gx = (c.xx - e.hdt * dx_) - c.gx;
         const double gy = (c.xy - e.hdt * dy_) - c.gy;
         const double gz = IS3D ? (c.xz - e.hdt * dz_) - c.gz : 0.0;
         if (MODE == M_NEWTON) {
-          double2* o1 = reinterpret_cast<double2*>(e.out1);
-          o1[q] = make_double2(-gr, -gi);
-          e.out1[2 * n + q] = -gx;
-          e.out1[3 * n + q] = -gy;
-          if (IS3D) e.out1[4 * n + q] = -gz;
+          st5(wplanes64(e.out1, n), -gr, -gi, -gx, -gy, -gz);
         } else {
-          reinterpret_cast<float2*>(e.o32a)[q] = make_float2((float)c.xr, (float)c.xi);
-          e.o32a[2 * n + q] = (float)c.xx;
-          e.o32a[3 * n + q] = (float)c.xy;
-          reinterpret_cast<float2*>(e.o32b)[q] = make_float2((float)-gr, (float)-gi);
-          e.o32b[2 * n + q] = (float)-gx;
-          e.o32b[3 * n + q] = (float)-gy;
+          const W32 a32 = wplanes32(e.o32a, n), b32 = wplanes32(e.o32b, n);
+          a32.psi(q, make_float2((float)c.xr, (float)c.xi));
+          a32.ph(2, q, (float)c.xx);
+          a32.ph(3, q, (float)c.xy);
+          b32.psi(q, make_float2((float)-gr, (float)-gi));
+          b32.ph(2, q, (float)-gx);
+          b32.ph(3, q, (float)-gy);
           if (IS3D) {
-            e.o32a[4 * n + q] = (float)c.xz;
-            e.o32b[4 * n + q] = (float)-gz;
+            a32.ph(4, q, (float)c.xz);
+            b32.ph(4, q, (float)-gz);
           }
         }
         red[0] += ((c.xr * c.xr + c.xi * c.xi) + c.xx * c.xx) + (c.xy * c.xy + c.xz * c.xz);
@@ -359,10 +367,7 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
           const double jz = (dz_ - c.gz) * e.inv_eps;
           apz = c.xz - e.hdt * jz;
         }
-        o0[q] = make_double2(apr, api);
-        e.out0[2 * n + q] = apx;
-        e.out0[3 * n + q] = apy;
-        if (IS3D) e.out0[4 * n + q] = apz;
+        st5(o0, apr, api, apx, apy, apz);
         if (e.ap0) {
           red[0] += ((apr * c.ar + api * c.ai) + apx * c.ax) + (apy * c.ay + apz * c.az);
         } else {
@@ -374,7 +379,7 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
   }
 }
 
-template <bool IS3D, bool PERT, int MODE>
+template <bool IS3D, bool PERT, int MODE, bool GEN>
 __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double, PERT> s, Epi e,
                                                 int tiles_x, int tiles_y, int kz, Red r) {
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -393,14 +398,14 @@ __global__ __launch_bounds__(BS) void k_stencil(Geo g, Bnd b, Mat mt, Src<double
   double red[RN::NV];
 #pragma unroll
   for (int a = 0; a < RN::NV; ++a) red[a] = 0.0;
-  stencil_tile<IS3D, PERT, MODE>(g, b, mt, s, e, tile_x, tile_y, zc, kz, red);
+  stencil_tile<IS3D, PERT, MODE, GEN>(g, b, mt, s, e, tile_x, tile_y, zc, kz, red);
   if constexpr (RN::NS + RN::NM > 0) publish<RN::NS, RN::NM>(red, r.pout);
 }
 
 // Batched films: block L (after the XCD remap) is work item L of the batch;
 // the item table names its film, whose descriptor gives the tile.  Partials
 // are published at index L, so each film's partials are one contiguous range.
-template <bool IS3D, int MODE>
+template <bool IS3D, int MODE, bool GEN>
 __global__ __launch_bounds__(BS) void k_bstencil(BatchK bk, const double* x, double* out0,
                                                  double* out1, const double* gam, float* o32a,
                                                  float* o32b, double* pout) {
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_bstencil(BatchK bk, const double* x, dou
   double red[RN::NV];
 #pragma unroll
   for (int a = 0; a < RN::NV; ++a) red[a] = 0.0;
-  stencil_tile<IS3D, false, MODE>(g, b, mt, s, e, tile_x, tile_y, zc, fd->kz, red);
+  stencil_tile<IS3D, false, MODE, GEN>(g, b, mt, s, e, tile_x, tile_y, zc, fd->kz, red);
   if constexpr (RN::NS + RN::NM > 0) publish_at<RN::NS, RN::NM>(red, pout, L);
 }
 
@@ -445,8 +450,10 @@ static int go_stencil(const Geo& g, const Bnd& b, const Mat& mt, const double* x
   Src<double, PERT> s{x, p, eps, g.n};
   const int tx = (g.nx + TX - 1) / TX, ty = (g.ny + TY - 1) / TY;
   const int nb = stencil_blocks(g, cfg);
-  launch((k_stencil<IS3D, PERT, MODE>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty,
-                     cfg.kz, r);
+  if (b.bx || b.by || b.bz)
+    launch((k_stencil<IS3D, PERT, MODE, true>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty, cfg.kz, r);
+  else
+    launch((k_stencil<IS3D, PERT, MODE, false>), dim3(nb), dim3(BS), 0, st, g, b, mt, s, e, tx, ty, cfg.kz, r);
   return nb;
 }
 
@@ -477,17 +484,19 @@ void launch_bstencil(const BatchK& bk, bool is3d, int mode, const double* x, dou
                      double* out1, const double* gam, float* o32a, float* o32b, double* pout,
                      hipStream_t st) {
   const dim3 gr(bk.nitems), bl(BS);
+#define TDGL_BST(D, M)                                                                              \
+  do {                                                                                              \
+    if (bk.gen) launch((k_bstencil<D, M, true>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout); \
+    else launch((k_bstencil<D, M, false>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);        \
+  } while (0)
   if (mode == M_PRED) {
-    if (is3d)
-      launch((k_bstencil<true, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
-    else
-      launch((k_bstencil<false, M_PRED>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+    if (is3d) TDGL_BST(true, M_PRED);
+    else TDGL_BST(false, M_PRED);
   } else {
-    if (is3d)
-      launch((k_bstencil<true, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
-    else
-      launch((k_bstencil<false, M_NEWTON32>), gr, bl, 0, st, bk, x, out0, out1, gam, o32a, o32b, pout);
+    if (is3d) TDGL_BST(true, M_NEWTON32);
+    else TDGL_BST(false, M_NEWTON32);
   }
+#undef TDGL_BST
 }
 
 // Per film (one block each): reduce the film's stencil partials, in item order.

@@ -362,6 +362,157 @@ __host__ __device__ __forceinline__ void stage32u(const uint8_t* flags, const V3
   cp.z = IS3D ? fmask(pz, p.sz & kz) : 0.f;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 (the plain path's stencils): one state -- or x + eps p, the FD JVP's
+// argument (tgcr.py:130) -- staged with the same masks as eff_cell's rules
+// give bit for bit; a mask applies to both words of a double.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ double fmask(double x, unsigned int m) {
+  const unsigned long long mm = ((unsigned long long)m << 32) | m;
+  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) & mm);
+}
+
+// One FP64 state-shaped vector of n nodes on 32-bit byte offsets: psi of node q
+// at 16 q, phi_x at 2 (8n) + 8 q, ...; the host keeps 5 (8n) below 2^32.
+struct V64 {
+  const char* base;
+  unsigned int pl;  // 8 n
+  __host__ __device__ __forceinline__ double2 psi(unsigned int q) const { return ldb<double2>(base, 16u * q); }
+  __host__ __device__ __forceinline__ double ph(int a, unsigned int q) const {
+    return ldb<double>(base, (unsigned int)a * pl + 8u * q);
+  }
+};
+__host__ __device__ __forceinline__ V64 planes64(const double* v, unsigned int n) {
+  return V64{reinterpret_cast<const char*>(v), 8u * n};
+}
+
+struct Plan64 {
+  unsigned int qij, mij, ok, sx, sy, kx, ky, kz;  // as in Plan32
+  double ax, ay, az;
+};
+__host__ __device__ __forceinline__ Plan64 plan64(const Geo& g, const Bnd& b, int i, int j) {
+  const Col32 c = col32(g, i, j);
+  Plan64 p;
+  p.qij = c.qij;
+  p.mij = c.mij;
+  p.ok = keep(c.okij);
+  p.sx = keep(!c.zx);
+  p.sy = keep(!c.zy);
+  bool kx = true, ky = true, kz = true;
+  p.ax = p.ay = p.az = -0.0;
+  if (c.fi) {
+    kx = g.per_x;
+    ky = kz = !g.per_x;
+    if (!g.per_x) {  // rhs.py:91-98
+      p.ay = c.lo_i ? -b.cz : b.cz;
+      p.az = c.lo_i ? b.cy : -b.cy;
+    }
+  } else if (c.fj) {
+    ky = g.per_y;
+    kx = kz = !g.per_y;
+    if (!g.per_y) {  // rhs.py:107-112
+      p.ax = c.lo_j ? b.cz : -b.cz;
+      p.az = c.lo_j ? -b.cx : b.cx;
+    }
+  }
+  p.kx = keep(c.okij && kx);
+  p.ky = keep(c.okij && ky);
+  p.kz = keep(c.okij && kz);
+  opaque(p.ok);
+  opaque(p.sx);
+  opaque(p.sy);
+  opaque(p.kx);
+  opaque(p.ky);
+  opaque(p.kz);
+  return p;
+}
+
+struct Pl64 {
+  unsigned int qoff, moff, sz;
+  bool fk;
+  unsigned int kx, ky, kz;
+  double ax, ay;
+};
+template <bool IS3D>
+__host__ __device__ __forceinline__ Pl64 plane64(const Geo& g, const Bnd& b, int k) {
+  Pl64 p;
+  p.fk = IS3D && ((k == 0) || (k == g.Nz));
+  const int sk = IS3D ? (p.fk ? (g.per_z ? (k == 0 ? g.Nz - 1 : 1) : (k == 0 ? 1 : g.Nz - 1)) : k) : 1;
+  p.qoff = IS3D ? (unsigned int)g.nx * (unsigned int)g.ny * (unsigned int)(sk - 1) : 0u;
+  p.moff = IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u;
+  p.sz = keep(!(IS3D && sk == g.Nz - 1));
+  p.kx = p.ky = keep(!g.per_z);
+  p.kz = keep(g.per_z);
+  p.ax = g.per_z ? -0.0 : (k == 0 ? -b.cy : b.cy);  // rhs.py:122-127
+  p.ay = g.per_z ? -0.0 : (k == 0 ? b.cx : -b.cx);
+  return p;
+}
+
+// Stage the cell of column plan c on plane p (0 <= k <= Nz; uniform field):
+// x, or x + eps p when PERT (two roundings, as Src::ld).
+template <bool IS3D, bool FK, bool PERT>
+__host__ __device__ __forceinline__ void stage64u(const uint8_t* flags, const V64& X, const V64& P,
+                                                  double eps, const Plan64& c, const Pl64& p,
+                                                  Cell<double>& cx) {
+  const unsigned int q = c.qij + p.qoff;
+  unsigned int kx, ky, kz, ok;
+  double ax, ay, az;
+  if (FK) {
+    ok = c.kx & c.ky & c.kz;
+    kx = ok & p.kx;
+    ky = ok & p.ky;
+    kz = ok & p.kz;
+    ax = p.ax;
+    ay = p.ay;
+    az = -0.0;
+  } else {
+    ok = c.ok;
+    kx = c.kx;
+    ky = c.ky;
+    kz = c.kz;
+    ax = c.ax;
+    ay = c.ay;
+    az = c.az;
+  }
+  double2 xa = X.psi(q);
+  double xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.0;
+  if (PERT) {
+    const double2 pa = P.psi(q);
+    xa.x = xa.x + eps * pa.x;
+    xa.y = xa.y + eps * pa.y;
+    xx = xx + eps * P.ph(2, q);
+    xy = xy + eps * P.ph(3, q);
+    if (IS3D) xz = xz + eps * P.ph(4, q);
+  }
+  if (flags) {
+    const uint8_t h = ldu(flags, c.mij + p.moff);
+    kx &= keep(!(h & kHoleX));
+    ky &= keep(!(h & kHoleY));
+    if (IS3D) kz &= keep(!(h & kHoleZ));
+  }
+  cx.pr = fmask(xa.x, ok);
+  cx.pi = fmask(xa.y, ok);
+  cx.x = fmask(fmask(xx, c.sx) + ax, kx);
+  cx.y = fmask(fmask(xy, c.sy) + ay, ky);
+  cx.z = IS3D ? fmask(fmask(xz, p.sz) + az, kz) : 0.0;
+}
+
+// FP64 cell with its plan (the CPU check compares this with eff_cell).
+template <bool IS3D, bool PERT>
+__host__ __device__ __forceinline__ Cell<double> cell64(const Geo& g, const Bnd& b, const uint8_t* flags,
+                                                        const V64& X, const V64& P, double eps, int i,
+                                                        int j, int k) {
+  const Plan64 c = plan64(g, b, i, j);
+  const Pl64 p = plane64<IS3D>(g, b, k);
+  Cell<double> r;
+  if (p.fk)
+    stage64u<IS3D, true, PERT>(flags, X, P, eps, c, p, r);
+  else
+    stage64u<IS3D, false, PERT>(flags, X, P, eps, c, p, r);
+  return r;
+}
+
 // One cell, column plan included (the CPU check compares this with
 // eff_cell_pair; kernels build the plan once per column).  `masks` picks the
 // uniform-field form (stage32u), which needs a uniform field.

@@ -4,8 +4,9 @@
 // included), on random 2-D and 3-D grids with periodic axes, uniform and
 // per-node applied fields and hole-boundary link bits; and the mask form of
 // the uniform-field staging (stage32u) must give exactly the bits of the
-// per-cell rules (stage32), signs of zeros included.  Host code only: hipcc
-// compiles the __host__ __device__ functions for the CPU.
+// per-cell rules (stage32), signs of zeros included.  The FP64 mask staging
+// (cell64, plain and x + eps p) must give exactly the bits of eff_cell.
+// Host code only: hipcc compiles the __host__ __device__ functions for the CPU.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -70,6 +71,15 @@ static long long check(std::mt19937& rng, int trial) {
   const Src<float, false> sx{x.data(), nullptr, 0.f, g.n};
   const Src<float, false> sp{p.data(), nullptr, 0.f, g.n};
   const V32 X = planes32(x.data(), (unsigned)g.n), P = planes32(p.data(), (unsigned)g.n);
+  std::vector<double> xd(len), pd(len);
+  for (long long e = 0; e < len; ++e) {
+    xd[e] = (double)x[e] + 1e-9 * ur(rng);
+    pd[e] = (double)p[e];
+  }
+  const double eps = 1e-4 * (1.0 + ur(rng));
+  const Src<double, false> s64{xd.data(), nullptr, 0.0, g.n};
+  const Src<double, true> s64p{xd.data(), pd.data(), eps, g.n};
+  const V64 X64 = planes64(xd.data(), (unsigned)g.n), P64 = planes64(pd.data(), (unsigned)g.n);
   long long cells = 0, bad = 0;
   for (int k = IS3D ? -1 : 0; k <= (IS3D ? g.Nz + 1 : 0); ++k)
     for (int j = -1; j <= g.Ny + 1; ++j)
@@ -87,6 +97,23 @@ static long long check(std::mt19937& rng, int trial) {
                      (int)IS3D, trial, g.Nx, g.Ny, g.Nz, g.per_x, g.per_y, g.per_z, i, j, k, c, u[c], w[c]);
             ++bad;
           }
+        if (!general && k >= 0 && k <= (IS3D ? g.Nz : 0)) {  // FP64 mask staging vs eff_cell
+          for (int pert = 0; pert < 2; ++pert) {
+            const Cell<double> ref = pert ? eff_cell<double, IS3D, true>(g, b, fl, s64p, i, j, k)
+                                          : eff_cell<double, IS3D, false>(g, b, fl, s64, i, j, k);
+            const Cell<double> got = pert ? cell64<IS3D, true>(g, b, fl, X64, P64, eps, i, j, k)
+                                          : cell64<IS3D, false>(g, b, fl, X64, P64, eps, i, j, k);
+            const double u64[5] = {ref.pr, ref.pi, ref.x, ref.y, ref.z};
+            const double w64[5] = {got.pr, got.pi, got.x, got.y, got.z};
+            for (int c = 0; c < 5; ++c)
+              if (memcmp(&u64[c], &w64[c], sizeof(double)) != 0) {
+                if (bad < 10)
+                  printf("FP64 MISMATCH 3d=%d pert=%d trial=%d N=(%d,%d,%d) per=(%d,%d,%d) cell=(%d,%d,%d) comp=%d: %.17g vs %.17g\n",
+                         (int)IS3D, pert, trial, g.Nx, g.Ny, g.Nz, g.per_x, g.per_y, g.per_z, i, j, k, c, u64[c], w64[c]);
+                ++bad;
+              }
+          }
+        }
         if (!general && k >= 0 && k <= (IS3D ? g.Nz : 0)) {  // the planes a sweep stages
           cell_pair32<IS3D>(g, b, fl, X, P, i, j, k, mx, mp, true);
           const float v[10] = {mx.pr, mx.pi, mx.x, mx.y, mx.z, mp.pr, mp.pi, mp.x, mp.y, mp.z};
