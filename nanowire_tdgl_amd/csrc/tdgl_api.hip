@@ -176,9 +176,12 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
   g.n = (long long)g.nx * g.ny * g.nz;
   g.nphi = g.is3d ? 3 : 2;
   ctx->len = (2 + g.nphi) * g.n;
-  if (g.mk * (long long)(grid->Nz + 1) >= (1LL << 31) || ctx->len >= (1LL << 31)) {
+  // node indices are 32-bit on the device, and the stencils and sweeps address
+  // a state vector with 32-bit byte offsets (8 B per FP64 entry)
+  if (g.mk * (long long)(grid->Nz + 1) >= (1LL << 31) || ctx->len >= (1LL << 31) ||
+      8 * ctx->len >= (1LL << 32)) {
     delete ctx;
-    return TDGL_E_INVALID;  // node indices are 32-bit on the device
+    return TDGL_E_INVALID;
   }
 
   Bnd& b = ctx->b;

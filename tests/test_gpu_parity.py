@@ -53,6 +53,17 @@ def test_eval_f_general_boundary_vectors():
     assert rel_err(F, d["F"]) < 1e-13
 
 
+def test_grid_beyond_32bit_offsets_rejected():
+    """State vectors are addressed with 32-bit byte offsets: a film of 40 B x n
+    >= 2^32 is refused at creation, one just below is accepted."""
+    from nanowire_tdgl_amd import _native
+    with pytest.raises(ValueError):
+        _native.Context(1025, 1025, 104, 0.5, 0.5, 0.5, 2.0)  # 1024 * 1024 * 103 nodes
+    ok = _native.Context(1025, 1025, 103, 0.5, 0.5, 0.5, 2.0)  # 1024 * 1024 * 102 nodes
+    assert ok.n == 1024 * 1024 * 102
+    ok.close()
+
+
 def test_permute_round_trip_and_complex_phi_rejected():
     T, engine_for, rhs, bbfv, _ = _lib()
     dev = device(S.EVALF_CASES["hole_zr"])
