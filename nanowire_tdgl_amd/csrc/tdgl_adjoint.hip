@@ -7,8 +7,12 @@
 // through the boundary rules (the ghost cells are affine in the interior
 // values).  The transpose is applied in two steps:
 //
-//   k_vjp_cells   each node m scatters w_m * coef into a full-grid cell
-//                 adjoint A[cell][component] (FP64 atomics);
+//   k_vjp_cells   each node m accumulates w_m * coef per (target cell,
+//                 component) -- its own cell, the six neighbours, the six
+//                 diagonal cells: 41 planes, each (plane, m) written by node
+//                 m's thread alone (no atomics, deterministic);
+//   k_cells_gather  every full-grid cell sums what its (up to 13) source nodes
+//                 left for it, in a fixed order: the cell adjoint A[cell][component];
 //   k_bc_gather   each interior node q gathers A over the cells whose values
 //                 come from q under the boundary rules of rhs.py:54-139 (its
 //                 own cell, the face ghosts copied from it, periodic images),
@@ -32,10 +36,14 @@ namespace tdgl {
 
 namespace {
 
-__device__ __forceinline__ void acc(double* A, long long m, int comp, double v) {
-  if (v != 0.0) atomicAdd(A + m * 5 + comp, v);
-}
 enum { CR = 0, CI = 1, LX = 2, LY = 3, LZ = 4 };
+// Target cells of a node's scatter, relative to the node: itself, -x, +x, -y,
+// +y, -z, +z (all five components each: plane slot * 5 + component), then the
+// diagonal cells of the curl cross terms (one component each: planes 35-40).
+enum { S0 = 0, SXM = 1, SXP = 2, SYM = 3, SYP = 4, SZM = 5, SZP = 6 };
+enum { D_XP_YM = 35, D_XP_ZM = 36, D_XM_YP = 37, D_YP_ZM = 38, D_XM_ZP = 39, D_YM_ZP = 40 };
+constexpr int kVjpPlanes = 41;
+__device__ __forceinline__ int plane(int slot, int comp) { return slot * 5 + comp; }
 
 template <bool IS3D>
 __device__ __forceinline__ void node_of(const Geo& g, long long q, int& i, int& j, int& k) {
@@ -50,9 +58,10 @@ __device__ __forceinline__ void node_of(const Geo& g, long long q, int& i, int& 
 // w: adjoint weights (internal layout).  A: full grid x 5, accumulated.
 // gsc: per interior node (internal order) d<w,F>/d(sc_q), or nullptr.
 // pk: per-block partials of d<w,F>/d(kappa) (uniform kappa `kappa`), or nullptr.
+// w: adjoint weights (internal layout).  B: kVjpPlanes x n, zeroed, accumulated.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt, const double* x,
-                                                           const double* w, double* A, double* gsc,
+                                                           const double* w, double* B, double* gsc,
                                                            double kappa, double* pk) {
   const long long n = g.n;
   const Src<double, false> s{x, nullptr, 0.0, n};
@@ -69,7 +78,8 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     const double mz = (IS3D && !(fl & kHoleZ)) ? w[4 * n + q] : 0.0;
 
     auto cell = [&](int ii, int jj, int kk) { return eff_cell<double, IS3D, false>(g, b, mt.flags, s, ii, jj, kk); };
-    auto mf = [&](int ii, int jj, int kk) { return mfull(g, ii, jj, kk); };
+    // B[plane * n + q] belongs to this thread (q is its node)
+    auto acc = [&](int pl, double v) { B[(long long)pl * n + q] += v; };
     const Cell<double> c0 = cell(i, j, k), xm = cell(i - 1, j, k), xp = cell(i + 1, j, k);
     const Cell<double> ym = cell(i, j - 1, k), yp = cell(i, j + 1, k);
     Cell<double> zm{}, zp{};
@@ -77,9 +87,6 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
       zm = cell(i, j, k - 1);
       zp = cell(i, j, k + 1);
     }
-    const long long m0 = mf(i, j, k), mxm = mf(i - 1, j, k), mxp = mf(i + 1, j, k);
-    const long long mym = mf(i, j - 1, k), myp = mf(i, j + 1, k);
-    const long long mzm = IS3D ? mf(i, j, k - 1) : 0, mzp = IS3D ? mf(i, j, k + 1) : 0;
 
     const double pr = c0.pr, pi = c0.pi;
     double sxp, cxp, sxm, cxm, syp, cyp, sym, cym, szp = 0, czp = 1, szm = 0, czm = 1;
@@ -93,31 +100,31 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     }
 
     // ---- psi equation: d psi_m = dgpsi dpsi_0 + sum of the six hopping terms + FPSI'
-    acc(A, m0, CR, mt.dgpsi * mr);
-    acc(A, m0, CI, mt.dgpsi * mi);
+    acc(plane(S0, CR), mt.dgpsi * mr);
+    acc(plane(S0, CI), mt.dgpsi * mi);
     // minus neighbour nb = m - a, its link t = phi_a(m - a):
     //   w_r = dq_r + dt q_i, w_i = dq_i - dt q_r;  d(ar, ai) = h (c w_r + s w_i, c w_i - s w_r)
-    auto minus = [&](double c, double sn, double h, const Cell<double>& nb, long long mnb, int lc) {
+    auto minus = [&](double c, double sn, double h, const Cell<double>& nb, int snb, int lc) {
       const double gr = h * (c * mr - sn * mi), gi = h * (sn * mr + c * mi);
-      acc(A, mnb, CR, gr);
-      acc(A, mnb, CI, gi);
-      acc(A, mnb, lc, gr * nb.pi - gi * nb.pr);
+      acc(plane(snb, CR), gr);
+      acc(plane(snb, CI), gi);
+      acc(plane(snb, lc), gr * nb.pi - gi * nb.pr);
     };
     // plus neighbour nb = m + a, the node's own link t = phi_a(m):
     //   w_r = dq_r - dt q_i, w_i = dq_i + dt q_r;  d(ar, ai) = h (c w_r - s w_i, c w_i + s w_r)
-    auto plus = [&](double c, double sn, double h, const Cell<double>& nb, long long mnb, int lc) {
+    auto plus = [&](double c, double sn, double h, const Cell<double>& nb, int snb, int lc) {
       const double gr = h * (c * mr + sn * mi), gi = h * (-sn * mr + c * mi);
-      acc(A, mnb, CR, gr);
-      acc(A, mnb, CI, gi);
-      acc(A, m0, lc, -gr * nb.pi + gi * nb.pr);
+      acc(plane(snb, CR), gr);
+      acc(plane(snb, CI), gi);
+      acc(plane(S0, lc), -gr * nb.pi + gi * nb.pr);
     };
-    minus(cxm, sxm, mt.ihx2, xm, mxm, LX);
-    plus(cxp, sxp, mt.ihx2, xp, mxp, LX);
-    minus(cym, sym, mt.ihy2, ym, mym, LY);
-    plus(cyp, syp, mt.ihy2, yp, myp, LY);
+    minus(cxm, sxm, mt.ihx2, xm, SXM, LX);
+    plus(cxp, sxp, mt.ihx2, xp, SXP, LX);
+    minus(cym, sym, mt.ihy2, ym, SYM, LY);
+    plus(cyp, syp, mt.ihy2, yp, SYP, LY);
     if (IS3D) {
-      minus(czm, szm, mt.ihz2, zm, mzm, LZ);
-      plus(czp, szp, mt.ihz2, zp, mzp, LZ);
+      minus(czm, szm, mt.ihz2, zm, SZM, LZ);
+      plus(czp, szp, mt.ihz2, zp, SZP, LZ);
     }
     // FPSI: [fr, fi] = M [dr, di], M symmetric
     const double m2 = pr * pr + pi * pi, a2 = pr * pr - pi * pi, b2 = 2.0 * pr * pi;
@@ -126,8 +133,8 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     const double tau = mt.has_mat ? (1.0 - sc) * mt.inv_tau : 0.0;
     const double M00 = sc * ((1.0 - 2.0 * m2) - a2) - tau, M11 = sc * ((1.0 - 2.0 * m2) + a2) - tau;
     const double M01 = -sc * b2;
-    acc(A, m0, CR, M00 * mr + M01 * mi);
-    acc(A, m0, CI, M01 * mr + M11 * mi);
+    acc(plane(S0, CR), M00 * mr + M01 * mi);
+    acc(plane(S0, CI), M01 * mr + M11 * mi);
     if (gsc) {  // d FPSI / d sc = (1 - |psi|^2) psi + psi / tau
       const double t1 = 1.0 - m2;
       gsc[q] = mr * (t1 * pr + pr * mt.inv_tau) + mi * (t1 * pi + pi * mt.inv_tau);
@@ -147,16 +154,16 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     }
     // supercurrent derivative along link a with own phase (c, sn), neighbour nb = m + a:
     //   (c U_i - s U_r) - dt (c T_r + s T_i)
-    auto dsup_t = [&](double c, double sn, double wa, const Cell<double>& nb, long long mnb, int lc) {
+    auto dsup_t = [&](double c, double sn, double wa, const Cell<double>& nb, int snb, int lc) {
       if (wa == 0.0) return;
       const double qr = nb.pr, qi = nb.pi;
       const double Tr = pr * qr + pi * qi, Ti = pr * qi - pi * qr;
       const double gUi = c * wa, gUr = -sn * wa;
-      acc(A, m0, CR, gUr * qr + gUi * qi);
-      acc(A, m0, CI, gUr * qi - gUi * qr);
-      acc(A, mnb, CR, gUr * pr - gUi * pi);
-      acc(A, mnb, CI, gUr * pi + gUi * pr);
-      acc(A, m0, lc, -(c * Tr + sn * Ti) * wa);
+      acc(plane(S0, CR), gUr * qr + gUi * qi);
+      acc(plane(S0, CI), gUr * qi - gUi * qr);
+      acc(plane(snb, CR), gUr * pr - gUi * pi);
+      acc(plane(snb, CI), gUr * pi + gUi * pr);
+      acc(plane(S0, lc), -(c * Tr + sn * Ti) * wa);
     };
     // the kappa^2 part of each link equation, for d/d kappa
     double kx = 0.0, ky = 0.0, kzv = 0.0;
@@ -164,82 +171,82 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
       // jx = 2cx (x[xm] - 2 x0 + x[xp]) + cy (x[ym] - 2 x0 + x[yp]) + cy ((-y[xp] + y0) + y[xp-y] - y[ym])
       //    + cz (x[zm] - 2 x0 + x[zp]) + cz ((-z[xp] + z0) + z[xp-z] - z[zm]) + dsup_x
       const double c2x = 2.0 * cx;
-      acc(A, mxm, LX, c2x * mx);
-      acc(A, mxp, LX, c2x * mx);
-      acc(A, mym, LX, cy * mx);
-      acc(A, myp, LX, cy * mx);
-      acc(A, m0, LX, (-2.0 * c2x - 2.0 * cy - (IS3D ? 2.0 * cz : 0.0)) * mx);
-      acc(A, mxp, LY, -cy * mx);
-      acc(A, m0, LY, cy * mx);
-      acc(A, mf(i + 1, j - 1, k), LY, cy * mx);
-      acc(A, mym, LY, -cy * mx);
+      acc(plane(SXM, LX), c2x * mx);
+      acc(plane(SXP, LX), c2x * mx);
+      acc(plane(SYM, LX), cy * mx);
+      acc(plane(SYP, LX), cy * mx);
+      acc(plane(S0, LX), (-2.0 * c2x - 2.0 * cy - (IS3D ? 2.0 * cz : 0.0)) * mx);
+      acc(plane(SXP, LY), -cy * mx);
+      acc(plane(S0, LY), cy * mx);
+      acc(D_XP_YM, cy * mx);
+      acc(plane(SYM, LY), -cy * mx);
       const Cell<double> dxy = cell(i + 1, j - 1, k);
       kx = c2x * (xm.x - 2.0 * c0.x + xp.x) + cy * (ym.x - 2.0 * c0.x + yp.x) +
            cy * (((-xp.y + c0.y) + dxy.y) - ym.y);
       if (IS3D) {
-        acc(A, mzm, LX, cz * mx);
-        acc(A, mzp, LX, cz * mx);
-        acc(A, mxp, LZ, -cz * mx);
-        acc(A, m0, LZ, cz * mx);
-        acc(A, mf(i + 1, j, k - 1), LZ, cz * mx);
-        acc(A, mzm, LZ, -cz * mx);
+        acc(plane(SZM, LX), cz * mx);
+        acc(plane(SZP, LX), cz * mx);
+        acc(plane(SXP, LZ), -cz * mx);
+        acc(plane(S0, LZ), cz * mx);
+        acc(D_XP_ZM, cz * mx);
+        acc(plane(SZM, LZ), -cz * mx);
         const Cell<double> dxz = cell(i + 1, j, k - 1);
         kx += cz * (zm.x - 2.0 * c0.x + zp.x) + cz * (((-xp.z + c0.z) + dxz.z) - zm.z);
       }
-      dsup_t(cxp, sxp, mx, xp, mxp, LX);
+      dsup_t(cxp, sxp, mx, xp, SXP, LX);
     }
     if (my != 0.0) {
       // jy = 2cy (y[ym] - 2 y0 + y[yp]) + cx (y[xm] - 2 y0 + y[xp]) + cx ((-x[yp] + x0) + x[yp-x] - x[xm])
       //    + cz (y[zm] - 2 y0 + y[zp]) + cz ((-z[yp] + z0) + z[yp-z] - z[zm]) + dsup_y
       const double c2y = 2.0 * cy;
-      acc(A, mym, LY, c2y * my);
-      acc(A, myp, LY, c2y * my);
-      acc(A, mxm, LY, cx * my);
-      acc(A, mxp, LY, cx * my);
-      acc(A, m0, LY, (-2.0 * c2y - 2.0 * cx - (IS3D ? 2.0 * cz : 0.0)) * my);
-      acc(A, myp, LX, -cx * my);
-      acc(A, m0, LX, cx * my);
-      acc(A, mf(i - 1, j + 1, k), LX, cx * my);
-      acc(A, mxm, LX, -cx * my);
+      acc(plane(SYM, LY), c2y * my);
+      acc(plane(SYP, LY), c2y * my);
+      acc(plane(SXM, LY), cx * my);
+      acc(plane(SXP, LY), cx * my);
+      acc(plane(S0, LY), (-2.0 * c2y - 2.0 * cx - (IS3D ? 2.0 * cz : 0.0)) * my);
+      acc(plane(SYP, LX), -cx * my);
+      acc(plane(S0, LX), cx * my);
+      acc(D_XM_YP, cx * my);
+      acc(plane(SXM, LX), -cx * my);
       const Cell<double> dyx = cell(i - 1, j + 1, k);
       ky = c2y * (ym.y - 2.0 * c0.y + yp.y) + cx * (xm.y - 2.0 * c0.y + xp.y) +
            cx * (((-yp.x + c0.x) + dyx.x) - xm.x);
       if (IS3D) {
-        acc(A, mzm, LY, cz * my);
-        acc(A, mzp, LY, cz * my);
-        acc(A, myp, LZ, -cz * my);
-        acc(A, m0, LZ, cz * my);
-        acc(A, mf(i, j + 1, k - 1), LZ, cz * my);
-        acc(A, mzm, LZ, -cz * my);
+        acc(plane(SZM, LY), cz * my);
+        acc(plane(SZP, LY), cz * my);
+        acc(plane(SYP, LZ), -cz * my);
+        acc(plane(S0, LZ), cz * my);
+        acc(D_YP_ZM, cz * my);
+        acc(plane(SZM, LZ), -cz * my);
         const Cell<double> dyz = cell(i, j + 1, k - 1);
         ky += cz * (zm.y - 2.0 * c0.y + zp.y) + cz * (((-yp.z + c0.z) + dyz.z) - zm.z);
       }
-      dsup_t(cyp, syp, my, yp, myp, LY);
+      dsup_t(cyp, syp, my, yp, SYP, LY);
     }
     if (IS3D && mz != 0.0) {
       // jz = 2cz (z[zm] - 2 z0 + z[zp]) + cx (z[xm] - 2 z0 + z[xp]) + cy (z[ym] - 2 z0 + z[yp])
       //    + cx ((-x[zp] + x0) + x[zp-x] - x[xm]) + cy ((-y[zp] + y0) + y[zp-y] - y[ym]) + dsup_z
       const double c2z = 2.0 * cz;
-      acc(A, mzm, LZ, c2z * mz);
-      acc(A, mzp, LZ, c2z * mz);
-      acc(A, mxm, LZ, cx * mz);
-      acc(A, mxp, LZ, cx * mz);
-      acc(A, mym, LZ, cy * mz);
-      acc(A, myp, LZ, cy * mz);
-      acc(A, m0, LZ, (-2.0 * c2z - 2.0 * cx - 2.0 * cy) * mz);
-      acc(A, mzp, LX, -cx * mz);
-      acc(A, m0, LX, cx * mz);
-      acc(A, mf(i - 1, j, k + 1), LX, cx * mz);
-      acc(A, mxm, LX, -cx * mz);
-      acc(A, mzp, LY, -cy * mz);
-      acc(A, m0, LY, cy * mz);
-      acc(A, mf(i, j - 1, k + 1), LY, cy * mz);
-      acc(A, mym, LY, -cy * mz);
+      acc(plane(SZM, LZ), c2z * mz);
+      acc(plane(SZP, LZ), c2z * mz);
+      acc(plane(SXM, LZ), cx * mz);
+      acc(plane(SXP, LZ), cx * mz);
+      acc(plane(SYM, LZ), cy * mz);
+      acc(plane(SYP, LZ), cy * mz);
+      acc(plane(S0, LZ), (-2.0 * c2z - 2.0 * cx - 2.0 * cy) * mz);
+      acc(plane(SZP, LX), -cx * mz);
+      acc(plane(S0, LX), cx * mz);
+      acc(D_XM_ZP, cx * mz);
+      acc(plane(SXM, LX), -cx * mz);
+      acc(plane(SZP, LY), -cy * mz);
+      acc(plane(S0, LY), cy * mz);
+      acc(D_YM_ZP, cy * mz);
+      acc(plane(SYM, LY), -cy * mz);
       const Cell<double> dzx = cell(i - 1, j, k + 1), dzy = cell(i, j - 1, k + 1);
       kzv = c2z * (zm.z - 2.0 * c0.z + zp.z) + cx * (xm.z - 2.0 * c0.z + xp.z) +
             cy * (ym.z - 2.0 * c0.z + yp.z) + cx * (((-zp.x + c0.x) + dzx.x) - xm.x) +
             cy * (((-zp.y + c0.y) + dzy.y) - ym.y);
-      dsup_t(czp, szp, mz, zp, mzp, LZ);
+      dsup_t(czp, szp, mz, zp, SZP, LZ);
     }
     if (pk) {  // the link operators are proportional to kappa^2
       const double kq = mt.k2 ? sqrt(mt.k2[q]) : kappa;
@@ -247,6 +254,51 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     }
   }
   if (pk) publish<1, 0>(kap, pk);
+}
+
+// Cell adjoint A[cell][component] of the full grid: each cell sums what its
+// source nodes (cell - offset, interior only) left in the planes of k_vjp_cells,
+// in a fixed order (the six neighbours' and the node's own slots, then the
+// diagonals); every cell and component is written.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A) {
+  const long long n = g.n;
+  const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
+  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
+       c += (long long)gridDim.x * kRedThreads) {
+    const int ci = (int)(c % g.mj);
+    const long long t = c / g.mj;
+    const int cj = (int)(t % (g.Ny + 1));
+    const int ck = IS3D ? (int)(t / (g.Ny + 1)) : 0;
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // the node that scattered into this cell with offset (di, dj, dk), if interior
+    auto src = [&](int di, int dj, int dk, long long& q) {
+      const int si = ci - di, sj = cj - dj, sk = ck - dk;
+      if (si < 1 || si > g.Nx - 1 || sj < 1 || sj > g.Ny - 1) return false;
+      if (IS3D && (sk < 1 || sk > g.Nz - 1)) return false;
+      q = qix(g, si, sj, sk);
+      return true;
+    };
+    constexpr int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+    for (int s = 0; s < (IS3D ? 7 : 5); ++s) {
+      long long q;
+      if (src(off[s][0], off[s][1], off[s][2], q))
+#pragma unroll
+        for (int comp = 0; comp < 5; ++comp) a[comp] += B[(long long)plane(s, comp) * n + q];
+    }
+    long long q;
+    if (src(1, -1, 0, q)) a[LY] += B[(long long)D_XP_YM * n + q];
+    if (src(-1, 1, 0, q)) a[LX] += B[(long long)D_XM_YP * n + q];
+    if (IS3D) {
+      if (src(1, 0, -1, q)) a[LZ] += B[(long long)D_XP_ZM * n + q];
+      if (src(0, 1, -1, q)) a[LZ] += B[(long long)D_YP_ZM * n + q];
+      if (src(-1, 0, 1, q)) a[LX] += B[(long long)D_XM_ZP * n + q];
+      if (src(0, -1, 1, q)) a[LY] += B[(long long)D_YM_ZP * n + q];
+    }
+#pragma unroll
+    for (int comp = 0; comp < 5; ++comp) A[c * 5 + comp] = a[comp];
+  }
 }
 
 // (J^T w)_q: the cell adjoint gathered over the cells whose values come from q.
@@ -356,6 +408,8 @@ __global__ __launch_bounds__(kRedThreads) void k_field_grad(Geo g, const uint8_t
 
 // Cell adjoint of B_z = (((x - x[+y]) - y) + y[+x]) / (hx hy) at every
 // interior node (physics/bfield.py:58-68), weights in the reference order.
+// Once per backward pass: neighbouring nodes share cells, so the four
+// contributions are FP64 atomics.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const double* gbz, double* A) {
   const long long n = g.n;
@@ -367,10 +421,11 @@ __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const do
     // reference order r = ((i-1) ny + (j-1)) nz + (k-1)
     const long long r = ((long long)(i - 1) * g.ny + (j - 1)) * g.nz + (IS3D ? k - 1 : 0);
     const double v = s * gbz[r];
-    acc(A, mfull(g, i, j, k), LX, v);
-    acc(A, mfull(g, i, j + 1, k), LX, -v);
-    acc(A, mfull(g, i, j, k), LY, -v);
-    acc(A, mfull(g, i + 1, j, k), LY, v);
+    if (v == 0.0) continue;
+    atomicAdd(A + mfull(g, i, j, k) * 5 + LX, v);
+    atomicAdd(A + mfull(g, i, j + 1, k) * 5 + LX, -v);
+    atomicAdd(A + mfull(g, i, j, k) * 5 + LY, -v);
+    atomicAdd(A + mfull(g, i + 1, j, k) * 5 + LY, v);
   }
 }
 
@@ -378,11 +433,19 @@ __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const do
 // launchers
 // --------------------------------------------------------------------------
 
+int vjp_planes() { return kVjpPlanes; }
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
-                      double* A, double* gsc, double kappa, double* pk, hipStream_t st) {
+                      double* B, double* gsc, double kappa, double* pk, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
-  else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, A, gsc, kappa, pk);
+  if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
+  else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
+}
+void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st) {
+  const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
+  long long nb = (dimx + kRedThreads - 1) / kRedThreads;
+  if (nb > 4096) nb = 4096;
+  if (g.is3d) launch(k_cells_gather<true>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A);
+  else launch(k_cells_gather<false>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A);
 }
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);

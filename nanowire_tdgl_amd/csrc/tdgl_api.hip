@@ -244,6 +244,7 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
   for (double* p : ctx->AP) hipFree(p);
   if (ctx->own_batch) tdgl_batch_destroy(ctx->own_batch);
   if (ctx->d_acell) hipFree(ctx->d_acell);
+  if (ctx->d_vjpb) hipFree(ctx->d_vjpb);
   ctx->prof.destroy();
   if (ctx->h_res) hipHostFree(ctx->h_res);
   if (ctx->h_la) hipHostFree(ctx->h_la);
@@ -480,14 +481,25 @@ int tdgl_precond_cheb32(tdgl_ctx* ctx, const float* x32, const float* v32, doubl
 
 namespace {
 
-int ensure_acell(tdgl_ctx* ctx) {
+// The cell adjoint, zeroed (clear) or to be overwritten by launch_cells_gather.
+int ensure_acell(tdgl_ctx* ctx, bool clear = true) {
   const Geo& g = ctx->g;
   const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
   if (!ctx->d_acell) {
     int rc;
     if ((rc = alloc(ctx, (void**)&ctx->d_acell, sizeof(double) * 5 * (size_t)dimx))) return rc;
   }
-  HIPCHK(hipMemsetAsync(ctx->d_acell, 0, sizeof(double) * 5 * (size_t)dimx, ctx->st));
+  if (clear) HIPCHK(hipMemsetAsync(ctx->d_acell, 0, sizeof(double) * 5 * (size_t)dimx, ctx->st));
+  return TDGL_OK;
+}
+// The per-node scatter planes of the VJP, zeroed.
+int ensure_vjpb(tdgl_ctx* ctx) {
+  const size_t bytes = sizeof(double) * (size_t)vjp_planes() * (size_t)ctx->g.n;
+  if (!ctx->d_vjpb) {
+    int rc;
+    if ((rc = alloc(ctx, (void**)&ctx->d_vjpb, bytes))) return rc;
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_vjpb, 0, bytes, ctx->st));
   return TDGL_OK;
 }
 
@@ -516,10 +528,11 @@ int tdgl_vjp_f(tdgl_ctx* ctx, const double* x, const double* w, double* gx, doub
                double* dkappa, double* dsc) {
   if (!ctx || !x || !w || !gx) return TDGL_E_INVALID;
   int rc;
-  if ((rc = ensure_acell(ctx))) return rc;
+  if ((rc = ensure_acell(ctx, false)) || (rc = ensure_vjpb(ctx))) return rc;
   Red r = red_next(ctx);
-  launch_vjp_cells(ctx->g, ctx->b, ctx->mt, x, w, ctx->d_acell, dsc, ctx->kappa_uniform,
+  launch_vjp_cells(ctx->g, ctx->b, ctx->mt, x, w, ctx->d_vjpb, dsc, ctx->kappa_uniform,
                    dkappa ? r.pout : nullptr, ctx->st);
+  launch_cells_gather(ctx->g, ctx->d_vjpb, ctx->d_acell, ctx->st);
   LAUNCHCHK();
   if (dkappa) {
     red_commit(ctx, stream_blocks(ctx->g));

@@ -147,6 +147,7 @@ struct tdgl_ctx {
   int kz_user = 0;    // tdgl_set_stencil_kz (0: the built-in choice)
   int tile_user = tdgl::kDefaultSweepShape;  // tdgl_set_sweep_tile (mixed-precision sweeps)
   double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
+  double* d_vjpb = nullptr;   // adjoint: vjp_planes() x n scatter planes
   double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
   double k2max = 0.0;         // largest kappa^2 of any node (preconditioner bounds)
   int pb_blocks = 0;  // capacity of d_pb in blocks

@@ -225,9 +225,13 @@ void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* 
 // in x += M u); u may alias r0v.
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st);
-// Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.
+// Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.  B:
+// vjp_planes() x n per-node scatter planes, zeroed before launch_vjp_cells,
+// turned into A by launch_cells_gather.
+int vjp_planes();
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
-                      double* A, double* gsc, double kappa, double* pk, hipStream_t st);
+                      double* B, double* gsc, double kappa, double* pk, hipStream_t st);
+void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st);
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);
