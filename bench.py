@@ -393,7 +393,7 @@ def section_adjoint(steps, every, fd=True):
            "forward_s_per_step": (c1 - c0) / steps, "adjoint_s_per_step": (c2 - c1) / steps,
            "grad": {"kappa": kappa.grad.item(), "Bz": bz.grad.item(),
                     "w_norm": float(w.grad.norm())},
-           "newton_iters": prob.stats.get("newton"), "gmres_iters": prob.stats.get("adjoint_gmres_iters")}
+           "newton_iters": prob.stats.get("newton"), "adjoint_krylov_iters": prob.stats.get("adjoint_krylov_iters")}
     log(f"[bench] adjoint: forward {out['forward_s_per_step']:.3f} s/step, adjoint "
         f"{out['adjoint_s_per_step']:.3f} s/step, grads {out['grad']}")
     if fd:

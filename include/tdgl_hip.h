@@ -201,6 +201,14 @@ int tdgl_vjp_f(tdgl_ctx* ctx, const double* x_dev, const double* w_dev, double* 
  * at the current field: gx_dev = (dB_z/dx)^T gbz (gbz in the reference interior
  * order, n values), dfield = d<gbz, B_z>/d(Bx, By, Bz) (host double[3] or NULL). */
 int tdgl_vjp_bz(tdgl_ctx* ctx, const double* gbz_dev, double* gx_dev, double* dfield);
+/* Adjoint step solve (no counterpart in the reference; the linear system of
+ * the trapezoidal step's adjoint): out_dev = mu with (I - hdt J(x1)^T) mu = rhs,
+ * by TGCR (the solver of tgcr.py:96-156, with the transposed operator) to
+ * ||residual|| <= tol ||rhs||, at most max_iter iterations (<= 0: the
+ * reference's default cap); *iters (may be NULL) = iterations taken.  Uses the
+ * current field and material.  TDGL_E_NONFINITE when TGCR stops short. */
+int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1_dev, const double* rhs_dev, double hdt,
+                       double tol, int64_t max_iter, double* out_dev, int* iters);
 
 /* Reductions over internal-layout vectors (the reference's np.vdot(...).real,
  * np.linalg.norm(., 2) and np.linalg.norm(., inf): tgcr.py:116,129,136,140,149,152;

@@ -29,7 +29,7 @@ EXPORTS = [
     "tdgl_set_stencil_kz", "tdgl_get_stencil_kz", "tdgl_set_sweep_tile", "tdgl_get_sweep_tile",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
-    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_adjoint_solve", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
@@ -113,6 +113,7 @@ def load() -> C.CDLL:
         "tdgl_batch_get_sweep_tile": ([P], C.c_int),
         "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
         "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
+        "tdgl_adjoint_solve": ([P, P, P, D, D, I64, P, C.POINTER(C.c_int)], C.c_int),
         "tdgl_get_stencil_kz": ([P], C.c_int),
         "tdgl_batch_create": ([P, C.c_int, C.POINTER(P)], C.c_int),
         "tdgl_batch_destroy": ([P], C.c_int),
@@ -301,6 +302,19 @@ class Context:
         self._call("tdgl_vjp_bz", C.c_void_p(gbz.data_ptr()), C.c_void_p(gx.data_ptr()),
                    C.cast(df, C.c_void_p) if field else None)
         return tuple(df) if field else None
+
+    def adjoint_solve(self, x1, rhs, hdt: float, tol: float, max_iter: int = 0, out=None):
+        """mu with (I - hdt J(x1)^T) mu = rhs (TGCR, ||residual|| <= tol ||rhs||);
+        returns (mu, iterations)."""
+        for v in (x1, rhs):
+            if v.dtype != self.torch.float64 or v.numel() != self.len:
+                raise ValueError("adjoint_solve takes float64 vectors of the internal state length")
+        out = self.torch.empty_like(rhs) if out is None else out
+        it = C.c_int(0)
+        self._call("tdgl_adjoint_solve", C.c_void_p(x1.data_ptr()), C.c_void_p(rhs.data_ptr()),
+                   C.c_double(hdt), C.c_double(tol), C.c_int64(int(max_iter)), C.c_void_p(out.data_ptr()),
+                   C.byref(it))
+        return out, int(it.value)
 
     def dot(self, a, b) -> float:
         out = C.c_double()

@@ -13,8 +13,8 @@ discrete adjoint runs backwards:
     dL/dtheta += dt/2 mu^T (dF/dtheta(x_{n+1}) + dF/dtheta(x_n))
 
 with J^T w and the parameter derivatives from the library's adjoint kernels
-(``tdgl_vjp_f``, ``tdgl_vjp_bz``); the linear adjoint solves use FP64 GMRES on
-the GPU.  States are checkpointed every ``checkpoint_every`` steps in the
+(``tdgl_vjp_f``, ``tdgl_vjp_bz``); the linear adjoint solves are the library's
+TGCR on the transposed step operator (``tdgl_adjoint_solve``, FP64).  States are checkpointed every ``checkpoint_every`` steps in the
 forward pass and each segment is recomputed (bit for bit: the FP64 path is
 deterministic) before its adjoint sweep.
 
@@ -38,61 +38,6 @@ from .core.material import MaterialMap
 from .core.parameters import SimulationParameters
 from .engine import FilmEngine
 from .physics.applied_field import AppliedField
-
-
-def gmres(apply, b, tol: float = 1e-12, restart: int = 120, maxit: int = 2000):
-    """Solve apply(x) = b (FP64 torch vectors on one device): restarted GMRES
-    with classical Gram-Schmidt done twice and Givens rotations."""
-    import torch
-    dev = b.device
-    x = torch.zeros_like(b)
-    bnorm = float(torch.linalg.vector_norm(b))
-    if bnorm == 0.0:
-        return x, 0
-    r = b.clone()
-    total = 0
-    V = torch.empty((restart + 1, b.numel()), dtype=b.dtype, device=dev)
-    while total < maxit:
-        beta = float(torch.linalg.vector_norm(r))
-        if beta <= tol * bnorm:
-            return x, total
-        V[0] = r / beta
-        H = np.zeros((restart + 1, restart))
-        cs, sn = np.zeros(restart), np.zeros(restart)
-        gvec = np.zeros(restart + 1)
-        gvec[0] = beta
-        jlast = 0
-        for j in range(restart):
-            w = apply(V[j])
-            h = V[:j + 1] @ w
-            w = w - V[:j + 1].T @ h
-            h2 = V[:j + 1] @ w
-            w = w - V[:j + 1].T @ h2
-            hv = (h + h2).cpu().numpy()
-            hn = float(torch.linalg.vector_norm(w))
-            H[:j + 1, j] = hv
-            H[j + 1, j] = hn
-            if hn > 0:
-                V[j + 1] = w / hn
-            for i in range(j):  # previous rotations
-                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
-                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
-                H[i, j] = t
-            d = np.hypot(H[j, j], H[j + 1, j])
-            cs[j], sn[j] = (H[j, j] / d, H[j + 1, j] / d) if d > 0 else (1.0, 0.0)
-            H[j, j] = d
-            H[j + 1, j] = 0.0
-            gvec[j + 1] = -sn[j] * gvec[j]
-            gvec[j] = cs[j] * gvec[j]
-            total += 1
-            jlast = j
-            if abs(gvec[j + 1]) <= tol * bnorm or hn == 0.0 or total >= maxit:
-                break
-        k = jlast + 1
-        y = np.linalg.solve(np.triu(H[:k, :k]), gvec[:k])
-        x = x + V[:k].T @ torch.as_tensor(y, dtype=b.dtype, device=dev)
-        r = b - apply(x)
-    return x, total
 
 
 def _internal_to_reference(params: SimulationParameters) -> np.ndarray:
@@ -218,11 +163,7 @@ class AdjointProblem:
                 self._set_field(bz, t)
                 x1, x0 = states[m + 1 - lo], states[m - lo]
 
-                def apply(v, x1=x1):
-                    ctx.vjp_f(x1, v, g)
-                    return v - hdt * g
-
-                mu, it = gmres(apply, lam, tol=self.adjoint_tol)
+                mu, it = ctx.adjoint_solve(x1, lam, hdt, self.adjoint_tol)
                 iters += it
                 mu = mu.contiguous()
                 f1, k1 = ctx.vjp_f(x1, mu, g, field=True, kappa=True, dsc=tmp)
@@ -232,7 +173,7 @@ class AdjointProblem:
                 lam = mu + hdt * g
                 d_kappa += hdt * (k1 + k0)
                 d_bz += hdt * (f1[2] + f0[2]) * self.profile(t)
-        self.stats["adjoint_gmres_iters"] = iters
+        self.stats["adjoint_krylov_iters"] = iters
         # dL/dw at the interior nodes: sc = base * (1 - w)
         d_sc_ref = np.empty(n)
         d_sc_ref[self._perm] = dsc.cpu().numpy()

@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
 // in a fixed order (the six neighbours' and the node's own slots, then the
 // diagonals); every cell and component is written.
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A) {
+__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, double* B, double* A) {
   const long long n = g.n;
   const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
   for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
@@ -280,21 +280,29 @@ __global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const doubl
       return true;
     };
     constexpr int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    // every (plane, node) entry is read by exactly one cell, which clears it:
+    // the planes are zero again for the next k_vjp_cells
+    auto take = [&](int pl, long long q) {
+      double* p = B + (long long)pl * n + q;
+      const double v = *p;
+      *p = 0.0;
+      return v;
+    };
 #pragma unroll
     for (int s = 0; s < (IS3D ? 7 : 5); ++s) {
       long long q;
       if (src(off[s][0], off[s][1], off[s][2], q))
 #pragma unroll
-        for (int comp = 0; comp < 5; ++comp) a[comp] += B[(long long)plane(s, comp) * n + q];
+        for (int comp = 0; comp < 5; ++comp) a[comp] += take(plane(s, comp), q);
     }
     long long q;
-    if (src(1, -1, 0, q)) a[LY] += B[(long long)D_XP_YM * n + q];
-    if (src(-1, 1, 0, q)) a[LX] += B[(long long)D_XM_YP * n + q];
+    if (src(1, -1, 0, q)) a[LY] += take(D_XP_YM, q);
+    if (src(-1, 1, 0, q)) a[LX] += take(D_XM_YP, q);
     if (IS3D) {
-      if (src(1, 0, -1, q)) a[LZ] += B[(long long)D_XP_ZM * n + q];
-      if (src(0, 1, -1, q)) a[LZ] += B[(long long)D_YP_ZM * n + q];
-      if (src(-1, 0, 1, q)) a[LX] += B[(long long)D_XM_ZP * n + q];
-      if (src(0, -1, 1, q)) a[LY] += B[(long long)D_YM_ZP * n + q];
+      if (src(1, 0, -1, q)) a[LZ] += take(D_XP_ZM, q);
+      if (src(0, 1, -1, q)) a[LZ] += take(D_YP_ZM, q);
+      if (src(-1, 0, 1, q)) a[LX] += take(D_XM_ZP, q);
+      if (src(0, -1, 1, q)) a[LY] += take(D_YM_ZP, q);
     }
 #pragma unroll
     for (int comp = 0; comp < 5; ++comp) A[c * 5 + comp] = a[comp];
@@ -429,9 +437,44 @@ __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const do
   }
 }
 
+// The adjoint step operator inside TGCR: Ap = r - hdt (J^T r) in place of
+// Ap = J^T r, and the partials of <Ap, Ap_0> (or ||Ap||^2 without Ap_0) that
+// the first Gram-Schmidt pass (or k_final_p) consumes -- what the FD-JVP
+// stencil's epilogue gives the forward solve.
+__global__ __launch_bounds__(kRedThreads) void k_adj_combine(long long len, const double* r, double* ap,
+                                                             double hdt, const double* ap0, Red rd) {
+  if (rd.res[RS_STOP] != 0.0) return;  // a queued iteration after the loop ended
+  double v[1] = {0.0};
+  for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
+       e += (long long)gridDim.x * kRedThreads) {
+    const double a = r[e] - hdt * ap[e];
+    ap[e] = a;
+    v[0] += a * (ap0 ? ap0[e] : a);
+  }
+  publish<1, 0>(v, rd.pout);
+}
+
+// y = s x (len doubles; y may alias x).
+__global__ __launch_bounds__(kRedThreads) void k_scale(long long len, double* y, double s, const double* x) {
+  for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
+       e += (long long)gridDim.x * kRedThreads)
+    y[e] = s * x[e];
+}
+
 // --------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------
+
+int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, const double* ap0,
+                       const Red& rd, hipStream_t st) {
+  const long long len = (long long)(2 + g.nphi) * g.n;
+  launch(k_adj_combine, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, r, ap, hdt, ap0, rd);
+  return stream_blocks(g);
+}
+void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st) {
+  const long long len = (long long)(2 + g.nphi) * g.n;
+  launch(k_scale, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, y, s, x);
+}
 
 int vjp_planes() { return kVjpPlanes; }
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
@@ -440,7 +483,7 @@ void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x
   if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
   else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
 }
-void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st) {
+void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st) {
   const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
   long long nb = (dimx + kRedThreads - 1) / kRedThreads;
   if (nb > 4096) nb = 4096;

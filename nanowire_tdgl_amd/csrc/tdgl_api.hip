@@ -492,13 +492,13 @@ int ensure_acell(tdgl_ctx* ctx, bool clear = true) {
   if (clear) HIPCHK(hipMemsetAsync(ctx->d_acell, 0, sizeof(double) * 5 * (size_t)dimx, ctx->st));
   return TDGL_OK;
 }
-// The per-node scatter planes of the VJP, zeroed.
+// The per-node scatter planes of the VJP: zeroed at allocation, and cleared
+// again by every launch_cells_gather.
 int ensure_vjpb(tdgl_ctx* ctx) {
+  if (ctx->d_vjpb) return TDGL_OK;
   const size_t bytes = sizeof(double) * (size_t)vjp_planes() * (size_t)ctx->g.n;
-  if (!ctx->d_vjpb) {
-    int rc;
-    if ((rc = alloc(ctx, (void**)&ctx->d_vjpb, bytes))) return rc;
-  }
+  int rc;
+  if ((rc = alloc(ctx, (void**)&ctx->d_vjpb, bytes))) return rc;
   HIPCHK(hipMemsetAsync(ctx->d_vjpb, 0, bytes, ctx->st));
   return TDGL_OK;
 }
@@ -638,11 +638,19 @@ double mgs_vectors(int i) {
   return v;
 }
 
+// The operator TGCR applies: the reference's finite-difference JVP of f at w_x
+// (default), or the adjoint step operator A^T = I - hdt J(x1)^T.
+struct TgcrOp {
+  int vjp = 0;
+  const double* x1 = nullptr;   // vjp: the linearisation point
+  double* acc_out = nullptr;    // x += sum alpha p accumulates here (default w_x)
+};
+
 // tgcr_matrix_free_trap (tgcr.py:96-156) on (I - dt/2 J) dx = r, r = -g held in
 // w_r, linearised at w_x with f(w_x) in w_f.  On success w_x += dx.
 // Returns TDGL_OK and sets *ok; *ok = 0 is the reference's empty-array return.
 int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double xx, int* ok,
-         tdgl_step_stats* s) {
+         tdgl_step_stats* s, const TgcrOp& op = TgcrOp{}) {
   const Geo& g = ctx->g;
   const double S = state_bytes(ctx);
   const long long N = (1 + g.nphi) * g.n;  // complex entries of the reference state
@@ -673,18 +681,29 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     const int i = (int)(kk - 1);  // every earlier iteration added a vector
     int rc2;
     if ((rc2 = ensure_vec(ctx, i))) return rc2;
-    Epi e{};
-    e.out0 = ctx->AP[i];
-    e.f0 = ctx->w_f;
-    e.ap0 = (i > 0) ? ctx->AP[0] : nullptr;
-    e.hdt = hdt;
-    e.tgcr = 1;
-    e.eps_num = eps_num;
-    // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
-    int tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S);
-    red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, 0.0, e,
-                                   ctx->cfg, red_next(ctx), ctx->st));
-    prof_end(ctx, tok);
+    int tok;
+    if (op.vjp) {  // Ap = r - hdt J(x1)^T r, then <Ap, Ap_0> (or ||Ap||^2)
+      tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S, 4);
+      launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, ctx->w_r, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st);
+      launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st);
+      launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st);
+      red_commit(ctx, launch_adj_combine(g, ctx->w_r, ctx->AP[i], hdt, (i > 0) ? ctx->AP[0] : nullptr,
+                                         red_next(ctx), ctx->st));
+      prof_end(ctx, tok);
+    } else {
+      Epi e{};
+      e.out0 = ctx->AP[i];
+      e.f0 = ctx->w_f;
+      e.ap0 = (i > 0) ? ctx->AP[0] : nullptr;
+      e.hdt = hdt;
+      e.tgcr = 1;
+      e.eps_num = eps_num;
+      // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
+      tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S);
+      red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, 0.0, e,
+                                     ctx->cfg, red_next(ctx), ctx->st));
+      prof_end(ctx, tok);
+    }
     // Gram-Schmidt against Ap_0 .. Ap_{i-1} (tgcr.py:135-140) in blocked
     // passes: each subtracts the previous pass's new vectors (the first: Ap_0,
     // whose beta the JVP reduced) and reduces the dots with the next
@@ -743,7 +762,8 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     return TDGL_OK;
   }
   const int tok = prof_begin(ctx, TDGL_K_ACCUM, (double)(nvec + 2) * S);  // read x, p_i; write x
-  red_commit(ctx, launch_accum(g, ctx->w_x, ctx->d_ptab, nvec, red_next(ctx), ctx->st));
+  red_commit(ctx, launch_accum(g, op.acc_out ? op.acc_out : ctx->w_x, ctx->d_ptab, nvec, red_next(ctx),
+                               ctx->st));
   prof_end(ctx, tok);
   red_finalize(ctx, 0, 1, RS_DXINF);
   LAUNCHCHK();
@@ -827,3 +847,45 @@ extern "C" int tdgl_profile_read(tdgl_ctx* ctx, int cls, double* ms, int64_t* n,
   if (bytes) *bytes = ctx->prof.bytes[cls];
   return TDGL_OK;
 }
+
+extern "C" {
+
+int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1, const double* rhs, double hdt, double tol,
+                       int64_t max_iter, double* out, int* iters) {
+  if (!ctx || !x1 || !rhs || !out || !(tol > 0.0) || !(hdt >= 0.0)) return TDGL_E_INVALID;
+  int rc;
+  if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0)) || (rc = ensure_acell(ctx, false)) ||
+      (rc = ensure_vjpb(ctx)))
+    return rc;
+  const Geo& g = ctx->g;
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  HIPCHK(hipMemsetAsync(out, 0, bytes, ctx->st));
+  if (iters) *iters = 0;
+  double b2;
+  if ((rc = tdgl_dot(ctx, rhs, rhs, &b2))) return rc;
+  if (!std::isfinite(b2)) return fail(ctx, TDGL_E_NONFINITE, "adjoint right-hand side is not finite");
+  if (b2 == 0.0) return TDGL_OK;
+  // unit right-hand side: TGCR's breakdown test (||Ap|| < 1e-14) is absolute
+  const double bn = sqrt(b2);
+  launch_scale(g, ctx->w_r, 1.0 / bn, rhs, ctx->st);
+  double rr;
+  if ((rc = tdgl_dot(ctx, ctx->w_r, ctx->w_r, &rr))) return rc;
+  HIPCHK(hipMemcpyAsync(ctx->d_res + RS_RR, &rr, sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  tdgl_solver_opts o{};
+  o.tol_gcr = tol;
+  o.tgcr_max_iter = max_iter;
+  tdgl_step_stats st{};
+  TgcrOp op;
+  op.vjp = 1;
+  op.x1 = x1;
+  op.acc_out = out;
+  int ok = 0;
+  if ((rc = tgcr(ctx, hdt, &o, rr, 0.0, &ok, &st, op))) return rc;
+  if (iters) *iters = st.tgcr_iters_last;
+  if (!ok) return fail(ctx, TDGL_E_NONFINITE, "adjoint TGCR did not reach the tolerance");
+  launch_scale(g, out, bn, out, ctx->st);
+  LAUNCHCHK();
+  return TDGL_OK;
+}
+
+}  // extern "C"

@@ -226,12 +226,17 @@ void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* 
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st);
 // Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.  B:
-// vjp_planes() x n per-node scatter planes, zeroed before launch_vjp_cells,
-// turned into A by launch_cells_gather.
+// vjp_planes() x n per-node scatter planes, zero before launch_vjp_cells
+// (zeroed once at allocation), turned into A by launch_cells_gather, which
+// clears them again.
 int vjp_planes();
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
                       double* B, double* gsc, double kappa, double* pk, hipStream_t st);
-void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st);
+void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st);
+// TGCR on the adjoint step operator: ap = r - hdt ap, partials of <ap, ap0> (or ||ap||^2).
+int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, const double* ap0,
+                       const Red& rd, hipStream_t st);
+void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st);
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);

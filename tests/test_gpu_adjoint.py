@@ -10,6 +10,10 @@
 * B_z observable adjoint: <g, B_z(x + p) - B_z(x)> (oracle, B_z affine in x)
   equals <(dB_z/dx)^T g, p>, and its field derivative matches a central
   difference.
+* Adjoint step solve (tdgl_adjoint_solve, TGCR on I - dt/2 J^T): the residual
+  mu - dt/2 J(x)^T mu - b, with J^T from tdgl_vjp_f, is <= 1e-11 of |b|;
+  the solve is linear in b (a scaled right-hand side gives the scaled mu to
+  1e-12) and a zero right-hand side gives zero in no iterations.
 * Whole trajectory: d L / d(kappa, Bz, w) of L = mean over the mid-plane of
   (B_z - T)^2 after several trapezoidal steps, from the adjoint behind a
   torch.autograd.Function with checkpointed recomputation, against central
@@ -36,6 +40,32 @@ def _rdot(a, b) -> float:
 def _engine(dev):
     from nanowire_tdgl_amd.engine import engine_for
     return engine_for(dev.params, dev.idx, dev.material)
+
+
+@pytest.mark.parametrize("name", ["hole_bz", "plain2d", "periodic_xyz", "trilayer_hole"])
+def test_adjoint_solve(name):
+    import torch
+    case = S.EVALF_CASES[name]
+    dev = device(case)
+    eng = _engine(dev)
+    eng.set_field(*case["field"])
+    n = dev.params.n_interior
+    nphi = 3 if dev.params.Nz > 1 else 2
+    ctx = eng.ctx
+    x1 = ctx.to_internal(rand_state(n, nphi, 81, 0.3, 0.2))
+    b = ctx.to_internal(rand_state(n, nphi, 82, 1e-3, 1e-3))  # small: the solve normalises
+    hdt = 0.1
+    mu, it = ctx.adjoint_solve(x1, b, hdt, 1e-12)
+    g = torch.empty_like(mu)
+    ctx.vjp_f(x1, mu, g)
+    res = float(torch.linalg.vector_norm(mu - hdt * g - b)) / float(torch.linalg.vector_norm(b))
+    print(f"{name}: adjoint TGCR {it} iterations, relative residual {res:.2e}")
+    assert it > 0 and res <= 1e-11
+    mu3, _ = ctx.adjoint_solve(x1, 3.0 * b, hdt, 1e-12)
+    lin = float(torch.linalg.vector_norm(mu3 - 3.0 * mu)) / float(torch.linalg.vector_norm(3.0 * mu))
+    assert lin <= 1e-12, lin
+    z, it0 = ctx.adjoint_solve(x1, torch.zeros_like(b), hdt, 1e-12)
+    assert it0 == 0 and not bool(z.any())
 
 
 @pytest.mark.parametrize("name", list(S.EVALF_CASES))
@@ -176,7 +206,7 @@ def test_trajectory_gradient_matches_finite_differences():
     w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
     loss = midplane_target_loss(prob, TrapezoidalBz.apply(prob, kappa, bz, w), width=3.0)
     loss.backward()
-    print("adjoint GMRES iterations:", prob.stats.get("adjoint_gmres_iters"))
+    print("adjoint TGCR iterations:", prob.stats.get("adjoint_krylov_iters"))
 
     def L(k, b, ww):
         _, bzo = prob.forward(k, b, ww)
