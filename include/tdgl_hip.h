@@ -205,10 +205,12 @@ int tdgl_vjp_bz(tdgl_ctx* ctx, const double* gbz_dev, double* gx_dev, double* df
  * the trapezoidal step's adjoint): out_dev = mu with (I - hdt J(x1)^T) mu = rhs,
  * by TGCR (the solver of tgcr.py:96-156, with the transposed operator) to
  * ||residual|| <= tol ||rhs||, at most max_iter iterations (<= 0: the
- * reference's default cap); *iters (may be NULL) = iterations taken.  Uses the
- * current field and material.  TDGL_E_NONFINITE when TGCR stops short. */
+ * reference's default cap); *iters (may be NULL) = iterations taken.
+ * precond: the Chebyshev right preconditioner's degree (-1 automatic, as the
+ * mixed method's; 0 none).  Uses the current field and material.
+ * TDGL_E_NONFINITE when TGCR stops short. */
 int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1_dev, const double* rhs_dev, double hdt,
-                       double tol, int64_t max_iter, double* out_dev, int* iters);
+                       double tol, int64_t max_iter, int precond, double* out_dev, int* iters);
 
 /* Reductions over internal-layout vectors (the reference's np.vdot(...).real,
  * np.linalg.norm(., 2) and np.linalg.norm(., inf): tgcr.py:116,129,136,140,149,152;

@@ -113,7 +113,7 @@ def load() -> C.CDLL:
         "tdgl_batch_get_sweep_tile": ([P], C.c_int),
         "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
         "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
-        "tdgl_adjoint_solve": ([P, P, P, D, D, I64, P, C.POINTER(C.c_int)], C.c_int),
+        "tdgl_adjoint_solve": ([P, P, P, D, D, I64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
         "tdgl_get_stencil_kz": ([P], C.c_int),
         "tdgl_batch_create": ([P, C.c_int, C.POINTER(P)], C.c_int),
         "tdgl_batch_destroy": ([P], C.c_int),
@@ -303,17 +303,19 @@ class Context:
                    C.cast(df, C.c_void_p) if field else None)
         return tuple(df) if field else None
 
-    def adjoint_solve(self, x1, rhs, hdt: float, tol: float, max_iter: int = 0, out=None):
-        """mu with (I - hdt J(x1)^T) mu = rhs (TGCR, ||residual|| <= tol ||rhs||);
-        returns (mu, iterations)."""
+    def adjoint_solve(self, x1, rhs, hdt: float, tol: float, max_iter: int = 0, out=None,
+                      precond="chebyshev"):
+        """mu with (I - hdt J(x1)^T) mu = rhs (TGCR, ||residual|| <= tol ||rhs||),
+        right-preconditioned as `precond` says (as for the mixed method:
+        "chebyshev" automatic degree, "none", or a degree); returns (mu, iterations)."""
         for v in (x1, rhs):
             if v.dtype != self.torch.float64 or v.numel() != self.len:
                 raise ValueError("adjoint_solve takes float64 vectors of the internal state length")
         out = self.torch.empty_like(rhs) if out is None else out
         it = C.c_int(0)
         self._call("tdgl_adjoint_solve", C.c_void_p(x1.data_ptr()), C.c_void_p(rhs.data_ptr()),
-                   C.c_double(hdt), C.c_double(tol), C.c_int64(int(max_iter)), C.c_void_p(out.data_ptr()),
-                   C.byref(it))
+                   C.c_double(hdt), C.c_double(tol), C.c_int64(int(max_iter)),
+                   C.c_int(precond_code(precond)), C.c_void_p(out.data_ptr()), C.byref(it))
         return out, int(it.value)
 
     def dot(self, a, b) -> float:

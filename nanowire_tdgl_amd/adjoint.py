@@ -14,7 +14,8 @@ discrete adjoint runs backwards:
 
 with J^T w and the parameter derivatives from the library's adjoint kernels
 (``tdgl_vjp_f``, ``tdgl_vjp_bz``); the linear adjoint solves are the library's
-TGCR on the transposed step operator (``tdgl_adjoint_solve``, FP64).  States are checkpointed every ``checkpoint_every`` steps in the
+TGCR on the transposed step operator (``tdgl_adjoint_solve``, FP64, with the
+mixed method's Chebyshev right preconditioner by default).  States are checkpointed every ``checkpoint_every`` steps in the
 forward pass and each segment is recomputed (bit for bit: the FP64 path is
 deterministic) before its adjoint sweep.
 
@@ -68,6 +69,7 @@ class AdjointProblem:
     newton_tol_dx: float = 1e-8
     tol_gcr: float = 1e-4
     adjoint_tol: float = 1e-12
+    adjoint_precond: object = "chebyshev"  # the adjoint TGCR's preconditioner (as the mixed method's)
 
     def __post_init__(self):
         self.device = Device(self.params)
@@ -163,7 +165,7 @@ class AdjointProblem:
                 self._set_field(bz, t)
                 x1, x0 = states[m + 1 - lo], states[m - lo]
 
-                mu, it = ctx.adjoint_solve(x1, lam, hdt, self.adjoint_tol)
+                mu, it = ctx.adjoint_solve(x1, lam, hdt, self.adjoint_tol, precond=self.adjoint_precond)
                 iters += it
                 mu = mu.contiguous()
                 f1, k1 = ctx.vjp_f(x1, mu, g, field=True, kappa=True, dsc=tmp)

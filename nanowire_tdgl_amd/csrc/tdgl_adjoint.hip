@@ -454,6 +454,22 @@ __global__ __launch_bounds__(kRedThreads) void k_adj_combine(long long len, cons
   publish<1, 0>(v, rd.pout);
 }
 
+// One Chebyshev step for A^T z = r, A^T v = v - hdt J^T v (g = J^T zp):
+// z = om (zp + (r - A^T zp) thinv) - (om - 1) zpp, zpp = nullptr meaning 0.
+__global__ __launch_bounds__(kRedThreads) void k_cheb64(long long len, double* z, const double* zp,
+                                                        const double* zpp, const double* r,
+                                                        const double* gv, double om, double thinv,
+                                                        double hdt, const double* stop) {
+  if (*stop != 0.0) return;  // a queued iteration after the loop ended
+  for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
+       e += (long long)gridDim.x * kRedThreads) {
+    const double a = zp[e] - hdt * gv[e];
+    double v = om * (zp[e] + (r[e] - a) * thinv);
+    if (zpp) v = v - (om - 1.0) * zpp[e];
+    z[e] = v;
+  }
+}
+
 // y = s x (len doubles; y may alias x).
 __global__ __launch_bounds__(kRedThreads) void k_scale(long long len, double* y, double s, const double* x) {
   for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
@@ -470,6 +486,11 @@ int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, co
   const long long len = (long long)(2 + g.nphi) * g.n;
   launch(k_adj_combine, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, r, ap, hdt, ap0, rd);
   return stream_blocks(g);
+}
+void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
+                   const double* gv, double om, double thinv, double hdt, const double* stop, hipStream_t st) {
+  const long long len = (long long)(2 + g.nphi) * g.n;
+  launch(k_cheb64, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, z, zp, zpp, r, gv, om, thinv, hdt, stop);
 }
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st) {
   const long long len = (long long)(2 + g.nphi) * g.n;

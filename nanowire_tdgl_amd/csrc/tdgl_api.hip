@@ -245,6 +245,8 @@ int tdgl_ctx_destroy(tdgl_ctx* ctx) {
   if (ctx->own_batch) tdgl_batch_destroy(ctx->own_batch);
   if (ctx->d_acell) hipFree(ctx->d_acell);
   if (ctx->d_vjpb) hipFree(ctx->d_vjpb);
+  for (double* p : ctx->d_adjt)
+    if (p) hipFree(p);
   ctx->prof.destroy();
   if (ctx->h_res) hipHostFree(ctx->h_res);
   if (ctx->h_la) hipHostFree(ctx->h_la);
@@ -644,6 +646,11 @@ struct TgcrOp {
   int vjp = 0;
   const double* x1 = nullptr;   // vjp: the linearisation point
   double* acc_out = nullptr;    // x += sum alpha p accumulates here (default w_x)
+  // vjp: right preconditioner M = the degree-d Chebyshev iterate for A^T z = r
+  // on [a, b] (p_k = M r_k, the direction's source in k_final_p); 0: none
+  int cheb = 0;
+  double thinv = 0.0;           // 1 / theta
+  double om[kMaxChebDeg + 1] = {0.0};
 };
 
 // tgcr_matrix_free_trap (tgcr.py:96-156) on (I - dt/2 J) dx = r, r = -g held in
@@ -682,12 +689,27 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     int rc2;
     if ((rc2 = ensure_vec(ctx, i))) return rc2;
     int tok;
-    if (op.vjp) {  // Ap = r - hdt J(x1)^T r, then <Ap, Ap_0> (or ||Ap||^2)
+    const double* zsrc = nullptr;  // the direction's source when preconditioned
+    if (op.vjp) {  // Ap = z - hdt J(x1)^T z (z = M r, or r), then <Ap, Ap_0> (or ||Ap||^2)
+      auto jt = [&](const double* v) {  // J(x1)^T v into AP[i]
+        launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, v, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st);
+        launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st);
+        launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st);
+      };
       tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S, 4);
-      launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, ctx->w_r, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st);
-      launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st);
-      launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st);
-      red_commit(ctx, launch_adj_combine(g, ctx->w_r, ctx->AP[i], hdt, (i > 0) ? ctx->AP[0] : nullptr,
+      const double* z = ctx->w_r;
+      if (op.cheb >= 1) {
+        double* const* T = ctx->d_adjt;
+        launch_scale(g, T[1], op.thinv, ctx->w_r, ctx->st);  // z_1 = r / theta
+        for (int m = 2; m <= op.cheb; ++m) {
+          jt(T[(m - 1) % 3]);
+          launch_cheb64(g, T[m % 3], T[(m - 1) % 3], m >= 3 ? T[(m - 2) % 3] : nullptr, ctx->w_r,
+                        ctx->AP[i], op.om[m], op.thinv, hdt, ctx->d_res + RS_STOP, ctx->st);
+        }
+        z = zsrc = T[op.cheb % 3];
+      }
+      jt(z);
+      red_commit(ctx, launch_adj_combine(g, z, ctx->AP[i], hdt, (i > 0) ? ctx->AP[0] : nullptr,
                                          red_next(ctx), ctx->st));
       prof_end(ctx, tok);
     } else {
@@ -725,7 +747,7 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     // read r, p_0..p_{i-1}, Ap; write p_i, Ap
     tok = prof_begin(ctx, TDGL_K_FINAL_P, (double)(i + 4) * S);
     red_commit(ctx, launch_final_p(g, ctx->w_r, ctx->d_ptab, i, ctx->AP[i], ctx->P[i], red_next(ctx),
-                                   ctx->st));
+                                   ctx->st, zsrc));
     prof_end(ctx, tok);
     tok = prof_begin(ctx, TDGL_K_UPDATE_R, 3.0 * S);  // read r, Ap; write r
     red_commit(ctx, launch_update_r(g, ctx->w_r, ctx->AP[i], RS_ALPHA + i, red_next(ctx), ctx->st));
@@ -851,8 +873,10 @@ extern "C" int tdgl_profile_read(tdgl_ctx* ctx, int cls, double* ms, int64_t* n,
 extern "C" {
 
 int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1, const double* rhs, double hdt, double tol,
-                       int64_t max_iter, double* out, int* iters) {
-  if (!ctx || !x1 || !rhs || !out || !(tol > 0.0) || !(hdt >= 0.0)) return TDGL_E_INVALID;
+                       int64_t max_iter, int precond, double* out, int* iters) {
+  if (!ctx || !x1 || !rhs || !out || !(tol > 0.0) || !(hdt >= 0.0) || precond < -1 ||
+      precond > kMaxChebDeg)
+    return TDGL_E_INVALID;
   int rc;
   if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0)) || (rc = ensure_acell(ctx, false)) ||
       (rc = ensure_vjpb(ctx)))
@@ -879,6 +903,21 @@ int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1, const double* rhs, doubl
   op.vjp = 1;
   op.x1 = x1;
   op.acc_out = out;
+  // Chebyshev right preconditioner on [1 - hdt, b] (the mixed method's, for the
+  // transposed operator: same spectrum); off when the interval nears 0
+  const double lo = 1.0 - hdt, hi = cheb_upper_bound(ctx, hdt);
+  if (precond != 0 && lo >= 0.1 && hi > lo) {
+    const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sig = th / de, s2 = sig * sig;
+    int d = precond > 0 ? precond : (int)ceil(acosh(4.0) / acosh(sig));
+    d = std::max(1, std::min(d, kMaxChebDeg));
+    op.cheb = d;
+    op.thinv = 1.0 / th;
+    op.om[1] = 1.0;
+    if (d >= 2) op.om[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
+    for (int m = 3; m <= d; ++m) op.om[m] = 1.0 / (1.0 - op.om[m - 1] / (4.0 * s2));
+    for (double*& p : ctx->d_adjt)
+      if (!p && (rc = alloc(ctx, (void**)&p, bytes))) return rc;
+  }
   int ok = 0;
   if ((rc = tgcr(ctx, hdt, &o, rr, 0.0, &ok, &st, op))) return rc;
   if (iters) *iters = st.tgcr_iters_last;

@@ -803,6 +803,11 @@ int batch_init(tdgl_batch* B) {
 }  // namespace
 
 namespace tdgl {
+double cheb_upper_bound(const tdgl_ctx* c, double hdt) { return cheb_upper(c, hdt); }
+}  // namespace tdgl
+
+
+namespace tdgl {
 
 // The context's own batch of one: its profile and tuning are the context's.
 int own_batch(tdgl_ctx* ctx) {

@@ -148,6 +148,7 @@ struct tdgl_ctx {
   int tile_user = tdgl::kDefaultSweepShape;  // tdgl_set_sweep_tile (mixed-precision sweeps)
   double* d_acell = nullptr;  // adjoint: full grid x 5 cell adjoint
   double* d_vjpb = nullptr;   // adjoint: vjp_planes() x n scatter planes
+  double* d_adjt[3] = {nullptr, nullptr, nullptr};  // adjoint solve: Chebyshev iterates
   double kappa_uniform = 0.0; // the kappa of the uniform link coefficients
   double k2max = 0.0;         // largest kappa^2 of any node (preconditioner bounds)
   int pb_blocks = 0;  // capacity of d_pb in blocks
@@ -161,6 +162,9 @@ int mixed_step_single(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opt
                       tdgl_step_stats* s);
 int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32);
 // One application of the mixed TGCR's Chebyshev preconditioner (tdgl_batch.hip).
+// Upper end b of the Chebyshev interval [1 - hdt, b] of I - hdt J (a
+// Gershgorin bound of the stencil; the spectrum of the transpose is the same).
+double cheb_upper_bound(const tdgl_ctx* c, double hdt);
 int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt, int degree,
                    float* w32, float* z32, double* info);
 }  // namespace tdgl

@@ -743,9 +743,11 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double
 // The p updates of tgcr.py:137 are applied here, per node, in the same j order,
 // which rounds exactly like the reference's vector-by-vector updates.
 template <bool IS3D>
+// zv (right preconditioning, M r): the direction's source in place of r; may alias pout.
 __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const double* rv,
                                                          const double* const* ptab, int nprev,
-                                                         double* ap, double* pout, Red r) {
+                                                         double* ap, double* pout, const double* zv,
+                                                         Red r) {
   if (r.res[RS_STOP] != 0.0) return;  // a queued iteration after the loop ended
   double s1[1];
   gather<1, 0>(r.pin, r.pcount, s1);
@@ -762,7 +764,7 @@ __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const doub
   double v[1] = {0.0};
   GRID_STRIDE(q) {
     const V5 rr = ld5<IS3D>(rv, n, q);
-    V5 p = rr;
+    V5 p = zv ? ld5<IS3D>(zv, n, q) : rr;
     for (int jj = 0; jj < nprev; ++jj) p = axmy(p, beta[jj], ld5<IS3D>(ptab[jj], n, q));
     const V5 a = scal(ld5<IS3D>(ap, n, q), inv);
     p = scal(p, inv);
@@ -893,8 +895,8 @@ int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn,
   return mgs_blocks(len);
 }
 int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
-                   double* pout, const Red& r, hipStream_t st) {
-  LAUNCH_V(k_final_p, g.n, rv, ptab, nprev, ap, pout, r);
+                   double* pout, const Red& r, hipStream_t st, const double* zv) {
+  LAUNCH_V(k_final_p, g.n, rv, ptab, nprev, ap, pout, zv, r);
   return stream_blocks(g);
 }
 int launch_update_r(const Geo& g, double* rv, const double* ap, int aslot, const Red& r,

@@ -166,7 +166,7 @@ int launch_mgs_block(const Geo& g, double* ap, const MgsVecs& v, int no, int nn,
 void launch_tgcr_check(double r0, double tol, long long k, long long maxk, const Red& r, hipStream_t st);
 int mgs_blocks(long long len);  // blocks of a blocked MGS pass over vectors of len doubles
 int launch_final_p(const Geo& g, const double* rv, const double* const* ptab, int nprev, double* ap,
-                   double* pout, const Red& r, hipStream_t st);
+                   double* pout, const Red& r, hipStream_t st, const double* zv = nullptr);
 int launch_update_r(const Geo& g, double* rv, const double* ap, int alpha_slot, const Red& r,
                     hipStream_t st);
 int launch_accum(const Geo& g, double* x, const double* const* ptab, int nvec, const Red& r,
@@ -237,6 +237,12 @@ void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st);
 int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, const double* ap0,
                        const Red& rd, hipStream_t st);
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st);
+// One Chebyshev step of the adjoint solve's preconditioner, FP64:
+// z = om (zp + (r - (zp - hdt g)) thinv) - (om - 1) zpp   (zpp = nullptr: 0),
+// g = J^T zp; returns at once when *stop (res[RS_STOP]) is set.
+void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
+                   const double* gv, double om, double thinv, double hdt, const double* stop,
+                   hipStream_t st);
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);

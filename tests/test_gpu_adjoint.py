@@ -42,8 +42,9 @@ def _engine(dev):
     return engine_for(dev.params, dev.idx, dev.material)
 
 
+@pytest.mark.parametrize("precond", ["chebyshev", "none", 3])
 @pytest.mark.parametrize("name", ["hole_bz", "plain2d", "periodic_xyz", "trilayer_hole"])
-def test_adjoint_solve(name):
+def test_adjoint_solve(name, precond):
     import torch
     case = S.EVALF_CASES[name]
     dev = device(case)
@@ -55,17 +56,19 @@ def test_adjoint_solve(name):
     x1 = ctx.to_internal(rand_state(n, nphi, 81, 0.3, 0.2))
     b = ctx.to_internal(rand_state(n, nphi, 82, 1e-3, 1e-3))  # small: the solve normalises
     hdt = 0.1
-    mu, it = ctx.adjoint_solve(x1, b, hdt, 1e-12)
+    mu, it = ctx.adjoint_solve(x1, b, hdt, 1e-12, precond=precond)
     g = torch.empty_like(mu)
     ctx.vjp_f(x1, mu, g)
     res = float(torch.linalg.vector_norm(mu - hdt * g - b)) / float(torch.linalg.vector_norm(b))
-    print(f"{name}: adjoint TGCR {it} iterations, relative residual {res:.2e}")
+    print(f"{name} ({precond}): adjoint TGCR {it} iterations, relative residual {res:.2e}")
     assert it > 0 and res <= 1e-11
-    mu3, _ = ctx.adjoint_solve(x1, 3.0 * b, hdt, 1e-12)
+    mu3, _ = ctx.adjoint_solve(x1, 3.0 * b, hdt, 1e-12, precond=precond)
     lin = float(torch.linalg.vector_norm(mu3 - 3.0 * mu)) / float(torch.linalg.vector_norm(3.0 * mu))
     assert lin <= 1e-12, lin
-    z, it0 = ctx.adjoint_solve(x1, torch.zeros_like(b), hdt, 1e-12)
+    z, it0 = ctx.adjoint_solve(x1, torch.zeros_like(b), hdt, 1e-12, precond=precond)
     assert it0 == 0 and not bool(z.any())
+    with pytest.raises(ValueError):
+        ctx.adjoint_solve(x1, b, hdt, 0.0, precond=precond)
 
 
 @pytest.mark.parametrize("name", list(S.EVALF_CASES))
