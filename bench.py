@@ -70,7 +70,7 @@ class Stepper:
     """The adaptive trapezoidal loop of the integrator, without history."""
 
     def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8, precision="f64",
-                 precond="chebyshev"):
+                 precond="chebyshev", **newton):
         from nanowire_tdgl_amd import _native
         from nanowire_tdgl_amd.engine import engine_for
         self.dev = dev
@@ -78,7 +78,8 @@ class Stepper:
         self.ctx = self.eng.ctx
         self.X = self.ctx.to_internal(x0)
         self.dt, self.cur, self.t, self.t_stop = dt, dt, 0.0, t_stop
-        self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision, precond=precond)
+        self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision, precond=precond,
+                                        **newton)
         self.log = []
 
     def accepted_steps(self, k: int) -> None:
@@ -110,6 +111,14 @@ class Stepper:
         b, j = self.ctx.observables(self.X)
         psi = self.ctx.to_reference(self.X).cpu().numpy()[:n]
         return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()
+
+
+# Newton step tolerance of the converged FP64 runs that the errors are also
+# measured against.  At the reference's 1e-3 (newton.py:84) the FP64 path's own
+# last Newton iterate sits ~1e-6 (|psi|^2) from the solution of the step
+# equations; FP64 runs at 1e-4 and 1e-5 agree to 1e-10.  Much below 1e-5 the
+# Newton iteration ends in TGCR's absolute breakdown test (tgcr.py:141-142).
+CONVERGED_NEWTON_DX = 1e-5
 
 
 def rel_errors(a, b) -> dict:
@@ -202,9 +211,19 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
     vu = n * steps / elu
     erru = rel_errors(u.observables(), fp64_stepper.observables())
     log(f"[bench] mixed, no preconditioner: {vu:.4g} cell-steps/s ({vu / v64:.2f}x FP64), err {erru}")
+    # the FP64 solution of the same step equations, Newton converged (untimed)
+    ref = Stepper(dev, x0, tol_gcr=tol, newton_tol_dx=CONVERGED_NEWTON_DX)
+    ref.accepted_steps(warmup + steps)
+    assert abs(ref.t - s.t) < 1e-9 and abs(fp64_stepper.t - s.t) < 1e-9
+    obs = ref.observables()
+    conv = {"mixed": rel_errors(s.observables(), obs),
+            "mixed_unpreconditioned": rel_errors(u.observables(), obs),
+            "fp64_same_tolerances": rel_errors(fp64_stepper.observables(), obs)}
+    log(f"[bench] vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}): {conv}")
     return {"value": v, "unit": "cell-steps/s", "ms_per_step": el / steps * 1e3,
             "speedup_vs_fp64": v / v64, "precond": "chebyshev (automatic degree)",
-            "rel_err_vs_fp64": err, "roofline": roof, "kernel_classes": pc,
+            "rel_err_vs_fp64": err, "rel_err_vs_fp64_converged": conv,
+            "roofline": roof, "kernel_classes": pc,
             "tgcr_iters_per_step": per_step(s, "tgcr_iters_total", steps),
             "stencil_sweeps_per_step": per_step(s, "jvp_evals", steps),
             "newton_iters_per_step": per_step(s, "newton_iters", steps),
@@ -326,21 +345,37 @@ def section_batched(attempts, warmup, fp64_steps):
     el64 = time.perf_counter() - c0
     v64 = float(ns.sum()) * fp64_steps / el64
     log(f"[bench] FP64 loop: {v64:.4g} cell-steps/s ({el64:.2f} s); batched/loop = {v / v64:.2f}")
-    # error of the batched mixed states against the FP64 loop (films at the same time)
+    # error of the batched mixed states against the FP64 loop (films at the same
+    # time), and against FP64 runs with Newton converged (untimed)
     worst = {"psi2": 0.0, "bz": 0.0, "js": 0.0}
+    keys = ("mixed", "fp64_same_tolerances")
+    conv = {k: dict(worst) for k in keys}
+    compared = 0
     for f, s in enumerate(steppers):
         if abs(t[f] - s.t) > 1e-9:
             continue
+        compared += 1
         eng = runner.engines[f]
         n = f_n = films[f].device.params.n_interior
         b, j = eng.ctx.observables(runner.batch.view(X, f))
         psi = eng.ctx.to_reference(runner.batch.view(X, f)).cpu().numpy()[:f_n]
-        e = rel_errors((np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()), s.observables())
+        mine = (np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy())
+        e = rel_errors(mine, s.observables())
         worst = {k: max(worst[k], e[k]) for k in worst}
-    log(f"[bench] batched vs FP64 loop, worst over films: {worst}")
+        ref = Stepper(films[f].device, films[f].x0, tol_gcr=tol, newton_tol_dx=CONVERGED_NEWTON_DX)
+        ref.accepted_steps(warmup + fp64_steps)
+        if abs(ref.t - s.t) > 1e-9:
+            continue
+        obs = ref.observables()
+        for k, e in zip(keys, (rel_errors(mine, obs), rel_errors(s.observables(), obs))):
+            conv[k] = {q: max(conv[k][q], e[q]) for q in e}
+    log(f"[bench] batched vs FP64 loop, worst over {compared} films: {worst}")
+    log(f"[bench] batched vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}), "
+        f"worst over films: {conv}")
     out.update({"fp64_loop_cell_steps_s": v64, "speedup_vs_fp64_loop": v / v64,
                 "speedup_unpreconditioned_vs_fp64_loop": vu / v64,
-                "rel_err_vs_fp64_worst_film": worst})
+                "films_compared": compared, "rel_err_vs_fp64_worst_film": worst,
+                "rel_err_vs_fp64_converged_worst_film": conv})
     return out
 
 

@@ -167,3 +167,28 @@ def test_mixed_trajectory_error_vs_fp64(cname):
     print("newton f64/mixed:", sum(s["newton_iters"] for s in a[3]),
           sum(s["newton_iters"] for s in b[3]),
           "tgcr:", sum(s["tgcr_iters_total"] for s in a[3]), sum(s["tgcr_iters_total"] for s in b[3]))
+
+
+def test_mixed_vs_converged_fp64():
+    """At the reference's Newton tolerances (newton.py:84: 1e-3 on ||g||_inf and
+    ||dx||_inf) the FP64 path's last iterate is a finite distance from the
+    solution of the step equations; the mixed path, with its exact Jacobian, ends
+    nearer to it.  Against an FP64 run whose Newton iteration is converged
+    (||dx||_inf <= 1e-5), the mixed trajectory is within 1e-6 relative in |psi|^2,
+    B_z and J_s (the headline film's settings on a 64x64x8 film with a 16x16 hole)."""
+    case = dict(p=dict(Nx=64, Ny=64, Nz=8, hx=0.5, hy=0.5, hz=0.5, kappa=4.0),
+                holes=[([(12.0, 12.0), (20.0, 12.0), (20.0, 20.0), (12.0, 20.0)], None)],
+                field=dict(Bz=0.4), seed=1, noise=1e-3, dt=0.2, tol_gcr=1e-8)
+    dev = device(case)
+    n = dev.params.n_interior
+    rng = np.random.default_rng(case["seed"])
+    X0 = np.concatenate([1.0 + case["noise"] * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                         np.zeros(3 * n, np.complex128)])
+    steps, tol = 12, case["tol_gcr"]
+    conv = _run(dev, X0, case["field"], "f64", steps, case["dt"], tol_gcr=tol, newton_tol_dx=1e-5)
+    loose = _run(dev, X0, case["field"], "f64", steps, case["dt"], tol_gcr=tol)
+    mixed = _run(dev, X0, case["field"], "mixed", steps, case["dt"], tol_gcr=tol)
+    for k, name in enumerate(("|psi|^2", "B_z", "J_s")):
+        em, ef = rel_err(mixed[k], conv[k]), rel_err(loose[k], conv[k])
+        print(f"{name}: vs converged FP64, mixed {em:.2e}, FP64 at the same tolerances {ef:.2e}")
+        assert em < 1e-6, name
