@@ -43,6 +43,8 @@ __device__ __forceinline__ void fast_sincos(float a, float& s, float& c) {
   c = __cosf(a);
 }
 constexpr int BS = 256;  // every sweep tile shape holds 256 nodes per plane
+// s_waitcnt vmcnt(0) alone (expcnt and lgkmcnt at their no-wait maxima)
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -74,11 +76,36 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   constexpr int TX = sweep_tx(TS), TY = sweep_ty(TS);
   static_assert(TX * TY == BS, "a sweep tile holds one node per thread and plane");
   constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
+  // Shared-memory rings of staged planes.  Plane k + 1 is committed at the top
+  // of step k, over the slot of plane k + 1 - RING.  Arrays read on planes
+  // k - 1..k + 1 (RING = 4) and arrays read on plane k only (NS = 3) then
+  // overwrite a plane last read two steps back, behind the one barrier per
+  // step; with RING = 3 (the 64x4 tile: four slots would leave room for two
+  // blocks per CU instead of three) a second barrier ends every step.
   constexpr int NS = IS3D ? 3 : 1;
-  constexpr int NZ = IS3D ? 3 : 1;
+  constexpr int kLds4 = (8 * 4 + 2 * 3) * CELLS * 4 + 3 * CELLS;
+  constexpr int RING = IS3D ? ((3 * kLds4 <= 160 * 1024) ? 4 : 3) : 1;
   constexpr int CZ = IS3D ? CELLS : 1;
-  __shared__ float X_r[NS][CELLS], X_i[NS][CELLS], X_x[NS][CELLS], X_y[NS][CELLS], X_z[NZ][CZ];
-  __shared__ float P_r[NS][CELLS], P_i[NS][CELLS], P_x[NS][CELLS], P_y[NS][CELLS], P_z[NZ][CZ];
+  // one object, so that a cell's address in every array is one register plus
+  // a constant offset
+  struct Rings {
+    float xr[RING][CELLS], xi[RING][CELLS], xz[RING][CZ];
+    float pr[RING][CELLS], pi[RING][CELLS], px[RING][CELLS], py[RING][CELLS], pz[RING][CZ];
+    float xx[NS][CELLS], xy[NS][CELLS];
+    uint8_t h[NS][CELLS];  // the staged cells' flags (the own node reads its own here)
+  };
+  __shared__ Rings S;
+  float(&X_r)[RING][CELLS] = S.xr;
+  float(&X_i)[RING][CELLS] = S.xi;
+  float(&X_z)[RING][CZ] = S.xz;
+  float(&P_r)[RING][CELLS] = S.pr;
+  float(&P_i)[RING][CELLS] = S.pi;
+  float(&P_x)[RING][CELLS] = S.px;
+  float(&P_y)[RING][CELLS] = S.py;
+  float(&P_z)[RING][CZ] = S.pz;
+  float(&X_x)[NS][CELLS] = S.xx;
+  float(&X_y)[NS][CELLS] = S.xy;
+  uint8_t(&H)[NS][CELLS] = S.h;
 
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
   const int f = bk.item_film32[L];
@@ -111,7 +138,6 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int i = i0 + tx, j = j0 + ty;
   const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
-  const unsigned int m_ij = (unsigned int)i + (unsigned int)g.mj * (unsigned int)j;  // full-grid column
   int kb = 0, ke = 1;
   if (IS3D) {
     kb = 1 + zc * fd->kz;
@@ -123,36 +149,54 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   // the columns this thread stages on every plane, as mask plans (uniform
   // field); a general applied field takes the per-cell rules, the column's
   // booleans recomputed on every plane (held across the march, they would
-  // occupy scalar registers as lane masks)
+  // occupy scalar registers as lane masks).  A plane's loads are issued a
+  // plane ahead; the mask rules apply when the cells go to shared memory, so
+  // the loads' latency overlaps the plane of arithmetic in between.
   const Fld32 F = fld32(b, &fd->b);
   const Plan32 pl0 = plan32(g, F, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
   const Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
-  auto stage_into = [&](int kp, Cell<float>& x0, Cell<float>& p0, Cell<float>& x1,
-                        Cell<float>& p1) __attribute__((always_inline)) {
-    if (GEN && F.gen) {
+  const bool hf = mt.flags != nullptr;
+  auto load_into = [&](int kp, Raw32& a0, Raw32& a1) __attribute__((always_inline)) {
+    if (GEN && F.gen) {  // general field: the rules at once, the cells kept in the load slots
+      auto pack = [](Raw32& r, const Cell<float>& v, const Cell<float>& w) {
+        r.xa = make_float2(v.pr, v.pi);
+        r.xx = v.x;
+        r.xy = v.y;
+        r.xz = v.z;
+        r.pa = make_float2(w.pr, w.pi);
+        r.px = w.x;
+        r.py = w.y;
+        r.pz = w.z;
+      };
       unsigned int h0 = hc0, h1 = hc1;
       opaque(h0);
       opaque(h1);
-      stage32<IS3D>(g, F, mt.flags, X, P, col32(g, i0 - 1 + h0 % CX, j0 - 1 + h0 / CX), kp, x0, p0);
-      if (has1) stage32<IS3D>(g, F, mt.flags, X, P, col32(g, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX), kp, x1, p1);
+      Cell<float> v, w;
+      const Col32 c0 = col32(g, i0 - 1 + h0 % CX, j0 - 1 + h0 / CX);
+      const unsigned int moff = IS3D ? (unsigned int)g.mk * (unsigned int)kp : 0u;
+      stage32<IS3D>(g, F, mt.flags, X, P, c0, kp, v, w);
+      pack(a0, v, w);
+      a0.h = hf ? (unsigned int)ldu(mt.flags, c0.mij + moff) : 0u;
+      const Col32 c1 = col32(g, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX);
+      stage32<IS3D>(g, F, mt.flags, X, P, c1, kp, v, w);
+      pack(a1, v, w);
+      a1.h = hf ? (unsigned int)ldu(mt.flags, c1.mij + moff) : 0u;
       return;
     }
+    // every lane issues both cells' loads: a column past the tile's cells
+    // (!has1) has a plan outside the grid, whose source node and flags index
+    // are valid, and its values are not written.  Loads on no branch let the
+    // own node's waits below count them exactly.
     const Pl32 pp = plane32<IS3D>(g, F, kp);
-    if (IS3D && pp.fk) {
-      stage32u<IS3D, true>(mt.flags, X, P, pl0, pp, x0, p0);
-      if (has1) stage32u<IS3D, true>(mt.flags, X, P, pl1, pp, x1, p1);
-    } else {
-      stage32u<IS3D, false>(mt.flags, X, P, pl0, pp, x0, p0);
-      if (has1) stage32u<IS3D, false>(mt.flags, X, P, pl1, pp, x1, p1);
-    }
+    load32u<IS3D>(mt.flags, X, P, pl0, pp, a0);
+    load32u<IS3D>(mt.flags, X, P, pl1, pp, a1);
   };
-  Cell<float> vx0, vp0, vx1, vp1;
-  auto stage = [&](int kp) __attribute__((always_inline)) { stage_into(kp, vx0, vp0, vx1, vp1); };
-  auto put = [&](int slot, int c, const Cell<float>& v, const Cell<float>& w) {
+  auto put = [&](int slot, int t, int c, const Cell<float>& v, const Cell<float>& w, unsigned int h) {
+    H[t][c] = (uint8_t)h;
     X_r[slot][c] = v.pr;
     X_i[slot][c] = v.pi;
-    X_x[slot][c] = v.x;
-    X_y[slot][c] = v.y;
+    X_x[t][c] = v.x;
+    X_y[t][c] = v.y;
     P_r[slot][c] = w.pr;
     P_i[slot][c] = w.pi;
     P_x[slot][c] = w.x;
@@ -162,76 +206,95 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
       P_z[slot][c] = w.z;
     }
   };
-  auto commit = [&](int slot) {
-    put(slot, hc0, vx0, vp0);
-    if (has1) put(slot, hc1, vx1, vp1);
+  // plane kp's cells, loaded by load_into, to their shared-memory slots
+  auto commit_from = [&](int kp, const Raw32& a0, const Raw32& a1) __attribute__((always_inline)) {
+    const int slot = kp % RING, t = kp % NS;
+    // every load of the plane has landed here (all are used below); waiting
+    // for all at once tells the compiler's wait tracking that no load is
+    // left in flight, so no register reuse later in the plane waits on one
+    vm_drain();
+    Cell<float> v0c, w0c, v1c, w1c;
+    if (GEN && F.gen) {
+      auto unpack = [](const Raw32& r, Cell<float>& v, Cell<float>& w) {
+        v = Cell<float>{r.xa.x, r.xa.y, r.xx, r.xy, r.xz};
+        w = Cell<float>{r.pa.x, r.pa.y, r.px, r.py, r.pz};
+      };
+      unpack(a0, v0c, w0c);
+      unpack(a1, v1c, w1c);
+    } else {
+      const Pl32 pp = plane32<IS3D>(g, F, kp);
+      if (IS3D && pp.fk) {
+        finish32u<IS3D, true>(a0, hf, pl0, pp, v0c, w0c);
+        finish32u<IS3D, true>(a1, hf, pl1, pp, v1c, w1c);
+      } else {
+        finish32u<IS3D, false>(a0, hf, pl0, pp, v0c, w0c);
+        finish32u<IS3D, false>(a1, hf, pl1, pp, v1c, w1c);
+      }
+    }
+    put(slot, t, hc0, v0c, w0c, a0.h);
+    if (has1) put(slot, t, hc1, v1c, w1c, a1.h);
   };
 
+  Raw32 r0, r1;
   if (IS3D) {
     // the first two planes' loads go out together (one memory round trip)
-    Cell<float> ax0, ap0, ax1, ap1;
-    stage_into(kb - 1, ax0, ap0, ax1, ap1);
-    stage(kb);
-    put((kb - 1) % 3, hc0, ax0, ap0);
-    if (has1) put((kb - 1) % 3, hc1, ax1, ap1);
-    commit(kb % 3);
-    stage(kb + 1);
+    Raw32 a0, a1;
+    load_into(kb - 1, a0, a1);
+    load_into(kb, r0, r1);
+    commit_from(kb - 1, a0, a1);
+    commit_from(kb, r0, r1);
+    load_into(kb + 1, r0, r1);
   } else {
-    stage(0);
+    load_into(0, r0, r1);
   }
 
   const float ihx2 = (float)mt.ihx2, ihy2 = (float)mt.ihy2, ihz2 = (float)mt.ihz2;
   const float dgpsi = (float)mt.dgpsi, inv_tau = (float)mt.inv_tau;
 
   for (int k = kb; k < ke; ++k) {
-    commit(IS3D ? (k + 1) % 3 : 0);
+    commit_from(IS3D ? k + 1 : 0, r0, r1);
     __syncthreads();
-    // own-node inputs, then the halo two planes ahead
-    unsigned int q = 0;
+    // own-node inputs (a lane past the grid's edge reads the nearest node and
+    // stores nothing), then the halo two planes ahead
+    const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
     uint8_t fl = 0;
     float scv = 1.f, k2v = 0.f;
-    float2 p0 = make_float2(0.f, 0.f);
-    float praw_x = 0.f, praw_y = 0.f, praw_z = 0.f;
+    if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
+    if (mt.k2) k2v = (float)ldu(mt.k2, q);
+    const float2 p0 = P.psi(q);
+    const float praw_x = P.ph(2, q), praw_y = P.ph(3, q), praw_z = IS3D ? P.ph(4, q) : 0.f;
     float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
     float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
-    if (act) {
-      q = qix32(g, i, j, k, IS3D);
-      if (mt.flags) fl = ldu(mt.flags, m_ij + (IS3D ? (unsigned int)g.mk * (unsigned int)k : 0u));
-      if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
-      if (mt.k2) k2v = (float)ldu(mt.k2, q);
-      p0 = P.psi(q);
-      praw_x = P.ph(2, q);
-      praw_y = P.ph(3, q);
-      if (IS3D) praw_z = P.ph(4, q);
-      if (has_v) {
-        v0 = CV.psi(q);
-        v_x = CV.ph(2, q);
-        v_y = CV.ph(3, q);
-        if (IS3D) v_z = CV.ph(4, q);
-      }
-      if (has_p) {
-        w0 = CP.psi(q);
-        w_x = CP.ph(2, q);
-        w_y = CP.ph(3, q);
-        if (IS3D) w_z = CP.ph(4, q);
-      }
+    if (has_v) {
+      v0 = CV.psi(q);
+      v_x = CV.ph(2, q);
+      v_y = CV.ph(3, q);
+      if (IS3D) v_z = CV.ph(4, q);
     }
-    if (IS3D && k + 2 <= ke) stage(k + 2);
+    if (has_p) {
+      w0 = CP.psi(q);
+      w_x = CP.ph(2, q);
+      w_y = CP.ph(3, q);
+      if (IS3D) w_z = CP.ph(4, q);
+    }
+    // the last plane loads plane ke again (in range, not written)
+    if (IS3D) load_into(min(k + 2, ke), r0, r1);
 
     if (act) {
-      const int s0 = IS3D ? k % 3 : 0, sm = IS3D ? (k + 2) % 3 : 0, sp1 = IS3D ? (k + 1) % 3 : 0;
+      const int s0 = k % RING, sm = (k + RING - 1) % RING, sp1 = (k + 1) % RING, t0 = k % NS;
       const int c0 = (ty + 1) * CX + (tx + 1);
       const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
+      fl = H[t0][c0];
       const float pr = X_r[s0][c0], pi = X_i[s0][c0];
       const float dr = P_r[s0][c0], di = P_i[s0][c0];
-      const float ex = X_x[s0][c0], ey = X_y[s0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
+      const float ex = X_x[t0][c0], ey = X_y[t0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
       const float dex = P_x[s0][c0], dey = P_y[s0][c0], dez = IS3D ? P_z[s0][c0] : 0.f;
 
       float sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0.f, czp = 1.f, szm = 0.f, czm = 1.f;
       fast_sincos(ex, sxp, cxp_);
-      fast_sincos(X_x[s0][cxm], sxm, cxm_);
+      fast_sincos(X_x[t0][cxm], sxm, cxm_);
       fast_sincos(ey, syp, cyp_);
-      fast_sincos(X_y[s0][cym], sym, cym_);
+      fast_sincos(X_y[t0][cym], sym, cym_);
       if (IS3D) {
         fast_sincos(ez, szp, czp);
         fast_sincos(X_z[sm][c0], szm, czm);
@@ -332,7 +395,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
         if (IS3D) out.ph(4, q, praw_z - hdt * jz);
       }
     }
-    __syncthreads();
+    if (IS3D && RING == 3) __syncthreads();
   }
 }
 

@@ -315,13 +315,37 @@ __host__ __device__ __forceinline__ Pl32 plane32(const Geo& g, const Fld32& F, i
   return p;
 }
 
-// Stage the cell of column plan c on plane p (uniform field).  FK: p is a z
-// face (the caller branches on p.fk, which is uniform).
-template <bool IS3D, bool FK>
-__host__ __device__ __forceinline__ void stage32u(const uint8_t* flags, const V32& X, const V32& P,
-                                                  const Plan32& c, const Pl32& p, Cell<float>& cx,
-                                                  Cell<float>& cp) {
+// One staged cell's loads, before any rule: the state and the perturbation at
+// the source node and the cell's hole bits (0 without flags).  A sweep issues
+// them a plane ahead and applies the rules (finish32u) when it writes the cell
+// to shared memory, so the loads' latency overlaps a plane of arithmetic.
+struct Raw32 {
+  float2 xa, pa;
+  float xx, xy, xz, px, py, pz;
+  unsigned int h;
+};
+
+template <bool IS3D>
+__host__ __device__ __forceinline__ void load32u(const uint8_t* flags, const V32& X, const V32& P,
+                                                 const Plan32& c, const Pl32& p, Raw32& r) {
   const unsigned int q = c.qij + p.qoff;
+  r.xa = X.psi(q);
+  r.pa = P.psi(q);
+  r.xx = X.ph(2, q);
+  r.xy = X.ph(3, q);
+  r.xz = IS3D ? X.ph(4, q) : 0.f;
+  r.px = P.ph(2, q);
+  r.py = P.ph(3, q);
+  r.pz = IS3D ? P.ph(4, q) : 0.f;
+  r.h = flags ? (unsigned int)ldu(flags, c.mij + p.moff) : 0u;
+}
+
+// The rules of column plan c on plane p applied to a cell's loads (uniform
+// field).  FK: p is a z face (the caller branches on p.fk, which is uniform).
+// HF: the film has hole flags (r.h was loaded).
+template <bool IS3D, bool FK>
+__host__ __device__ __forceinline__ void finish32u(const Raw32& r, bool hf, const Plan32& c,
+                                                   const Pl32& p, Cell<float>& cx, Cell<float>& cp) {
   unsigned int kx, ky, kz, ok;
   float ax, ay, az;
   if (FK) {  // only cells on no x / y face stay: their x / y rule keeps every component
@@ -341,25 +365,37 @@ __host__ __device__ __forceinline__ void stage32u(const uint8_t* flags, const V3
     ay = c.ay;
     az = c.az;
   }
-  const float2 xa = X.psi(q), pa = P.psi(q);
-  const float xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.f;
-  const float px = P.ph(2, q), py = P.ph(3, q), pz = IS3D ? P.ph(4, q) : 0.f;
-  if (flags) {
-    const uint8_t h = ldu(flags, c.mij + p.moff);
-    kx &= keep(!(h & kHoleX));
-    ky &= keep(!(h & kHoleY));
-    if (IS3D) kz &= keep(!(h & kHoleZ));
+  if (hf) {  // keep(!(h & bit)) as (bit of h) - 1: all ones when clear, zero when set
+    unsigned int hx = (r.h & 1u) - 1u, hy = ((r.h >> 1) & 1u) - 1u, hz = ((r.h >> 2) & 1u) - 1u;
+    static_assert(kHoleX == 1 && kHoleY == 2 && kHoleZ == 4, "hole bits");
+    opaque(hx);
+    opaque(hy);
+    opaque(hz);
+    kx &= hx;
+    ky &= hy;
+    if (IS3D) kz &= hz;
   }
-  cx.pr = fmask(xa.x, ok);
-  cx.pi = fmask(xa.y, ok);
-  cx.x = fmask(fmask(xx, c.sx) + ax, kx);
-  cx.y = fmask(fmask(xy, c.sy) + ay, ky);
-  cx.z = IS3D ? fmask(fmask(xz, p.sz) + az, kz) : 0.f;
-  cp.pr = fmask(pa.x, ok);
-  cp.pi = fmask(pa.y, ok);
-  cp.x = fmask(px, c.sx & kx);
-  cp.y = fmask(py, c.sy & ky);
-  cp.z = IS3D ? fmask(pz, p.sz & kz) : 0.f;
+  cx.pr = fmask(r.xa.x, ok);
+  cx.pi = fmask(r.xa.y, ok);
+  cx.x = fmask(fmask(r.xx, c.sx) + ax, kx);
+  cx.y = fmask(fmask(r.xy, c.sy) + ay, ky);
+  cx.z = IS3D ? fmask(fmask(r.xz, p.sz) + az, kz) : 0.f;
+  cp.pr = fmask(r.pa.x, ok);
+  cp.pi = fmask(r.pa.y, ok);
+  cp.x = fmask(r.px, c.sx & kx);
+  cp.y = fmask(r.py, c.sy & ky);
+  cp.z = IS3D ? fmask(r.pz, p.sz & kz) : 0.f;
+}
+
+// Stage the cell of column plan c on plane p (uniform field): loads and rules
+// together.  FK: p is a z face.
+template <bool IS3D, bool FK>
+__host__ __device__ __forceinline__ void stage32u(const uint8_t* flags, const V32& X, const V32& P,
+                                                  const Plan32& c, const Pl32& p, Cell<float>& cx,
+                                                  Cell<float>& cp) {
+  Raw32 r;
+  load32u<IS3D>(flags, X, P, c, p, r);
+  finish32u<IS3D, FK>(r, flags != nullptr, c, p, cx, cp);
 }
 
 // ---------------------------------------------------------------------------
