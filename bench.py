@@ -70,7 +70,7 @@ class Stepper:
     """The adaptive trapezoidal loop of the integrator, without history."""
 
     def __init__(self, dev, x0, dt=0.2, t_stop=200.0, tol_gcr=1e-8, precision="f64",
-                 precond="chebyshev", **newton):
+                 precond="chebyshev", predictor="euler", **newton):
         from nanowire_tdgl_amd import _native
         from nanowire_tdgl_amd.engine import engine_for
         self.dev = dev
@@ -80,6 +80,8 @@ class Stepper:
         self.dt, self.cur, self.t, self.t_stop = dt, dt, 0.0, t_stop
         self.opts = _native.solver_opts(tol_gcr=tol_gcr, precision=precision, precond=precond,
                                         **newton)
+        self.extrapolate = predictor == "extrapolate"
+        self.x_prev, self.dt_prev = None, 0.0
         self.log = []
 
     def accepted_steps(self, k: int) -> None:
@@ -87,23 +89,28 @@ class Stepper:
         while done < k:
             dta = min(self.cur, self.t_stop - self.t)
             self.eng.set_field(*self.dev.applied_field.evaluate(self.t, self.t_stop))
-            st = self.ctx.trap_step(self.X, dta, self.opts)
+            x_in = self.X.clone() if self.extrapolate else None
+            st = self.ctx.trap_step(self.X, dta, self.opts, x_prev=self.x_prev, dt_prev=self.dt_prev)
             self.log.append(st.as_dict())
             if st.converged:
                 self.t += dta
                 self.cur = self.dt
                 done += 1
+                if x_in is not None:
+                    self.x_prev, self.dt_prev = x_in, dta
             else:
                 self.cur /= 10.0
                 if self.cur < 1e-8:
                     raise RuntimeError(f"dt reduced below dt_min at t={self.t:.6e}")
 
     def snapshot(self):
-        return (self.X.clone(), self.t, self.cur, len(self.log))
+        xp = None if self.x_prev is None else self.x_prev.clone()
+        return (self.X.clone(), self.t, self.cur, len(self.log), xp, self.dt_prev)
 
     def restore(self, snap) -> None:
-        X, self.t, self.cur, nlog = snap
+        X, self.t, self.cur, nlog, xp, self.dt_prev = snap
         self.X.copy_(X)
+        self.x_prev = None if xp is None else xp.clone()
         del self.log[nlog:]
 
     def observables(self):
@@ -168,11 +175,12 @@ def roofline_of(per_class: dict) -> dict:
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": bpl}
 
 
-def timed_run(dev, x0, precision, steps, warmup, tol, profile=None, precond="chebyshev"):
+def timed_run(dev, x0, precision, steps, warmup, tol, profile=None, precond="chebyshev",
+              predictor="euler"):
     """Warm up, time `steps` accepted steps, then (optionally) re-run them
     instrumented from the same state.  Returns (Stepper, seconds, per_class, roofline)."""
     import torch
-    s = Stepper(dev, x0, tol_gcr=tol, precision=precision, precond=precond)
+    s = Stepper(dev, x0, tol_gcr=tol, precision=precision, precond=precond, predictor=predictor)
     s.accepted_steps(warmup)
     torch.cuda.synchronize()
     snap = s.snapshot()
@@ -220,6 +228,23 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
             "mixed_unpreconditioned": rel_errors(u.observables(), obs),
             "fp64_same_tolerances": rel_errors(fp64_stepper.observables(), obs)}
     log(f"[bench] vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}): {conv}")
+    # Newton started from the extrapolated last step instead of X + dt f(X),
+    # for the mixed method and, as its like-for-like baseline, for FP64
+    ex = {}
+    for prec in ("mixed", "f64"):
+        e, ele, _, _ = timed_run(dev, x0, prec, steps, warmup, tol, profile=False, predictor="extrapolate")
+        assert abs(e.t - s.t) < 1e-9
+        ex[prec] = {"value": n * steps / ele, "newton_iters_per_step": per_step(e, "newton_iters", steps),
+                    "tgcr_iters_per_step": per_step(e, "tgcr_iters_total", steps),
+                    "rel_err_vs_fp64_converged": rel_errors(e.observables(), obs)}
+    ex["mixed"]["speedup_vs_fp64_plain"] = ex["mixed"]["value"] / v64
+    ex["mixed"]["speedup_vs_fp64_extrapolating"] = ex["mixed"]["value"] / ex["f64"]["value"]
+    log(f"[bench] extrapolating predictor: mixed {ex['mixed']['value']:.4g} cell-steps/s "
+        f"({ex['mixed']['speedup_vs_fp64_plain']:.2f}x FP64 plain, "
+        f"{ex['mixed']['speedup_vs_fp64_extrapolating']:.2f}x FP64 extrapolating at {ex['f64']['value']:.4g}); "
+        f"Newton/step {ex['mixed']['newton_iters_per_step']:.2f} (reference predictor "
+        f"{per_step(s, 'newton_iters', steps):.2f}); err vs converged mixed {ex['mixed']['rel_err_vs_fp64_converged']}, "
+        f"FP64 {ex['f64']['rel_err_vs_fp64_converged']}")
     return {"value": v, "unit": "cell-steps/s", "ms_per_step": el / steps * 1e3,
             "speedup_vs_fp64": v / v64, "precond": "chebyshev (automatic degree)",
             "rel_err_vs_fp64": err, "rel_err_vs_fp64_converged": conv,
@@ -227,6 +252,7 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
             "tgcr_iters_per_step": per_step(s, "tgcr_iters_total", steps),
             "stencil_sweeps_per_step": per_step(s, "jvp_evals", steps),
             "newton_iters_per_step": per_step(s, "newton_iters", steps),
+            "extrapolating_predictor": ex,
             "unpreconditioned": {"value": vu, "speedup_vs_fp64": vu / v64, "rel_err_vs_fp64": erru,
                                  "tgcr_iters_per_step": per_step(u, "tgcr_iters_total", steps)}}
 
@@ -321,12 +347,46 @@ def section_batched(attempts, warmup, fp64_steps):
     torch.cuda.synchronize()
     vu = accu / (time.perf_counter() - c0)
     log(f"[bench] batched mixed, no preconditioner: {vu:.4g} cell-steps/s")
+    # the same attempts with Newton started from each film's extrapolated last
+    # step; one untimed attempt first gives every film a last step
+    X.copy_(X0)
+    t[:] = t0v
+    hist = {"x": None, "dt": np.zeros(len(films))}
+
+    def attempt_ex():
+        runner.set_fields(t, 300 * dt)
+        x_in = X.clone()
+        st = runner.batch.trap_step(X, dt, opts, x_prev=hist["x"], dt_prev=hist["dt"])
+        ok = np.array([s.converged for s in st], dtype=bool)
+        t[ok] += dt
+        if hist["x"] is None or ok.all():
+            hist["x"] = x_in
+        else:
+            for f in np.flatnonzero(ok):
+                runner.batch.view(hist["x"], f).copy_(runner.batch.view(x_in, f))
+        hist["dt"][ok] = dt
+        return st, ok
+
+    attempt_ex()
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    acce, newton_ex = 0.0, 0
+    for _ in range(attempts):
+        st, ok = attempt_ex()
+        acce += float(ns[ok].sum())
+        newton_ex += sum(s.newton_iters for s in st)
+    torch.cuda.synchronize()
+    ve = acce / (time.perf_counter() - c0)
+    log(f"[bench] batched mixed, extrapolating predictor: {ve:.4g} cell-steps/s "
+        f"({newton_ex / (attempts * len(films)):.2f} Newton iterations per film-step)")
     X.copy_(Xp)
     t[:] = tp
     out = {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
                        "Bz~U[0.05,0.8], holes on even films; dt=0.2, TGCR tol 1e-4",
            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
            "mixed_batched_cell_steps_s": v, "mixed_batched_unpreconditioned_cell_steps_s": vu,
+           "mixed_batched_extrapolating_cell_steps_s": ve,
+           "newton_iters_per_film_step_extrapolating": newton_ex / (attempts * len(films)),
            "roofline": roof, "kernel_classes": pc,
            "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16"}[tune["shape"]],
                                    "kz": tune["kz"]},
@@ -373,6 +433,7 @@ def section_batched(attempts, warmup, fp64_steps):
     log(f"[bench] batched vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}), "
         f"worst over films: {conv}")
     out.update({"fp64_loop_cell_steps_s": v64, "speedup_vs_fp64_loop": v / v64,
+                "speedup_extrapolating_vs_fp64_loop": ve / v64,
                 "speedup_unpreconditioned_vs_fp64_loop": vu / v64,
                 "films_compared": compared, "rel_err_vs_fp64_worst_film": worst,
                 "rel_err_vs_fp64_converged_worst_film": conv})

@@ -87,6 +87,18 @@ typedef struct tdgl_solver_opts {
    * is unchanged.  Off when dt/2 >= 0.9 (the interval would reach 0). */
   int32_t mixed_precond;
   int32_t mixed_reserved;
+  /* Predictor (no counterpart in the reference, which starts Newton from
+   * X + dt f(X), integrators.py:156-158): with x_prev non-NULL, a film whose
+   * dt_prev entry is > 0 starts from the linear extrapolation
+   * X + (dt / dt_prev) (X - x_prev) of its last accepted step instead.
+   * x_prev: the state before that step (device, internal layout, the same
+   * length and offsets as X_dev); dt_prev: host array of the step's sizes, one
+   * entry per film (one for tdgl_trap_step); an entry <= 0 keeps the
+   * reference predictor for that film.  gamma = X + dt/2 f(X), the equations
+   * and the convergence test are unchanged: only Newton's starting point moves.
+   * Both NULL (the default) reproduce the reference. */
+  const double* x_prev;
+  const double* dt_prev;
 } tdgl_solver_opts;
 
 /* Per-step counters (the reference keeps none; they make iteration-count

@@ -54,7 +54,24 @@ class SolverOpts(C.Structure):
                 ("newton_max_iter", C.c_int32), ("precision", C.c_int32),
                 ("tol_gcr", C.c_double), ("eps_mf", C.c_double),
                 ("tgcr_max_iter", C.c_int64), ("mixed_tol_floor", C.c_double),
-                ("mixed_precond", C.c_int32), ("mixed_reserved", C.c_int32)]
+                ("mixed_precond", C.c_int32), ("mixed_reserved", C.c_int32),
+                ("x_prev", C.c_void_p), ("dt_prev", C.c_void_p)]
+
+
+def _with_history(opts: "SolverOpts", x_prev, dt_prev, nfilm: int, total_len: int, torch):
+    """A copy of opts whose predictor extrapolates from the last accepted step
+    (x_prev: the state before it, dt_prev: its size per film); returns the copy
+    and the host array that must outlive the call."""
+    import numpy as np
+    o = SolverOpts.from_buffer_copy(opts)
+    if x_prev is None:
+        return o, None
+    if x_prev.dtype != torch.float64 or x_prev.numel() != total_len or not x_prev.is_contiguous():
+        raise ValueError("x_prev must be a contiguous float64 tensor shaped like the state")
+    d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt_prev, dtype=np.float64), (nfilm,)))
+    o.x_prev = x_prev.data_ptr()
+    o.dt_prev = d.ctypes.data
+    return o, d
 
 
 class StepStats(C.Structure):
@@ -328,10 +345,15 @@ class Context:
         self._call("tdgl_norm_inf", C.c_void_p(a.data_ptr()), C.byref(out))
         return out.value
 
-    def trap_step(self, X, dt: float, opts: SolverOpts) -> StepStats:
+    def trap_step(self, X, dt: float, opts: SolverOpts, x_prev=None, dt_prev: float = 0.0) -> StepStats:
+        """One step attempt; with x_prev (the state before the last accepted
+        step, of size dt_prev > 0) Newton starts from the extrapolation
+        X + dt/dt_prev (X - x_prev) instead of the reference's X + dt f(X)."""
         st = StepStats()
-        self._call("tdgl_trap_step", C.c_void_p(X.data_ptr()), C.c_double(dt), C.byref(opts),
+        o, keep = _with_history(opts, x_prev, dt_prev, 1, self.len, self.torch)
+        self._call("tdgl_trap_step", C.c_void_p(X.data_ptr()), C.c_double(dt), C.byref(o),
                    C.byref(st))
+        del keep
         return st
 
     def euler_step(self, X, dt: float) -> None:
@@ -450,17 +472,22 @@ class Batch:
         o = self.offset(f)
         return X[o:o + self.contexts[f].len]
 
-    def trap_step(self, X, dt, opts: SolverOpts, active=None) -> list:
+    def trap_step(self, X, dt, opts: SolverOpts, active=None, x_prev=None, dt_prev=None) -> list:
+        """One step attempt per active film; x_prev / dt_prev (per film, <= 0:
+        none) as Context.trap_step's, packed like X."""
         import numpy as np
         F = self.nfilm
         if X.dtype != self.torch.float64 or X.numel() != self.total_len:
             raise ValueError("X must be a float64 tensor of total_len elements")
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (F,)))
         a = None if active is None else np.ascontiguousarray(np.asarray(active, dtype=np.int32))
+        o, keep = _with_history(opts, x_prev, 0.0 if dt_prev is None else dt_prev, F, self.total_len,
+                                self.torch)
         stats = (StepStats * F)()
         self._call("tdgl_batch_trap_step", C.c_void_p(X.data_ptr()), C.c_void_p(d.ctypes.data),
-                   C.c_void_p(a.ctypes.data) if a is not None else None, C.byref(opts),
+                   C.c_void_p(a.ctypes.data) if a is not None else None, C.byref(o),
                    C.cast(stats, C.c_void_p))
+        del keep
         return list(stats)
 
     def profile_enable(self, on: bool = True) -> None:

@@ -800,6 +800,7 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   if (!ctx || !X || !o || !s) return TDGL_E_INVALID;
   if (o->precision != TDGL_PREC_F64 && o->precision != TDGL_PREC_MIXED)
     return fail(ctx, TDGL_E_INVALID, "precision must be TDGL_PREC_F64 or TDGL_PREC_MIXED");
+  if (o->x_prev && !o->dt_prev) return fail(ctx, TDGL_E_INVALID, "x_prev needs dt_prev");
   if (o->precision == TDGL_PREC_MIXED) return mixed_step_single(ctx, X, dt, o, s);
   memset(s, 0, sizeof(*s));
   int rc;
@@ -820,6 +821,13 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
   LAUNCHCHK();
   s->rhs_evals++;
   s->bytes_moved += 3.0 * S;
+  if (o->x_prev && o->dt_prev[0] > 0.0) {  // Newton starts from the extrapolated state
+    const int tk = prof_begin(ctx, TDGL_K_PRED, 3.0 * S);
+    launch_extrap64(X, o->x_prev, dt / o->dt_prev[0], ctx->w_x, ctx->len, ctx->st);
+    prof_end(ctx, tk);
+    LAUNCHCHK();
+    s->bytes_moved += 3.0 * S;
+  }
   if ((rc = newton_eval(ctx, hdt, s))) return rc;
   if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
   for (int it = 1; it <= o->newton_max_iter; ++it) {

@@ -52,6 +52,7 @@ struct tdgl_batch {
   int* d_act = nullptr;
   double* d_res = nullptr;
   double* d_dtv = nullptr;
+  double* d_ext = nullptr;  // per film: dt / dt_prev of the extrapolating predictor (0: off)
   int* d_fstat = nullptr;
   int* d_maxk = nullptr;
   int* d_iter = nullptr;
@@ -667,6 +668,23 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
         S[f].bytes_moved += 3.0 * s64(B, f);
       }
   }
+  if (o->x_prev) {  // Newton starts from the extrapolated state where a film has a last step
+    std::vector<double> ratio(F, 0.0);
+    double by = 0.0;
+    for (int f = 0; f < F; ++f)
+      if (act[f] && o->dt_prev[f] > 0.0) {
+        ratio[f] = dt[f] / o->dt_prev[f];
+        by += 3.0 * s64(B, f);
+        S[f].bytes_moved += 3.0 * s64(B, f);
+      }
+    if (by > 0.0) {
+      BCHK(hipMemcpyAsync(B->d_ext, ratio.data(), sizeof(double) * F, hipMemcpyHostToDevice, B->ps));
+      const int tok = B->prof->begin(B->ps, TDGL_K_PRED, by);
+      launch_bextrap(kview(B), X, o->x_prev, B->d_ext, B->w_x, B->ps);
+      B->prof->end(B->ps, tok);
+      BCHK(hipGetLastError());
+    }
+  }
   if ((rc = newton_eval(B, act, S))) return rc;
   if ((rc = report(B))) return rc;
   for (int f = 0; f < F; ++f)
@@ -776,6 +794,7 @@ int batch_init(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_act, sizeof(int) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_res, sizeof(double) * kResLen * (size_t)F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_dtv, sizeof(double) * 2 * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_ext, sizeof(double) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_fstat, sizeof(int) * 4 * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_maxk, sizeof(int) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_iter, sizeof(int)))) return rc;
@@ -930,7 +949,7 @@ int tdgl_batch_destroy(tdgl_batch* B) {
   if (B->ps) (void)hipStreamDestroy(B->ps);
   for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_item_film32, (void*)B->d_sblk_film,
                   (void*)B->d_act,
-                  (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_fstat, (void*)B->d_maxk,
+                  (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_ext, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
                   (void*)B->d_H, (void*)B->d_rep, (void*)B->w_x, (void*)B->w_gam, (void*)B->x32,
                   (void*)B->r32, (void*)B->r0_32, (void*)B->d_qtab, (void*)B->d_cheb,
@@ -1019,6 +1038,7 @@ int tdgl_batch_trap_step(tdgl_batch* B, double* X, const double* dt, const int* 
   if (!B || !X || !dt || !o || !stats) return TDGL_E_INVALID;
   if (o->precision != TDGL_PREC_MIXED)
     return bfail(B, TDGL_E_INVALID, "batches run the mixed-precision method (TDGL_PREC_MIXED)");
+  if (o->x_prev && !o->dt_prev) return bfail(B, TDGL_E_INVALID, "x_prev needs dt_prev");
   for (int f = 0; f < B->nfilm; ++f)
     if (!(dt[f] > 0.0) && (!active || active[f])) return bfail(B, TDGL_E_INVALID, "dt must be > 0");
   return batch_step(B, X, dt, active, o, stats);

@@ -798,9 +798,30 @@ __global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, con
   if (threadIdx.x == 0) pm[blockIdx.x] = v[0];
 }
 
+// out = x + r (x - xp) with r = ratio[f]: Newton's start extrapolated from the
+// film's last accepted step (r <= 0: out keeps the reference predictor).
+__global__ __launch_bounds__(kRedThreads) void k_bextrap(BatchK bk, const double* x, const double* xp,
+                                                         const double* ratio, double* out) {
+  FILM_STREAM(bk);
+  (void)n4;
+  const double r = ratio[f];
+  if (!(r > 0.0)) return;
+  // the film's own elements only: the caller's padding is never read
+  const long long len = fd->len;
+  const double* X = x + off;
+  const double* XP = xp + off;
+  double* O = out + off;
+  for (long long i = e0; i < len; i += estride) O[i] = X[i] + r * (X[i] - XP[i]);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
+void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,
+                    hipStream_t st) {
+  launch(k_bextrap, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, xp, ratio, out);
+}
 
 template <bool IS3D, bool CHEB, bool GEN>
 void launch_sweep_g(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,

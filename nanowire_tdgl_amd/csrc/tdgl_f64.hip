@@ -18,6 +18,8 @@
 //   k_obs      : B = curl(phi) and J_s (check-path observables).
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 
 #include "tdgl_device.hpp"
@@ -825,6 +827,19 @@ __global__ __launch_bounds__(kRedThreads) void k_accum(long long n, double* x,
     v[0] = fmax(v[0], fmax(hypot(d.a, d.b), fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)))));
   }
   publish<0, 1>(v, r.pout);
+}
+
+// out = x + r (x - xp) over a whole state vector (the extrapolating predictor)
+__global__ __launch_bounds__(kRedThreads) void k_extrap64(const double* x, const double* xp, double r,
+                                                          double* out, long long len) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += stride)
+    out[i] = x[i] + r * (x[i] - xp[i]);
+}
+void launch_extrap64(const double* x, const double* xp, double ratio, double* out, long long len,
+                     hipStream_t st) {
+  const long long blocks = std::min<long long>((len + kRedThreads - 1) / kRedThreads, 4096);
+  launch(k_extrap64, dim3((unsigned)blocks), dim3(kRedThreads), 0, st, x, xp, ratio, out, len);
 }
 
 template <bool IS3D>

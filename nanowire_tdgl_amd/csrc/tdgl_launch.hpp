@@ -204,6 +204,12 @@ struct ChebStep {
 void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                     const int* iter, const ChebStep& cs, hipStream_t st);
 // x (FP64) += s z with s = 1/theta of the film when scale_th, else 1; partials of ||s z||_inf.
+// out = x + ratio (x - xp): the extrapolating predictor (ratio[f] <= 0: film
+// f keeps out as it is)
+void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,
+                    hipStream_t st);
+void launch_extrap64(const double* x, const double* xp, double ratio, double* out, long long len,
+                     hipStream_t st);
 void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
                   hipStream_t st);
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,

@@ -92,14 +92,18 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
                       tgcr_max_iter: Optional[int] = None, mixed_tol_floor: float = 1e-6,
                       dt_min: float = 1e-8, max_attempts: Optional[int] = None,
                       tune: bool = False, runner: Optional[BatchRunner] = None,
-                      keep_log: bool = False, precond="chebyshev") -> BatchResult:
+                      keep_log: bool = False, precond="chebyshev",
+                      predictor: str = "euler") -> BatchResult:
     """Advance every film from t_start to t_stop (mixed precision, lockstep).
 
     The Newton/TGCR parameters mean what they mean in
     :func:`nanowire_tdgl_amd.solvers.integrators.trapezoidal`; ``tune`` runs
     the stencil autotuner first; ``max_attempts`` bounds the number of
     lockstep attempts (None: until every film reaches t_stop); ``precond`` as
-    in ``trapezoidal`` (the automatic degree is the largest any film asks for)."""
+    in ``trapezoidal`` (the automatic degree is the largest any film asks for);
+    ``predictor`` as in ``trapezoidal``, per film."""
+    if predictor not in ("euler", "extrapolate"):
+        raise ValueError("predictor must be 'euler' or 'extrapolate'")
     runner = runner or BatchRunner([f.device for f in films])
     F = len(films)
     X = runner.pack([f.x0 for f in films])
@@ -111,6 +115,7 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     t = np.full(F, float(t_start))
     cur = np.full(F, float(dt))
     steps = np.zeros(F, dtype=np.int64)
+    x_prev, dt_prev = None, np.zeros(F)
     attempts = 0
     log = []
     while True:
@@ -119,8 +124,17 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
             break
         dta = np.where(active, np.minimum(cur, t_stop - t), dt)
         runner.set_fields(t, t_stop)
-        st = runner.batch.trap_step(X, dta, opts, active.astype(np.int32))
+        x_in = X.clone() if predictor == "extrapolate" else None
+        st = runner.batch.trap_step(X, dta, opts, active.astype(np.int32), x_prev=x_prev,
+                                    dt_prev=dt_prev)
         attempts += 1
+        if x_in is not None:  # a converged film's history becomes the step it just took
+            for f in range(F):
+                if active[f] and st[f].converged:
+                    if x_prev is None:
+                        x_prev = x_in.clone()
+                    runner.batch.view(x_prev, f).copy_(runner.batch.view(x_in, f))
+                    dt_prev[f] = dta[f]
         if keep_log:
             log.append([s.as_dict() for s in st])
         for f in range(F):

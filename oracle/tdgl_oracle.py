@@ -788,16 +788,21 @@ def newton_trap(f: Callable, x0, gamma, dt, tol_f=1e-3, tol_dx=1e-3, max_iter=20
 def trapezoidal(x0, film: Film, field_at: Callable, t_start, t_stop, dt, *,
                 newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,
                 eps_mf=1e-4, save_every=1, adaptive=True, dt_min=1e-8, tgcr_max_iter=None,
-                max_accepted=None, log=None):
+                max_accepted=None, log=None, predictor="euler"):
     """Adaptive trapezoidal stepping (integrators.py:92-206).  ``field_at(t)``
     returns (Bx, By, Bz); the field is frozen at t_n for the implicit solve.
-    ``max_accepted`` stops after that many accepted steps (a bounded sample)."""
+    ``max_accepted`` stops after that many accepted steps (a bounded sample).
+    ``predictor="extrapolate"`` (an extension with no counterpart in the
+    reference): after the first accepted step Newton starts from
+    X + (dt / dt_prev) (X - X_prev), X_prev the state before the last accepted
+    step and dt_prev its size, instead of X + dt f(X) (integrators.py:156)."""
     g, idx, mat = film.g, film.idx, film.material
     X = np.array(x0, dtype=np.complex128)
     t = t_start
     cur = dt
     times, hist = [t], [X.copy()]
     step = 0
+    x_prev, dt_prev = None, 0.0
     while t < t_stop - 1e-14:
         dta = min(cur, t_stop - t)
         u = boundary_vectors(*field_at(t), g, idx)
@@ -807,6 +812,8 @@ def trapezoidal(x0, film: Film, field_at: Callable, t_start, t_stop, dt, *,
 
         fn = f(X)
         xp = X + dta * fn
+        if predictor == "extrapolate" and x_prev is not None:
+            xp = X + (dta / dt_prev) * (X - x_prev)
         gam = X + (dta / 2.0) * fn
         st = {}
         xn, conv, it = newton_trap(f, xp, gam, dta, newton_tol_f, newton_tol_dx,
@@ -814,6 +821,7 @@ def trapezoidal(x0, film: Film, field_at: Callable, t_start, t_stop, dt, *,
         if log is not None:
             log.append((bool(conv), int(it), float(dta), list(st.get("tgcr", []))))
         if conv:
+            x_prev, dt_prev = X, dta
             X = xn
             t += dta
             step += 1

@@ -49,7 +49,7 @@ def _films():
     return films
 
 
-def _single(film, steps, dt, precision, **kw):
+def _single(film, steps, dt, precision, extrapolate=False, **kw):
     from nanowire_tdgl_amd import _native
     from nanowire_tdgl_amd.engine import engine_for
     d = film.device
@@ -57,11 +57,14 @@ def _single(film, steps, dt, precision, **kw):
     X = eng.ctx.to_internal(film.x0)
     opts = _native.solver_opts(precision=precision, **kw)
     stats = []
+    x_prev = None
     for s in range(steps):
         eng.set_field(*d.applied_field.evaluate(s * dt, 100.0))
-        st = eng.ctx.trap_step(X, dt, opts)
+        x_in = X.clone() if extrapolate else None
+        st = eng.ctx.trap_step(X, dt, opts, x_prev=x_prev, dt_prev=dt)
         assert st.converged == 1
         stats.append(st.as_dict())
+        x_prev = x_in
     return eng, X, stats
 
 
@@ -93,6 +96,40 @@ def test_batch_reproduces_each_film_bitwise(precond):
             assert sf.tgcr_iters_total == ref["tgcr_iters_total"]
     for f in range(len(films)):
         assert runner.batch.view(X, f).equal(got[f]), f
+
+
+def test_batch_extrapolating_predictor_reproduces_each_film_bitwise():
+    """Newton started from each film's extrapolated last step: the batch still
+    reproduces every film's batch-of-one steps bit for bit (the first step has
+    no history and takes the reference predictor), with fewer Newton
+    iterations than the reference predictor."""
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    steps, dt = 4, 0.2
+    singles = [_single(f, steps, dt, "mixed", extrapolate=True, precond=6) for f in films]
+    plain = [_single(f, steps, dt, "mixed", precond=6) for f in films]
+    runner = BatchRunner([f.device for f in films])
+    X = runner.pack([f.x0 for f in films])
+    opts = _native.solver_opts(precision="mixed", precond=6)
+    x_prev = None
+    for s in range(steps):
+        runner.set_fields([s * dt] * len(films), 100.0)
+        x_in = X.clone()
+        st = runner.batch.trap_step(X, dt, opts, x_prev=x_prev, dt_prev=[dt] * len(films))
+        x_prev = x_in
+        for f, sf in enumerate(st):
+            assert sf.converged == 1
+            assert sf.newton_iters == singles[f][2][s]["newton_iters"]
+            assert sf.tgcr_iters_total == singles[f][2][s]["tgcr_iters_total"]
+    for f in range(len(films)):
+        assert runner.batch.view(X, f).equal(singles[f][1]), f
+    ne = sum(r["newton_iters"] for s in singles for r in s[2])
+    nr = sum(r["newton_iters"] for s in plain for r in s[2])
+    print(f"Newton iterations over the films: extrapolating {ne}, reference predictor {nr}")
+    assert ne < nr
+    with pytest.raises(ValueError):
+        runner.batch.trap_step(X, dt, opts, x_prev=X[:10], dt_prev=dt)
 
 
 def test_inactive_films_are_untouched():

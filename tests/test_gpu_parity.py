@@ -200,6 +200,34 @@ def test_trajectory_matches_reference(which):
     assert err < 1e-9
 
 
+def test_extrapolating_predictor_trajectory_matches_oracle():
+    """predictor="extrapolate" (no counterpart in the reference): the FP64 path
+    against the oracle's restatement of the same predictor, trap_hole case:
+    identical accept decisions and Newton iteration counts, states to 1e-9."""
+    T, engine_for, rhs, bbfv, integ = _lib()
+    from nanowire_tdgl_amd.solvers.runner import _make_eval_u
+    case = S.TRAP_HOLE
+    d = golden("trap_hole")
+    dev = device(case)
+    f = oracle_film(case)
+    t_stop = 2.0  # ten steps (the oracle's share of the test stays a few seconds)
+    eval_u = _make_eval_u(T.AppliedField(**case["field"]), dev.params, dev.idx, t_stop)
+    log = []
+    times, hist = integ.trapezoidal(d["x0"], dev.params, dev.idx, eval_u, 0.0, t_stop, case["dt"],
+                                    tol_gcr=case["tol_gcr"], progress=False, material=dev.material,
+                                    step_log=log, predictor="extrapolate")
+    olog = []
+    otimes, ohist = O.trapezoidal(d["x0"], f, lambda t: O.applied_field_at(t, t_stop, (0.0, 0.0, case["field"]["Bz"])),
+                                  0.0, t_stop, case["dt"], tol_gcr=case["tol_gcr"], log=olog,
+                                  predictor="extrapolate")
+    assert [(bool(r["converged"]), r["newton_iters"]) for r in log] == [(c, it) for c, it, _, _ in olog]
+    assert np.allclose(times, otimes, rtol=0, atol=1e-12)
+    err = rel_err(hist, ohist)
+    print(f"extrapolating predictor: {sum(r['newton_iters'] for r in log)} Newton iterations, "
+          f"state error vs oracle {err:.2e}")
+    assert err < 1e-9
+
+
 def _c1_run(t_stop):
     T, engine_for, rhs, bbfv, integ = _lib()
     from nanowire_tdgl_amd.solvers.runner import _make_eval_u

@@ -42,7 +42,7 @@ def test_loader_sets_signatures():
     assert ctypes.sizeof(_native.StepStats) == 8 * 4 + 5 * 8
     text = (ROOT / "include" / "tdgl_hip.h").read_text()
     assert len(_native.KERNEL_CLASSES) == int(re.search(r"#define TDGL_K_COUNT (\d+)", text).group(1))
-    assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8 + 4 + 4
+    assert ctypes.sizeof(_native.SolverOpts) == 8 + 8 + 4 + 4 + 8 + 8 + 8 + 8 + 4 + 4 + 8 + 8
 
 
 def test_precond_option_codes():

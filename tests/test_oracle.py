@@ -245,6 +245,37 @@ def test_trapezoidal_hole_trajectory_matches_reference():
     assert tg == [int(k) for k, _ in d["tgcr"]]
 
 
+def test_extrapolating_predictor_moves_only_newtons_start():
+    """predictor="extrapolate" (an extension with no counterpart in the
+    reference) starts Newton from X + dt/dt_prev (X - X_prev).  The step
+    equations and the convergence test are the reference's, so: the first step
+    (no history yet) takes the reference's Newton iterations, every step is
+    accepted as in the reference, the trajectory stays within 1e-7 of the
+    reference's states (the loose Newton tolerance's reach), and the
+    extrapolated start saves Newton iterations on this film."""
+    d = golden("trap_hole")
+    case = S.TRAP_HOLE
+    f = oracle_film(case)
+    af = case["field"]
+
+    def field_at(t):
+        return O.applied_field_at(t, float(d["t_stop"]), (0.0, 0.0, af["Bz"]))
+
+    log = []
+    times, hist = O.trapezoidal(d["x0"], f, field_at, 0.0, float(d["t_stop"]), case["dt"],
+                                tol_gcr=case["tol_gcr"], log=log, predictor="extrapolate")
+    ref = [(bool(c), int(it)) for c, it, _ in d["newton"]]
+    assert np.array_equal(times, d["times"])
+    assert [c for c, *_ in log] == [c for c, _ in ref]
+    assert log[0][1] == ref[0][1]
+    err = rel_err(hist[:, d["sel"]], d["states"])
+    assert 0 < err < 1e-7
+    assert sum(it for _, it, _, _ in log) < 0.8 * sum(it for _, it in ref)
+    with pytest.raises(ValueError):
+        from nanowire_tdgl_amd.solvers import integrators
+        integrators.trapezoidal(d["x0"], None, None, None, 0.0, 1.0, 0.1, predictor="midpoint")
+
+
 def test_euler_and_observables_match_reference():
     d = golden("euler_small")
     case = S.EULER_SMALL
