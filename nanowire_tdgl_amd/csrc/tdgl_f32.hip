@@ -257,12 +257,20 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     // own-node inputs (a lane past the grid's edge reads the nearest node and
     // stores nothing), then the halo two planes ahead
     const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
-    uint8_t fl = 0;
+    const int t0 = k % NS, c0 = (ty + 1) * CX + (tx + 1);
+    const uint8_t fl = H[t0][c0];
     float scv = 1.f, k2v = 0.f;
     if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
     if (mt.k2) k2v = (float)ldu(mt.k2, q);
-    const float2 p0 = P.psi(q);
-    const float praw_x = P.ph(2, q), praw_y = P.ph(3, q), praw_z = IS3D ? P.ph(4, q) : 0.f;
+    // p at the own node is its staged cell (read below): the rules keep psi and
+    // zero a link only on the last interior column / row / plane (the normal
+    // zeroing) or on a hole link; there the raw value is loaded
+    const bool rx = i == g.Nx - 1 || (fl & kHoleX), ry = j == g.Ny - 1 || (fl & kHoleY);
+    const bool rz = IS3D && (k == g.Nz - 1 || (fl & kHoleZ));
+    float lx = 0.f, ly = 0.f, lz = 0.f;
+    if (rx) lx = P.ph(2, q);
+    if (ry) ly = P.ph(3, q);
+    if (rz) lz = P.ph(4, q);
     float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
     float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
     if (has_v) {
@@ -281,14 +289,14 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     if (IS3D) load_into(min(k + 2, ke), r0, r1);
 
     if (act) {
-      const int s0 = k % RING, sm = (k + RING - 1) % RING, sp1 = (k + 1) % RING, t0 = k % NS;
-      const int c0 = (ty + 1) * CX + (tx + 1);
+      const int s0 = k % RING, sm = (k + RING - 1) % RING, sp1 = (k + 1) % RING;
       const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
-      fl = H[t0][c0];
       const float pr = X_r[s0][c0], pi = X_i[s0][c0];
       const float dr = P_r[s0][c0], di = P_i[s0][c0];
       const float ex = X_x[t0][c0], ey = X_y[t0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
       const float dex = P_x[s0][c0], dey = P_y[s0][c0], dez = IS3D ? P_z[s0][c0] : 0.f;
+      const float2 p0 = make_float2(dr, di);
+      const float praw_x = rx ? lx : dex, praw_y = ry ? ly : dey, praw_z = rz ? lz : dez;
 
       float sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0.f, czp = 1.f, szm = 0.f, czm = 1.f;
       fast_sincos(ex, sxp, cxp_);
