@@ -36,11 +36,13 @@ def solve(device: Device, t_start: float = 0.0, t_stop: float = 10.0, dt: float 
           tol_gcr: float = 1e-4, eps_mf: float = 1e-4, adaptive: bool = True,
           log_metadata: bool = True, log_dir: str | Path = "logs",
           tgcr_max_iter: Optional[int] = None, precision: str = "f64",
-          precond="chebyshev") -> Solution:
+          precond="chebyshev", predictor: str = "euler") -> Solution:
     """Run a TDGL simulation on the GPU (runner.py:40-163).  ``tgcr_max_iter``
     (None = the reference's n_state cap), ``precision`` ("f64", or "mixed":
-    FP32 Krylov solve inside FP64 Newton) and ``precond`` (the mixed solve's
-    Chebyshev preconditioner: "chebyshev", "none" or a degree) are extensions."""
+    FP32 Krylov solve inside FP64 Newton), ``precond`` (the mixed solve's
+    Chebyshev preconditioner: "chebyshev", "none" or a degree) and
+    ``predictor`` ("euler": the reference's Newton start; "extrapolate": the
+    extrapolated last step) are extensions."""
     params, idx = device.params, device.idx
     material = getattr(device, "material", None)
     if x0 is None:
@@ -68,7 +70,7 @@ def solve(device: Device, t_start: float = 0.0, t_stop: float = 10.0, dt: float 
                                         eps_mf=eps_mf, save_every=save_every, adaptive=adaptive,
                                         progress=progress, verbose=verbose, material=material,
                                         tgcr_max_iter=tgcr_max_iter, step_log=log,
-                                        precision=precision, precond=precond)
+                                        precision=precision, precond=precond, predictor=predictor)
         else:
             raise ValueError(f"Unknown method {method!r}. Use 'euler' or 'trapezoidal'.")
 
