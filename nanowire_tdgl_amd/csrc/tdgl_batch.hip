@@ -69,7 +69,8 @@ struct tdgl_batch {
   float* r0_32 = nullptr;
   float* t32[2] = {nullptr, nullptr};  // Chebyshev recurrence vectors
   double* d_cheb = nullptr;            // nfilm * kChebStride: 1/theta, omega_1 .. omega_d
-  int cheb_deg = 0;                    // this step's preconditioner degree (0: none)
+  int cheb_deg = 0;                    // this step's largest preconditioner degree (0: none)
+  std::vector<int> cheb_degf;          // per film: its own degree
   std::vector<float*> Q;
   float** d_qtab = nullptr;  // kMaxKrylov entries
 
@@ -327,44 +328,51 @@ double cheb_upper(const tdgl_ctx* c, double hdt) {
 // a = 1 - dt/2 (J's positive eigenvalues stay below 1: the (1 - |psi|^2) psi
 // term at psi = 0) and b = cheb_upper; sigma = (b + a)/(b - a),
 // omega_1 = 1, omega_2 = 2 sigma^2 / (2 sigma^2 - 1),
-// omega_{m+1} = 1 / (1 - omega_m / (4 sigma^2)).  The degree is common to the
-// batch: the request, or the largest over the active films of the degree
-// that brings the polynomial's deviation 1 / T_d(sigma) down to 0.25.
+// omega_{m+1} = 1 / (1 - omega_m / (4 sigma^2)).  Each film runs its own
+// degree d_f (table slot kChebDeg): the request, or the degree that brings
+// its polynomial's deviation 1 / T_d(sigma) down to 0.25; the sweeps of a
+// step run to the largest, films past their degree sit them out.
 int setup_cheb(tdgl_batch* B, int request, const std::vector<double>& dtv,
                const std::vector<int>& act, double* interval0 = nullptr) {
   B->cheb_deg = 0;
   if (request == 0) return TDGL_OK;
   const int F = B->nfilm;
   std::vector<double> lo(F), hi(F);
+  std::vector<int> deg(F, 1);
   int dmax = 1;
   for (int f = 0; f < F; ++f) {
     const double hdt = dtv[2 * f + 1];
     lo[f] = 1.0 - hdt;
     hi[f] = cheb_upper(B->films[f], hdt);
-    if (!act[f]) continue;
-    if (lo[f] < 0.1) return TDGL_OK;  // dt too large for a positive interval: no preconditioner
-    const double sig = (hi[f] + lo[f]) / (hi[f] - lo[f]);
-    const int d = (int)ceil(acosh(4.0) / acosh(sig));
-    dmax = std::max(dmax, d);
+    if (act[f] && lo[f] < 0.1) return TDGL_OK;  // dt too large for a positive interval: no preconditioner
+    int d = 1;
+    if (request > 0) {
+      d = request;
+    } else if (lo[f] < hi[f]) {
+      const double sig = (hi[f] + lo[f]) / (hi[f] - lo[f]);
+      d = (int)ceil(acosh(4.0) / acosh(sig));
+    }
+    deg[f] = std::max(1, std::min(d, kMaxChebDeg));
+    if (act[f]) dmax = std::max(dmax, deg[f]);
   }
-  int d = request > 0 ? request : dmax;
-  d = std::max(1, std::min(d, kMaxChebDeg));
   std::vector<double> tab((size_t)F * kChebStride, 0.0);
   for (int f = 0; f < F; ++f) {
+    double* t = tab.data() + (size_t)f * kChebStride;
+    t[kChebDeg] = (double)deg[f];
     if (!(lo[f] < hi[f])) continue;
     const double th = 0.5 * (hi[f] + lo[f]), de = 0.5 * (hi[f] - lo[f]);
     const double sig = th / de, s2 = sig * sig;
-    double* t = tab.data() + (size_t)f * kChebStride;
     t[0] = 1.0 / th;
     t[1] = 1.0;
-    if (d >= 2) t[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
-    for (int mm = 3; mm <= d; ++mm) t[mm] = 1.0 / (1.0 - t[mm - 1] / (4.0 * s2));
+    if (deg[f] >= 2) t[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
+    for (int mm = 3; mm <= deg[f]; ++mm) t[mm] = 1.0 / (1.0 - t[mm - 1] / (4.0 * s2));
   }
   int rc;
   if ((rc = ensure_cheb(B))) return rc;
   BCHK(hipMemcpyAsync(B->d_cheb, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, B->ps));
   BCHK(hipStreamSynchronize(B->ps));  // tab is a host temporary
-  B->cheb_deg = d;
+  B->cheb_deg = dmax;
+  B->cheb_degf = deg;
   if (interval0) {
     interval0[0] = lo[0];
     interval0[1] = hi[0];
@@ -383,8 +391,8 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
                const float* v, float* out) {
   const BatchK bk = kview(B);
   const int d = B->cheb_deg;
-  const double S = prof ? sum_over(B, act, s32) : 0.0;
   if (d == 0) {
+    const double S = prof ? sum_over(B, act, s32) : 0.0;
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S, 1, true) : -1;
     launch_bjvp32(bk, B->is3d, x32, v, B->d_qtab, B->d_iter, out, B->ps);
     if (prof) B->prof->end(B->ps, tok);
@@ -393,29 +401,33 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
   for (int m = 1; m <= d; ++m) {
     ChebStep cs{};
     cs.m = m;
-    const float* src = m == 1 ? v : B->t32[(m - 2) % 2];
+    cs.w = 1;
+    cs.v = v;
+    cs.out = out;
+    cs.mid = B->t32[(m - 1) % 2];
     cs.prev = m == 1 ? nullptr : (m == 2 ? v : B->t32[(m - 1) % 2]);
-    cs.fin = m == d;
-    cs.v = cs.fin ? v : nullptr;
-    cs.out = cs.fin ? out : B->t32[(m - 1) % 2];
-    // bytes: x, source, output; prev (m >= 2); v at the last sweep unless it is prev (m >= 3)
-    const double by = S * (3.0 + (m >= 2 ? 1.0 : 0.0) + (cs.fin && m >= 3 ? 1.0 : 0.0));
+    const float* src = m == 1 ? v : B->t32[(m - 2) % 2];
+    // bytes per film still sweeping: x, source, output; prev (m >= 2); v at
+    // its last sweep unless it is prev (m >= 3)
+    double by = 0.0;
+    if (prof)
+      for (int f = 0; f < B->nfilm; ++f)
+        if (act[f] && B->cheb_degf[f] >= m)
+          by += s32(B, f) * (3.0 + (m >= 2 ? 1.0 : 0.0) + (m == B->cheb_degf[f] && m >= 3 ? 1.0 : 0.0));
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }
 }
 
-// z = M u: the Chebyshev iterate z_d of A z = u (z_1 = u / theta,
+// z = M u: the Chebyshev iterate z_{d_f} of A z = u (z_1 = u / theta,
 // z_m = omega_m (z_{m-1} + (u - A z_{m-1}) / theta) - (omega_m - 1) z_{m-2}).
-// z_m lives in t32[m % 2]; returns z_d's buffer, or nullptr for d = 1
-// (z = u / theta, applied by the consumer).
-const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, const float* x32,
-                       const float* u) {
+// z_m lives in t32[m % 2], so film f's result is in t32[d_f % 2] (d_f >= 2) or
+// is u / theta (d_f = 1, applied by the consumer).
+void enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, const float* x32, const float* u) {
   const int d = B->cheb_deg;
-  if (d < 2) return nullptr;
+  if (d < 2) return;
   const BatchK bk = kview(B);
-  const double S = prof ? sum_over(B, act, s32) : 0.0;
   for (int m = 2; m <= d; ++m) {
     ChebStep cs{};
     cs.m = m;
@@ -425,23 +437,26 @@ const float* enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, co
     cs.src_th = m == 2;
     cs.prev = m == 2 ? nullptr : (m == 3 ? u : B->t32[m % 2]);
     cs.prev_th = m == 3;
-    cs.out = B->t32[m % 2];
-    // bytes: x, source, output; u unless it is the source (m >= 3); prev unless it is u (m >= 4)
-    const double by = S * (3.0 + (m >= 3 ? 1.0 : 0.0) + (m >= 4 ? 1.0 : 0.0));
+    cs.mid = B->t32[m % 2];
+    // bytes per film still sweeping: x, source, output; u unless it is the
+    // source (m >= 3); prev unless it is u (m >= 4)
+    double by = 0.0;
+    if (prof)
+      for (int f = 0; f < B->nfilm; ++f)
+        if (act[f] && B->cheb_degf[f] >= m) by += s32(B, f) * (3.0 + (m >= 3 ? 1.0 : 0.0) + (m >= 4 ? 1.0 : 0.0));
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
     launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }
-  return B->t32[d % 2];
 }
 
 // x += M u for the films in `ok`, u = R y in r0_32, with ||M u||_inf partials.
 void enqueue_precond_update(tdgl_batch* B, bool prof, const std::vector<int>& ok) {
   const BatchK bk = kview(B);
-  const float* z = enqueue_z(B, prof, ok, B->x32, B->r0_32);
+  enqueue_z(B, prof, ok, B->x32, B->r0_32);
   const double S = prof ? sum_over(B, ok, s32) : 0.0;
   const int tok = prof ? B->prof->begin(B->ps, TDGL_K_ACCUM, S + 2.0 * sum_over(B, ok, s64), 2) : -1;
-  launch_baddz(bk, B->w_x, z ? z : B->r0_32, z ? 0 : 1, B->d_pm, B->ps);
+  launch_baddz(bk, B->w_x, B->r0_32, B->t32[0], B->t32[1], B->d_pm, B->ps);
   launch_bcolsum(bk, B->d_iter, B->d_pm, CS_DXINF, 1, B->ps);
   if (prof) B->prof->end(B->ps, tok);
 }
@@ -605,7 +620,7 @@ int tgcr(tdgl_batch* B, const std::vector<int>& act0, double tol, std::vector<td
   for (int f = 0; f < F; ++f) {
     if (!act0[f]) continue;
     const double kit = rep(B, f)[8];
-    const int d = B->cheb_deg;
+    const int d = B->cheb_deg ? B->cheb_degf[f] : 0;
     S[f].jvp_evals += (int)kit * std::max(d, 1);
     // iteration i: W (3 without a preconditioner), dots (i + 1), update (i + 3), finish 4
     const double wb = d == 0 ? 3.0 : 3.0 * d + (d >= 2 ? d - 1.0 : 0.0) + (d >= 3 ? 1.0 : 0.0);
@@ -745,9 +760,10 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
         if (d == 0) {
           S[f].bytes_moved += (kk + 1.0) * s32(B, f) + 2.0 * s64(B, f);
         } else {
-          const double zb = d >= 2 ? 3.0 * (d - 1) + (d >= 3 ? 1.0 : 0.0) + (d >= 4 ? 2.0 * (d - 3) : 0.0) : 0.0;
+          const int df = B->cheb_degf[f];
+          const double zb = df >= 2 ? 3.0 * (df - 1) + (df >= 3 ? 1.0 : 0.0) + (df >= 4 ? 2.0 * (df - 3) : 0.0) : 0.0;
           S[f].bytes_moved += (kk + 2.0 + zb + 1.0) * s32(B, f) + 2.0 * s64(B, f);
-          S[f].jvp_evals += std::max(d - 1, 0);
+          S[f].jvp_evals += std::max(df - 1, 0);
         }
       }
       if ((rc = newton_eval(B, ok, S))) return rc;  // newton.py:109-113
@@ -902,7 +918,8 @@ int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt,
   if ((rc = setup_cheb(B, degree, dtv, one, iv))) return done(rc);
   if (B->cheb_deg == 0) return done(bfail(B, TDGL_E_INVALID, "no preconditioner for this degree and dt"));
   enqueue_w(B, false, one, x32, v32, w32);
-  const float* z = enqueue_z(B, false, one, x32, v32);
+  enqueue_z(B, false, one, x32, v32);
+  const float* z = B->cheb_deg >= 2 ? B->t32[B->cheb_deg % 2] : nullptr;  // one film: its own degree
   if (z && hipMemcpyAsync(z32, z, sizeof(float) * (size_t)B->hf[0].len, hipMemcpyDeviceToDevice, B->ps) != hipSuccess)
     return done(bfail(B, TDGL_E_HIP, "hipMemcpyAsync"));
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess)

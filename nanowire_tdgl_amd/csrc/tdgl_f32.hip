@@ -116,14 +116,20 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   const Mat mt = fd->mt;
   const long long off = fd->off;
   const unsigned int n = (unsigned int)g.n;  // the host keeps 20 n below 2^32
-  const W32 out = wplanes32((out_direct ? out_direct : Q[*iter]) + off, n);
+  // the film's own Chebyshev degree: past it the film sits the sweep out; its
+  // last W sweep writes v - rho to the final vector
+  const double* cf = CHEB ? bk.cheb + (long long)f * kChebStride : nullptr;
+  const int df = CHEB ? (int)cf[kChebDeg] : 0;
+  if (CHEB && cs.m > df) return;
+  const bool fin = CHEB && cs.w && cs.m == df;
+  float* const obase = CHEB ? (fin ? (cs.out ? cs.out : Q[*iter]) : cs.mid) : (out_direct ? out_direct : Q[*iter]);
+  const W32 out = wplanes32(obase + off, n);
   const V32 X = planes32(x32 + off, n), P = planes32(p32 + off, n);
   const float hdt = (float)bk.dtv[2 * f + 1];
   float c_it = 0.f, c_om = 0.f, c_ss = 1.f, c_sp = 0.f;
-  const bool has_v = CHEB && cs.v, has_p = CHEB && cs.prev;
+  const bool has_v = CHEB && cs.v && (fin || cs.cv), has_p = CHEB && cs.prev;
   V32 CV{}, CP{};
   if (CHEB) {
-    const double* cf = bk.cheb + (long long)f * kChebStride;
     c_it = (float)cf[0];
     c_om = (float)cf[cs.m];
     c_ss = cs.src_th ? c_it : 1.f;
@@ -390,7 +396,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
           const float y = c_ss * pv;
           const float ay = c_ss * (pv - hdt * jv);
           const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
-          return cs.fin ? vv - res : res;
+          return fin ? vv - res : res;
         };
         out.psi(q, make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
         out.ph(2, q, step(praw_x, jx, v_x, w_x));
@@ -781,12 +787,16 @@ __global__ __launch_bounds__(kRedThreads) void k_baccum_u(BatchK bk, float* u, c
   }
 }
 
-// x (FP64) += s z, s = 1/theta (degree-1 preconditioner: M = 1/theta) or 1;
-// partials of ||s z||_inf (complex modulus on psi, |.| on phi).
-__global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, const float* z,
-                                                       int scale_th, double* pm) {
+// x (FP64) += z_f, the film's Chebyshev iterate z_{d_f}: u / theta for d_f = 1
+// (M = 1/theta), else the buffer t[d_f % 2] its last z sweep wrote; partials
+// of ||z_f||_inf (complex modulus on psi, |.| on phi).
+__global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, const float* u,
+                                                       const float* t0, const float* t1, double* pm) {
   FILM_STREAM(bk);
-  const double s = scale_th ? bk.cheb[(long long)f * kChebStride] : 1.0;
+  const double* cf = bk.cheb + (long long)f * kChebStride;
+  const int df = (int)cf[kChebDeg];
+  const double s = df < 2 ? cf[0] : 1.0;
+  const float* z = df < 2 ? u : ((df & 1) ? t1 : t0);
   const float4* Z = reinterpret_cast<const float4*>(z + off);
   double2* X = reinterpret_cast<double2*>(x + off);
   const long long npsi = 2 * fd->g.n;
@@ -849,22 +859,22 @@ void launch_sweep(const BatchK& bk, const float* x32, const float* p32, float* c
 }
 void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                    const int* iter, float* out_direct, hipStream_t st) {
-  const ChebStep none{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0};
+  const ChebStep none{};
   if (is3d) launch_sweep<true, false>(bk, x32, p32, Q, iter, out_direct, none, st);
   else launch_sweep<false, false>(bk, x32, p32, Q, iter, out_direct, none, st);
 }
 void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                     const int* iter, const ChebStep& cs, hipStream_t st) {
-  if (is3d) launch_sweep<true, true>(bk, x32, p32, Q, iter, cs.out, cs, st);
-  else launch_sweep<false, true>(bk, x32, p32, Q, iter, cs.out, cs, st);
+  if (is3d) launch_sweep<true, true>(bk, x32, p32, Q, iter, nullptr, cs, st);
+  else launch_sweep<false, true>(bk, x32, p32, Q, iter, nullptr, cs, st);
 }
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st) {
   launch(k_baccum_u, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, u, r0v, Q, fstat);
 }
-void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
+void launch_baddz(const BatchK& bk, double* x, const float* u, const float* t0, const float* t1, double* pm,
                   hipStream_t st) {
-  launch(k_baddz, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, z, scale_th, pm);
+  launch(k_baddz, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, x, u, t0, t1, pm);
 }
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                   double* pm, hipStream_t st) {

@@ -66,6 +66,7 @@ __host__ __device__ constexpr int sweep_ty(int s) { return s == 1 ? 8 : (s == 2 
 // the per-film coefficient stride (1/theta, then omega_1 .. omega_d).
 constexpr int kMaxChebDeg = 64;
 constexpr int kChebStride = kMaxChebDeg + 2;
+constexpr int kChebDeg = kChebStride - 1;  // slot of the film's degree (as a double)
 constexpr int RS_ALPHA = RS_BETA + kMaxKrylov;
 constexpr int RS_DBETA = RS_ALPHA + kMaxKrylov;  // mixed: second-pass coefficients
 // FP64 TGCR: 1.0 once the iteration loop has ended (converged, breakdown,
@@ -194,23 +195,29 @@ void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p
 //   out = fin ? v - res : res
 // (prev == nullptr: no prev term; s' = 1/theta when prev_th).  out may alias
 // prev; out == nullptr writes Q[*iter].
+// One sweep m of a film's Chebyshev recurrence.  Each film runs its own
+// degree d_f (the cheb table's slot kChebDeg): a film with d_f < m sits the
+// sweep out; W sweeps (w = 1) write `mid` before the film's last sweep and
+// v - rho into `out` (nullptr: Q[*iter]) at m = d_f; z sweeps write `mid`.
 struct ChebStep {
   const float* v;
   const float* prev;
   float* out;
+  float* mid;
   int m;
-  int src_th, prev_th, cv, fin;
+  int src_th, prev_th, cv, w;
 };
 void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
                     const int* iter, const ChebStep& cs, hipStream_t st);
-// x (FP64) += s z with s = 1/theta of the film when scale_th, else 1; partials of ||s z||_inf.
 // out = x + ratio (x - xp): the extrapolating predictor (ratio[f] <= 0: film
 // f keeps out as it is)
 void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,
                     hipStream_t st);
 void launch_extrap64(const double* x, const double* xp, double ratio, double* out, long long len,
                      hipStream_t st);
-void launch_baddz(const BatchK& bk, double* x, const float* z, int scale_th, double* pm,
+// x (FP64) += z_f with z_f the film's Chebyshev iterate: u / theta for degree
+// 1, else t[d_f % 2]; partials of ||z_f||_inf.
+void launch_baddz(const BatchK& bk, double* x, const float* u, const float* t0, const float* t1, double* pm,
                   hipStream_t st);
 void launch_bmdot(const BatchK& bk, int pass, float* const* Q, const float* rv, const int* iter,
                   double* pm, hipStream_t st);

@@ -100,7 +100,7 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     :func:`nanowire_tdgl_amd.solvers.integrators.trapezoidal`; ``tune`` runs
     the stencil autotuner first; ``max_attempts`` bounds the number of
     lockstep attempts (None: until every film reaches t_stop); ``precond`` as
-    in ``trapezoidal`` (the automatic degree is the largest any film asks for);
+    in ``trapezoidal`` (each film takes its own automatic degree);
     ``predictor`` as in ``trapezoidal``, per film."""
     if predictor not in ("euler", "extrapolate"):
         raise ValueError("predictor must be 'euler' or 'extrapolate'")

@@ -3,8 +3,8 @@
 * A film's arithmetic does not depend on the other films of its batch: each
   film owns its work items, streaming blocks and reduction order, so a batch
   reproduces every film's own batch-of-one step bit for bit (with or without
-  a Chebyshev preconditioner of a fixed degree; the automatic degree is the
-  largest any film of the batch asks for).
+  a Chebyshev preconditioner, of a fixed degree or of each film's own
+  automatic degree).
 * Films switched off for an attempt are left untouched.
 * Against FP64 steps of the same films: |psi|^2, B_z and J_s agree to 1e-6
   relative (of the largest entry) after several steps.
@@ -74,9 +74,8 @@ def _observables(eng, X, n):
     return np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy()
 
 
-@pytest.mark.parametrize("precond", ["none", 6])
+@pytest.mark.parametrize("precond", ["none", 6, "chebyshev"])
 def test_batch_reproduces_each_film_bitwise(precond):
-    # a fixed preconditioner degree: the automatic one is the batch's largest
     from nanowire_tdgl_amd import _native
     from nanowire_tdgl_amd.solvers.batch import BatchRunner
     films = _films()
