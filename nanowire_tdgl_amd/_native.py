@@ -58,6 +58,13 @@ class SolverOpts(C.Structure):
                 ("x_prev", C.c_void_p), ("dt_prev", C.c_void_p)]
 
 
+def _is_vec(t, dtype, numel: int) -> bool:
+    """A contiguous device tensor of that dtype and length (what a kernel
+    argument must be: a host tensor's pointer is no device address)."""
+    return (getattr(t, "dtype", None) == dtype and t.numel() == numel and t.is_cuda
+            and t.is_contiguous())
+
+
 def _with_history(opts: "SolverOpts", x_prev, dt_prev, nfilm: int, total_len: int, torch):
     """A copy of opts whose predictor extrapolates from the last accepted step
     (x_prev: the state before it, dt_prev: its size per film); returns the copy
@@ -66,8 +73,8 @@ def _with_history(opts: "SolverOpts", x_prev, dt_prev, nfilm: int, total_len: in
     o = SolverOpts.from_buffer_copy(opts)
     if x_prev is None:
         return o, None
-    if x_prev.dtype != torch.float64 or x_prev.numel() != total_len or not x_prev.is_contiguous():
-        raise ValueError("x_prev must be a contiguous float64 tensor shaped like the state")
+    if not _is_vec(x_prev, torch.float64, total_len):
+        raise ValueError("x_prev must be a contiguous float64 device tensor shaped like the state")
     d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt_prev, dtype=np.float64), (nfilm,)))
     o.x_prev = x_prev.data_ptr()
     o.dt_prev = d.ctypes.data
@@ -276,18 +283,25 @@ class Context:
         self._call("tdgl_set_field_vectors", *[C.c_void_p(v.ctypes.data) for v in a])
 
     # -- compute -------------------------------------------------------------
+    def _vecs64(self, what: str, *vs) -> None:
+        for v in vs:
+            if not _is_vec(v, self.torch.float64, self.len):
+                raise ValueError(f"{what} takes float64 device vectors of the internal state length")
+
     def rhs(self, x, f) -> None:
+        self._vecs64("rhs", x, f)
         self._call("tdgl_rhs", C.c_void_p(x.data_ptr()), C.c_void_p(f.data_ptr()))
 
     def jvp_trap_fd(self, x, f0, p, eps: float, dt: float, ap) -> None:
+        self._vecs64("jvp_trap_fd", x, f0, p, ap)
         self._call("tdgl_jvp_trap_fd", C.c_void_p(x.data_ptr()), C.c_void_p(f0.data_ptr()),
                    C.c_void_p(p.data_ptr()), C.c_double(eps), C.c_double(dt),
                    C.c_void_p(ap.data_ptr()))
 
     def jvp_trap32(self, x32, p32, dt: float, ap32) -> None:
         for v in (x32, p32, ap32):
-            if v.dtype != self.torch.float32 or v.numel() != self.len:
-                raise ValueError("jvp_trap32 takes float32 vectors of the internal state length")
+            if not _is_vec(v, self.torch.float32, self.len):
+                raise ValueError("jvp_trap32 takes float32 device vectors of the internal state length")
         self._call("tdgl_jvp_trap32", C.c_void_p(x32.data_ptr()), C.c_void_p(p32.data_ptr()),
                    C.c_double(dt), C.c_void_p(ap32.data_ptr()))
 
@@ -296,8 +310,8 @@ class Context:
         w32 = A M v, z32 = M v (A = I - dt/2 J(x32)).  Returns the interval
         (a, b), the degree d, and whether z32 was left unwritten (d = 1: z = v/theta)."""
         for v in (x32, v32, w32, z32):
-            if v.dtype != self.torch.float32 or v.numel() != self.len:
-                raise ValueError("precond_cheb32 takes float32 vectors of the internal state length")
+            if not _is_vec(v, self.torch.float32, self.len):
+                raise ValueError("precond_cheb32 takes float32 device vectors of the internal state length")
         info = (C.c_double * 4)()
         self._call("tdgl_precond_cheb32", C.c_void_p(x32.data_ptr()), C.c_void_p(v32.data_ptr()),
                    C.c_double(dt), C.c_int(int(degree)), C.c_void_p(w32.data_ptr()),
@@ -326,8 +340,8 @@ class Context:
         right-preconditioned as `precond` says (as for the mixed method:
         "chebyshev" automatic degree, "none", or a degree); returns (mu, iterations)."""
         for v in (x1, rhs):
-            if v.dtype != self.torch.float64 or v.numel() != self.len:
-                raise ValueError("adjoint_solve takes float64 vectors of the internal state length")
+            if not _is_vec(v, self.torch.float64, self.len):
+                raise ValueError("adjoint_solve takes float64 device vectors of the internal state length")
         out = self.torch.empty_like(rhs) if out is None else out
         it = C.c_int(0)
         self._call("tdgl_adjoint_solve", C.c_void_p(x1.data_ptr()), C.c_void_p(rhs.data_ptr()),
@@ -349,6 +363,8 @@ class Context:
         """One step attempt; with x_prev (the state before the last accepted
         step, of size dt_prev > 0) Newton starts from the extrapolation
         X + dt/dt_prev (X - x_prev) instead of the reference's X + dt f(X)."""
+        if not _is_vec(X, self.torch.float64, self.len):
+            raise ValueError("X must be a contiguous float64 device tensor of the internal state length")
         st = StepStats()
         o, keep = _with_history(opts, x_prev, dt_prev, 1, self.len, self.torch)
         self._call("tdgl_trap_step", C.c_void_p(X.data_ptr()), C.c_double(dt), C.byref(o),
@@ -357,6 +373,7 @@ class Context:
         return st
 
     def euler_step(self, X, dt: float) -> None:
+        self._vecs64("euler_step", X)
         self._call("tdgl_euler_step", C.c_void_p(X.data_ptr()), C.c_double(dt))
 
     def profile_enable(self, on: bool = True) -> None:
@@ -372,6 +389,7 @@ class Context:
         return out
 
     def observables(self, x, want_b=True, want_j=True):
+        self._vecs64("observables", x)
         t = self.torch
         b = t.empty(3 * self.n, dtype=t.float64, device=self.device) if want_b else None
         j = t.empty(3 * self.n, dtype=t.float64, device=self.device) if want_j else None
@@ -477,8 +495,8 @@ class Batch:
         none) as Context.trap_step's, packed like X."""
         import numpy as np
         F = self.nfilm
-        if X.dtype != self.torch.float64 or X.numel() != self.total_len:
-            raise ValueError("X must be a float64 tensor of total_len elements")
+        if not _is_vec(X, self.torch.float64, self.total_len):
+            raise ValueError("X must be a contiguous float64 device tensor of total_len elements")
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (F,)))
         a = None if active is None else np.ascontiguousarray(np.asarray(active, dtype=np.int32))
         o, keep = _with_history(opts, x_prev, 0.0 if dt_prev is None else dt_prev, F, self.total_len,

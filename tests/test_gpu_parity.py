@@ -64,6 +64,29 @@ def test_grid_beyond_32bit_offsets_rejected():
     ok.close()
 
 
+def test_host_tensors_rejected():
+    """A host tensor's pointer is no device address: every entry point that
+    takes device vectors rejects one (and wrong lengths) before any launch."""
+    import torch
+    from nanowire_tdgl_amd import _native
+    T, engine_for, *_ = _lib()
+    dev = device(S.EVALF_CASES["c1_bz"])
+    ctx = engine_for(dev.params, dev.idx, dev.material).ctx
+    X = ctx.to_internal(rand_state(dev.params.n_interior, 3, 5, 0.1, 0.1))
+    host = X.cpu()
+    opts = _native.solver_opts()
+    for call in (lambda: ctx.trap_step(host, 0.1, opts),
+                 lambda: ctx.trap_step(X[:-1], 0.1, opts),
+                 lambda: ctx.trap_step(X, 0.1, opts, x_prev=host, dt_prev=0.1),
+                 lambda: ctx.euler_step(host, 0.1),
+                 lambda: ctx.observables(host),
+                 lambda: ctx.rhs(X, host),
+                 lambda: ctx.jvp_trap32(X.float(), host.float(), 0.1, X.float())):
+        with pytest.raises(ValueError):
+            call()
+    assert torch.isfinite(X).all()
+
+
 def test_permute_round_trip_and_complex_phi_rejected():
     T, engine_for, rhs, bbfv, _ = _lib()
     dev = device(S.EVALF_CASES["hole_zr"])
