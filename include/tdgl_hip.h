@@ -82,7 +82,8 @@ typedef struct tdgl_solver_opts {
    * on the interval [1 - dt/2, 1 + dt/2 ||J||_inf] (||J||_inf bounded from the
    * stencil coefficients with |psi| <= 2).  0: none; > 0: polynomial degree
    * (minus one) = this many stencil sweeps per TGCR iteration; < 0: the
-   * degree that contracts the spectrum to within 0.25 of 1.  TGCR's stop test
+   * degree that contracts the spectrum to within 0.25 of 1 (in a batch, each
+   * film's own).  TGCR's stop test
    * stays on the true residual of (I - dt/2 J) dx = -g; the Newton residual
    * is unchanged.  Off when dt/2 >= 0.9 (the interval would reach 0). */
   int32_t mixed_precond;
