@@ -223,26 +223,36 @@ def test_trajectory_matches_reference(which):
     assert err < 1e-9
 
 
-def test_extrapolating_predictor_trajectory_matches_oracle():
+@pytest.mark.parametrize("which", ["trap_hole", "trap_dtcut"])
+def test_extrapolating_predictor_trajectory_matches_oracle(which):
     """predictor="extrapolate" (no counterpart in the reference): the FP64 path
-    against the oracle's restatement of the same predictor, trap_hole case:
-    identical accept decisions and Newton iteration counts, states to 1e-9."""
+    against the oracle's restatement of the same predictor: identical accept
+    decisions and Newton iteration counts, states to 1e-9.  trap_dtcut cuts dt
+    (TGCR capped at 4 iterations), so its extrapolations use dt / dt_prev != 1."""
     T, engine_for, rhs, bbfv, integ = _lib()
     from nanowire_tdgl_amd.solvers.runner import _make_eval_u
-    case = S.TRAP_HOLE
-    d = golden("trap_hole")
+    case = {"trap_hole": S.TRAP_HOLE, "trap_dtcut": S.TRAP_DTCUT}[which]
+    d = golden(which)
     dev = device(case)
     f = oracle_film(case)
-    t_stop = 2.0  # ten steps (the oracle's share of the test stays a few seconds)
-    eval_u = _make_eval_u(T.AppliedField(**case["field"]), dev.params, dev.idx, t_stop)
+    # a few seconds of oracle work; on trap_dtcut every attempt at this t_stop
+    # is cut three times before one is accepted
+    t_stop = {"trap_hole": 2.0, "trap_dtcut": 0.003}[which]
+    field = case["field"]
+    eval_u = _make_eval_u(T.AppliedField(**field), dev.params, dev.idx, t_stop)
     log = []
     times, hist = integ.trapezoidal(d["x0"], dev.params, dev.idx, eval_u, 0.0, t_stop, case["dt"],
                                     tol_gcr=case["tol_gcr"], progress=False, material=dev.material,
-                                    step_log=log, predictor="extrapolate")
+                                    tgcr_max_iter=case["tgcr_max_iter"], step_log=log,
+                                    predictor="extrapolate")
+    frac = field.get("t_on_fraction", 2.0 / 3.0)
     olog = []
-    otimes, ohist = O.trapezoidal(d["x0"], f, lambda t: O.applied_field_at(t, t_stop, (0.0, 0.0, case["field"]["Bz"])),
+    otimes, ohist = O.trapezoidal(d["x0"], f, lambda t: O.applied_field_at(t, t_stop, (0.0, 0.0, field["Bz"]),
+                                                                          t_on_fraction=frac),
                                   0.0, t_stop, case["dt"], tol_gcr=case["tol_gcr"], log=olog,
-                                  predictor="extrapolate")
+                                  tgcr_max_iter=case["tgcr_max_iter"], predictor="extrapolate")
+    if which == "trap_dtcut":
+        assert any(not c for c, *_ in olog)  # the case does cut dt
     assert [(bool(r["converged"]), r["newton_iters"]) for r in log] == [(c, it) for c, it, _, _ in olog]
     assert np.allclose(times, otimes, rtol=0, atol=1e-12)
     err = rel_err(hist, ohist)
