@@ -70,6 +70,12 @@ class AdjointProblem:
     tol_gcr: float = 1e-4
     adjoint_tol: float = 1e-12
     adjoint_precond: object = "chebyshev"  # the adjoint TGCR's preconditioner (as the mixed method's)
+    # Newton's start in the forward steps: "extrapolate" (X + (X - X_prev),
+    # fewer Newton iterations) or "euler" (X + dt f, the reference's); with the
+    # tight Newton tolerances above both give the same trajectory to ~1e-10.
+    # Checkpoints keep X_prev, so the recomputed segments repeat the forward
+    # steps bit for bit either way.
+    predictor: str = "extrapolate"
 
     def __post_init__(self):
         self.device = Device(self.params)
@@ -102,9 +108,9 @@ class AdjointProblem:
         self.ctx.set_field(0.0, 0.0, bz * self.profile(t))
 
     # -- forward --------------------------------------------------------------
-    def _step(self, X, bz: float, t: float) -> None:
+    def _step(self, X, bz: float, t: float, x_prev=None) -> None:
         self._set_field(bz, t)
-        st = self.ctx.trap_step(X, self.dt, self._opts)
+        st = self.ctx.trap_step(X, self.dt, self._opts, x_prev=x_prev, dt_prev=self.dt)
         if not st.converged:
             raise RuntimeError(f"trapezoidal step at t={t:.4g} did not converge at dt={self.dt} "
                                "(the adjoint assumes a fixed step; loosen the tolerances)")
@@ -116,12 +122,19 @@ class AdjointProblem:
         import torch
         self.set_parameters(kappa, w_full)
         self._opts = _native.solver_opts(self.newton_tol_f, self.newton_tol_dx, 20, self.tol_gcr)
+        if self.predictor not in ("euler", "extrapolate"):
+            raise ValueError("predictor must be 'euler' or 'extrapolate'")
+        ex = self.predictor == "extrapolate"
         X = self.ctx.to_internal(self.x0)
-        self.checkpoints = {0: X.clone()}
+        # checkpoint n: (X_n, X_{n-1}) -- the second only for the extrapolating predictor
+        self.checkpoints = {0: (X.clone(), None)}
+        prev = None
         for n in range(self.steps):
-            self._step(X, bz, n * self.dt)
+            x_in = X.clone() if ex else None
+            self._step(X, bz, n * self.dt, prev)
+            prev = x_in
             if (n + 1) % self.checkpoint_every == 0:
-                self.checkpoints[n + 1] = X.clone()
+                self.checkpoints[n + 1] = (X.clone(), prev)
         self.X_final = X
         self.ctx.set_field(0.0, 0.0, bz * self.profile(self.t_stop))
         b, _ = self.ctx.observables(X, want_b=True, want_j=False)
@@ -153,12 +166,15 @@ class AdjointProblem:
             if lo >= hi:
                 continue
             # recompute the segment's states from its checkpoint
-            X = self.checkpoints[lo].clone()
+            X0, prev = self.checkpoints[lo]
+            X = X0.clone()
             states = [X.clone()]
             for m in range(lo, hi):
-                self._step(X, bz, m * dt)
+                self._step(X, bz, m * dt, prev)
+                if self.predictor == "extrapolate":
+                    prev = states[-1]
                 states.append(X.clone())
-            if hi in self.checkpoints and not torch.equal(states[-1], self.checkpoints[hi]):
+            if hi in self.checkpoints and not torch.equal(states[-1], self.checkpoints[hi][0]):
                 raise RuntimeError("recomputed segment differs from its checkpoint")
             for m in range(hi - 1, lo - 1, -1):
                 t = m * dt
