@@ -515,6 +515,26 @@ def section_adjoint(steps, every, fd=True, predictor=None):
                      "rel_diff_kappa": abs(fdk - out["grad"]["kappa"]) / abs(fdk),
                      "rel_diff_Bz": abs(fdb - out["grad"]["Bz"]) / abs(fdb)}
         log(f"[bench] adjoint vs central differences: {out['fd']}")
+        # 20 random interior mask cells (seed 4), the same eps
+        gw = w.grad.numpy()
+        full = prob.device.idx.interior_to_full
+        cells = np.random.default_rng(4).choice(full, size=20, replace=False)
+
+        def Lw(wv):
+            _, bzo = prob.forward(2.5, 0.3, wv)
+            return float(midplane_target_loss(prob, bzo))
+
+        diffs = []
+        for m in cells:
+            wp, wm = w0.copy(), w0.copy()
+            wp[m] += eps
+            wm[m] -= eps
+            diffs.append(abs((Lw(wp) - Lw(wm)) / (2 * eps) - gw[m]))
+        scale = float(np.max(np.abs(gw)))
+        out["fd"]["mask_cells"] = {"cells": 20, "max_abs_diff": float(max(diffs)), "grad_scale": scale,
+                                   "max_diff_over_scale": float(max(diffs)) / scale}
+        log(f"[bench] adjoint vs central differences on 20 mask cells: max |diff| {max(diffs):.3e} "
+            f"({max(diffs) / scale:.2e} of max |dL/dw|)")
     return out
 
 
