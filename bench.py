@@ -525,11 +525,13 @@ def section_adjoint(steps, every, fd=True, predictor=None):
             return float(midplane_target_loss(prob, bzo))
 
         diffs = []
-        for m in cells:
+        for c, m in enumerate(cells):
             wp, wm = w0.copy(), w0.copy()
             wp[m] += eps
             wm[m] -= eps
             diffs.append(abs((Lw(wp) - Lw(wm)) / (2 * eps) - gw[m]))
+            if (c + 1) % 5 == 0:
+                log(f"[bench] mask cells: {c + 1} of 20 done")
         scale = float(np.max(np.abs(gw)))
         out["fd"]["mask_cells"] = {"cells": 20, "max_abs_diff": float(max(diffs)), "grad_scale": scale,
                                    "max_diff_over_scale": float(max(diffs)) / scale}
