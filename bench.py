@@ -540,6 +540,29 @@ def section_adjoint(steps, every, fd=True, predictor=None):
     return out
 
 
+def whole_run(total: int = 1000, chunk: int = 100) -> None:
+    """The production film's configured run (1000 steps, t = 0..200) for the
+    FP64 plain, mixed, and mixed extrapolating methods, timed in chunks;
+    prints the rate of every chunk and of the whole run."""
+    import torch
+    dev, x0 = build_film()
+    n = dev.params.n_interior
+    for prec, pred in (("f64", "euler"), ("mixed", "euler"), ("mixed", "extrapolate")):
+        s = Stepper(dev, x0, precision=prec, predictor=pred)
+        tot = 0.0
+        for done in range(chunk, total + 1, chunk):
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            s.accepted_steps(chunk)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - c0
+            tot += el
+            nit = sum(r["newton_iters"] for r in s.log[-chunk:]) / chunk
+            log(f"[bench] {prec} {pred}: steps {done}, t = {s.t:.1f}: {n * chunk / el:.4g} cell-steps/s, "
+                f"{nit:.2f} Newton iterations per step")
+        log(f"[bench] {prec} {pred}: {total} steps in {tot:.1f} s = {n * total / tot:.4g} cell-steps/s")
+
+
 def calibrate_copy(mib: int, reps: int) -> None:
     """Copy a resident `mib` MiB buffer `reps` times (one copy kernel per
     rep): the reference point that scales rocprofv3's FETCH_SIZE and
@@ -558,6 +581,8 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate the PMC byte counters")
+    ap.add_argument("--whole-run", action="store_true",
+                    help="only time the production film's whole 1000-step run per method, in chunks")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -575,6 +600,9 @@ def main() -> None:
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 5)
+        return
+    if args.whole_run:
+        whole_run()
         return
     global PROFILE
     PROFILE = not args.no_profile
