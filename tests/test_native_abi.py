@@ -71,3 +71,17 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_native, "_lib", None)
     with pytest.raises(ImportError, match="build it"):
         _native.load()
+
+
+def test_device_vector_check_rejects_host_and_strided_tensors():
+    """The bindings hand kernels raw pointers; _is_vec admits only contiguous
+    device tensors of the right dtype and length (checked here without a GPU:
+    every host tensor fails it)."""
+    import torch
+    from nanowire_tdgl_amd._native import _is_vec
+    x = torch.zeros(10, dtype=torch.float64)
+    assert not _is_vec(x, torch.float64, 10)          # host memory
+    assert not _is_vec(x[::2], torch.float64, 5)      # strided
+    assert not _is_vec(x.float(), torch.float64, 10)  # dtype
+    assert not _is_vec(x, torch.float64, 9)           # length
+    assert not _is_vec(None, torch.float64, 10)
