@@ -131,6 +131,27 @@ def test_batch_extrapolating_predictor_reproduces_each_film_bitwise():
         runner.batch.trap_step(X, dt, opts, x_prev=X[:10], dt_prev=dt)
 
 
+def test_trapezoidal_batch_extrapolating_matches_single_films():
+    """trapezoidal_batch's per-film history bookkeeping (predictor="extrapolate")
+    against each film's own trapezoidal run with the same predictor: the same
+    final states bit for bit."""
+    from nanowire_tdgl_amd.solvers import integrators
+    from nanowire_tdgl_amd.solvers.batch import trapezoidal_batch
+    from nanowire_tdgl_amd.solvers.runner import _make_eval_u
+    films = _films()
+    t_end, dt = 1.2, 0.2
+    res = trapezoidal_batch(films, 0.0, t_end, dt, predictor="extrapolate")
+    for f, film in enumerate(films):
+        d = film.device
+        eval_u = _make_eval_u(d.applied_field, d.params, d.idx, t_end)
+        log = []
+        times, hist = integrators.trapezoidal(film.x0, d.params, d.idx, eval_u, 0.0, t_end, dt,
+                                              precision="mixed", predictor="extrapolate",
+                                              material=d.material, progress=False, step_log=log)
+        assert res.steps[f] == sum(1 for r in log if r["converged"])
+        assert np.array_equal(hist[:, -1], res.states[f]), f
+
+
 def test_inactive_films_are_untouched():
     from nanowire_tdgl_amd import _native
     from nanowire_tdgl_amd.solvers.batch import BatchRunner
