@@ -22,6 +22,8 @@ Pinning (what checks each function; see ``tests/test_oracle.py``):
   trilayer, 2-D, periodic), closed forms (FPSI values, zero-gauge Laplacian,
   LPHI diagonal, uniform periodic equilibrium, C4 antisymmetry), and the
   CSR-vs-stencil cross-check;
+* jvp_stencil                 -- central differences of eval_f_csr on the ten
+  golden eval_f inputs (second-order convergence to it), hole rows zero;
 * tgcr / newton / trapezoidal -- golden ``tgcr_trap``/``newton_trap``/``trap_*``
   (values and iteration counts), dense linear solves, scalar Newton problems;
 * forward_euler, observables   -- golden ``euler_small`` and ``obs_aniso``.
@@ -664,46 +666,6 @@ def jvp_stencil(X, P, g: Grid, idx: dict, u, mat: Optional[Material] = None) -> 
         if hm.size and q + 1 < len(res):
             res[q + 1][hm] = 0.0
     return np.concatenate(res)
-
-
-def tgcr_trap_exact(jv: Callable, b, dt, tol=1e-4, max_iter=None, stats=None):
-    """TGCR of tgcr.py:96-156 with an exact product jv(p) = J p in place of the
-    finite difference (the mixed-precision path's operator)."""
-    N = len(b)
-    if max_iter is None:
-        max_iter = max(N, int(round(0.2 * N)))
-    x = np.zeros_like(b)
-    r = b.copy()
-    r0 = np.linalg.norm(r)
-    rk = r0
-    if r0 == 0.0:
-        return x
-    P, AP = [], []
-    k = 0
-    while rk / r0 > tol and k < max_iter:
-        k += 1
-        p = r.copy()
-        Ap = p - (dt / 2.0) * jv(p)
-        for pj, apj in zip(P, AP):
-            beta = np.vdot(Ap, apj).real
-            p = p - beta * pj
-            Ap = Ap - beta * apj
-        nrm = np.linalg.norm(Ap)
-        if nrm < 1e-14:
-            break
-        Ap /= nrm
-        p /= nrm
-        P.append(p)
-        AP.append(Ap)
-        alpha = np.vdot(r, Ap).real
-        x = x + alpha * p
-        r = r - alpha * Ap
-        rk = np.linalg.norm(r)
-    if stats is not None:
-        stats["tgcr_iters"] = k
-    if rk > tol * r0:
-        return np.array([], dtype=np.complex128)
-    return x
 
 
 eval_f = eval_f_csr

@@ -99,6 +99,62 @@ def test_eval_f_matches_reference(name):
     assert rel_err(Fs, d["F"]) < 1e-15
 
 
+@pytest.mark.parametrize("name", list(S.EVALF_CASES))
+def test_jvp_stencil_central_differences(name):
+    """jvp_stencil is the exact derivative of eval_f (rhs.py:147-239) along P:
+    the central difference (F(X + hP) - F(X - hP)) / 2h of the reference's own
+    eval_f (bit-exact against tdgl3d above) converges to it at second order.
+    A dropped or mis-signed term leaves an O(1) gap that does not shrink with h."""
+    d = golden(f"evalf_{name}")
+    f = oracle_film(S.EVALF_CASES[name])
+    u = O.boundary_vectors(*d["field"], f.g, f.idx)
+    X = d["X"]
+    n = f.g.n_interior
+    nphi = 3 if f.g.is_3d else 2
+    rng = np.random.default_rng(100 + len(name))
+    P = np.concatenate([rng.standard_normal(n) + 1j * rng.standard_normal(n),
+                        rng.standard_normal(nphi * n).astype(np.complex128)])
+    jp = O.jvp_stencil(X, P, f.g, f.idx, u, f.material)
+    scale = float(np.max(np.abs(jp)))
+
+    def gap(h):
+        fd = (O.eval_f_csr(X + h * P, f.g, f.idx, u, f.material)
+              - O.eval_f_csr(X - h * P, f.g, f.idx, u, f.material)) / (2.0 * h)
+        return float(np.max(np.abs(fd - jp)))
+
+    e1, e2 = gap(1e-3), gap(5e-4)
+    assert e2 <= 1e-6 * scale, (e1, e2, scale)
+    assert 3.5 <= e1 / e2 <= 4.5, (e1, e2)
+    # the hole rows of the link blocks are exactly zero, as eval_f's are
+    for q, ax in enumerate("xyz"):
+        hm = f.idx[f"hole_{ax}_bc_interior"]
+        if hm.size and q < nphi:
+            assert np.all(jp[(q + 1) * n + hm] == 0.0)
+
+
+def test_planted_jvp_mistake_is_caught(monkeypatch):
+    """Dropping the conj(d psi) part of d[(1-|psi|^2) psi] in the oracle's JVP
+    must fail the central-difference pin."""
+    orig = O.jvp_stencil
+    d = golden("evalf_c1_bz")
+    f = oracle_film(S.EVALF_CASES["c1_bz"])
+    u = O.boundary_vectors(*d["field"], f.g, f.idx)
+    n = f.g.n_interior
+    rng = np.random.default_rng(1)
+    X = d["X"]
+    P = np.concatenate([rng.standard_normal(n) + 1j * rng.standard_normal(n),
+                        rng.standard_normal(3 * n).astype(np.complex128)])
+    jp = orig(X, P, f.g, f.idx, u, f.material)
+    c = X[:n]
+    wrong = jp.copy()
+    wrong[:n] += c * c * np.conj(P[:n])  # undo the -psi^2 conj(d psi) term
+    h = 5e-4
+    fd = (O.eval_f_csr(X + h * P, f.g, f.idx, u, f.material)
+          - O.eval_f_csr(X - h * P, f.g, f.idx, u, f.material)) / (2.0 * h)
+    assert np.max(np.abs(fd - jp)) <= 1e-6 * np.max(np.abs(jp))
+    assert np.max(np.abs(fd - wrong)) > 1e-2 * np.max(np.abs(jp))
+
+
 def test_fpsi_closed_form():
     g = O.Grid(6, 6, 1, kappa=2.0)
     idx = O.construct_indices(g)
