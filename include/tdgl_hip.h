@@ -241,6 +241,26 @@ int tdgl_norm_inf(tdgl_ctx* ctx, const double* a_dev, double* out);
 int tdgl_trap_step(tdgl_ctx* ctx, double* X_dev, double dt, const tdgl_solver_opts* opts,
                    tdgl_step_stats* stats);
 
+/* newton_gcr_trap (src/tdgl3d/solvers/newton.py:76-120) on its own, FP64, at
+ * the current field: Newton on g(x) = x - dt/2 f(x) - gamma from x_dev (in:
+ * the start x0; out: the last iterate -- the converged one, the one whose
+ * TGCR failed, or the last of newton_max_iter iterations, as the reference
+ * returns it).  stats->converged / newton_iters are the reference's
+ * (converged, iterations).  opts->precision must be TDGL_PREC_F64. */
+int tdgl_newton_trap(tdgl_ctx* ctx, double* x_dev, const double* gamma_dev, double dt,
+                     const tdgl_solver_opts* opts, tdgl_step_stats* stats);
+
+/* tgcr_matrix_free_trap (src/tdgl3d/solvers/tgcr.py:96-156) on its own, FP64,
+ * at the current field: dx_dev with (I - dt/2 J(x_lin)) dx = b, J p by the
+ * finite difference of eval_f at x_lin (opts->eps_mf), tolerance
+ * opts->tol_gcr, at most opts->tgcr_max_iter iterations (<= 0: the
+ * reference's default).  *converged = 0 is the reference's empty-array return
+ * (dx_dev is then zero); stats->tgcr_iters_last = its iteration count.
+ * opts->precision must be TDGL_PREC_F64. */
+int tdgl_tgcr_trap(tdgl_ctx* ctx, const double* x_lin_dev, const double* b_dev, double dt,
+                   const tdgl_solver_opts* opts, double* dx_dev, int* converged,
+                   tdgl_step_stats* stats);
+
 /* One Forward-Euler step X += dt f(X) (integrators.py:78-83). */
 int tdgl_euler_step(tdgl_ctx* ctx, double* X_dev, double dt);
 

@@ -14,6 +14,16 @@ from .core.state import StateVector
 from .physics.applied_field import AppliedField
 from .solvers.runner import solve
 
+# Analysis tools (tdgl3d re-exports the same six names, src/tdgl3d/__init__.py:17-25)
+from .analysis import (
+    check_steady_state,
+    compute_convergence_metrics,
+    count_hole_flux_quanta,
+    count_vortices_plaquette,
+    count_vortices_polygon,
+    find_vortex_cores,
+)
+
 __version__ = "0.1.0"
 
 __all__ = [
@@ -26,4 +36,11 @@ __all__ = [
     "MaterialMap",
     "AppliedField",
     "solve",
+    # Analysis
+    "check_steady_state",
+    "compute_convergence_metrics",
+    "count_vortices_plaquette",
+    "count_vortices_polygon",
+    "count_hole_flux_quanta",
+    "find_vortex_cores",
 ]

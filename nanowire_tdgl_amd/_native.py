@@ -29,7 +29,7 @@ EXPORTS = [
     "tdgl_set_stencil_kz", "tdgl_get_stencil_kz", "tdgl_set_sweep_tile", "tdgl_get_sweep_tile",
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
-    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_adjoint_solve", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_euler_step",
+    "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_adjoint_solve", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_newton_trap", "tdgl_tgcr_trap", "tdgl_euler_step",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
@@ -124,6 +124,9 @@ def load() -> C.CDLL:
         "tdgl_dot": ([P, P, P, C.POINTER(D)], C.c_int),
         "tdgl_norm_inf": ([P, P, C.POINTER(D)], C.c_int),
         "tdgl_trap_step": ([P, P, D, C.POINTER(SolverOpts), C.POINTER(StepStats)], C.c_int),
+        "tdgl_newton_trap": ([P, P, P, D, C.POINTER(SolverOpts), C.POINTER(StepStats)], C.c_int),
+        "tdgl_tgcr_trap": ([P, P, P, D, C.POINTER(SolverOpts), P, C.POINTER(C.c_int),
+                            C.POINTER(StepStats)], C.c_int),
         "tdgl_euler_step": ([P, P, D], C.c_int),
         "tdgl_observables": ([P, P, P, P], C.c_int),
         "tdgl_profile_enable": ([P, C.c_int], C.c_int),
@@ -256,6 +259,7 @@ class Context:
     def to_reference(self, st):
         """internal FP64 buffer -> complex128 reference-layout torch tensor (device)."""
         t = self.torch
+        self._vecs64("to_reference", st)
         out = t.empty((1 + self.nphi) * self.n, dtype=t.complex128, device=self.device)
         self._call("tdgl_permute_out", C.c_void_p(st.data_ptr()), C.c_void_p(out.data_ptr()))
         return out
@@ -319,7 +323,11 @@ class Context:
         return {"a": info[0], "b": info[1], "degree": int(info[2]), "z_is_v_over_theta": info[3] == 1.0}
 
     def vjp_f(self, x, w, gx, field=False, kappa=False, dsc=None):
-        """gx = J(x)^T w; returns (d/d(Bx,By,Bz) or None, d/dkappa or None)."""
+        """gx = J(x)^T w; returns (d/d(Bx,By,Bz) or None, d/dkappa or None);
+        dsc (optional) receives d<w, F>/d sc per interior node (n values)."""
+        self._vecs64("vjp_f", x, w, gx)
+        if dsc is not None and not _is_vec(dsc, self.torch.float64, self.n):
+            raise ValueError("vjp_f: dsc must be a contiguous float64 device tensor of n_interior values")
         df = (C.c_double * 3)() if field else None
         dk = C.c_double() if kappa else None
         self._call("tdgl_vjp_f", C.c_void_p(x.data_ptr()), C.c_void_p(w.data_ptr()),
@@ -329,6 +337,9 @@ class Context:
 
     def vjp_bz(self, gbz, gx, field=False):
         """gx = (dB_z/dx)^T gbz (gbz in reference order); returns d/d(Bx,By,Bz) or None."""
+        if not _is_vec(gbz, self.torch.float64, self.n):
+            raise ValueError("vjp_bz: gbz must be a contiguous float64 device tensor of n_interior values")
+        self._vecs64("vjp_bz", gx)
         df = (C.c_double * 3)() if field else None
         self._call("tdgl_vjp_bz", C.c_void_p(gbz.data_ptr()), C.c_void_p(gx.data_ptr()),
                    C.cast(df, C.c_void_p) if field else None)
@@ -343,6 +354,7 @@ class Context:
             if not _is_vec(v, self.torch.float64, self.len):
                 raise ValueError("adjoint_solve takes float64 device vectors of the internal state length")
         out = self.torch.empty_like(rhs) if out is None else out
+        self._vecs64("adjoint_solve", out)
         it = C.c_int(0)
         self._call("tdgl_adjoint_solve", C.c_void_p(x1.data_ptr()), C.c_void_p(rhs.data_ptr()),
                    C.c_double(hdt), C.c_double(tol), C.c_int64(int(max_iter)),
@@ -350,11 +362,13 @@ class Context:
         return out, int(it.value)
 
     def dot(self, a, b) -> float:
+        self._vecs64("dot", a, b)
         out = C.c_double()
         self._call("tdgl_dot", C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.byref(out))
         return out.value
 
     def norm_inf(self, a) -> float:
+        self._vecs64("norm_inf", a)
         out = C.c_double()
         self._call("tdgl_norm_inf", C.c_void_p(a.data_ptr()), C.byref(out))
         return out.value
@@ -371,6 +385,25 @@ class Context:
                    C.byref(st))
         del keep
         return st
+
+    def newton_trap(self, x, gamma, dt: float, opts: SolverOpts) -> StepStats:
+        """newton_gcr_trap on the device: x (in: x0, out: the last iterate)."""
+        self._vecs64("newton_trap", x, gamma)
+        st = StepStats()
+        self._call("tdgl_newton_trap", C.c_void_p(x.data_ptr()), C.c_void_p(gamma.data_ptr()),
+                   C.c_double(dt), C.byref(opts), C.byref(st))
+        return st
+
+    def tgcr_trap(self, x_lin, b, dt: float, opts: SolverOpts, dx) -> tuple:
+        """tgcr_matrix_free_trap on the device: dx (written); returns
+        (converged, stats)."""
+        self._vecs64("tgcr_trap", x_lin, b, dx)
+        st = StepStats()
+        ok = C.c_int(0)
+        self._call("tdgl_tgcr_trap", C.c_void_p(x_lin.data_ptr()), C.c_void_p(b.data_ptr()),
+                   C.c_double(dt), C.byref(opts), C.c_void_p(dx.data_ptr()), C.byref(ok),
+                   C.byref(st))
+        return bool(ok.value), st
 
     def euler_step(self, X, dt: float) -> None:
         self._vecs64("euler_step", X)

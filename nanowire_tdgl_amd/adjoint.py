@@ -89,6 +89,8 @@ class AdjointProblem:
         self.t_stop = self.steps * self.dt
         self._perm = _internal_to_reference(self.params)
         self.stats: dict = {}
+        # forward runs so far: backward() needs the checkpoints of the run it differentiates
+        self.generation = 0
 
     # -- parameters -----------------------------------------------------------
     def set_parameters(self, kappa: float, w_full: np.ndarray) -> None:
@@ -120,6 +122,8 @@ class AdjointProblem:
     def forward(self, kappa: float, bz: float, w_full: np.ndarray):
         """Run the steps; returns (final internal state, B_z observable (reference order))."""
         import torch
+        # a new run: outputs of earlier runs can no longer be differentiated
+        self.generation += 1
         self.set_parameters(kappa, w_full)
         self._opts = _native.solver_opts(self.newton_tol_f, self.newton_tol_dx, 20, self.tol_gcr)
         if self.predictor not in ("euler", "extrapolate"):
@@ -224,15 +228,24 @@ def _fn():
         def forward(ctx, kappa, bz, w, problem):
             ctx.problem = problem
             _, bzo = problem.forward(float(kappa), float(bz), w.detach().cpu().numpy())
-            ctx.dev = bzo.device
+            ctx.generation = problem.generation
+            # each gradient goes back on its input's device and dtype
+            ctx.like = [(v.device, v.dtype) if torch.is_tensor(v) else None for v in (kappa, bz, w)]
             return bzo
 
         @staticmethod
         def backward(ctx, g):
             p = ctx.problem
-            dk, db, dw = p.backward(g.detach().to(torch.float64))
-            t = lambda v: torch.as_tensor(v, dtype=torch.float64)  # noqa: E731
-            return t(dk), t(db), torch.as_tensor(dw, dtype=torch.float64), None
+            if p.generation != ctx.generation:
+                raise RuntimeError(
+                    "the AdjointProblem ran forward again after this output was computed; its "
+                    "checkpoints now belong to the later run: call backward before the next forward")
+            dk, db, dw = p.backward(g.detach().to(torch.float64).contiguous())
+            out = []
+            for v, like in zip((dk, db, dw), ctx.like):
+                out.append(None if like is None else
+                           torch.as_tensor(v, dtype=torch.float64).to(device=like[0], dtype=like[1]))
+            return out[0], out[1], out[2], None
 
     _FN = _TrapBz
     return _FN

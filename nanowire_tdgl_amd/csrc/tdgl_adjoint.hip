@@ -416,24 +416,39 @@ __global__ __launch_bounds__(kRedThreads) void k_field_grad(Geo g, const uint8_t
 
 // Cell adjoint of B_z = (((x - x[+y]) - y) + y[+x]) / (hx hy) at every
 // interior node (physics/bfield.py:58-68), weights in the reference order.
-// Once per backward pass: neighbouring nodes share cells, so the four
-// contributions are FP64 atomics.
+// Gathered per full-grid cell (no atomics, fixed order): the x link of a cell
+// takes +v from its own node and -v from the node below it in y; the y link
+// takes -v from its own node and +v from the node before it in x.  Every cell
+// and component is written.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_bz_cells(Geo g, Bnd b, const double* gbz, double* A) {
-  const long long n = g.n;
+  const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
   const double s = 1.0 / (b.hx * b.hy);
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    int i, j, k;
-    node_of<IS3D>(g, q, i, j, k);
-    // reference order r = ((i-1) ny + (j-1)) nz + (k-1)
-    const long long r = ((long long)(i - 1) * g.ny + (j - 1)) * g.nz + (IS3D ? k - 1 : 0);
-    const double v = s * gbz[r];
-    if (v == 0.0) continue;
-    atomicAdd(A + mfull(g, i, j, k) * 5 + LX, v);
-    atomicAdd(A + mfull(g, i, j + 1, k) * 5 + LX, -v);
-    atomicAdd(A + mfull(g, i, j, k) * 5 + LY, -v);
-    atomicAdd(A + mfull(g, i + 1, j, k) * 5 + LY, v);
+  // s * gbz at interior node (i, j, k), reference order r = ((i-1) ny + (j-1)) nz + (k-1)
+  auto v = [&](int i, int j, int k, double& out) {
+    if (i < 1 || i > g.Nx - 1 || j < 1 || j > g.Ny - 1) return false;
+    if (IS3D && (k < 1 || k > g.Nz - 1)) return false;
+    out = s * gbz[((long long)(i - 1) * g.ny + (j - 1)) * g.nz + (IS3D ? k - 1 : 0)];
+    return true;
+  };
+  for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
+       c += (long long)gridDim.x * kRedThreads) {
+    const int ci = (int)(c % g.mj);
+    const long long t = c / g.mj;
+    const int cj = (int)(t % (g.Ny + 1));
+    const int ck = IS3D ? (int)(t / (g.Ny + 1)) : 0;
+    double ax = 0.0, ay = 0.0, w;
+    if (v(ci, cj, ck, w)) {
+      ax += w;
+      ay -= w;
+    }
+    if (v(ci, cj - 1, ck, w)) ax -= w;
+    if (v(ci - 1, cj, ck, w)) ay += w;
+    A[c * 5 + CR] = 0.0;
+    A[c * 5 + CI] = 0.0;
+    A[c * 5 + LX] = ax;
+    A[c * 5 + LY] = ay;
+    A[c * 5 + LZ] = 0.0;
   }
 }
 
@@ -523,9 +538,11 @@ int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, doubl
   return stream_blocks(g);
 }
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st) {
-  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) launch(k_bz_cells<true>, gr, bl, 0, st, g, b, gbz, A);
-  else launch(k_bz_cells<false>, gr, bl, 0, st, g, b, gbz, A);
+  const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
+  long long nb = (dimx + kRedThreads - 1) / kRedThreads;
+  if (nb > 4096) nb = 4096;
+  if (g.is3d) launch(k_bz_cells<true>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, b, gbz, A);
+  else launch(k_bz_cells<false>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, b, gbz, A);
 }
 
 }  // namespace tdgl

@@ -483,15 +483,15 @@ int tdgl_precond_cheb32(tdgl_ctx* ctx, const float* x32, const float* v32, doubl
 
 namespace {
 
-// The cell adjoint, zeroed (clear) or to be overwritten by launch_cells_gather.
-int ensure_acell(tdgl_ctx* ctx, bool clear = true) {
+// The cell adjoint; every kernel that fills it (k_cells_gather, k_bz_cells)
+// writes every cell and component.
+int ensure_acell(tdgl_ctx* ctx) {
   const Geo& g = ctx->g;
   const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
   if (!ctx->d_acell) {
     int rc;
     if ((rc = alloc(ctx, (void**)&ctx->d_acell, sizeof(double) * 5 * (size_t)dimx))) return rc;
   }
-  if (clear) HIPCHK(hipMemsetAsync(ctx->d_acell, 0, sizeof(double) * 5 * (size_t)dimx, ctx->st));
   return TDGL_OK;
 }
 // The per-node scatter planes of the VJP: zeroed at allocation, and cleared
@@ -530,7 +530,7 @@ int tdgl_vjp_f(tdgl_ctx* ctx, const double* x, const double* w, double* gx, doub
                double* dkappa, double* dsc) {
   if (!ctx || !x || !w || !gx) return TDGL_E_INVALID;
   int rc;
-  if ((rc = ensure_acell(ctx, false)) || (rc = ensure_vjpb(ctx))) return rc;
+  if ((rc = ensure_acell(ctx)) || (rc = ensure_vjpb(ctx))) return rc;
   Red r = red_next(ctx);
   launch_vjp_cells(ctx->g, ctx->b, ctx->mt, x, w, ctx->d_vjpb, dsc, ctx->kappa_uniform,
                    dkappa ? r.pout : nullptr, ctx->st);
@@ -793,6 +793,41 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
   return TDGL_OK;
 }
 
+// The Newton loop of newton_gcr_trap (newton.py:94-120) on w_x (the start,
+// then each iterate) and w_gam.  On return w_x holds the last iterate:
+// the converged one, the one whose TGCR failed (newton.py:102-106), or the
+// last after max_iter iterations.
+int newton_solve(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, tdgl_step_stats* s) {
+  int rc;
+  if ((rc = newton_eval(ctx, hdt, s))) return rc;
+  if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
+  for (int it = 1; it <= o->newton_max_iter; ++it) {
+    const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
+    int ok = 0;
+    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
+    if (!ok) {  // newton.py:102-106
+      s->tgcr_failed = 1;
+      s->newton_iters = it;
+      s->converged = 0;
+      return TDGL_OK;
+    }
+    if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
+    if ((rc = readback(ctx, RS_XX, 4))) return rc;  // xx, rr, ginf, dxinf
+    s->g_inf = ctx->h_res[RS_GINF];
+    s->dx_inf = ctx->h_res[RS_DXINF];
+    if (!std::isfinite(s->g_inf) || !std::isfinite(s->dx_inf))
+      return fail(ctx, TDGL_E_NONFINITE, "Newton residual is not finite");
+    if (s->g_inf <= o->newton_tol_f && s->dx_inf <= o->newton_tol_dx) {  // newton.py:118
+      s->converged = 1;
+      s->newton_iters = it;
+      return TDGL_OK;
+    }
+  }
+  s->converged = 0;
+  s->newton_iters = o->newton_max_iter;
+  return TDGL_OK;
+}
+
 }  // namespace
 
 extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opts* o,
@@ -828,34 +863,67 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
     LAUNCHCHK();
     s->bytes_moved += 3.0 * S;
   }
-  if ((rc = newton_eval(ctx, hdt, s))) return rc;
-  if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
-  for (int it = 1; it <= o->newton_max_iter; ++it) {
-    const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
-    int ok = 0;
-    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
-    if (!ok) {  // newton.py:102-106
-      s->tgcr_failed = 1;
-      s->newton_iters = it;
-      s->converged = 0;
-      return TDGL_OK;
-    }
-    if ((rc = newton_eval(ctx, hdt, s))) return rc;  // newton.py:109-113
-    if ((rc = readback(ctx, RS_XX, 4))) return rc;  // xx, rr, ginf, dxinf
-    s->g_inf = ctx->h_res[RS_GINF];
-    s->dx_inf = ctx->h_res[RS_DXINF];
-    if (!std::isfinite(s->g_inf) || !std::isfinite(s->dx_inf))
-      return fail(ctx, TDGL_E_NONFINITE, "Newton residual is not finite");
-    if (s->g_inf <= o->newton_tol_f && s->dx_inf <= o->newton_tol_dx) {  // newton.py:118
-      HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
-      s->bytes_moved += 2.0 * S;
-      s->converged = 1;
-      s->newton_iters = it;
-      return TDGL_OK;
-    }
+  if ((rc = newton_solve(ctx, hdt, o, s))) return rc;
+  if (s->converged) {
+    HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
+    s->bytes_moved += 2.0 * S;
   }
-  s->converged = 0;
-  s->newton_iters = o->newton_max_iter;
+  return TDGL_OK;
+}
+
+// newton_gcr_trap (newton.py:76-120) as an entry point of its own: Newton from
+// x (in: x0, out: the last iterate, also when Newton fails) on
+// g(x) = x - dt/2 f(x) - gamma.
+extern "C" int tdgl_newton_trap(tdgl_ctx* ctx, double* x, const double* gamma, double dt,
+                                const tdgl_solver_opts* o, tdgl_step_stats* s) {
+  if (!ctx || !x || !gamma || !o || !s) return TDGL_E_INVALID;
+  if (o->precision != TDGL_PREC_F64)
+    return fail(ctx, TDGL_E_INVALID, "tdgl_newton_trap runs the plain FP64 method only");
+  memset(s, 0, sizeof(*s));
+  int rc;
+  if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0))) return rc;
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  HIPCHK(hipMemcpyAsync(ctx->w_x, x, bytes, hipMemcpyDeviceToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->w_gam, gamma, bytes, hipMemcpyDeviceToDevice, ctx->st));
+  if ((rc = newton_solve(ctx, dt / 2.0, o, s))) return rc;
+  HIPCHK(hipMemcpyAsync(x, ctx->w_x, bytes, hipMemcpyDeviceToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return TDGL_OK;
+}
+
+// tgcr_matrix_free_trap (tgcr.py:96-156) as an entry point of its own: dx with
+// (I - dt/2 J(x_lin)) dx = b, J by the finite difference of f at x_lin.
+extern "C" int tdgl_tgcr_trap(tdgl_ctx* ctx, const double* x_lin, const double* b, double dt,
+                              const tdgl_solver_opts* o, double* dx, int* converged,
+                              tdgl_step_stats* s) {
+  if (!ctx || !x_lin || !b || !o || !dx || !converged || !s) return TDGL_E_INVALID;
+  if (o->precision != TDGL_PREC_F64)
+    return fail(ctx, TDGL_E_INVALID, "tdgl_tgcr_trap runs the plain FP64 method only");
+  memset(s, 0, sizeof(*s));
+  int rc;
+  if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0))) return rc;
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  HIPCHK(hipMemcpyAsync(ctx->w_x, x_lin, bytes, hipMemcpyDeviceToDevice, ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->w_r, b, bytes, hipMemcpyDeviceToDevice, ctx->st));
+  HIPCHK(hipMemsetAsync(dx, 0, bytes, ctx->st));  // x = 0 (tgcr.py:114)
+  Epi e{};
+  e.out0 = ctx->w_f;  // f(x_lin), the base of the finite difference
+  launch_stencil(M_RHS, false, ctx->g, ctx->b, ctx->mt, ctx->w_x, nullptr, 0.0, e, ctx->cfg,
+                 red_next(ctx), ctx->st);
+  s->rhs_evals++;
+  red_commit(ctx, launch_dot(ctx->g, ctx->w_x, ctx->w_x, red_next(ctx), ctx->st));
+  red_finalize(ctx, 1, 0, RS_XX);
+  red_commit(ctx, launch_dot(ctx->g, ctx->w_r, ctx->w_r, red_next(ctx), ctx->st));
+  red_finalize(ctx, 1, 0, RS_RR);  // the JVP forms eps from res[RS_RR]
+  LAUNCHCHK();
+  if ((rc = readback(ctx, RS_XX, 2))) return rc;
+  TgcrOp op;
+  op.acc_out = dx;
+  int ok = 0;
+  if ((rc = tgcr(ctx, dt / 2.0, o, ctx->h_res[RS_RR], ctx->h_res[RS_XX], &ok, s, op))) return rc;
+  s->tgcr_failed = !ok;
+  *converged = ok;
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return TDGL_OK;
 }
 
@@ -886,7 +954,7 @@ int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1, const double* rhs, doubl
       precond > kMaxChebDeg)
     return TDGL_E_INVALID;
   int rc;
-  if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0)) || (rc = ensure_acell(ctx, false)) ||
+  if ((rc = ensure_work(ctx)) || (rc = ensure_vec(ctx, 0)) || (rc = ensure_acell(ctx)) ||
       (rc = ensure_vjpb(ctx)))
     return rc;
   const Geo& g = ctx->g;

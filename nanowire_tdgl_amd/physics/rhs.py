@@ -52,6 +52,32 @@ def eval_f(X, params: SimulationParameters, idx: GridIndices, u: BoundaryVectors
     return ctx.to_reference(f).cpu().numpy()
 
 
+class DeviceRHS:
+    """The callable f(X) -> dX/dt that tdgl3d's Newton and TGCR consume
+    (integrators.py:23-32), bound to one film and one set of boundary vectors.
+    Called with a numpy state it returns eval_f's numpy result; the device
+    drop-ins of ``newton_gcr_trap`` and ``tgcr_matrix_free_trap`` recognise it
+    and run their whole solve in the library instead of calling it."""
+
+    __slots__ = ("params", "idx", "u", "material")
+
+    def __init__(self, params: SimulationParameters, idx: GridIndices, u: BoundaryVectors,
+                 material: Optional[MaterialMap] = None):
+        self.params = params
+        self.idx = idx
+        self.u = u
+        self.material = material
+
+    def __call__(self, X) -> np.ndarray:
+        return eval_f(X, self.params, self.idx, self.u, self.material)
+
+    def bind(self):
+        """The film's device context with this closure's field loaded."""
+        eng = engine_for(self.params, self.idx, self.material)
+        eng.set_boundary(self.u)
+        return eng.ctx
+
+
 def device_observables(X, params, idx, u, material=None):
     """B = curl(phi) with the BCs applied and J_s without, at the interior
     nodes (the observables of core/solution.py:94-217), from the device."""

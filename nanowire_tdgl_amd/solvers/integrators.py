@@ -18,6 +18,15 @@ from ..core.material import MaterialMap
 from ..core.parameters import SimulationParameters
 from ..engine import engine_for
 from ..mesh.indices import GridIndices
+from ..physics.rhs import DeviceRHS
+
+
+def _make_eval_f(params: SimulationParameters, idx: GridIndices, u,
+                 material: Optional[MaterialMap] = None) -> DeviceRHS:
+    """The closure f(X) -> dX/dt for the current boundary vectors
+    (integrators.py:23-32); newton_gcr_trap and tgcr_matrix_free_trap run it
+    on the device."""
+    return DeviceRHS(params, idx, u, material)
 
 
 def _bar(total, desc, enabled, unit="it"):

@@ -237,3 +237,42 @@ def test_trajectory_gradient_matches_finite_differences(predictor):
         assert abs(gw[m] - fd) <= 1e-5 * scale
     ghost = np.setdiff1d(np.arange(prob.params.dim_x), full)
     assert np.all(gw[ghost] == 0.0)
+
+
+def test_backward_is_deterministic_and_guards_stale_forwards():
+    """Two backward passes of the same forward give the same bits for
+    dL/d(kappa, Bz, w): the VJP and the B_z cell adjoint gather in a fixed
+    order (no atomics).  Gradients come back on their inputs' devices and
+    dtypes, and a second forward before backward makes the first output's
+    backward raise (its checkpoints belong to the later run)."""
+    import torch
+    from nanowire_tdgl_amd.adjoint import TrapezoidalBz, midplane_target_loss
+    prob, w0 = _problem(steps=4, every=2)
+    grads = []
+    for _ in range(2):
+        kappa = torch.tensor(2.5, dtype=torch.float64, device="cuda", requires_grad=True)
+        bz = torch.tensor(0.3, dtype=torch.float64, device="cuda", requires_grad=True)
+        w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
+        loss = midplane_target_loss(prob, TrapezoidalBz.apply(prob, kappa, bz, w), width=3.0)
+        loss.backward()
+        assert kappa.grad.device.type == "cuda" and bz.grad.device.type == "cuda"
+        assert w.grad.device.type == "cpu" and w.grad.dtype == torch.float64
+        grads.append((kappa.grad.item(), bz.grad.item(), w.grad.numpy().copy()))
+    assert grads[0][0] == grads[1][0] and grads[0][1] == grads[1][1]
+    assert np.array_equal(grads[0][2], grads[1][2])
+    # the B_z adjoint alone, twice: identical bits
+    ctx = prob.ctx
+    gb = torch.as_tensor(np.random.default_rng(6).standard_normal(prob.params.n_interior),
+                         device=ctx.device)
+    a, b = ctx.empty_state(), ctx.empty_state()
+    fa = ctx.vjp_bz(gb, a, field=True)
+    fb = ctx.vjp_bz(gb, b, field=True)
+    assert torch.equal(a, b) and fa == fb
+    # stale forward
+    kappa = torch.tensor(2.5, dtype=torch.float64, requires_grad=True)
+    bz = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
+    out1 = TrapezoidalBz.apply(prob, kappa, bz, w)
+    TrapezoidalBz.apply(prob, kappa, bz, w)
+    with pytest.raises(RuntimeError, match="forward again"):
+        midplane_target_loss(prob, out1, width=3.0).backward()

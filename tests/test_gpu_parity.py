@@ -81,7 +81,22 @@ def test_host_tensors_rejected():
                  lambda: ctx.euler_step(host, 0.1),
                  lambda: ctx.observables(host),
                  lambda: ctx.rhs(X, host),
-                 lambda: ctx.jvp_trap32(X.float(), host.float(), 0.1, X.float())):
+                 lambda: ctx.jvp_trap32(X.float(), host.float(), 0.1, X.float()),
+                 lambda: ctx.to_reference(host),
+                 lambda: ctx.to_reference(X[:-1]),
+                 lambda: ctx.vjp_f(X, host, torch.empty_like(X)),
+                 lambda: ctx.vjp_f(X, X, torch.empty_like(X), dsc=torch.empty(3, dtype=X.dtype,
+                                                                                device=X.device)),
+                 lambda: ctx.vjp_f(X, X, torch.empty_like(X), dsc=torch.empty(ctx.n, dtype=X.dtype)),
+                 lambda: ctx.vjp_bz(torch.zeros(ctx.n - 1, dtype=X.dtype, device=X.device),
+                                    torch.empty_like(X)),
+                 lambda: ctx.vjp_bz(torch.zeros(ctx.n, dtype=X.dtype), torch.empty_like(X)),
+                 lambda: ctx.vjp_bz(torch.zeros(ctx.n, dtype=X.dtype, device=X.device), host),
+                 lambda: ctx.dot(X, host),
+                 lambda: ctx.dot(X[:-1], X[:-1]),
+                 lambda: ctx.norm_inf(host),
+                 lambda: ctx.adjoint_solve(X, X, 0.1, 1e-8, out=host),
+                 lambda: ctx.adjoint_solve(X, X, 0.1, 1e-8, out=X[:-1])):
         with pytest.raises(ValueError):
             call()
     assert torch.isfinite(X).all()
@@ -376,3 +391,49 @@ def test_film128_hole_step_invariants():
     # the psi relaxation inside the hole drives psi down
     sc = dev.material.interior_sc_mask
     assert np.abs(Xn[:n][sc == 0]).max() < np.abs(X[:n][sc == 0]).min()
+
+
+def test_newton_and_tgcr_drop_ins_match_reference():
+    """newton_gcr_trap and tgcr_matrix_free_trap with tdgl3d's signatures, fed
+    the closure of integrators._make_eval_f, run on the device and reproduce
+    tdgl3d's own outputs (golden tgcr_trap / newton_trap on the 16x16x8 film,
+    Bz = 0.3): dx and x to 1e-12, the iteration count and the converged flag.
+    Any other callable is refused; max_iter = 0 gives the empty array."""
+    T, engine_for, rhs, bbfv, integ = _lib()
+    from nanowire_tdgl_amd.solvers.newton import newton_gcr_trap
+    from nanowire_tdgl_amd.solvers.tgcr import tgcr_matrix_free_trap
+    dev = device(S.EVALF_CASES["c1_bz"])
+    u = rhs.BoundaryVectors.from_field(bbfv(0.0, 0.0, 0.3, dev.params, dev.idx))
+    f = integ._make_eval_f(dev.params, dev.idx, u)
+    d = golden("tgcr_trap")
+    dx = tgcr_matrix_free_trap(f, d["x_lin"], d["b"], float(d["dt"]), tol=1e-8, eps_mf=1e-4)
+    assert dx.dtype == np.complex128 and dx.shape == d["dx"].shape
+    assert rel_err(dx, d["dx"]) < 1e-12, rel_err(dx, d["dx"])
+    ctx = f.bind()
+    from nanowire_tdgl_amd import _native
+    ok, st = ctx.tgcr_trap(ctx.to_internal(d["x_lin"]), ctx.to_internal(d["b"]), float(d["dt"]),
+                           _native.solver_opts(tol_gcr=1e-8), ctx.empty_state())
+    assert ok and st.tgcr_iters_last == int(d["n_iter"])
+    assert tgcr_matrix_free_trap(f, d["x_lin"], d["b"], float(d["dt"]), max_iter=0).size == 0
+    assert tgcr_matrix_free_trap(f, d["x_lin"], d["b"], float(d["dt"]), tol=1e-8, max_iter=2).size == 0
+    z = tgcr_matrix_free_trap(f, d["x_lin"], np.zeros_like(d["b"]), float(d["dt"]))
+    assert z.shape == d["b"].shape and not np.any(z)
+    d = golden("newton_trap")
+    x, conv, it = newton_gcr_trap(f, d["x0"], d["gamma"], float(d["dt"]), tol_f=1e-3, tol_dx=1e-3,
+                                  max_iter=20, tol_gcr=1e-8, eps_mf=1e-4)
+    assert conv == bool(d["converged"]) and it == int(d["iters"])
+    assert rel_err(x, d["x"]) < 1e-12, rel_err(x, d["x"])
+    # one Newton iteration is not enough: the reference returns the iterate after it
+    x1, conv1, it1 = newton_gcr_trap(f, d["x0"], d["gamma"], float(d["dt"]), max_iter=1,
+                                     tol_gcr=1e-8)
+    of = oracle_film(S.EVALF_CASES["c1_bz"])
+    uo = O.boundary_vectors(0.0, 0.0, 0.3, of.g, of.idx)
+    want = O.newton_trap(lambda Y: O.eval_f_csr(Y, of.g, of.idx, uo), d["x0"], d["gamma"],
+                         float(d["dt"]), 1e-3, 1e-3, 1, 1e-8, 1e-4)
+    assert (conv1, it1) == (want[1], want[2]) == (False, 1)
+    assert rel_err(x1, want[0]) < 1e-12
+    for bad in (lambda Y: f(Y), O.eval_f_csr):
+        with pytest.raises(TypeError):
+            newton_gcr_trap(bad, d["x0"], d["gamma"], 0.1)
+        with pytest.raises(TypeError):
+            tgcr_matrix_free_trap(bad, d["x0"], d["gamma"], 0.1)
