@@ -264,6 +264,16 @@ int tdgl_tgcr_trap(tdgl_ctx* ctx, const double* x_lin_dev, const double* b_dev, 
 /* One Forward-Euler step X += dt f(X) (integrators.py:78-83). */
 int tdgl_euler_step(tdgl_ctx* ctx, double* X_dev, double dt);
 
+/* tdgl_euler_step without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  X + dt f(X) is staged in a file-scope device
+ * array of tdgl_noalloc_max_len() doubles per GPU (32 MiB; its address is
+ * resolved when the first context on that GPU is created) instead of a work
+ * buffer the library allocates.  TDGL_E_INVALID when the state is longer.
+ * Every call on one GPU shares that array: calls that overlap on a GPU race. */
+int tdgl_euler_step_noalloc(tdgl_ctx* ctx, double* X_dev, double dt);
+int64_t tdgl_noalloc_max_len(void);
+
 /* Check-path observables on an internal-layout state, written in the
  * reference interior order (3 x n doubles each):
  *   B  = curl(phi) on the expanded state with BCs at the current field

@@ -30,6 +30,7 @@ EXPORTS = [
     "tdgl_state_len", "tdgl_n_interior", "tdgl_set_material", "tdgl_set_field",
     "tdgl_set_field_vectors", "tdgl_permute_in", "tdgl_permute_out", "tdgl_rhs",
     "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_adjoint_solve", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_newton_trap", "tdgl_tgcr_trap", "tdgl_euler_step",
+    "tdgl_euler_step_noalloc", "tdgl_noalloc_max_len",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
@@ -128,6 +129,8 @@ def load() -> C.CDLL:
         "tdgl_tgcr_trap": ([P, P, P, D, C.POINTER(SolverOpts), P, C.POINTER(C.c_int),
                             C.POINTER(StepStats)], C.c_int),
         "tdgl_euler_step": ([P, P, D], C.c_int),
+        "tdgl_euler_step_noalloc": ([P, P, D], C.c_int),
+        "tdgl_noalloc_max_len": ([], I64),
         "tdgl_observables": ([P, P, P, P], C.c_int),
         "tdgl_profile_enable": ([P, C.c_int], C.c_int),
         "tdgl_profile_read": ([P, C.c_int, C.POINTER(D), C.POINTER(I64), C.POINTER(D)], C.c_int),
@@ -408,6 +411,16 @@ class Context:
     def euler_step(self, X, dt: float) -> None:
         self._vecs64("euler_step", X)
         self._call("tdgl_euler_step", C.c_void_p(X.data_ptr()), C.c_double(dt))
+
+    def euler_step_noalloc(self, X, dt: float) -> None:
+        """euler_step staged in the library's per-GPU scratch array instead of
+        a work buffer: allocates nothing (for graph capture and tight loops).
+        States up to noalloc_max_len() doubles; overlapping calls on one GPU race."""
+        self._vecs64("euler_step_noalloc", X)
+        self._call("tdgl_euler_step_noalloc", C.c_void_p(X.data_ptr()), C.c_double(dt))
+
+    def noalloc_max_len(self) -> int:
+        return int(self.lib.tdgl_noalloc_max_len())
 
     def profile_enable(self, on: bool = True) -> None:
         self._call("tdgl_profile_enable", C.c_int(1 if on else 0))

@@ -20,7 +20,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libtdglhip.so"
 
 # FP64 parity sources: no FMA contraction, so products and sums round like numpy.
-SOURCES_EXACT = ["tdgl_f64.hip", "tdgl_api.hip", "tdgl_batch.hip", "tdgl_adjoint.hip"]
+SOURCES_EXACT = ["tdgl_f64.hip", "tdgl_api.hip", "tdgl_batch.hip", "tdgl_adjoint.hip",
+                 "tdgl_noalloc.hip"]
 # Mixed-precision sources (FP32 stencil / Krylov): contraction allowed.
 SOURCES_FAST: list[str] = ["tdgl_f32.hip"]
 

@@ -160,6 +160,13 @@ int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out) {
   ctx = new tdgl_ctx();
   ctx->grid = *grid;
   HIPCHK(hipGetDevice(&ctx->device));
+  {
+    const int rc0 = noalloc_init_device(ctx->device, &ctx->err);
+    if (rc0 != TDGL_OK) {
+      delete ctx;
+      return rc0;
+    }
+  }
   Geo& g = ctx->g;
   g.Nx = grid->Nx;
   g.Ny = grid->Ny;

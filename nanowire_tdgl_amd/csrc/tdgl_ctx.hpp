@@ -167,4 +167,6 @@ int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, f
 double cheb_upper_bound(const tdgl_ctx* c, double hdt);
 int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt, int degree,
                    float* w32, float* z32, double* info);
+// Resolve the no-alloc scratch array's address on `device` (tdgl_noalloc.hip).
+int noalloc_init_device(int device, std::string* err);
 }  // namespace tdgl
