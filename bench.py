@@ -1,32 +1,40 @@
 #!/usr/bin/env python3
 """Benchmark: trapezoidal TDGL steps on one MI355X.
 
-Headline workload (the plain FP64 production film): 128x128x16 cells, h = 0.5,
-kappa = 4.0, Bz = 0.4 applied through the boundary links (AppliedField
-default on-window, t_stop = 1000 dt), a centred 32x32-cell square hole through
-all z (tau = 0.1 relaxation in the hole), psi = 1 + 1e-3 complex Gaussian
-noise (seed 1), phi = 0; dt = 0.2, TGCR tolerance 1e-8, Newton defaults.
+Headline workload (the batched mixed-precision step): 64 uneven films from
+seed 3 -- Nx, Ny uniform in [32, 160], Nz in [4, 24], h = 0.5, kappa ~ U[1, 5],
+Bz ~ U[0.05, 0.8], a centred rectangular hole on the even films, psi = 1 +
+1e-3 noise, phi = 0 -- advanced in lockstep attempts (one native call per
+attempt for all films), dt = 0.2 with the integrator's per-film dt/10 retry,
+TGCR tolerance 1e-4, Newton's defaults, the Chebyshev-preconditioned mixed
+TGCR (FP32 JVP and Krylov vectors; FP64 Newton residual, Gram coefficients and
+state update), tiles picked by the autotuner.  A step (`--steps`) is one
+lockstep attempt over the batch; `value` is cell-steps/s: interior nodes of
+the films whose attempt was accepted, summed over the timed attempts, over
+the wall time, states resident in HBM.
 
-A step is one accepted trapezoidal step (predictor + Newton + TGCR, with the
-reference's dt/10 retries counted in the time).  `value` is cell-steps/s:
-interior nodes x accepted steps / wall time, state resident in HBM.
-
-One JSON line on stdout.  `roofline` is the dominant kernel class, measured
-with HIP events around every launch on the library's stream over a second,
-instrumented pass of the same K steps.  `cpu_baseline` is the CPU oracle
-(faithful FP64 restatement of tdgl3d) timed on a bounded sample on this host.
+One JSON line on stdout.  `roofline` is the dominant kernel class (the
+Chebyshev sweeps), measured with HIP events on each launch over a second,
+instrumented pass of the same attempts.  The line also carries the same
+attempts looped over the plain FP64 step, and the per-film error of |psi|^2,
+B_z and J_s against the oracle of this comparison: the plain FP64 GPU step
+with Newton converged (||dx||_inf <= 1e-5; the solution of the same step
+equations), with the FP64 loop at the same tolerances beside it.
+`cpu_baseline` is the CPU oracle (faithful FP64 restatement of tdgl3d) timed
+on whole steps of the batch's smallest films on this host, one thread and all
+cores.
 
 Secondary sections of the same line (each with its own roofline):
-  mixed   -- the same film and steps in mixed precision; ratio and error of
-             |psi|^2, B_z, J_s against the FP64 run;
-  large   -- 256x256x64 film with 1% insulating spheres (kappa 3, Bz 0.5,
-             TGCR tol 1e-4): FP64 vs mixed steps and their error;
-  batched -- 64 uneven films (seed 3) in lockstep mixed-precision attempts,
-             against looping the plain FP64 step over the same films;
-  adjoint -- 96x96x12 film (kappa 2.5, Bz 0.3, mask w = 0.05 U(0,1) seed 4):
-             forward FP64 steps with checkpoints, the hand-written adjoint of
-             L = mean_midplane (B_z - T)^2, seconds per step of each, and
-             central differences (eps = 1e-6) for kappa and Bz.
+  fp64_plain -- the 128x128x16 production film with a 32x32 hole, plain FP64
+                (the parity path), TGCR tol 1e-8;
+  mixed      -- the same film and steps in mixed precision;
+  large      -- 256x256x64 film with 1% insulating spheres (kappa 3, Bz 0.5,
+                TGCR tol 1e-4): FP64 vs mixed steps and their error;
+  adjoint    -- 96x96x12 film (kappa 2.5, Bz 0.3, mask w = 0.05 U(0,1) seed
+                4): forward FP64 steps with checkpoints every 20, the
+                hand-written adjoint of L = mean_midplane (B_z - T)^2, seconds
+                per step of each, and central differences (eps = 1e-6) for
+                kappa, Bz and random mask cells.
 """
 
 from __future__ import annotations
@@ -285,181 +293,221 @@ def section_large(steps, warmup):
             "stencil_sweeps_per_step_mixed": per_step(b, "jvp_evals", steps)}
 
 
-def section_batched(attempts, warmup, fp64_steps):
+BATCH_DT = 0.2
+BATCH_T_STOP = 300 * BATCH_DT  # the configured 300 steps: the field's on-window ends at 2/3 of it
+BATCH_TOL = 1e-4
+
+
+def _quantiles(vals) -> dict:
+    v = np.asarray(vals, dtype=np.float64)
+    return {"median": float(np.median(v)), "p90": float(np.quantile(v, 0.9)), "max": float(v.max())}
+
+
+class BatchAttempts:
+    """Lockstep attempts over the batch with the integrator's per-film control
+    flow (integrators.py:150-197): the field at the film's own time, dt/10 on
+    a failed attempt, dt restored after an accepted one."""
+
+    def __init__(self, runner, films, opts, predictor="euler"):
+        self.runner, self.opts = runner, opts
+        self.F = len(films)
+        self.ns = np.array([f.device.params.n_interior for f in films], dtype=np.float64)
+        self.X = runner.pack([f.x0 for f in films])
+        self.t = np.zeros(self.F)
+        self.cur = np.full(self.F, BATCH_DT)
+        self.extrapolate = predictor == "extrapolate"
+        self.x_prev, self.dt_prev = None, np.zeros(self.F)
+        self.newton = 0
+
+    def snapshot(self):
+        xp = None if self.x_prev is None else self.x_prev.clone()
+        return (self.X.clone(), self.t.copy(), self.cur.copy(), xp, self.dt_prev.copy())
+
+    def restore(self, snap) -> None:
+        X, self.t, self.cur, xp, self.dt_prev = snap
+        self.X.copy_(X)
+        self.t, self.cur, self.dt_prev = self.t.copy(), self.cur.copy(), self.dt_prev.copy()
+        self.x_prev = None if xp is None else xp.clone()
+
+    def attempt(self) -> float:
+        """One attempt for every film; returns the cell-steps it accepted."""
+        b = self.runner.batch
+        self.runner.set_fields(self.t, BATCH_T_STOP)
+        dta = self.cur.copy()
+        x_in = self.X.clone() if self.extrapolate else None
+        st = b.trap_step(self.X, dta, self.opts, x_prev=self.x_prev, dt_prev=self.dt_prev)
+        ok = np.array([q.converged for q in st], dtype=bool)
+        self.newton += sum(q.newton_iters for q in st)
+        if x_in is not None:
+            if self.x_prev is None:
+                self.x_prev = x_in
+            else:
+                for f in np.flatnonzero(ok):
+                    b.view(self.x_prev, f).copy_(b.view(x_in, f))
+            self.dt_prev[ok] = dta[ok]
+        self.t[ok] += dta[ok]
+        self.cur[ok] = BATCH_DT
+        self.cur[~ok] /= 10.0
+        if (self.cur < 1e-8).any():
+            raise RuntimeError("dt reduced below dt_min in the batch")
+        return float(self.ns[ok].sum())
+
+    def run(self, k: int) -> float:
+        return sum(self.attempt() for _ in range(k))
+
+
+def section_batched(attempts, warmup, fp64=True):
+    """The headline: the 64-film batch in mixed precision."""
     import torch
     from nanowire_tdgl_amd import _native, workloads
     from nanowire_tdgl_amd.solvers.batch import BatchRunner
     films = workloads.batch_films()
     ns = np.array([f.device.params.n_interior for f in films])
-    tol, dt = 1e-4, 0.2
     runner = BatchRunner([f.device for f in films])
     tune = runner.tune(reps=3)
-    X = runner.pack([f.x0 for f in films])
-    opts = _native.solver_opts(tol_gcr=tol, precision="mixed")
-    t = np.zeros(len(films))
+    opts = _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed")
 
-    def attempt(o=opts):
-        runner.set_fields(t, 300 * dt)
-        st = runner.batch.trap_step(X, dt, o)
-        ok = np.array([s.converged for s in st], dtype=bool)
-        t[ok] += dt
-        return st, ok
+    def timed(b: BatchAttempts):
+        b.run(warmup)
+        torch.cuda.synchronize()
+        snap = b.snapshot()
+        n0 = b.newton
+        c0 = time.perf_counter()
+        acc = b.run(attempts)
+        torch.cuda.synchronize()
+        el_ = time.perf_counter() - c0
+        b.newton_timed = (b.newton - n0) / (attempts * len(films))
+        return snap, acc, el_
 
-    for _ in range(warmup):
-        attempt()
-    torch.cuda.synchronize()
-    X0, t0v = X.clone(), t.copy()
-    c0 = time.perf_counter()
-    acc = 0.0
-    for _ in range(attempts):
-        st, ok = attempt()
-        acc += float(ns[ok].sum())
-    torch.cuda.synchronize()
-    el = time.perf_counter() - c0
+    mix = BatchAttempts(runner, films, opts)
+    snap, acc, el = timed(mix)
     v = acc / el
+    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films, {attempts} attempts "
+        f"in {el:.3f} s")
     pc = roof = None
-    if PROFILE:  # instrumented repeat from the same state
-        X.copy_(X0)
-        t[:] = t0v
+    if PROFILE:  # instrumented repeat of the same attempts from the same state
+        final = mix.snapshot()
+        mix.restore(snap)
         runner.batch.profile_enable(True)
-        for _ in range(attempts):
-            attempt()
+        mix.run(attempts)
         pc = runner.batch.profile_read()
         runner.batch.profile_enable(False)
         roof = with_traffic(roofline_of(pc), "cheb_sweeps_mixed_sections.md", "batched")
-    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films ({el:.2f} s)")
-    # the same attempts without the preconditioner, after the same warm-up (its
-    # larger Krylov basis is allocated and its iteration graph captured there)
-    Xp, tp = X.clone(), t.copy()
-    opts_none = _native.solver_opts(tol_gcr=tol, precision="mixed", precond="none")
-    X.copy_(X0)
-    t[:] = t0v
-    for _ in range(max(warmup, 1)):
-        attempt(opts_none)
-    X.copy_(X0)
-    t[:] = t0v
-    torch.cuda.synchronize()
-    c0 = time.perf_counter()
-    accu = 0.0
-    for _ in range(attempts):
-        st, ok = attempt(opts_none)
-        accu += float(ns[ok].sum())
-    torch.cuda.synchronize()
-    vu = accu / (time.perf_counter() - c0)
+        mix.restore(final)
+        log(f"[bench] batched dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
+            f"{roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
+    # the same attempts without the preconditioner, and with the extrapolating predictor
+    unp = BatchAttempts(runner, films, _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed",
+                                                           precond="none"))
+    _, accu, elu = timed(unp)
+    vu = accu / elu
     log(f"[bench] batched mixed, no preconditioner: {vu:.4g} cell-steps/s")
-    # the same attempts with Newton started from each film's extrapolated last
-    # step; one untimed attempt first gives every film a last step
-    X.copy_(X0)
-    t[:] = t0v
-    hist = {"x": None, "dt": np.zeros(len(films))}
-
-    def attempt_ex():
-        runner.set_fields(t, 300 * dt)
-        x_in = X.clone()
-        st = runner.batch.trap_step(X, dt, opts, x_prev=hist["x"], dt_prev=hist["dt"])
-        ok = np.array([s.converged for s in st], dtype=bool)
-        t[ok] += dt
-        if hist["x"] is None or ok.all():
-            hist["x"] = x_in
-        else:
-            for f in np.flatnonzero(ok):
-                runner.batch.view(hist["x"], f).copy_(runner.batch.view(x_in, f))
-        hist["dt"][ok] = dt
-        return st, ok
-
-    attempt_ex()
-    torch.cuda.synchronize()
-    c0 = time.perf_counter()
-    acce, newton_ex = 0.0, 0
-    for _ in range(attempts):
-        st, ok = attempt_ex()
-        acce += float(ns[ok].sum())
-        newton_ex += sum(s.newton_iters for s in st)
-    torch.cuda.synchronize()
-    ve = acce / (time.perf_counter() - c0)
-    log(f"[bench] batched mixed, extrapolating predictor: {ve:.4g} cell-steps/s "
-        f"({newton_ex / (attempts * len(films)):.2f} Newton iterations per film-step)")
-    X.copy_(Xp)
-    t[:] = tp
+    ex = BatchAttempts(runner, films, opts, predictor="extrapolate")
+    ex.run(1)  # every film gets a last step to extrapolate from
+    _, acce, ele = timed(ex)
+    ve = acce / ele
+    log(f"[bench] batched mixed, extrapolating predictor: {ve:.4g} cell-steps/s")
     out = {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
-                       "Bz~U[0.05,0.8], holes on even films; dt=0.2, TGCR tol 1e-4",
+                       "Bz~U[0.05,0.8], holes on even films; psi = 1 + 1e-3 noise, phi = 0; dt=0.2 "
+                       "(per-film dt/10 retry), TGCR tol 1e-4, Chebyshev-preconditioned mixed TGCR, "
+                       "autotuned tiles",
            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
-           "mixed_batched_cell_steps_s": v, "mixed_batched_unpreconditioned_cell_steps_s": vu,
-           "mixed_batched_extrapolating_cell_steps_s": ve,
-           "newton_iters_per_film_step_extrapolating": newton_ex / (attempts * len(films)),
+           "warmup": warmup, "seconds": el, "value": v,
+           "newton_iters_per_film_attempt": mix.newton_timed,
            "roofline": roof, "kernel_classes": pc,
+           "unpreconditioned_cell_steps_s": vu,
+           "extrapolating_cell_steps_s": ve,
+           "extrapolating_newton_iters_per_film_attempt": ex.newton_timed,
            "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16"}[tune["shape"]],
-                                   "kz": tune["kz"]},
+                                   "kz": tune["kz"]}, "cached": tune.get("cached", False),
                         "jvp_ms_by_shape_and_kz": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
-    if fp64_steps <= 0:
+    if not fp64:
         return out
-    # baseline: loop the plain FP64 step over the films, same steps from the same start
-    steppers = [Stepper(f.device, f.x0, tol_gcr=tol) for f in films]
-    for s in steppers:
-        s.accepted_steps(warmup)
+    # baseline: the plain FP64 step looped over the films, same steps from the same start
+    steppers = [Stepper(f.device, f.x0, t_stop=BATCH_T_STOP, tol_gcr=BATCH_TOL) for f in films]
+    for s_ in steppers:
+        s_.accepted_steps(warmup)
     torch.cuda.synchronize()
     c0 = time.perf_counter()
-    for s in steppers:
-        s.accepted_steps(fp64_steps)
+    for s_ in steppers:
+        s_.accepted_steps(attempts)
     torch.cuda.synchronize()
     el64 = time.perf_counter() - c0
-    v64 = float(ns.sum()) * fp64_steps / el64
+    v64 = float(ns.sum()) * attempts / el64
     log(f"[bench] FP64 loop: {v64:.4g} cell-steps/s ({el64:.2f} s); batched/loop = {v / v64:.2f}")
-    # error of the batched mixed states against the FP64 loop (films at the same
-    # time), and against FP64 runs with Newton converged (untimed)
-    worst = {"psi2": 0.0, "bz": 0.0, "js": 0.0}
-    keys = ("mixed", "fp64_same_tolerances")
-    conv = {k: dict(worst) for k in keys}
+    # per-film error against the oracle of the comparison (FP64, Newton converged)
+    # and against the FP64 loop at the same tolerances, films at the same time
+    errs = {"mixed_vs_fp64_converged": [], "fp64_same_tol_vs_fp64_converged": [],
+            "mixed_vs_fp64_same_tol": []}
     compared = 0
-    for f, s in enumerate(steppers):
-        if abs(t[f] - s.t) > 1e-9:
+    for f, s_ in enumerate(steppers):
+        eng = runner.engines[f]
+        Xf = runner.batch.view(mix.X, f)
+        ref = Stepper(films[f].device, films[f].x0, t_stop=BATCH_T_STOP, tol_gcr=BATCH_TOL,
+                      newton_tol_dx=CONVERGED_NEWTON_DX)
+        ref.accepted_steps(warmup + attempts)
+        if abs(mix.t[f] - ref.t) > 1e-9 or abs(s_.t - ref.t) > 1e-9:
             continue
         compared += 1
-        eng = runner.engines[f]
-        n = f_n = films[f].device.params.n_interior
-        b, j = eng.ctx.observables(runner.batch.view(X, f))
-        psi = eng.ctx.to_reference(runner.batch.view(X, f)).cpu().numpy()[:f_n]
+        n = ns[f]
+        b, j = eng.ctx.observables(Xf)
+        psi = eng.ctx.to_reference(Xf).cpu().numpy()[:n]
         mine = (np.abs(psi) ** 2, b.cpu().numpy()[2 * n:], j.cpu().numpy())
-        e = rel_errors(mine, s.observables())
-        worst = {k: max(worst[k], e[k]) for k in worst}
-        ref = Stepper(films[f].device, films[f].x0, tol_gcr=tol, newton_tol_dx=CONVERGED_NEWTON_DX)
-        ref.accepted_steps(warmup + fp64_steps)
-        if abs(ref.t - s.t) > 1e-9:
-            continue
-        obs = ref.observables()
-        for k, e in zip(keys, (rel_errors(mine, obs), rel_errors(s.observables(), obs))):
-            conv[k] = {q: max(conv[k][q], e[q]) for q in e}
-    log(f"[bench] batched vs FP64 loop, worst over {compared} films: {worst}")
-    log(f"[bench] batched vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}), "
-        f"worst over films: {conv}")
+        obs, same = ref.observables(), s_.observables()
+        errs["mixed_vs_fp64_converged"].append(rel_errors(mine, obs))
+        errs["fp64_same_tol_vs_fp64_converged"].append(rel_errors(same, obs))
+        errs["mixed_vs_fp64_same_tol"].append(rel_errors(mine, same))
+    dist = {k: {q: _quantiles([e[q] for e in v_]) for q in ("psi2", "bz", "js")}
+            for k, v_ in errs.items() if v_}
+    log(f"[bench] batched error distribution over {compared} films: {dist}")
     out.update({"fp64_loop_cell_steps_s": v64, "speedup_vs_fp64_loop": v / v64,
                 "speedup_extrapolating_vs_fp64_loop": ve / v64,
                 "speedup_unpreconditioned_vs_fp64_loop": vu / v64,
-                "films_compared": compared, "rel_err_vs_fp64_worst_film": worst,
-                "rel_err_vs_fp64_converged_worst_film": conv})
+                "films_compared": compared,
+                "error_oracle": ("plain FP64 GPU step with Newton converged (||dx||_inf <= 1e-5), "
+                                 "same films, steps and fields"),
+                "rel_err": dist,
+                "rel_err_psi2_per_film": [float(f"{e['psi2']:.3g}") for e in errs["mixed_vs_fp64_converged"]]})
     return out
 
 
-def cpu_baseline(step_stats: dict, n_interior: int, budget: float) -> dict:
+def cpu_baseline(budget: float, workers: int) -> dict:
+    """The oracle on whole steps of the batch's smallest films: one thread,
+    then `workers` processes of one thread each."""
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    out = subprocess.run([sys.executable, "-m", "oracle.cpu_sample", "--budget", str(budget)],
-                         cwd=str(ROOT), env=env, capture_output=True, text=True, timeout=600)
-    if out.returncode != 0:
-        raise RuntimeError(out.stderr[-2000:])
-    s = json.loads(out.stdout.strip().splitlines()[-1])
-    evals_step = 1 + 2 * step_stats["newton_iters"] + 2 * step_stats["tgcr_iters_total"]
-    sec_per_step = s["elapsed_s"] * evals_step / max(s["evals"], 1)
-    if s["finished_step"]:
-        sec_per_step = s["elapsed_s"]
-    return {"value": n_interior / sec_per_step, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (CSR eval_f per call, FD-JVP TGCR with MGS, Newton) on step 1 of "
-                       f"this film for {s['elapsed_s']:.1f} s: {s['evals']} of the step's "
-                       f"{evals_step} eval_f calls (count from the GPU run's Newton/TGCR "
-                       f"iterations); time scaled by calls; 1 thread"),
-            "sec_per_step": sec_per_step}
+    runs = {}
+    for w in (1, workers):
+        out = subprocess.run([sys.executable, "-m", "oracle.cpu_sample", "--budget", str(budget),
+                              "--workers", str(w)], cwd=str(ROOT), env=env, capture_output=True,
+                             text=True, timeout=1200)
+        if out.returncode != 0:
+            raise RuntimeError(out.stderr[-2000:])
+        runs[w] = json.loads(out.stdout.strip().splitlines()[-1])
+        log(f"[bench] CPU oracle, {w} worker(s): {runs[w]['value']:.4g} cell-steps/s "
+            f"({runs[w]['films']} film steps in {runs[w]['elapsed_s']:.1f} s)")
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    one, allc = runs[1], runs[workers]
+    return {"value": allc["value"], "unit": "cell-steps/s", "cores": workers, "kind": "port",
+            "cpu_model": model, "cpu_count": os.cpu_count(),
+            "sample": (f"oracle (CSR eval_f rebuilt per call, FD-JVP TGCR with MGS, Newton, dt/10 "
+                       f"retry) on whole first steps (dt=0.2, field on, TGCR tol 1e-4) of the "
+                       f"batch's films, smallest first, for a {budget:.0f} s budget: "
+                       f"{allc['films']} film steps ({allc['cell_steps']} cell-steps) in "
+                       f"{allc['elapsed_s']:.1f} s on {workers} one-thread processes"),
+            "one_thread": {"value": one["value"], "films": one["films"],
+                           "cell_steps": one["cell_steps"], "seconds": one["elapsed_s"]},
+            "all_cores": {"value": allc["value"], "workers": workers, "films": allc["films"],
+                          "cell_steps": allc["cell_steps"], "seconds": allc["elapsed_s"]}}
 
 
-def section_adjoint(steps, every, fd=True, predictor=None):
+def section_adjoint(steps, every, fd=True, predictor=None, mask_cells=20):
     import torch
     import nanowire_tdgl_amd as T
     from nanowire_tdgl_amd.adjoint import AdjointProblem, TrapezoidalBz, midplane_target_loss
@@ -515,10 +563,10 @@ def section_adjoint(steps, every, fd=True, predictor=None):
                      "rel_diff_kappa": abs(fdk - out["grad"]["kappa"]) / abs(fdk),
                      "rel_diff_Bz": abs(fdb - out["grad"]["Bz"]) / abs(fdb)}
         log(f"[bench] adjoint vs central differences: {out['fd']}")
-        # 20 random interior mask cells (seed 4), the same eps
+        # random interior mask cells (seed 4), the same eps
         gw = w.grad.numpy()
         full = prob.device.idx.interior_to_full
-        cells = np.random.default_rng(4).choice(full, size=20, replace=False)
+        cells = np.random.default_rng(4).choice(full, size=20, replace=False)[:mask_cells]
 
         def Lw(wv):
             _, bzo = prob.forward(2.5, 0.3, wv)
@@ -531,11 +579,11 @@ def section_adjoint(steps, every, fd=True, predictor=None):
             wm[m] -= eps
             diffs.append(abs((Lw(wp) - Lw(wm)) / (2 * eps) - gw[m]))
             if (c + 1) % 5 == 0:
-                log(f"[bench] mask cells: {c + 1} of 20 done")
+                log(f"[bench] mask cells: {c + 1} of {len(cells)} done")
         scale = float(np.max(np.abs(gw)))
-        out["fd"]["mask_cells"] = {"cells": 20, "max_abs_diff": float(max(diffs)), "grad_scale": scale,
+        out["fd"]["mask_cells"] = {"cells": len(cells), "max_abs_diff": float(max(diffs)), "grad_scale": scale,
                                    "max_diff_over_scale": float(max(diffs)) / scale}
-        log(f"[bench] adjoint vs central differences on 20 mask cells: max |diff| {max(diffs):.3e} "
+        log(f"[bench] adjoint vs central differences on {len(cells)} mask cells: max |diff| {max(diffs):.3e} "
             f"({max(diffs) / scale:.2e} of max |dL/dw|)")
     return out
 
@@ -577,6 +625,34 @@ def calibrate_copy(mib: int, reps: int) -> None:
                       "bytes_written_per_copy": mib * (1 << 20), "copies": reps}), flush=True)
 
 
+def section_fp64_plain(dev, x0, steps, warmup):
+    """The plain FP64 method (the parity path) on the 128x128x16 production film."""
+    n = dev.params.n_interior
+    s, el, per_class, roof = timed_run(dev, x0, "f64", steps, warmup, 1e-8)
+    timed = s.timed
+    value = n * steps / el
+    log(f"[bench] FP64 plain, 128x128x16: {el:.3f} s for {steps} steps ({len(timed)} attempts): "
+        f"{value:.4g} cell-steps/s")
+    if roof:
+        tr = pmc_traffic("pmc_traffic_film128_hole.md").get(roof["kernel"])
+        if tr:
+            roof["traffic"] = tr
+            roof["traffic_source"] = ("profiles/pmc_traffic_film128_hole.md (rocprofv3 FETCH_SIZE x 2.0 + "
+                                      "WRITE_SIZE, calibrated on a 1 GiB copy)")
+        log(f"[bench] FP64 dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
+            f"{roof['achieved']:.0f} GB/s, traffic {roof['traffic']}")
+    step_bytes = sum(r["bytes_moved"] for r in timed)
+    out = {"workload": "plain FP64 trapezoidal step, 128x128x16 film, h=0.5, kappa=4, Bz=0.4, "
+                       "32x32 hole, psi = 1 + 1e-3 noise (seed 1), dt=0.2, TGCR tol 1e-8",
+           "n_interior": n, "steps": steps, "warmup": warmup, "value": value,
+           "ms_per_step": el / steps * 1e3, "roofline": roof, "kernel_classes": per_class,
+           "attempts": len(timed),
+           "newton_iters_per_step": sum(r["newton_iters"] for r in timed) / steps,
+           "tgcr_iters_per_step": sum(r["tgcr_iters_total"] for r in timed) / steps,
+           "whole_step_algorithmic_GBps": step_bytes / el / 1e9}
+    return out, s, el
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -584,18 +660,20 @@ def main() -> None:
     ap.add_argument("--whole-run", action="store_true",
                     help="only time the production film's whole 1000-step run per method, in chunks")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20, help="timed lockstep attempts of the batch")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed attempts before them")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-workers", type=int, default=min(os.cpu_count() or 1, 16),
+                    help="processes of the all-cores CPU figure (default: all cores, at most 16)")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--large-steps", type=int, default=2)
-    ap.add_argument("--batch-attempts", type=int, default=3)
-    ap.add_argument("--batch-fp64-steps", type=int, default=-1,
-                    help="steps of the FP64 loop baseline in the batched section "
-                         "(default: --batch-attempts; 0 skips the baseline)")
-    ap.add_argument("--adjoint-steps", type=int, default=40)
-    ap.add_argument("--sections", default="mixed,large,batched,adjoint",
+    ap.add_argument("--no-batch-fp64", action="store_true",
+                    help="skip the FP64 loop baseline and the error report of the batch")
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-mask-cells", type=int, default=6)
+    ap.add_argument("--sections", default="fp64_plain,mixed,large,adjoint",
                     help="comma list of secondary sections to run (empty: headline only)")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -611,60 +689,44 @@ def main() -> None:
               "this path is a design note, not a measured run", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    dev, x0 = build_film()
-    n = dev.params.n_interior
-    log(f"[bench] film {dev.params.Nx}x{dev.params.Ny}x{dev.params.Nz}, n_interior={n}")
-    s, el, per_class, roof = timed_run(dev, x0, "f64", args.steps, args.warmup, 1e-8)
-    timed = s.timed
-    value = n * args.steps / el
-    log(f"[bench] {el:.3f} s for {args.steps} steps ({len(timed)} attempts): {value:.4g} cell-steps/s")
-    if roof:
-        tr = pmc_traffic("pmc_traffic_film128_hole.md").get(roof["kernel"])
-        if tr:
-            roof["traffic"] = tr
-            roof["traffic_source"] = ("profiles/pmc_traffic_film128_hole.md (rocprofv3 FETCH_SIZE x 2.0 + "
-                                      "WRITE_SIZE, calibrated on a 1 GiB copy)")
-        log(f"[bench] dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
-            f"{roof['achieved']:.0f} GB/s, traffic {roof['traffic']}")
+    import torch  # noqa: F401
+    head = section_batched(args.steps, args.warmup, fp64=not args.no_batch_fp64)
     sections = [x for x in args.sections.split(",") if x]
     extra = {}
-    if "mixed" in sections:
-        extra["mixed"] = section_mixed(dev, x0, s, el, args.steps, args.warmup, 1e-8)
+    if "fp64_plain" in sections or "mixed" in sections:
+        dev, x0 = build_film()
+        fp, s64, el64 = section_fp64_plain(dev, x0, args.fp64_steps, 2)
+        if "fp64_plain" in sections:
+            extra["fp64_plain"] = fp
+        if "mixed" in sections:
+            extra["mixed"] = section_mixed(dev, x0, s64, el64, args.fp64_steps, 2, 1e-8)
     if "large" in sections:
         extra["large"] = section_large(args.large_steps, 1)
-    if "batched" in sections:
-        fp64_steps = args.batch_attempts if args.batch_fp64_steps < 0 else args.batch_fp64_steps
-        extra["batched"] = section_batched(args.batch_attempts, 1, fp64_steps)
     if "adjoint" in sections:
-        extra["adjoint"] = section_adjoint(args.adjoint_steps, 20)
+        extra["adjoint"] = section_adjoint(args.adjoint_steps, 20, mask_cells=args.adjoint_mask_cells)
 
-    first = s.log[0]
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU oracle sample ({args.cpu_budget:.0f} s budget)")
-        cpu = cpu_baseline(first, n, args.cpu_budget)
-        log(f"[bench] CPU baseline {cpu['value']:.4g} cell-steps/s ({cpu['sec_per_step']:.1f} s/step)")
+        log(f"[bench] CPU oracle sample ({args.cpu_budget:.0f} s budget per run)")
+        cpu = cpu_baseline(args.cpu_budget, args.cpu_workers)
 
-    step_bytes = sum(r["bytes_moved"] for r in timed)
+    value = head.pop("value")
+    el = head.pop("seconds")
+    roof = head.pop("roofline")
     res = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "mixed",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 trapezoidal step, 128x128x16 film, h=0.5, kappa=4, "
-                               "Bz=0.4, 32x32 hole, dt=0.2, TGCR tol 1e-8",
-                   "n_interior": n, "precision": "f64"},
+        "config": {"workload": head.pop("workload"), "films": head.pop("films"),
+                   "n_interior_total": head.pop("n_interior_total"), "precision": "mixed",
+                   "step": "one lockstep trapezoidal attempt over all films"},
         "roofline": roof,
-        "cpu_baseline": ({k: v for k, v in cpu.items() if k != "sec_per_step"} if cpu else None),
-        "step_counters": {
-            "attempts": len(timed),
-            "newton_iters_per_step": sum(r["newton_iters"] for r in timed) / args.steps,
-            "tgcr_iters_per_step": sum(r["tgcr_iters_total"] for r in timed) / args.steps,
-            "whole_step_algorithmic_GBps": step_bytes / el / 1e9,
-        },
-        "kernel_classes": per_class,
+        "cpu_baseline": cpu,
     }
+    if cpu:
+        res["speedup_vs_cpu_all_cores"] = value / cpu["value"]
+    res["batched"] = head
     res.update(extra)
     print(json.dumps(res), flush=True)
 

@@ -167,6 +167,18 @@ def load() -> C.CDLL:
     return lib
 
 
+_digest = None
+
+
+def library_digest() -> str:
+    """Digest of the built library (tuning results are only valid for one build)."""
+    global _digest
+    if _digest is None:
+        import hashlib
+        _digest = hashlib.sha256(LIB_PATH.read_bytes()).hexdigest()[:16]
+    return _digest
+
+
 def exported_symbols() -> list[str]:
     """Names from include/tdgl_hip.h that the built library exports."""
     lib = C.CDLL(str(LIB_PATH))

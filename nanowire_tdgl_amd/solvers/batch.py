@@ -10,7 +10,11 @@ at t_stop.  The films advance in lockstep attempts: one
 
 from __future__ import annotations
 
+import hashlib
+import json
+import os
 from dataclasses import dataclass, field
+from pathlib import Path
 from typing import Optional, Sequence
 
 import numpy as np
@@ -54,15 +58,43 @@ class BatchRunner:
         for d, e, t in zip(self.devices, self.engines, times):
             e.set_field(*d.applied_field.evaluate(t, t_stop))
 
+    def tune_key(self, candidates, shapes) -> str:
+        """Cache key of a tuning: the GPU, the library build, the precision and
+        every film's grid (Nx, Ny, Nz), with the candidates searched."""
+        import torch
+        dims = [[e.params.Nx, e.params.Ny, e.params.Nz] for e in self.engines]
+        blob = json.dumps([torch.cuda.get_device_name(self.batch.device), _native.library_digest(),
+                           "mixed", dims, list(candidates), list(shapes)])
+        return hashlib.sha256(blob.encode()).hexdigest()[:32]
+
+    def _apply(self, kzs, shape) -> None:
+        self.batch.set_tile(int(shape))
+        for f, k in enumerate(kzs):
+            self.batch.set_kz(f, int(k))
+
     def tune(self, X=None, candidates=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), shapes=(0, 1, 2),
-             reps: int = 5) -> dict:
+             reps: int = 5, cache: Optional[str] = "default") -> dict:
         """Autotuner: pick the tile of the mixed-precision sweeps -- its shape
         (0 = 64x4, 1 = 32x8, 2 = 16x16 nodes per plane) and its z depth kz.
 
         Every shape with every candidate depth policy kz_f = min(K, nz_f), and
         the current setting, is timed on the batch's own FP32 JVP stencil; the
-        fastest is kept.  Returns {"kz": [...], "shape": s, "ms": best,
-        "trials": {"<shape>:<K>": ms}}."""
+        fastest is kept.  The pick is cached in a JSON file (``cache``: a path,
+        "default" = $NANOWIRE_TDGL_AMD_TUNE_CACHE or
+        ~/.cache/nanowire_tdgl_amd/autotune.json, None = no cache) under
+        :meth:`tune_key`; a later tuning of the same films on the same GPU
+        reads it instead of timing.  Returns {"kz": [...], "shape": s, "ms":
+        best, "trials": {"<shape>:<K>": ms}, "cached": bool}."""
+        path = _cache_path(cache)
+        key = self.tune_key(candidates, shapes) if path else None
+        if path and path.exists():
+            try:
+                hit = json.loads(path.read_text()).get(key)
+            except (OSError, ValueError):
+                hit = None
+            if hit and len(hit["kz"]) == self.batch.nfilm:
+                self._apply(hit["kz"], hit["shape"])
+                return dict(hit, cached=True)
         b = self.batch
         F = b.nfilm
         names = {0: "64x4", 1: "32x8", 2: "16x16"}
@@ -80,10 +112,36 @@ class BatchRunner:
                 if ms < best_ms:
                     best_ms, best = ms, (kzs, sh)
         kzs, sh = best
-        b.set_tile(sh)
-        for f, k in enumerate(kzs):
-            b.set_kz(f, k)
-        return {"kz": kzs, "shape": sh, "ms": best_ms, "trials": trials}
+        self._apply(kzs, sh)
+        res = {"kz": [int(k) for k in kzs], "shape": int(sh), "ms": best_ms, "trials": trials}
+        if path:
+            _cache_store(path, key, res)
+        return dict(res, cached=False)
+
+
+def _cache_path(cache) -> Optional[Path]:
+    if cache is None:
+        return None
+    if cache == "default":
+        env = os.environ.get("NANOWIRE_TDGL_AMD_TUNE_CACHE")
+        return Path(env) if env else Path.home() / ".cache" / "nanowire_tdgl_amd" / "autotune.json"
+    return Path(cache)
+
+
+def _cache_store(path: Path, key: str, res: dict) -> None:
+    """Add one tuning to the JSON cache (written to a temporary file, then renamed)."""
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        data = json.loads(path.read_text()) if path.exists() else {}
+    except (OSError, ValueError):
+        data = {}
+    data[key] = res
+    tmp = path.with_suffix(path.suffix + f".{os.getpid()}.tmp")
+    try:
+        tmp.write_text(json.dumps(data))
+        os.replace(tmp, path)
+    except OSError:
+        pass  # an unwritable cache only costs the next run a tuning
 
 
 def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float, dt: float, *,

@@ -49,21 +49,42 @@ def film256_spheres(Nx: int = 256, Ny: int = 256, Nz: int = 64, fraction: float 
     return dev, _noise_state(p.n_interior, 2, 1e-3, mask)
 
 
-def batch_films(nfilm: int = 64, seed: int = 3, nmin: int = 32, nmax: int = 160,
-                zmin: int = 4, zmax: int = 24) -> list[BatchFilm]:
+def batch_film_specs(nfilm: int = 64, seed: int = 3, nmin: int = 32, nmax: int = 160,
+                     zmin: int = 4, zmax: int = 24) -> list[dict]:
+    """The batch's films as plain data: grid parameters, Bz, the hole polygon
+    (or None) and the seed of the initial state's noise."""
     rng = np.random.default_rng(seed)
-    films = []
+    specs = []
     for f in range(nfilm):
         Nx, Ny = (int(v) for v in rng.integers(nmin, nmax + 1, size=2))
         Nz = int(rng.integers(zmin, zmax + 1))
         kappa = float(rng.uniform(1.0, 5.0))
         bz = float(rng.uniform(0.05, 0.8))
-        p = SimulationParameters(Nx=Nx, Ny=Ny, Nz=Nz, hx=0.5, hy=0.5, hz=0.5, kappa=kappa)
-        dev = Device(p, applied_field=AppliedField(Bz=bz))
+        hole = None
         if f % 2 == 0:
-            sx, sy = rng.uniform(0.1, 0.3, size=2) * Nx * p.hx
-            cx, cy = Nx * p.hx / 2.0, Ny * p.hy / 2.0
-            dev.add_hole([(cx - sx / 2, cy - sy / 2), (cx + sx / 2, cy - sy / 2),
-                          (cx + sx / 2, cy + sy / 2), (cx - sx / 2, cy + sy / 2)])
-        films.append(BatchFilm(dev, _noise_state(p.n_interior, seed + 1000 + f, 1e-3)))
+            sx, sy = rng.uniform(0.1, 0.3, size=2) * Nx * 0.5
+            cx, cy = Nx * 0.5 / 2.0, Ny * 0.5 / 2.0
+            hole = [(cx - sx / 2, cy - sy / 2), (cx + sx / 2, cy - sy / 2),
+                    (cx + sx / 2, cy + sy / 2), (cx - sx / 2, cy + sy / 2)]
+        specs.append({"params": dict(Nx=Nx, Ny=Ny, Nz=Nz, hx=0.5, hy=0.5, hz=0.5, kappa=kappa),
+                      "bz": bz, "hole": hole, "state_seed": seed + 1000 + f})
+    return specs
+
+
+def batch_state(spec: dict) -> np.ndarray:
+    """Initial state of one batch film: psi = 1 + 1e-3 noise, phi = 0."""
+    p = spec["params"]
+    n = (p["Nx"] - 1) * (p["Ny"] - 1) * (p["Nz"] - 1)
+    return _noise_state(n, spec["state_seed"], 1e-3)
+
+
+def batch_films(nfilm: int = 64, seed: int = 3, nmin: int = 32, nmax: int = 160,
+                zmin: int = 4, zmax: int = 24) -> list[BatchFilm]:
+    films = []
+    for spec in batch_film_specs(nfilm, seed, nmin, nmax, zmin, zmax):
+        p = SimulationParameters(**spec["params"])
+        dev = Device(p, applied_field=AppliedField(Bz=spec["bz"]))
+        if spec["hole"]:
+            dev.add_hole(spec["hole"])
+        films.append(BatchFilm(dev, batch_state(spec)))
     return films

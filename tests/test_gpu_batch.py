@@ -197,7 +197,7 @@ def test_batch_error_against_fp64():
             assert err < 1e-6, (f, name)
 
 
-def test_stencil_depth_does_not_change_the_jvp():
+def test_stencil_depth_does_not_change_the_jvp(tmp_path):
     import torch
     from nanowire_tdgl_amd.solvers.batch import BatchRunner
     films = _films()
@@ -219,9 +219,20 @@ def test_stencil_depth_does_not_change_the_jvp():
     assert all(o.equal(outs[0]) for o in outs[1:])
     with pytest.raises(ValueError):
         ctx.set_sweep_tile(3)
-    info = runner.tune(reps=2)
+    cache = tmp_path / "autotune.json"
+    info = runner.tune(reps=2, cache=str(cache))
     assert len(info["kz"]) == 3 and info["ms"] > 0 and info["shape"] in (0, 1, 2)
-    assert runner.batch.get_tile() == info["shape"]
+    assert runner.batch.get_tile() == info["shape"] and not info["cached"]
+    assert cache.exists()
+    # a second tuning of the same films reads the pick from the cache
+    other = BatchRunner([f.device for f in films])
+    again = other.tune(reps=2, cache=str(cache))
+    assert again["cached"] and again["kz"] == info["kz"] and again["shape"] == info["shape"]
+    assert other.batch.get_tile() == info["shape"]
+    assert [other.batch.get_kz(f) for f in range(3)] == info["kz"]
+    # different films: a different key, tuned afresh
+    fewer = BatchRunner([f.device for f in films[:2]])
+    assert not fewer.tune(reps=1, cache=str(cache))["cached"]
 
 
 def test_graph_replay_matches_eager_launches():

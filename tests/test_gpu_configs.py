@@ -65,7 +65,7 @@ def test_c4_batch_of_64_films_against_fp64():
     films = workloads.batch_films()
     assert len(films) == 64
     runner = BatchRunner([f.device for f in films])
-    tune = runner.tune(reps=2)
+    tune = runner.tune(reps=2, cache=None)
     print(f"autotuned tile shape {tune['shape']}, kz {tune['kz'][:8]}..., {tune['ms']:.3f} ms per sweep")
     res = trapezoidal_batch(films, 0.0, T_FIELD_STOP, 0.2, max_attempts=3, runner=runner)
     print(f"attempts {res.attempts}, accepted steps per film {sorted(set(res.steps))}")
