@@ -418,9 +418,9 @@ def section_batched(attempts, warmup, fp64=True):
            "unpreconditioned_cell_steps_s": vu,
            "extrapolating_cell_steps_s": ve,
            "extrapolating_newton_iters_per_film_attempt": ex.newton_timed,
-           "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16"}[tune["shape"]],
-                                   "kz": tune["kz"]}, "cached": tune.get("cached", False),
-                        "jvp_ms_by_shape_and_kz": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
+           "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[tune["shape"]],
+                                   "split": tune["split"]}, "cached": tune.get("cached", False),
+                        "jvp_ms_by_shape_and_split": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
     if not fp64:
         return out
     # baseline: the plain FP64 step looped over the films, same steps from the same start

@@ -126,13 +126,17 @@ typedef struct tdgl_step_stats {
 int tdgl_ctx_create(const tdgl_grid* grid, tdgl_ctx** out);
 int tdgl_ctx_destroy(tdgl_ctx* ctx);
 int tdgl_set_stream(tdgl_ctx* ctx, void* hip_stream);
-/* Tuning (no counterpart in the reference): z planes per stencil work item
- * (tiles are 64 x 4 x kz nodes) for this context's kernels, FP64 and mixed. */
+/* Tuning (no counterpart in the reference): z planes per FP64 stencil work
+ * item (tiles are 64 x 4 x kz nodes), including the mixed method's FP64
+ * residual stencil; the mixed sweeps size their z pieces to their work
+ * (tdgl_batch_set_sweep_split). */
 int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz);
 int tdgl_get_stencil_kz(const tdgl_ctx* ctx);
 /* Tuning (no counterpart in the reference): tile shape of the mixed-precision
  * stencil sweeps, nodes per plane: 0 = 64 x 4 (default), 1 = 32 x 8,
- * 2 = 16 x 16.  Results are the same bits for every shape. */
+ * 2 = 16 x 16, 3 = each film the one of these with the least work per plane
+ * (256 lanes per tile, idle ones included, plus a third of a lane per staged
+ * halo cell).  Results are the same bits for every shape. */
 int tdgl_set_sweep_tile(tdgl_ctx* ctx, int shape);
 int tdgl_get_sweep_tile(const tdgl_ctx* ctx);
 const char* tdgl_last_error(const tdgl_ctx* ctx);
@@ -300,13 +304,22 @@ int tdgl_batch_set_stream(tdgl_batch* batch, void* hip_stream);
 const char* tdgl_batch_last_error(const tdgl_batch* batch);
 int64_t tdgl_batch_total_len(const tdgl_batch* batch);
 int64_t tdgl_batch_offset(const tdgl_batch* batch, int film);
-/* Tuning: z planes per stencil work item of film `film` (the autotuner's knob). */
+/* Tuning: z planes per FP64 stencil work item of film `film` (the batch's
+ * predictor and Newton residual stencils). */
 int tdgl_batch_set_kz(tdgl_batch* batch, int film, int kz);
 int tdgl_batch_get_kz(const tdgl_batch* batch, int film);
 /* Tuning: tile shape of the batch's FP32 sweeps (as tdgl_set_sweep_tile; the
  * autotuner's other knob). */
 int tdgl_batch_set_sweep_tile(tdgl_batch* batch, int shape);
 int tdgl_batch_get_sweep_tile(const tdgl_batch* batch);
+/* Tuning: how each FP32 sweep's work is split into blocks.  Every tile column
+ * of the films taking part in a sweep is cut along z into near-equal pieces of
+ * at most P planes, one block per piece, P = (the sweep's tile-planes / the
+ * blocks the device holds at once) clamped to [min_planes, max_planes]: deep
+ * pieces when every film sweeps, shallow ones when a sweep covers a few films.
+ * Defaults 8 and 2; results are the same bits for every split. */
+int tdgl_batch_set_sweep_split(tdgl_batch* batch, int max_planes, int min_planes);
+int tdgl_batch_get_sweep_split(const tdgl_batch* batch, int* max_planes, int* min_planes);
 /* Average milliseconds of `reps` launches of the batch's FP32 JVP stencil over
  * its work buffers, as currently laid out (what the autotuner measures). */
 int tdgl_batch_time_stencil(tdgl_batch* batch, int reps, double* ms);

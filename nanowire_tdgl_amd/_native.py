@@ -34,7 +34,8 @@ EXPORTS = [
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
     "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
-    "tdgl_batch_set_sweep_tile", "tdgl_batch_get_sweep_tile",
+    "tdgl_batch_set_sweep_tile", "tdgl_batch_get_sweep_tile", "tdgl_batch_set_sweep_split",
+    "tdgl_batch_get_sweep_split",
     "tdgl_batch_time_stencil", "tdgl_batch_set_graphs",
     "tdgl_batch_trap_step", "tdgl_batch_profile_enable", "tdgl_batch_profile_read",
 ]
@@ -141,6 +142,8 @@ def load() -> C.CDLL:
         "tdgl_get_sweep_tile": ([P], C.c_int),
         "tdgl_batch_set_sweep_tile": ([P, C.c_int], C.c_int),
         "tdgl_batch_get_sweep_tile": ([P], C.c_int),
+        "tdgl_batch_set_sweep_split": ([P, C.c_int, C.c_int], C.c_int),
+        "tdgl_batch_get_sweep_split": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "tdgl_vjp_f": ([P, P, P, P, P, P, P], C.c_int),
         "tdgl_vjp_bz": ([P, P, P, P], C.c_int),
         "tdgl_adjoint_solve": ([P, P, P, D, D, I64, C.c_int, P, C.POINTER(C.c_int)], C.c_int),
@@ -248,7 +251,8 @@ class Context:
         return int(self.lib.tdgl_get_stencil_kz(self.h))
 
     def set_sweep_tile(self, shape: int) -> None:
-        """Tile shape of the mixed-precision sweeps: 0 = 64x4, 1 = 32x8, 2 = 16x16."""
+        """Tile shape of the mixed-precision sweeps: 0 = 64x4, 1 = 32x8, 2 = 16x16,
+        3 = the film's own best (fewest idle lanes and staged cells)."""
         self._call("tdgl_set_sweep_tile", C.c_int(int(shape)))
 
     def get_sweep_tile(self) -> int:
@@ -524,6 +528,15 @@ class Batch:
 
     def get_tile(self) -> int:
         return int(self.lib.tdgl_batch_get_sweep_tile(self.h))
+
+    def set_split(self, max_planes: int, min_planes: int) -> None:
+        """Depth range of the FP32 sweeps' pieces (same bits for every split)."""
+        self._call("tdgl_batch_set_sweep_split", C.c_int(int(max_planes)), C.c_int(int(min_planes)))
+
+    def get_split(self) -> tuple:
+        a, b = C.c_int(), C.c_int()
+        self._call("tdgl_batch_get_sweep_split", C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def set_graphs(self, on: bool) -> None:
         self._call("tdgl_batch_set_graphs", C.c_int(1 if on else 0))

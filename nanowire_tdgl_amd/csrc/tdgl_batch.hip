@@ -39,15 +39,14 @@ struct tdgl_batch {
   std::vector<FilmDesc> hf;  // host copy of the descriptors
   std::vector<int> kz;       // z planes per stencil block, per film
   int shape = kDefaultSweepShape;  // FP32 sweep tile shape
-  std::vector<int> item_film, item_film32, sblk_film;
-  int nitems = 0, nitems32 = 0, nsblk = 0;
+  std::vector<int> item_film, sblk_film;
+  int nitems = 0, nsblk = 0;
   long long total_len = 0;  // elements of a batched state vector
   long long hlen = 0;       // doubles of the packed TGCR matrices
   std::vector<int> kcap;    // per film: min(n_state, kMaxKrylov)
 
   FilmDesc* d_films = nullptr;
   int* d_item_film = nullptr;
-  int* d_item_film32 = nullptr;
   int* d_sblk_film = nullptr;
   int* d_act = nullptr;
   double* d_res = nullptr;
@@ -88,6 +87,18 @@ struct tdgl_batch {
   int gen = 0;  // some film has general applied-field vectors (set by refresh)
   int body_gen = -1;
 
+  // Work pieces of the FP32 sweeps (SweepPart): part[0] the plain JVP over
+  // the films of the step, part[m] Chebyshev sweep m over the films whose
+  // degree reaches m; rebuilt when that set or the tuning changes.
+  std::vector<SweepPart> part;
+  std::vector<int> part_key;
+  int4* d_pseg = nullptr;
+  size_t pseg_cap = 0;
+  int part_gen = 0, body_part = -1;
+  int part_lmax = 8;  // deepest sweep piece, in planes (the autotuner's knob)
+  int part_lmin = 2;  // shallowest sweep piece when a sweep has little work (the autotuner's knob)
+  int cus = 0;        // compute units of the device
+
   ProfSink own_prof;
   ProfSink* prof = &own_prof;
 };
@@ -126,7 +137,6 @@ BatchK kview(const tdgl_batch* B) {
   BatchK k;
   k.films = B->d_films;
   k.item_film = B->d_item_film;
-  k.item_film32 = B->d_item_film32;
   k.sblk_film = B->d_sblk_film;
   k.act = B->d_act;
   k.res = B->d_res;
@@ -134,7 +144,6 @@ BatchK kview(const tdgl_batch* B) {
   k.cheb = B->d_cheb;
   k.nfilm = B->nfilm;
   k.nitems = B->nitems;
-  k.nitems32 = B->nitems32;
   k.shape = B->shape;
   k.gen = B->gen;
   k.nsblk = B->nsblk;
@@ -172,7 +181,6 @@ int build_layout(tdgl_batch* B) {
   const int F = B->nfilm;
   B->hf.assign(F, FilmDesc{});
   B->item_film.clear();
-  B->item_film32.clear();
   B->sblk_film.clear();
   long long off = 0, hoff = 0;
   for (int f = 0; f < F; ++f) {
@@ -191,11 +199,9 @@ int build_layout(tdgl_batch* B) {
     d.item0 = (int)B->item_film.size();
     d.nitems = d.tiles_x * d.tiles_y * d.tz;
     B->item_film.insert(B->item_film.end(), d.nitems, f);
-    d.tiles32_x = (c->g.nx + sweep_tx(B->shape) - 1) / sweep_tx(B->shape);
-    d.tiles32_y = (c->g.ny + sweep_ty(B->shape) - 1) / sweep_ty(B->shape);
-    d.item32_0 = (int)B->item_film32.size();
-    d.nitems32 = d.tiles32_x * d.tiles32_y * d.tz;
-    B->item_film32.insert(B->item_film32.end(), d.nitems32, f);
+    d.shape32 = B->shape == kSweepShapeFilm ? sweep_film_shape(c->g.nx, c->g.ny) : B->shape;
+    d.tiles32_x = (c->g.nx + sweep_tx(d.shape32) - 1) / sweep_tx(d.shape32);
+    d.tiles32_y = (c->g.ny + sweep_ty(d.shape32) - 1) / sweep_ty(d.shape32);
     d.sb0 = (int)B->sblk_film.size();
     d.nsb = (int)std::min<long long>(kStreamMaxPerFilm, std::max<long long>(1, (c->len + kStreamChunk - 1) / kStreamChunk));
     B->sblk_film.insert(B->sblk_film.end(), d.nsb, f);
@@ -210,21 +216,18 @@ int build_layout(tdgl_batch* B) {
     B->body = nullptr;
   }
   B->nitems = (int)B->item_film.size();
-  B->nitems32 = (int)B->item_film32.size();
   B->nsblk = (int)B->sblk_film.size();
-  for (void* p : {(void*)B->d_item_film, (void*)B->d_item_film32, (void*)B->d_sblk_film, (void*)B->d_pitem,
-                  (void*)B->d_pm})
+  B->part_key.clear();  // the sweep tiles may have changed
+  for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
     if (p) BCHK(hipFree(p));
-  B->d_item_film = B->d_item_film32 = B->d_sblk_film = nullptr;
+  B->d_item_film = B->d_sblk_film = nullptr;
   B->d_pitem = B->d_pm = nullptr;
   int rc;
   if ((rc = balloc(B, (void**)&B->d_item_film, sizeof(int) * B->nitems))) return rc;
-  if ((rc = balloc(B, (void**)&B->d_item_film32, sizeof(int) * B->nitems32))) return rc;
   if ((rc = balloc(B, (void**)&B->d_sblk_film, sizeof(int) * B->nsblk))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pitem, sizeof(double) * 3 * (size_t)B->nitems))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pm, sizeof(double) * (kMaxKrylov + 2) * (size_t)B->nsblk))) return rc;
   BCHK(hipMemcpy(B->d_item_film, B->item_film.data(), sizeof(int) * B->nitems, hipMemcpyHostToDevice));
-  BCHK(hipMemcpy(B->d_item_film32, B->item_film32.data(), sizeof(int) * B->nitems32, hipMemcpyHostToDevice));
   BCHK(hipMemcpy(B->d_sblk_film, B->sblk_film.data(), sizeof(int) * B->nsblk, hipMemcpyHostToDevice));
   return TDGL_OK;
 }
@@ -282,6 +285,99 @@ int ensure_q(tdgl_batch* B, int i) {
 int set_act(tdgl_batch* B, const std::vector<int>& act) {
   memcpy(B->h_act, act.data(), sizeof(int) * B->nfilm);
   BCHK(hipMemcpyAsync(B->d_act, B->h_act, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->ps));
+  return TDGL_OK;
+}
+
+// One sweep's pieces: every FP32 tile column of the films in `in`, cut along z
+// into near-equal pieces of at most Lp planes, one block per piece.  Lp adapts
+// to the sweep's work W (tile-planes): W / (the blocks the device holds at
+// once), within [lmin, lmax] -- a sweep over every film gets deep pieces (few
+// restaged halo planes), a sweep over a few films shallow ones (every CU
+// busy).  The pieces are listed film by film (within a film, neighbouring
+// tiles of one z piece next to each other: their shared halo cells meet in
+// L2), and the sweep kernel gives each XCD one contiguous run of the list
+// (xcd_remap); inside each run the deeper pieces come first, so each XCD's
+// dispatcher starts its longest blocks first.
+void part_build(const tdgl_batch* B, const std::vector<char>& in, long long gres, int lmin, int lmax,
+                std::vector<int4>& seg, int& nblocks) {
+  long long W = 0;
+  for (int f = 0; f < B->nfilm; ++f)
+    if (in[f]) W += (long long)B->hf[f].tiles32_x * B->hf[f].tiles32_y * (B->is3d ? B->hf[f].g.nz : 1);
+  nblocks = 0;
+  if (W == 0) return;
+  const long long Lp = std::max<long long>(lmin, std::min<long long>(lmax, (W + gres - 1) / gres));
+  const size_t seg0 = seg.size();
+  for (int f = 0; f < B->nfilm; ++f) {
+    if (!in[f]) continue;
+    const FilmDesc& d = B->hf[f];
+    const int cols = d.tiles32_x * d.tiles32_y;
+    if (!B->is3d) {
+      for (int t = 0; t < cols; ++t) seg.push_back(make_int4(f, t, 0, 1));
+      continue;
+    }
+    const int nz = d.g.nz, np = (int)((nz + Lp - 1) / Lp);
+    for (int q = 0; q < np; ++q) {
+      const int k0 = 1 + (int)((long long)nz * q / np), k1 = 1 + (int)((long long)nz * (q + 1) / np);
+      for (int t = 0; t < cols; ++t) seg.push_back(make_int4(f, t, k0, k1));
+    }
+  }
+  nblocks = (int)(seg.size() - seg0);
+  // the runs of xcd_remap (tdgl_device.hpp): XCD x holds per or per - 1 blocks
+  const int G = nblocks, nx = 8, per = (G + nx - 1) / nx, full = G % nx == 0 ? nx : G % nx;
+  for (int x = 0, base = 0; x < nx; ++x) {
+    const int len = x < full ? per : per - 1;
+    std::stable_sort(seg.begin() + seg0 + base, seg.begin() + seg0 + base + len,
+                     [](const int4& a, const int4& b) { return a.w - a.z > b.w - b.z; });
+    base += len;
+  }
+}
+
+// The sweep pieces for the films in `act` at the current Chebyshev degrees
+// (rebuilt only when the films, degrees, tile shape or tuning change).
+int ensure_parts(tdgl_batch* B, const std::vector<int>& act) {
+  const int F = B->nfilm, D = B->cheb_deg;
+  std::vector<int> key = {B->shape, B->part_lmax, B->part_lmin, D};
+  for (int f = 0; f < F; ++f) key.push_back(act[f] ? 1 + (D ? B->cheb_degf[f] : 0) : 0);
+  if (key == B->part_key) return TDGL_OK;
+  const long long gres = (long long)B->cus * sweep_blocks_per_cu(B->is3d, B->shape, D > 0);
+  std::vector<int4> seg;
+  std::vector<int> g0(D + 1), nb(D + 1);
+  for (int m = 0; m <= D; ++m) {
+    std::vector<char> in(F);
+    for (int f = 0; f < F; ++f) in[f] = act[f] && (m == 0 || B->cheb_degf[f] >= m);
+    g0[m] = (int)seg.size();
+    part_build(B, in, gres, B->part_lmin, B->part_lmax, seg, nb[m]);
+    // every tile-plane of every film taking part exactly once, every piece
+    // inside its film's tiles and planes
+    std::vector<long long> cover(F, 0);
+    bool ok = (int)seg.size() - g0[m] == nb[m];
+    for (int q = g0[m]; ok && q < (int)seg.size(); ++q) {
+      const int4 v = seg[q];
+      ok = v.x >= 0 && v.x < F && in[v.x];
+      if (!ok) break;
+      const FilmDesc& d = B->hf[v.x];
+      const int k0 = B->is3d ? 1 : 0, k1 = B->is3d ? d.g.Nz : 1;
+      ok = v.y >= 0 && v.y < d.tiles32_x * d.tiles32_y && v.z >= k0 && v.z < v.w && v.w <= k1;
+      cover[v.x] += v.w - v.z;
+    }
+    for (int f = 0; ok && f < F; ++f)
+      ok = cover[f] == (in[f] ? (long long)B->hf[f].tiles32_x * B->hf[f].tiles32_y *
+                                    (B->is3d ? B->hf[f].g.nz : 1) : 0LL);
+    if (!ok) return bfail(B, TDGL_E_INVALID, "internal error: inconsistent sweep pieces");
+  }
+  BCHK(hipStreamSynchronize(B->ps));  // no queued sweep still reads the old tables
+  if (seg.size() > B->pseg_cap) {
+    if (B->d_pseg) BCHK(hipFree(B->d_pseg));
+    B->d_pseg = nullptr;
+    int rc;
+    if ((rc = balloc(B, (void**)&B->d_pseg, sizeof(int4) * seg.size()))) return rc;
+    B->pseg_cap = seg.size();
+  }
+  if (!seg.empty()) BCHK(hipMemcpy(B->d_pseg, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice));
+  B->part.assign(D + 1, SweepPart{});
+  for (int m = 0; m <= D; ++m) B->part[m] = SweepPart{B->d_pseg + g0[m], nb[m]};
+  B->part_key = key;
+  ++B->part_gen;
   return TDGL_OK;
 }
 
@@ -394,7 +490,7 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
   if (d == 0) {
     const double S = prof ? sum_over(B, act, s32) : 0.0;
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_JVP, 3.0 * S, 1, true) : -1;
-    launch_bjvp32(bk, B->is3d, x32, v, B->d_qtab, B->d_iter, out, B->ps);
+    launch_bjvp32(bk, B->part[0], B->is3d, x32, v, B->d_qtab, B->d_iter, out, B->ps);
     if (prof) B->prof->end(B->ps, tok);
     return;
   }
@@ -415,7 +511,7 @@ void enqueue_w(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
         if (act[f] && B->cheb_degf[f] >= m)
           by += s32(B, f) * (3.0 + (m >= 2 ? 1.0 : 0.0) + (m == B->cheb_degf[f] && m >= 3 ? 1.0 : 0.0));
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
-    launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+    launch_bcheb32(bk, B->part[m], B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }
 }
@@ -445,7 +541,7 @@ void enqueue_z(tdgl_batch* B, bool prof, const std::vector<int>& act, const floa
       for (int f = 0; f < B->nfilm; ++f)
         if (act[f] && B->cheb_degf[f] >= m) by += s32(B, f) * (3.0 + (m >= 3 ? 1.0 : 0.0) + (m >= 4 ? 1.0 : 0.0));
     const int tok = prof ? B->prof->begin(B->ps, TDGL_K_CHEB, by, 1, true) : -1;
-    launch_bcheb32(bk, B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
+    launch_bcheb32(bk, B->part[m], B->is3d, x32, src, B->d_qtab, B->d_iter, cs, B->ps);
     if (prof) B->prof->end(B->ps, tok);
   }
 }
@@ -520,7 +616,8 @@ int launch_body(tdgl_batch* B, double tol) {
     BCHK(hipGetLastError());
     return TDGL_OK;
   }
-  if (B->body && (B->body_tol != tol || B->body_deg != B->cheb_deg || B->body_gen != B->gen)) {
+  if (B->body && (B->body_tol != tol || B->body_deg != B->cheb_deg || B->body_gen != B->gen ||
+                  B->body_part != B->part_gen)) {
     BCHK(hipGraphExecDestroy(B->body));
     B->body = nullptr;
   }
@@ -537,6 +634,7 @@ int launch_body(tdgl_batch* B, double tol) {
     B->body_tol = tol;
     B->body_deg = B->cheb_deg;
     B->body_gen = B->gen;
+    B->body_part = B->part_gen;
   }
   BCHK(hipGraphLaunch(B->body, B->ps));
   return TDGL_OK;
@@ -668,6 +766,7 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
   std::vector<int> act(F);
   for (int f = 0; f < F; ++f) act[f] = active ? (active[f] != 0) : 1;
   if ((rc = setup_cheb(B, o->mixed_precond, dtv, act))) return rc;
+  if ((rc = ensure_parts(B, act))) return rc;
 
   // predictor + gamma (integrators.py:156-158)
   if ((rc = set_act(B, act))) return rc;
@@ -824,6 +923,11 @@ int batch_init(tdgl_batch* B) {
   BCHK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
   BCHK(hipEventCreateWithFlags(&B->ev_act[0], hipEventDisableTiming));
   BCHK(hipEventCreateWithFlags(&B->ev_act[1], hipEventDisableTiming));
+  {
+    int dev = 0;
+    BCHK(hipGetDevice(&dev));
+    BCHK(hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   B->kz.resize(F);
   B->kcap.resize(F);
   if (nfilm_is_own(B)) B->shape = B->films[0]->tile_user;
@@ -893,7 +997,12 @@ int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, f
   if ((rc = set_act(B, one))) return rc;
   if (hipMemcpyAsync(B->d_dtv, dtv, sizeof(dtv), hipMemcpyHostToDevice, B->ps) != hipSuccess)
     return TDGL_E_HIP;
-  launch_bjvp32(kview(B), B->is3d, x32, p32, nullptr, nullptr, ap32, B->ps);
+  B->cheb_deg = 0;
+  if ((rc = ensure_parts(B, one))) {
+    ctx->err = B->err;
+    return rc;
+  }
+  launch_bjvp32(kview(B), B->part[0], B->is3d, x32, p32, nullptr, nullptr, ap32, B->ps);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess) return TDGL_E_HIP;
   return sync_out(B);
 }
@@ -917,6 +1026,7 @@ int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt,
   double iv[2] = {0.0, 0.0};
   if ((rc = setup_cheb(B, degree, dtv, one, iv))) return done(rc);
   if (B->cheb_deg == 0) return done(bfail(B, TDGL_E_INVALID, "no preconditioner for this degree and dt"));
+  if ((rc = ensure_parts(B, one))) return done(rc);
   enqueue_w(B, false, one, x32, v32, w32);
   enqueue_z(B, false, one, x32, v32);
   const float* z = B->cheb_deg >= 2 ? B->t32[B->cheb_deg % 2] : nullptr;  // one film: its own degree
@@ -964,7 +1074,8 @@ int tdgl_batch_destroy(tdgl_batch* B) {
   for (hipEvent_t e : {B->ev_in, B->ev_out, B->ev_act[0], B->ev_act[1]})
     if (e) (void)hipEventDestroy(e);
   if (B->ps) (void)hipStreamDestroy(B->ps);
-  for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_item_film32, (void*)B->d_sblk_film,
+  for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_pseg,
+                  (void*)B->d_sblk_film,
                   (void*)B->d_act,
                   (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_ext, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
@@ -1008,6 +1119,20 @@ int tdgl_batch_get_kz(const tdgl_batch* B, int f) {
   return B->kz[f];
 }
 
+int tdgl_batch_set_sweep_split(tdgl_batch* B, int max_planes, int min_planes) {
+  if (!B || min_planes < 1 || max_planes < min_planes || max_planes > 4096) return TDGL_E_INVALID;
+  B->part_lmax = max_planes;
+  B->part_lmin = min_planes;
+  B->part_key.clear();
+  return TDGL_OK;
+}
+int tdgl_batch_get_sweep_split(const tdgl_batch* B, int* max_planes, int* min_planes) {
+  if (!B) return TDGL_E_INVALID;
+  if (max_planes) *max_planes = B->part_lmax;
+  if (min_planes) *min_planes = B->part_lmin;
+  return TDGL_OK;
+}
+
 int tdgl_batch_set_sweep_tile(tdgl_batch* B, int shape) {
   if (!B || shape < 0 || shape >= kSweepShapes) return TDGL_E_INVALID;
   B->shape = shape;
@@ -1027,13 +1152,15 @@ int tdgl_batch_time_stencil(tdgl_batch* B, int reps, double* ms) {
   std::vector<double> dtv(2 * B->nfilm, 0.1);
   if ((rc = set_act(B, all))) return rc;
   BCHK(hipMemcpyAsync(B->d_dtv, dtv.data(), sizeof(double) * dtv.size(), hipMemcpyHostToDevice, B->ps));
+  B->cheb_deg = 0;  // the plain JVP sweep over every film (the next step sets the degrees again)
+  if ((rc = ensure_parts(B, all))) return rc;
   hipEvent_t a, b;
   BCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
   BCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
-  launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);  // warm-up
+  launch_bjvp32(kview(B), B->part[0], B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);  // warm-up
   BCHK(hipEventRecord(a, B->ps));
   for (int r = 0; r < reps; ++r)
-    launch_bjvp32(kview(B), B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);
+    launch_bjvp32(kview(B), B->part[0], B->is3d, B->x32, B->r32, nullptr, nullptr, B->Q[0], B->ps);
   BCHK(hipEventRecord(b, B->ps));
   BCHK(hipEventSynchronize(b));
   float t = 0.f;

@@ -70,10 +70,13 @@ __device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70);
 // per-cell rules (stage32), every other film's by the mask plans (stage32u),
 // which give the same bits.
 template <bool IS3D, bool CHEB, int TS, bool GEN>
-__global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, const float* p32,
-                                               float* const* Q, const int* iter, float* out_direct,
-                                               ChebStep cs) {
-  constexpr int TX = sweep_tx(TS), TY = sweep_ty(TS);
+__global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, const float* x32,
+                                               const float* p32, float* const* Q, const int* iter,
+                                               float* out_direct, ChebStep cs) {
+  // TS = kSweepShapeFilm: each film's own shape (FilmDesc::shape32), the
+  // shared memory sized for the largest (64 x 4: 66 x 6 staged cells)
+  constexpr bool RT = TS == kSweepShapeFilm;
+  constexpr int TX = RT ? 64 : sweep_tx(TS), TY = RT ? 4 : sweep_ty(TS);
   static_assert(TX * TY == BS, "a sweep tile holds one node per thread and plane");
   constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
   // Shared-memory rings of staged planes.  Plane k + 1 is committed at the top
@@ -107,11 +110,14 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   float(&X_y)[NS][CELLS] = S.xy;
   uint8_t(&H)[NS][CELLS] = S.h;
 
+  // The block's piece of the sweep (film, tile column, z planes [kb, ke)):
+  // SweepPart sizes the pieces to the sweep's work.
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int f = bk.item_film32[L];
+  const int4 seg = part.seg[L];
+  const int f = seg.x;
   if (!bk.act[f]) return;
   const FilmDesc* fd = bk.films + f;
-  const Geo g = fd->g;  // by value: held in registers for the whole item
+  const Geo g = fd->g;  // by value: held in registers for the whole piece
   const Bnd b = fd->b;
   const Mat mt = fd->mt;
   const long long off = fd->off;
@@ -137,21 +143,20 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     if (has_v) CV = planes32(cs.v + off, n);
     if (has_p) CP = planes32(cs.prev + off, n);
   }
-  const int li = (int)L - fd->item32_0;
-  const int tile_x = li % fd->tiles32_x, rest = li / fd->tiles32_x;
-  const int tile_y = rest % fd->tiles32_y, zc = rest / fd->tiles32_y;
-  const int i0 = 1 + tile_x * TX, j0 = 1 + tile_y * TY;
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int tile_x = seg.y % fd->tiles32_x, tile_y = seg.y / fd->tiles32_x;
+  // the film's tile: TXr x TYr nodes, CXr staged cells per row
+  const int tsh = RT ? fd->shape32 : 0;
+  const int TXr = RT ? (64 >> tsh) : TX, TYr = RT ? (4 << tsh) : TY, CXr = TXr + 2;
+  const int cellsr = RT ? CXr * (TYr + 2) : CELLS;
+  const int i0 = 1 + tile_x * TXr, j0 = 1 + tile_y * TYr;
+  const int tx = RT ? (int)(threadIdx.x & (unsigned)(TXr - 1)) : (int)(threadIdx.x % TX);
+  const int ty = RT ? (int)(threadIdx.x >> (6 - tsh)) : (int)(threadIdx.x / TX);
   const int i = i0 + tx, j = j0 + ty;
   const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
-  int kb = 0, ke = 1;
-  if (IS3D) {
-    kb = 1 + zc * fd->kz;
-    ke = min(kb + fd->kz, g.Nz);
-  }
+  const int kb = seg.z, ke = seg.w;
 
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;
-  const bool has1 = hc1 < CELLS;
+  const bool has1 = hc1 < cellsr;
   // the columns this thread stages on every plane, as mask plans (uniform
   // field); a general applied field takes the per-cell rules, the column's
   // booleans recomputed on every plane (held across the march, they would
@@ -159,8 +164,8 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
   // plane ahead; the mask rules apply when the cells go to shared memory, so
   // the loads' latency overlaps the plane of arithmetic in between.
   const Fld32 F = fld32(b, &fd->b);
-  const Plan32 pl0 = plan32(g, F, i0 - 1 + hc0 % CX, j0 - 1 + hc0 / CX);
-  const Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
+  const Plan32 pl0 = plan32(g, F, i0 - 1 + hc0 % CXr, j0 - 1 + hc0 / CXr);
+  const Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CXr, j0 - 1 + hc1 / CXr);
   const bool hf = mt.flags != nullptr;
   auto load_into = [&](int kp, Raw32& a0, Raw32& a1) __attribute__((always_inline)) {
     if (GEN && F.gen) {  // general field: the rules at once, the cells kept in the load slots
@@ -178,12 +183,12 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
       opaque(h0);
       opaque(h1);
       Cell<float> v, w;
-      const Col32 c0 = col32(g, i0 - 1 + h0 % CX, j0 - 1 + h0 / CX);
+      const Col32 c0 = col32(g, i0 - 1 + h0 % CXr, j0 - 1 + h0 / CXr);
       const unsigned int moff = IS3D ? (unsigned int)g.mk * (unsigned int)kp : 0u;
       stage32<IS3D>(g, F, mt.flags, X, P, c0, kp, v, w);
       pack(a0, v, w);
       a0.h = hf ? (unsigned int)ldu(mt.flags, c0.mij + moff) : 0u;
-      const Col32 c1 = col32(g, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX);
+      const Col32 c1 = col32(g, i0 - 1 + h1 % CXr, j0 - 1 + h1 / CXr);
       stage32<IS3D>(g, F, mt.flags, X, P, c1, kp, v, w);
       pack(a1, v, w);
       a1.h = hf ? (unsigned int)ldu(mt.flags, c1.mij + moff) : 0u;
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
     // own-node inputs (a lane past the grid's edge reads the nearest node and
     // stores nothing), then the halo two planes ahead
     const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
-    const int t0 = k % NS, c0 = (ty + 1) * CX + (tx + 1);
+    const int t0 = k % NS, c0 = (ty + 1) * CXr + (tx + 1);
     const uint8_t fl = H[t0][c0];
     float scv = 1.f, k2v = 0.f;
     if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
@@ -296,7 +301,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
 
     if (act) {
       const int s0 = k % RING, sm = (k + RING - 1) % RING, sp1 = (k + 1) % RING;
-      const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CX, cyp = c0 + CX;
+      const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CXr, cyp = c0 + CXr;
       const float pr = X_r[s0][c0], pi = X_i[s0][c0];
       const float dr = P_r[s0][c0], di = P_i[s0][c0];
       const float ex = X_x[t0][c0], ey = X_y[t0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
@@ -369,7 +374,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, const float* x32, c
                  cy * (P_x[s0][cym] - 2.f * dex + P_x[s0][cyp]);
       float jy = 2.f * cy * (P_y[s0][cym] - 2.f * dey + P_y[s0][cyp]) +
                  cx * (P_y[s0][cxm] - 2.f * dey + P_y[s0][cxp]);
-      jx += cy * (((-P_y[s0][cxp] + dey) + P_y[s0][cxp - CX]) - P_y[s0][cym]);
+      jx += cy * (((-P_y[s0][cxp] + dey) + P_y[s0][cxp - CXr]) - P_y[s0][cym]);
       jy += cx * (((-P_x[s0][cyp] + dex) + P_x[s0][cyp - 1]) - P_x[s0][cxm]);
       float jz = 0.f;
       if (IS3D) {
@@ -842,31 +847,56 @@ void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const d
 }
 
 template <bool IS3D, bool CHEB, bool GEN>
-void launch_sweep_g(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,
-                    float* out, const ChebStep& cs, hipStream_t st) {
-  const dim3 gr(bk.nitems32), bl(BS);
+void launch_sweep_g(const BatchK& bk, const SweepPart& part, const float* x32, const float* p32,
+                    float* const* Q, const int* iter, float* out, const ChebStep& cs, hipStream_t st) {
+  const dim3 gr(part.nblocks), bl(BS);
   switch (bk.shape) {
-    case 1: launch((k_bjvp32<IS3D, CHEB, 1, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    case 2: launch((k_bjvp32<IS3D, CHEB, 2, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
-    default: launch((k_bjvp32<IS3D, CHEB, 0, GEN>), gr, bl, 0, st, bk, x32, p32, Q, iter, out, cs); break;
+    case 1: launch((k_bjvp32<IS3D, CHEB, 1, GEN>), gr, bl, 0, st, bk, part, x32, p32, Q, iter, out, cs); break;
+    case 2: launch((k_bjvp32<IS3D, CHEB, 2, GEN>), gr, bl, 0, st, bk, part, x32, p32, Q, iter, out, cs); break;
+    case kSweepShapeFilm:
+      launch((k_bjvp32<IS3D, CHEB, kSweepShapeFilm, GEN>), gr, bl, 0, st, bk, part, x32, p32, Q, iter, out, cs);
+      break;
+    default: launch((k_bjvp32<IS3D, CHEB, 0, GEN>), gr, bl, 0, st, bk, part, x32, p32, Q, iter, out, cs); break;
   }
 }
 template <bool IS3D, bool CHEB>
-void launch_sweep(const BatchK& bk, const float* x32, const float* p32, float* const* Q, const int* iter,
-                  float* out, const ChebStep& cs, hipStream_t st) {
-  if (bk.gen) launch_sweep_g<IS3D, CHEB, true>(bk, x32, p32, Q, iter, out, cs, st);
-  else launch_sweep_g<IS3D, CHEB, false>(bk, x32, p32, Q, iter, out, cs, st);
+void launch_sweep(const BatchK& bk, const SweepPart& part, const float* x32, const float* p32,
+                  float* const* Q, const int* iter, float* out, const ChebStep& cs, hipStream_t st) {
+  if (part.nblocks <= 0) return;
+  if (bk.gen) launch_sweep_g<IS3D, CHEB, true>(bk, part, x32, p32, Q, iter, out, cs, st);
+  else launch_sweep_g<IS3D, CHEB, false>(bk, part, x32, p32, Q, iter, out, cs, st);
 }
-void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
-                   const int* iter, float* out_direct, hipStream_t st) {
+void launch_bjvp32(const BatchK& bk, const SweepPart& part, bool is3d, const float* x32, const float* p32,
+                   float* const* Q, const int* iter, float* out_direct, hipStream_t st) {
   const ChebStep none{};
-  if (is3d) launch_sweep<true, false>(bk, x32, p32, Q, iter, out_direct, none, st);
-  else launch_sweep<false, false>(bk, x32, p32, Q, iter, out_direct, none, st);
+  if (is3d) launch_sweep<true, false>(bk, part, x32, p32, Q, iter, out_direct, none, st);
+  else launch_sweep<false, false>(bk, part, x32, p32, Q, iter, out_direct, none, st);
 }
-void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
-                    const int* iter, const ChebStep& cs, hipStream_t st) {
-  if (is3d) launch_sweep<true, true>(bk, x32, p32, Q, iter, nullptr, cs, st);
-  else launch_sweep<false, true>(bk, x32, p32, Q, iter, nullptr, cs, st);
+void launch_bcheb32(const BatchK& bk, const SweepPart& part, bool is3d, const float* x32, const float* p32,
+                    float* const* Q, const int* iter, const ChebStep& cs, hipStream_t st) {
+  if (is3d) launch_sweep<true, true>(bk, part, x32, p32, Q, iter, nullptr, cs, st);
+  else launch_sweep<false, true>(bk, part, x32, p32, Q, iter, nullptr, cs, st);
+}
+int sweep_blocks_per_cu(bool is3d, int shape, bool cheb) {
+  int n = 0;
+  hipError_t e;
+  auto q = [&](auto kern) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, BS, 0); };
+  if (is3d) {
+    if (cheb) {
+      if (shape == 1) q(k_bjvp32<true, true, 1, false>);
+      else if (shape == 2) q(k_bjvp32<true, true, 2, false>);
+      else if (shape == kSweepShapeFilm) q(k_bjvp32<true, true, kSweepShapeFilm, false>);
+      else q(k_bjvp32<true, true, 0, false>);
+    } else {
+      if (shape == 1) q(k_bjvp32<true, false, 1, false>);
+      else if (shape == 2) q(k_bjvp32<true, false, 2, false>);
+      else if (shape == kSweepShapeFilm) q(k_bjvp32<true, false, kSweepShapeFilm, false>);
+      else q(k_bjvp32<true, false, 0, false>);
+    }
+  } else {
+    q(k_bjvp32<false, true, 0, false>);
+  }
+  return (e == hipSuccess && n > 0) ? n : 1;
 }
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st) {

@@ -58,10 +58,29 @@ constexpr int kMaxKrylov = 2048;
 // picks).  Inside a solve, 64 x 4 was the fastest on single films (128x128x16:
 // 15.5 us against 19.1 us per Chebyshev sweep for 32 x 8; equal on 256x256x64),
 // although 32 x 8 wins when the sweep is timed alone on cache-resident vectors.
-constexpr int kSweepShapes = 3;
+// FP32 sweep tile shapes: 0 = 64 x 4, 1 = 32 x 8, 2 = 16 x 16 nodes per plane;
+// kSweepShapeFilm = each film its own (sweep_film_shape)
+constexpr int kSweepShapeFilm = 3;
+constexpr int kSweepShapes = 4;
 constexpr int kDefaultSweepShape = 0;
 __host__ __device__ constexpr int sweep_tx(int s) { return s == 1 ? 32 : (s == 2 ? 16 : 64); }
 __host__ __device__ constexpr int sweep_ty(int s) { return s == 1 ? 8 : (s == 2 ? 16 : 4); }
+// The fixed shape of a film under kSweepShapeFilm: the one with the least
+// work per plane, counting each tile as its 256 lanes (idle ones included)
+// plus a third of a lane per staged cell ((TX + 2)(TY + 2) per tile).
+inline int sweep_film_shape(long long nx, long long ny) {
+  int best = 0;
+  double bc = 0.0;
+  for (int s = 0; s < 3; ++s) {
+    const long long tx = (nx + sweep_tx(s) - 1) / sweep_tx(s), ty = (ny + sweep_ty(s) - 1) / sweep_ty(s);
+    const double c = (double)(tx * ty) * (256.0 + (sweep_tx(s) + 2) * (sweep_ty(s) + 2) / 3.0);
+    if (s == 0 || c < bc) {
+      best = s;
+      bc = c;
+    }
+  }
+  return best;
+}
 // Chebyshev polynomial preconditioner of the mixed TGCR: largest degree and
 // the per-film coefficient stride (1/theta, then omega_1 .. omega_d).
 constexpr int kMaxChebDeg = 64;
@@ -130,7 +149,6 @@ struct Epi {
 struct BatchK {
   const FilmDesc* films;
   const int* item_film;  // FP64 stencil block -> film
-  const int* item_film32;  // FP32 sweep block -> film
   const int* sblk_film;  // streaming block -> film
   int* act;              // per film: takes part in the current stage
   double* res;           // nfilm * kResLen scalar slots
@@ -138,10 +156,17 @@ struct BatchK {
   const double* cheb;    // per film, kChebStride doubles: 1/theta, omega_1 .. omega_d
   int nfilm;
   int nitems;            // FP64 stencil blocks of the batch
-  int nitems32;          // FP32 sweep blocks of the batch
   int shape;             // FP32 sweep tile shape (sweep_tx / sweep_ty)
   int gen;               // some film has general applied-field vectors (per-cell ghost rules)
   int nsblk;             // streaming blocks of the batch
+};
+
+// Work of one FP32 sweep launch: block b runs the piece seg[b] = (film, tile
+// column = tile_x + tiles32_x * tile_y, first plane, end plane); the launch
+// has nblocks blocks.
+struct SweepPart {
+  const int4* seg;
+  int nblocks;
 };
 
 struct StencilCfg {
@@ -186,8 +211,10 @@ void launch_bfin(const BatchK& bk, int nsum, int nmax, const double* pin, int sl
 void launch_breport(const BatchK& bk, const int* fstat, double* out, hipStream_t st);
 constexpr int kReportLen = 10;  // doubles per film in the report
 // Mixed-precision TGCR kernels (tdgl_f32.hip).
-void launch_bjvp32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
-                   const int* iter, float* out_direct, hipStream_t st);
+void launch_bjvp32(const BatchK& bk, const SweepPart& part, bool is3d, const float* x32, const float* p32,
+                   float* const* Q, const int* iter, float* out_direct, hipStream_t st);
+// Resident blocks per CU of the FP32 sweep kernel (shape, preconditioner epilogue or not).
+int sweep_blocks_per_cu(bool is3d, int shape, bool cheb);
 // One step of a Chebyshev recurrence for A = I - dt/2 J(x) (the stencil of
 // k_bjvp32 with another epilogue).  With y = s p (s = 1/theta when src_th,
 // else 1) and the film's omega_m, 1/theta:
@@ -207,8 +234,8 @@ struct ChebStep {
   int m;
   int src_th, prev_th, cv, w;
 };
-void launch_bcheb32(const BatchK& bk, bool is3d, const float* x32, const float* p32, float* const* Q,
-                    const int* iter, const ChebStep& cs, hipStream_t st);
+void launch_bcheb32(const BatchK& bk, const SweepPart& part, bool is3d, const float* x32, const float* p32,
+                    float* const* Q, const int* iter, const ChebStep& cs, hipStream_t st);
 // out = x + ratio (x - xp): the extrapolating predictor (ratio[f] <= 0: film
 // f keeps out as it is)
 void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,

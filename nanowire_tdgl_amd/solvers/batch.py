@@ -58,65 +58,66 @@ class BatchRunner:
         for d, e, t in zip(self.devices, self.engines, times):
             e.set_field(*d.applied_field.evaluate(t, t_stop))
 
-    def tune_key(self, candidates, shapes) -> str:
+    def tune_key(self, splits, shapes) -> str:
         """Cache key of a tuning: the GPU, the library build, the precision and
         every film's grid (Nx, Ny, Nz), with the candidates searched."""
         import torch
         dims = [[e.params.Nx, e.params.Ny, e.params.Nz] for e in self.engines]
         blob = json.dumps([torch.cuda.get_device_name(self.batch.device), _native.library_digest(),
-                           "mixed", dims, list(candidates), list(shapes)])
+                           "mixed", dims, [list(x) for x in splits], list(shapes)])
         return hashlib.sha256(blob.encode()).hexdigest()[:32]
 
-    def _apply(self, kzs, shape) -> None:
+    def _apply(self, shape, split) -> None:
         self.batch.set_tile(int(shape))
-        for f, k in enumerate(kzs):
-            self.batch.set_kz(f, int(k))
+        self.batch.set_split(int(split[0]), int(split[1]))
 
-    def tune(self, X=None, candidates=(1, 2, 3, 4, 6, 8, 12, 16, 24, 32), shapes=(0, 1, 2),
+    def tune(self, X=None, splits=((8, 2), (8, 4), (6, 2), (12, 2), (16, 4)), shapes=(0, 1, 2, 3),
              reps: int = 5, cache: Optional[str] = "default") -> dict:
-        """Autotuner: pick the tile of the mixed-precision sweeps -- its shape
-        (0 = 64x4, 1 = 32x8, 2 = 16x16 nodes per plane) and its z depth kz.
+        """Autotuner: pick the mixed-precision sweeps' tile shape (0 = 64x4,
+        1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film the shape with the
+        fewest idle lanes and staged cells) and work split (the deepest and
+        the shallowest z piece per block; tdgl_batch_set_sweep_split).
 
-        Every shape with every candidate depth policy kz_f = min(K, nz_f), and
-        the current setting, is timed on the batch's own FP32 JVP stencil; the
-        fastest is kept.  The pick is cached in a JSON file (``cache``: a path,
-        "default" = $NANOWIRE_TDGL_AMD_TUNE_CACHE or
+        Every shape with every split, and the current setting, is timed on the
+        batch's own FP32 JVP sweep over all films; the fastest is kept (every
+        setting gives the same bits).  The pick is cached in a JSON file
+        (``cache``: a path, "default" = $NANOWIRE_TDGL_AMD_TUNE_CACHE or
         ~/.cache/nanowire_tdgl_amd/autotune.json, None = no cache) under
-        :meth:`tune_key`; a later tuning of the same films on the same GPU
-        reads it instead of timing.  Returns {"kz": [...], "shape": s, "ms":
-        best, "trials": {"<shape>:<K>": ms}, "cached": bool}."""
+        :meth:`tune_key`; a later tuning of the same films on the same GPU reads
+        it instead of timing.  Returns {"shape": s, "split": [max, min], "kz": [...]
+        (the FP64 stencils' z depths, not tuned), "ms": best, "trials":
+        {"<shape>:<g>x<l>": ms}, "cached": bool}."""
+        b = self.batch
+        F = b.nfilm
+        kz = [b.get_kz(f) for f in range(F)]
         path = _cache_path(cache)
-        key = self.tune_key(candidates, shapes) if path else None
+        key = self.tune_key(splits, shapes) if path else None
         if path and path.exists():
             try:
                 hit = json.loads(path.read_text()).get(key)
             except (OSError, ValueError):
                 hit = None
-            if hit and len(hit["kz"]) == self.batch.nfilm:
-                self._apply(hit["kz"], hit["shape"])
-                return dict(hit, cached=True)
-        b = self.batch
-        F = b.nfilm
-        names = {0: "64x4", 1: "32x8", 2: "16x16"}
-        best = ([b.get_kz(f) for f in range(F)], b.get_tile())
+            if hit and "split" in hit:
+                self._apply(hit["shape"], hit["split"])
+                return dict(hit, kz=kz, cached=True)
+        names = {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}
+        best = (b.get_tile(), b.get_split())
         best_ms = b.time_stencil(reps)
         trials = {"default": best_ms}
         for sh in shapes:
             b.set_tile(sh)
-            for K in candidates:
-                kzs = [max(1, min(K, e.params.Nz - 1 if e.params.Nz > 1 else 1)) for e in self.engines]
-                for f, k in enumerate(kzs):
-                    b.set_kz(f, k)
+            for g, l in splits:
+                b.set_split(g, l)
                 ms = b.time_stencil(reps)
-                trials[f"{names[sh]}:{K}"] = ms
+                trials[f"{names[sh]}:{g}x{l}"] = ms
                 if ms < best_ms:
-                    best_ms, best = ms, (kzs, sh)
-        kzs, sh = best
-        self._apply(kzs, sh)
-        res = {"kz": [int(k) for k in kzs], "shape": int(sh), "ms": best_ms, "trials": trials}
+                    best_ms, best = ms, (sh, (g, l))
+        self._apply(*best)
+        res = {"shape": int(best[0]), "split": [int(best[1][0]), int(best[1][1])], "ms": best_ms,
+               "trials": trials}
         if path:
             _cache_store(path, key, res)
-        return dict(res, cached=False)
+        return dict(res, kz=kz, cached=False)
 
 
 def _cache_path(cache) -> Optional[Path]:

@@ -207,7 +207,7 @@ def test_stencil_depth_does_not_change_the_jvp(tmp_path):
     x = 0.5 + 0.1 * torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32)
     p = torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32) - 0.5
     outs = []
-    for shape in (0, 1, 2):
+    for shape in (0, 1, 2, 3):
         ctx.set_sweep_tile(shape)
         assert ctx.get_sweep_tile() == shape
         for kz in (1, 3, 8):
@@ -218,18 +218,34 @@ def test_stencil_depth_does_not_change_the_jvp(tmp_path):
             outs.append(out)
     assert all(o.equal(outs[0]) for o in outs[1:])
     with pytest.raises(ValueError):
-        ctx.set_sweep_tile(3)
+        ctx.set_sweep_tile(4)
+    # the work split of the batch's sweeps changes no node's arithmetic either:
+    # a whole mixed step gives the same bits for every split
+    from nanowire_tdgl_amd import _native
+    steps = []
+    for shape, split in ((0, (8, 2)), (0, (1, 1)), (0, (3, 3)), (3, (2, 2)), (3, (64, 1)), (1, (8, 2))):
+        runner.batch.set_tile(shape)
+        runner.batch.set_split(*split)
+        assert runner.batch.get_split() == split
+        Xb = runner.pack([f.x0 for f in films])
+        st = runner.batch.trap_step(Xb, 0.2, _native.solver_opts(precision="mixed"))
+        assert all(q.converged for q in st)
+        steps.append(Xb)
+    assert all(o.equal(steps[0]) for o in steps[1:])
+    with pytest.raises(ValueError):
+        runner.batch.set_split(2, 4)
     cache = tmp_path / "autotune.json"
     info = runner.tune(reps=2, cache=str(cache))
-    assert len(info["kz"]) == 3 and info["ms"] > 0 and info["shape"] in (0, 1, 2)
+    assert len(info["kz"]) == 3 and info["ms"] > 0 and info["shape"] in (0, 1, 2, 3)
     assert runner.batch.get_tile() == info["shape"] and not info["cached"]
+    assert list(runner.batch.get_split()) == info["split"]
     assert cache.exists()
     # a second tuning of the same films reads the pick from the cache
     other = BatchRunner([f.device for f in films])
     again = other.tune(reps=2, cache=str(cache))
-    assert again["cached"] and again["kz"] == info["kz"] and again["shape"] == info["shape"]
+    assert again["cached"] and again["split"] == info["split"] and again["shape"] == info["shape"]
     assert other.batch.get_tile() == info["shape"]
-    assert [other.batch.get_kz(f) for f in range(3)] == info["kz"]
+    assert list(other.batch.get_split()) == info["split"]
     # different films: a different key, tuned afresh
     fewer = BatchRunner([f.device for f in films[:2]])
     assert not fewer.tune(reps=1, cache=str(cache))["cached"]

@@ -66,7 +66,7 @@ def test_c4_batch_of_64_films_against_fp64():
     assert len(films) == 64
     runner = BatchRunner([f.device for f in films])
     tune = runner.tune(reps=2, cache=None)
-    print(f"autotuned tile shape {tune['shape']}, kz {tune['kz'][:8]}..., {tune['ms']:.3f} ms per sweep")
+    print(f"autotuned tile shape {tune['shape']}, split {tune['split']}, {tune['ms']:.3f} ms per sweep")
     res = trapezoidal_batch(films, 0.0, T_FIELD_STOP, 0.2, max_attempts=3, runner=runner)
     print(f"attempts {res.attempts}, accepted steps per film {sorted(set(res.steps))}")
     worst = {"psi2": 0.0, "bz": 0.0, "js": 0.0}
