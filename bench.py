@@ -296,6 +296,10 @@ def section_large(steps, warmup):
 BATCH_DT = 0.2
 BATCH_T_STOP = 300 * BATCH_DT  # the configured 300 steps: the field's on-window ends at 2/3 of it
 BATCH_TOL = 1e-4
+WORKLOAD_BATCH = ("64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
+                  "Bz~U[0.05,0.8], holes on even films; psi = 1 + 1e-3 noise, phi = 0; dt=0.2 "
+                  "(per-film dt/10 retry), TGCR tol 1e-4, Chebyshev-preconditioned mixed TGCR, "
+                  "autotuned tiles")
 
 
 def _quantiles(vals) -> dict:
@@ -356,7 +360,7 @@ class BatchAttempts:
         return sum(self.attempt() for _ in range(k))
 
 
-def section_batched(attempts, warmup, fp64=True):
+def section_batched(attempts, warmup, fp64=True, variants=True):
     """The headline: the 64-film batch in mixed precision."""
     import torch
     from nanowire_tdgl_amd import _native, workloads
@@ -368,6 +372,7 @@ def section_batched(attempts, warmup, fp64=True):
     opts = _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed")
 
     def timed(b: BatchAttempts):
+        b.start = b.snapshot()
         b.run(warmup)
         torch.cuda.synchronize()
         snap = b.snapshot()
@@ -385,17 +390,22 @@ def section_batched(attempts, warmup, fp64=True):
     log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films, {attempts} attempts "
         f"in {el:.3f} s")
     pc = roof = None
-    if PROFILE:  # instrumented repeat of the same attempts from the same state
+    if PROFILE:  # instrumented repeat of the warm-up and timed attempts from the same state
         final = mix.snapshot()
-        mix.restore(snap)
+        mix.restore(mix.start)
         runner.batch.profile_enable(True)
-        mix.run(attempts)
+        mix.run(warmup + attempts)
         pc = runner.batch.profile_read()
         runner.batch.profile_enable(False)
         roof = with_traffic(roofline_of(pc), "cheb_sweeps_mixed_sections.md", "batched")
         mix.restore(final)
         log(f"[bench] batched dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
             f"{roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
+    if not variants:
+        out = {"workload": WORKLOAD_BATCH, "films": len(films), "n_interior_total": int(ns.sum()),
+               "attempts": attempts, "warmup": warmup, "seconds": el, "value": v,
+               "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc}
+        return out
     # the same attempts without the preconditioner, and with the extrapolating predictor
     unp = BatchAttempts(runner, films, _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed",
                                                            precond="none"))
@@ -407,10 +417,7 @@ def section_batched(attempts, warmup, fp64=True):
     _, acce, ele = timed(ex)
     ve = acce / ele
     log(f"[bench] batched mixed, extrapolating predictor: {ve:.4g} cell-steps/s")
-    out = {"workload": "64 films from seed 3: Nx,Ny in [32,160], Nz in [4,24], h=0.5, kappa~U[1,5], "
-                       "Bz~U[0.05,0.8], holes on even films; psi = 1 + 1e-3 noise, phi = 0; dt=0.2 "
-                       "(per-film dt/10 retry), TGCR tol 1e-4, Chebyshev-preconditioned mixed TGCR, "
-                       "autotuned tiles",
+    out = {"workload": WORKLOAD_BATCH,
            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
            "warmup": warmup, "seconds": el, "value": v,
            "newton_iters_per_film_attempt": mix.newton_timed,
@@ -669,12 +676,17 @@ def main() -> None:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-batch-fp64", action="store_true",
                     help="skip the FP64 loop baseline and the error report of the batch")
+    ap.add_argument("--no-batch-variants", action="store_true",
+                    help="skip the batch's runs without the preconditioner and with the "
+                         "extrapolating predictor (the profiled command of the headline)")
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--adjoint-steps", type=int, default=400)
     ap.add_argument("--adjoint-mask-cells", type=int, default=6)
     ap.add_argument("--sections", default="fp64_plain,mixed,large,adjoint",
                     help="comma list of secondary sections to run (empty: headline only)")
+    ap.add_argument("--sections-only", action="store_true",
+                    help="profiling aid: run only the secondary sections (no headline value)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 5)
@@ -690,7 +702,11 @@ def main() -> None:
         sys.exit(2)
 
     import torch  # noqa: F401
-    head = section_batched(args.steps, args.warmup, fp64=not args.no_batch_fp64)
+    if args.sections_only:
+        head = None
+    else:
+        head = section_batched(args.steps, args.warmup, fp64=not args.no_batch_fp64,
+                               variants=not args.no_batch_variants)
     sections = [x for x in args.sections.split(",") if x]
     extra = {}
     if "fp64_plain" in sections or "mixed" in sections:
@@ -710,6 +726,9 @@ def main() -> None:
         log(f"[bench] CPU oracle sample ({args.cpu_budget:.0f} s budget per run)")
         cpu = cpu_baseline(args.cpu_budget, args.cpu_workers)
 
+    if head is None:
+        print(json.dumps({"sections_only": True, **extra}), flush=True)
+        return
     value = head.pop("value")
     el = head.pop("seconds")
     roof = head.pop("roofline")

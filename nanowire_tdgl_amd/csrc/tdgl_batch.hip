@@ -296,8 +296,8 @@ int set_act(tdgl_batch* B, const std::vector<int>& act) {
 // busy).  The pieces are listed film by film (within a film, neighbouring
 // tiles of one z piece next to each other: their shared halo cells meet in
 // L2), and the sweep kernel gives each XCD one contiguous run of the list
-// (xcd_remap); inside each run the deeper pieces come first, so each XCD's
-// dispatcher starts its longest blocks first.
+// (xcd_remap); inside each run the deeper pieces come first (by depth class,
+// two planes wide), so each XCD's dispatcher starts its longest blocks first.
 void part_build(const tdgl_batch* B, const std::vector<char>& in, long long gres, int lmin, int lmax,
                 std::vector<int4>& seg, int& nblocks) {
   long long W = 0;
@@ -326,8 +326,10 @@ void part_build(const tdgl_batch* B, const std::vector<char>& in, long long gres
   const int G = nblocks, nx = 8, per = (G + nx - 1) / nx, full = G % nx == 0 ? nx : G % nx;
   for (int x = 0, base = 0; x < nx; ++x) {
     const int len = x < full ? per : per - 1;
+    // by depth class (pieces within a plane or so of each other keep their
+    // order: a column's z pieces differ by one plane and share halo planes)
     std::stable_sort(seg.begin() + seg0 + base, seg.begin() + seg0 + base + len,
-                     [](const int4& a, const int4& b) { return a.w - a.z > b.w - b.z; });
+                     [](const int4& a, const int4& b) { return (a.w - a.z + 1) / 2 > (b.w - b.z + 1) / 2; });
     base += len;
   }
 }
