@@ -132,3 +132,16 @@ def test_applied_field_evaluate():
     assert ramp.evaluate(0.25, 1.0)[2] == pytest.approx(1.0)
     custom = T.AppliedField(field_func=lambda t, ts: (0.0, 0.0, t / ts))
     assert custom.evaluate(0.5, 1.0)[2] == pytest.approx(0.5)
+
+
+def test_bench_reads_the_committed_traffic_lines():
+    """bench.py attaches the HBM traffic of each section's dominant kernel from
+    the committed rocprofv3 summaries: every line it reads parses, per kernel."""
+    import bench
+    sweeps = bench.pmc_traffic("cheb_sweeps_mixed_sections.md")
+    for section in ("batched", "mixed", "large"):
+        assert sweeps[section]["cheb"] > 1e6, section
+    fp64 = bench.pmc_traffic("pmc_traffic_film128_hole.md")
+    assert fp64["mgs"] > 1e6
+    roof = {"kernel": "cheb", "traffic": None}
+    assert bench.with_traffic(dict(roof), "cheb_sweeps_mixed_sections.md", "batched")["traffic"] > 1e6
