@@ -165,9 +165,16 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   // the loads' latency overlaps the plane of arithmetic in between.
   const Fld32 F = fld32(b, &fd->b);
   const Plan32 pl0 = plan32(g, F, i0 - 1 + hc0 % CXr, j0 - 1 + hc0 / CXr);
-  const Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CXr, j0 - 1 + hc1 / CXr);
+  Plan32 pl1 = plan32(g, F, i0 - 1 + hc1 % CXr, j0 - 1 + hc1 / CXr);
+  if (!has1) {  // no second cell: its loads all read node 0 (one line each), unused
+    pl1.qij = 0u;
+    pl1.mij = 0u;
+  }
   const bool hf = mt.flags != nullptr;
-  auto load_into = [&](int kp, Raw32& a0, Raw32& a1) __attribute__((always_inline)) {
+  // live = false: the march's last step, which has no plane to fetch; it
+  // issues the same loads (so that the waits count them exactly) from node 0
+  // of plane kp (one line each), and the values are never committed
+  auto load_into = [&](int kp, Raw32& a0, Raw32& a1, bool live = true) __attribute__((always_inline)) {
     if (GEN && F.gen) {  // general field: the rules at once, the cells kept in the load slots
       auto pack = [](Raw32& r, const Cell<float>& v, const Cell<float>& w) {
         r.xa = make_float2(v.pr, v.pi);
@@ -199,8 +206,13 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
     // are valid, and its values are not written.  Loads on no branch let the
     // own node's waits below count them exactly.
     const Pl32 pp = plane32<IS3D>(g, F, kp);
-    load32u<IS3D>(mt.flags, X, P, pl0, pp, a0);
-    load32u<IS3D>(mt.flags, X, P, pl1, pp, a1);
+    Plan32 c0 = pl0, c1 = pl1;
+    if (!live) {
+      c0.qij = c1.qij = 0u;
+      c0.mij = c1.mij = 0u;
+    }
+    load32u<IS3D>(mt.flags, X, P, c0, pp, a0);
+    load32u<IS3D>(mt.flags, X, P, c1, pp, a1);
   };
   auto put = [&](int slot, int t, int c, const Cell<float>& v, const Cell<float>& w, unsigned int h) {
     H[t][c] = (uint8_t)h;
@@ -217,9 +229,10 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
       P_z[slot][c] = w.z;
     }
   };
-  // plane kp's cells, loaded by load_into, to their shared-memory slots
-  auto commit_from = [&](int kp, const Raw32& a0, const Raw32& a1) __attribute__((always_inline)) {
-    const int slot = kp % RING, t = kp % NS;
+  // plane kp's cells, loaded by load_into, to ring slot `slot` (and slot t of
+  // the arrays read on one plane only)
+  auto commit_from = [&](int kp, int slot, int t, const Raw32& a0, const Raw32& a1)
+                         __attribute__((always_inline)) {
     // every load of the plane has landed here (all are used below); waiting
     // for all at once tells the compiler's wait tracking that no load is
     // left in flight, so no register reuse later in the plane waits on one
@@ -246,14 +259,18 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
     if (has1) put(slot, t, hc1, v1c, w1c, a1.h);
   };
 
+  // Ring slots: plane kp sits in slot (kp - kb + 1) % RING ((kp - kb + 1) % NS
+  // for the one-plane arrays).  With RING == NS (the 64x4 tile) the march is
+  // unrolled by RING, so that every copy of the step knows its slots at
+  // compile time.
   Raw32 r0, r1;
   if (IS3D) {
     // the first two planes' loads go out together (one memory round trip)
     Raw32 a0, a1;
     load_into(kb - 1, a0, a1);
     load_into(kb, r0, r1);
-    commit_from(kb - 1, a0, a1);
-    commit_from(kb, r0, r1);
+    commit_from(kb - 1, 0, 0, a0, a1);
+    commit_from(kb, 1, 1, r0, r1);
     load_into(kb + 1, r0, r1);
   } else {
     load_into(0, r0, r1);
@@ -262,13 +279,15 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   const float ihx2 = (float)mt.ihx2, ihy2 = (float)mt.ihy2, ihz2 = (float)mt.ihz2;
   const float dgpsi = (float)mt.dgpsi, inv_tau = (float)mt.inv_tau;
 
-  for (int k = kb; k < ke; ++k) {
-    commit_from(IS3D ? k + 1 : 0, r0, r1);
+  // one step of the march: plane k from slots s0 (plane k), sm (k - 1) and sp1
+  // (k + 1, committed first); t0 / t1: planes k / k + 1 in the one-plane arrays
+  auto step_k = [&](int k, int s0, int sm, int sp1, int t0, int t1) __attribute__((always_inline)) {
+    commit_from(IS3D ? k + 1 : 0, sp1, t1, r0, r1);
     __syncthreads();
     // own-node inputs (a lane past the grid's edge reads the nearest node and
     // stores nothing), then the halo two planes ahead
     const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
-    const int t0 = k % NS, c0 = (ty + 1) * CXr + (tx + 1);
+    const int c0 = (ty + 1) * CXr + (tx + 1);
     const uint8_t fl = H[t0][c0];
     float scv = 1.f, k2v = 0.f;
     if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
@@ -296,11 +315,10 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
       w_y = CP.ph(3, q);
       if (IS3D) w_z = CP.ph(4, q);
     }
-    // the last plane loads plane ke again (in range, not written)
-    if (IS3D) load_into(min(k + 2, ke), r0, r1);
+    // the halo two planes ahead (on the last step, the one-line stand-in)
+    if (IS3D) load_into(min(k + 2, ke), r0, r1, k + 2 <= ke);
 
     if (act) {
-      const int s0 = k % RING, sm = (k + RING - 1) % RING, sp1 = (k + 1) % RING;
       const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CXr, cyp = c0 + CXr;
       const float pr = X_r[s0][c0], pi = X_i[s0][c0];
       const float dr = P_r[s0][c0], di = P_i[s0][c0];
@@ -353,9 +371,9 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
 
       float cx, cy, cz;
       if (mt.k2) {
-        cx = k2v / (float)mt.hx2;
-        cy = k2v / (float)mt.hy2;
-        cz = IS3D ? k2v / (float)mt.hz2 : 0.f;
+        cx = k2v * (float)mt.ihx2;
+        cy = k2v * (float)mt.ihy2;
+        cz = IS3D ? k2v * (float)mt.ihz2 : 0.f;
       } else {
         cx = (float)mt.cx;
         cy = (float)mt.cy;
@@ -415,6 +433,21 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
       }
     }
     if (IS3D && RING == 3) __syncthreads();
+  };
+  if constexpr (IS3D && RING == NS) {
+    static_assert(RING == 3, "the unrolled march has three slots");
+    for (int k = kb; k < ke; k += 3) {
+      step_k(k, 1, 0, 2, 1, 2);
+      if (k + 1 >= ke) break;
+      step_k(k + 1, 2, 1, 0, 2, 0);
+      if (k + 2 >= ke) break;
+      step_k(k + 2, 0, 2, 1, 0, 1);
+    }
+  } else {
+    for (int k = kb; k < ke; ++k) {
+      const int r = k - kb + 1;
+      step_k(k, r % RING, (r + RING - 1) % RING, (r + 1) % RING, r % NS, (r + 1) % NS);
+    }
   }
 }
 
