@@ -166,12 +166,18 @@ int sync_out(tdgl_batch* B) {
 // z planes per stencil work item: deep enough that the two extra staged planes
 // are a small overhead (measured best at 8 on 256x256x64 films and on the
 // 64-film batch; 16 and 4 are 10 % slower there inside a solve), shallow
-// enough that small films still spread over the CUs (2 on 128x128x16: 512
-// work items).
-int default_kz(const Geo& g) {
-  if (!g.is3d) return 1;
+// enough that the work still spreads over the CUs: about 512 work items over
+// the whole launch (2 on a 128x128x16 film alone).  A batch's launches cover
+// all its films, so the rule applies to the batch's tile-planes together
+// (8 on the 64-film batch, 2.0 staged cells per node instead of 2.8 with each
+// film sized alone; attempts 1.6 % faster).
+long long stencil_tile_planes(const Geo& g) {
   const long long tiles = (long long)((g.nx + kStencilTX - 1) / kStencilTX) * ((g.ny + kStencilTY - 1) / kStencilTY);
-  long long kz = tiles * g.nz / 512;
+  return tiles * (g.is3d ? g.nz : 1);
+}
+int default_kz(const Geo& g, long long launch_tile_planes) {
+  if (!g.is3d) return 1;
+  long long kz = launch_tile_planes / 512;
   kz = std::max(2LL, std::min<long long>(kz, 8));
   return (int)std::min<long long>(kz, g.nz);
 }
@@ -933,9 +939,11 @@ int batch_init(tdgl_batch* B) {
   B->kz.resize(F);
   B->kcap.resize(F);
   if (nfilm_is_own(B)) B->shape = B->films[0]->tile_user;
+  long long planes = 0;
+  for (int f = 0; f < F; ++f) planes += stencil_tile_planes(B->films[f]->g);
   for (int f = 0; f < F; ++f) {
     const Geo& g = B->films[f]->g;
-    B->kz[f] = B->films[f]->kz_user > 0 && nfilm_is_own(B) ? B->films[f]->kz_user : default_kz(g);
+    B->kz[f] = B->films[f]->kz_user > 0 && nfilm_is_own(B) ? B->films[f]->kz_user : default_kz(g, planes);
     B->kcap[f] = (int)std::min<long long>((long long)(1 + g.nphi) * g.n, kMaxKrylov);
   }
   return build_layout(B);
@@ -985,15 +993,20 @@ int mixed_step_single(tdgl_ctx* ctx, double* X, double dt, const tdgl_solver_opt
   return rc;
 }
 
+// The single-film FP32 entry points take and return vectors in the internal
+// layout; the sweeps work in the mixed method's (each node's links together),
+// so the vectors pass through the batch's work buffers on the way.
 int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, float* ap32) {
   int rc;
   if ((rc = own_batch(ctx))) return rc;
   tdgl_batch* B = ctx->own_batch;
   B->st = ctx->st;
-  if ((rc = sync_in(B)) || (rc = refresh(B))) {
+  if ((rc = ensure_work(B)) || (rc = sync_in(B)) || (rc = refresh(B))) {
     ctx->err = B->err;
     return rc;
   }
+  const long long n = B->hf[0].g.n;
+  const int nphi = B->hf[0].g.nphi;
   const double dtv[2] = {dt, dt / 2.0};
   std::vector<int> one(1, 1);
   if ((rc = set_act(B, one))) return rc;
@@ -1004,7 +1017,10 @@ int jvp32_single(tdgl_ctx* ctx, const float* x32, const float* p32, double dt, f
     ctx->err = B->err;
     return rc;
   }
-  launch_bjvp32(kview(B), B->part[0], B->is3d, x32, p32, nullptr, nullptr, ap32, B->ps);
+  launch_l32_permute(x32, B->x32, n, nphi, true, B->ps);
+  launch_l32_permute(p32, B->r32, n, nphi, true, B->ps);
+  launch_bjvp32(kview(B), B->part[0], B->is3d, B->x32, B->r32, nullptr, nullptr, B->r0_32, B->ps);
+  launch_l32_permute(B->r0_32, ap32, n, nphi, false, B->ps);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess) return TDGL_E_HIP;
   return sync_out(B);
 }
@@ -1029,11 +1045,15 @@ int precond_single(tdgl_ctx* ctx, const float* x32, const float* v32, double dt,
   if ((rc = setup_cheb(B, degree, dtv, one, iv))) return done(rc);
   if (B->cheb_deg == 0) return done(bfail(B, TDGL_E_INVALID, "no preconditioner for this degree and dt"));
   if ((rc = ensure_parts(B, one))) return done(rc);
-  enqueue_w(B, false, one, x32, v32, w32);
-  enqueue_z(B, false, one, x32, v32);
+  const long long n = B->hf[0].g.n;
+  const int nphi = B->hf[0].g.nphi;
+  launch_l32_permute(x32, B->x32, n, nphi, true, B->ps);
+  launch_l32_permute(v32, B->r0_32, n, nphi, true, B->ps);
+  enqueue_w(B, false, one, B->x32, B->r0_32, B->r32);
+  enqueue_z(B, false, one, B->x32, B->r0_32);
+  launch_l32_permute(B->r32, w32, n, nphi, false, B->ps);
   const float* z = B->cheb_deg >= 2 ? B->t32[B->cheb_deg % 2] : nullptr;  // one film: its own degree
-  if (z && hipMemcpyAsync(z32, z, sizeof(float) * (size_t)B->hf[0].len, hipMemcpyDeviceToDevice, B->ps) != hipSuccess)
-    return done(bfail(B, TDGL_E_HIP, "hipMemcpyAsync"));
+  if (z) launch_l32_permute(z, z32, n, nphi, false, B->ps);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(B->ps) != hipSuccess)
     return done(bfail(B, TDGL_E_HIP, "preconditioner kernels failed"));
   // degree 1: z = v / theta

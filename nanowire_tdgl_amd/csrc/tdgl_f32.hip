@@ -129,8 +129,8 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   if (CHEB && cs.m > df) return;
   const bool fin = CHEB && cs.w && cs.m == df;
   float* const obase = CHEB ? (fin ? (cs.out ? cs.out : Q[*iter]) : cs.mid) : (out_direct ? out_direct : Q[*iter]);
-  const W32 out = wplanes32(obase + off, n);
-  const V32 X = planes32(x32 + off, n), P = planes32(p32 + off, n);
+  const W32 out = wplanes32(obase + off, n, IS3D);
+  const V32 X = planes32(x32 + off, n, IS3D), P = planes32(p32 + off, n, IS3D);
   const float hdt = (float)bk.dtv[2 * f + 1];
   float c_it = 0.f, c_om = 0.f, c_ss = 1.f, c_sp = 0.f;
   const bool has_v = CHEB && cs.v && (fin || cs.cv), has_p = CHEB && cs.prev;
@@ -140,8 +140,8 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
     c_om = (float)cf[cs.m];
     c_ss = cs.src_th ? c_it : 1.f;
     c_sp = cs.prev ? (c_om - 1.f) * (cs.prev_th ? c_it : 1.f) : 0.f;
-    if (has_v) CV = planes32(cs.v + off, n);
-    if (has_p) CP = planes32(cs.prev + off, n);
+    if (has_v) CV = planes32(cs.v + off, n, IS3D);
+    if (has_p) CP = planes32(cs.prev + off, n, IS3D);
   }
   const int tile_x = seg.y % fd->tiles32_x, tile_y = seg.y / fd->tiles32_x;
   // the film's tile: TXr x TYr nodes, CXr staged cells per row
@@ -305,15 +305,17 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
     float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
     if (has_v) {
       v0 = CV.psi(q);
-      v_x = CV.ph(2, q);
-      v_y = CV.ph(3, q);
-      if (IS3D) v_z = CV.ph(4, q);
+      const float3 l = CV.links<IS3D>(q);
+      v_x = l.x;
+      v_y = l.y;
+      v_z = l.z;
     }
     if (has_p) {
       w0 = CP.psi(q);
-      w_x = CP.ph(2, q);
-      w_y = CP.ph(3, q);
-      if (IS3D) w_z = CP.ph(4, q);
+      const float3 l = CP.links<IS3D>(q);
+      w_x = l.x;
+      w_y = l.y;
+      w_z = l.z;
     }
     // the halo two planes ahead (on the last step, the one-line stand-in)
     if (IS3D) load_into(min(k + 2, ke), r0, r1, k + 2 <= ke);
@@ -422,14 +424,11 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
           return fin ? vv - res : res;
         };
         out.psi(q, make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
-        out.ph(2, q, step(praw_x, jx, v_x, w_x));
-        out.ph(3, q, step(praw_y, jy, v_y, w_y));
-        if (IS3D) out.ph(4, q, step(praw_z, jz, v_z, w_z));
+        out.links<IS3D>(q, step(praw_x, jx, v_x, w_x), step(praw_y, jy, v_y, w_y),
+                        IS3D ? step(praw_z, jz, v_z, w_z) : 0.f);
       } else {
         out.psi(q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
-        out.ph(2, q, praw_x - hdt * jx);
-        out.ph(3, q, praw_y - hdt * jy);
-        if (IS3D) out.ph(4, q, praw_z - hdt * jz);
+        out.links<IS3D>(q, praw_x - hdt * jx, praw_y - hdt * jy, IS3D ? praw_z - hdt * jz : 0.f);
       }
     }
     if (IS3D && RING == 3) __syncthreads();
@@ -763,6 +762,26 @@ __global__ __launch_bounds__(1024) void k_btrisolve(BatchK bk, const int* fstat,
   }
 }
 
+// x (FP64, internal layout) += d for FP32 elements el .. el + 3 of the mixed
+// layout (psi elements in place, link elements to their node's phi planes,
+// the film's padding in place).
+__device__ __forceinline__ void add4_internal(double* x, long long el, long long n, int nphi, double d0,
+                                              double d1, double d2, double d3) {
+  if (el + 3 < 2 * n) {
+    double2* X = reinterpret_cast<double2*>(x + el);
+    const double2 xa = X[0], xb = X[1];
+    X[0] = make_double2(xa.x + d0, xa.y + d1);
+    X[1] = make_double2(xb.x + d2, xb.y + d3);
+    return;
+  }
+  const double d[4] = {d0, d1, d2, d3};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const long long i = internal_of_l32(el + t, n, nphi);
+    x[i] += d[t];
+  }
+}
+
 // x (FP64) += gamma r_0 + sum_{m<K-1} coef_m Q_m; partials of ||dx||_inf
 // (complex modulus on psi, |.| on phi).
 __global__ __launch_bounds__(kRedThreads) void k_baccum(BatchK bk, double* x, const float* r0v,
@@ -772,7 +791,7 @@ __global__ __launch_bounds__(kRedThreads) void k_baccum(BatchK bk, double* x, co
   const double gam = res[RS_GAMMA];
   const double* coef = res + RS_BETA;
   const float4* R0 = reinterpret_cast<const float4*>(r0v + off);
-  double2* X = reinterpret_cast<double2*>(x + off);
+  double* X = x + off;
   const long long npsi = 2 * fd->g.n;  // psi occupies elements [0, 2n)
   double v[1] = {0.0};
   for (long long e = e0; e < n4; e += estride) {
@@ -786,10 +805,8 @@ __global__ __launch_bounds__(kRedThreads) void k_baccum(BatchK bk, double* x, co
       d2 += c * a.z;
       d3 += c * a.w;
     }
-    const double2 xa = X[2 * e], xb = X[2 * e + 1];
-    X[2 * e] = make_double2(xa.x + d0, xa.y + d1);
-    X[2 * e + 1] = make_double2(xb.x + d2, xb.y + d3);
     const long long el = 4 * e;
+    add4_internal(X, el, fd->g.n, fd->g.nphi, d0, d1, d2, d3);
     const double ma = (el < npsi) ? hypot(d0, d1) : fmax(fabs(d0), fabs(d1));
     const double mb = (el + 2 < npsi) ? hypot(d2, d3) : fmax(fabs(d2), fabs(d3));
     v[0] = fmax(v[0], fmax(ma, mb));
@@ -836,16 +853,14 @@ __global__ __launch_bounds__(kRedThreads) void k_baddz(BatchK bk, double* x, con
   const double s = df < 2 ? cf[0] : 1.0;
   const float* z = df < 2 ? u : ((df & 1) ? t1 : t0);
   const float4* Z = reinterpret_cast<const float4*>(z + off);
-  double2* X = reinterpret_cast<double2*>(x + off);
+  double* X = x + off;
   const long long npsi = 2 * fd->g.n;
   double v[1] = {0.0};
   for (long long e = e0; e < n4; e += estride) {
     const float4 a = Z[e];
     const double d0 = s * a.x, d1 = s * a.y, d2 = s * a.z, d3 = s * a.w;
-    const double2 xa = X[2 * e], xb = X[2 * e + 1];
-    X[2 * e] = make_double2(xa.x + d0, xa.y + d1);
-    X[2 * e + 1] = make_double2(xb.x + d2, xb.y + d3);
     const long long el = 4 * e;
+    add4_internal(X, el, fd->g.n, fd->g.nphi, d0, d1, d2, d3);
     const double ma = (el < npsi) ? hypot(d0, d1) : fmax(fabs(d0), fabs(d1));
     const double mb = (el + 2 < npsi) ? hypot(d2, d3) : fmax(fabs(d2), fabs(d3));
     v[0] = fmax(v[0], fmax(ma, mb));
@@ -870,9 +885,25 @@ __global__ __launch_bounds__(kRedThreads) void k_bextrap(BatchK bk, const double
   for (long long i = e0; i < len; i += estride) O[i] = X[i] + r * (X[i] - XP[i]);
 }
 
+// One film's FP32 vector between the internal layout and the mixed method's
+// (to_l32: out = the mixed layout of in; else the internal layout of in).
+__global__ __launch_bounds__(256) void k_l32_permute(const float* in, float* out, long long n, int nphi,
+                                                     int to_l32) {
+  const long long len = (2 + nphi) * n;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+       e += (long long)gridDim.x * blockDim.x)
+    out[e] = in[to_l32 ? internal_of_l32(e, n, nphi) : l32_of_internal(e, n, nphi)];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
+void launch_l32_permute(const float* in, float* out, long long n, int nphi, bool to_l32, hipStream_t st) {
+  const long long len = (2 + nphi) * n;
+  const int blocks = (int)std::min<long long>((len + 255) / 256, 4096);
+  launch(k_l32_permute, dim3(blocks), dim3(256), 0, st, in, out, n, nphi, to_l32 ? 1 : 0);
+}
 
 void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,
                     hipStream_t st) {

@@ -341,17 +341,12 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
         if (MODE == M_NEWTON) {
           st5(wplanes64(e.out1, n), -gr, -gi, -gx, -gy, -gz);
         } else {
-          const W32 a32 = wplanes32(e.o32a, n), b32 = wplanes32(e.o32b, n);
+          // the mixed method's FP32 layout (each node's links together)
+          const W32 a32 = wplanes32(e.o32a, n, IS3D), b32 = wplanes32(e.o32b, n, IS3D);
           a32.psi(q, make_float2((float)c.xr, (float)c.xi));
-          a32.ph(2, q, (float)c.xx);
-          a32.ph(3, q, (float)c.xy);
+          a32.links<IS3D>(q, (float)c.xx, (float)c.xy, (float)c.xz);
           b32.psi(q, make_float2((float)-gr, (float)-gi));
-          b32.ph(2, q, (float)-gx);
-          b32.ph(3, q, (float)-gy);
-          if (IS3D) {
-            a32.ph(4, q, (float)c.xz);
-            b32.ph(4, q, (float)-gz);
-          }
+          b32.links<IS3D>(q, (float)-gx, (float)-gy, (float)-gz);
         }
         red[0] += ((c.xr * c.xr + c.xi * c.xi) + c.xx * c.xx) + (c.xy * c.xy + c.xz * c.xz);
         red[1] += ((gr * gr + gi * gi) + gx * gx) + (gy * gy + gz * gz);

@@ -238,6 +238,9 @@ void launch_bcheb32(const BatchK& bk, const SweepPart& part, bool is3d, const fl
                     float* const* Q, const int* iter, const ChebStep& cs, hipStream_t st);
 // out = x + ratio (x - xp): the extrapolating predictor (ratio[f] <= 0: film
 // f keeps out as it is)
+// one film's FP32 vector between the internal layout and the mixed method's
+// (each node's link components together; tdgl_stage32.hpp)
+void launch_l32_permute(const float* in, float* out, long long n, int nphi, bool to_l32, hipStream_t st);
 void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,
                     hipStream_t st);
 void launch_extrap64(const double* x, const double* xp, double ratio, double* out, long long len,

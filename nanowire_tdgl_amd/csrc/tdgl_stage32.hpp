@@ -48,32 +48,60 @@ __host__ __device__ __forceinline__ void stu(T* base, unsigned int e, T v) {
   stb<T>(base, e * (unsigned int)sizeof(T), v);
 }
 
-// One FP32 state-shaped vector of n nodes: a base pointer and the byte size of
-// one phi plane (4 n); psi of node q is at byte 8 q, phi_x at 2 (4n) + 4 q,
-// phi_y at 3 (4n) + 4 q, phi_z at 4 (4n) + 4 q.  One 64-bit base per vector
+// One FP32 state-shaped vector of n nodes, the mixed method's layout: psi of
+// node q at byte 8 q (the first 2 n floats, as in the internal layout), then
+// each node's link components together: phi_x, phi_y (, phi_z) of node q at
+// byte 8 n + ls q, ls = 4 nphi.  A node's links are then one load (dwordx3,
+// dwordx2 in 2-D) where three planes took three.  One 64-bit base per vector
 // keeps the kernels' uniform state small; the host keeps 5 (4n) below 2^32.
 struct V32 {
   const char* base;
-  unsigned int pl;  // 4 n
+  unsigned int lk;  // 8 n: where the links start
+  unsigned int ls;  // 4 nphi: bytes of one node's links
   __host__ __device__ __forceinline__ float2 psi(unsigned int q) const { return ldb<float2>(base, 8u * q); }
   __host__ __device__ __forceinline__ float ph(int a, unsigned int q) const {  // a = 2, 3, 4
-    return ldb<float>(base, (unsigned int)a * pl + 4u * q);
+    return ldb<float>(base, lk + ls * q + 4u * (unsigned int)(a - 2));
+  }
+  // phi_x, phi_y, phi_z of node q (phi_z = 0 in 2-D)
+  template <bool IS3D>
+  __host__ __device__ __forceinline__ float3 links(unsigned int q) const {
+    if (IS3D) return ldb<float3>(base, lk + 12u * q);
+    const float2 v = ldb<float2>(base, lk + 8u * q);
+    return make_float3(v.x, v.y, 0.f);
   }
 };
-__host__ __device__ __forceinline__ V32 planes32(const float* v, unsigned int n) {
-  return V32{reinterpret_cast<const char*>(v), 4u * n};
+__host__ __device__ __forceinline__ V32 planes32(const float* v, unsigned int n, bool is3d) {
+  return V32{reinterpret_cast<const char*>(v), 8u * n, is3d ? 12u : 8u};
 }
 // The same for an output vector.
 struct W32 {
   char* base;
-  unsigned int pl;
+  unsigned int lk, ls;
   __host__ __device__ __forceinline__ void psi(unsigned int q, float2 v) const { stb<float2>(base, 8u * q, v); }
   __host__ __device__ __forceinline__ void ph(int a, unsigned int q, float v) const {
-    stb<float>(base, (unsigned int)a * pl + 4u * q, v);
+    stb<float>(base, lk + ls * q + 4u * (unsigned int)(a - 2), v);
+  }
+  template <bool IS3D>
+  __host__ __device__ __forceinline__ void links(unsigned int q, float x, float y, float z) const {
+    if (IS3D) stb<float3>(base, lk + 12u * q, make_float3(x, y, z));
+    else stb<float2>(base, lk + 8u * q, make_float2(x, y));
   }
 };
-__host__ __device__ __forceinline__ W32 wplanes32(float* v, unsigned int n) {
-  return W32{reinterpret_cast<char*>(v), 4u * n};
+__host__ __device__ __forceinline__ W32 wplanes32(float* v, unsigned int n, bool is3d) {
+  return W32{reinterpret_cast<char*>(v), 8u * n, is3d ? 12u : 8u};
+}
+// Element index in the mixed method's FP32 layout of element e of the internal
+// layout (psi in [0, 2n) as is; phi_a of node q, internal (2 + a) n + q, at
+// 2n + nphi q + a) -- the maps the FP32 <-> FP64 kernels use.
+__host__ __device__ __forceinline__ long long l32_of_internal(long long e, long long n, int nphi) {
+  if (e < 2 * n || e >= (2 + nphi) * n) return e;
+  const long long a = (e - 2 * n) / n, q = (e - 2 * n) - a * n;
+  return 2 * n + nphi * q + a;
+}
+__host__ __device__ __forceinline__ long long internal_of_l32(long long e, long long n, int nphi) {
+  if (e < 2 * n || e >= (2 + nphi) * n) return e;
+  const long long q = (e - 2 * n) / nphi, a = (e - 2 * n) - q * nphi;
+  return 2 * n + a * n + q;
 }
 
 __host__ __device__ __forceinline__ unsigned int qix32(const Geo& g, int i, int j, int k, bool is3d) {
@@ -331,12 +359,13 @@ __host__ __device__ __forceinline__ void load32u(const uint8_t* flags, const V32
   const unsigned int q = c.qij + p.qoff;
   r.xa = X.psi(q);
   r.pa = P.psi(q);
-  r.xx = X.ph(2, q);
-  r.xy = X.ph(3, q);
-  r.xz = IS3D ? X.ph(4, q) : 0.f;
-  r.px = P.ph(2, q);
-  r.py = P.ph(3, q);
-  r.pz = IS3D ? P.ph(4, q) : 0.f;
+  const float3 xl = X.links<IS3D>(q), pk = P.links<IS3D>(q);
+  r.xx = xl.x;
+  r.xy = xl.y;
+  r.xz = xl.z;
+  r.px = pk.x;
+  r.py = pk.y;
+  r.pz = pk.z;
   r.h = flags ? (unsigned int)ldu(flags, c.mij + p.moff) : 0u;
 }
 

@@ -70,7 +70,18 @@ static long long check(std::mt19937& rng, int trial) {
   const uint8_t* fl = trial % 2 ? flags.data() : nullptr;
   const Src<float, false> sx{x.data(), nullptr, 0.f, g.n};
   const Src<float, false> sp{p.data(), nullptr, 0.f, g.n};
-  const V32 X = planes32(x.data(), (unsigned)g.n), P = planes32(p.data(), (unsigned)g.n);
+  // the FP32 sweeps read the mixed method's layout (each node's links together)
+  std::vector<float> xl(len), pl(len);
+  for (long long e = 0; e < len; ++e) {
+    const long long i = internal_of_l32(e, g.n, g.nphi);
+    xl[e] = x[i];
+    pl[e] = p[i];
+    if (l32_of_internal(i, g.n, g.nphi) != e) {
+      printf("LAYOUT MAP MISMATCH n=%lld nphi=%d e=%lld\n", g.n, g.nphi, e);
+      return -1;
+    }
+  }
+  const V32 X = planes32(xl.data(), (unsigned)g.n, IS3D), P = planes32(pl.data(), (unsigned)g.n, IS3D);
   std::vector<double> xd(len), pd(len);
   for (long long e = 0; e < len; ++e) {
     xd[e] = (double)x[e] + 1e-9 * ur(rng);
