@@ -205,9 +205,14 @@ int build_layout(tdgl_batch* B) {
     d.item0 = (int)B->item_film.size();
     d.nitems = d.tiles_x * d.tiles_y * d.tz;
     B->item_film.insert(B->item_film.end(), d.nitems, f);
-    d.shape32 = B->shape == kSweepShapeFilm ? sweep_film_shape(c->g.nx, c->g.ny) : B->shape;
-    d.tiles32_x = (c->g.nx + sweep_tx(d.shape32) - 1) / sweep_tx(d.shape32);
-    d.tiles32_y = (c->g.ny + sweep_ty(d.shape32) - 1) / sweep_ty(d.shape32);
+    if (B->shape == kSweepShapeFilm) {
+      sweep_film_tile(c->g.nx, c->g.ny, d.tw32, d.th32);
+    } else {
+      d.tw32 = sweep_tx(B->shape);
+      d.th32 = sweep_ty(B->shape);
+    }
+    d.tiles32_x = (c->g.nx + d.tw32 - 1) / d.tw32;
+    d.tiles32_y = (c->g.ny + d.th32 - 1) / d.th32;
     d.sb0 = (int)B->sblk_film.size();
     d.nsb = (int)std::min<long long>(kStreamMaxPerFilm, std::max<long long>(1, (c->len + kStreamChunk - 1) / kStreamChunk));
     B->sblk_film.insert(B->sblk_film.end(), d.nsb, f);

@@ -73,12 +73,13 @@ template <bool IS3D, bool CHEB, int TS, bool GEN>
 __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, const float* x32,
                                                const float* p32, float* const* Q, const int* iter,
                                                float* out_direct, ChebStep cs) {
-  // TS = kSweepShapeFilm: each film's own shape (FilmDesc::shape32), the
-  // shared memory sized for the largest (64 x 4: 66 x 6 staged cells)
+  // TS = kSweepShapeFilm: each film's own tile (FilmDesc::tw32 x th32, at most
+  // 256 nodes), the shared memory sized for the 64 x 4 tile's 66 x 6 cells
   constexpr bool RT = TS == kSweepShapeFilm;
   constexpr int TX = RT ? 64 : sweep_tx(TS), TY = RT ? 4 : sweep_ty(TS);
   static_assert(TX * TY == BS, "a sweep tile holds one node per thread and plane");
   constexpr int CX = TX + 2, CELLS = CX * (TY + 2);
+  static_assert(!RT || CELLS == kSweepCells, "per-film tiles fit the 64 x 4 tile's cells");
   // Shared-memory rings of staged planes.  Plane k + 1 is committed at the top
   // of step k, over the slot of plane k + 1 - RING.  Arrays read on planes
   // k - 1..k + 1 (RING = 4) and arrays read on plane k only (NS = 3) then
@@ -144,15 +145,17 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
     if (has_p) CP = planes32(cs.prev + off, n, IS3D);
   }
   const int tile_x = seg.y % fd->tiles32_x, tile_y = seg.y / fd->tiles32_x;
-  // the film's tile: TXr x TYr nodes, CXr staged cells per row
-  const int tsh = RT ? fd->shape32 : 0;
-  const int TXr = RT ? (64 >> tsh) : TX, TYr = RT ? (4 << tsh) : TY, CXr = TXr + 2;
+  // the film's tile: TXr x TYr nodes, CXr staged cells per row.  With a
+  // per-film tile the lanes past TXr x TYr compute nothing (their c0 below
+  // stays inside the cells: (255 / TXr + 1)(TXr + 2) + TXr < kSweepCells for
+  // TXr >= 16)
+  const int TXr = RT ? fd->tw32 : TX, TYr = RT ? fd->th32 : TY, CXr = TXr + 2;
   const int cellsr = RT ? CXr * (TYr + 2) : CELLS;
   const int i0 = 1 + tile_x * TXr, j0 = 1 + tile_y * TYr;
-  const int tx = RT ? (int)(threadIdx.x & (unsigned)(TXr - 1)) : (int)(threadIdx.x % TX);
-  const int ty = RT ? (int)(threadIdx.x >> (6 - tsh)) : (int)(threadIdx.x / TX);
+  const int tx = RT ? (int)(threadIdx.x % (unsigned)TXr) : (int)(threadIdx.x % TX);
+  const int ty = RT ? (int)(threadIdx.x / (unsigned)TXr) : (int)(threadIdx.x / TX);
   const int i = i0 + tx, j = j0 + ty;
-  const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1);
+  const bool act = (i <= g.Nx - 1) && (j <= g.Ny - 1) && (!RT || ty < TYr);
   const int kb = seg.z, ke = seg.w;
 
   const int hc0 = threadIdx.x, hc1 = threadIdx.x + BS;

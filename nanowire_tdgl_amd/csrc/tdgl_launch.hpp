@@ -59,27 +59,38 @@ constexpr int kMaxKrylov = 2048;
 // 15.5 us against 19.1 us per Chebyshev sweep for 32 x 8; equal on 256x256x64),
 // although 32 x 8 wins when the sweep is timed alone on cache-resident vectors.
 // FP32 sweep tile shapes: 0 = 64 x 4, 1 = 32 x 8, 2 = 16 x 16 nodes per plane;
-// kSweepShapeFilm = each film its own (sweep_film_shape)
+// kSweepShapeFilm = each film a tile of its own (sweep_film_tile)
 constexpr int kSweepShapeFilm = 3;
 constexpr int kSweepShapes = 4;
 constexpr int kDefaultSweepShape = 0;
 __host__ __device__ constexpr int sweep_tx(int s) { return s == 1 ? 32 : (s == 2 ? 16 : 64); }
 __host__ __device__ constexpr int sweep_ty(int s) { return s == 1 ? 8 : (s == 2 ? 16 : 4); }
-// The fixed shape of a film under kSweepShapeFilm: the one with the least
-// work per plane, counting each tile as its 256 lanes (idle ones included)
-// plus a third of a lane per staged cell ((TX + 2)(TY + 2) per tile).
-inline int sweep_film_shape(long long nx, long long ny) {
-  int best = 0;
-  double bc = 0.0;
-  for (int s = 0; s < 3; ++s) {
-    const long long tx = (nx + sweep_tx(s) - 1) / sweep_tx(s), ty = (ny + sweep_ty(s) - 1) / sweep_ty(s);
-    const double c = (double)(tx * ty) * (256.0 + (sweep_tx(s) + 2) * (sweep_ty(s) + 2) / 3.0);
-    if (s == 0 || c < bc) {
-      best = s;
-      bc = c;
+// Staged cells a sweep block holds per plane: the 64 x 4 tile's (66 x 6).
+constexpr int kSweepCells = 66 * 6;
+// The tile of a film under kSweepShapeFilm: tw x th nodes per plane, any width
+// from 16 to 64 (tw th <= 256 lanes, (tw + 2)(th + 2) staged cells within
+// kSweepCells), the one with the least work per plane.  A sweep's time is
+// close to proportional to its lanes (idle ones included: 17.8, 18.3 and
+// 20.4 ps per lane and plane for 64 x 4, 32 x 8 and 16 x 16 tiles on the
+// films of a 64-film batch), a little more per lane for narrow tiles, whose
+// waves span more rows: a tile costs 23 + 64 / tw.  On a tie the wider tile.
+// A film 67 nodes wide takes tiles 34 wide (2 across, 99 % of the columns
+// busy) where 64 x 4 tiles keep 52 % busy.
+inline void sweep_film_tile(long long nx, long long ny, int& tw, int& th) {
+  double bc = -1.0;
+  tw = 64;
+  th = 4;
+  for (int w = 64; w >= 16; --w)
+    for (int h = 1; w * h <= 256; ++h) {
+      if ((w + 2) * (h + 2) > kSweepCells) continue;
+      const long long t = ((nx + w - 1) / w) * ((ny + h - 1) / h);
+      const double c = (double)t * (23.0 + 64.0 / w);
+      if (bc < 0.0 || c < bc) {
+        bc = c;
+        tw = w;
+        th = h;
+      }
     }
-  }
-  return best;
 }
 // Chebyshev polynomial preconditioner of the mixed TGCR: largest degree and
 // the per-film coefficient stride (1/theta, then omega_1 .. omega_d).

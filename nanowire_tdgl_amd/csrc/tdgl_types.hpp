@@ -57,7 +57,7 @@ struct FilmDesc {
   long long len;                 // elements of one state vector: (2 + nphi) n
   int tiles_x, tiles_y, kz, tz;  // FP64 stencil decomposition: tiles_x*tiles_y*tz work items
   int item0, nitems;             // the film's FP64 stencil blocks
-  int shape32;                   // FP32 sweep tile shape of the film (sweep_tx / sweep_ty)
+  int tw32, th32;                // FP32 sweep tile of the film: nodes per row, rows
   int tiles32_x, tiles32_y;      // FP32 sweep tiles per plane
   int sb0, nsb;                  // the film's streaming blocks
   long long hoff;                // the film's packed triangular TGCR matrix in the batch's H buffer
