@@ -368,7 +368,11 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
     films = workloads.batch_films()
     ns = np.array([f.device.params.n_interior for f in films])
     runner = BatchRunner([f.device for f in films])
-    tune = runner.tune(reps=3)
+    tune = runner.tune(reps=10)
+    tune_info = {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[tune["shape"]],
+                            "split": tune["split"]}, "cached": tune.get("cached", False),
+                 "jvp_ms_by_shape_and_split": {str(k): round(m, 4) for k, m in tune.get("trials", {}).items()}}
+    log(f"[bench] batch autotune: {tune_info['picked']} (cached {tune_info['cached']})")
     opts = _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed")
 
     def timed(b: BatchAttempts):
@@ -404,7 +408,8 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
     if not variants:
         out = {"workload": WORKLOAD_BATCH, "films": len(films), "n_interior_total": int(ns.sum()),
                "attempts": attempts, "warmup": warmup, "seconds": el, "value": v,
-               "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc}
+               "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc,
+               "autotune": tune_info}
         return out
     # the same attempts without the preconditioner, and with the extrapolating predictor
     unp = BatchAttempts(runner, films, _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed",
@@ -425,9 +430,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
            "unpreconditioned_cell_steps_s": vu,
            "extrapolating_cell_steps_s": ve,
            "extrapolating_newton_iters_per_film_attempt": ex.newton_timed,
-           "autotune": {"picked": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[tune["shape"]],
-                                   "split": tune["split"]}, "cached": tune.get("cached", False),
-                        "jvp_ms_by_shape_and_split": {str(k): round(m, 4) for k, m in tune["trials"].items()}}}
+           "autotune": tune_info}
     if not fp64:
         return out
     # baseline: the plain FP64 step looped over the films, same steps from the same start
