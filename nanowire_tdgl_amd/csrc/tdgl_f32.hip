@@ -234,12 +234,14 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   };
   // plane kp's cells, loaded by load_into, to ring slot `slot` (and slot t of
   // the arrays read on one plane only)
-  auto commit_from = [&](int kp, int slot, int t, const Raw32& a0, const Raw32& a1)
+  auto commit_from = [&](int kp, int slot, int t, const Raw32& a0, const Raw32& a1, bool drain = true)
                          __attribute__((always_inline)) {
-    // every load of the plane has landed here (all are used below); waiting
-    // for all at once tells the compiler's wait tracking that no load is
-    // left in flight, so no register reuse later in the plane waits on one
-    vm_drain();
+    // drain: every load of the plane has landed here (all are used below);
+    // waiting for all at once tells the compiler's wait tracking that no
+    // load is left in flight, so no register reuse later in the plane waits
+    // on one.  The prologue's commits leave the next plane's loads in flight
+    // (the compiler's own waits cover the committed plane's).
+    if (drain) vm_drain();
     Cell<float> v0c, w0c, v1c, w1c;
     if (GEN && F.gen) {
       auto unpack = [](const Raw32& r, Cell<float>& v, Cell<float>& w) {
@@ -268,13 +270,15 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   // compile time.
   Raw32 r0, r1;
   if (IS3D) {
-    // the first two planes' loads go out together (one memory round trip)
-    Raw32 a0, a1;
+    // the first three planes' loads go out together (one memory round trip
+    // before the first step: the third plane's loads stay in flight while
+    // the first two are committed)
+    Raw32 a0, a1, b0, b1;
     load_into(kb - 1, a0, a1);
-    load_into(kb, r0, r1);
-    commit_from(kb - 1, 0, 0, a0, a1);
-    commit_from(kb, 1, 1, r0, r1);
+    load_into(kb, b0, b1);
     load_into(kb + 1, r0, r1);
+    commit_from(kb - 1, 0, 0, a0, a1, false);
+    commit_from(kb, 1, 1, b0, b1, false);
   } else {
     load_into(0, r0, r1);
   }
