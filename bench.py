@@ -405,11 +405,15 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         mix.restore(final)
         log(f"[bench] batched dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.1f} us/launch, "
             f"{roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of peak), traffic {roof['traffic']}")
+    # the whole attempt's algorithmic bytes (every kernel class, averaged over the instrumented
+    # repeat's warm-up and timed attempts) over the timed wall time per attempt
+    whole = (sum(c["bytes"] for c in pc.values()) / (warmup + attempts) / (el / attempts) / 1e9
+             if pc else None)
     if not variants:
         out = {"workload": WORKLOAD_BATCH, "films": len(films), "n_interior_total": int(ns.sum()),
                "attempts": attempts, "warmup": warmup, "seconds": el, "value": v,
                "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc,
-               "autotune": tune_info}
+               "whole_attempt_algorithmic_GBps": whole, "autotune": tune_info}
         return out
     # the same attempts without the preconditioner, and with the extrapolating predictor
     unp = BatchAttempts(runner, films, _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed",
@@ -426,7 +430,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
            "films": len(films), "n_interior_total": int(ns.sum()), "attempts": attempts,
            "warmup": warmup, "seconds": el, "value": v,
            "newton_iters_per_film_attempt": mix.newton_timed,
-           "roofline": roof, "kernel_classes": pc,
+           "roofline": roof, "kernel_classes": pc, "whole_attempt_algorithmic_GBps": whole,
            "unpreconditioned_cell_steps_s": vu,
            "extrapolating_cell_steps_s": ve,
            "extrapolating_newton_iters_per_film_attempt": ex.newton_timed,
