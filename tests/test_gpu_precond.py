@@ -213,7 +213,7 @@ def test_sweep_tiles_agree_bitwise(name):
     v = _round32(rand_state(n, nphi, 53, 1.0, 1.0))
     gx, gv = ctx.to_internal(x).float(), ctx.to_internal(v).float()
     ref = None
-    for shape in (0, 1, 2):
+    for shape in (0, 1, 2, 3):
         for kz in (1, 2, 5):
             ctx.set_sweep_tile(shape)
             ctx.set_stencil_kz(kz)
@@ -223,6 +223,39 @@ def test_sweep_tiles_agree_bitwise(name):
                 ref = (w, z)
             assert w.equal(ref[0]) and z.equal(ref[1]), (shape, kz)
     ctx.set_sweep_tile(0)
+
+
+@pytest.mark.parametrize("dims", [(17, 40, 5), (18, 35, 4), (34, 21, 6), (51, 23, 5), (68, 30, 4),
+                                  (101, 19, 3), (160, 9, 2)])
+def test_per_film_tiles_agree_with_64x4(dims):
+    """Per-film tiles of any width (16 to 64 nodes, lanes past W x R idle) give
+    the 64x4 tile's bits, on widths that leave partial tiles and idle lanes."""
+    import torch
+    import nanowire_tdgl_amd as T
+    from nanowire_tdgl_amd.engine import engine_for
+    Nx, Ny, Nz = dims
+    p = T.SimulationParameters(Nx=Nx, Ny=Ny, Nz=Nz, hx=0.5, hy=0.5, hz=0.5, kappa=2.5)
+    dev = T.Device(p, applied_field=T.AppliedField(Bz=0.35, t_on_fraction=1.0))
+    if Nx > 30:
+        dev.add_hole([(3.0, 2.0), (0.25 * Nx, 2.0), (0.25 * Nx, 0.3 * Ny), (3.0, 0.3 * Ny)])
+    eng = engine_for(p, dev.idx, dev.material)
+    eng.set_field(0.0, 0.0, 0.35)
+    ctx = eng.ctx
+    g = torch.Generator(device="cuda").manual_seed(Nx)
+    x = 0.5 + 0.1 * torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32)
+    v = torch.rand(ctx.len, device="cuda", generator=g, dtype=torch.float32) - 0.5
+    outs = []
+    for shape in (0, 3):
+        ctx.set_sweep_tile(shape)
+        ap = torch.empty_like(x)
+        ctx.jvp_trap32(x, v, 0.2, ap)
+        w, z = torch.zeros_like(x), torch.zeros_like(x)
+        ctx.precond_cheb32(x, v, 0.2, 6, w, z)
+        outs.append((ap, w, z))
+    ctx.set_sweep_tile(0)
+    assert torch.isfinite(outs[0][0]).all()
+    for a, b in zip(outs[0], outs[1]):
+        assert a.equal(b), dims
 
 
 def test_preconditioner_rejects_bad_arguments():
