@@ -134,9 +134,10 @@ int tdgl_set_stencil_kz(tdgl_ctx* ctx, int kz);
 int tdgl_get_stencil_kz(const tdgl_ctx* ctx);
 /* Tuning (no counterpart in the reference): tile shape of the mixed-precision
  * stencil sweeps, nodes per plane: 0 = 64 x 4 (default), 1 = 32 x 8,
- * 2 = 16 x 16, 3 = each film the one of these with the least work per plane
- * (256 lanes per tile, idle ones included, plus a third of a lane per staged
- * halo cell).  Results are the same bits for every shape. */
+ * 2 = 16 x 16, 3 = each film a tile of its own, W x R nodes with W from 16
+ * to 64 and W R <= 256, the one with the fewest lanes per plane (idle ones
+ * included, a little more per lane for narrow tiles).  Results are the same
+ * bits for every shape. */
 int tdgl_set_sweep_tile(tdgl_ctx* ctx, int shape);
 int tdgl_get_sweep_tile(const tdgl_ctx* ctx);
 const char* tdgl_last_error(const tdgl_ctx* ctx);

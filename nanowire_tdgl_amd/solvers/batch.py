@@ -74,8 +74,9 @@ class BatchRunner:
     def tune(self, X=None, splits=((8, 2), (8, 4), (6, 2), (12, 2), (16, 4)), shapes=(0, 1, 2, 3),
              reps: int = 5, cache: Optional[str] = "default") -> dict:
         """Autotuner: pick the mixed-precision sweeps' tile shape (0 = 64x4,
-        1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film the shape with the
-        fewest idle lanes and staged cells) and work split (the deepest and
+        1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film a tile of its own
+        width, 16 to 64 nodes, with the fewest lanes per plane) and work split
+        (the deepest and
         the shallowest z piece per block; tdgl_batch_set_sweep_split).
 
         Every shape with every split, and the current setting, is timed on the
