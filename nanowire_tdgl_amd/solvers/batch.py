@@ -76,8 +76,8 @@ class BatchRunner:
         """Autotuner: pick the mixed-precision sweeps' tile shape (0 = 64x4,
         1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film a tile of its own
         width, 16 to 64 nodes, with the fewest lanes per plane) and work split
-        (the deepest and
-        the shallowest z piece per block; tdgl_batch_set_sweep_split).
+        (the deepest and the shallowest z piece per block;
+        tdgl_batch_set_sweep_split).
 
         Every shape with every split, and the current setting, is timed on the
         batch's own FP32 JVP sweep over all films; the fastest is kept (every
