@@ -22,7 +22,8 @@ with Newton converged (||dx||_inf <= 1e-5; the solution of the same step
 equations), with the FP64 loop at the same tolerances beside it.
 `cpu_baseline` is the CPU oracle (faithful FP64 restatement of tdgl3d) timed
 on whole steps of the batch's smallest films on this host, one thread and all
-cores.
+cores.  `batched.full_run` is the batch's whole configured run (every film's
+300 steps, t = 0 to 60: the field is on until t = 40, then the films relax).
 
 Secondary sections of the same line (each with its own roofline):
   fp64_plain -- the 128x128x16 production film with a 32x32 hole, plain FP64
@@ -55,6 +56,8 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # Off with --no-profile: no second, event-instrumented pass in any section.
 PROFILE = True
+# Off with --no-batch-full-run: the batch's whole configured run (300 steps per film).
+FULL_RUN = True
 METRIC = "cell-steps/s mixed vs FP64 at 1e-6 rel err; achieved HBM GB/s; adjoint s/step"
 
 
@@ -333,15 +336,21 @@ class BatchAttempts:
         self.t, self.cur, self.dt_prev = self.t.copy(), self.cur.copy(), self.dt_prev.copy()
         self.x_prev = None if xp is None else xp.clone()
 
+    def done(self) -> bool:
+        return not (self.t < BATCH_T_STOP - 1e-14).any()
+
     def attempt(self) -> float:
-        """One attempt for every film; returns the cell-steps it accepted."""
+        """One attempt for every film that has not reached t_stop (dt = min(dt,
+        t_stop - t), integrators.py:151); returns the cell-steps it accepted."""
         b = self.runner.batch
+        active = self.t < BATCH_T_STOP - 1e-14
         self.runner.set_fields(self.t, BATCH_T_STOP)
-        dta = self.cur.copy()
+        dta = np.where(active, np.minimum(self.cur, BATCH_T_STOP - self.t), BATCH_DT)
         x_in = self.X.clone() if self.extrapolate else None
-        st = b.trap_step(self.X, dta, self.opts, x_prev=self.x_prev, dt_prev=self.dt_prev)
-        ok = np.array([q.converged for q in st], dtype=bool)
-        self.newton += sum(q.newton_iters for q in st)
+        st = b.trap_step(self.X, dta, self.opts, active.astype(np.int32), x_prev=self.x_prev,
+                         dt_prev=self.dt_prev)
+        ok = np.array([q.converged for q in st], dtype=bool) & active
+        self.newton += sum(q.newton_iters for q, a in zip(st, active) if a)
         if x_in is not None:
             if self.x_prev is None:
                 self.x_prev = x_in
@@ -351,7 +360,7 @@ class BatchAttempts:
             self.dt_prev[ok] = dta[ok]
         self.t[ok] += dta[ok]
         self.cur[ok] = BATCH_DT
-        self.cur[~ok] /= 10.0
+        self.cur[active & ~ok] /= 10.0
         if (self.cur < 1e-8).any():
             raise RuntimeError("dt reduced below dt_min in the batch")
         return float(self.ns[ok].sum())
@@ -415,6 +424,21 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
                "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc,
                "whole_attempt_algorithmic_GBps": whole, "autotune": tune_info}
         return out
+    full = None
+    if FULL_RUN:  # the configured run: every film from t = 0 to its 300th step
+        fr = BatchAttempts(runner, films, opts)
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        accf, nat = 0.0, 0
+        while not fr.done():
+            accf += fr.attempt()
+            nat += 1
+        torch.cuda.synchronize()
+        elf = time.perf_counter() - c0
+        full = {"steps_per_film": int(round(BATCH_T_STOP / BATCH_DT)), "attempts": nat, "seconds": elf,
+                "cell_steps_s": accf / elf, "newton_iters_per_film_step": fr.newton / (len(films) * BATCH_T_STOP / BATCH_DT)}
+        log(f"[bench] batched mixed, whole configured run: {full['cell_steps_s']:.4g} cell-steps/s "
+            f"({nat} attempts in {elf:.2f} s)")
     # the same attempts without the preconditioner, and with the extrapolating predictor
     unp = BatchAttempts(runner, films, _native.solver_opts(tol_gcr=BATCH_TOL, precision="mixed",
                                                            precond="none"))
@@ -431,6 +455,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
            "warmup": warmup, "seconds": el, "value": v,
            "newton_iters_per_film_attempt": mix.newton_timed,
            "roofline": roof, "kernel_classes": pc, "whole_attempt_algorithmic_GBps": whole,
+           "full_run": full,
            "unpreconditioned_cell_steps_s": vu,
            "extrapolating_cell_steps_s": ve,
            "extrapolating_newton_iters_per_film_attempt": ex.newton_timed,
@@ -681,6 +706,8 @@ def main() -> None:
     ap.add_argument("--cpu-workers", type=int, default=min(os.cpu_count() or 1, 16),
                     help="processes of the all-cores CPU figure (default: all cores, at most 16)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-batch-full-run", action="store_true",
+                    help="skip the batch's whole configured run (300 steps per film)")
     ap.add_argument("--no-batch-fp64", action="store_true",
                     help="skip the FP64 loop baseline and the error report of the batch")
     ap.add_argument("--no-batch-variants", action="store_true",
@@ -701,8 +728,9 @@ def main() -> None:
     if args.whole_run:
         whole_run()
         return
-    global PROFILE
+    global PROFILE, FULL_RUN
     PROFILE = not args.no_profile
+    FULL_RUN = not args.no_batch_full_run
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU split of "
               "this path is a design note, not a measured run", file=sys.stderr)
