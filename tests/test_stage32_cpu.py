@@ -34,3 +34,20 @@ def test_stage32_matches_boundary_rules(tmp_path):
     run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     print(run.stdout[-2000:])
     assert run.returncode == 0 and run.stdout.strip().startswith("OK"), run.stdout[-2000:]
+
+
+@pytest.mark.skipif(_hipcc() is None, reason="hipcc not installed")
+def test_per_film_sweep_tiles_fit_the_block(tmp_path):
+    """sweep_film_tile (tdgl_launch.hpp) on every film from 1 to 400 nodes
+    across and along: the tile fits the 256-lane block and its shared memory,
+    every lane's LDS cell stays inside the staged cells, and the pick is the
+    cost model's argmin."""
+    exe = tmp_path / "sweep_tile_check"
+    cmd = [_hipcc(), "-O1", "-std=c++17", f"-I{ROOT / 'include'}",
+           f"-I{ROOT / 'nanowire_tdgl_amd' / 'csrc'}", str(ROOT / "tests" / "sweep_tile_check.cpp"),
+           "-o", str(exe)]
+    build = subprocess.run(cmd, capture_output=True, text=True, cwd=str(tmp_path))
+    assert build.returncode == 0, build.stderr[-3000:]
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(run.stdout[-2000:])
+    assert run.returncode == 0 and run.stdout.strip().startswith("OK"), run.stdout[-2000:]
