@@ -619,11 +619,12 @@ def section_adjoint(steps, every, fd=True, predictor=None, mask_cells=20):
             diffs.append(abs((Lw(wp) - Lw(wm)) / (2 * eps) - gw[m]))
             if (c + 1) % 5 == 0:
                 log(f"[bench] mask cells: {c + 1} of {len(cells)} done")
-        scale = float(np.max(np.abs(gw)))
-        out["fd"]["mask_cells"] = {"cells": len(cells), "max_abs_diff": float(max(diffs)), "grad_scale": scale,
-                                   "max_diff_over_scale": float(max(diffs)) / scale}
-        log(f"[bench] adjoint vs central differences on {len(cells)} mask cells: max |diff| {max(diffs):.3e} "
-            f"({max(diffs) / scale:.2e} of max |dL/dw|)")
+        if diffs:
+            scale = float(np.max(np.abs(gw)))
+            out["fd"]["mask_cells"] = {"cells": len(cells), "max_abs_diff": float(max(diffs)),
+                                       "grad_scale": scale, "max_diff_over_scale": float(max(diffs)) / scale}
+            log(f"[bench] adjoint vs central differences on {len(cells)} mask cells: max |diff| "
+                f"{max(diffs):.3e} ({max(diffs) / scale:.2e} of max |dL/dw|)")
     return out
 
 
