@@ -239,6 +239,53 @@ def test_trajectory_gradient_matches_finite_differences(predictor):
     assert np.all(gw[ghost] == 0.0)
 
 
+@pytest.mark.parametrize("variant", ["hole", "periodic_x", "film2d"])
+def test_trajectory_gradient_variants(variant):
+    """The trajectory gradient against central differences on the boundary
+    branches the plain film does not reach: a hole (zeroed links, hole rows),
+    a periodic axis (wrapped ghosts, no field flux on it) and a 2-D film (no z
+    links)."""
+    import torch
+    import nanowire_tdgl_amd as T
+    from nanowire_tdgl_amd.adjoint import AdjointProblem, TrapezoidalBz, midplane_target_loss
+    nz = 1 if variant == "film2d" else 5
+    p = T.SimulationParameters(Nx=16, Ny=14, Nz=nz, hx=0.5, hy=0.5, hz=0.5, kappa=2.5,
+                               periodic_x=(variant == "periodic_x"))
+    holes = [([(2.0, 2.0), (4.0, 2.0), (4.0, 4.0), (2.0, 4.0)], None)] if variant == "hole" else []
+    n = p.n_interior
+    rng = np.random.default_rng(7)
+    x0 = np.concatenate([1.0 + 0.01 * (rng.standard_normal(n) + 1j * rng.standard_normal(n)),
+                         np.zeros(p.n_state - n, complex)])
+    w0 = 0.05 * np.random.default_rng(8).uniform(size=p.dim_x)
+    prob = AdjointProblem(p, x0, dt=0.2, steps=4, checkpoint_every=2, holes=holes)
+    kappa = torch.tensor(2.5, dtype=torch.float64, requires_grad=True)
+    bz = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
+    loss = midplane_target_loss(prob, TrapezoidalBz.apply(prob, kappa, bz, w), width=3.0)
+    loss.backward()
+
+    def L(k, b, ww):
+        _, bzo = prob.forward(k, b, ww)
+        return float(midplane_target_loss(prob, bzo, width=3.0))
+
+    eps = 1e-4
+    fd_k = (L(2.5 + eps, 0.3, w0) - L(2.5 - eps, 0.3, w0)) / (2 * eps)
+    fd_b = (L(2.5, 0.3 + eps, w0) - L(2.5, 0.3 - eps, w0)) / (2 * eps)
+    print(f"{variant}: dL/dkappa adjoint {kappa.grad.item():.10e} fd {fd_k:.10e}; "
+          f"dL/dBz adjoint {bz.grad.item():.10e} fd {fd_b:.10e}")
+    assert abs(kappa.grad.item() - fd_k) <= 1e-5 * abs(fd_k)
+    assert abs(bz.grad.item() - fd_b) <= 1e-5 * abs(fd_b)
+    gw = w.grad.numpy()
+    full = prob.device.idx.interior_to_full
+    scale = np.max(np.abs(gw))
+    for m in np.random.default_rng(9).choice(full, size=3, replace=False):
+        wp, wm = w0.copy(), w0.copy()
+        wp[m] += eps
+        wm[m] -= eps
+        fd = (L(2.5, 0.3, wp) - L(2.5, 0.3, wm)) / (2 * eps)
+        assert abs(gw[m] - fd) <= 1e-5 * scale, (m, gw[m], fd)
+
+
 def test_backward_is_deterministic_and_guards_stale_forwards():
     """Two backward passes of the same forward give the same bits for
     dL/d(kappa, Bz, w): the VJP and the B_z cell adjoint gather in a fixed
