@@ -97,6 +97,54 @@ def test_batch_reproduces_each_film_bitwise(precond):
         assert runner.batch.view(X, f).equal(got[f]), f
 
 
+@pytest.mark.parametrize("tile", [0, 3])
+def test_batch_with_general_field_film_reproduces_each_film_bitwise(tile):
+    """One film of the batch carries full-grid field vectors (a non-uniform
+    B_z on its faces): the batch then runs the general-field sweeps, per-cell
+    ghost rules for that film and the uniform-field masks for the others, and
+    still reproduces every film's own run bit for bit."""
+    from nanowire_tdgl_amd import _native
+    from nanowire_tdgl_amd.engine import engine_for
+    from nanowire_tdgl_amd.physics.applied_field import build_boundary_field_vectors
+    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    films = _films()
+    d1 = films[1].device
+    Bx, By, Bz = (np.array(v, dtype=np.float64) for v in build_boundary_field_vectors(0.0, 0.0, 0.6, d1.params, d1.idx))
+    Bz = Bz * (1.0 + 0.3 * np.random.default_rng(11).uniform(size=Bz.shape))  # non-uniform on the faces
+    steps, dt = 2, 0.2
+    opts = _native.solver_opts(precision="mixed")
+
+    def fields(engines):
+        for f, (film, eng) in enumerate(zip(films, engines)):
+            if f == 1:
+                eng.ctx.set_field_vectors(Bx, By, Bz)
+                eng._field = None
+            else:
+                eng.set_field(*film.device.applied_field.evaluate(0.0, 100.0))
+
+    engines = [engine_for(f.device.params, f.device.idx, f.device.material) for f in films]
+    want = []
+    for f, (film, eng) in enumerate(zip(films, engines)):
+        eng.ctx.set_sweep_tile(tile)
+        fields(engines)
+        X = eng.ctx.to_internal(film.x0)
+        for _ in range(steps):
+            st = eng.ctx.trap_step(X, dt, opts)
+            assert st.converged == 1
+        want.append(X)
+    runner = BatchRunner([f.device for f in films])
+    runner.batch.set_tile(tile)
+    X = runner.pack([f.x0 for f in films])
+    for _ in range(steps):
+        fields(runner.engines)
+        st = runner.batch.trap_step(X, dt, opts)
+        assert all(q.converged for q in st)
+    for f in range(len(films)):
+        assert runner.batch.view(X, f).equal(want[f]), f
+    for eng in engines:
+        eng.ctx.set_sweep_tile(0)
+
+
 def test_batch_extrapolating_predictor_reproduces_each_film_bitwise():
     """Newton started from each film's extrapolated last step: the batch still
     reproduces every film's batch-of-one steps bit for bit (the first step has
