@@ -49,6 +49,7 @@ struct tdgl_batch {
   int* d_item_film = nullptr;
   int* d_sblk_film = nullptr;
   int* d_act = nullptr;
+  int* d_sel = nullptr;     // per film: selected for a batched copy (k_bcopy_films)
   double* d_res = nullptr;
   double* d_dtv = nullptr;
   double* d_ext = nullptr;  // per film: dt / dt_prev of the extrapolating predictor (0: off)
@@ -74,7 +75,7 @@ struct tdgl_batch {
   float** d_qtab = nullptr;  // kMaxKrylov entries
 
   double* h_rep = nullptr;  // pinned: nfilm * kReportLen
-  int* h_act = nullptr;     // pinned: 3 * nfilm (set_act staging + two readback slots)
+  int* h_act = nullptr;     // pinned: 4 * nfilm (set_act staging, two readback slots, copy selection)
 
   // Work runs on a private stream (graph capture needs a non-legacy stream);
   // events order it after, and the caller's stream after it.
@@ -296,6 +297,18 @@ int ensure_q(tdgl_batch* B, int i) {
 int set_act(tdgl_batch* B, const std::vector<int>& act) {
   memcpy(B->h_act, act.data(), sizeof(int) * B->nfilm);
   BCHK(hipMemcpyAsync(B->d_act, B->h_act, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->ps));
+  return TDGL_OK;
+}
+
+// dst[film] = src[film] for the films in `sel`, in one launch (a device copy
+// per film cost a launch each).  The selection's pinned staging slot is
+// rewritten only after a later report() has synchronised the stream.
+int copy_films(tdgl_batch* B, const std::vector<int>& sel, const double* src, double* dst) {
+  int* h = B->h_act + 3 * (size_t)B->nfilm;
+  memcpy(h, sel.data(), sizeof(int) * B->nfilm);
+  BCHK(hipMemcpyAsync(B->d_sel, h, sizeof(int) * B->nfilm, hipMemcpyHostToDevice, B->ps));
+  launch_bcopy_films(kview(B), B->d_sel, src, dst, B->ps);
+  BCHK(hipGetLastError());
   return TDGL_OK;
 }
 
@@ -880,6 +893,8 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
       }
       if ((rc = newton_eval(B, ok, S))) return rc;  // newton.py:109-113
       if ((rc = report(B))) return rc;
+      std::vector<int> conv(F, 0);
+      int nconv = 0;
       for (int f = 0; f < F; ++f) {
         if (!ok[f]) continue;
         const double* r = rep(B, f);
@@ -888,15 +903,16 @@ int batch_step_impl(tdgl_batch* B, double* X, const double* dt, const int* activ
         if (!std::isfinite(S[f].g_inf) || !std::isfinite(S[f].dx_inf))
           return bfail(B, TDGL_E_NONFINITE, "Newton residual is not finite");
         if (S[f].g_inf <= o->newton_tol_f && S[f].dx_inf <= o->newton_tol_dx) {  // newton.py:118
-          const FilmDesc& d = B->hf[f];
-          BCHK(hipMemcpyAsync(X + d.off, B->w_x + d.off, sizeof(double) * (size_t)d.len,
-                              hipMemcpyDeviceToDevice, B->ps));
+          conv[f] = 1;
+          ++nconv;
           S[f].bytes_moved += 2.0 * s64(B, f);
           S[f].converged = 1;
           S[f].newton_iters = it;
           newton[f] = 0;
         }
       }
+      // the films that converged take their iterate: one launch for all of them
+      if (nconv && (rc = copy_films(B, conv, B->w_x, X))) return rc;
     }
     bool any = false;
     for (int f = 0; f < F; ++f) any = any || newton[f];
@@ -920,6 +936,7 @@ int batch_init(tdgl_batch* B) {
   int rc;
   if ((rc = balloc(B, (void**)&B->d_films, sizeof(FilmDesc) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_act, sizeof(int) * F))) return rc;
+  if ((rc = balloc(B, (void**)&B->d_sel, sizeof(int) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_res, sizeof(double) * kResLen * (size_t)F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_dtv, sizeof(double) * 2 * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_ext, sizeof(double) * F))) return rc;
@@ -930,7 +947,7 @@ int batch_init(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_rep, sizeof(double) * kReportLen * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_cheb, sizeof(double) * kChebStride * F))) return rc;
   BCHK(hipHostMalloc((void**)&B->h_rep, sizeof(double) * kReportLen * F, hipHostMallocDefault));
-  BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * 3 * F, hipHostMallocDefault));
+  BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * 4 * F, hipHostMallocDefault));
   BCHK(hipStreamCreateWithFlags(&B->ps, hipStreamNonBlocking));
   BCHK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
   BCHK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
@@ -1103,7 +1120,7 @@ int tdgl_batch_destroy(tdgl_batch* B) {
   if (B->ps) (void)hipStreamDestroy(B->ps);
   for (void* p : {(void*)B->d_films, (void*)B->d_item_film, (void*)B->d_pseg,
                   (void*)B->d_sblk_film,
-                  (void*)B->d_act,
+                  (void*)B->d_act, (void*)B->d_sel,
                   (void*)B->d_res, (void*)B->d_dtv, (void*)B->d_ext, (void*)B->d_fstat, (void*)B->d_maxk,
                   (void*)B->d_iter, (void*)B->d_nact, (void*)B->d_pitem, (void*)B->d_pm,
                   (void*)B->d_H, (void*)B->d_rep, (void*)B->w_x, (void*)B->w_gam, (void*)B->x32,

@@ -892,6 +892,19 @@ __global__ __launch_bounds__(kRedThreads) void k_bextrap(BatchK bk, const double
   for (long long i = e0; i < len; i += estride) O[i] = X[i] + r * (X[i] - XP[i]);
 }
 
+// dst = src on the films selected by sel (the film's own elements; padding is
+// never touched).
+__global__ __launch_bounds__(kRedThreads) void k_bcopy_films(BatchK bk, const int* sel, const double* src,
+                                                             double* dst) {
+  const int f = bk.sblk_film[blockIdx.x];
+  if (!sel[f]) return;
+  const FilmDesc* fd = bk.films + f;
+  const long long off = fd->off, len = fd->len;
+  const long long e0 = (long long)(blockIdx.x - fd->sb0) * kRedThreads + threadIdx.x;
+  const long long estride = (long long)fd->nsb * kRedThreads;
+  for (long long i = e0; i < len; i += estride) dst[off + i] = src[off + i];
+}
+
 // One film's FP32 vector between the internal layout and the mixed method's
 // (to_l32: out = the mixed layout of in; else the internal layout of in).
 __global__ __launch_bounds__(256) void k_l32_permute(const float* in, float* out, long long n, int nphi,
@@ -910,6 +923,10 @@ void launch_l32_permute(const float* in, float* out, long long n, int nphi, bool
   const long long len = (2 + nphi) * n;
   const int blocks = (int)std::min<long long>((len + 255) / 256, 4096);
   launch(k_l32_permute, dim3(blocks), dim3(256), 0, st, in, out, n, nphi, to_l32 ? 1 : 0);
+}
+
+void launch_bcopy_films(const BatchK& bk, const int* sel, const double* src, double* dst, hipStream_t st) {
+  launch(k_bcopy_films, dim3(bk.nsblk), dim3(kRedThreads), 0, st, bk, sel, src, dst);
 }
 
 void launch_bextrap(const BatchK& bk, const double* x, const double* xp, const double* ratio, double* out,

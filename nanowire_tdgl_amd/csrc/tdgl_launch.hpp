@@ -247,6 +247,8 @@ struct ChebStep {
 };
 void launch_bcheb32(const BatchK& bk, const SweepPart& part, bool is3d, const float* x32, const float* p32,
                     float* const* Q, const int* iter, const ChebStep& cs, hipStream_t st);
+// dst = src on the films with sel[f] != 0 (each film's own elements)
+void launch_bcopy_films(const BatchK& bk, const int* sel, const double* src, double* dst, hipStream_t st);
 // out = x + ratio (x - xp): the extrapolating predictor (ratio[f] <= 0: film
 // f keeps out as it is)
 // one film's FP32 vector between the internal layout and the mixed method's
