@@ -469,7 +469,9 @@ class Batch:
     ctx.len]`` in the internal layout.
     """
 
-    def __init__(self, contexts):
+    def __init__(self, contexts, stream=None):
+        """``stream``: the torch stream the batch orders its work after and
+        before (default: the first context's)."""
         import torch
         self.lib = load()
         self.torch = torch
@@ -483,7 +485,7 @@ class Batch:
             raise ValueError(f"tdgl_batch_create failed (code {rc}): films must all be 3-D or all 2-D")
         self.h = h
         self.device = self.contexts[0].device
-        self.stream = self.contexts[0].stream
+        self.stream = self.contexts[0].stream if stream is None else stream
         self._call("tdgl_batch_set_stream", C.c_void_p(self.stream.cuda_stream))
 
     def _call(self, name: str, *args) -> None:

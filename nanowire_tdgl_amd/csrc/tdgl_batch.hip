@@ -73,6 +73,7 @@ struct tdgl_batch {
   std::vector<int> cheb_degf;          // per film: its own degree
   std::vector<float*> Q;
   float** d_qtab = nullptr;  // kMaxKrylov entries
+  std::vector<void*> retired;  // replaced device buffers, freed with the batch
 
   double* h_rep = nullptr;  // pinned: nfilm * kReportLen
   int* h_act = nullptr;     // pinned: 4 * nfilm (set_act staging, two readback slots, copy selection)
@@ -118,12 +119,20 @@ int bfail(tdgl_batch* B, int code, const std::string& msg) {
       return bfail(B, TDGL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
   } while (0)
 
+// Device buffers are cleared and filled on the batch's own stream, and buffers
+// a batch replaces are kept until it is destroyed: no call of a step touches
+// the legacy stream or frees memory (which would synchronise the device), so
+// batches stepped from several host threads at once do not disturb each
+// other's work or graph captures.
 int balloc(tdgl_batch* B, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return bfail(B, TDGL_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  e = hipMemset(*p, 0, bytes);
-  if (e != hipSuccess) return bfail(B, TDGL_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  e = hipMemsetAsync(*p, 0, bytes, B->ps);
+  if (e != hipSuccess) return bfail(B, TDGL_E_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
   return TDGL_OK;
+}
+void retire(tdgl_batch* B, void* p) {
+  if (p) B->retired.push_back(p);
 }
 
 constexpr int kStencilTX = 64, kStencilTY = 4;
@@ -231,7 +240,7 @@ int build_layout(tdgl_batch* B) {
   B->nsblk = (int)B->sblk_film.size();
   B->part_key.clear();  // the sweep tiles may have changed
   for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
-    if (p) BCHK(hipFree(p));
+    retire(B, p);
   B->d_item_film = B->d_sblk_film = nullptr;
   B->d_pitem = B->d_pm = nullptr;
   int rc;
@@ -239,8 +248,11 @@ int build_layout(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_sblk_film, sizeof(int) * B->nsblk))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pitem, sizeof(double) * 3 * (size_t)B->nitems))) return rc;
   if ((rc = balloc(B, (void**)&B->d_pm, sizeof(double) * (kMaxKrylov + 2) * (size_t)B->nsblk))) return rc;
-  BCHK(hipMemcpy(B->d_item_film, B->item_film.data(), sizeof(int) * B->nitems, hipMemcpyHostToDevice));
-  BCHK(hipMemcpy(B->d_sblk_film, B->sblk_film.data(), sizeof(int) * B->nsblk, hipMemcpyHostToDevice));
+  BCHK(hipMemcpyAsync(B->d_item_film, B->item_film.data(), sizeof(int) * B->nitems, hipMemcpyHostToDevice,
+                      B->ps));
+  BCHK(hipMemcpyAsync(B->d_sblk_film, B->sblk_film.data(), sizeof(int) * B->nsblk, hipMemcpyHostToDevice,
+                      B->ps));
+  BCHK(hipStreamSynchronize(B->ps));  // the host tables may change before the next copy
   return TDGL_OK;
 }
 
@@ -289,7 +301,8 @@ int ensure_q(tdgl_batch* B, int i) {
     if ((rc = balloc(B, (void**)&q, sizeof(float) * (size_t)B->total_len))) return rc;
     B->Q.push_back(q);
     const int k = (int)B->Q.size() - 1;
-    BCHK(hipMemcpy(B->d_qtab + k, &B->Q[k], sizeof(float*), hipMemcpyHostToDevice));
+    BCHK(hipMemcpyAsync(B->d_qtab + k, &B->Q[k], sizeof(float*), hipMemcpyHostToDevice, B->ps));
+    BCHK(hipStreamSynchronize(B->ps));  // B->Q may move on its next push_back
   }
   return TDGL_OK;
 }
@@ -393,13 +406,16 @@ int ensure_parts(tdgl_batch* B, const std::vector<int>& act) {
   }
   BCHK(hipStreamSynchronize(B->ps));  // no queued sweep still reads the old tables
   if (seg.size() > B->pseg_cap) {
-    if (B->d_pseg) BCHK(hipFree(B->d_pseg));
+    retire(B, B->d_pseg);
     B->d_pseg = nullptr;
     int rc;
     if ((rc = balloc(B, (void**)&B->d_pseg, sizeof(int4) * seg.size()))) return rc;
     B->pseg_cap = seg.size();
   }
-  if (!seg.empty()) BCHK(hipMemcpy(B->d_pseg, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice));
+  if (!seg.empty()) {
+    BCHK(hipMemcpyAsync(B->d_pseg, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice, B->ps));
+    BCHK(hipStreamSynchronize(B->ps));  // seg is a local
+  }
   B->part.assign(D + 1, SweepPart{});
   for (int m = 0; m <= D; ++m) B->part[m] = SweepPart{B->d_pseg + g0[m], nb[m]};
   B->part_key = key;
@@ -934,6 +950,7 @@ bool nfilm_is_own(const tdgl_batch* B) { return B->nfilm == 1 && B->own_stream_f
 int batch_init(tdgl_batch* B) {
   const int F = B->nfilm;
   int rc;
+  BCHK(hipStreamCreateWithFlags(&B->ps, hipStreamNonBlocking));
   if ((rc = balloc(B, (void**)&B->d_films, sizeof(FilmDesc) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_act, sizeof(int) * F))) return rc;
   if ((rc = balloc(B, (void**)&B->d_sel, sizeof(int) * F))) return rc;
@@ -948,7 +965,6 @@ int batch_init(tdgl_batch* B) {
   if ((rc = balloc(B, (void**)&B->d_cheb, sizeof(double) * kChebStride * F))) return rc;
   BCHK(hipHostMalloc((void**)&B->h_rep, sizeof(double) * kReportLen * F, hipHostMallocDefault));
   BCHK(hipHostMalloc((void**)&B->h_act, sizeof(int) * 4 * F, hipHostMallocDefault));
-  BCHK(hipStreamCreateWithFlags(&B->ps, hipStreamNonBlocking));
   BCHK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
   BCHK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
   BCHK(hipEventCreateWithFlags(&B->ev_act[0], hipEventDisableTiming));
@@ -1128,6 +1144,7 @@ int tdgl_batch_destroy(tdgl_batch* B) {
                   (void*)B->t32[0], (void*)B->t32[1]})
     if (p) (void)hipFree(p);
   for (float* q : B->Q) (void)hipFree(q);
+  for (void* p : B->retired) (void)hipFree(p);
   if (B->h_rep) (void)hipHostFree(B->h_rep);
   if (B->h_act) (void)hipHostFree(B->h_act);
   B->own_prof.destroy();

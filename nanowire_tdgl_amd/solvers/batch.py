@@ -41,12 +41,14 @@ class BatchResult:
 
 
 class BatchRunner:
-    """A tdgl_batch over Devices, with their FilmEngines kept alive."""
+    """A tdgl_batch over Devices, with their FilmEngines kept alive; ``stream``
+    (a torch stream) is the one the batch's steps are ordered on (default: the
+    stream current when the first film's engine was made)."""
 
-    def __init__(self, devices: Sequence):
+    def __init__(self, devices: Sequence, stream=None):
         self.devices = list(devices)
         self.engines = [engine_for(d.params, d.idx, getattr(d, "material", None)) for d in self.devices]
-        self.batch = _native.Batch([e.ctx for e in self.engines])
+        self.batch = _native.Batch([e.ctx for e in self.engines], stream=stream)
 
     def pack(self, x0s):
         return self.batch.pack([e.ctx.to_internal(x) for e, x in zip(self.engines, x0s)])
@@ -121,6 +123,79 @@ class BatchRunner:
         return dict(res, kz=kz, cached=False)
 
 
+def partition_films(sizes: Sequence[int], groups: int) -> list:
+    """Split films (their node counts ``sizes``) into ``groups`` groups of
+    near-equal total size: the largest film first, each film to the group with
+    the fewest nodes so far (ties to the lowest group).  Returns the groups as
+    ascending lists of film indices; no group is empty when groups <= films."""
+    if groups < 1:
+        raise ValueError("groups must be >= 1")
+    groups = min(groups, len(sizes))
+    load = [0] * groups
+    parts: list = [[] for _ in range(groups)]
+    for i in sorted(range(len(sizes)), key=lambda i: (-sizes[i], i)):
+        k = min(range(groups), key=lambda k: (load[k], k))
+        parts[k].append(i)
+        load[k] += int(sizes[i])
+    return [sorted(p) for p in parts]
+
+
+class BatchGroups:
+    """The films of a batch split into G sub-batches that run concurrently.
+
+    Each group (:func:`partition_films`, balanced by nodes) is a
+    :class:`BatchRunner` of its own on a torch stream of its own, stepped from
+    a host thread of its own (the library's calls release the GIL), so one
+    group's host work -- the Newton control flow, the converged-film
+    bookkeeping, the readbacks of a lockstep attempt -- overlaps the other
+    groups' kernels, and their kernels share the GPU.  A film's arithmetic does
+    not depend on the other films of its batch, so every film's result is the
+    same bit for bit whatever the grouping."""
+
+    def __init__(self, devices: Sequence, groups: int):
+        import torch
+        devices = list(devices)
+        sizes = [d.params.n_interior for d in devices]
+        self.parts = partition_films(sizes, groups)
+        self.streams = [torch.cuda.Stream() for _ in self.parts]
+        self.runners = []
+        for part, st in zip(self.parts, self.streams):
+            with torch.cuda.stream(st):
+                self.runners.append(BatchRunner([devices[i] for i in part], stream=st))
+
+    def __len__(self) -> int:
+        return len(self.parts)
+
+    def run(self, fn) -> list:
+        """``fn(k, runner)`` for every group k at once, each in its own thread
+        with the group's stream current; returns the results in group order
+        once every group has finished (the first exception is raised again)."""
+        import threading
+        import torch
+        out: list = [None] * len(self.parts)
+        err: list = []
+
+        def work(k):
+            try:
+                with torch.cuda.stream(self.streams[k]):
+                    out[k] = fn(k, self.runners[k])
+                    self.streams[k].synchronize()
+            except BaseException as e:  # re-raised in the caller's thread
+                err.append(e)
+
+        if len(self.parts) == 1:
+            work(0)
+        else:
+            threads = [threading.Thread(target=work, args=(k,)) for k in range(len(self.parts))]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+        if err:
+            raise err[0]
+        return out
+
+
 def _cache_path(cache) -> Optional[Path]:
     if cache is None:
         return None
@@ -153,7 +228,7 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
                       dt_min: float = 1e-8, max_attempts: Optional[int] = None,
                       tune: bool = False, runner: Optional[BatchRunner] = None,
                       keep_log: bool = False, precond="chebyshev",
-                      predictor: str = "euler") -> BatchResult:
+                      predictor: str = "euler", groups: int = 1) -> BatchResult:
     """Advance every film from t_start to t_stop (mixed precision, lockstep).
 
     The Newton/TGCR parameters mean what they mean in
@@ -161,17 +236,58 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     the stencil autotuner first; ``max_attempts`` bounds the number of
     lockstep attempts (None: until every film reaches t_stop); ``precond`` as
     in ``trapezoidal`` (each film takes its own automatic degree);
-    ``predictor`` as in ``trapezoidal``, per film."""
+    ``predictor`` as in ``trapezoidal``, per film.
+
+    ``groups`` > 1 runs the films as that many concurrent sub-batches
+    (:class:`BatchGroups`; ``runner`` may then be a BatchGroups of as many
+    groups): every film's result is the same bit for bit, the lockstep and
+    ``max_attempts`` hold per group, ``attempts`` is the largest group's count
+    and ``log[a][f]`` is None for a film whose group made fewer attempts."""
     if predictor not in ("euler", "extrapolate"):
         raise ValueError("predictor must be 'euler' or 'extrapolate'")
-    runner = runner or BatchRunner([f.device for f in films])
-    F = len(films)
-    X = runner.pack([f.x0 for f in films])
-    if tune:
-        runner.tune()
     opts = _native.solver_opts(newton_tol_f, newton_tol_dx, newton_max_iter, tol_gcr, 1e-4,
                                tgcr_max_iter, precision="mixed", mixed_tol_floor=mixed_tol_floor,
                                precond=precond)
+    kw = dict(t_start=t_start, t_stop=t_stop, dt=dt, dt_min=dt_min, max_attempts=max_attempts,
+              keep_log=keep_log, predictor=predictor, opts=opts)
+    if groups == 1 and not isinstance(runner, BatchGroups):
+        runner = runner or BatchRunner([f.device for f in films])
+        X = runner.pack([f.x0 for f in films])
+        if tune:
+            runner.tune()
+        r = _advance(runner, X, list(range(len(films))), **kw)
+        r.states = [runner.unpack(X, f) for f in range(len(films))]
+        return r
+    if runner is not None and not isinstance(runner, BatchGroups):
+        raise ValueError("with groups > 1, runner must be a BatchGroups")
+    grp = runner or BatchGroups([f.device for f in films], groups)
+    import torch
+    Xs = []
+    for part, r, st in zip(grp.parts, grp.runners, grp.streams):
+        with torch.cuda.stream(st):
+            Xs.append(r.pack([films[i].x0 for i in part]))
+        if tune:  # each group timed alone, before they run together
+            r.tune()
+    torch.cuda.synchronize()
+    res = grp.run(lambda k, r: _advance(r, Xs[k], grp.parts[k], **kw))
+    F = len(films)
+    states, times, steps = [None] * F, [0.0] * F, [0] * F
+    attempts = max(r.attempts for r in res)
+    log = [[None] * F for _ in range(attempts)] if keep_log else []
+    for part, r, run, X in zip(grp.parts, res, grp.runners, Xs):
+        for j, f in enumerate(part):
+            states[f], times[f], steps[f] = run.unpack(X, j), r.times[j], r.steps[j]
+            for a, row in enumerate(r.log):
+                log[a][f] = row[j]
+    return BatchResult(states, times, steps, attempts, log)
+
+
+def _advance(runner: BatchRunner, X, ids, *, t_start, t_stop, dt, dt_min, max_attempts, keep_log,
+             predictor, opts) -> BatchResult:
+    """The lockstep attempts of trapezoidal_batch over one tdgl_batch, on its
+    packed state X (advanced in place); ``ids`` are the films' numbers in the
+    caller's list (for messages).  The result's states are left empty."""
+    F = runner.batch.nfilm
     t = np.full(F, float(t_start))
     cur = np.full(F, float(dt))
     steps = np.zeros(F, dtype=np.int64)
@@ -207,6 +323,6 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
             else:
                 cur[f] /= 10.0
                 if cur[f] < dt_min:
-                    raise RuntimeError(f"film {f}: dt reduced below dt_min={dt_min:.1e} at t={t[f]:.6e}")
-    states = [runner.unpack(X, f) for f in range(F)]
-    return BatchResult(states, list(t), list(steps), attempts, log)
+                    raise RuntimeError(f"film {ids[f]}: dt reduced below dt_min={dt_min:.1e} "
+                                       f"at t={t[f]:.6e}")
+    return BatchResult([], list(t), list(steps), attempts, log)

@@ -145,3 +145,22 @@ def test_bench_reads_the_committed_traffic_lines():
     assert fp64["mgs"] > 1e6
     roof = {"kernel": "cheb", "traffic": None}
     assert bench.with_traffic(dict(roof), "cheb_sweeps_mixed_sections.md", "batched")["traffic"] > 1e6
+
+
+def test_partition_films_covers_every_film_once_and_balances():
+    """The concurrent-group split: every film in exactly one group, groups
+    ascending, none empty, and the heaviest group within one film of the mean
+    (largest-first greedy)."""
+    from nanowire_tdgl_amd.solvers.batch import partition_films
+    rng = np.random.default_rng(5)
+    for F, G in [(1, 1), (1, 3), (5, 2), (64, 3), (64, 8), (7, 7)]:
+        sizes = [int(x) for x in rng.integers(100, 100000, F)]
+        parts = partition_films(sizes, G)
+        assert len(parts) == min(G, F)
+        assert sorted(i for p in parts for i in p) == list(range(F))
+        assert all(p == sorted(p) and p for p in parts)
+        loads = [sum(sizes[i] for i in p) for p in parts]
+        assert max(loads) <= sum(sizes) / len(parts) + max(sizes)
+    assert partition_films([5, 9, 1, 3, 3, 7], 3) == [[1], [4, 5], [0, 2, 3]]
+    with pytest.raises(ValueError):
+        partition_films([1, 2], 0)
