@@ -8,10 +8,14 @@ Bz ~ U[0.05, 0.8], a centred rectangular hole on the even films, psi = 1 +
 attempt for all films), dt = 0.2 with the integrator's per-film dt/10 retry,
 TGCR tolerance 1e-4, Newton's defaults, the Chebyshev-preconditioned mixed
 TGCR (FP32 JVP and Krylov vectors; FP64 Newton residual, Gram coefficients and
-state update), tiles picked by the autotuner.  A step (`--steps`) is one
-lockstep attempt over the batch; `value` is cell-steps/s: interior nodes of
-the films whose attempt was accepted, summed over the timed attempts, over
-the wall time, states resident in HBM.
+state update), tiles picked by the autotuner.  The films run as
+`--batch-groups` (default 4) concurrent sub-batches balanced by nodes, each
+its own batch on its own stream and host thread (trapezoidal_batch's
+groups=); every film's arithmetic, and so every film's state, is the same bit
+for bit as in one batch, whose figure is reported beside it.  A step
+(`--steps`) is one lockstep attempt of every group; `value` is cell-steps/s:
+interior nodes of the films whose attempt was accepted, summed over the timed
+attempts, over the wall time, states resident in HBM.
 
 One JSON line on stdout.  `roofline` is the dominant kernel class (the
 Chebyshev sweeps), measured with HIP events on each launch over a second,
@@ -53,11 +57,21 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (default
+# 4), sharing them round robin once they run out; two concurrent sub-batches on
+# one queue run one after the other.  The headline's groups, the one-batch run
+# and the default stream need 6, so the process asks for 8 before HIP starts.
+HW_QUEUES = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < HW_QUEUES:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 HBM_PEAK_GBS = 8000.0
 # Off with --no-profile: no second, event-instrumented pass in any section.
 PROFILE = True
 # Off with --no-batch-full-run: the batch's whole configured run (300 steps per film).
 FULL_RUN = True
+# --batch-groups: the headline's concurrent sub-batches (1 = the films as one batch).
+BATCH_GROUPS = 4
 METRIC = "cell-steps/s mixed vs FP64 at 1e-6 rel err; achieved HBM GB/s; adjoint s/step"
 
 
@@ -369,11 +383,53 @@ class BatchAttempts:
         return sum(self.attempt() for _ in range(k))
 
 
+class GroupAttempts:
+    """BatchAttempts over the films split into concurrent sub-batches
+    (BatchGroups: node-balanced groups, each its own tdgl_batch on its own
+    stream, stepped from its own thread).  An attempt is one attempt of every
+    group, so every film makes the same attempts, with the same bits, as in
+    one batch."""
+
+    def __init__(self, grp, films, opts, predictor="euler"):
+        import torch
+        self.grp = grp
+        self.parts = []
+        for k, r in enumerate(grp.runners):
+            with torch.cuda.stream(grp.streams[k]):
+                self.parts.append(BatchAttempts(r, [films[i] for i in grp.parts[k]], opts, predictor))
+        torch.cuda.synchronize()
+
+    @property
+    def newton(self) -> int:
+        return sum(b.newton for b in self.parts)
+
+    def snapshot(self):
+        return [b.snapshot() for b in self.parts]
+
+    def done(self) -> bool:
+        return all(b.done() for b in self.parts)
+
+    def run(self, k: int) -> float:
+        return sum(self.grp.run(lambda g, r: self.parts[g].run(k)))
+
+    def run_to_end(self):
+        """Every group until its films reach t_stop; returns (cell-steps, most attempts)."""
+        def go(g, r):
+            b, acc, n = self.parts[g], 0.0, 0
+            while not b.done():
+                acc += b.attempt()
+                n += 1
+            return acc, n
+        res = self.grp.run(go)
+        return sum(a for a, _ in res), max(n for _, n in res)
+
+
 def section_batched(attempts, warmup, fp64=True, variants=True):
-    """The headline: the 64-film batch in mixed precision."""
+    """The headline: the 64-film batch in mixed precision, as BATCH_GROUPS
+    concurrent sub-batches (the single batch's figure beside it)."""
     import torch
     from nanowire_tdgl_amd import _native, workloads
-    from nanowire_tdgl_amd.solvers.batch import BatchRunner
+    from nanowire_tdgl_amd.solvers.batch import BatchGroups, BatchRunner
     films = workloads.batch_films()
     ns = np.array([f.device.params.n_interior for f in films])
     runner = BatchRunner([f.device for f in films])
@@ -397,11 +453,30 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         b.newton_timed = (b.newton - n0) / (attempts * len(films))
         return snap, acc, el_
 
+
     mix = BatchAttempts(runner, films, opts)
-    snap, acc, el = timed(mix)
-    v = acc / el
-    log(f"[bench] batched mixed: {v:.4g} cell-steps/s over {len(films)} films, {attempts} attempts "
-        f"in {el:.3f} s")
+    snap, acc, el1 = timed(mix)
+    v1 = acc / el1
+    log(f"[bench] batched mixed, one batch: {v1:.4g} cell-steps/s over {len(films)} films, "
+        f"{attempts} attempts in {el1:.3f} s")
+    grp, groups_info = None, None
+    if BATCH_GROUPS > 1:
+        grp = BatchGroups([f.device for f in films], BATCH_GROUPS)
+        picks = [r.tune(reps=10) for r in grp.runners]  # each group tuned alone
+        gm = GroupAttempts(grp, films, opts)
+        _, accg, el = timed(gm)
+        v = accg / el
+        if accg != acc:
+            raise RuntimeError("the grouped attempts accepted other steps than the single batch")
+        groups_info = {"groups": len(grp), "films_per_group": [len(p) for p in grp.parts],
+                       "nodes_per_group": [int(ns[p].sum()) for p in grp.parts],
+                       "tile_per_group": [{0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[t["shape"]]
+                                          for t in picks],
+                       "one_batch_cell_steps_s": v1, "one_batch_seconds": el1}
+        log(f"[bench] batched mixed, {len(grp)} concurrent groups: {v:.4g} cell-steps/s "
+            f"({attempts} attempts in {el:.3f} s; {v / v1:.3f}x one batch)")
+    else:
+        v, el = v1, el1
     pc = roof = None
     if PROFILE:  # instrumented repeat of the warm-up and timed attempts from the same state
         final = mix.snapshot()
@@ -422,21 +497,27 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         out = {"workload": WORKLOAD_BATCH, "films": len(films), "n_interior_total": int(ns.sum()),
                "attempts": attempts, "warmup": warmup, "seconds": el, "value": v,
                "newton_iters_per_film_attempt": mix.newton_timed, "roofline": roof, "kernel_classes": pc,
-               "whole_attempt_algorithmic_GBps": whole, "autotune": tune_info}
+               "whole_attempt_algorithmic_GBps": whole, "concurrent_groups": groups_info,
+               "autotune": tune_info}
         return out
     full = None
     if FULL_RUN:  # the configured run: every film from t = 0 to its 300th step
-        fr = BatchAttempts(runner, films, opts)
         torch.cuda.synchronize()
         c0 = time.perf_counter()
-        accf, nat = 0.0, 0
-        while not fr.done():
-            accf += fr.attempt()
-            nat += 1
+        if grp is not None:
+            fr = GroupAttempts(grp, films, opts)
+            accf, nat = fr.run_to_end()
+        else:
+            fr = BatchAttempts(runner, films, opts)
+            accf, nat = 0.0, 0
+            while not fr.done():
+                accf += fr.attempt()
+                nat += 1
         torch.cuda.synchronize()
         elf = time.perf_counter() - c0
         full = {"steps_per_film": int(round(BATCH_T_STOP / BATCH_DT)), "attempts": nat, "seconds": elf,
-                "cell_steps_s": accf / elf, "newton_iters_per_film_step": fr.newton / (len(films) * BATCH_T_STOP / BATCH_DT)}
+                "groups": len(grp) if grp is not None else 1, "cell_steps_s": accf / elf,
+                "newton_iters_per_film_step": fr.newton / (len(films) * BATCH_T_STOP / BATCH_DT)}
         log(f"[bench] batched mixed, whole configured run: {full['cell_steps_s']:.4g} cell-steps/s "
             f"({nat} attempts in {elf:.2f} s)")
     # the same attempts without the preconditioner, and with the extrapolating predictor
@@ -445,7 +526,8 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
     _, accu, elu = timed(unp)
     vu = accu / elu
     log(f"[bench] batched mixed, no preconditioner: {vu:.4g} cell-steps/s")
-    ex = BatchAttempts(runner, films, opts, predictor="extrapolate")
+    ex = (GroupAttempts(grp, films, opts, predictor="extrapolate") if grp is not None else
+          BatchAttempts(runner, films, opts, predictor="extrapolate"))
     ex.run(1)  # every film gets a last step to extrapolate from
     _, acce, ele = timed(ex)
     ve = acce / ele
@@ -455,6 +537,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
            "warmup": warmup, "seconds": el, "value": v,
            "newton_iters_per_film_attempt": mix.newton_timed,
            "roofline": roof, "kernel_classes": pc, "whole_attempt_algorithmic_GBps": whole,
+           "concurrent_groups": groups_info,
            "full_run": full,
            "unpreconditioned_cell_steps_s": vu,
            "extrapolating_cell_steps_s": ve,
@@ -583,7 +666,7 @@ def section_adjoint(steps, every, fd=True, predictor=None, mask_cells=20):
                        f"(seed 4), psi = 1 + 0.01 noise (seed 4); {steps} FP64 steps dt=0.2, "
                        f"checkpoints every {every}; L = mean over k=6 of (B_z - T)^2, T = 0.2 "
                        "exp(-r^2/(2 15^2))",
-           "n_interior": n, "steps": steps, "loss": float(loss),
+           "n_interior": n, "steps": steps, "loss": float(loss.detach()),
            "forward_s_per_step": (c1 - c0) / steps, "adjoint_s_per_step": (c2 - c1) / steps,
            "grad": {"kappa": kappa.grad.item(), "Bz": bz.grad.item(),
                     "w_norm": float(w.grad.norm())},
@@ -694,6 +777,7 @@ def section_fp64_plain(dev, x0, steps, warmup):
 
 
 def main() -> None:
+    global PROFILE, FULL_RUN, BATCH_GROUPS
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel used to calibrate the PMC byte counters")
@@ -709,6 +793,8 @@ def main() -> None:
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-batch-full-run", action="store_true",
                     help="skip the batch's whole configured run (300 steps per film)")
+    ap.add_argument("--batch-groups", type=int, default=BATCH_GROUPS,
+                    help="concurrent sub-batches of the headline (1: the films as one batch)")
     ap.add_argument("--no-batch-fp64", action="store_true",
                     help="skip the FP64 loop baseline and the error report of the batch")
     ap.add_argument("--no-batch-variants", action="store_true",
@@ -729,9 +815,9 @@ def main() -> None:
     if args.whole_run:
         whole_run()
         return
-    global PROFILE, FULL_RUN
     PROFILE = not args.no_profile
     FULL_RUN = not args.no_batch_full_run
+    BATCH_GROUPS = max(1, args.batch_groups)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU split of "
               "this path is a design note, not a measured run", file=sys.stderr)

@@ -302,6 +302,11 @@ int tdgl_observables(tdgl_ctx* ctx, const double* x_dev, double* b_dev, double* 
 int tdgl_batch_create(tdgl_ctx* const* films, int nfilm, tdgl_batch** out);
 int tdgl_batch_destroy(tdgl_batch* batch);
 int tdgl_batch_set_stream(tdgl_batch* batch, void* hip_stream);
+/* The batch's own (non-blocking) stream, the one its kernels run on: work that
+ * a caller orders on it (as the caller's stream of tdgl_batch_set_stream)
+ * needs no event between the two.  No counterpart in the reference: it lets
+ * sub-batches stepped concurrently each keep to one stream. */
+int tdgl_batch_work_stream(const tdgl_batch* batch, void** hip_stream);
 const char* tdgl_batch_last_error(const tdgl_batch* batch);
 int64_t tdgl_batch_total_len(const tdgl_batch* batch);
 int64_t tdgl_batch_offset(const tdgl_batch* batch, int film);

@@ -1158,6 +1158,12 @@ int tdgl_batch_set_stream(tdgl_batch* B, void* s) {
   return TDGL_OK;
 }
 
+int tdgl_batch_work_stream(const tdgl_batch* B, void** s) {
+  if (!B || !s) return TDGL_E_INVALID;
+  *s = (void*)B->ps;
+  return TDGL_OK;
+}
+
 const char* tdgl_batch_last_error(const tdgl_batch* B) { return B ? B->err.c_str() : "null batch"; }
 
 int64_t tdgl_batch_total_len(const tdgl_batch* B) { return B ? B->total_len : -1; }
