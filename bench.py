@@ -57,14 +57,6 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (default
-# 4), sharing them round robin once they run out; two concurrent sub-batches on
-# one queue run one after the other.  The headline's groups, the one-batch run
-# and the default stream need 6, so the process asks for 8 before HIP starts.
-HW_QUEUES = 8
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < HW_QUEUES:
-    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
-
 HBM_PEAK_GBS = 8000.0
 # Off with --no-profile: no second, event-instrumented pass in any section.
 PROFILE = True
@@ -339,6 +331,7 @@ class BatchAttempts:
         self.extrapolate = predictor == "extrapolate"
         self.x_prev, self.dt_prev = None, np.zeros(self.F)
         self.newton = 0
+        self.work = np.zeros(self.F)  # per film: nodes x JVP and Chebyshev sweeps so far
 
     def snapshot(self):
         xp = None if self.x_prev is None else self.x_prev.clone()
@@ -365,6 +358,7 @@ class BatchAttempts:
                          dt_prev=self.dt_prev)
         ok = np.array([q.converged for q in st], dtype=bool) & active
         self.newton += sum(q.newton_iters for q, a in zip(st, active) if a)
+        self.work += self.ns * np.array([q.jvp_evals if a else 0 for q, a in zip(st, active)])
         if x_in is not None:
             if self.x_prev is None:
                 self.x_prev = x_in

@@ -307,6 +307,12 @@ int tdgl_batch_set_stream(tdgl_batch* batch, void* hip_stream);
  * needs no event between the two.  No counterpart in the reference: it lets
  * sub-batches stepped concurrently each keep to one stream. */
 int tdgl_batch_work_stream(const tdgl_batch* batch, void** hip_stream);
+/* Move the batch's work to a new stream that has a hardware queue of its own
+ * (made with a CU mask of every CU; HIP otherwise shares its
+ * GPU_MAX_HW_QUEUES queues among a process's streams, and batches stepped at
+ * once on one queue run one after the other).  The new stream is blocking
+ * (ordered after work on the legacy stream).  No counterpart in the reference. */
+int tdgl_batch_own_queue(tdgl_batch* batch);
 const char* tdgl_batch_last_error(const tdgl_batch* batch);
 int64_t tdgl_batch_total_len(const tdgl_batch* batch);
 int64_t tdgl_batch_offset(const tdgl_batch* batch, int film);

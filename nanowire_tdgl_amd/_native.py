@@ -32,7 +32,7 @@ EXPORTS = [
     "tdgl_jvp_trap_fd", "tdgl_jvp_trap32", "tdgl_precond_cheb32", "tdgl_vjp_f", "tdgl_vjp_bz", "tdgl_adjoint_solve", "tdgl_dot", "tdgl_norm_inf", "tdgl_trap_step", "tdgl_newton_trap", "tdgl_tgcr_trap", "tdgl_euler_step",
     "tdgl_euler_step_noalloc", "tdgl_noalloc_max_len",
     "tdgl_observables", "tdgl_profile_enable", "tdgl_profile_read",
-    "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_work_stream",
+    "tdgl_batch_create", "tdgl_batch_destroy", "tdgl_batch_set_stream", "tdgl_batch_work_stream", "tdgl_batch_own_queue",
     "tdgl_batch_last_error",
     "tdgl_batch_total_len", "tdgl_batch_offset", "tdgl_batch_set_kz", "tdgl_batch_get_kz",
     "tdgl_batch_set_sweep_tile", "tdgl_batch_get_sweep_tile", "tdgl_batch_set_sweep_split",
@@ -153,6 +153,7 @@ def load() -> C.CDLL:
         "tdgl_batch_destroy": ([P], C.c_int),
         "tdgl_batch_set_stream": ([P, P], C.c_int),
         "tdgl_batch_work_stream": ([P, P], C.c_int),
+        "tdgl_batch_own_queue": ([P], C.c_int),
         "tdgl_batch_last_error": ([P], C.c_char_p),
         "tdgl_batch_total_len": ([P], I64),
         "tdgl_batch_offset": ([P, C.c_int], I64),
@@ -546,6 +547,11 @@ class Batch:
         """Order the batch's steps after and before ``stream`` (a torch stream)."""
         self.stream = stream
         self._call("tdgl_batch_set_stream", C.c_void_p(stream.cuda_stream))
+
+    def own_queue(self) -> None:
+        """Move the batch's work to a stream with a hardware queue of its own
+        (tdgl_batch_own_queue); take :meth:`work_stream` again afterwards."""
+        self._call("tdgl_batch_own_queue")
 
     def work_stream(self):
         """The batch's own stream (its kernels run there), as a torch stream."""

@@ -1158,6 +1158,32 @@ int tdgl_batch_set_stream(tdgl_batch* B, void* s) {
   return TDGL_OK;
 }
 
+// HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues and
+// shares a queue between streams once they run out; batches stepped at once on
+// one queue run one after the other.  A stream made with a CU mask always gets
+// a queue of its own, so the batch's work moves to one with every CU enabled.
+// The new stream is a blocking one (it orders after the legacy stream's work).
+int tdgl_batch_own_queue(tdgl_batch* B) {
+  if (!B) return TDGL_E_INVALID;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  BCHK(hipGetDevice(&dev));
+  BCHK(hipGetDeviceProperties(&prop, dev));
+  const int ncu = prop.multiProcessorCount;
+  std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+  for (int i = 0; i < ncu; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
+  hipStream_t s = nullptr;
+  BCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+  BCHK(hipStreamSynchronize(B->ps));
+  if (B->body) {
+    (void)hipGraphExecDestroy(B->body);
+    B->body = nullptr;
+  }
+  BCHK(hipStreamDestroy(B->ps));
+  B->ps = s;
+  return TDGL_OK;
+}
+
 int tdgl_batch_work_stream(const tdgl_batch* B, void** s) {
   if (!B || !s) return TDGL_E_INVALID;
   *s = (void*)B->ps;

@@ -123,49 +123,57 @@ class BatchRunner:
         return dict(res, kz=kz, cached=False)
 
 
-def partition_films(sizes: Sequence[int], groups: int) -> list:
-    """Split films (their node counts ``sizes``) into ``groups`` groups of
-    near-equal total size: the largest film first, each film to the group with
-    the fewest nodes so far (ties to the lowest group).  Returns the groups as
-    ascending lists of film indices; no group is empty when groups <= films."""
+def partition_films(sizes: Sequence[float], groups: int) -> list:
+    """Split films (their weights ``sizes``: node counts, or measured work)
+    into ``groups`` groups of near-equal total weight: the heaviest film first,
+    each film to the group with the least weight so far (ties to the lowest
+    group).  Returns the groups as ascending lists of film indices; no group is
+    empty when groups <= films."""
     if groups < 1:
         raise ValueError("groups must be >= 1")
     groups = min(groups, len(sizes))
-    load = [0] * groups
+    load = [0.0] * groups
     parts: list = [[] for _ in range(groups)]
     for i in sorted(range(len(sizes)), key=lambda i: (-sizes[i], i)):
         k = min(range(groups), key=lambda k: (load[k], k))
         parts[k].append(i)
-        load[k] += int(sizes[i])
+        load[k] += float(sizes[i])
     return [sorted(p) for p in parts]
 
 
 class BatchGroups:
     """The films of a batch split into G sub-batches that run concurrently.
 
-    Each group (:func:`partition_films`, balanced by nodes) is a
-    :class:`BatchRunner` of its own on a torch stream of its own, stepped from
-    a host thread of its own (the library's calls release the GIL), so one
-    group's host work -- the Newton control flow, the converged-film
+    Each group (:func:`partition_films`: balanced by nodes, or by ``weights``
+    such as each film's measured work) is a :class:`BatchRunner` of its own,
+    stepped from a host thread of its own (the library's calls release the
+    GIL), and the group's host-side torch work runs on its batch's own stream.
+    One group's host work -- the Newton control flow, the converged-film
     bookkeeping, the readbacks of a lockstep attempt -- overlaps the other
     groups' kernels, and their kernels share the GPU.  A film's arithmetic does
     not depend on the other films of its batch, so every film's result is the
     same bit for bit whatever the grouping.
 
-    Each group's stream needs a hardware queue of its own: HIP maps a process's
-    streams onto GPU_MAX_HW_QUEUES queues (default 4, read when HIP starts)
-    and shares them once they run out, and two groups on one queue run one
-    after the other.  Set it to at least G + 2 before the process first uses
-    the GPU (a warning says so otherwise).  On the 64-film batch, 2 to 4
-    groups gain 11-17 % over one batch; 5 and more lose."""
+    Each group needs a hardware queue of its own: HIP maps a process's streams
+    onto GPU_MAX_HW_QUEUES queues (default 4) and shares them once they run
+    out, and two groups on one queue run one after the other.  With
+    ``own_queues`` (the default) every group's batch moves to a stream with a
+    queue of its own (tdgl_batch_own_queue).  Those streams are blocking: while
+    the groups run, no other thread may use the legacy (default) stream, or a
+    group's graph capture fails.  Without ``own_queues`` set GPU_MAX_HW_QUEUES
+    to at least G + 2 before the process first uses the GPU (a warning says so
+    otherwise).  On the 64-film batch, 2 to 4 groups gain 11-17 % over one
+    batch; 5 and more lose."""
 
-    def __init__(self, devices: Sequence, groups: int, work_streams: bool = True):
-        import torch
+    def __init__(self, devices: Sequence, groups: int, weights: Optional[Sequence[float]] = None,
+                 own_queues: bool = True):
         devices = list(devices)
-        sizes = [d.params.n_interior for d in devices]
+        sizes = [d.params.n_interior for d in devices] if weights is None else list(weights)
+        if len(sizes) != len(devices):
+            raise ValueError("one weight per film")
         self.parts = partition_films(sizes, groups)
         hwq = os.environ.get("GPU_MAX_HW_QUEUES", "")
-        if len(self.parts) > 1 and int(hwq if hwq.isdigit() else 4) < len(self.parts) + 2:
+        if not own_queues and len(self.parts) > 1 and int(hwq if hwq.isdigit() else 4) < len(self.parts) + 2:
             import warnings
             warnings.warn(f"{len(self.parts)} concurrent groups with GPU_MAX_HW_QUEUES={hwq or 4}: groups "
                           "may share a hardware queue and run one after the other; set it to at least "
@@ -173,13 +181,11 @@ class BatchGroups:
                           stacklevel=2)
         self.runners, self.streams = [], []
         for part in self.parts:
-            if work_streams:  # the group's host-side work joins its batch's own stream
-                r = BatchRunner([devices[i] for i in part])
-                st = r.batch.work_stream()
-                r.batch.set_stream(st)
-            else:
-                st = torch.cuda.Stream()
-                r = BatchRunner([devices[i] for i in part], stream=st)
+            r = BatchRunner([devices[i] for i in part])
+            if own_queues and len(self.parts) > 1:
+                r.batch.own_queue()
+            st = r.batch.work_stream()
+            r.batch.set_stream(st)
             self.runners.append(r)
             self.streams.append(st)
 
