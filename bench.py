@@ -440,10 +440,12 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         torch.cuda.synchronize()
         snap = b.snapshot()
         n0 = b.newton
+        w0 = time.monotonic_ns()  # the window on the clock kernel traces use
         c0 = time.perf_counter()
         acc = b.run(attempts)
         torch.cuda.synchronize()
         el_ = time.perf_counter() - c0
+        b.window_ns = [w0, time.monotonic_ns()]
         b.newton_timed = (b.newton - n0) / (attempts * len(films))
         return snap, acc, el_
 
@@ -466,7 +468,9 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
                        "nodes_per_group": [int(ns[p].sum()) for p in grp.parts],
                        "tile_per_group": [{0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[t["shape"]]
                                           for t in picks],
-                       "one_batch_cell_steps_s": v1, "one_batch_seconds": el1}
+                       "one_batch_cell_steps_s": v1, "one_batch_seconds": el1,
+                       "timed_window_monotonic_ns": gm.window_ns,
+                       "one_batch_timed_window_monotonic_ns": mix.window_ns}
         log(f"[bench] batched mixed, {len(grp)} concurrent groups: {v:.4g} cell-steps/s "
             f"({attempts} attempts in {el:.3f} s; {v / v1:.3f}x one batch)")
     else:
