@@ -359,14 +359,53 @@ void part_build(const tdgl_batch* B, const std::vector<char>& in, long long gres
     }
   }
   nblocks = (int)(seg.size() - seg0);
+  const int nx = 8;
+  if (kXcdBalance && B->is3d && nblocks >= 8 * nx) {
+    // Equal block counts per XCD give unequal work (films differ in piece
+    // depth and tile width), and the sweep lasts as long as its busiest XCD.
+    // So the list is cut into 8 contiguous runs of near-equal estimated cost,
+    // a block's set-up counted as ~2.3 planes and a plane's cost as its tile's
+    // lanes per node (23 + 64/W, the per-film tile model); each run is padded
+    // to the longest with empty pieces (film -1: the block returns at once).
+    std::vector<double> cost((size_t)nblocks);
+    double total = 0.0;
+    for (int q = 0; q < nblocks; ++q) {
+      const int4 v = seg[seg0 + q];
+      const int tw = B->shape == kSweepShapeFilm ? B->hf[v.x].tw32 : sweep_tx(B->shape);
+      cost[q] = ((v.w - v.z) + 2.3) * (23.0 + 64.0 / tw);
+      total += cost[q];
+    }
+    std::vector<int> cut(nx + 1, nblocks);
+    cut[0] = 0;
+    double acc = 0.0;
+    for (int q = 0, x = 1; q < nblocks && x < nx; ++q) {
+      acc += cost[q];
+      if (acc >= total * x / nx) cut[x++] = q + 1;
+    }
+    int per = 0;
+    for (int x = 0; x < nx; ++x) per = std::max(per, cut[x + 1] - cut[x]);
+    std::vector<int4> runs;
+    runs.reserve((size_t)per * nx);
+    for (int x = 0; x < nx; ++x) {
+      for (int q = cut[x]; q < cut[x + 1]; ++q) runs.push_back(seg[seg0 + q]);
+      for (int q = cut[x + 1] - cut[x]; q < per; ++q) runs.push_back(make_int4(-1, 0, 0, 0));
+    }
+    seg.resize(seg0);
+    seg.insert(seg.end(), runs.begin(), runs.end());
+    nblocks = per * nx;
+  }
   // the runs of xcd_remap (tdgl_device.hpp): XCD x holds per or per - 1 blocks
-  const int G = nblocks, nx = 8, per = (G + nx - 1) / nx, full = G % nx == 0 ? nx : G % nx;
+  const int G = nblocks, per = (G + nx - 1) / nx, full = G % nx == 0 ? nx : G % nx;
   for (int x = 0, base = 0; x < nx; ++x) {
     const int len = x < full ? per : per - 1;
     // by depth class (pieces within a plane or so of each other keep their
-    // order: a column's z pieces differ by one plane and share halo planes)
+    // order: a column's z pieces differ by one plane and share halo planes);
+    // empty pieces last
     std::stable_sort(seg.begin() + seg0 + base, seg.begin() + seg0 + base + len,
-                     [](const int4& a, const int4& b) { return (a.w - a.z + 1) / 2 > (b.w - b.z + 1) / 2; });
+                     [](const int4& a, const int4& b) {
+                       const int ca = a.x < 0 ? -1 : (a.w - a.z + 1) / 2, cb = b.x < 0 ? -1 : (b.w - b.z + 1) / 2;
+                       return ca > cb;
+                     });
     base += len;
   }
 }
@@ -392,6 +431,7 @@ int ensure_parts(tdgl_batch* B, const std::vector<int>& act) {
     bool ok = (int)seg.size() - g0[m] == nb[m];
     for (int q = g0[m]; ok && q < (int)seg.size(); ++q) {
       const int4 v = seg[q];
+      if (v.x == -1 && v.y == 0 && v.z == 0 && v.w == 0) continue;  // an empty piece
       ok = v.x >= 0 && v.x < F && in[v.x];
       if (!ok) break;
       const FilmDesc& d = B->hf[v.x];

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   const unsigned int L = xcd_remap(blockIdx.x, gridDim.x);
   const int4 seg = part.seg[L];
   const int f = seg.x;
-  if (!bk.act[f]) return;
+  if (f < 0 || !bk.act[f]) return;  // an empty piece (XCD run padding) or a film at rest
   const FilmDesc* fd = bk.films + f;
   const Geo g = fd->g;  // by value: held in registers for the whole piece
   const Bnd b = fd->b;

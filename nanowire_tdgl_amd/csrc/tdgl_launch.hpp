@@ -61,6 +61,9 @@ constexpr int kMaxKrylov = 2048;
 // FP32 sweep tile shapes: 0 = 64 x 4, 1 = 32 x 8, 2 = 16 x 16 nodes per plane;
 // kSweepShapeFilm = each film a tile of its own (sweep_film_tile)
 constexpr int kSweepShapeFilm = 3;
+// The sweep piece lists give each XCD a run of near-equal estimated cost
+// (padded with empty pieces) rather than an equal count of pieces.
+constexpr bool kXcdBalance = true;
 constexpr int kSweepShapes = 4;
 constexpr int kDefaultSweepShape = 0;
 __host__ __device__ constexpr int sweep_tx(int s) { return s == 1 ? 32 : (s == 2 ? 16 : 64); }

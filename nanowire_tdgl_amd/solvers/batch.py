@@ -73,7 +73,8 @@ class BatchRunner:
         self.batch.set_tile(int(shape))
         self.batch.set_split(int(split[0]), int(split[1]))
 
-    def tune(self, X=None, splits=((8, 2), (8, 4), (6, 2), (12, 2), (16, 4)), shapes=(0, 1, 2, 3),
+    def tune(self, X=None, splits=((8, 2), (8, 4), (6, 2), (10, 4), (12, 2), (12, 4), (16, 4)),
+             shapes=(0, 1, 2, 3),
              reps: int = 5, cache: Optional[str] = "default") -> dict:
         """Autotuner: pick the mixed-precision sweeps' tile shape (0 = 64x4,
         1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film a tile of its own
