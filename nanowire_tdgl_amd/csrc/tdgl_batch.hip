@@ -239,8 +239,11 @@ int build_layout(tdgl_batch* B) {
   B->nitems = (int)B->item_film.size();
   B->nsblk = (int)B->sblk_film.size();
   B->part_key.clear();  // the sweep tiles may have changed
+  // build_layout runs at creation and from the configuration calls (tile
+  // shape, z depth), which wait for the batch's stream first: never inside a
+  // step, so the old tables are freed at once
   for (void* p : {(void*)B->d_item_film, (void*)B->d_sblk_film, (void*)B->d_pitem, (void*)B->d_pm})
-    retire(B, p);
+    if (p) BCHK(hipFree(p));
   B->d_item_film = B->d_sblk_film = nullptr;
   B->d_pitem = B->d_pm = nullptr;
   int rc;
