@@ -164,3 +164,13 @@ def test_partition_films_covers_every_film_once_and_balances():
     assert partition_films([5, 9, 1, 3, 3, 7], 3) == [[1], [4, 5], [0, 2, 3]]
     with pytest.raises(ValueError):
         partition_films([1, 2], 0)
+
+
+def test_trapezoidal_batch_argument_errors():
+    """Host-side checks of trapezoidal_batch, raised before any device work:
+    an unknown predictor, and a plain runner handed to a grouped run."""
+    from nanowire_tdgl_amd.solvers.batch import trapezoidal_batch
+    with pytest.raises(ValueError, match="predictor"):
+        trapezoidal_batch([], 0.0, 1.0, 0.1, predictor="bdf2")
+    with pytest.raises(ValueError, match="BatchGroups"):
+        trapezoidal_batch([], 0.0, 1.0, 0.1, groups=2, runner=object())
