@@ -69,6 +69,12 @@ __device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70);
 // batch has general applied-field vectors; its cells are staged by the
 // per-cell rules (stage32), every other film's by the mask plans (stage32u),
 // which give the same bits.
+// compile-time tag: whether a wave's lanes stage a second cell
+template <bool B>
+struct SecondCell {
+  static constexpr bool value = B;
+};
+
 template <bool IS3D, bool CHEB, int TS, bool GEN>
 __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, const float* x32,
                                                const float* p32, float* const* Q, const int* iter,
@@ -177,7 +183,8 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   // live = false: the march's last step, which has no plane to fetch; it
   // issues the same loads (so that the waits count them exactly) from node 0
   // of plane kp (one line each), and the values are never committed
-  auto load_into = [&](int kp, Raw32& a0, Raw32& a1, bool live = true) __attribute__((always_inline)) {
+  auto load_into = [&](auto two, int kp, Raw32& a0, Raw32& a1, bool live = true) __attribute__((always_inline)) {
+    constexpr bool H1 = decltype(two)::value;
     if (GEN && F.gen) {  // general field: the rules at once, the cells kept in the load slots
       auto pack = [](Raw32& r, const Cell<float>& v, const Cell<float>& w) {
         r.xa = make_float2(v.pr, v.pi);
@@ -198,10 +205,12 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
       stage32<IS3D>(g, F, mt.flags, X, P, c0, kp, v, w);
       pack(a0, v, w);
       a0.h = hf ? (unsigned int)ldu(mt.flags, c0.mij + moff) : 0u;
-      const Col32 c1 = col32(g, i0 - 1 + h1 % CXr, j0 - 1 + h1 / CXr);
-      stage32<IS3D>(g, F, mt.flags, X, P, c1, kp, v, w);
-      pack(a1, v, w);
-      a1.h = hf ? (unsigned int)ldu(mt.flags, c1.mij + moff) : 0u;
+      if constexpr (H1) {
+        const Col32 c1 = col32(g, i0 - 1 + h1 % CXr, j0 - 1 + h1 / CXr);
+        stage32<IS3D>(g, F, mt.flags, X, P, c1, kp, v, w);
+        pack(a1, v, w);
+        a1.h = hf ? (unsigned int)ldu(mt.flags, c1.mij + moff) : 0u;
+      }
       return;
     }
     // every lane issues both cells' loads: a column past the tile's cells
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
       c0.mij = c1.mij = 0u;
     }
     load32u<IS3D>(mt.flags, X, P, c0, pp, a0);
-    load32u<IS3D>(mt.flags, X, P, c1, pp, a1);
+    if constexpr (H1) load32u<IS3D>(mt.flags, X, P, c1, pp, a1);
   };
   auto put = [&](int slot, int t, int c, const Cell<float>& v, const Cell<float>& w, unsigned int h) {
     H[t][c] = (uint8_t)h;
@@ -234,8 +243,9 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
   };
   // plane kp's cells, loaded by load_into, to ring slot `slot` (and slot t of
   // the arrays read on one plane only)
-  auto commit_from = [&](int kp, int slot, int t, const Raw32& a0, const Raw32& a1, bool drain = true)
+  auto commit_from = [&](auto two, int kp, int slot, int t, const Raw32& a0, const Raw32& a1, bool drain = true)
                          __attribute__((always_inline)) {
+    constexpr bool H1 = decltype(two)::value;
     // drain: every load of the plane has landed here (all are used below);
     // waiting for all at once tells the compiler's wait tracking that no
     // load is left in flight, so no register reuse later in the plane waits
@@ -249,212 +259,222 @@ __global__ __launch_bounds__(BS, 3) void k_bjvp32(BatchK bk, SweepPart part, con
         w = Cell<float>{r.pa.x, r.pa.y, r.px, r.py, r.pz};
       };
       unpack(a0, v0c, w0c);
-      unpack(a1, v1c, w1c);
+      if constexpr (H1) unpack(a1, v1c, w1c);
     } else {
       const Pl32 pp = plane32<IS3D>(g, F, kp);
       if (IS3D && pp.fk) {
         finish32u<IS3D, true>(a0, hf, pl0, pp, v0c, w0c);
-        finish32u<IS3D, true>(a1, hf, pl1, pp, v1c, w1c);
+        if constexpr (H1) finish32u<IS3D, true>(a1, hf, pl1, pp, v1c, w1c);
       } else {
         finish32u<IS3D, false>(a0, hf, pl0, pp, v0c, w0c);
-        finish32u<IS3D, false>(a1, hf, pl1, pp, v1c, w1c);
+        if constexpr (H1) finish32u<IS3D, false>(a1, hf, pl1, pp, v1c, w1c);
       }
     }
     put(slot, t, hc0, v0c, w0c, a0.h);
-    if (has1) put(slot, t, hc1, v1c, w1c, a1.h);
+    if constexpr (H1) {
+      if (has1) put(slot, t, hc1, v1c, w1c, a1.h);
+    }
   };
 
   // Ring slots: plane kp sits in slot (kp - kb + 1) % RING ((kp - kb + 1) % NS
   // for the one-plane arrays).  With RING == NS (the 64x4 tile) the march is
   // unrolled by RING, so that every copy of the step knows its slots at
   // compile time.
-  Raw32 r0, r1;
-  if (IS3D) {
-    // the first three planes' loads go out together (one memory round trip
-    // before the first step: the third plane's loads stay in flight while
-    // the first two are committed)
-    Raw32 a0, a1, b0, b1;
-    load_into(kb - 1, a0, a1);
-    load_into(kb, b0, b1);
-    load_into(kb + 1, r0, r1);
-    commit_from(kb - 1, 0, 0, a0, a1, false);
-    commit_from(kb, 1, 1, b0, b1, false);
-  } else {
-    load_into(0, r0, r1);
-  }
-
-  const float ihx2 = (float)mt.ihx2, ihy2 = (float)mt.ihy2, ihz2 = (float)mt.ihz2;
-  const float dgpsi = (float)mt.dgpsi, inv_tau = (float)mt.inv_tau;
-
-  // one step of the march: plane k from slots s0 (plane k), sm (k - 1) and sp1
-  // (k + 1, committed first); t0 / t1: planes k / k + 1 in the one-plane arrays
-  auto step_k = [&](int k, int s0, int sm, int sp1, int t0, int t1) __attribute__((always_inline)) {
-    commit_from(IS3D ? k + 1 : 0, sp1, t1, r0, r1);
-    __syncthreads();
-    // own-node inputs (a lane past the grid's edge reads the nearest node and
-    // stores nothing), then the halo two planes ahead
-    const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
-    const int c0 = (ty + 1) * CXr + (tx + 1);
-    const uint8_t fl = H[t0][c0];
-    float scv = 1.f, k2v = 0.f;
-    if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
-    if (mt.k2) k2v = (float)ldu(mt.k2, q);
-    // p at the own node is its staged cell (read below): the rules keep psi and
-    // zero a link only on the last interior column / row / plane (the normal
-    // zeroing) or on a hole link; there the raw value is loaded
-    const bool rx = i == g.Nx - 1 || (fl & kHoleX), ry = j == g.Ny - 1 || (fl & kHoleY);
-    const bool rz = IS3D && (k == g.Nz - 1 || (fl & kHoleZ));
-    float lx = 0.f, ly = 0.f, lz = 0.f;
-    if (rx) lx = P.ph(2, q);
-    if (ry) ly = P.ph(3, q);
-    if (rz) lz = P.ph(4, q);
-    float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
-    float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
-    if (has_v) {
-      v0 = CV.psi(q);
-      const float3 l = CV.links<IS3D>(q);
-      v_x = l.x;
-      v_y = l.y;
-      v_z = l.z;
+  auto march = [&](auto two) __attribute__((always_inline)) {
+    Raw32 r0, r1;
+    if (IS3D) {
+      // the first three planes' loads go out together (one memory round trip
+      // before the first step: the third plane's loads stay in flight while
+      // the first two are committed)
+      Raw32 a0, a1, b0, b1;
+      load_into(two, kb - 1, a0, a1);
+      load_into(two, kb, b0, b1);
+      load_into(two, kb + 1, r0, r1);
+      commit_from(two, kb - 1, 0, 0, a0, a1, false);
+      commit_from(two, kb, 1, 1, b0, b1, false);
+    } else {
+      load_into(two, 0, r0, r1);
     }
-    if (has_p) {
-      w0 = CP.psi(q);
-      const float3 l = CP.links<IS3D>(q);
-      w_x = l.x;
-      w_y = l.y;
-      w_z = l.z;
-    }
-    // the halo two planes ahead (on the last step, the one-line stand-in)
-    if (IS3D) load_into(min(k + 2, ke), r0, r1, k + 2 <= ke);
 
-    if (act) {
-      const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CXr, cyp = c0 + CXr;
-      const float pr = X_r[s0][c0], pi = X_i[s0][c0];
-      const float dr = P_r[s0][c0], di = P_i[s0][c0];
-      const float ex = X_x[t0][c0], ey = X_y[t0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
-      const float dex = P_x[s0][c0], dey = P_y[s0][c0], dez = IS3D ? P_z[s0][c0] : 0.f;
-      const float2 p0 = make_float2(dr, di);
-      const float praw_x = rx ? lx : dex, praw_y = ry ? ly : dey, praw_z = rz ? lz : dez;
+    const float ihx2 = (float)mt.ihx2, ihy2 = (float)mt.ihy2, ihz2 = (float)mt.ihz2;
+    const float dgpsi = (float)mt.dgpsi, inv_tau = (float)mt.inv_tau;
 
-      float sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0.f, czp = 1.f, szm = 0.f, czm = 1.f;
-      fast_sincos(ex, sxp, cxp_);
-      fast_sincos(X_x[t0][cxm], sxm, cxm_);
-      fast_sincos(ey, syp, cyp_);
-      fast_sincos(X_y[t0][cym], sym, cym_);
-      if (IS3D) {
-        fast_sincos(ez, szp, czp);
-        fast_sincos(X_z[sm][c0], szm, czm);
+    // one step of the march: plane k from slots s0 (plane k), sm (k - 1) and sp1
+    // (k + 1, committed first); t0 / t1: planes k / k + 1 in the one-plane arrays
+    auto step_k = [&](int k, int s0, int sm, int sp1, int t0, int t1) __attribute__((always_inline)) {
+      commit_from(two, IS3D ? k + 1 : 0, sp1, t1, r0, r1);
+      __syncthreads();
+      // own-node inputs (a lane past the grid's edge reads the nearest node and
+      // stores nothing), then the halo two planes ahead
+      const unsigned int q = qix32(g, min(i, g.Nx - 1), min(j, g.Ny - 1), k, IS3D);
+      const int c0 = (ty + 1) * CXr + (tx + 1);
+      const uint8_t fl = H[t0][c0];
+      float scv = 1.f, k2v = 0.f;
+      if (mt.has_mat && !mt.sc_bits) scv = (float)ldu(mt.sc, q);
+      if (mt.k2) k2v = (float)ldu(mt.k2, q);
+      // p at the own node is its staged cell (read below): the rules keep psi and
+      // zero a link only on the last interior column / row / plane (the normal
+      // zeroing) or on a hole link; there the raw value is loaded
+      const bool rx = i == g.Nx - 1 || (fl & kHoleX), ry = j == g.Ny - 1 || (fl & kHoleY);
+      const bool rz = IS3D && (k == g.Nz - 1 || (fl & kHoleZ));
+      float lx = 0.f, ly = 0.f, lz = 0.f;
+      if (rx) lx = P.ph(2, q);
+      if (ry) ly = P.ph(3, q);
+      if (rz) lz = P.ph(4, q);
+      float2 v0 = make_float2(0.f, 0.f), w0 = make_float2(0.f, 0.f);
+      float v_x = 0.f, v_y = 0.f, v_z = 0.f, w_x = 0.f, w_y = 0.f, w_z = 0.f;
+      if (has_v) {
+        v0 = CV.psi(q);
+        const float3 l = CV.links<IS3D>(q);
+        v_x = l.x;
+        v_y = l.y;
+        v_z = l.z;
       }
+      if (has_p) {
+        w0 = CP.psi(q);
+        const float3 l = CP.links<IS3D>(q);
+        w_x = l.x;
+        w_y = l.y;
+        w_z = l.z;
+      }
+      // the halo two planes ahead (on the last step, the one-line stand-in)
+      if (IS3D) load_into(two, min(k + 2, ke), r0, r1, k + 2 <= ke);
 
-      // d[e^{-i t} psi(-a)] = e^{-i t}(d psi(-a) - i dt psi(-a)),
-      // d[e^{i t} psi(+a)]  = e^{i t}(d psi(+a) + i dt psi(+a))
-      float ar = dgpsi * dr, ai = dgpsi * di;
-      auto minus = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi, float h) {
-        const float wr = dqr + dt_ * qi, wi = dqi - dt_ * qr;
-        ar += (c * wr + s * wi) * h;
-        ai += (c * wi - s * wr) * h;
-      };
-      auto plus = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi, float h) {
-        const float wr = dqr - dt_ * qi, wi = dqi + dt_ * qr;
-        ar += (c * wr - s * wi) * h;
-        ai += (c * wi + s * wr) * h;
-      };
-      minus(cxm_, sxm, P_x[s0][cxm], X_r[s0][cxm], X_i[s0][cxm], P_r[s0][cxm], P_i[s0][cxm], ihx2);
-      plus(cxp_, sxp, dex, X_r[s0][cxp], X_i[s0][cxp], P_r[s0][cxp], P_i[s0][cxp], ihx2);
-      minus(cym_, sym, P_y[s0][cym], X_r[s0][cym], X_i[s0][cym], P_r[s0][cym], P_i[s0][cym], ihy2);
-      plus(cyp_, syp, dey, X_r[s0][cyp], X_i[s0][cyp], P_r[s0][cyp], P_i[s0][cyp], ihy2);
-      if (IS3D) {
-        minus(czm, szm, P_z[sm][c0], X_r[sm][c0], X_i[sm][c0], P_r[sm][c0], P_i[sm][c0], ihz2);
-        plus(czp, szp, dez, X_r[sp1][c0], X_i[sp1][c0], P_r[sp1][c0], P_i[sp1][c0], ihz2);
-      }
-      // d[(1-|psi|^2) psi] = (1 - 2|psi|^2) dpsi - psi^2 conj(dpsi)
-      const float m2 = pr * pr + pi * pi, a2 = pr * pr - pi * pi, b2 = 2.f * pr * pi;
-      float fr = (1.f - 2.f * m2) * dr - (a2 * dr + b2 * di);
-      float fi = (1.f - 2.f * m2) * di - (b2 * dr - a2 * di);
-      if (mt.has_mat) {
-        const float sc = mt.sc_bits ? ((fl & kInsulator) ? 0.f : 1.f) : scv;
-        fr = sc * fr - (1.f - sc) * dr * inv_tau;
-        fi = sc * fi - (1.f - sc) * di * inv_tau;
-      }
-      const float jr = ar + fr, ji = ai + fi;
+      if (act) {
+        const int cxm = c0 - 1, cxp = c0 + 1, cym = c0 - CXr, cyp = c0 + CXr;
+        const float pr = X_r[s0][c0], pi = X_i[s0][c0];
+        const float dr = P_r[s0][c0], di = P_i[s0][c0];
+        const float ex = X_x[t0][c0], ey = X_y[t0][c0], ez = IS3D ? X_z[s0][c0] : 0.f;
+        const float dex = P_x[s0][c0], dey = P_y[s0][c0], dez = IS3D ? P_z[s0][c0] : 0.f;
+        const float2 p0 = make_float2(dr, di);
+        const float praw_x = rx ? lx : dex, praw_y = ry ? ly : dey, praw_z = rz ? lz : dez;
 
-      float cx, cy, cz;
-      if (mt.k2) {
-        cx = k2v * (float)mt.ihx2;
-        cy = k2v * (float)mt.ihy2;
-        cz = IS3D ? k2v * (float)mt.ihz2 : 0.f;
-      } else {
-        cx = (float)mt.cx;
-        cy = (float)mt.cy;
-        cz = (float)mt.cz;
-      }
-      // d Im[e^{-i t} conj(psi) psi1] = Im[e^{-i t} U] - dt Re[e^{-i t} T],
-      // T = conj(psi) psi1, U = conj(dpsi) psi1 + conj(psi) dpsi1
-      auto dsup = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi) {
-        const float Tr = pr * qr + pi * qi, Ti = pr * qi - pi * qr;
-        const float Ur = (dr * qr + di * qi) + (pr * dqr + pi * dqi);
-        const float Ui = (dr * qi - di * qr) + (pr * dqi - pi * dqr);
-        return (c * Ui - s * Ur) - dt_ * (c * Tr + s * Ti);
-      };
-      // link operators on the perturbation: 2 D_aa along the link, D_bb across
-      float jx = 2.f * cx * (P_x[s0][cxm] - 2.f * dex + P_x[s0][cxp]) +
-                 cy * (P_x[s0][cym] - 2.f * dex + P_x[s0][cyp]);
-      float jy = 2.f * cy * (P_y[s0][cym] - 2.f * dey + P_y[s0][cyp]) +
-                 cx * (P_y[s0][cxm] - 2.f * dey + P_y[s0][cxp]);
-      jx += cy * (((-P_y[s0][cxp] + dey) + P_y[s0][cxp - CXr]) - P_y[s0][cym]);
-      jy += cx * (((-P_x[s0][cyp] + dex) + P_x[s0][cyp - 1]) - P_x[s0][cxm]);
-      float jz = 0.f;
-      if (IS3D) {
-        jx += cz * (P_x[sm][c0] - 2.f * dex + P_x[sp1][c0]);
-        jy += cz * (P_y[sm][c0] - 2.f * dey + P_y[sp1][c0]);
-        jx += cz * (((-P_z[s0][cxp] + dez) + P_z[sm][cxp]) - P_z[sm][c0]);
-        jy += cz * (((-P_z[s0][cyp] + dez) + P_z[sm][cyp]) - P_z[sm][c0]);
-        jz = 2.f * cz * (P_z[sm][c0] - 2.f * dez + P_z[sp1][c0]) +
-             cx * (P_z[s0][cxm] - 2.f * dez + P_z[s0][cxp]) +
-             cy * (P_z[s0][cym] - 2.f * dez + P_z[s0][cyp]);
-        jz += cx * (((-P_x[sp1][c0] + dex) + P_x[sp1][cxm]) - P_x[s0][cxm]);
-        jz += cy * (((-P_y[sp1][c0] + dey) + P_y[sp1][cym]) - P_y[s0][cym]);
-        jz += dsup(czp, szp, dez, X_r[sp1][c0], X_i[sp1][c0], P_r[sp1][c0], P_i[sp1][c0]);
-      }
-      jx += dsup(cxp_, sxp, dex, X_r[s0][cxp], X_i[s0][cxp], P_r[s0][cxp], P_i[s0][cxp]);
-      jy += dsup(cyp_, syp, dey, X_r[s0][cyp], X_i[s0][cyp], P_r[s0][cyp], P_i[s0][cyp]);
-      if (fl & kHoleX) jx = 0.f;
-      if (fl & kHoleY) jy = 0.f;
-      if (IS3D && (fl & kHoleZ)) jz = 0.f;
+        float sxp, cxp_, sxm, cxm_, syp, cyp_, sym, cym_, szp = 0.f, czp = 1.f, szm = 0.f, czm = 1.f;
+        fast_sincos(ex, sxp, cxp_);
+        fast_sincos(X_x[t0][cxm], sxm, cxm_);
+        fast_sincos(ey, syp, cyp_);
+        fast_sincos(X_y[t0][cym], sym, cym_);
+        if (IS3D) {
+          fast_sincos(ez, szp, czp);
+          fast_sincos(X_z[sm][c0], szm, czm);
+        }
 
-      if (CHEB) {
-        const float cvf = cs.cv ? 1.f : 0.f;
-        auto step = [&](float pv, float jv, float vv, float wv) {
-          const float y = c_ss * pv;
-          const float ay = c_ss * (pv - hdt * jv);
-          const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
-          return fin ? vv - res : res;
+        // d[e^{-i t} psi(-a)] = e^{-i t}(d psi(-a) - i dt psi(-a)),
+        // d[e^{i t} psi(+a)]  = e^{i t}(d psi(+a) + i dt psi(+a))
+        float ar = dgpsi * dr, ai = dgpsi * di;
+        auto minus = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi, float h) {
+          const float wr = dqr + dt_ * qi, wi = dqi - dt_ * qr;
+          ar += (c * wr + s * wi) * h;
+          ai += (c * wi - s * wr) * h;
         };
-        out.psi(q, make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
-        out.links<IS3D>(q, step(praw_x, jx, v_x, w_x), step(praw_y, jy, v_y, w_y),
-                        IS3D ? step(praw_z, jz, v_z, w_z) : 0.f);
-      } else {
-        out.psi(q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
-        out.links<IS3D>(q, praw_x - hdt * jx, praw_y - hdt * jy, IS3D ? praw_z - hdt * jz : 0.f);
+        auto plus = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi, float h) {
+          const float wr = dqr - dt_ * qi, wi = dqi + dt_ * qr;
+          ar += (c * wr - s * wi) * h;
+          ai += (c * wi + s * wr) * h;
+        };
+        minus(cxm_, sxm, P_x[s0][cxm], X_r[s0][cxm], X_i[s0][cxm], P_r[s0][cxm], P_i[s0][cxm], ihx2);
+        plus(cxp_, sxp, dex, X_r[s0][cxp], X_i[s0][cxp], P_r[s0][cxp], P_i[s0][cxp], ihx2);
+        minus(cym_, sym, P_y[s0][cym], X_r[s0][cym], X_i[s0][cym], P_r[s0][cym], P_i[s0][cym], ihy2);
+        plus(cyp_, syp, dey, X_r[s0][cyp], X_i[s0][cyp], P_r[s0][cyp], P_i[s0][cyp], ihy2);
+        if (IS3D) {
+          minus(czm, szm, P_z[sm][c0], X_r[sm][c0], X_i[sm][c0], P_r[sm][c0], P_i[sm][c0], ihz2);
+          plus(czp, szp, dez, X_r[sp1][c0], X_i[sp1][c0], P_r[sp1][c0], P_i[sp1][c0], ihz2);
+        }
+        // d[(1-|psi|^2) psi] = (1 - 2|psi|^2) dpsi - psi^2 conj(dpsi)
+        const float m2 = pr * pr + pi * pi, a2 = pr * pr - pi * pi, b2 = 2.f * pr * pi;
+        float fr = (1.f - 2.f * m2) * dr - (a2 * dr + b2 * di);
+        float fi = (1.f - 2.f * m2) * di - (b2 * dr - a2 * di);
+        if (mt.has_mat) {
+          const float sc = mt.sc_bits ? ((fl & kInsulator) ? 0.f : 1.f) : scv;
+          fr = sc * fr - (1.f - sc) * dr * inv_tau;
+          fi = sc * fi - (1.f - sc) * di * inv_tau;
+        }
+        const float jr = ar + fr, ji = ai + fi;
+
+        float cx, cy, cz;
+        if (mt.k2) {
+          cx = k2v * (float)mt.ihx2;
+          cy = k2v * (float)mt.ihy2;
+          cz = IS3D ? k2v * (float)mt.ihz2 : 0.f;
+        } else {
+          cx = (float)mt.cx;
+          cy = (float)mt.cy;
+          cz = (float)mt.cz;
+        }
+        // d Im[e^{-i t} conj(psi) psi1] = Im[e^{-i t} U] - dt Re[e^{-i t} T],
+        // T = conj(psi) psi1, U = conj(dpsi) psi1 + conj(psi) dpsi1
+        auto dsup = [&](float c, float s, float dt_, float qr, float qi, float dqr, float dqi) {
+          const float Tr = pr * qr + pi * qi, Ti = pr * qi - pi * qr;
+          const float Ur = (dr * qr + di * qi) + (pr * dqr + pi * dqi);
+          const float Ui = (dr * qi - di * qr) + (pr * dqi - pi * dqr);
+          return (c * Ui - s * Ur) - dt_ * (c * Tr + s * Ti);
+        };
+        // link operators on the perturbation: 2 D_aa along the link, D_bb across
+        float jx = 2.f * cx * (P_x[s0][cxm] - 2.f * dex + P_x[s0][cxp]) +
+                   cy * (P_x[s0][cym] - 2.f * dex + P_x[s0][cyp]);
+        float jy = 2.f * cy * (P_y[s0][cym] - 2.f * dey + P_y[s0][cyp]) +
+                   cx * (P_y[s0][cxm] - 2.f * dey + P_y[s0][cxp]);
+        jx += cy * (((-P_y[s0][cxp] + dey) + P_y[s0][cxp - CXr]) - P_y[s0][cym]);
+        jy += cx * (((-P_x[s0][cyp] + dex) + P_x[s0][cyp - 1]) - P_x[s0][cxm]);
+        float jz = 0.f;
+        if (IS3D) {
+          jx += cz * (P_x[sm][c0] - 2.f * dex + P_x[sp1][c0]);
+          jy += cz * (P_y[sm][c0] - 2.f * dey + P_y[sp1][c0]);
+          jx += cz * (((-P_z[s0][cxp] + dez) + P_z[sm][cxp]) - P_z[sm][c0]);
+          jy += cz * (((-P_z[s0][cyp] + dez) + P_z[sm][cyp]) - P_z[sm][c0]);
+          jz = 2.f * cz * (P_z[sm][c0] - 2.f * dez + P_z[sp1][c0]) +
+               cx * (P_z[s0][cxm] - 2.f * dez + P_z[s0][cxp]) +
+               cy * (P_z[s0][cym] - 2.f * dez + P_z[s0][cyp]);
+          jz += cx * (((-P_x[sp1][c0] + dex) + P_x[sp1][cxm]) - P_x[s0][cxm]);
+          jz += cy * (((-P_y[sp1][c0] + dey) + P_y[sp1][cym]) - P_y[s0][cym]);
+          jz += dsup(czp, szp, dez, X_r[sp1][c0], X_i[sp1][c0], P_r[sp1][c0], P_i[sp1][c0]);
+        }
+        jx += dsup(cxp_, sxp, dex, X_r[s0][cxp], X_i[s0][cxp], P_r[s0][cxp], P_i[s0][cxp]);
+        jy += dsup(cyp_, syp, dey, X_r[s0][cyp], X_i[s0][cyp], P_r[s0][cyp], P_i[s0][cyp]);
+        if (fl & kHoleX) jx = 0.f;
+        if (fl & kHoleY) jy = 0.f;
+        if (IS3D && (fl & kHoleZ)) jz = 0.f;
+
+        if (CHEB) {
+          const float cvf = cs.cv ? 1.f : 0.f;
+          auto step = [&](float pv, float jv, float vv, float wv) {
+            const float y = c_ss * pv;
+            const float ay = c_ss * (pv - hdt * jv);
+            const float res = c_om * (y + (cvf * vv - ay) * c_it) - c_sp * wv;
+            return fin ? vv - res : res;
+          };
+          out.psi(q, make_float2(step(p0.x, jr, v0.x, w0.x), step(p0.y, ji, v0.y, w0.y)));
+          out.links<IS3D>(q, step(praw_x, jx, v_x, w_x), step(praw_y, jy, v_y, w_y),
+                          IS3D ? step(praw_z, jz, v_z, w_z) : 0.f);
+        } else {
+          out.psi(q, make_float2(p0.x - hdt * jr, p0.y - hdt * ji));
+          out.links<IS3D>(q, praw_x - hdt * jx, praw_y - hdt * jy, IS3D ? praw_z - hdt * jz : 0.f);
+        }
+      }
+      if (IS3D && RING == 3) __syncthreads();
+    };
+    if constexpr (IS3D && RING == NS) {
+      static_assert(RING == 3, "the unrolled march has three slots");
+      for (int k = kb; k < ke; k += 3) {
+        step_k(k, 1, 0, 2, 1, 2);
+        if (k + 1 >= ke) break;
+        step_k(k + 1, 2, 1, 0, 2, 0);
+        if (k + 2 >= ke) break;
+        step_k(k + 2, 0, 2, 1, 0, 1);
+      }
+    } else {
+      for (int k = kb; k < ke; ++k) {
+        const int r = k - kb + 1;
+        step_k(k, r % RING, (r + RING - 1) % RING, (r + 1) % RING, r % NS, (r + 1) % NS);
       }
     }
-    if (IS3D && RING == 3) __syncthreads();
   };
-  if constexpr (IS3D && RING == NS) {
-    static_assert(RING == 3, "the unrolled march has three slots");
-    for (int k = kb; k < ke; k += 3) {
-      step_k(k, 1, 0, 2, 1, 2);
-      if (k + 1 >= ke) break;
-      step_k(k + 1, 2, 1, 0, 2, 0);
-      if (k + 2 >= ke) break;
-      step_k(k + 2, 0, 2, 1, 0, 1);
-    }
-  } else {
-    for (int k = kb; k < ke; ++k) {
-      const int r = k - kb + 1;
-      step_k(k, r % RING, (r + RING - 1) % RING, (r + 1) % RING, r % NS, (r + 1) % NS);
-    }
-  }
+  // a wave none of whose lanes stages a second cell runs the march without
+  // the second cell's loads (the branch is uniform per wave; every wave still
+  // passes the same barriers)
+  const unsigned int wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
+  if (wave0 + BS < (unsigned int)cellsr) march(SecondCell<true>{});
+  else march(SecondCell<false>{});
 }
 
 // ---------------------------------------------------------------------------
