@@ -87,7 +87,15 @@ typedef struct tdgl_solver_opts {
    * stays on the true residual of (I - dt/2 J) dx = -g; the Newton residual
    * is unchanged.  Off when dt/2 >= 0.9 (the interval would reach 0). */
   int32_t mixed_precond;
-  int32_t mixed_reserved;
+  /* TDGL_PREC_F64 only (no counterpart in the reference): the same Chebyshev
+   * polynomial as a right preconditioner of the FP64 TGCR, with the
+   * reference's finite-difference JVP (eps from ||r||) as the operator in its
+   * recurrence; TGCR keeps every direction p_k = M r_k it adds, so the update
+   * and the stop test stay those of the preconditioned system.  0 (the
+   * default): the reference's unpreconditioned TGCR; < 0: the automatic
+   * degree; > 0: that degree.  Everything stays FP64; Newton converges to the
+   * same discrete solution. */
+  int32_t f64_precond;
   /* Predictor (no counterpart in the reference, which starts Newton from
    * X + dt f(X), integrators.py:156-158): with x_prev non-NULL, a film whose
    * dt_prev entry is > 0 starts from the linear extrapolation

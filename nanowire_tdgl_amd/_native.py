@@ -57,7 +57,7 @@ class SolverOpts(C.Structure):
                 ("newton_max_iter", C.c_int32), ("precision", C.c_int32),
                 ("tol_gcr", C.c_double), ("eps_mf", C.c_double),
                 ("tgcr_max_iter", C.c_int64), ("mixed_tol_floor", C.c_double),
-                ("mixed_precond", C.c_int32), ("mixed_reserved", C.c_int32),
+                ("mixed_precond", C.c_int32), ("f64_precond", C.c_int32),
                 ("x_prev", C.c_void_p), ("dt_prev", C.c_void_p)]
 
 
@@ -627,11 +627,14 @@ def precond_code(precond) -> int:
 
 def solver_opts(newton_tol_f=1e-3, newton_tol_dx=1e-3, newton_max_iter=20, tol_gcr=1e-4,
                 eps_mf=1e-4, tgcr_max_iter=None, precision=PREC_F64,
-                mixed_tol_floor=1e-6, precond="chebyshev") -> SolverOpts:
+                mixed_tol_floor=1e-6, precond="chebyshev", f64_precond="none") -> SolverOpts:
+    """``precond``: the mixed method's TGCR preconditioner; ``f64_precond``:
+    the FP64 method's ("none", the default, is the reference's TGCR;
+    "chebyshev" or a degree: tdgl_solver_opts.f64_precond)."""
     if isinstance(precision, str):
         precision = {"f64": PREC_F64, "fp64": PREC_F64, "mixed": PREC_MIXED}[precision]
     return SolverOpts(float(newton_tol_f), float(newton_tol_dx), int(newton_max_iter),
                       int(precision), float(tol_gcr), float(eps_mf),
                       int(tgcr_max_iter) if tgcr_max_iter is not None else 0,
-                      float(mixed_tol_floor), precond_code(precond), 0)
+                      float(mixed_tol_floor), precond_code(precond), precond_code(f64_precond))
 

@@ -70,6 +70,11 @@ class AdjointProblem:
     tol_gcr: float = 1e-4
     adjoint_tol: float = 1e-12
     adjoint_precond: object = "chebyshev"  # the adjoint TGCR's preconditioner (as the mixed method's)
+    # The forward steps' FP64 TGCR: "chebyshev" (the same polynomial as a right
+    # preconditioner, the finite-difference JVP in its recurrence; an FP64 step
+    # that converges to the same solution with far fewer Krylov iterations) or
+    # "none" (the reference's TGCR).
+    forward_precond: object = "chebyshev"
     # Newton's start in the forward steps: "extrapolate" (X + (X - X_prev),
     # fewer Newton iterations) or "euler" (X + dt f, the reference's); with the
     # tight Newton tolerances above both give the same trajectory to ~1e-10.
@@ -125,7 +130,8 @@ class AdjointProblem:
         # a new run: outputs of earlier runs can no longer be differentiated
         self.generation += 1
         self.set_parameters(kappa, w_full)
-        self._opts = _native.solver_opts(self.newton_tol_f, self.newton_tol_dx, 20, self.tol_gcr)
+        self._opts = _native.solver_opts(self.newton_tol_f, self.newton_tol_dx, 20, self.tol_gcr,
+                                         f64_precond=self.forward_precond)
         if self.predictor not in ("euler", "extrapolate"):
             raise ValueError("predictor must be 'euler' or 'extrapolate'")
         ex = self.predictor == "extrapolate"

@@ -469,16 +469,17 @@ __global__ __launch_bounds__(kRedThreads) void k_adj_combine(long long len, cons
   publish<1, 0>(v, rd.pout);
 }
 
-// One Chebyshev step for A^T z = r, A^T v = v - hdt J^T v (g = J^T zp):
-// z = om (zp + (r - A^T zp) thinv) - (om - 1) zpp, zpp = nullptr meaning 0.
+// One Chebyshev step for A z = r: z = om (zp + (r - A zp) thinv) - (om - 1) zpp,
+// zpp = nullptr meaning 0.  gv holds J^T zp (the adjoint, A^T v = v - hdt J^T v)
+// or, with ap_given, A zp itself.
 __global__ __launch_bounds__(kRedThreads) void k_cheb64(long long len, double* z, const double* zp,
                                                         const double* zpp, const double* r,
                                                         const double* gv, double om, double thinv,
-                                                        double hdt, const double* stop) {
+                                                        double hdt, int ap_given, const double* stop) {
   if (*stop != 0.0) return;  // a queued iteration after the loop ended
   for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
        e += (long long)gridDim.x * kRedThreads) {
-    const double a = zp[e] - hdt * gv[e];
+    const double a = ap_given ? gv[e] : zp[e] - hdt * gv[e];
     double v = om * (zp[e] + (r[e] - a) * thinv);
     if (zpp) v = v - (om - 1.0) * zpp[e];
     z[e] = v;
@@ -503,9 +504,11 @@ int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, co
   return stream_blocks(g);
 }
 void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
-                   const double* gv, double om, double thinv, double hdt, const double* stop, hipStream_t st) {
+                   const double* gv, double om, double thinv, double hdt, const double* stop, hipStream_t st,
+                   bool ap_given) {
   const long long len = (long long)(2 + g.nphi) * g.n;
-  launch(k_cheb64, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, z, zp, zpp, r, gv, om, thinv, hdt, stop);
+  launch(k_cheb64, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, z, zp, zpp, r, gv, om, thinv, hdt,
+         ap_given ? 1 : 0, stop);
 }
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st) {
   const long long len = (long long)(2 + g.nphi) * g.n;

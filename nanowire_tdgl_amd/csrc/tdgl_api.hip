@@ -653,12 +653,36 @@ struct TgcrOp {
   int vjp = 0;
   const double* x1 = nullptr;   // vjp: the linearisation point
   double* acc_out = nullptr;    // x += sum alpha p accumulates here (default w_x)
-  // vjp: right preconditioner M = the degree-d Chebyshev iterate for A^T z = r
-  // on [a, b] (p_k = M r_k, the direction's source in k_final_p); 0: none
+  // right preconditioner M = the degree-d Chebyshev iterate for A z = r (A^T z
+  // = r for vjp) on [a, b] (p_k = M r_k, the direction's source in k_final_p);
+  // 0: none
   int cheb = 0;
   double thinv = 0.0;           // 1 / theta
   double om[kMaxChebDeg + 1] = {0.0};
 };
+
+// The Chebyshev right preconditioner on [1 - hdt, b] (the mixed method's
+// interval: J^T has J's spectrum); request < 0: the automatic degree (the
+// spectrum within 0.25 of 1), > 0: that degree; off (op.cheb = 0) when the
+// interval nears 0.  Allocates the recurrence's three vectors.
+int cheb_op(tdgl_ctx* ctx, double hdt, int request, TgcrOp& op) {
+  op.cheb = 0;
+  const double lo = 1.0 - hdt, hi = cheb_upper_bound(ctx, hdt);
+  if (request == 0 || !(lo >= 0.1) || !(hi > lo)) return TDGL_OK;
+  const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sig = th / de, s2 = sig * sig;
+  int d = request > 0 ? request : (int)ceil(acosh(4.0) / acosh(sig));
+  d = std::max(1, std::min(d, kMaxChebDeg));
+  op.cheb = d;
+  op.thinv = 1.0 / th;
+  op.om[1] = 1.0;
+  if (d >= 2) op.om[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
+  for (int m = 3; m <= d; ++m) op.om[m] = 1.0 / (1.0 - op.om[m - 1] / (4.0 * s2));
+  const size_t bytes = sizeof(double) * (size_t)ctx->len;
+  int rc;
+  for (double*& p : ctx->d_adjt)
+    if (!p && (rc = alloc(ctx, (void**)&p, bytes))) return rc;
+  return TDGL_OK;
+}
 
 // tgcr_matrix_free_trap (tgcr.py:96-156) on (I - dt/2 J) dx = r, r = -g held in
 // w_r, linearised at w_x with f(w_x) in w_f.  On success w_x += dx.
@@ -681,6 +705,8 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     return TDGL_OK;
   }
   const double xn = sqrt(xx);
+  if (op.cheb >= 1)  // the preconditioned breakdown test is relative to ||r_0||
+    HIPCHK(hipMemcpyAsync(ctx->d_res + RS_R0, &r0, sizeof(double), hipMemcpyHostToDevice, ctx->st));
   // Lookahead: without profiling, iteration k + 1 is queued before iteration
   // k's residual comes back, so the GPU never waits for the host.  The device
   // evaluates the loop's exit test itself (k_tgcr_check -> res[RS_STOP]); the
@@ -727,9 +753,26 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
       e.hdt = hdt;
       e.tgcr = 1;
       e.eps_num = eps_num;
+      const double* z = ctx->w_r;
+      if (op.cheb >= 1) {  // z = M r by the recurrence, A v = v - hdt J v by the FD JVP (eps from ||r||)
+        double* const* T = ctx->d_adjt;
+        tok = prof_begin(ctx, TDGL_K_JVP, 3.0 * S * op.cheb, 2 * op.cheb - 1);
+        launch_scale(g, T[1], op.thinv, ctx->w_r, ctx->st);  // z_1 = r / theta
+        Epi ei = e;
+        ei.ap0 = nullptr;
+        for (int m = 2; m <= op.cheb; ++m) {
+          // A z_{m-1} into Ap_i (its partials land in the unread buffer and are not committed)
+          launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, T[(m - 1) % 3], 0.0, ei, ctx->cfg,
+                         red_next(ctx), ctx->st);
+          launch_cheb64(g, T[m % 3], T[(m - 1) % 3], m >= 3 ? T[(m - 2) % 3] : nullptr, ctx->w_r,
+                        ctx->AP[i], op.om[m], op.thinv, hdt, ctx->d_res + RS_STOP, ctx->st, true);
+        }
+        prof_end(ctx, tok);
+        z = zsrc = T[op.cheb % 3];
+      }
       // bytes: read x, p, f0 (+ Ap_0 for beta_0), write Ap
       tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S);
-      red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, ctx->w_r, 0.0, e,
+      red_commit(ctx, launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, z, 0.0, e,
                                      ctx->cfg, red_next(ctx), ctx->st));
       prof_end(ctx, tok);
     }
@@ -775,7 +818,7 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     HIPCHK(hipEventSynchronize(ctx->ev_la[k & 1]));
     if (!ctx->prof.pending.empty()) prof_harvest(ctx);
     const double* h = ctx->h_la + 4 * (k & 1);  // brk, (xx), rr
-    s->jvp_evals++;
+    s->jvp_evals += std::max(1, op.cheb);
     s->bytes_moved += ((i > 0 ? 5.0 : 4.0) + mgs_vectors(i)) * S;
     if (h[0] != 0.0) break;  // tgcr.py:141-142
     s->bytes_moved += (double)(i + 4) * S + 3.0 * S;
@@ -804,14 +847,15 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
 // then each iterate) and w_gam.  On return w_x holds the last iterate:
 // the converged one, the one whose TGCR failed (newton.py:102-106), or the
 // last after max_iter iterations.
-int newton_solve(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, tdgl_step_stats* s) {
+int newton_solve(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, tdgl_step_stats* s,
+                 const TgcrOp& op = TgcrOp{}) {
   int rc;
   if ((rc = newton_eval(ctx, hdt, s))) return rc;
   if ((rc = readback(ctx, RS_XX, 3))) return rc;  // xx, rr, ginf
   for (int it = 1; it <= o->newton_max_iter; ++it) {
     const double rr = ctx->h_res[RS_RR], xx = ctx->h_res[RS_XX];
     int ok = 0;
-    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s))) return rc;
+    if ((rc = tgcr(ctx, hdt, o, rr, xx, &ok, s, op))) return rc;
     if (!ok) {  // newton.py:102-106
       s->tgcr_failed = 1;
       s->newton_iters = it;
@@ -844,6 +888,8 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
     return fail(ctx, TDGL_E_INVALID, "precision must be TDGL_PREC_F64 or TDGL_PREC_MIXED");
   if (o->x_prev && !o->dt_prev) return fail(ctx, TDGL_E_INVALID, "x_prev needs dt_prev");
   if (o->precision == TDGL_PREC_MIXED) return mixed_step_single(ctx, X, dt, o, s);
+  if (o->f64_precond < -1 || o->f64_precond > kMaxChebDeg)
+    return fail(ctx, TDGL_E_INVALID, "f64_precond must be -1 (automatic), 0 (none) or a degree up to 64");
   memset(s, 0, sizeof(*s));
   int rc;
   if ((rc = ensure_work(ctx))) return rc;
@@ -870,7 +916,9 @@ extern "C" int tdgl_trap_step(tdgl_ctx* ctx, double* X, double dt, const tdgl_so
     LAUNCHCHK();
     s->bytes_moved += 3.0 * S;
   }
-  if ((rc = newton_solve(ctx, hdt, o, s))) return rc;
+  TgcrOp op;  // the FP64 TGCR's optional Chebyshev right preconditioner
+  if (o->f64_precond && (rc = cheb_op(ctx, hdt, o->f64_precond, op))) return rc;
+  if ((rc = newton_solve(ctx, hdt, o, s, op))) return rc;
   if (s->converged) {
     HIPCHK(hipMemcpyAsync(X, ctx->w_x, sizeof(double) * ctx->len, hipMemcpyDeviceToDevice, ctx->st));
     s->bytes_moved += 2.0 * S;
@@ -986,21 +1034,8 @@ int tdgl_adjoint_solve(tdgl_ctx* ctx, const double* x1, const double* rhs, doubl
   op.vjp = 1;
   op.x1 = x1;
   op.acc_out = out;
-  // Chebyshev right preconditioner on [1 - hdt, b] (the mixed method's, for the
-  // transposed operator: same spectrum); off when the interval nears 0
-  const double lo = 1.0 - hdt, hi = cheb_upper_bound(ctx, hdt);
-  if (precond != 0 && lo >= 0.1 && hi > lo) {
-    const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sig = th / de, s2 = sig * sig;
-    int d = precond > 0 ? precond : (int)ceil(acosh(4.0) / acosh(sig));
-    d = std::max(1, std::min(d, kMaxChebDeg));
-    op.cheb = d;
-    op.thinv = 1.0 / th;
-    op.om[1] = 1.0;
-    if (d >= 2) op.om[2] = 2.0 * s2 / (2.0 * s2 - 1.0);
-    for (int m = 3; m <= d; ++m) op.om[m] = 1.0 / (1.0 - op.om[m - 1] / (4.0 * s2));
-    for (double*& p : ctx->d_adjt)
-      if (!p && (rc = alloc(ctx, (void**)&p, bytes))) return rc;
-  }
+  // Chebyshev right preconditioner (the transposed operator has the same spectrum)
+  if ((rc = cheb_op(ctx, hdt, precond, op))) return rc;
   int ok = 0;
   if ((rc = tgcr(ctx, hdt, &o, rr, 0.0, &ok, &st, op))) return rc;
   if (iters) *iters = st.tgcr_iters_last;

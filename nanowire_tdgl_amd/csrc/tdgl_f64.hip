@@ -735,7 +735,9 @@ __global__ __launch_bounds__(kMgsThreads) void k_mgs_block(long long len, double
   }
 }
 
-// ||Ap|| from the previous partials; breakdown test (tgcr.py:140-142);
+// ||Ap|| from the previous partials; breakdown test (tgcr.py:140-142: ||Ap|| <
+// 1e-14; right-preconditioned, ||Ap|| < 1e-14 ||r_0||, as for a unit right-hand
+// side, since Ap = A M r is then of the residual's size);
 // p_k = (r - sum_j beta_j p_j) / ||Ap|| ; Ap_k = Ap / ||Ap|| ; partials of alpha = <r, Ap_k>.
 // The p updates of tgcr.py:137 are applied here, per node, in the same j order,
 // which rounds exactly like the reference's vector-by-vector updates.
@@ -750,7 +752,7 @@ __global__ __launch_bounds__(kRedThreads) void k_final_p(long long n, const doub
   gather<1, 0>(r.pin, r.pcount, s1);
   const double nrm = sqrt(s1[0]);
   const double inv = 1.0 / nrm;
-  const bool brk = nrm < 1e-14;
+  const bool brk = nrm < (zv ? 1e-14 * r.res[RS_R0] : 1e-14);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     r.res[RS_NRM] = nrm;
     r.res[RS_INV] = inv;

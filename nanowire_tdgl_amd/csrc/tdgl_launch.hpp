@@ -46,7 +46,7 @@ enum ResSlot : int {
   RS_NRM2 = 10,  // mixed: ||Ap||^2 after the re-orthogonalisation pass
   RS_RW1 = 11,   // mixed: <r, Ap> after the first pass
   RS_RW2 = 12,   // mixed: <r, Ap> after the second pass
-  RS_R0 = 13,    // mixed: ||r_0||_2 of the current TGCR solve
+  RS_R0 = 13,    // ||r_0||_2 of the current TGCR solve (mixed; FP64 when right-preconditioned)
   RS_RK = 14,    // mixed: ||r_k||_2
   RS_GAMMA = 15, // mixed: coefficient of r_0 in the Newton update
   RS_BETA = 16,  // beta_j (mixed: first-pass coefficients; after the solve, the update's), j < kMaxKrylov
@@ -296,12 +296,14 @@ void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st);
 int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, const double* ap0,
                        const Red& rd, hipStream_t st);
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st);
-// One Chebyshev step of the adjoint solve's preconditioner, FP64:
-// z = om (zp + (r - (zp - hdt g)) thinv) - (om - 1) zpp   (zpp = nullptr: 0),
-// g = J^T zp; returns at once when *stop (res[RS_STOP]) is set.
+// One Chebyshev step of the FP64 preconditioners:
+// z = om (zp + (r - A zp) thinv) - (om - 1) zpp   (zpp = nullptr: 0), with
+// A zp = zp - hdt g, g = J^T zp (the adjoint solve), or A zp = g (ap_given: the
+// FP64 step's, g from the finite-difference JVP); returns at once when *stop
+// (res[RS_STOP]) is set.
 void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
                    const double* gv, double om, double thinv, double hdt, const double* stop,
-                   hipStream_t st);
+                   hipStream_t st, bool ap_given = false);
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);

@@ -200,12 +200,16 @@ def _problem(steps=6, every=2):
     return AdjointProblem(p, x0, dt=0.2, steps=steps, checkpoint_every=every), w0
 
 
-@pytest.mark.parametrize("predictor", ["extrapolate", "euler"])
-def test_trajectory_gradient_matches_finite_differences(predictor):
+@pytest.mark.parametrize("predictor,forward_precond",
+                         [("extrapolate", "chebyshev"), ("euler", "chebyshev"), ("extrapolate", "none")])
+def test_trajectory_gradient_matches_finite_differences(predictor, forward_precond):
+    """Both predictors, and the forward's FP64 TGCR with and without its
+    Chebyshev preconditioner (the reference's)."""
     import torch
     from nanowire_tdgl_amd.adjoint import TrapezoidalBz, midplane_target_loss
     prob, w0 = _problem()
     prob.predictor = predictor
+    prob.forward_precond = forward_precond
     kappa = torch.tensor(2.5, dtype=torch.float64, requires_grad=True)
     bz = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
     w = torch.tensor(w0, dtype=torch.float64, requires_grad=True)
