@@ -58,7 +58,9 @@ __device__ __forceinline__ void node_of(const Geo& g, long long q, int& i, int& 
 // w: adjoint weights (internal layout).  A: full grid x 5, accumulated.
 // gsc: per interior node (internal order) d<w,F>/d(sc_q), or nullptr.
 // pk: per-block partials of d<w,F>/d(kappa) (uniform kappa `kappa`), or nullptr.
-// w: adjoint weights (internal layout).  B: kVjpPlanes x n, zeroed, accumulated.
+// w: adjoint weights (internal layout).  B: kVjpPlanes x n, every (plane, node)
+// entry written: a node's 41 sums are accumulated in registers, in the order of
+// the terms below, and stored once.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt, const double* x,
                                                            const double* w, double* B, double* gsc,
@@ -78,8 +80,11 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
     const double mz = (IS3D && !(fl & kHoleZ)) ? w[4 * n + q] : 0.0;
 
     auto cell = [&](int ii, int jj, int kk) { return eff_cell<double, IS3D, false>(g, b, mt.flags, s, ii, jj, kk); };
-    // B[plane * n + q] belongs to this thread (q is its node)
-    auto acc = [&](int pl, double v) { B[(long long)pl * n + q] += v; };
+    // the node's entries of the planes (B[plane * n + q] belongs to this thread)
+    double bp[kVjpPlanes];
+#pragma unroll
+    for (int pl = 0; pl < kVjpPlanes; ++pl) bp[pl] = 0.0;
+    auto acc = [&](int pl, double v) { bp[pl] += v; };
     const Cell<double> c0 = cell(i, j, k), xm = cell(i - 1, j, k), xp = cell(i + 1, j, k);
     const Cell<double> ym = cell(i, j - 1, k), yp = cell(i, j + 1, k);
     Cell<double> zm{}, zp{};
@@ -252,6 +257,8 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
       const double kq = mt.k2 ? sqrt(mt.k2[q]) : kappa;
       kap[0] += (2.0 / kq) * ((mx * kx + my * ky) + mz * kzv);
     }
+#pragma unroll
+    for (int pl = 0; pl < kVjpPlanes; ++pl) B[(long long)pl * n + q] = bp[pl];
   }
   if (pk) publish<1, 0>(kap, pk);
 }
@@ -261,7 +268,7 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
 // in a fixed order (the six neighbours' and the node's own slots, then the
 // diagonals); every cell and component is written.
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, double* B, double* A) {
+__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A) {
   const long long n = g.n;
   const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
   for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
@@ -280,14 +287,9 @@ __global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, double* B, 
       return true;
     };
     constexpr int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    // every (plane, node) entry is read by exactly one cell, which clears it:
-    // the planes are zero again for the next k_vjp_cells
-    auto take = [&](int pl, long long q) {
-      double* p = B + (long long)pl * n + q;
-      const double v = *p;
-      *p = 0.0;
-      return v;
-    };
+    // every (plane, node) entry is read by exactly one cell (k_vjp_cells
+    // writes them all on every call)
+    auto take = [&](int pl, long long q) { return B[(long long)pl * n + q]; };
 #pragma unroll
     for (int s = 0; s < (IS3D ? 7 : 5); ++s) {
       long long q;

@@ -501,8 +501,8 @@ int ensure_acell(tdgl_ctx* ctx) {
   }
   return TDGL_OK;
 }
-// The per-node scatter planes of the VJP: zeroed at allocation, and cleared
-// again by every launch_cells_gather.
+// The per-node planes of the VJP (every entry written by each launch_vjp_cells,
+// read once by the launch_cells_gather after it).
 int ensure_vjpb(tdgl_ctx* ctx) {
   if (ctx->d_vjpb) return TDGL_OK;
   const size_t bytes = sizeof(double) * (size_t)vjp_planes() * (size_t)ctx->g.n;
