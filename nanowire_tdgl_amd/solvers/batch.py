@@ -184,7 +184,12 @@ class BatchGroups:
         for part in self.parts:
             r = BatchRunner([devices[i] for i in part])
             if own_queues and len(self.parts) > 1:
-                r.batch.own_queue()
+                try:
+                    r.batch.own_queue()
+                except _native.TDGLError as e:  # a runtime without CU-masked streams: shared queues
+                    import warnings
+                    warnings.warn(f"no stream with a queue of its own ({e}); the group shares HIP's "
+                                  "hardware queues", RuntimeWarning, stacklevel=2)
             st = r.batch.work_stream()
             r.batch.set_stream(st)
             self.runners.append(r)
