@@ -756,16 +756,20 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
       const double* z = ctx->w_r;
       if (op.cheb >= 1) {  // z = M r by the recurrence, A v = v - hdt J v by the FD JVP (eps from ||r||)
         double* const* T = ctx->d_adjt;
-        tok = prof_begin(ctx, TDGL_K_JVP, 3.0 * S * op.cheb, 2 * op.cheb - 1);
+        tok = prof_begin(ctx, TDGL_K_JVP, 4.0 * S * op.cheb, op.cheb);
         launch_scale(g, T[1], op.thinv, ctx->w_r, ctx->st);  // z_1 = r / theta
         Epi ei = e;
         ei.ap0 = nullptr;
+        ei.cr = ctx->w_r;
+        ei.cthinv = op.thinv;
         for (int m = 2; m <= op.cheb; ++m) {
-          // A z_{m-1} into Ap_i (its partials land in the unread buffer and are not committed)
+          // z_m from A z_{m-1} in the stencil's epilogue (its partials land in the
+          // unread buffer and are not committed)
+          ei.cz = T[m % 3];
+          ei.czpp = m >= 3 ? T[(m - 2) % 3] : nullptr;
+          ei.com = op.om[m];
           launch_stencil(M_JVP, true, g, ctx->b, ctx->mt, ctx->w_x, T[(m - 1) % 3], 0.0, ei, ctx->cfg,
                          red_next(ctx), ctx->st);
-          launch_cheb64(g, T[m % 3], T[(m - 1) % 3], m >= 3 ? T[(m - 2) % 3] : nullptr, ctx->w_r,
-                        ctx->AP[i], op.om[m], op.thinv, hdt, ctx->d_res + RS_STOP, ctx->st, true);
         }
         prof_end(ctx, tok);
         z = zsrc = T[op.cheb % 3];

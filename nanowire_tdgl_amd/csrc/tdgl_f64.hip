@@ -364,7 +364,29 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
           const double jz = (dz_ - c.gz) * e.inv_eps;
           apz = c.xz - e.hdt * jz;
         }
-        st5(o0, apr, api, apx, apy, apz);
+        if (e.cz) {  // the recurrence step, as k_cheb64 with A p given (the same expression)
+          const V64 R = planes64(e.cr, n);
+          const double2 rp = R.psi(q);
+          const double rx = R.ph(2, q), ry = R.ph(3, q), rz = IS3D ? R.ph(4, q) : 0.0;
+          double2 zq = make_double2(0.0, 0.0);
+          double zx = 0.0, zy = 0.0, zz = 0.0;
+          if (e.czpp) {
+            const V64 Z = planes64(e.czpp, n);
+            zq = Z.psi(q);
+            zx = Z.ph(2, q);
+            zy = Z.ph(3, q);
+            if (IS3D) zz = Z.ph(4, q);
+          }
+          auto step = [&](double p, double r, double a, double zpp) {
+            double v = e.com * (p + (r - a) * e.cthinv);
+            if (e.czpp) v = v - (e.com - 1.0) * zpp;
+            return v;
+          };
+          st5(wplanes64(e.cz, n), step(c.xr, rp.x, apr, zq.x), step(c.xi, rp.y, api, zq.y),
+              step(c.xx, rx, apx, zx), step(c.xy, ry, apy, zy), IS3D ? step(c.xz, rz, apz, zz) : 0.0);
+        } else {
+          st5(o0, apr, api, apx, apy, apz);
+        }
         if (e.ap0) {
           red[0] += ((apr * c.ar + api * c.ai) + apx * c.ax) + (apy * c.ay + apz * c.az);
         } else {

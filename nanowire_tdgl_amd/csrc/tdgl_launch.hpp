@@ -157,6 +157,12 @@ struct Epi {
   // host's formula, same rounding), and a return when res[RS_STOP] is set
   int tgcr;
   double eps_num;        // eps_mf * (1 + ||x||)
+  // JVP with cz: one step of the FP64 TGCR's Chebyshev recurrence in place of
+  // storing Ap: cz = com (p + (cr - A p) cthinv) - (com - 1) czpp (czpp = nullptr: 0)
+  double* cz;
+  const double* cr;
+  const double* czpp;
+  double com, cthinv;
 };
 
 // Kernel-side view of a batch of films (all device pointers).
