@@ -64,7 +64,8 @@ __device__ __forceinline__ void node_of(const Geo& g, long long q, int& i, int& 
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt, const double* x,
                                                            const double* w, double* B, double* gsc,
-                                                           double kappa, double* pk) {
+                                                           double kappa, double* pk, const double* stop) {
+  if (stop && *stop != 0.0) return;  // a queued TGCR iteration after the loop ended
   const long long n = g.n;
   const Src<double, false> s{x, nullptr, 0.0, n};
   double kap[1] = {0.0};
@@ -268,7 +269,9 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
 // in a fixed order (the six neighbours' and the node's own slots, then the
 // diagonals); every cell and component is written.
 template <bool IS3D>
-__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A) {
+__global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A,
+                                                              const double* stop) {
+  if (stop && *stop != 0.0) return;
   const long long n = g.n;
   const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
   for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
@@ -314,7 +317,9 @@ __global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const doubl
 // (J^T w)_q: the cell adjoint gathered over the cells whose values come from q.
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_bc_gather(Geo g, const uint8_t* flags,
-                                                           const double* A, double* out) {
+                                                           const double* A, double* out,
+                                                           const double* stop, AdjCheb ch) {
+  if (stop && *stop != 0.0) return;
   const long long n = g.n;
   for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
        q += (long long)gridDim.x * kRedThreads) {
@@ -374,10 +379,24 @@ __global__ __launch_bounds__(kRedThreads) void k_bc_gather(Geo g, const uint8_t*
         }
       }
     }
-    reinterpret_cast<double2*>(out)[q] = make_double2(gr, gi);
-    out[2 * n + q] = gx;
-    out[3 * n + q] = gy;
-    if (IS3D) out[4 * n + q] = gz;
+    if (ch.z) {  // the recurrence step, J^T zp = (gr, gi, gx, gy, gz)
+      auto step = [&](long long e, double gv) {
+        const double a = ch.zp[e] - ch.hdt * gv;
+        double v = ch.om * (ch.zp[e] + (ch.r[e] - a) * ch.thinv);
+        if (ch.zpp) v = v - (ch.om - 1.0) * ch.zpp[e];
+        ch.z[e] = v;
+      };
+      step(2 * q, gr);
+      step(2 * q + 1, gi);
+      step(2 * n + q, gx);
+      step(3 * n + q, gy);
+      if (IS3D) step(4 * n + q, gz);
+    } else {
+      reinterpret_cast<double2*>(out)[q] = make_double2(gr, gi);
+      out[2 * n + q] = gx;
+      out[3 * n + q] = gy;
+      if (IS3D) out[4 * n + q] = gz;
+    }
   }
 }
 
@@ -471,23 +490,6 @@ __global__ __launch_bounds__(kRedThreads) void k_adj_combine(long long len, cons
   publish<1, 0>(v, rd.pout);
 }
 
-// One Chebyshev step for A z = r: z = om (zp + (r - A zp) thinv) - (om - 1) zpp,
-// zpp = nullptr meaning 0.  gv holds J^T zp (the adjoint, A^T v = v - hdt J^T v)
-// or, with ap_given, A zp itself.
-__global__ __launch_bounds__(kRedThreads) void k_cheb64(long long len, double* z, const double* zp,
-                                                        const double* zpp, const double* r,
-                                                        const double* gv, double om, double thinv,
-                                                        double hdt, int ap_given, const double* stop) {
-  if (*stop != 0.0) return;  // a queued iteration after the loop ended
-  for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
-       e += (long long)gridDim.x * kRedThreads) {
-    const double a = ap_given ? gv[e] : zp[e] - hdt * gv[e];
-    double v = om * (zp[e] + (r[e] - a) * thinv);
-    if (zpp) v = v - (om - 1.0) * zpp[e];
-    z[e] = v;
-  }
-}
-
 // y = s x (len doubles; y may alias x).
 __global__ __launch_bounds__(kRedThreads) void k_scale(long long len, double* y, double s, const double* x) {
   for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
@@ -505,13 +507,6 @@ int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, co
   launch(k_adj_combine, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, r, ap, hdt, ap0, rd);
   return stream_blocks(g);
 }
-void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
-                   const double* gv, double om, double thinv, double hdt, const double* stop, hipStream_t st,
-                   bool ap_given) {
-  const long long len = (long long)(2 + g.nphi) * g.n;
-  launch(k_cheb64, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, z, zp, zpp, r, gv, om, thinv, hdt,
-         ap_given ? 1 : 0, stop);
-}
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st) {
   const long long len = (long long)(2 + g.nphi) * g.n;
   launch(k_scale, dim3(stream_blocks(g)), dim3(kRedThreads), 0, st, len, y, s, x);
@@ -519,22 +514,23 @@ void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_
 
 int vjp_planes() { return kVjpPlanes; }
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
-                      double* B, double* gsc, double kappa, double* pk, hipStream_t st) {
+                      double* B, double* gsc, double kappa, double* pk, hipStream_t st, const double* stop) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
-  else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk);
+  if (g.is3d) launch(k_vjp_cells<true>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk, stop);
+  else launch(k_vjp_cells<false>, gr, bl, 0, st, g, b, mt, x, w, B, gsc, kappa, pk, stop);
 }
-void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st) {
+void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st, const double* stop) {
   const long long dimx = g.mk * (g.is3d ? (long long)(g.Nz + 1) : 1LL);
   long long nb = (dimx + kRedThreads - 1) / kRedThreads;
   if (nb > 4096) nb = 4096;
-  if (g.is3d) launch(k_cells_gather<true>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A);
-  else launch(k_cells_gather<false>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A);
+  if (g.is3d) launch(k_cells_gather<true>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A, stop);
+  else launch(k_cells_gather<false>, dim3((unsigned)nb), dim3(kRedThreads), 0, st, g, B, A, stop);
 }
-void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st) {
+void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st,
+                      const double* stop, const AdjCheb& ch) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
-  if (g.is3d) launch(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out);
-  else launch(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out);
+  if (g.is3d) launch(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out, stop, ch);
+  else launch(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out, stop, ch);
 }
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);

@@ -724,10 +724,12 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
     int tok;
     const double* zsrc = nullptr;  // the direction's source when preconditioned
     if (op.vjp) {  // Ap = z - hdt J(x1)^T z (z = M r, or r), then <Ap, Ap_0> (or ||Ap||^2)
-      auto jt = [&](const double* v) {  // J(x1)^T v into AP[i]
-        launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, v, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st);
-        launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st);
-        launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st);
+      const double* stop = ctx->d_res + RS_STOP;
+      // J(x1)^T v into AP[i], or (ch.z) the recurrence step formed from it
+      auto jt = [&](const double* v, const AdjCheb& ch) {
+        launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, v, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st, stop);
+        launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st, stop);
+        launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st, stop, ch);
       };
       tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S, 4);
       const double* z = ctx->w_r;
@@ -735,13 +737,19 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
         double* const* T = ctx->d_adjt;
         launch_scale(g, T[1], op.thinv, ctx->w_r, ctx->st);  // z_1 = r / theta
         for (int m = 2; m <= op.cheb; ++m) {
-          jt(T[(m - 1) % 3]);
-          launch_cheb64(g, T[m % 3], T[(m - 1) % 3], m >= 3 ? T[(m - 2) % 3] : nullptr, ctx->w_r,
-                        ctx->AP[i], op.om[m], op.thinv, hdt, ctx->d_res + RS_STOP, ctx->st);
+          AdjCheb ch;
+          ch.z = T[m % 3];
+          ch.zp = T[(m - 1) % 3];
+          ch.r = ctx->w_r;
+          ch.zpp = m >= 3 ? T[(m - 2) % 3] : nullptr;
+          ch.om = op.om[m];
+          ch.thinv = op.thinv;
+          ch.hdt = hdt;
+          jt(ch.zp, ch);
         }
         z = zsrc = T[op.cheb % 3];
       }
-      jt(z);
+      jt(z, AdjCheb{});
       red_commit(ctx, launch_adj_combine(g, z, ctx->AP[i], hdt, (i > 0) ? ctx->AP[0] : nullptr,
                                          red_next(ctx), ctx->st));
       prof_end(ctx, tok);

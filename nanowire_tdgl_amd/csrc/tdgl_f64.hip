@@ -364,7 +364,7 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
           const double jz = (dz_ - c.gz) * e.inv_eps;
           apz = c.xz - e.hdt * jz;
         }
-        if (e.cz) {  // the recurrence step, as k_cheb64 with A p given (the same expression)
+        if (e.cz) {  // the recurrence step: z = om (p + (r - A p) thinv) - (om - 1) zpp
           const V64 R = planes64(e.cr, n);
           const double2 rp = R.psi(q);
           const double rx = R.ph(2, q), ry = R.ph(3, q), rz = IS3D ? R.ph(4, q) : 0.0;

@@ -291,26 +291,30 @@ void launch_baccum(const BatchK& bk, double* x, const float* r0v, float* const* 
 void launch_baccum_u(const BatchK& bk, float* u, const float* r0v, float* const* Q, const int* fstat,
                      hipStream_t st);
 // Adjoint kernels (tdgl_adjoint.hip).  A: full grid x 5 cell adjoint.  B:
-// vjp_planes() x n per-node scatter planes, zero before launch_vjp_cells
-// (zeroed once at allocation), turned into A by launch_cells_gather, which
-// clears them again.
+// vjp_planes() x n per-node planes, all written by launch_vjp_cells and turned
+// into A by launch_cells_gather.  stop (non-null inside TGCR): res[RS_STOP];
+// the kernels of an iteration queued after the loop ended return at once.
 int vjp_planes();
 void launch_vjp_cells(const Geo& g, const Bnd& b, const Mat& mt, const double* x, const double* w,
-                      double* B, double* gsc, double kappa, double* pk, hipStream_t st);
-void launch_cells_gather(const Geo& g, double* B, double* A, hipStream_t st);
+                      double* B, double* gsc, double kappa, double* pk, hipStream_t st,
+                      const double* stop = nullptr);
+void launch_cells_gather(const Geo& g, const double* B, double* A, hipStream_t st,
+                         const double* stop = nullptr);
+// One step of the adjoint TGCR's Chebyshev recurrence done where J^T zp is
+// formed: z = om (zp + (r - (zp - hdt J^T zp)) thinv) - (om - 1) zpp (zpp = nullptr: 0).
+struct AdjCheb {
+  double* z = nullptr;  // nullptr: write J^T w itself
+  const double* zp = nullptr;
+  const double* r = nullptr;
+  const double* zpp = nullptr;
+  double om = 0.0, thinv = 0.0, hdt = 0.0;
+};
 // TGCR on the adjoint step operator: ap = r - hdt ap, partials of <ap, ap0> (or ||ap||^2).
 int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, const double* ap0,
                        const Red& rd, hipStream_t st);
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st);
-// One Chebyshev step of the FP64 preconditioners:
-// z = om (zp + (r - A zp) thinv) - (om - 1) zpp   (zpp = nullptr: 0), with
-// A zp = zp - hdt g, g = J^T zp (the adjoint solve), or A zp = g (ap_given: the
-// FP64 step's, g from the finite-difference JVP); returns at once when *stop
-// (res[RS_STOP]) is set.
-void launch_cheb64(const Geo& g, double* z, const double* zp, const double* zpp, const double* r,
-                   const double* gv, double om, double thinv, double hdt, const double* stop,
-                   hipStream_t st, bool ap_given = false);
-void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st);
+void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st,
+                      const double* stop = nullptr, const AdjCheb& ch = AdjCheb{});
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
