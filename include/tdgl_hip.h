@@ -87,8 +87,9 @@ typedef struct tdgl_solver_opts {
    * stays on the true residual of (I - dt/2 J) dx = -g; the Newton residual
    * is unchanged.  Off when dt/2 >= 0.9 (the interval would reach 0). */
   int32_t mixed_precond;
-  /* TDGL_PREC_F64 only (no counterpart in the reference): the same Chebyshev
-   * polynomial as a right preconditioner of the FP64 TGCR, with the
+  /* TDGL_PREC_F64 steps of tdgl_trap_step only (no counterpart in the
+   * reference; tdgl_newton_trap and tdgl_tgcr_trap stay the reference's): the
+   * same Chebyshev polynomial as a right preconditioner of the FP64 TGCR, with the
    * reference's finite-difference JVP (eps from ||r||) as the operator in its
    * recurrence; TGCR keeps every direction p_k = M r_k it adds, so the update
    * and the stop test stay those of the preconditioned system.  0 (the
