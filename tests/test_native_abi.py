@@ -85,3 +85,14 @@ def test_device_vector_check_rejects_host_and_strided_tensors():
     assert not _is_vec(x.float(), torch.float64, 10)  # dtype
     assert not _is_vec(x, torch.float64, 9)           # length
     assert not _is_vec(None, torch.float64, 10)
+
+
+def test_solver_opts_preconditioner_codes():
+    """solver_opts: the mixed method's preconditioner defaults to the automatic
+    Chebyshev degree, the FP64 method's to none (the reference's TGCR)."""
+    from nanowire_tdgl_amd import _native
+    o = _native.solver_opts()
+    assert (o.mixed_precond, o.f64_precond) == (-1, 0)
+    o = _native.solver_opts(precision="f64", f64_precond="chebyshev", precond="none")
+    assert (o.mixed_precond, o.f64_precond) == (0, -1)
+    assert _native.solver_opts(f64_precond=5).f64_precond == 5
