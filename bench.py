@@ -32,7 +32,8 @@ cores.  `batched.full_run` is the batch's whole configured run (every film's
 Secondary sections of the same line (each with its own roofline):
   fp64_plain -- the 128x128x16 production film with a 32x32 hole, plain FP64
                 (the parity path), TGCR tol 1e-8;
-  mixed      -- the same film and steps in mixed precision;
+  mixed      -- the same film and steps in mixed precision (and, beside it, FP64
+                with its TGCR preconditioned by the same polynomial);
   large      -- 256x256x64 film with 1% insulating spheres (kappa 3, Bz 0.5,
                 TGCR tol 1e-4): FP64 vs mixed steps and their error;
   adjoint    -- 96x96x12 film (kappa 2.5, Bz 0.3, mask w = 0.05 U(0,1) seed
@@ -193,11 +194,11 @@ def roofline_of(per_class: dict) -> dict:
 
 
 def timed_run(dev, x0, precision, steps, warmup, tol, profile=None, precond="chebyshev",
-              predictor="euler"):
+              predictor="euler", **opts):
     """Warm up, time `steps` accepted steps, then (optionally) re-run them
     instrumented from the same state.  Returns (Stepper, seconds, per_class, roofline)."""
     import torch
-    s = Stepper(dev, x0, tol_gcr=tol, precision=precision, precond=precond, predictor=predictor)
+    s = Stepper(dev, x0, tol_gcr=tol, precision=precision, precond=precond, predictor=predictor, **opts)
     s.accepted_steps(warmup)
     torch.cuda.synchronize()
     snap = s.snapshot()
@@ -236,6 +237,12 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
     vu = n * steps / elu
     erru = rel_errors(u.observables(), fp64_stepper.observables())
     log(f"[bench] mixed, no preconditioner: {vu:.4g} cell-steps/s ({vu / v64:.2f}x FP64), err {erru}")
+    # FP64 with its TGCR right-preconditioned by the same polynomial (an FP64
+    # step of this build, not the reference's method)
+    fpc, elp, _, _ = timed_run(dev, x0, "f64", steps, warmup, tol, profile=False, f64_precond="chebyshev")
+    vp = n * steps / elp
+    log(f"[bench] FP64, preconditioned TGCR: {vp:.4g} cell-steps/s ({vp / v64:.2f}x FP64 plain; "
+        f"mixed {v / vp:.2f}x it)")
     # the FP64 solution of the same step equations, Newton converged (untimed)
     ref = Stepper(dev, x0, tol_gcr=tol, newton_tol_dx=CONVERGED_NEWTON_DX)
     ref.accepted_steps(warmup + steps)
@@ -243,7 +250,8 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
     obs = ref.observables()
     conv = {"mixed": rel_errors(s.observables(), obs),
             "mixed_unpreconditioned": rel_errors(u.observables(), obs),
-            "fp64_same_tolerances": rel_errors(fp64_stepper.observables(), obs)}
+            "fp64_same_tolerances": rel_errors(fp64_stepper.observables(), obs),
+            "fp64_preconditioned": rel_errors(fpc.observables(), obs)}
     log(f"[bench] vs FP64 with Newton converged (dx <= {CONVERGED_NEWTON_DX:g}): {conv}")
     # Newton started from the extrapolated last step instead of X + dt f(X),
     # for the mixed method and, as its like-for-like baseline, for FP64
@@ -271,7 +279,10 @@ def section_mixed(dev, x0, fp64_stepper, fp64_el, steps, warmup, tol):
             "newton_iters_per_step": per_step(s, "newton_iters", steps),
             "extrapolating_predictor": ex,
             "unpreconditioned": {"value": vu, "speedup_vs_fp64": vu / v64, "rel_err_vs_fp64": erru,
-                                 "tgcr_iters_per_step": per_step(u, "tgcr_iters_total", steps)}}
+                                 "tgcr_iters_per_step": per_step(u, "tgcr_iters_total", steps)},
+            "fp64_preconditioned": {"value": vp, "speedup_vs_fp64": vp / v64, "mixed_speedup_over_it": v / vp,
+                                    "tgcr_iters_per_step": per_step(fpc, "tgcr_iters_total", steps),
+                                    "fd_jvps_per_step": per_step(fpc, "jvp_evals", steps)}}
 
 
 def section_large(steps, warmup):
