@@ -566,6 +566,19 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
     el64 = time.perf_counter() - c0
     v64 = float(ns.sum()) * attempts / el64
     log(f"[bench] FP64 loop: {v64:.4g} cell-steps/s ({el64:.2f} s); batched/loop = {v / v64:.2f}")
+    # the same loop with the FP64 TGCR preconditioned by the mixed method's polynomial (an FP64
+    # step of this build): what the batch gains from precision and batching alone
+    psteppers = [Stepper(f.device, f.x0, t_stop=BATCH_T_STOP, tol_gcr=BATCH_TOL, f64_precond="chebyshev")
+                 for f in films]
+    for s_ in psteppers:
+        s_.accepted_steps(warmup)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    for s_ in psteppers:
+        s_.accepted_steps(attempts)
+    torch.cuda.synchronize()
+    v64p = float(ns.sum()) * attempts / (time.perf_counter() - c0)
+    log(f"[bench] FP64 loop, preconditioned TGCR: {v64p:.4g} cell-steps/s; batched/it = {v / v64p:.2f}")
     # per-film error against the oracle of the comparison (FP64, Newton converged)
     # and against the FP64 loop at the same tolerances, films at the same time
     errs = {"mixed_vs_fp64_converged": [], "fp64_same_tol_vs_fp64_converged": [],
@@ -592,6 +605,8 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
             for k, v_ in errs.items() if v_}
     log(f"[bench] batched error distribution over {compared} films: {dist}")
     out.update({"fp64_loop_cell_steps_s": v64, "speedup_vs_fp64_loop": v / v64,
+                "fp64_preconditioned_loop_cell_steps_s": v64p,
+                "speedup_vs_fp64_preconditioned_loop": v / v64p,
                 "speedup_extrapolating_vs_fp64_loop": ve / v64,
                 "speedup_unpreconditioned_vs_fp64_loop": vu / v64,
                 "films_compared": compared,
