@@ -554,7 +554,8 @@ class Batch:
         self._call("tdgl_batch_own_queue")
 
     def work_stream(self):
-        """The batch's own stream (its kernels run there), as a torch stream."""
+        """The batch's own stream (its kernels run there), as a torch stream;
+        valid while the batch lives and until :meth:`own_queue` replaces it."""
         p = C.c_void_p()
         self._call("tdgl_batch_work_stream", C.byref(p))
         return self.torch.cuda.ExternalStream(p.value, device=self.device)
