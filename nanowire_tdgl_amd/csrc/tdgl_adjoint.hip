@@ -267,12 +267,46 @@ __global__ __launch_bounds__(kRedThreads) void k_vjp_cells(Geo g, Bnd b, Mat mt,
 // Cell adjoint A[cell][component] of the full grid: each cell sums what its
 // source nodes (cell - offset, interior only) left in the planes of k_vjp_cells,
 // in a fixed order (the six neighbours' and the node's own slots, then the
-// diagonals); every cell and component is written.
+// diagonals).  Every (plane, node) entry is read by exactly one cell
+// (k_vjp_cells writes them all on every call).
+template <bool IS3D>
+__device__ __forceinline__ void gather_cell(const Geo& g, const double* B, int ci, int cj, int ck,
+                                            double (&a)[5]) {
+  const long long n = g.n;
+#pragma unroll
+  for (int comp = 0; comp < 5; ++comp) a[comp] = 0.0;
+  // the node that scattered into this cell with offset (di, dj, dk), if interior
+  auto src = [&](int di, int dj, int dk, long long& q) {
+    const int si = ci - di, sj = cj - dj, sk = ck - dk;
+    if (si < 1 || si > g.Nx - 1 || sj < 1 || sj > g.Ny - 1) return false;
+    if (IS3D && (sk < 1 || sk > g.Nz - 1)) return false;
+    q = qix(g, si, sj, sk);
+    return true;
+  };
+  constexpr int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  auto take = [&](int pl, long long q) { return B[(long long)pl * n + q]; };
+#pragma unroll
+  for (int s = 0; s < (IS3D ? 7 : 5); ++s) {
+    long long q;
+    if (src(off[s][0], off[s][1], off[s][2], q))
+#pragma unroll
+      for (int comp = 0; comp < 5; ++comp) a[comp] += take(plane(s, comp), q);
+  }
+  long long q;
+  if (src(1, -1, 0, q)) a[LY] += take(D_XP_YM, q);
+  if (src(-1, 1, 0, q)) a[LX] += take(D_XM_YP, q);
+  if (IS3D) {
+    if (src(1, 0, -1, q)) a[LZ] += take(D_XP_ZM, q);
+    if (src(0, 1, -1, q)) a[LZ] += take(D_YP_ZM, q);
+    if (src(-1, 0, 1, q)) a[LX] += take(D_XM_ZP, q);
+    if (src(0, -1, 1, q)) a[LY] += take(D_YM_ZP, q);
+  }
+}
+
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const double* B, double* A,
                                                               const double* stop) {
   if (stop && *stop != 0.0) return;
-  const long long n = g.n;
   const long long dimx = g.mk * (IS3D ? (long long)(g.Nz + 1) : 1LL);
   for (long long c = (long long)blockIdx.x * kRedThreads + threadIdx.x; c < dimx;
        c += (long long)gridDim.x * kRedThreads) {
@@ -280,124 +314,131 @@ __global__ __launch_bounds__(kRedThreads) void k_cells_gather(Geo g, const doubl
     const long long t = c / g.mj;
     const int cj = (int)(t % (g.Ny + 1));
     const int ck = IS3D ? (int)(t / (g.Ny + 1)) : 0;
-    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    // the node that scattered into this cell with offset (di, dj, dk), if interior
-    auto src = [&](int di, int dj, int dk, long long& q) {
-      const int si = ci - di, sj = cj - dj, sk = ck - dk;
-      if (si < 1 || si > g.Nx - 1 || sj < 1 || sj > g.Ny - 1) return false;
-      if (IS3D && (sk < 1 || sk > g.Nz - 1)) return false;
-      q = qix(g, si, sj, sk);
-      return true;
-    };
-    constexpr int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    // every (plane, node) entry is read by exactly one cell (k_vjp_cells
-    // writes them all on every call)
-    auto take = [&](int pl, long long q) { return B[(long long)pl * n + q]; };
-#pragma unroll
-    for (int s = 0; s < (IS3D ? 7 : 5); ++s) {
-      long long q;
-      if (src(off[s][0], off[s][1], off[s][2], q))
-#pragma unroll
-        for (int comp = 0; comp < 5; ++comp) a[comp] += take(plane(s, comp), q);
-    }
-    long long q;
-    if (src(1, -1, 0, q)) a[LY] += take(D_XP_YM, q);
-    if (src(-1, 1, 0, q)) a[LX] += take(D_XM_YP, q);
-    if (IS3D) {
-      if (src(1, 0, -1, q)) a[LZ] += take(D_XP_ZM, q);
-      if (src(0, 1, -1, q)) a[LZ] += take(D_YP_ZM, q);
-      if (src(-1, 0, 1, q)) a[LX] += take(D_XM_ZP, q);
-      if (src(0, -1, 1, q)) a[LY] += take(D_YM_ZP, q);
-    }
+    double a[5];
+    gather_cell<IS3D>(g, B, ci, cj, ck, a);
 #pragma unroll
     for (int comp = 0; comp < 5; ++comp) A[c * 5 + comp] = a[comp];
   }
 }
 
-// (J^T w)_q: the cell adjoint gathered over the cells whose values come from q.
+// (J^T w)_q: the cell adjoint gathered over the cells whose values come from q
+// (its own cell and the face ghosts copied from it); cell(ii, jj, kk, a) gives
+// a cell's five components.  Then J^T w is stored, or (ch.z) the recurrence
+// step formed from it.
+template <bool IS3D, class CellA>
+__device__ __forceinline__ void bc_node(const Geo& g, const uint8_t* flags, long long q, double* out,
+                                        const AdjCheb& ch, CellA cell) {
+  const long long n = g.n;
+  int i, j, k;
+  node_of<IS3D>(g, q, i, j, k);
+  auto F = [&](int ii, int jj, int kk) { return flags ? flags[mfull(g, ii, jj, kk)] : (uint8_t)0; };
+  const uint8_t f0 = F(i, j, k);
+  double a0[5];
+  cell(i, j, k, a0);
+  double gr = a0[CR], gi = a0[CI];
+  double gx = (i != g.Nx - 1 && !(f0 & kHoleX)) ? a0[LX] : 0.0;
+  double gy = (j != g.Ny - 1 && !(f0 & kHoleY)) ? a0[LY] : 0.0;
+  double gz = (IS3D && k != g.Nz - 1 && !(f0 & kHoleZ)) ? a0[LZ] : 0.0;
+  double ag[5];
+  // x faces
+  for (int side = 0; side < 2; ++side) {
+    const int gi_ = side == 0 ? 0 : g.Nx;
+    const int src = g.per_x ? (side == 0 ? g.Nx - 1 : 1) : (side == 0 ? 1 : g.Nx - 1);
+    if (i != src) continue;
+    const uint8_t fg = F(gi_, j, k);
+    cell(gi_, j, k, ag);
+    gr += ag[CR];
+    gi += ag[CI];
+    if (g.per_x) {
+      if (side == 1 && g.Nx - 1 != 1 && !(fg & kHoleX)) gx += ag[LX];
+    } else {
+      if (j != g.Ny - 1 && !(fg & kHoleY)) gy += ag[LY];
+      if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += ag[LZ];
+    }
+  }
+  // y faces
+  for (int side = 0; side < 2; ++side) {
+    const int gj = side == 0 ? 0 : g.Ny;
+    const int src = g.per_y ? (side == 0 ? g.Ny - 1 : 1) : (side == 0 ? 1 : g.Ny - 1);
+    if (j != src) continue;
+    const uint8_t fg = F(i, gj, k);
+    cell(i, gj, k, ag);
+    gr += ag[CR];
+    gi += ag[CI];
+    if (g.per_y) {
+      if (side == 1 && g.Ny - 1 != 1 && !(fg & kHoleY)) gy += ag[LY];
+    } else {
+      if (i != g.Nx - 1 && !(fg & kHoleX)) gx += ag[LX];
+      if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += ag[LZ];
+    }
+  }
+  // z faces
+  if (IS3D) {
+    for (int side = 0; side < 2; ++side) {
+      const int gk = side == 0 ? 0 : g.Nz;
+      const int src = g.per_z ? (side == 0 ? g.Nz - 1 : 1) : (side == 0 ? 1 : g.Nz - 1);
+      if (k != src) continue;
+      const uint8_t fg = F(i, j, gk);
+      cell(i, j, gk, ag);
+      gr += ag[CR];
+      gi += ag[CI];
+      if (g.per_z) {
+        if (side == 1 && g.Nz - 1 != 1 && !(fg & kHoleZ)) gz += ag[LZ];
+      } else {
+        if (i != g.Nx - 1 && !(fg & kHoleX)) gx += ag[LX];
+        if (j != g.Ny - 1 && !(fg & kHoleY)) gy += ag[LY];
+      }
+    }
+  }
+  if (ch.z) {  // the recurrence step, J^T zp = (gr, gi, gx, gy, gz)
+    auto step = [&](long long e, double gv) {
+      const double a = ch.zp[e] - ch.hdt * gv;
+      double v = ch.om * (ch.zp[e] + (ch.r[e] - a) * ch.thinv);
+      if (ch.zpp) v = v - (ch.om - 1.0) * ch.zpp[e];
+      ch.z[e] = v;
+    };
+    step(2 * q, gr);
+    step(2 * q + 1, gi);
+    step(2 * n + q, gx);
+    step(3 * n + q, gy);
+    if (IS3D) step(4 * n + q, gz);
+  } else {
+    reinterpret_cast<double2*>(out)[q] = make_double2(gr, gi);
+    out[2 * n + q] = gx;
+    out[3 * n + q] = gy;
+    if (IS3D) out[4 * n + q] = gz;
+  }
+}
+
+// J^T w from the materialised cell adjoint A (k_cells_gather).
 template <bool IS3D>
 __global__ __launch_bounds__(kRedThreads) void k_bc_gather(Geo g, const uint8_t* flags,
                                                            const double* A, double* out,
                                                            const double* stop, AdjCheb ch) {
   if (stop && *stop != 0.0) return;
-  const long long n = g.n;
-  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < n;
-       q += (long long)gridDim.x * kRedThreads) {
-    int i, j, k;
-    node_of<IS3D>(g, q, i, j, k);
-    auto F = [&](int ii, int jj, int kk) { return flags ? flags[mfull(g, ii, jj, kk)] : (uint8_t)0; };
-    auto a = [&](int ii, int jj, int kk, int c) { return A[mfull(g, ii, jj, kk) * 5 + c]; };
-    const uint8_t f0 = F(i, j, k);
-    double gr = a(i, j, k, CR), gi = a(i, j, k, CI);
-    double gx = (i != g.Nx - 1 && !(f0 & kHoleX)) ? a(i, j, k, LX) : 0.0;
-    double gy = (j != g.Ny - 1 && !(f0 & kHoleY)) ? a(i, j, k, LY) : 0.0;
-    double gz = (IS3D && k != g.Nz - 1 && !(f0 & kHoleZ)) ? a(i, j, k, LZ) : 0.0;
-    // x faces
-    for (int side = 0; side < 2; ++side) {
-      const int gi_ = side == 0 ? 0 : g.Nx;
-      const int src = g.per_x ? (side == 0 ? g.Nx - 1 : 1) : (side == 0 ? 1 : g.Nx - 1);
-      if (i != src) continue;
-      const uint8_t fg = F(gi_, j, k);
-      gr += a(gi_, j, k, CR);
-      gi += a(gi_, j, k, CI);
-      if (g.per_x) {
-        if (side == 1 && g.Nx - 1 != 1 && !(fg & kHoleX)) gx += a(gi_, j, k, LX);
-      } else {
-        if (j != g.Ny - 1 && !(fg & kHoleY)) gy += a(gi_, j, k, LY);
-        if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += a(gi_, j, k, LZ);
-      }
-    }
-    // y faces
-    for (int side = 0; side < 2; ++side) {
-      const int gj = side == 0 ? 0 : g.Ny;
-      const int src = g.per_y ? (side == 0 ? g.Ny - 1 : 1) : (side == 0 ? 1 : g.Ny - 1);
-      if (j != src) continue;
-      const uint8_t fg = F(i, gj, k);
-      gr += a(i, gj, k, CR);
-      gi += a(i, gj, k, CI);
-      if (g.per_y) {
-        if (side == 1 && g.Ny - 1 != 1 && !(fg & kHoleY)) gy += a(i, gj, k, LY);
-      } else {
-        if (i != g.Nx - 1 && !(fg & kHoleX)) gx += a(i, gj, k, LX);
-        if (IS3D && k != g.Nz - 1 && !(fg & kHoleZ)) gz += a(i, gj, k, LZ);
-      }
-    }
-    // z faces
-    if (IS3D) {
-      for (int side = 0; side < 2; ++side) {
-        const int gk = side == 0 ? 0 : g.Nz;
-        const int src = g.per_z ? (side == 0 ? g.Nz - 1 : 1) : (side == 0 ? 1 : g.Nz - 1);
-        if (k != src) continue;
-        const uint8_t fg = F(i, j, gk);
-        gr += a(i, j, gk, CR);
-        gi += a(i, j, gk, CI);
-        if (g.per_z) {
-          if (side == 1 && g.Nz - 1 != 1 && !(fg & kHoleZ)) gz += a(i, j, gk, LZ);
-        } else {
-          if (i != g.Nx - 1 && !(fg & kHoleX)) gx += a(i, j, gk, LX);
-          if (j != g.Ny - 1 && !(fg & kHoleY)) gy += a(i, j, gk, LY);
-        }
-      }
-    }
-    if (ch.z) {  // the recurrence step, J^T zp = (gr, gi, gx, gy, gz)
-      auto step = [&](long long e, double gv) {
-        const double a = ch.zp[e] - ch.hdt * gv;
-        double v = ch.om * (ch.zp[e] + (ch.r[e] - a) * ch.thinv);
-        if (ch.zpp) v = v - (ch.om - 1.0) * ch.zpp[e];
-        ch.z[e] = v;
-      };
-      step(2 * q, gr);
-      step(2 * q + 1, gi);
-      step(2 * n + q, gx);
-      step(3 * n + q, gy);
-      if (IS3D) step(4 * n + q, gz);
-    } else {
-      reinterpret_cast<double2*>(out)[q] = make_double2(gr, gi);
-      out[2 * n + q] = gx;
-      out[3 * n + q] = gy;
-      if (IS3D) out[4 * n + q] = gz;
-    }
-  }
+  auto cell = [&](int ii, int jj, int kk, double (&a)[5]) {
+    const long long c = mfull(g, ii, jj, kk);
+#pragma unroll
+    for (int comp = 0; comp < 5; ++comp) a[comp] = A[c * 5 + comp];
+  };
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < g.n;
+       q += (long long)gridDim.x * kRedThreads)
+    bc_node<IS3D>(g, flags, q, out, ch, cell);
+}
+
+// J^T w straight from the planes: each node gathers the cells it needs (its
+// own, and at a face the ghosts copied from it) itself, so the cell adjoint is
+// never stored.  Each of those cells is needed by this node only, and each is
+// summed as k_cells_gather sums it: the same bits as the two kernels.
+template <bool IS3D>
+__global__ __launch_bounds__(kRedThreads) void k_planes_gather(Geo g, const uint8_t* flags,
+                                                               const double* B, double* out,
+                                                               const double* stop, AdjCheb ch) {
+  if (stop && *stop != 0.0) return;
+  auto cell = [&](int ii, int jj, int kk, double (&a)[5]) { gather_cell<IS3D>(g, B, ii, jj, kk, a); };
+  for (long long q = (long long)blockIdx.x * kRedThreads + threadIdx.x; q < g.n;
+       q += (long long)gridDim.x * kRedThreads)
+    bc_node<IS3D>(g, flags, q, out, ch, cell);
 }
 
 // d/d(cx, cy, cz) of the face ghosts' transverse links (rhs.py:91-98, 107-112,
@@ -531,6 +572,12 @@ void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, doubl
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);
   if (g.is3d) launch(k_bc_gather<true>, gr, bl, 0, st, g, flags, A, out, stop, ch);
   else launch(k_bc_gather<false>, gr, bl, 0, st, g, flags, A, out, stop, ch);
+}
+void launch_planes_gather(const Geo& g, const uint8_t* flags, const double* B, double* out, hipStream_t st,
+                          const double* stop, const AdjCheb& ch) {
+  const dim3 gr(stream_blocks(g)), bl(kRedThreads);
+  if (g.is3d) launch(k_planes_gather<true>, gr, bl, 0, st, g, flags, B, out, stop, ch);
+  else launch(k_planes_gather<false>, gr, bl, 0, st, g, flags, B, out, stop, ch);
 }
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st) {
   const dim3 gr(stream_blocks(g)), bl(kRedThreads);

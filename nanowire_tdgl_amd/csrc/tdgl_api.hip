@@ -728,8 +728,7 @@ int tgcr(tdgl_ctx* ctx, double hdt, const tdgl_solver_opts* o, double rr, double
       // J(x1)^T v into AP[i], or (ch.z) the recurrence step formed from it
       auto jt = [&](const double* v, const AdjCheb& ch) {
         launch_vjp_cells(g, ctx->b, ctx->mt, op.x1, v, ctx->d_vjpb, nullptr, 0.0, nullptr, ctx->st, stop);
-        launch_cells_gather(g, ctx->d_vjpb, ctx->d_acell, ctx->st, stop);
-        launch_bc_gather(g, ctx->mt.flags, ctx->d_acell, ctx->AP[i], ctx->st, stop, ch);
+        launch_planes_gather(g, ctx->mt.flags, ctx->d_vjpb, ctx->AP[i], ctx->st, stop, ch);
       };
       tok = prof_begin(ctx, TDGL_K_JVP, (i > 0 ? 5.0 : 4.0) * S, 4);
       const double* z = ctx->w_r;

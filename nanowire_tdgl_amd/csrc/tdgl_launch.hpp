@@ -315,6 +315,10 @@ int launch_adj_combine(const Geo& g, const double* r, double* ap, double hdt, co
 void launch_scale(const Geo& g, double* y, double s, const double* x, hipStream_t st);
 void launch_bc_gather(const Geo& g, const uint8_t* flags, const double* A, double* out, hipStream_t st,
                       const double* stop = nullptr, const AdjCheb& ch = AdjCheb{});
+// launch_cells_gather + launch_bc_gather in one pass over the nodes (the cell
+// adjoint is not stored; the same bits).
+void launch_planes_gather(const Geo& g, const uint8_t* flags, const double* B, double* out, hipStream_t st,
+                          const double* stop = nullptr, const AdjCheb& ch = AdjCheb{});
 int launch_field_grad(const Geo& g, const uint8_t* flags, const double* A, double* pout, hipStream_t st);
 void launch_bz_cells(const Geo& g, const Bnd& b, const double* gbz, double* A, hipStream_t st);
 void launch_permute_in(const Geo& g, const void* ref, double* st_out, int* bad, hipStream_t st);
