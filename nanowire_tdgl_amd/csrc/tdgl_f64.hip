@@ -125,6 +125,25 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
     pl1 = plan64(g, b, i0 - 1 + hc1 % CX, j0 - 1 + hc1 / CX);
   }
   Cell<double> v0, v1;
+  // GEN: each cell staged whole (eff_cell); otherwise a plane's loads
+  // (load64u) and rules (finish64u) in one stage, except in the prologue,
+  // whose three planes' loads all go out before the first rules
+  const bool hf = mt.flags != nullptr;
+  auto ld = [&](int kp, Raw64& x0, Raw64& x1) __attribute__((always_inline)) {
+    const Pl64 pp = plane64<IS3D>(g, b, kp);
+    load64u<IS3D, PERT>(mt.flags, X, P, pl0, pp, x0);
+    if (has1) load64u<IS3D, PERT>(mt.flags, X, P, pl1, pp, x1);
+  };
+  auto fin = [&](int kp, const Raw64& x0, const Raw64& x1) __attribute__((always_inline)) {
+    const Pl64 pp = plane64<IS3D>(g, b, kp);
+    if (IS3D && pp.fk) {
+      finish64u<IS3D, true, PERT>(x0, hf, s.eps, pl0, pp, v0);
+      if (has1) finish64u<IS3D, true, PERT>(x1, hf, s.eps, pl1, pp, v1);
+    } else {
+      finish64u<IS3D, false, PERT>(x0, hf, s.eps, pl0, pp, v0);
+      if (has1) finish64u<IS3D, false, PERT>(x1, hf, s.eps, pl1, pp, v1);
+    }
+  };
   auto stage = [&](int kp) __attribute__((always_inline)) {
     if constexpr (GEN) {
       unsigned int h0 = hc0, h1 = hc1;
@@ -134,18 +153,13 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
       if (has1)
         v1 = eff_cell<double, IS3D, PERT>(g, b, mt.flags, s, i0 - 1 + h1 % CX, j0 - 1 + h1 / CX, kp);
     } else {
-      const Pl64 pp = plane64<IS3D>(g, b, kp);
-      if (IS3D && pp.fk) {
-        stage64u<IS3D, true, PERT>(mt.flags, X, P, s.eps, pl0, pp, v0);
-        if (has1) stage64u<IS3D, true, PERT>(mt.flags, X, P, s.eps, pl1, pp, v1);
-      } else {
-        stage64u<IS3D, false, PERT>(mt.flags, X, P, s.eps, pl0, pp, v0);
-        if (has1) stage64u<IS3D, false, PERT>(mt.flags, X, P, s.eps, pl1, pp, v1);
-      }
+      Raw64 x0, x1;
+      ld(kp, x0, x1);
+      fin(kp, x0, x1);
     }
   };
   const unsigned int m_ij = (unsigned int)i + (unsigned int)g.mj * (unsigned int)j;  // full-grid column
-  auto commit = [&](int slot) __attribute__((always_inline)) {
+  auto write = [&](int slot) __attribute__((always_inline)) {
     l_pr[slot][hc0] = v0.pr;
     l_pi[slot][hc0] = v0.pi;
     l_x[slot][hc0] = v0.x;
@@ -159,13 +173,26 @@ __device__ __forceinline__ void stencil_tile(const Geo& g, const Bnd& b, const M
       if (IS3D) l_z[slot][hc1] = v1.z;
     }
   };
+  auto commit = [&](int slot) __attribute__((always_inline)) { write(slot); };
 
   if (IS3D) {
-    stage(kb - 1);
-    commit((kb - 1) % 3);
-    stage(kb);
-    commit(kb % 3);
-    stage(kb + 1);
+    if constexpr (GEN) {
+      stage(kb - 1);
+      commit((kb - 1) % 3);
+      stage(kb);
+      commit(kb % 3);
+      stage(kb + 1);
+    } else {  // the three planes' loads together: one memory round trip before the first step
+      Raw64 a0{}, a1{}, b0{}, b1{}, c0{}, c1{};
+      ld(kb - 1, a0, a1);
+      ld(kb, b0, b1);
+      ld(kb + 1, c0, c1);
+      fin(kb - 1, a0, a1);
+      write((kb - 1) % 3);
+      fin(kb, b0, b1);
+      write(kb % 3);
+      fin(kb + 1, c0, c1);  // committed at the first step
+    }
   } else {
     stage(0);
   }

@@ -514,13 +514,32 @@ __host__ __device__ __forceinline__ Pl64 plane64(const Geo& g, const Bnd& b, int
   return p;
 }
 
-// Stage the cell of column plan c on plane p (0 <= k <= Nz; uniform field):
-// x, or x + eps p when PERT (two roundings, as Src::ld).
-template <bool IS3D, bool FK, bool PERT>
-__host__ __device__ __forceinline__ void stage64u(const uint8_t* flags, const V64& X, const V64& P,
-                                                  double eps, const Plan64& c, const Pl64& p,
-                                                  Cell<double>& cx) {
+// The raw loads of a staged FP64 cell (x, p and the hole flags at the cell), and
+// the column's and plane's rules applied to them (stage64u below is the two).
+struct Raw64 {
+  double2 xa, pa;
+  double xx, xy, xz, px, py, pz;
+  unsigned int h;
+};
+template <bool IS3D, bool PERT>
+__host__ __device__ __forceinline__ void load64u(const uint8_t* flags, const V64& X, const V64& P,
+                                                 const Plan64& c, const Pl64& p, Raw64& r) {
   const unsigned int q = c.qij + p.qoff;
+  r.xa = X.psi(q);
+  r.xx = X.ph(2, q);
+  r.xy = X.ph(3, q);
+  r.xz = IS3D ? X.ph(4, q) : 0.0;
+  if (PERT) {
+    r.pa = P.psi(q);
+    r.px = P.ph(2, q);
+    r.py = P.ph(3, q);
+    if (IS3D) r.pz = P.ph(4, q);
+  }
+  r.h = flags ? (unsigned int)ldu(flags, c.mij + p.moff) : 0u;
+}
+template <bool IS3D, bool FK, bool PERT>
+__host__ __device__ __forceinline__ void finish64u(const Raw64& r, bool hf, double eps, const Plan64& c,
+                                                   const Pl64& p, Cell<double>& cx) {
   unsigned int kx, ky, kz, ok;
   double ax, ay, az;
   if (FK) {
@@ -540,27 +559,36 @@ __host__ __device__ __forceinline__ void stage64u(const uint8_t* flags, const V6
     ay = c.ay;
     az = c.az;
   }
-  double2 xa = X.psi(q);
-  double xx = X.ph(2, q), xy = X.ph(3, q), xz = IS3D ? X.ph(4, q) : 0.0;
+  double2 xa = r.xa;
+  double xx = r.xx, xy = r.xy, xz = r.xz;
   if (PERT) {
-    const double2 pa = P.psi(q);
-    xa.x = xa.x + eps * pa.x;
-    xa.y = xa.y + eps * pa.y;
-    xx = xx + eps * P.ph(2, q);
-    xy = xy + eps * P.ph(3, q);
-    if (IS3D) xz = xz + eps * P.ph(4, q);
+    xa.x = xa.x + eps * r.pa.x;
+    xa.y = xa.y + eps * r.pa.y;
+    xx = xx + eps * r.px;
+    xy = xy + eps * r.py;
+    if (IS3D) xz = xz + eps * r.pz;
   }
-  if (flags) {
-    const uint8_t h = ldu(flags, c.mij + p.moff);
-    kx &= keep(!(h & kHoleX));
-    ky &= keep(!(h & kHoleY));
-    if (IS3D) kz &= keep(!(h & kHoleZ));
+  if (hf) {
+    kx &= keep(!(r.h & kHoleX));
+    ky &= keep(!(r.h & kHoleY));
+    if (IS3D) kz &= keep(!(r.h & kHoleZ));
   }
   cx.pr = fmask(xa.x, ok);
   cx.pi = fmask(xa.y, ok);
   cx.x = fmask(fmask(xx, c.sx) + ax, kx);
   cx.y = fmask(fmask(xy, c.sy) + ay, ky);
   cx.z = IS3D ? fmask(fmask(xz, p.sz) + az, kz) : 0.0;
+}
+
+// Stage the cell of column plan c on plane p (0 <= k <= Nz; uniform field):
+// x, or x + eps p when PERT (two roundings, as Src::ld).
+template <bool IS3D, bool FK, bool PERT>
+__host__ __device__ __forceinline__ void stage64u(const uint8_t* flags, const V64& X, const V64& P,
+                                                  double eps, const Plan64& c, const Pl64& p,
+                                                  Cell<double>& cx) {
+  Raw64 r;
+  load64u<IS3D, PERT>(flags, X, P, c, p, r);
+  finish64u<IS3D, FK, PERT>(r, flags != nullptr, eps, c, p, cx);
 }
 
 // FP64 cell with its plan (the CPU check compares this with eff_cell).
