@@ -469,7 +469,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
     grp, groups_info = None, None
     if BATCH_GROUPS > 1:
         grp = BatchGroups([f.device for f in films], BATCH_GROUPS)
-        picks = [r.tune(reps=10) for r in grp.runners]  # each group tuned alone
+        gt = grp.tune()  # the groups timed together
         gm = GroupAttempts(grp, films, opts)
         _, accg, el = timed(gm)
         v = accg / el
@@ -477,13 +477,15 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
             raise RuntimeError("the grouped attempts accepted other steps than the single batch")
         groups_info = {"groups": len(grp), "films_per_group": [len(p) for p in grp.parts],
                        "nodes_per_group": [int(ns[p].sum()) for p in grp.parts],
-                       "tile_per_group": [{0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[t["shape"]]
-                                          for t in picks],
+                       "autotune": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[gt["shape"]],
+                                    "split": gt["split"],
+                                    "jvp_ms_by_shape_and_split": {k: round(m, 4) for k, m in gt["trials"].items()}},
                        "one_batch_cell_steps_s": v1, "one_batch_seconds": el1,
                        "timed_window_monotonic_ns": gm.window_ns,
                        "one_batch_timed_window_monotonic_ns": mix.window_ns}
         log(f"[bench] batched mixed, {len(grp)} concurrent groups: {v:.4g} cell-steps/s "
-            f"({attempts} attempts in {el:.3f} s; {v / v1:.3f}x one batch)")
+            f"({attempts} attempts in {el:.3f} s; {v / v1:.3f}x one batch; tiles "
+            f"{groups_info['autotune']['shape']}, split {gt['split']})")
     else:
         v, el = v1, el1
     pc = roof = None

@@ -40,6 +40,11 @@ class BatchResult:
     log: list = field(default_factory=list)  # per attempt: list of per-film stat dicts
 
 
+# The piece splits (deepest / shallowest z piece per block) the autotuners try.
+SPLITS = ((8, 2), (8, 4), (6, 2), (10, 4), (12, 2), (12, 4), (16, 4))
+SHAPE_NAMES = {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}
+
+
 class BatchRunner:
     """A tdgl_batch over Devices, with their FilmEngines kept alive; ``stream``
     (a torch stream) is the one the batch's steps are ordered on (default: the
@@ -73,8 +78,7 @@ class BatchRunner:
         self.batch.set_tile(int(shape))
         self.batch.set_split(int(split[0]), int(split[1]))
 
-    def tune(self, X=None, splits=((8, 2), (8, 4), (6, 2), (10, 4), (12, 2), (12, 4), (16, 4)),
-             shapes=(0, 1, 2, 3),
+    def tune(self, X=None, splits=SPLITS, shapes=(0, 1, 2, 3),
              reps: int = 5, cache: Optional[str] = "default") -> dict:
         """Autotuner: pick the mixed-precision sweeps' tile shape (0 = 64x4,
         1 = 32x8, 2 = 16x16 nodes per plane, 3 = each film a tile of its own
@@ -104,7 +108,7 @@ class BatchRunner:
             if hit and "split" in hit:
                 self._apply(hit["shape"], hit["split"])
                 return dict(hit, kz=kz, cached=True)
-        names = {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}
+        names = SHAPE_NAMES
         best = (b.get_tile(), b.get_split())
         best_ms = b.time_stencil(reps)
         trials = {"default": best_ms}
@@ -197,6 +201,42 @@ class BatchGroups:
 
     def __len__(self) -> int:
         return len(self.parts)
+
+    # Grouped runs try pieces of at least 4 planes: the other groups' sweeps
+    # cover a sweep's tail, so shallower pieces only add per-block set-up.
+    GROUP_SPLITS = ((8, 4), (12, 4), (16, 4), (16, 8), (24, 4), (32, 8))
+
+    def tune(self, splits=GROUP_SPLITS, shapes=(3,), reps: int = 40, repeats: int = 3) -> dict:
+        """The autotuner for groups that run together: every group takes each
+        tile shape and piece split, their FP32 JVP sweeps over all their films
+        are timed at once (the groups start together at a barrier; the slowest
+        group's event time, the best of ``repeats``), and the fastest setting
+        is kept for every group.  A group timed alone wants shallower pieces
+        (more blocks for its share of the GPU) than it does beside the others,
+        whose sweeps fill its start and tail.  Returns {"shape", "split", "ms",
+        "trials"}."""
+        import threading
+        best, trials = None, {}
+        for sh in shapes:
+            for g, l in splits:
+                for r in self.runners:
+                    r.batch.set_tile(int(sh))
+                    r.batch.set_split(int(g), int(l))
+                ms = float("inf")
+                for _ in range(repeats):
+                    gate = threading.Barrier(len(self.runners))
+
+                    def timed(k, r):
+                        gate.wait()
+                        return r.batch.time_stencil(reps)
+
+                    ms = min(ms, max(self.run(timed)))
+                trials[f"{SHAPE_NAMES[sh]}:{g}x{l}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, int(sh), (int(g), int(l)))
+        for r in self.runners:
+            r._apply(best[1], best[2])
+        return {"shape": best[1], "split": list(best[2]), "ms": best[0], "trials": trials}
 
     def run(self, fn) -> list:
         """``fn(k, runner)`` for every group k at once, each in its own thread

@@ -332,6 +332,11 @@ def test_concurrent_groups_reproduce_one_batch_bitwise(groups):
     one = trapezoidal_batch(films, 0.0, 0.8, 0.2, keep_log=True)
     grp = BatchGroups([f.device for f in films], groups)
     assert sorted(i for p in grp.parts for i in p) == list(range(len(films)))
+    # the groups' autotuner (timed together) leaves every group on its pick
+    pick = grp.tune(reps=2, repeats=1)
+    assert tuple(pick["split"]) in BatchGroups.GROUP_SPLITS
+    assert all(r.batch.get_tile() == pick["shape"] and list(r.batch.get_split()) == pick["split"]
+               for r in grp.runners)
     for pred in ("euler", "extrapolate"):
         ref = one if pred == "euler" else trapezoidal_batch(films, 0.0, 0.8, 0.2, keep_log=True,
                                                             predictor=pred)
