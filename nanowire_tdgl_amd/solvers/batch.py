@@ -338,9 +338,9 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     for part, r, st in zip(grp.parts, grp.runners, grp.streams):
         with torch.cuda.stream(st):
             Xs.append(r.pack([films[i].x0 for i in part]))
-        if tune:  # each group timed alone, before they run together
-            r.tune()
     torch.cuda.synchronize()
+    if tune:  # the groups timed together
+        grp.tune()
     res = grp.run(lambda k, r: _advance(r, Xs[k], grp.parts[k], **kw))
     F = len(films)
     states, times, steps = [None] * F, [0.0] * F, [0] * F
