@@ -1217,7 +1217,10 @@ int tdgl_batch_own_queue(tdgl_batch* B) {
   for (int i = 0; i < ncu; ++i) mask[(size_t)i / 32] |= 1u << (i % 32);
   hipStream_t s = nullptr;
   BCHK(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
-  BCHK(hipStreamSynchronize(B->ps));
+  if (const hipError_t e = hipStreamSynchronize(B->ps); e != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    return bfail(B, TDGL_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  }
   if (B->body) {
     (void)hipGraphExecDestroy(B->body);
     B->body = nullptr;

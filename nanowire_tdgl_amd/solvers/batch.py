@@ -333,6 +333,9 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     if runner is not None and not isinstance(runner, BatchGroups):
         raise ValueError("with groups > 1, runner must be a BatchGroups")
     grp = runner or BatchGroups([f.device for f in films], groups)
+    if sorted(i for part in grp.parts for i in part) != list(range(len(films))):
+        raise ValueError(f"the BatchGroups runner holds {sum(map(len, grp.parts))} films; "
+                         f"{len(films)} were given")
     import torch
     Xs = []
     for part, r, st in zip(grp.parts, grp.runners, grp.streams):

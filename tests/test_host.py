@@ -168,9 +168,14 @@ def test_partition_films_covers_every_film_once_and_balances():
 
 def test_trapezoidal_batch_argument_errors():
     """Host-side checks of trapezoidal_batch, raised before any device work:
-    an unknown predictor, and a plain runner handed to a grouped run."""
-    from nanowire_tdgl_amd.solvers.batch import trapezoidal_batch
+    an unknown predictor, a plain runner handed to a grouped run, and groups
+    built for another number of films."""
+    from nanowire_tdgl_amd.solvers.batch import BatchGroups, partition_films, trapezoidal_batch
     with pytest.raises(ValueError, match="predictor"):
         trapezoidal_batch([], 0.0, 1.0, 0.1, predictor="bdf2")
     with pytest.raises(ValueError, match="BatchGroups"):
         trapezoidal_batch([], 0.0, 1.0, 0.1, groups=2, runner=object())
+    grp = BatchGroups.__new__(BatchGroups)  # the partition only: no batches on the device
+    grp.parts = partition_films([5, 4, 3], 2)
+    with pytest.raises(ValueError, match="holds 3 films; 2 were given"):
+        trapezoidal_batch([object(), object()], 0.0, 1.0, 0.1, groups=2, runner=grp)
