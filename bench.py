@@ -390,7 +390,7 @@ class BatchAttempts:
 
 class GroupAttempts:
     """BatchAttempts over the films split into concurrent sub-batches
-    (BatchGroups: node-balanced groups, each its own tdgl_batch on its own
+    (BatchGroups: each group its own tdgl_batch on its own
     stream, stepped from its own thread).  An attempt is one attempt of every
     group, so every film makes the same attempts, with the same bits, as in
     one batch."""
@@ -449,6 +449,7 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         b.start = b.snapshot()
         b.run(warmup)
         torch.cuda.synchronize()
+        b.work_warm = b.work.copy() if isinstance(b, BatchAttempts) else None
         snap = b.snapshot()
         n0 = b.newton
         w0 = time.monotonic_ns()  # the window on the clock kernel traces use
@@ -468,7 +469,11 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         f"{attempts} attempts in {el1:.3f} s")
     grp, groups_info = None, None
     if BATCH_GROUPS > 1:
-        grp = BatchGroups([f.device for f in films], BATCH_GROUPS)
+        # films alike in sweeps per attempt share a group (their sweeps stay
+        # full), the groups balanced by that work: measured over the one
+        # batch's warm-up attempts, before any timed attempt
+        wk = mix.work_warm
+        grp = BatchGroups([f.device for f in films], BATCH_GROUPS, weights=wk, keys=wk / ns)
         gt = grp.tune()  # the groups timed together
         gm = GroupAttempts(grp, films, opts)
         _, accg, el = timed(gm)
@@ -477,6 +482,9 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
             raise RuntimeError("the grouped attempts accepted other steps than the single batch")
         groups_info = {"groups": len(grp), "films_per_group": [len(p) for p in grp.parts],
                        "nodes_per_group": [int(ns[p].sum()) for p in grp.parts],
+                       "partition": "films sorted by sweeps per attempt, cut into runs of equal sweep work "
+                                    f"(both over the one batch's {warmup} warm-up attempts)",
+                       "work_share_per_group": [round(float(wk[p].sum() / wk.sum()), 3) for p in grp.parts],
                        "autotune": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[gt["shape"]],
                                     "split": gt["split"],
                                     "jvp_ms_by_shape_and_split": {k: round(m, 4) for k, m in gt["trials"].items()}},

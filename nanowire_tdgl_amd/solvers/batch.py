@@ -128,15 +128,32 @@ class BatchRunner:
         return dict(res, kz=kz, cached=False)
 
 
-def partition_films(sizes: Sequence[float], groups: int) -> list:
+def partition_films(sizes: Sequence[float], groups: int, keys: Optional[Sequence[float]] = None) -> list:
     """Split films (their weights ``sizes``: node counts, or measured work)
-    into ``groups`` groups of near-equal total weight: the heaviest film first,
-    each film to the group with the least weight so far (ties to the lowest
-    group).  Returns the groups as ascending lists of film indices; no group is
-    empty when groups <= films."""
+    into ``groups`` groups of near-equal total weight.  Without ``keys``: the
+    heaviest film first, each film to the group with the least weight so far
+    (ties to the lowest group).  With ``keys`` (one per film, such as a film's
+    sweeps per attempt): the films sorted by key are cut into contiguous runs
+    of near-equal weight, each film to the run its weight's midpoint falls in,
+    so films alike in key share a group.  Returns the groups as ascending lists
+    of film indices; no group is empty when groups <= films."""
     if groups < 1:
         raise ValueError("groups must be >= 1")
     groups = min(groups, len(sizes))
+    if keys is not None:
+        if len(keys) != len(sizes):
+            raise ValueError("one key per film")
+        order = sorted(range(len(sizes)), key=lambda i: (float(keys[i]), i))
+        tot = float(sum(float(w) for w in sizes)) or 1.0
+        parts = [[] for _ in range(groups)]
+        acc, k = 0.0, 0
+        for pos, i in enumerate(order):
+            want = int((acc + 0.5 * float(sizes[i])) * groups / tot)
+            # never skip a group, and leave every later group a film
+            k = min(k + (1 if parts[k] else 0), max(k, want, groups - (len(order) - pos)), groups - 1)
+            parts[k].append(i)
+            acc += float(sizes[i])
+        return [sorted(p) for p in parts]
     load = [0.0] * groups
     parts: list = [[] for _ in range(groups)]
     for i in sorted(range(len(sizes)), key=lambda i: (-sizes[i], i)):
@@ -150,7 +167,10 @@ class BatchGroups:
     """The films of a batch split into G sub-batches that run concurrently.
 
     Each group (:func:`partition_films`: balanced by nodes, or by ``weights``
-    such as each film's measured work) is a :class:`BatchRunner` of its own,
+    such as each film's measured work; with ``keys``, such as each film's
+    sweeps per attempt, films alike in key share a group, so a group's films
+    stop their Chebyshev sweeps and TGCR iterations at similar counts and its
+    sweeps stay full) is a :class:`BatchRunner` of its own,
     stepped from a host thread of its own (the library's calls release the
     GIL), and the group's host-side torch work runs on its batch's own stream.
     One group's host work -- the Newton control flow, the converged-film
@@ -171,12 +191,12 @@ class BatchGroups:
     batch; 5 and more lose."""
 
     def __init__(self, devices: Sequence, groups: int, weights: Optional[Sequence[float]] = None,
-                 own_queues: bool = True):
+                 own_queues: bool = True, keys: Optional[Sequence[float]] = None):
         devices = list(devices)
         sizes = [d.params.n_interior for d in devices] if weights is None else list(weights)
         if len(sizes) != len(devices):
             raise ValueError("one weight per film")
-        self.parts = partition_films(sizes, groups)
+        self.parts = partition_films(sizes, groups, keys)
         hwq = os.environ.get("GPU_MAX_HW_QUEUES", "")
         if not own_queues and len(self.parts) > 1 and int(hwq if hwq.isdigit() else 4) < len(self.parts) + 2:
             import warnings

@@ -321,17 +321,18 @@ def test_graph_replay_matches_eager_launches():
     assert outs[0].equal(outs[1])
 
 
-@pytest.mark.parametrize("groups", [2, 3, 8])
-def test_concurrent_groups_reproduce_one_batch_bitwise(groups):
+@pytest.mark.parametrize("groups,keyed", [(2, False), (3, False), (8, False), (2, True)])
+def test_concurrent_groups_reproduce_one_batch_bitwise(groups, keyed):
     """trapezoidal_batch(groups=G): the films run as G sub-batches on their own
     streams and host threads at once (graph capture on), and every film ends
     in the same state, time and step count as in one batch, bit for bit; the
     per-film logs agree too, attempt by attempt.  More groups than films give
-    one film per group."""
+    one film per group; films grouped by a key (here the plane count) too."""
     from nanowire_tdgl_amd.solvers.batch import BatchGroups, trapezoidal_batch
     films = _films() + _films()[:2]
     one = trapezoidal_batch(films, 0.0, 0.8, 0.2, keep_log=True)
-    grp = BatchGroups([f.device for f in films], groups)
+    keys = [f.device.params.Nz for f in films] if keyed else None
+    grp = BatchGroups([f.device for f in films], groups, keys=keys)
     assert len(grp) == min(groups, len(films))
     assert sorted(i for p in grp.parts for i in p) == list(range(len(films)))
     # the groups' autotuner (timed together) leaves every group on its pick

@@ -166,6 +166,35 @@ def test_partition_films_covers_every_film_once_and_balances():
         partition_films([1, 2], 0)
 
 
+def test_partition_films_by_key_is_contiguous_in_key_order():
+    """With keys, each group is a contiguous run of the films sorted by (key,
+    index), every film in exactly one group, none empty, and the heaviest group
+    within one film of the mean; a hand-worked case pins the cut rule (each film
+    to the run its weight's midpoint falls in)."""
+    from nanowire_tdgl_amd.solvers.batch import partition_films
+    rng = np.random.default_rng(6)
+    for _ in range(500):
+        F, G = int(rng.integers(1, 70)), int(rng.integers(1, 10))
+        sizes = [int(x) for x in rng.integers(100, 100000, F)]
+        keys = [float(x) for x in rng.integers(0, 20, F)]
+        parts = partition_films(sizes, G, keys)
+        assert len(parts) == min(G, F)
+        assert sorted(i for p in parts for i in p) == list(range(F))
+        assert all(p == sorted(p) and p for p in parts)
+        for a, b in zip(parts[:-1], parts[1:]):
+            assert max((keys[i], i) for i in a) < min((keys[i], i) for i in b)
+        loads = [sum(sizes[i] for i in p) for p in parts]
+        assert max(loads) <= sum(sizes) / len(parts) + max(sizes)
+    # keys order 2, 0, 3, 1 (weights 1, 1, 6, 2 of 10): midpoints 0.5, 1.5, 5, 9
+    # of 10 fall in halves 0, 0, 1, 1
+    assert partition_films([1, 2, 1, 6], 2, keys=[1.0, 3.0, 0.0, 2.0]) == [[0, 2], [1, 3]]
+    # the heavy film's midpoint (52 of 104) falls in run 2 of 4, but it takes
+    # run 1, which no film would hold otherwise
+    assert partition_films([1, 1, 100, 1, 1], 4, keys=[0, 1, 2, 3, 4]) == [[0, 1], [2], [3], [4]]
+    with pytest.raises(ValueError, match="one key per film"):
+        partition_films([1, 2], 2, keys=[1.0])
+
+
 def test_trapezoidal_batch_argument_errors():
     """Host-side checks of trapezoidal_batch, raised before any device work:
     an unknown predictor, a plain runner handed to a grouped run, and groups
