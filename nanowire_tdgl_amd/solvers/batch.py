@@ -226,12 +226,13 @@ class BatchGroups:
     # cover a sweep's tail, so shallower pieces only add per-block set-up.
     GROUP_SPLITS = ((8, 4), (12, 4), (16, 4), (16, 8), (24, 4), (32, 8))
 
-    def tune(self, splits=GROUP_SPLITS, shapes=(3,), reps: int = 40, repeats: int = 3) -> dict:
+    def tune(self, splits=GROUP_SPLITS, shapes=(3,), reps: int = 40, repeats: int = 5) -> dict:
         """The autotuner for groups that run together: every group takes each
         tile shape and piece split, their FP32 JVP sweeps over all their films
         are timed at once (the groups start together at a barrier; the slowest
-        group's event time, the best of ``repeats``), and the fastest setting
-        is kept for every group.  A group timed alone wants shallower pieces
+        group's event time, the median of ``repeats``: the settings differ by a
+        few per cent, and a best-of count picks whichever setting had one
+        lucky overlap), and the fastest setting is kept for every group.  A group timed alone wants shallower pieces
         (more blocks for its share of the GPU) than it does beside the others,
         whose sweeps fill its start and tail.  Returns {"shape", "split", "ms",
         "trials"}."""
@@ -242,7 +243,7 @@ class BatchGroups:
                 for r in self.runners:
                     r.batch.set_tile(int(sh))
                     r.batch.set_split(int(g), int(l))
-                ms = float("inf")
+                runs = []
                 for _ in range(repeats):
                     gate = threading.Barrier(len(self.runners))
 
@@ -250,7 +251,8 @@ class BatchGroups:
                         gate.wait()
                         return r.batch.time_stencil(reps)
 
-                    ms = min(ms, max(self.run(timed)))
+                    runs.append(max(self.run(timed)))
+                ms = float(np.median(runs))
                 trials[f"{SHAPE_NAMES[sh]}:{g}x{l}"] = ms
                 if best is None or ms < best[0]:
                     best = (ms, int(sh), (int(g), int(l)))
