@@ -338,8 +338,9 @@ def trapezoidal_batch(films: Sequence[BatchFilm], t_start: float, t_stop: float,
     ``predictor`` as in ``trapezoidal``, per film.
 
     ``groups`` > 1 runs the films as that many concurrent sub-batches
-    (:class:`BatchGroups`; ``runner`` may then be a BatchGroups of as many
-    groups): every film's result is the same bit for bit, the lockstep and
+    (:class:`BatchGroups`, balanced by nodes; ``runner`` may instead be a
+    BatchGroups built for these films, such as one grouped by ``keys`` measured
+    on an earlier run): every film's result is the same bit for bit, the lockstep and
     ``max_attempts`` hold per group, ``attempts`` is the largest group's count
     and ``log[a][f]`` is None for a film whose group made fewer attempts."""
     if predictor not in ("euler", "extrapolate"):
