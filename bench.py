@@ -473,7 +473,11 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
         # full), the groups balanced by that work: measured over the one
         # batch's warm-up attempts, before any timed attempt
         wk = mix.work_warm
-        grp = BatchGroups([f.device for f in films], BATCH_GROUPS, weights=wk, keys=wk / ns)
+        if wk.sum() > 0:
+            grp = BatchGroups([f.device for f in films], BATCH_GROUPS, weights=wk, keys=wk / ns)
+        else:  # no warm-up attempt to measure: node-balanced groups
+            wk = ns.astype(np.float64)
+            grp = BatchGroups([f.device for f in films], BATCH_GROUPS)
         gt = grp.tune()  # the groups timed together
         gm = GroupAttempts(grp, films, opts)
         _, accg, el = timed(gm)
@@ -482,8 +486,9 @@ def section_batched(attempts, warmup, fp64=True, variants=True):
             raise RuntimeError("the grouped attempts accepted other steps than the single batch")
         groups_info = {"groups": len(grp), "films_per_group": [len(p) for p in grp.parts],
                        "nodes_per_group": [int(ns[p].sum()) for p in grp.parts],
-                       "partition": "films sorted by sweeps per attempt, cut into runs of equal sweep work "
-                                    f"(both over the one batch's {warmup} warm-up attempts)",
+                       "partition": ("films sorted by sweeps per attempt, cut into runs of equal sweep work "
+                                     f"(both over the one batch's {warmup} warm-up attempts)" if warmup > 0
+                                     else "balanced by nodes (no warm-up attempt to measure)"),
                        "work_share_per_group": [round(float(wk[p].sum() / wk.sum()), 3) for p in grp.parts],
                        "autotune": {"shape": {0: "64x4", 1: "32x8", 2: "16x16", 3: "per-film"}[gt["shape"]],
                                     "split": gt["split"],
