@@ -61,13 +61,13 @@ def _run_single(dev, x0, t_end, dt, precision, **newton):
 
 def test_c4_batch_of_64_films_against_fp64():
     from nanowire_tdgl_amd import workloads
-    from nanowire_tdgl_amd.solvers.batch import BatchRunner, trapezoidal_batch
+    from nanowire_tdgl_amd.solvers.batch import BatchGroups, BatchRunner, trapezoidal_batch
     films = workloads.batch_films()
     assert len(films) == 64
     runner = BatchRunner([f.device for f in films])
     tune = runner.tune(reps=2, cache=None)
     print(f"autotuned tile shape {tune['shape']}, split {tune['split']}, {tune['ms']:.3f} ms per sweep")
-    res = trapezoidal_batch(films, 0.0, T_FIELD_STOP, 0.2, max_attempts=3, runner=runner)
+    res = trapezoidal_batch(films, 0.0, T_FIELD_STOP, 0.2, max_attempts=3, runner=runner, keep_log=True)
     print(f"attempts {res.attempts}, accepted steps per film {sorted(set(res.steps))}")
     worst = {"psi2": 0.0, "bz": 0.0, "js": 0.0}
     for f, film in enumerate(films):
@@ -89,6 +89,16 @@ def test_c4_batch_of_64_films_against_fp64():
         ctx, X, t = _run_single(dev, films[f].x0, res.times[f], 0.2, "mixed")
         assert abs(t - res.times[f]) < 1e-9
         assert np.array_equal(ctx.to_reference(X).cpu().numpy(), res.states[f]), f
+    # as 4 concurrent groups of films alike in sweeps per attempt (the sweep
+    # counts of these attempts, weighted by nodes): every film bit for bit
+    ns = np.array([f.device.params.n_interior for f in films], dtype=np.float64)
+    sweeps = np.array([sum(row[f]["jvp_evals"] for row in res.log) for f in range(64)], dtype=np.float64)
+    grp = BatchGroups([f.device for f in films], 4, weights=sweeps * ns, keys=sweeps)
+    assert len(grp) == 4
+    res4 = trapezoidal_batch(films, 0.0, T_FIELD_STOP, 0.2, max_attempts=3, runner=grp, groups=4)
+    assert res4.steps == res.steps and res4.times == res.times
+    for f in range(64):
+        assert np.array_equal(res4.states[f], res.states[f]), f
 
 
 def test_c3_large_film_mixed_against_fp64():
