@@ -4,7 +4,9 @@
   even films), mixed precision in lockstep attempts with the autotuned tiles:
   after three attempts every film is within 1e-6 (relative to its largest
   entry) of its own FP64 run with Newton converged in |psi|^2, B_z and J_s,
-  and two films are bit for bit their own single-film mixed runs.
+  two films are bit for bit their own single-film mixed runs, and the same
+  attempts as 4 concurrent groups (films grouped by their sweeps per attempt)
+  leave every film bit for bit as the one batch does.
 * 256x256x64 film with 1% insulating spheres: two mixed steps against two
   FP64 steps with Newton converged, within 1e-6 in |psi|^2, B_z and J_s; the
   link variables stay real and every value is finite.
